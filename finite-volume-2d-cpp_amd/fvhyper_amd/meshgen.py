@@ -1,0 +1,317 @@
+"""Seeded synthetic meshes and the gmsh MSH 4.1 ASCII writer.
+
+Every workload mesh of this project comes from these generators, so the
+reference reader (fvhyper/src/mesh.cpp:264-617) and this library's reader see
+the same file.  A generator returns a :class:`MeshArrays` whose arrays are in
+*file order*: nodes, cells and boundary elements exactly as
+:func:`write_msh` writes them, which is the order the reference numbers them
+in (mesh.cpp:356-413, 418-483, 487-570).
+
+Random numbers come from an in-repo splitmix64 stream with
+``u = (x >> 11) * 2**-53`` so that the same seed gives the same mesh on every
+machine (no implementation-defined library distributions).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import List, Sequence
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """The first ``n`` outputs of splitmix64 started at ``seed`` (uint64)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + _GOLDEN * (np.arange(1, n + 1, dtype=np.uint64))
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, n: int) -> np.ndarray:
+    """``n`` doubles in [0, 1) from :func:`splitmix64` (53-bit mantissas)."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+@dataclasses.dataclass
+class MeshArrays:
+    """A 2-D mesh in file order.
+
+    x, y        node coordinates (float64, length N)
+    cells       (C, 4) int32 node indices, 0-based; column 3 is -1 for triangles
+    bnd         (B, 2) int32 node indices of the boundary line elements
+    bnd_tag     (B,) int32 index into ``names`` of each boundary element
+    names       physical names of the boundary groups (BC keys)
+    """
+
+    x: np.ndarray
+    y: np.ndarray
+    cells: np.ndarray
+    bnd: np.ndarray
+    bnd_tag: np.ndarray
+    names: List[str]
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.cells.shape[0])
+
+    @property
+    def n_bnd(self) -> int:
+        return int(self.bnd.shape[0])
+
+
+def _split_quads(q00, q10, q11, q01, diag_ac):
+    """Two triangles per quad; ``diag_ac`` picks the 00-11 diagonal."""
+    n = q00.shape[0]
+    t = np.empty((2 * n, 4), dtype=np.int32)
+    t[:, 3] = -1
+    t[0::2, 0] = q00
+    t[0::2, 1] = q10
+    t[0::2, 2] = np.where(diag_ac, q11, q01)
+    t[1::2, 0] = np.where(diag_ac, q00, q10)
+    t[1::2, 1] = q11
+    t[1::2, 2] = q01
+    return t
+
+
+def _incircle(ax, ay, bx, by, cx, cy, dx, dy):
+    """> 0 when d lies inside the circumcircle of the ccw triangle abc."""
+    adx, ady = ax - dx, ay - dy
+    bdx, bdy = bx - dx, by - dy
+    cdx, cdy = cx - dx, cy - dy
+    ad = adx * adx + ady * ady
+    bd = bdx * bdx + bdy * bdy
+    cd = cdx * cdx + cdy * cdy
+    return (adx * (bdy * cd - bd * cdy) - ady * (bdx * cd - bd * cdx) + ad * (bdx * cdy - bdy * cdx))
+
+
+def _grid_mesh(nx, ny, x0, y0, hx, hy, keep, jitter, seed, delaunay, boundary_name):
+    """Structured (nx x ny) grid of squares, masked by ``keep(ic, jc)``.
+
+    Interior nodes move by U(-jitter, +jitter)*h per coordinate; nodes on the
+    domain boundary stay put.  Each square becomes two triangles: along the
+    00-11 diagonal, or, with ``delaunay``, along whichever diagonal passes the
+    empty-circumcircle test for the jittered corners.  ``boundary_name(mx, my,
+    nx_out, ny_out)`` names a boundary edge from its midpoint and outward
+    normal.
+    """
+    ic, jc = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    cell_mask = keep(ic, jc)                      # (ny, nx)
+    # nodes used by kept squares
+    node_used = np.zeros((ny + 1, nx + 1), dtype=bool)
+    node_used[:-1, :-1] |= cell_mask
+    node_used[:-1, 1:] |= cell_mask
+    node_used[1:, :-1] |= cell_mask
+    node_used[1:, 1:] |= cell_mask
+    # boundary nodes: a used node that touches a missing square (or the frame)
+    padded = np.zeros((ny + 2, nx + 2), dtype=bool)
+    padded[1:-1, 1:-1] = cell_mask
+    touch_missing = ~(padded[:-1, :-1] & padded[:-1, 1:] & padded[1:, :-1] & padded[1:, 1:])
+    on_bnd = node_used & touch_missing
+    node_id = -np.ones((ny + 1, nx + 1), dtype=np.int64)
+    order = np.flatnonzero(node_used.ravel())
+    node_id.ravel()[order] = np.arange(order.size)
+    jj, ii = np.divmod(order, nx + 1)
+    x = x0 + ii * hx
+    y = y0 + jj * hy
+    x = x.astype(np.float64)
+    y = y.astype(np.float64)
+    if jitter > 0.0:
+        u = uniform01(seed, 2 * order.size).reshape(order.size, 2)
+        interior = ~on_bnd.ravel()[order]
+        x = np.where(interior, x + (2.0 * u[:, 0] - 1.0) * jitter * hx, x)
+        y = np.where(interior, y + (2.0 * u[:, 1] - 1.0) * jitter * hy, y)
+    # cells, row-major over kept squares
+    kj, ki = np.nonzero(cell_mask)
+    q00 = node_id[kj, ki]
+    q10 = node_id[kj, ki + 1]
+    q11 = node_id[kj + 1, ki + 1]
+    q01 = node_id[kj + 1, ki]
+    if delaunay:
+        # ccw triangle (00, 10, 11); 01 inside its circumcircle -> use 10-01
+        inside = _incircle(x[q00], y[q00], x[q10], y[q10], x[q11], y[q11], x[q01], y[q01]) > 0
+        diag_ac = ~inside
+    else:
+        diag_ac = np.ones(q00.shape[0], dtype=bool)
+    cells = _split_quads(q00, q10, q11, q01, diag_ac)
+    # boundary edges: square sides whose other side is missing
+    edges = []
+    tags = []
+    names: List[str] = []
+    sides = [  # (di, dj) of the neighbour square, node pair of the shared side, outward normal
+        (0, -1, (q00, q10), (0.0, -1.0)),
+        (1, 0, (q10, q11), (1.0, 0.0)),
+        (0, 1, (q11, q01), (0.0, 1.0)),
+        (-1, 0, (q01, q00), (-1.0, 0.0)),
+    ]
+    for di, dj, (na, nb), (ox, oy) in sides:
+        ni, nj = ki + di, kj + dj
+        outside = (ni < 0) | (ni >= nx) | (nj < 0) | (nj >= ny)
+        nbr_kept = np.zeros_like(outside)
+        inside_idx = ~outside
+        nbr_kept[inside_idx] = cell_mask[nj[inside_idx], ni[inside_idx]]
+        is_b = ~nbr_kept
+        if not np.any(is_b):
+            continue
+        a, b = na[is_b], nb[is_b]
+        # midpoint of the undisplaced side (boundary nodes are not jittered)
+        mx = x0 + (ki[is_b] + (0.5 if dj != 0 else (1.0 if di > 0 else 0.0))) * hx
+        my = y0 + (kj[is_b] + (0.5 if di != 0 else (1.0 if dj > 0 else 0.0))) * hy
+        for k in range(a.size):
+            nm = boundary_name(float(mx[k]), float(my[k]), ox, oy)
+            if nm not in names:
+                names.append(nm)
+            edges.append((int(a[k]), int(b[k])))
+            tags.append(names.index(nm))
+    bnd = np.asarray(edges, dtype=np.int32).reshape(-1, 2)
+    bnd_tag = np.asarray(tags, dtype=np.int32)
+    # group boundary elements by name (one line-element block per group)
+    order_b = np.argsort(bnd_tag, kind="stable")
+    return MeshArrays(x, y, cells, bnd[order_b], bnd_tag[order_b], names)
+
+
+def sod_tube(nx: int = 400, ny: int = 40, length: float = 1.0, height: float = 0.1,
+             jitter: float = 0.2, seed: int = 20240601) -> MeshArrays:
+    """Shock-tube channel [0,length]x[0,height]: nx x ny squares split into
+    two triangles each along a fixed diagonal, interior nodes jittered by
+    U(+-jitter*h).  Top/bottom are ``wall`` (slip), ends ``transmissive``.
+    The 400 x 40 default is the 32,000-cell Sod workload."""
+    hx, hy = length / nx, height / ny
+
+    def name(mx, my, ox, oy):
+        return "transmissive" if ox != 0.0 else "wall"
+
+    return _grid_mesh(nx, ny, 0.0, 0.0, hx, hy, lambda i, j: np.ones_like(i, dtype=bool),
+                      jitter, seed, False, name)
+
+
+def forward_step(n_per_unit: int = 440, jitter: float = 0.2, seed: int = 20240602,
+                 step_x: float = 0.6, step_h: float = 0.2, length: float = 3.0,
+                 height: float = 1.0, step_name: str = "wall") -> MeshArrays:
+    """Mach-3 forward-facing step channel [0,3]x[0,1] minus [0.6,3]x[0,0.2].
+
+    A square lattice of spacing 1/n_per_unit, interior nodes jittered by
+    U(+-jitter*h), each square split along its Delaunay diagonal.  With the
+    default 440 per unit the mean edge length is ~1/400 and the mesh has
+    975,744 triangles.  Boundaries: ``inlet`` (x=0), ``outlet`` (x=3), the
+    step face x=0.6 named ``step_name``, every other side ``wall``.
+    """
+    h = 1.0 / n_per_unit
+    nx = int(round(length * n_per_unit))
+    ny = int(round(height * n_per_unit))
+    sx = int(round(step_x * n_per_unit))
+    sy = int(round(step_h * n_per_unit))
+    if abs(sx * h - step_x) > 1e-12 or abs(sy * h - step_h) > 1e-12:
+        raise ValueError("step corner must fall on the lattice")
+
+    def keep(i, j):
+        return ~((i >= sx) & (j < sy))
+
+    def name(mx, my, ox, oy):
+        if ox < 0 and abs(mx) < 0.5 * h:
+            return "inlet"
+        if ox > 0 and abs(mx - length) < 0.5 * h:
+            return "outlet"
+        if ox > 0 and abs(mx - step_x) < 0.5 * h:
+            return step_name
+        return "wall"
+
+    return _grid_mesh(nx, ny, 0.0, 0.0, h, h, keep, jitter, seed, True, name)
+
+
+def mixed_channel(nx: int = 12, ny: int = 6, jitter: float = 0.15, seed: int = 7,
+                  tri_fraction: float = 0.5, names=("wall", "transmissive")) -> MeshArrays:
+    """A small jittered channel [0,1]x[0,0.5] of quads, a seeded fraction of
+    them split into triangles: exercises 3- and 4-edge cells together."""
+    m = _grid_mesh(nx, ny, 0.0, 0.0, 1.0 / nx, 0.5 / ny,
+                   lambda i, j: np.ones_like(i, dtype=bool), jitter, seed, False,
+                   lambda mx, my, ox, oy: names[1] if ox != 0.0 else names[0])
+    tri = m.cells
+    quads = np.empty((tri.shape[0] // 2, 4), dtype=np.int32)
+    quads[:, 0] = tri[0::2, 0]
+    quads[:, 1] = tri[0::2, 1]
+    quads[:, 2] = tri[0::2, 2]
+    quads[:, 3] = tri[1::2, 2]
+    split = uniform01(seed + 1, quads.shape[0]) < tri_fraction
+    out = []
+    for k in range(quads.shape[0]):
+        if split[k]:
+            out.append(tri[2 * k])
+            out.append(tri[2 * k + 1])
+        else:
+            out.append(quads[k])
+    m.cells = np.asarray(out, dtype=np.int32)
+    return m
+
+
+# ---------------------------------------------------------------------------
+# MSH 4.1 ASCII writer: exactly the subset the reference reader parses
+# (mesh.cpp:264-617): single-space tokens, 1-based tags, curve entities with
+# one physical tag at field 8, one node block, line blocks then cell blocks.
+# ---------------------------------------------------------------------------
+
+def write_msh(path: str, m: MeshArrays) -> None:
+    nb_names = len(m.names)
+    n_nodes = m.x.size
+    lines: List[str] = []
+    lines += ["$MeshFormat", "4.1 0 8", "$EndMeshFormat"]
+    lines += ["$PhysicalNames", str(nb_names + 1)]
+    for k, nm in enumerate(m.names):
+        if " " in nm or '"' in nm:
+            raise ValueError("physical names may not hold spaces or quotes")
+        lines.append('1 %d "%s"' % (k + 1, nm))
+    lines.append('2 %d "internal"' % (nb_names + 1))
+    lines.append("$EndPhysicalNames")
+    lines += ["$Entities", "0 %d 1 0" % nb_names]
+    for k in range(nb_names):
+        lines.append("%d 0 0 0 0 0 0 1 %d 0" % (k + 1, k + 1))
+    lines.append("1 0 0 0 0 0 0 1 %d 0" % (nb_names + 1))
+    lines.append("$EndEntities")
+    lines += ["$Nodes", "1 %d 1 %d" % (n_nodes, n_nodes), "2 1 0 %d" % n_nodes]
+    body = [("\n".join(str(i) for i in range(1, n_nodes + 1)))] if n_nodes else []
+    coords = "\n".join("%.17g %.17g 0" % (a, b) for a, b in zip(m.x.tolist(), m.y.tolist()))
+    # element blocks: one per contiguous run of boundary tag, then cells by type run
+    blocks = []
+    tag = 1
+    b = 0
+    while b < m.n_bnd:
+        e = b
+        while e < m.n_bnd and m.bnd_tag[e] == m.bnd_tag[b]:
+            e += 1
+        rows = ["%d %d %d" % (tag + r, m.bnd[b + r, 0] + 1, m.bnd[b + r, 1] + 1) for r in range(e - b)]
+        blocks.append("1 %d 1 %d\n" % (int(m.bnd_tag[b]) + 1, e - b) + "\n".join(rows))
+        tag += e - b
+        b = e
+    c = 0
+    is_tri = m.cells[:, 3] < 0
+    while c < m.n_cells:
+        e = c
+        while e < m.n_cells and is_tri[e] == is_tri[c]:
+            e += 1
+        if is_tri[c]:
+            rows = ["%d %d %d %d" % (tag + r, m.cells[c + r, 0] + 1, m.cells[c + r, 1] + 1,
+                                     m.cells[c + r, 2] + 1) for r in range(e - c)]
+            blocks.append("2 1 2 %d\n" % (e - c) + "\n".join(rows))
+        else:
+            rows = ["%d %d %d %d %d" % (tag + r, m.cells[c + r, 0] + 1, m.cells[c + r, 1] + 1,
+                                        m.cells[c + r, 2] + 1, m.cells[c + r, 3] + 1)
+                    for r in range(e - c)]
+            blocks.append("2 1 3 %d\n" % (e - c) + "\n".join(rows))
+        tag += e - c
+        c = e
+    n_el = m.n_bnd + m.n_cells
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+        if n_nodes:
+            f.write(body[0] + "\n" + coords + "\n")
+        f.write("$EndNodes\n")
+        f.write("$Elements\n%d %d 1 %d\n" % (len(blocks), n_el, n_el))
+        for blk in blocks:
+            f.write(blk + "\n")
+        f.write("$EndElements\n")

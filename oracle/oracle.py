@@ -1,0 +1,325 @@
+"""Python access to the CPU checkers: the C restatement (liboracle.so) and the
+compiled reference harness (_ref/fvh_ref).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product package.
+
+Solver configuration is a plain dict shared with the product's Python layer:
+  flux   'roe_split' (shockTube/naca association) | 'roe_joined' (forwardStep)
+  phi    'zero' | 'michalak' | 'venkat' | 'minmod'   (+ phi_yt for Michalak)
+  dt     'fixed' | 'sum_cfl' | 'edge_min'           (+ dt_fixed, cfl)
+  grad, lim, lin, gdt   the solver:: flags (explicit.h:35-40)
+  K      limiter_k_value (explicit.cpp:27 fixes 7.5)
+  gamma, alpha (list of RK stage coefficients)
+  bcs    {physical name: (kind, rho, u, v, p)}, kind in
+         'copy' | 'mirror' | 'inlet_outlet' | 'farfield'
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+from typing import Dict, List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "fvh_ref")
+
+FLUX = {"roe_split": 0, "roe_joined": 1}
+PHI = {"zero": 0, "michalak": 1, "venkat": 2, "minmod": 3}
+DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2}
+BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3}
+RK5 = [0.0695, 0.1602, 0.2898, 0.506, 1.0]
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [
+        ("flux", C.c_int), ("phi", C.c_int), ("phi_yt", C.c_double),
+        ("dt_kind", C.c_int), ("dt_fixed", C.c_double), ("cfl", C.c_double),
+        ("grad", C.c_int), ("lim", C.c_int), ("lin", C.c_int), ("gdt", C.c_int),
+        ("K", C.c_double), ("gamma", C.c_double),
+        ("n_alpha", C.c_int), ("alpha", C.c_double * 8),
+        ("n_bc", C.c_int), ("bc_kind", C.c_int * 16), ("bc_param", (C.c_double * 4) * 16),
+    ]
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+        ("n_real", C.c_int32), ("n_cells", C.c_int32), ("n_edges", C.c_int32), ("n_bnd", C.c_int32),
+        ("edge_nodes", C.POINTER(C.c_int32)), ("edge_cells", C.POINTER(C.c_int32)),
+        ("edge_len", C.POINTER(C.c_double)), ("edge_nx", C.POINTER(C.c_double)),
+        ("edge_ny", C.POINTER(C.c_double)), ("edge_cx", C.POINTER(C.c_double)),
+        ("edge_cy", C.POINTER(C.c_double)),
+        ("cell_nodes", C.POINTER(C.c_int32)), ("cell_tri", C.POINTER(C.c_uint8)),
+        ("cell_area", C.POINTER(C.c_double)), ("cell_cx", C.POINTER(C.c_double)),
+        ("cell_cy", C.POINTER(C.c_double)),
+        ("bnd_edge", C.POINTER(C.c_int32)), ("bnd_bc", C.POINTER(C.c_int32)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle oracle)")
+        L = C.CDLL(LIB_PATH)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.orc_mesh_build.restype = C.POINTER(OrcMesh)
+        L.orc_mesh_build.argtypes = [C.c_int32, dp, dp, C.c_int32, ip, C.c_int32, ip, ip,
+                                     C.POINTER(C.c_int)]
+        L.orc_mesh_free.argtypes = [C.POINTER(OrcMesh)]
+        cfgp = C.POINTER(OrcConfig)
+        mp = C.POINTER(OrcMesh)
+        L.orc_calc_dt.argtypes = [mp, cfgp, dp, dp]
+        L.orc_min_dt.argtypes = [mp, dp]
+        L.orc_calc_gradients.argtypes = [mp, dp, dp, dp]
+        L.orc_calc_limiters.argtypes = [mp, cfgp, dp, dp, dp, dp, dp, dp]
+        L.orc_calc_time_derivatives.argtypes = [mp, cfgp, dp, dp, dp, dp, dp]
+        L.orc_update_cells.argtypes = [C.c_int64, dp, dp, dp, dp, C.c_double]
+        L.orc_update_bounds.argtypes = [mp, cfgp, dp, dp, dp, dp]
+        L.orc_calc_residuals.argtypes = [mp, dp, dp]
+        L.orc_flux.argtypes = [cfgp, dp, dp, dp, dp]
+        L.orc_phi.argtypes = [cfgp, C.c_double]
+        L.orc_phi.restype = C.c_double
+        L.orc_bc.argtypes = [cfgp, C.c_int, dp, dp, dp]
+        L.orc_run.argtypes = [mp, cfgp, dp, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+                              C.POINTER(C.c_uint32), C.POINTER(C.c_double), dp, dp, C.c_void_p]
+        L.orc_run.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def make_config(cfg: Dict, names: List[str]) -> OrcConfig:
+    """OrcConfig from the shared dict; BC slots follow ``names`` order."""
+    c = OrcConfig()
+    c.flux = FLUX[cfg.get("flux", "roe_split")]
+    c.phi = PHI[cfg.get("phi", "zero")]
+    c.phi_yt = float(cfg.get("phi_yt", 2.0))
+    c.dt_kind = DT[cfg.get("dt", "fixed")]
+    c.dt_fixed = float(cfg.get("dt_fixed", 2e-5))
+    c.cfl = float(cfg.get("cfl", 1.0))
+    c.grad = int(bool(cfg.get("grad", False)))
+    c.lim = int(bool(cfg.get("lim", False)))
+    c.lin = int(bool(cfg.get("lin", False)))
+    c.gdt = int(bool(cfg.get("gdt", False)))
+    c.K = float(cfg.get("K", 7.5))
+    c.gamma = float(cfg.get("gamma", 1.4))
+    alpha = list(cfg.get("alpha", RK5))
+    c.n_alpha = len(alpha)
+    for i, a in enumerate(alpha):
+        c.alpha[i] = a
+    bcs = cfg["bcs"]
+    c.n_bc = len(names)
+    for s, nm in enumerate(names):
+        if nm not in bcs:
+            raise KeyError("Physical name %s not in the boundary table" % nm)
+        kind, *par = bcs[nm]
+        c.bc_kind[s] = BC[kind]
+        par = list(par) + [0.0] * (4 - len(par))
+        for k in range(4):
+            c.bc_param[s][k] = float(par[k])
+    return c
+
+
+def _arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Mesh:
+    """Oracle mesh in the reference numbering, built from file-order arrays."""
+
+    def __init__(self, m):
+        self.arrays = m
+        L = lib()
+        err = C.c_int(0)
+        x = np.ascontiguousarray(m.x, dtype=np.float64)
+        y = np.ascontiguousarray(m.y, dtype=np.float64)
+        cells = np.ascontiguousarray(m.cells, dtype=np.int32).ravel()
+        bnd = np.ascontiguousarray(m.bnd, dtype=np.int32).ravel()
+        tags = np.ascontiguousarray(m.bnd_tag, dtype=np.int32)
+        p = L.orc_mesh_build(x.size, x, y, m.cells.shape[0], cells, m.bnd.shape[0], bnd, tags,
+                             C.byref(err))
+        if not p:
+            raise ValueError("invalid edge ref: boundary element not on a cell edge")
+        self._p = p
+        s = p.contents
+        self.n_real, self.n_cells, self.n_edges, self.n_bnd = s.n_real, s.n_cells, s.n_edges, s.n_bnd
+        E, N = s.n_edges, s.n_cells
+        self.edge_nodes = _arr(s.edge_nodes, 2 * E, np.int32).reshape(E, 2)
+        self.edge_cells = _arr(s.edge_cells, 2 * E, np.int32).reshape(E, 2)
+        self.edge_len = _arr(s.edge_len, E, np.float64)
+        self.edge_nx = _arr(s.edge_nx, E, np.float64)
+        self.edge_ny = _arr(s.edge_ny, E, np.float64)
+        self.edge_cx = _arr(s.edge_cx, E, np.float64)
+        self.edge_cy = _arr(s.edge_cy, E, np.float64)
+        self.cell_nodes = _arr(s.cell_nodes, 4 * N, np.int32).reshape(N, 4)
+        self.cell_tri = _arr(s.cell_tri, N, np.uint8)
+        self.cell_area = _arr(s.cell_area, N, np.float64)
+        self.cell_cx = _arr(s.cell_cx, N, np.float64)
+        self.cell_cy = _arr(s.cell_cy, N, np.float64)
+        self.bnd_edge = _arr(s.bnd_edge, s.n_bnd, np.int32)
+        self.names = list(m.names)
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(self._p)
+        except Exception:
+            pass
+
+    # -- phases ----------------------------------------------------------
+    def phases(self, cfg: Dict, q: np.ndarray, alpha: float = 0.5) -> Dict[str, np.ndarray]:
+        """One stage of every phase from state ``q`` (AoS, 4*n_cells):
+        the 'phases' mode of the reference harness."""
+        L = lib()
+        c = make_config(cfg, self.names)
+        q = np.ascontiguousarray(q, dtype=np.float64).copy()
+        N = self.n_cells
+        n = 4 * N
+        out = {}
+        dt = np.zeros(N)
+        L.orc_calc_dt(self._p, C.byref(c), q, dt)
+        out["dt_local"] = dt.copy()
+        if c.gdt:
+            L.orc_min_dt(self._p, dt)
+        out["dt"] = dt
+        gx, gy = np.zeros(n), np.zeros(n)
+        if c.grad:
+            L.orc_calc_gradients(self._p, q, gx, gy)
+        out["gx"], out["gy"] = gx, gy
+        lim, qmin, qmax = np.zeros(n), np.zeros(n), np.zeros(n)
+        if c.lim:
+            L.orc_calc_limiters(self._p, C.byref(c), q, gx, gy, lim, qmin, qmax)
+        out["lim"], out["qmin"], out["qmax"] = lim, qmin, qmax
+        qt = np.zeros(n)
+        L.orc_calc_time_derivatives(self._p, C.byref(c), q, gx, gy, lim, qt)
+        out["qt"] = qt
+        qk = np.zeros(n)
+        L.orc_update_cells(N, qk, q, qt, dt, float(alpha))
+        out["qk_cells"] = qk.copy()
+        L.orc_update_bounds(self._p, C.byref(c), qk, gx, gy, lim)
+        out["qk"] = qk
+        R = np.zeros(4)
+        L.orc_calc_residuals(self._p, qt, R)
+        out["R"] = R
+        return out
+
+    def run(self, cfg: Dict, q0: np.ndarray, max_step: int, max_time: float = 1e10,
+            tol: float = 1e-16, print_interval: int = 1):
+        """run() of explicit.cpp:657-817 from initial state q0 (all cells).
+        Returns (q, steps, time, R, R0, qt)."""
+        L = lib()
+        c = make_config(cfg, self.names)
+        q = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        steps = C.c_uint32(0)
+        t = C.c_double(0.0)
+        R = np.zeros(4)
+        R0 = np.zeros(4)
+        qt = np.zeros_like(q)
+        rc = L.orc_run(self._p, C.byref(c), q, int(max_step), float(max_time), float(tol),
+                       int(print_interval), C.byref(steps), C.byref(t), R, R0,
+                       qt.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise MemoryError("orc_run failed")
+        return q, int(steps.value), float(t.value), R, R0, qt
+
+
+def flux(cfg: Dict, qi, qj, n) -> np.ndarray:
+    c = make_config(dict(cfg, bcs={}), [])
+    f = np.zeros(4)
+    lib().orc_flux(C.byref(c), f, np.ascontiguousarray(qi, dtype=np.float64),
+                   np.ascontiguousarray(qj, dtype=np.float64), np.ascontiguousarray(n, dtype=np.float64))
+    return f
+
+
+def phi(cfg: Dict, r: float) -> float:
+    c = make_config(dict(cfg, bcs={}), [])
+    return float(lib().orc_phi(C.byref(c), float(r)))
+
+
+def bc(kind_tuple, gamma: float, qint, n) -> np.ndarray:
+    c = make_config({"gamma": gamma, "bcs": {"b": kind_tuple}}, ["b"])
+    b = np.zeros(4)
+    lib().orc_bc(C.byref(c), 0, b, np.ascontiguousarray(qint, dtype=np.float64),
+                 np.ascontiguousarray(n, dtype=np.float64))
+    return b
+
+
+# ---------------------------------------------------------------------------
+# Compiled reference harness (_ref/fvh_ref)
+# ---------------------------------------------------------------------------
+
+def read_dump(path: str) -> Dict[str, np.ndarray]:
+    out = {}
+    with open(path, "rb") as f:
+        data = f.read()
+    p = 0
+    while p < len(data):
+        (ln,) = struct.unpack_from("<I", data, p)
+        p += 4
+        name = data[p:p + ln].decode()
+        p += ln
+        typ = chr(data[p])
+        p += 1
+        (n,) = struct.unpack_from("<Q", data, p)
+        p += 8
+        if typ == "d":
+            out[name] = np.frombuffer(data, dtype="<f8", count=n, offset=p).copy()
+            p += 8 * n
+        else:
+            out[name] = np.frombuffer(data, dtype="<u4", count=n, offset=p).copy()
+            p += 4 * n
+    return out
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(mode: str, workdir: str, mesh_name: str, flags: str = "", timeout: float = 600,
+            **kv) -> Dict:
+    """Run the reference harness in ``workdir`` (which holds mesh_name.msh).
+    Returns the dump dict plus 'stdout'."""
+    out = os.path.join(workdir, mesh_name + "." + mode + ".dump")
+    args = [REF_BIN, mode, mesh_name, out] + ["%s=%s" % (k, v) for k, v in kv.items()]
+    env = dict(os.environ)
+    env["FVH_REF_FLAGS"] = flags
+    r = subprocess.run(args, cwd=workdir, env=env, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError("fvh_ref failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    d = read_dump(out)
+    d["stdout"] = r.stdout
+    return d
+
+
+def ref_args(cfg: Dict) -> Dict[str, str]:
+    """Map a config dict onto the harness's plugin selection.  Only the
+    combinations the reference examples provide are expressible."""
+    kv = {}
+    kv["flux"] = {"roe_split": "st", "roe_joined": "fs"}[cfg.get("flux", "roe_split")]
+    kv["dt"] = {"fixed": "st", "sum_cfl": "fs", "edge_min": "nc"}[cfg.get("dt", "fixed")]
+    phi_ = cfg.get("phi", "zero")
+    kv["phi"] = {"zero": "zero", "michalak": "fs", "venkat": "venk", "minmod": "minmod"}[phi_]
+    if phi_ == "michalak" and float(cfg.get("phi_yt", 2.0)) != 2.0:
+        raise ValueError("the reference's Michalak limiter fixes y_t = 2")
+    if cfg.get("dt", "fixed") == "fixed" and float(cfg.get("dt_fixed", 2e-5)) != 2e-5:
+        raise ValueError("the reference's fixed dt is 2e-5")
+    if float(cfg.get("K", 7.5)) != 7.5:
+        raise ValueError("the reference fixes K = 7.5")
+    kv["gamma"] = repr(float(cfg.get("gamma", 1.4)))
+    kv["cfl_fs"] = repr(float(cfg.get("cfl", 1.5)))
+    kv["cfl_nc"] = repr(float(cfg.get("cfl", 1.0)))
+    return kv
+
+
+def ref_flags(cfg: Dict) -> str:
+    return ",".join(k for k in ("grad", "lim", "lin", "gdt") if cfg.get(k, False))
