@@ -1,0 +1,46 @@
+// fvh_internal.h -- host-side types shared by the mesh builder and the solver.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fvh.h"
+
+// Mesh in the reference's numbering and geometry (fvhyper/mesh.h:136-206),
+// single device: no MPI ghost cells, no halo maps.
+struct fvh_mesh {
+    std::vector<double> x, y;                 // nodesX, nodesY (file order)
+    int32_t n_real = 0;                       // nRealCells
+    int32_t n_cells = 0;                      // real + boundary ghost cells
+    std::vector<int32_t> edge_nodes;          // edgesNodes, 2 per edge
+    std::vector<int32_t> edge_cells;          // edgesCells, 2 per edge
+    std::vector<double> e_len, e_nx, e_ny, e_cx, e_cy;
+    std::vector<int32_t> cell_nodes;          // cellsNodes, 4 per cell
+    std::vector<uint8_t> cell_tri;            // cellsIsTriangle
+    std::vector<double> c_area, c_cx, c_cy;   // cellsAreas, cellsCenters
+    std::vector<int32_t> bnd_edge;            // boundaryEdges
+    std::vector<int32_t> bnd_tag;             // index into names per boundary element
+    std::vector<std::string> names;           // physical names of boundary groups
+    std::vector<int32_t> perm;                // device position of each reference cell (RCM), lazily
+    int32_t n_edges() const { return (int32_t)e_len.size(); }
+};
+
+namespace fvh {
+
+void set_error(const std::string& msg);
+
+// Builds reference numbering + geometry from file-order arrays.  Returns an
+// FVH_* code; on error the message is set.
+int32_t build_mesh(fvh_mesh& m, int32_t n_nodes, const double* x, const double* y,
+                   int32_t n_cells, const int32_t* cells, int32_t n_bnd,
+                   const int32_t* bnd, const int32_t* bnd_tag,
+                   const std::vector<std::string>& names);
+
+int32_t read_msh(const char* path, fvh_mesh& m);
+
+// Reverse Cuthill-McKee order of the real cells: order[k] = reference cell
+// placed at device position k.  Deterministic (ties by cell index).
+std::vector<int32_t> rcm_order(const fvh_mesh& m);
+
+}  // namespace fvh

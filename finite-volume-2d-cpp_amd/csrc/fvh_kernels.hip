@@ -1,0 +1,504 @@
+// fvh_kernels.hip -- gfx950 kernels of the explicit residual-and-update path.
+// Built with -ffp-contract=off: no multiply-add is fused, so each FP64
+// expression rounds exactly as the reference's x86-64 build does.
+#include "fvh_kernels.h"
+
+#include <float.h>
+
+namespace fvh {
+
+namespace {
+
+constexpr double kInf = __builtin_inf();
+
+__device__ inline bool stopped(const Ctrl* ctrl) { return ctrl != nullptr && ctrl->stop != 0; }
+
+// NaN-ignoring minimum over a 256-thread block (std::min(a,b) keeps a unless
+// b < a, so NaN never wins once a finite value is held).
+__device__ inline double block_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_xor(v, off, 64));
+    __shared__ double part[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    double r = part[0];
+    for (int w = 1; w < kBlock / 64; ++w) r = smin(r, part[w]);
+    return r;
+}
+
+// Fixed-pattern sum of 4 values over a 256-thread block (deterministic).
+__device__ inline void block_sum4(double* v) {
+    for (int k = 0; k < 4; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    __shared__ double part[kBlock / 64][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) part[wave][k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+        double s = part[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) s += part[w][k];
+        v[k] = s;
+    }
+}
+
+template <class T>
+__device__ inline Q4<T> ld(const Q4<T>* p) { return *p; }
+
+// ---------------------------------------------------------------------------
+// time step: calc_dt kinds + block minimum for the global-dt reduction
+// ---------------------------------------------------------------------------
+template <class T, int KIND>
+__global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, PhysParams pp,
+                                               const Q4<T>* __restrict__ q, T* __restrict__ dt,
+                                               double* __restrict__ blk_min, const Ctrl* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    double v = kInf;
+    if (c < m.n_total) {
+        T r;
+        if (KIND == 0) {
+            r = T(pp.dt_fixed);
+        } else {
+            const T g = T(pp.gamma);
+            const Q4<T> qc = ld(q + c);
+            const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+            if (KIND == 1) {
+                // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184)
+                T s = T(0);
+                for (int h = h0; h < h1; ++h) {
+                    const uint32_t ue = m.he_edge[h];
+                    const Q4<T> geo = m.e_geo[ue & kEdgeMask & 0x3fffffffu];
+                    const Q4<T> qn = ld(q + m.he_nbr[h]);
+                    const bool side = (ue & kSideBit) != 0;
+                    const T* qi = side ? qn.v : qc.v;
+                    const T* qj = side ? qc.v : qn.v;
+                    const T ei = eig_sum(g, qi, geo.v[0], geo.v[1]);
+                    const T ej = eig_sum(g, qj, geo.v[0], geo.v[1]);
+                    s = s + smax(ei, ej) * geo.v[2];
+                }
+                r = T(pp.cfl) * m.c_area[c] / s;
+            } else {
+                // edge-minimum local form (naca/main.cpp:151-182)
+                r = T(1);
+                const T A = m.c_area[c];
+                for (int h = h0; h < h1; ++h) {
+                    const Q4<T> geo = m.e_geo[m.he_edge[h] & 0x3fffffffu];
+                    const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+                    const T d = T(pp.cfl) * (A / geo.v[2]) / sabs(eo);
+                    r = smin(r, d);
+                }
+            }
+        }
+        dt[c] = r;
+        v = (double)r;
+    }
+    v = block_min(v);
+    if (threadIdx.x == 0) blk_min[blockIdx.x] = v;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dt_min(const double* __restrict__ blk_min, int nblk,
+                                                   const T* __restrict__ dt_cells, int32_t dev0,
+                                                   T* __restrict__ dt_global, const Ctrl* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    double v = kInf;
+    for (int i = threadIdx.x; i < nblk; i += kBlock) v = smin(v, blk_min[i]);
+    v = block_min(v);
+    if (threadIdx.x == 0) {
+        // min_dt starts from dt[0] (explicit.cpp:557): a NaN there survives
+        const T d0 = dt_cells[dev0];
+        dt_global[0] = (d0 != d0) ? d0 : T(v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green-Gauss gradient + neighbour min/max + limiter + reconstruction
+// increments, one pass per cell (explicit.cpp:112-258, 307-313)
+// ---------------------------------------------------------------------------
+template <class T, int PHI>
+__global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp, int grad, int lim_on,
+                                                    int lin, const Q4<T>* __restrict__ q,
+                                                    Q4<T>* __restrict__ incr, Q4<T>* __restrict__ dbg_gx,
+                                                    Q4<T>* __restrict__ dbg_gy, Q4<T>* __restrict__ dbg_lim,
+                                                    const Ctrl* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    const Q4<T> qc = ld(q + c);
+    const G2<T> cc = m.c_c[c];
+    T gx[4] = {T(0), T(0), T(0), T(0)};
+    T gy[4] = {T(0), T(0), T(0), T(0)};
+    // limiters are reset to 1 when computed (explicit.cpp:184-186) and stay
+    // zero-initialised otherwise (explicit.cpp:673)
+    const T lim0 = lim_on ? T(1) : T(0);
+    T lim[4] = {lim0, lim0, lim0, lim0};
+    if (c < m.n_real) {
+        if (grad) {
+            for (int h = h0; h < h1; ++h) {
+                const int nb = m.he_nbr[h];
+                if (nb == c) continue;   // the reference skips i == j edges
+                const uint32_t ue = m.he_edge[h];
+                const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
+                const Q4<T> qn = ld(q + nb);
+                const bool side = (ue & kSideBit) != 0;
+                const T* qi = side ? qn.v : qc.v;
+                const T* qj = side ? qc.v : qn.v;
+                const T w = geo.v[3];
+                const T w1 = T(1) - w;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T f = (qi[k] * w1 + qj[k] * w) * geo.v[2];
+                    if (!side) {
+                        gx[k] = gx[k] + f * geo.v[0];
+                        gy[k] = gy[k] + f * geo.v[1];
+                    } else {
+                        gx[k] = gx[k] - f * geo.v[0];
+                        gy[k] = gy[k] - f * geo.v[1];
+                    }
+                }
+            }
+            const T ia = m.c_inv_area[c];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                gx[k] = gx[k] * ia;
+                gy[k] = gy[k] * ia;
+            }
+        }
+        if (lim_on) {
+            T qmn[4], qmx[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
+            for (int h = h0; h < h1; ++h) {
+                const Q4<T> qn = ld(q + m.he_nbr[h]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    qmn[k] = smin(qmn[k], qn.v[k]);
+                    qmx[k] = smax(qmx[k], qn.v[k]);
+                }
+            }
+            const T Ka = T(pp.K) * m.c_sqrt_area[c];
+            const T K3a = Ka * Ka * Ka;
+            const T yt = T(pp.phi_yt);
+            const T tol = T(1e-15);
+            for (int h = h0; h < h1; ++h) {
+                const G2<T> ec = m.e_c[m.he_edge[h] & 0x3fffffffu];
+                const T dx = ec.v[0] - cc.v[0];
+                const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T dqg = gx[k] * dx + gy[k] * dy;
+                    const T dmax = qmx[k] - qc.v[k];
+                    const T dmin = qmn[k] - qc.v[k];
+                    const T spread = (dmax - dmin) * (dmax - dmin);
+                    T sig;
+                    if (spread <= K3a) {
+                        sig = T(1);
+                    } else if (spread <= T(2) * K3a) {
+                        const T y = (spread / K3a - T(1));
+                        sig = T(2) * y * y * y - T(3) * y * y + T(1);
+                    } else {
+                        sig = T(0);
+                    }
+                    T l = T(1);
+                    if (sig < T(1)) {
+                        if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
+                        else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+                    }
+                    l = sig + (T(1) - sig) * l;
+                    lim[k] = smin(lim[k], l);
+                }
+            }
+        }
+    }
+    if (dbg_gx) {
+        Q4<T> a, b, d;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
+        dbg_gx[c] = a;
+        dbg_gy[c] = b;
+        dbg_lim[c] = d;
+    }
+    if (lin) {
+        for (int h = h0; h < h1; ++h) {
+            const G2<T> ec = m.e_c[m.he_edge[h] & 0x3fffffffu];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+            Q4<T> inc;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
+            incr[h] = inc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Edge flux gather + RK stage update + boundary update (explicit.cpp:261-417)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBcBit = 0x40000000u;
+
+template <class T, bool JOINED, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_flux(MeshView<T> m, PhysParams pp, BcTable<T> bcs,
+                                                 const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
+                                                 Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ incr,
+                                                 const T* __restrict__ dt, int dt_stride, T alpha,
+                                                 double* __restrict__ res_blk, Q4<T>* __restrict__ dbg_qt,
+                                                 const Ctrl* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const T g = T(pp.gamma);
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < m.n_real) {
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const Q4<T> qc = ld(qin + c);
+        const T A = m.c_area[c];
+        T qt[4] = {T(0), T(0), T(0), T(0)};
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
+            const Q4<T> qn = ld(qin + m.he_nbr[h]);
+            T a[4], b[4];
+            if (LIN) {
+                const Q4<T> ia = incr[h];
+                const Q4<T> ib = incr[m.he_twin[h]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    a[k] = qc.v[k] + ia.v[k];
+                    b[k] = qn.v[k] + ib.v[k];
+                }
+                if (pp.positivity) {
+                    if (!(a[0] > T(0)) || !(pressure(g, a) > T(0)))
+                        for (int k = 0; k < 4; ++k) a[k] = qc.v[k];
+                    if (!(b[0] > T(0)) || !(pressure(g, b) > T(0)))
+                        for (int k = 0; k < 4; ++k) b[k] = qn.v[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { a[k] = qc.v[k]; b[k] = qn.v[k]; }
+            }
+            const bool side = (ue & kSideBit) != 0;
+            T f[4];
+            roe_flux<T, JOINED>(g, f, side ? b : a, side ? a : b, geo.v[0], geo.v[1]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T t = f[k] * geo.v[2] / A;
+                qt[k] = side ? qt[k] + t : qt[k] - t;
+            }
+        }
+        const T d = dt[(int64_t)c * dt_stride];
+        const Q4<T> qs = ld(q0 + c);
+        Q4<T> out;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
+        qout[c] = out;
+        if (dbg_qt) {
+            Q4<T> t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
+            dbg_qt[c] = t;
+        }
+        if (res_blk) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) rs[k] = (double)qt[k] * (double)qt[k] * (double)A;
+        }
+        // ghost cells of this cell's boundary edges (update_bounds, :380-417)
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            if (!(ue & kBcBit)) continue;
+            const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
+            T qi[4];
+            if (LIN) {
+                const Q4<T> ia = incr[h];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + ia.v[k];
+                if (pp.positivity && (!(qi[0] > T(0)) || !(pressure(g, qi) > T(0))))
+                    for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+            }
+            Q4<T> gs;
+            boundary(bcs.p[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
+            qout[m.he_nbr[h]] = gs;
+        }
+    }
+    if (res_blk) {
+        block_sum4(rs);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) res_blk[4 * blockIdx.x + k] = rs[k];
+    }
+}
+
+// Initial boundary update with zero gradients and limiters (explicit.cpp:712).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParams pp, int lin,
+                                                        BcTable<T> bcs, Q4<T>* __restrict__ q) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_real) return;
+    const T g = T(pp.gamma);
+    const Q4<T> qc = ld(q + c);
+    const G2<T> cc = m.c_c[c];
+    for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        if (!(ue & kBcBit)) continue;
+        const int e = ue & 0x3fffffffu;
+        const Q4<T> geo = m.e_geo[e];
+        T qi[4];
+        if (lin) {
+            const G2<T> ec = m.e_c[e];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+            const T zero = T(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qi[k] = qc.v[k] + (zero * dx + zero * dy) * zero;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qi[k] = qc.v[k];
+        }
+        Q4<T> gs;
+        boundary(bcs.p[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
+        q[m.he_nbr[h]] = gs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// run() bookkeeping on the device (explicit.cpp:715-801)
+// ---------------------------------------------------------------------------
+__global__ void k_ctrl_begin(Ctrl* ctrl, uint32_t max_step, double max_time, double tol) {
+    if (ctrl->stop) return;
+    double Rmax = 0.0;
+    if (ctrl->step > 0) {
+        for (int k = 0; k < 4; ++k) Rmax = smax(ctrl->R[k], Rmax);
+    } else {
+        Rmax = 1.0;
+    }
+    if ((ctrl->step >= max_step) | (ctrl->time >= max_time) | (Rmax < tol)) ctrl->stop = 1;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ctrl_end(Ctrl* ctrl, const double* __restrict__ res_blk, int nblk,
+                                                     int need_res, const T* __restrict__ dt_src, int32_t dt_index) {
+    if (ctrl->stop) return;
+    if (need_res) {
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < nblk; i += kBlock)
+            for (int k = 0; k < 4; ++k) s[k] += res_blk[4 * i + k];
+        block_sum4(s);
+        if (threadIdx.x == 0) {
+            double R[4];
+            for (int k = 0; k < 4; ++k) R[k] = sqrt(s[k]);
+            if (ctrl->step == 0) {
+                for (int k = 0; k < 4; ++k) { ctrl->R0[k] = R[k]; ctrl->R[k] = R[k]; }
+            } else {
+                for (int k = 0; k < 4; ++k) ctrl->R[k] = R[k] / ctrl->R0[k];
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        const double d0 = (double)dt_src[dt_index];
+        ctrl->dt0 = d0;
+        ctrl->step += 1;
+        ctrl->time += d0;
+        for (int k = 0; k < 4; ++k)
+            if (!(ctrl->R[k] == ctrl->R[k])) ctrl->nonfinite = 1;
+    }
+}
+
+inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class T>
+void launch_dt(hipStream_t s, const MeshView<T>& m, int dt_kind, const PhysParams& pp,
+               const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
+    const int nb = nblocks(m.n_total);
+    if (nb == 0) return;
+    if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
+    else hipLaunchKernelGGL((k_dt<T, 2>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
+}
+
+void launch_dt_min(hipStream_t s, const double* blk_min, int nblk, const void* dt_cells, int dt_is_f32,
+                   int32_t dev0, double* dt_global, const Ctrl* ctrl) {
+    if (dt_is_f32)
+        hipLaunchKernelGGL((k_dt_min<float>), dim3(1), dim3(kBlock), 0, s, blk_min, nblk,
+                           (const float*)dt_cells, dev0, (float*)dt_global, ctrl);
+    else
+        hipLaunchKernelGGL((k_dt_min<double>), dim3(1), dim3(kBlock), 0, s, blk_min, nblk,
+                           (const double*)dt_cells, dev0, dt_global, ctrl);
+}
+
+template <class T>
+void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind, bool grad,
+                    bool lim, bool lin, const Q4<T>* q, Q4<T>* incr, Q4<T>* dbg_gx, Q4<T>* dbg_gy,
+                    Q4<T>* dbg_lim, const Ctrl* ctrl) {
+    const int nb = nblocks(m.n_total);
+    if (nb == 0) return;
+#define FVH_GL(P) hipLaunchKernelGGL((k_gradlim<T, P>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)grad, \
+                                     (int)lim, (int)lin, q, incr, dbg_gx, dbg_gy, dbg_lim, ctrl)
+    switch (phi_kind) {
+        case PHI_ZERO: FVH_GL(PHI_ZERO); break;
+        case PHI_MICHALAK: FVH_GL(PHI_MICHALAK); break;
+        case PHI_VENKAT: FVH_GL(PHI_VENKAT); break;
+        default: FVH_GL(PHI_MINMOD); break;
+    }
+#undef FVH_GL
+}
+
+template <class T>
+void launch_flux(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool joined, bool lin,
+                 const BcTable<T>& bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, const Q4<T>* incr,
+                 const T* dt, int dt_stride, const double* /*dt_global*/, T alpha, double* res_blk,
+                 Q4<T>* dbg_qt, const Ctrl* ctrl) {
+    const int nb = nblocks(m.n_real);
+    if (nb == 0) return;
+#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, pp, bcs, qin, q0, \
+                                        qout, incr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
+    if (joined) {
+        if (lin) FVH_FX(true, true); else FVH_FX(true, false);
+    } else {
+        if (lin) FVH_FX(false, true); else FVH_FX(false, false);
+    }
+#undef FVH_FX
+}
+
+template <class T>
+void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
+                        const BcTable<T>& bcs, Q4<T>* q) {
+    const int nb = nblocks(m.n_real);
+    if (nb == 0) return;
+    hipLaunchKernelGGL((k_bounds_init<T>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q);
+}
+
+void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol) {
+    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(1), 0, s, ctrl, max_step, max_time, tol);
+}
+
+void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,
+                     const void* dt_src, int dt_is_f32, int32_t dt_index) {
+    if (dt_is_f32)
+        hipLaunchKernelGGL((k_ctrl_end<float>), dim3(1), dim3(kBlock), 0, s, ctrl, res_blk, nblk, need_res,
+                           (const float*)dt_src, dt_index);
+    else
+        hipLaunchKernelGGL((k_ctrl_end<double>), dim3(1), dim3(kBlock), 0, s, ctrl, res_blk, nblk, need_res,
+                           (const double*)dt_src, dt_index);
+}
+
+#define FVH_INST(T)                                                                                         \
+    template void launch_dt<T>(hipStream_t, const MeshView<T>&, int, const PhysParams&, const Q4<T>*, T*,    \
+                               double*, const Ctrl*);                                                       \
+    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const PhysParams&, int, bool, bool,    \
+                                    bool, const Q4<T>*, Q4<T>*, Q4<T>*, Q4<T>*, Q4<T>*, const Ctrl*);       \
+    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool, bool,            \
+                                 const BcTable<T>&, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,       \
+                                 const T*, int, const double*, T, double*, Q4<T>*, const Ctrl*);            \
+    template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,           \
+                                        const BcTable<T>&, Q4<T>*);
+FVH_INST(double)
+FVH_INST(float)
+
+}  // namespace fvh

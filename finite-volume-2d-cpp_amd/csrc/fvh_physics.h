@@ -1,0 +1,213 @@
+// fvh_physics.h -- the per-edge, per-boundary and per-cell arithmetic that
+// fvhyper leaves to user plugins, as host+device functions selected by kind.
+//
+// The same source serves the GPU kernels and the host (plugin probing in the
+// drop-in layer).  With FP contraction off and IEEE division/sqrt on both
+// sides, the two give identical bits, which is what lets the FP64 path match
+// the reference CPU run exactly.  Expressions keep the association of the
+// reference plugins on purpose: a reordering that is equal in exact
+// arithmetic changes the last bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#ifndef FVH_HD
+#define FVH_HD __host__ __device__ inline
+#endif
+
+namespace fvh {
+
+// std::min / std::max as libstdc++ defines them (NaN and signed-zero
+// behaviour included): min(a,b) = (b<a)?b:a, max(a,b) = (a<b)?b:a.
+template <class T> FVH_HD T smin(T a, T b) { return (b < a) ? b : a; }
+template <class T> FVH_HD T smax(T a, T b) { return (a < b) ? b : a; }
+template <class T> FVH_HD T sabs(T a) { return fabs(a); }
+
+template <class T> struct Vec4 { T v[4]; };
+
+// Pressure of an ideal gas, calc_p (examples/forwardStep/main.cpp:37-39).
+template <class T>
+FVH_HD T pressure(T g, const T* q) {
+    return (g - T(1)) * (q[3] - T(0.5) / q[0] * (q[1] * q[1] + q[2] * q[2]));
+}
+
+// Roe flux-difference splitting with the "cures" entropy fix.
+//  joined=false: examples/shockTube/main.cpp:68-135 (also naca/main.cpp:71-137)
+//  joined=true : examples/forwardStep/main.cpp:72-135
+// The two differ only in how the central flux is associated.
+template <class T, bool JOINED>
+FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1) {
+    const T pi = pressure(g, qi);
+    const T pj = pressure(g, qj);
+    const T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
+    const T Vj = (qj[1] * n0 + qj[2] * n1) / qj[0];
+    if (JOINED) {
+        f[0] = (qi[0] * Vi + qj[0] * Vj) * T(0.5);
+        f[1] = (qi[1] * Vi + pi * n0 + qj[1] * Vj + pj * n0) * T(0.5);
+        f[2] = (qi[2] * Vi + pi * n1 + qj[2] * Vj + pj * n1) * T(0.5);
+        f[3] = ((qi[3] + pi) * Vi + (qj[3] + pj) * Vj) * T(0.5);
+    } else {
+        f[0] = qi[0] * Vi;
+        f[1] = qi[1] * Vi + pi * n0;
+        f[2] = qi[2] * Vi + pi * n1;
+        f[3] = (qi[3] + pi) * Vi;
+        f[0] = f[0] + qj[0] * Vj;
+        f[1] = f[1] + (qj[1] * Vj + pj * n0);
+        f[2] = f[2] + (qj[2] * Vj + pj * n1);
+        f[3] = f[3] + (qj[3] + pj) * Vj;
+        f[0] = f[0] * T(0.5);
+        f[1] = f[1] * T(0.5);
+        f[2] = f[2] * T(0.5);
+        f[3] = f[3] * T(0.5);
+    }
+    const T pL = pi, pR = pj;
+    const T uL = qi[1] / qi[0];
+    const T uR = qj[1] / qj[0];
+    const T vL = qi[2] / qi[0];
+    const T vR = qj[2] / qj[0];
+    const T srhoL = sqrt(qi[0]);
+    const T srhoR = sqrt(qj[0]);
+    const T rho = srhoR * srhoL;
+    const T den = srhoL + srhoR;
+    const T u = (uL * srhoL + uR * srhoR) / den;
+    const T v = (vL * srhoL + vR * srhoR) / den;
+    const T h = ((qi[3] + pL) / qi[0] * srhoL + (qj[3] + pR) / qj[0] * srhoR) / den;
+    const T q2 = u * u + v * v;
+    const T c = sqrt((g - T(1)) * (h - T(0.5) * q2));
+    const T V = u * n0 + v * n1;
+    const T VR = uR * n0 + vR * n1;
+    const T VL = uL * n0 + vL * n1;
+    const T lcm = sabs(smin(V - c, VL - c));
+    const T lc = sabs(V);
+    const T lcp = sabs(smax(V + c, VR + c));
+    const T dp = pR - pL;
+    const T dV = VR - VL;
+    const T rc = rho * c;
+    const T c2x2 = T(2) * c * c;
+    const T kF1 = lcm * (dp - rc * dV) / c2x2;
+    const T kF0 = lc * ((qj[0] - qi[0]) - dp / (c * c));
+    const T kFr = lc * rho;
+    const T kF5 = lcp * (dp + rc * dV) / c2x2;
+    f[0] = f[0] - T(0.5) * (kF1 + kF0 + kF5);
+    f[1] = f[1] - T(0.5) * (kF1 * (u - c * n0) + kF0 * u + kFr * (uR - uL - dV * n0) + kF5 * (u + c * n0));
+    f[2] = f[2] - T(0.5) * (kF1 * (v - c * n1) + kF0 * v + kFr * (vR - vL - dV * n1) + kF5 * (v + c * n1));
+    f[3] = f[3] - T(0.5) * (kF1 * (h - c * V) + kF0 * q2 * T(0.5) + kFr * (u * (uR - uL) + v * (vR - vL) - V * dV) + kF5 * (h + c * V));
+}
+
+// limiter_func kinds.
+//  zero:      examples/shockTube/main.cpp:60-62
+//  Michalak:  examples/forwardStep/main.cpp:57-66 (y_t a parameter, 2 there)
+//  minmod:    examples/burgers/main.cpp:51-53
+//  Venkatakrishnan: (r^2 + 2r)/(r^2 + r + 2); no reference example.
+enum PhiKind { PHI_ZERO = 0, PHI_MICHALAK = 1, PHI_VENKAT = 2, PHI_MINMOD = 3 };
+
+template <class T, int PHI>
+FVH_HD T phi(T y, T yt) {
+    if (PHI == PHI_ZERO) return T(0);
+    if (PHI == PHI_MICHALAK) {
+        if (y >= yt) return T(1);
+        const T a = T(1) / (yt * yt) - T(2) / (yt * yt * yt);
+        const T b = T(-1.5) * a * yt - T(0.5) / yt;
+        return a * y * y * y + b * y * y + y;
+    }
+    if (PHI == PHI_VENKAT) return (y * y + T(2) * y) / (y * y + y + T(2));
+    return smax(T(0), smin(T(1), y));
+}
+
+// Boundary-condition kinds.  b receives the ghost state, q is the interior
+// face state, (n0, n1) the outward unit normal.
+//  copy:    examples/shockTube/main.cpp:158-163
+//  mirror:  examples/forwardStep/main.cpp:262-269 (= naca/main.cpp:260-267)
+//  inlet/outlet characteristic: examples/forwardStep/main.cpp:194-261
+//  farfield characteristic:     examples/naca/main.cpp:192-259
+// Both characteristic kinds keep the reference's b[2] = b[1]*(...) spelling
+// in their subsonic branches; the freestream (rho, u, v, p) is a parameter.
+enum BcKind { BC_COPY = 0, BC_MIRROR = 1, BC_INLET_OUTLET = 2, BC_FARFIELD = 3 };
+
+template <class T>
+struct BcParam {
+    int kind;
+    T rho, u, v, p;
+};
+
+template <class T>
+FVH_HD void boundary(const BcParam<T>& P, T g, T* b, const T* q, T n0, T n1) {
+    if (P.kind == BC_COPY) {
+        b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[3];
+        return;
+    }
+    if (P.kind == BC_MIRROR) {
+        const T un = n0 * q[1] + n1 * q[2];
+        b[0] = q[0];
+        b[1] = q[1] - T(2) * n0 * un;
+        b[2] = q[2] - T(2) * n1 * un;
+        b[3] = q[3];
+        return;
+    }
+    T bv[4];
+    bv[0] = P.rho;
+    bv[1] = bv[0] * P.u;
+    bv[2] = bv[0] * P.v;
+    bv[3] = P.p / (g - T(1)) + T(0.5) / bv[0] * (bv[1] * bv[1] + bv[2] * bv[2]);
+    const T U0 = q[1] / q[0];
+    const T U1 = q[2] / q[0];
+    const T u_norm = sqrt(U0 * U0 + U1 * U1);
+    const T u_dot_n = U0 * n0 + U1 * n1;
+    const T p = pressure(g, q);
+    const T c = sqrt(g * p / q[0]);
+    const T mach = u_norm / c;
+    if (mach > T(1)) {
+        if (u_dot_n < T(0)) {
+            b[0] = bv[0]; b[1] = bv[1]; b[2] = bv[2]; b[3] = bv[3];
+        } else {
+            b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[3];
+        }
+        return;
+    }
+    const T pa = pressure(g, bv);
+    const T rhoa = bv[0];
+    const T ua = bv[1] / bv[0];
+    const T va = bv[2] / bv[0];
+    const T pd = p;
+    const T rhod = q[0];
+    const T ud = q[1] / q[0];
+    const T vd = q[2] / q[0];
+    const T rho0 = q[0];
+    const T c0 = c;
+    if (u_dot_n < T(0)) {
+        if (P.kind == BC_INLET_OUTLET) {
+            const T pb = pd;
+            b[0] = bv[0]; b[1] = bv[1]; b[2] = bv[2];
+            b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+        } else {
+            const T pb = T(0.5) * (pa + pd - rho0 * c0 * (n0 * (ua - ud) + n1 * (va - vd)));
+            b[0] = rhoa + (pb - pa) / (c0 * c0);
+            b[1] = b[0] * (ua - n0 * (pa - pb) / (rho0 * c0));
+            b[2] = b[1] * (va - n1 * (pa - pb) / (rho0 * c0));
+            b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+        }
+    } else {
+        const T pb = pa;
+        b[0] = rhod + (pb - pd) / (c0 * c0);
+        b[1] = b[0] * (ud + n0 * (pd - pb) / (rho0 * c0));
+        b[2] = b[1] * (va + n1 * (pd - pb) / (rho0 * c0));
+        b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+    }
+}
+
+// Per-edge wave-speed terms of the calc_dt kinds.
+//  summed CFL (forwardStep/main.cpp:167-171): c + |u.n|
+template <class T>
+FVH_HD T eig_sum(T g, const T* q, T n0, T n1) {
+    const T c = sqrt(pressure(g, q) * g / q[0]);
+    return c + sabs(q[1] / q[0] * n0 + q[2] / q[0] * n1);
+}
+//  edge-minimum local (naca/main.cpp:169-173): (c+|u|) n0 + (c+|v|) n1
+template <class T>
+FVH_HD T eig_edge(T g, const T* q, T n0, T n1) {
+    const T c = sqrt(pressure(g, q) * g / q[0]);
+    return (c + sabs(q[1] / q[0])) * n0 + (c + sabs(q[2] / q[0])) * n1;
+}
+
+}  // namespace fvh

@@ -1,0 +1,203 @@
+"""Python face of libfvh: Mesh and Solver over the C ABI.
+
+Mirrors fvhyper's operator surface for the explicit path: the mesh object
+(fvhyper/mesh.h:136-206 members, exported in reference order), the
+solverOptions/run pair (fvhyper/explicit.h:152-181) and the per-phase
+functions (explicit.h:88-129, here one stage of all of them at once).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+_INT_FIELDS = {"edgesNodes", "edgesCells", "cellsNodes", "cellsIsTriangle", "boundaryEdges",
+               "boundaryTags", "devicePermutation"}
+
+
+class Mesh:
+    """A mesh in the reference numbering (real cells, then boundary ghosts)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        L = _lib.load()
+        c = (C.c_int64 * 6)()
+        check(L.fvh_mesh_counts(self._h, c))
+        self.n_nodes, self.n_real, self.n_cells, self.n_edges, self.n_bnd, n_names = list(c)
+        buf = C.create_string_buffer(256)
+        self.names: List[str] = []
+        for k in range(n_names):
+            check(L.fvh_mesh_name(self._h, k, buf, 256))
+            self.names.append(buf.value.decode())
+
+    @classmethod
+    def read_msh(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        check(_lib.load().fvh_mesh_read_msh(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, m) -> "Mesh":
+        """From a meshgen.MeshArrays (file-order arrays)."""
+        L = _lib.load()
+        x = np.ascontiguousarray(m.x, dtype=np.float64)
+        y = np.ascontiguousarray(m.y, dtype=np.float64)
+        cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+        bnd = np.ascontiguousarray(m.bnd, dtype=np.int32).reshape(-1, 2)
+        tags = np.ascontiguousarray(m.bnd_tag, dtype=np.int32)
+        names = (C.c_char_p * max(1, len(m.names)))(*[n.encode() for n in m.names])
+        h = C.c_void_p()
+        check(L.fvh_mesh_from_arrays(x.size, dptr(x), dptr(y), cells.shape[0], iptr(cells), bnd.shape[0],
+                                     iptr(bnd), iptr(tags), len(m.names), names, C.byref(h)))
+        return cls(h)
+
+    def export(self, field: str) -> np.ndarray:
+        L = _lib.load()
+        n = check(L.fvh_mesh_export(self._h, field.encode(), None, 0))
+        dt = np.int32 if field in _INT_FIELDS else np.float64
+        out = np.zeros(n, dtype=dt)
+        check(L.fvh_mesh_export(self._h, field.encode(), out.ctypes.data_as(C.c_void_p), n))
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().fvh_mesh_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def make_config(cfg: Dict, precision: str = "f64", renumber: bool = True) -> _lib.Config:
+    """fvh_config from the shared configuration dict (see oracle/oracle.py)."""
+    c = _lib.Config()
+    _lib.load().fvh_config_default(C.byref(c))
+    c.precision = {"f64": 0, "f32": 1}[precision]
+    c.flux = _lib.FLUX[cfg.get("flux", "roe_split")]
+    c.gamma = float(cfg.get("gamma", 1.4))
+    c.phi = _lib.PHI[cfg.get("phi", "zero")]
+    c.phi_yt = float(cfg.get("phi_yt", 2.0))
+    c.K = float(cfg.get("K", 7.5))
+    c.dt_kind = _lib.DT[cfg.get("dt", "fixed")]
+    c.dt_fixed = float(cfg.get("dt_fixed", 2e-5))
+    c.cfl = float(cfg.get("cfl", 1.0))
+    flags = 0
+    if cfg.get("grad", False):
+        flags |= _lib.FLAG_GRADIENTS
+    if cfg.get("lim", False):
+        flags |= _lib.FLAG_LIMITERS
+    if cfg.get("lin", False):
+        flags |= _lib.FLAG_LINEAR
+    if cfg.get("gdt", False):
+        flags |= _lib.FLAG_GLOBAL_DT
+    if cfg.get("positivity", False):
+        flags |= _lib.FLAG_POSITIVITY
+    c.flags = flags
+    alpha = list(cfg.get("alpha", _lib.RK5))
+    if not 1 <= len(alpha) <= _lib.FVH_MAX_ALPHA:
+        raise ValueError("1..%d RK stages supported" % _lib.FVH_MAX_ALPHA)
+    c.n_alpha = len(alpha)
+    for i, a in enumerate(alpha):
+        c.alpha[i] = float(a)
+    bcs = cfg.get("bcs", {})
+    if len(bcs) > _lib.FVH_MAX_BC:
+        raise ValueError("at most %d boundary conditions" % _lib.FVH_MAX_BC)
+    c.n_bc = len(bcs)
+    for k, (name, spec) in enumerate(bcs.items()):
+        kind, *par = spec
+        par = list(par) + [0.0] * (4 - len(par))
+        c.bc[k].name = name.encode()
+        c.bc[k].kind = _lib.BC[kind]
+        c.bc[k].rho, c.bc[k].u, c.bc[k].v, c.bc[k].p = [float(v) for v in par[:4]]
+    c.renumber = 1 if renumber else 0
+    return c
+
+
+class Solver:
+    """Explicit FV solver on the current HIP device (one stream per handle)."""
+
+    def __init__(self, mesh: Mesh, cfg: Dict, precision: str = "f64", renumber: bool = True):
+        self.mesh = mesh
+        self.cfg = dict(cfg)
+        self.precision = precision
+        L = _lib.load()
+        c = make_config(cfg, precision, renumber)
+        h = C.c_void_p()
+        check(L.fvh_solver_create(mesh._h, C.byref(c), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().fvh_solver_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    @property
+    def n_state(self) -> int:
+        return 4 * self.mesh.n_cells
+
+    def set_state(self, q: np.ndarray) -> None:
+        q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+        if q.size != self.n_state:
+            raise ValueError("state holds %d values, mesh needs %d" % (q.size, self.n_state))
+        check(_lib.load().fvh_solver_set_state(self._h, dptr(q)))
+
+    def get_state(self) -> np.ndarray:
+        q = np.zeros(self.n_state)
+        check(_lib.load().fvh_solver_get_state(self._h, dptr(q)))
+        return q
+
+    def apply_bounds(self) -> None:
+        check(_lib.load().fvh_solver_apply_bounds(self._h))
+
+    def phases(self, alpha: float = 0.5) -> Dict[str, np.ndarray]:
+        n = self.n_state
+        out = {"dt": np.zeros(self.mesh.n_cells), "gx": np.zeros(n), "gy": np.zeros(n),
+               "lim": np.zeros(n), "qt": np.zeros(n), "qk": np.zeros(n)}
+        check(_lib.load().fvh_solver_phases(self._h, float(alpha), dptr(out["dt"]), dptr(out["gx"]),
+                                            dptr(out["gy"]), dptr(out["lim"]), dptr(out["qt"]),
+                                            dptr(out["qk"])))
+        return out
+
+    def run(self, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
+            print_interval: int = 1) -> Dict:
+        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol))
+        st = _lib.Status()
+        check(_lib.load().fvh_solver_run(self._h, C.byref(o), C.byref(st)))
+        return self._status_dict(st)
+
+    def advance(self, n: int) -> None:
+        check(_lib.load().fvh_solver_advance(self._h, int(n)))
+
+    def synchronize(self) -> None:
+        check(_lib.load().fvh_solver_synchronize(self._h))
+
+    def status(self) -> Dict:
+        st = _lib.Status()
+        check(_lib.load().fvh_solver_status(self._h, C.byref(st)))
+        return self._status_dict(st)
+
+    @staticmethod
+    def _status_dict(st) -> Dict:
+        return {"step": int(st.step), "stopped": bool(st.stopped), "time": float(st.time),
+                "R": np.array(list(st.R)), "R0": np.array(list(st.R0))}
+
+    def profile(self, enable: bool) -> None:
+        check(_lib.load().fvh_solver_profile(self._h, 1 if enable else 0))
+
+    def kernel_times(self) -> Dict[str, Dict[str, float]]:
+        L = _lib.load()
+        counts = (C.c_int64 * 16)()
+        ms = (C.c_double * 16)()
+        n = check(L.fvh_solver_kernel_times(self._h, 16, counts, ms))
+        return {L.fvh_solver_kernel_name(k).decode(): {"count": int(counts[k]), "total_ms": float(ms[k])}
+                for k in range(n)}
+
+    def stage_bytes(self) -> float:
+        return float(_lib.load().fvh_solver_stage_bytes(self._h))

@@ -1,0 +1,195 @@
+/*
+ * fvh.h -- C ABI of the MI355X (gfx950) explicit finite-volume library.
+ *
+ * One C-ABI shared library (libfvh.so) replaces fvhyper's explicit
+ * residual-and-update path on one GPU.  Plain pointers and sizes only; every
+ * entry point returns FVH_OK (0) or a negative FVH_E* code and leaves a
+ * message for fvh_last_error().  No C++ exception crosses this boundary.
+ *
+ * States cross the boundary in the reference layout: AoS doubles,
+ * q[4*cell + k], over the reference's cell numbering (real cells in file
+ * order, then one boundary ghost cell per boundary element,
+ * fvhyper/src/mesh.cpp:623-669).  The library renumbers cells internally
+ * (reverse Cuthill-McKee) and inverts the permutation on the way out.
+ *
+ * Threading: a solver owns one HIP stream; a handle is not thread-safe, two
+ * handles may be used from two threads.
+ */
+#ifndef FVH_H
+#define FVH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVH_ABI_VERSION 1
+
+enum {
+    FVH_OK = 0,
+    FVH_E_ARG = -1,       /* invalid argument */
+    FVH_E_IO = -2,        /* file cannot be read */
+    FVH_E_FORMAT = -3,    /* malformed or unsupported mesh file */
+    FVH_E_MESH = -4,      /* inconsistent mesh (boundary edge not in mesh, ...) */
+    FVH_E_BC = -5,        /* physical name without a boundary condition */
+    FVH_E_HIP = -6,       /* HIP runtime error */
+    FVH_E_UNSUPPORTED = -7,
+    FVH_E_STATE = -8      /* non-finite state detected */
+};
+
+/* Roe flux with the entropy cure; the two reference spellings differ in the
+ * association of the central flux, which changes the last bits. */
+enum { FVH_FLUX_ROE_SPLIT = 0,   /* examples/shockTube/main.cpp:68-135, naca/main.cpp:71-137 */
+       FVH_FLUX_ROE_JOINED = 1   /* examples/forwardStep/main.cpp:72-135 */ };
+enum { FVH_PHI_ZERO = 0, FVH_PHI_MICHALAK = 1, FVH_PHI_VENKATAKRISHNAN = 2, FVH_PHI_MINMOD = 3 };
+enum { FVH_DT_FIXED = 0,         /* shockTube/main.cpp:140-147 */
+       FVH_DT_SUM_CFL = 1,       /* forwardStep/main.cpp:140-185 */
+       FVH_DT_EDGE_MIN = 2       /* naca/main.cpp:142-183 */ };
+enum { FVH_BC_COPY = 0,          /* shockTube/main.cpp:158-163 */
+       FVH_BC_MIRROR = 1,        /* forwardStep/main.cpp:262-269 */
+       FVH_BC_INLET_OUTLET = 2,  /* forwardStep/main.cpp:194-261 */
+       FVH_BC_FARFIELD = 3       /* naca/main.cpp:192-259 */ };
+enum { FVH_F64 = 0, FVH_F32 = 1 };
+
+/* solver:: flags of fvhyper/explicit.h:35-40 */
+#define FVH_FLAG_GRADIENTS   1u   /* do_calc_gradients */
+#define FVH_FLAG_LIMITERS    2u   /* do_calc_limiters */
+#define FVH_FLAG_LINEAR      4u   /* linear_interpolate */
+#define FVH_FLAG_GLOBAL_DT   8u   /* global_dt */
+/* Extension without a reference counterpart: where a limited linear
+ * reconstruction yields a face state with rho <= 0 or p <= 0, that face uses
+ * the cell value instead (first order).  Off for reference parity. */
+#define FVH_FLAG_POSITIVITY 16u
+
+#define FVH_MAX_ALPHA 8
+#define FVH_MAX_BC 16
+#define FVH_NAME_LEN 64
+
+typedef struct fvh_bc_spec {
+    char name[FVH_NAME_LEN];   /* physical name in the mesh (boundaries::bounds key) */
+    int32_t kind;              /* FVH_BC_* */
+    double rho, u, v, p;       /* freestream of the characteristic kinds */
+} fvh_bc_spec;
+
+typedef struct fvh_config {
+    int32_t precision;         /* FVH_F64 (reference-exact) or FVH_F32 */
+    int32_t flux;              /* FVH_FLUX_* */
+    double gamma;
+    int32_t phi;               /* FVH_PHI_* (limiter_func) */
+    double phi_yt;             /* Michalak threshold, 2 in the reference */
+    double K;                  /* limiter_k_value, explicit.cpp:27 (7.5) */
+    int32_t dt_kind;           /* FVH_DT_* (calc_dt) */
+    double dt_fixed;
+    double cfl;
+    uint32_t flags;            /* FVH_FLAG_* */
+    int32_t n_alpha;           /* RK stage coefficients, explicit.cpp:681-687 */
+    double alpha[FVH_MAX_ALPHA];
+    int32_t n_bc;
+    fvh_bc_spec bc[FVH_MAX_BC];
+    int32_t renumber;          /* 1: reverse Cuthill-McKee device order (default) */
+    int32_t block_size;        /* 0: library default */
+} fvh_config;
+
+/* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */
+typedef struct fvh_run_options {
+    double max_time;
+    uint32_t max_step;
+    uint32_t print_interval;
+    double tolerance;
+} fvh_run_options;
+
+typedef struct fvh_status {
+    uint32_t step;             /* steps taken */
+    uint32_t stopped;          /* 1 once the stop test of explicit.cpp:718-726 held */
+    double time;               /* sum of dt[cell 0] over steps (explicit.cpp:800) */
+    double R[4];               /* residual, normalised by R0 after step 0 */
+    double R0[4];              /* step-0 residual (explicit.cpp:752-754) */
+} fvh_status;
+
+typedef struct fvh_mesh fvh_mesh;
+typedef struct fvh_solver fvh_solver;
+
+/* Message of the last failing call on this thread. */
+const char* fvh_last_error(void);
+int32_t fvh_abi_version(void);
+
+/* ---- mesh (replaces fvhyper::mesh::read_file, fvhyper/mesh.h:193,
+ *      fvhyper/src/mesh.cpp:803-855; single device, so no halo maps) ---- */
+
+/* Read a gmsh MSH 4.1 ASCII file (the subset of mesh.cpp:264-617). */
+int32_t fvh_mesh_read_msh(const char* path, fvh_mesh** out);
+
+/* Same mesh from file-order arrays: x,y per node; cells 4 node indices each
+ * (0-based, the 4th -1 for a triangle); bnd 2 node indices per boundary
+ * element with bnd_tag indexing names[]. */
+int32_t fvh_mesh_from_arrays(int32_t n_nodes, const double* x, const double* y,
+                             int32_t n_cells, const int32_t* cells,
+                             int32_t n_bnd, const int32_t* bnd, const int32_t* bnd_tag,
+                             int32_t n_names, const char* const* names, fvh_mesh** out);
+void fvh_mesh_free(fvh_mesh* m);
+
+/* counts[0..5] = nodes, real cells, cells incl. ghosts, edges, boundary
+ * elements, physical names */
+int32_t fvh_mesh_counts(const fvh_mesh* m, int64_t* counts);
+
+/* Copy one mesh member in reference order into dst (capacity n elements).
+ * double fields: nodesX nodesY edgesLengths edgesNormalsX edgesNormalsY
+ *   edgesCentersX edgesCentersY cellsAreas cellsCentersX cellsCentersY
+ * int32 fields: edgesNodes(2/edge) edgesCells(2/edge) cellsNodes(4/cell)
+ *   cellsIsTriangle boundaryEdges boundaryTags devicePermutation
+ * Returns the number of elements (dst may be NULL to query it). */
+int64_t fvh_mesh_export(const fvh_mesh* m, const char* field, void* dst, int64_t n);
+
+/* Physical name idx into buf (NUL-terminated). */
+int32_t fvh_mesh_name(const fvh_mesh* m, int32_t idx, char* buf, int32_t len);
+
+/* ---- solver (replaces fvhyper::run and its phases, fvhyper/explicit.h:88-181) */
+
+void fvh_config_default(fvh_config* c);
+
+/* Bind the BC table by physical name (error FVH_E_BC as mesh.cpp:662-664),
+ * upload geometry and allocate device state on the current HIP device. */
+int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** out);
+void fvh_solver_free(fvh_solver* s);
+
+/* State in the reference layout (4 doubles per cell, all cells). */
+int32_t fvh_solver_set_state(fvh_solver* s, const double* q);
+int32_t fvh_solver_get_state(fvh_solver* s, double* q);
+
+/* Fill ghost cells from boundary conditions with zero gradients
+ * (update_bounds(q, 0, 0, 0), explicit.cpp:712) and reset step/time. */
+int32_t fvh_solver_apply_bounds(fvh_solver* s);
+
+/* One stage of every phase from the current state, results in reference
+ * layout (explicit.cpp:729-747 with a single alpha); any output may be NULL.
+ * dt: 1 value per cell; the others 4 per cell.  The state is not changed. */
+int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, double* gy,
+                          double* lim, double* qt, double* qk);
+
+/* The run() loop of explicit.cpp:657-817 from the current state: applies the
+ * initial boundary update, then steps until the stop test holds. */
+int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st);
+
+/* Enqueue n full steps (dt, all RK stages, swap) without residuals or stop
+ * test; returns without waiting.  Used for fixed-length timing. */
+int32_t fvh_solver_advance(fvh_solver* s, int32_t n);
+int32_t fvh_solver_synchronize(fvh_solver* s);
+int32_t fvh_solver_status(fvh_solver* s, fvh_status* st);
+
+/* Kernel timing with HIP events on the solver's stream.  While enabled, each
+ * launch is bracketed by events; fvh_solver_kernel_times returns, per kernel
+ * (names in fvh_solver_kernel_name), the launch count and total ms. */
+int32_t fvh_solver_profile(fvh_solver* s, int32_t enable);
+int32_t fvh_solver_kernel_times(fvh_solver* s, int32_t max, int64_t* counts, double* total_ms);
+const char* fvh_solver_kernel_name(int32_t k);
+
+/* Algorithmic bytes per RK stage for this mesh and config (the HBM roofline
+ * numerator): first order C(3Vs+s+g) + E(3g+8) + B(2Vs+2g+4); second order
+ * adds C(7Vs+2g) + E(2g). */
+double fvh_solver_stage_bytes(const fvh_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

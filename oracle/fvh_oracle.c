@@ -535,6 +535,14 @@ void orc_calc_time_derivatives(const orc_mesh* m, const orc_config* cf, const do
         } else {
             for (int k = 0; k < V; ++k) { qi[k] = q[V * i + k]; qj[k] = q[V * j + k]; }
         }
+        if (cf->lin && cf->pos) {
+            /* extension (no reference counterpart): fall back to the cell
+             * value where the reconstructed face state is not physical */
+            if (!(qi[0] > 0) || !(pressure(cf->gamma, qi) > 0))
+                for (int k = 0; k < V; ++k) qi[k] = q[V * i + k];
+            if (!(qj[0] > 0) || !(pressure(cf->gamma, qj) > 0))
+                for (int k = 0; k < V; ++k) qj[k] = q[V * j + k];
+        }
         orc_flux(cf, f, qi, qj, nn);
         for (int k = 0; k < V; ++k) {
             qt[V * i + k] -= f[k] * le / m->cell_area[i];
@@ -567,6 +575,8 @@ void orc_update_bounds(const orc_mesh* m, const orc_config* cf, double* q,
                 int64_t kk = V * (int64_t)ii + k;
                 qint[k] = q[kk] + (gx[kk] * d0 + gy[kk] * d1) * lim[kk];
             }
+            if (cf->pos && (!(qint[0] > 0) || !(pressure(cf->gamma, qint) > 0)))
+                for (int k = 0; k < V; ++k) qint[k] = q[V * ii + k];
         } else {
             for (int k = 0; k < V; ++k) qint[k] = q[V * ii + k];
         }

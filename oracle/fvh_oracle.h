@@ -37,6 +37,8 @@ typedef struct {
     double dt_fixed;
     double cfl;
     int grad, lim, lin, gdt;   /* solver:: flags (explicit.h:35-40) */
+    int pos;             /* extension: first-order face state where the
+                            reconstruction has rho <= 0 or p <= 0 */
     double K;            /* limiter_k_value (explicit.cpp:27) */
     double gamma;
     int n_alpha;

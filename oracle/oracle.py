@@ -39,7 +39,7 @@ class OrcConfig(C.Structure):
     _fields_ = [
         ("flux", C.c_int), ("phi", C.c_int), ("phi_yt", C.c_double),
         ("dt_kind", C.c_int), ("dt_fixed", C.c_double), ("cfl", C.c_double),
-        ("grad", C.c_int), ("lim", C.c_int), ("lin", C.c_int), ("gdt", C.c_int),
+        ("grad", C.c_int), ("lim", C.c_int), ("lin", C.c_int), ("gdt", C.c_int), ("pos", C.c_int),
         ("K", C.c_double), ("gamma", C.c_double),
         ("n_alpha", C.c_int), ("alpha", C.c_double * 8),
         ("n_bc", C.c_int), ("bc_kind", C.c_int * 16), ("bc_param", (C.c_double * 4) * 16),
@@ -110,6 +110,7 @@ def make_config(cfg: Dict, names: List[str]) -> OrcConfig:
     c.lim = int(bool(cfg.get("lim", False)))
     c.lin = int(bool(cfg.get("lin", False)))
     c.gdt = int(bool(cfg.get("gdt", False)))
+    c.pos = int(bool(cfg.get("positivity", False)))
     c.K = float(cfg.get("K", 7.5))
     c.gamma = float(cfg.get("gamma", 1.4))
     alpha = list(cfg.get("alpha", RK5))

@@ -1,0 +1,75 @@
+"""Shared test helpers: seeded physical states, mesh/config cases, comparisons."""
+import numpy as np
+
+from fvhyper_amd import meshgen
+
+FREESTREAM_STEP = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+FREESTREAM_NACA = ("farfield", 1.4, 0.8, 0.0175, 1.0)
+
+
+def random_state(n_cells: int, seed: int, gamma: float = 1.4, mach: float = 0.5) -> np.ndarray:
+    """Smooth-ish random physical states: rho in [0.5,1.5], |u| ~ mach*c, p in [0.5,1.5]."""
+    u = meshgen.uniform01(seed, 4 * n_cells).reshape(n_cells, 4)
+    rho = 0.5 + u[:, 0]
+    p = 0.5 + u[:, 3]
+    c = np.sqrt(gamma * p / rho)
+    vx = (2 * u[:, 1] - 1) * mach * c
+    vy = (2 * u[:, 2] - 1) * mach * c
+    q = np.stack([rho, rho * vx, rho * vy, p / (gamma - 1) + 0.5 * rho * (vx * vx + vy * vy)], 1)
+    return q.ravel()
+
+
+def bits_equal(a, b) -> bool:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def same_values(a, b) -> bool:
+    """Bit-identical on every non-NaN entry and NaN in the same places (a
+    run that blows up must blow up identically; NaN payloads may differ)."""
+    a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float64).ravel()
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+def rel_err(a, b) -> float:
+    """max_{i,k} |a-b| / max(|a_ik|, max_i |a_.k|): per-variable infinity-norm floor."""
+    a = np.asarray(a, dtype=np.float64).reshape(-1, 4)
+    b = np.asarray(b, dtype=np.float64).reshape(-1, 4)
+    scale = np.maximum(np.abs(a), np.max(np.abs(a), axis=0, keepdims=True))
+    scale = np.where(scale == 0, 1.0, scale)
+    return float(np.max(np.abs(a - b) / scale)) if a.size else 0.0
+
+
+# (name, mesh factory, config) cases shared by the parity tests
+BC_ALL = {"wall": ("mirror",), "transmissive": ("copy",), "inlet": FREESTREAM_STEP,
+          "outlet": FREESTREAM_STEP}
+
+CONFIGS = {
+    "first_split_sum": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, bcs=BC_ALL),
+    "first_joined_fixed": dict(flux="roe_joined", dt="fixed", dt_fixed=2e-5, gdt=True, bcs=BC_ALL),
+    "second_michalak_local": dict(flux="roe_split", dt="edge_min", cfl=1.0, phi="michalak", grad=True,
+                                  lim=True, lin=True, gdt=False,
+                                  bcs={"wall": ("mirror",), "transmissive": FREESTREAM_NACA,
+                                       "inlet": FREESTREAM_NACA, "outlet": ("copy",)}),
+    "second_venkat_global": dict(flux="roe_joined", dt="sum_cfl", cfl=0.8, phi="venkat", grad=True,
+                                 lim=True, lin=True, gdt=True, bcs=BC_ALL),
+    "second_minmod_farfield": dict(flux="roe_joined", dt="sum_cfl", cfl=0.6, phi="minmod", grad=True,
+                                   lim=True, lin=True, gdt=True,
+                                   bcs={"wall": FREESTREAM_NACA, "transmissive": ("mirror",),
+                                        "inlet": FREESTREAM_NACA, "outlet": FREESTREAM_NACA}),
+    "grad_only_lin": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, grad=True, lim=False, lin=True,
+                          gdt=True, bcs=BC_ALL),
+}
+
+MESHES = {
+    "mixed": lambda: meshgen.mixed_channel(nx=12, ny=6),
+    "sod": lambda: meshgen.sod_tube(nx=40, ny=4),
+    "step": lambda: meshgen.forward_step(n_per_unit=20),
+}

@@ -1,0 +1,106 @@
+"""GPU parity: libfvh.so (HIP on gfx950, through the C ABI) against the CPU
+oracle on the same meshes and states.  The FP64 path must be bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fvhyper_amd import Mesh, Solver, meshgen, workloads
+from helpers import CONFIGS, MESHES, bits_equal, random_state, rel_err, same_values
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(mesh_name, cfg, precision="f64", renumber=True):
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg, precision=precision, renumber=renumber)
+    return m, om, pm, s
+
+
+@pytest.mark.parametrize("mesh_name", sorted(MESHES))
+@pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
+def test_phases_bit_exact(mesh_name, cfg_name):
+    print("case", mesh_name, cfg_name, flush=True)
+    cfg = CONFIGS[cfg_name]
+    m, om, pm, s = _pair(mesh_name, cfg)
+    q = random_state(om.n_cells, seed=101)
+    s.set_state(q)
+    g = s.phases(alpha=0.2898)
+    o = om.phases(cfg, q, alpha=0.2898)
+    for key in ("dt", "gx", "gy", "lim", "qt", "qk"):
+        assert bits_equal(g[key], o[key]), (key, np.max(np.abs(g[key] - o[key])))
+
+
+@pytest.mark.parametrize("renumber", [True, False])
+@pytest.mark.parametrize("mesh_name", sorted(MESHES))
+@pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
+def test_run_bit_exact(mesh_name, cfg_name, renumber):
+    print("case", mesh_name, cfg_name, renumber, flush=True)
+    cfg = CONFIGS[cfg_name]
+    m, om, pm, s = _pair(mesh_name, cfg, renumber=renumber)
+    q0 = random_state(om.n_cells, seed=202, mach=0.3)
+    s.set_state(q0)
+    st = s.run(max_step=12, print_interval=5)
+    qg = s.get_state()
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 12, print_interval=5)
+    assert st["step"] == steps == 12
+    assert same_values([st["time"]], [t])
+    # some stress configs blow up; the GPU must then blow up identically
+    assert same_values(qg, qo), rel_err(qo, qg)
+    # residual sums run in another order: equal to rounding
+    np.testing.assert_allclose(st["R0"], R0, rtol=1e-12)
+    if np.all(np.isfinite(R)):
+        np.testing.assert_allclose(st["R"], R, rtol=1e-10)
+
+
+def test_run_tolerance_stop_matches():
+    """Stop on the residual test (explicit.cpp:718-726): same step count."""
+    cfg = CONFIGS["first_split_sum"]
+    m, om, pm, s = _pair("sod", cfg)
+    q0 = workloads.initial_state(om, workloads.sod_ic())
+    s.set_state(q0)
+    st = s.run(max_step=400, tol=0.5)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 400, tol=0.5)
+    assert st["step"] == steps < 400
+    assert bits_equal(s.get_state(), qo)
+
+
+def test_run_max_time_stop_matches():
+    cfg = CONFIGS["second_venkat_global"]
+    m, om, pm, s = _pair("step", cfg)
+    q0 = workloads.initial_state(om, workloads.uniform_ic(1.4, 3.0, 0.0, 1.0))
+    s.set_state(q0)
+    st = s.run(max_time=0.01)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 100000, max_time=0.01)
+    assert st["step"] == steps and st["time"] == t
+    assert bits_equal(s.get_state(), qo)
+
+
+def test_sod_full_size_explicit_euler():
+    """The 32,000-cell Sod workload, explicit Euler, CFL 0.5, to t = 0.2."""
+    m, cfg, ic = workloads.sod()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_time=0.2, print_interval=100)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 100000, max_time=0.2, print_interval=100)
+    print("sod steps", steps, "time", t, flush=True)
+    assert st["step"] == steps and st["time"] == t
+    assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
+
+
+def test_forward_step_full_size_two_steps():
+    """The 975,744-cell Mach-3 step workload, two RK5 steps, bit-exact."""
+    m, cfg, ic = workloads.forward_step()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_step=2)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 2)
+    assert st["step"] == steps == 2 and st["time"] == t
+    assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
