@@ -43,4 +43,9 @@ int32_t read_msh(const char* path, fvh_mesh& m);
 // placed at device position k.  Deterministic (ties by cell index).
 std::vector<int32_t> rcm_order(const fvh_mesh& m);
 
+// Recursive coordinate bisection of the real cells' centres into tiles of
+// `tile` cells (all full but the last), tiles in recursion order, cells in
+// ascending reference index within a tile.  order[k] as for rcm_order.
+std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile);
+
 }  // namespace fvh

@@ -1,22 +1,28 @@
 // fvh_kernels.h -- gfx950 kernels of the explicit residual-and-update path.
 //
-// Data layout in HBM (device cell order: reverse Cuthill-McKee of the real
-// cells, then the boundary ghost cells in boundary-element order):
+// Data layout in HBM (device cell order: a locality renumbering of the real
+// cells -- recursive-bisection tiles or reverse Cuthill-McKee -- then the
+// boundary ghost cells in boundary-element order):
 //   state      Q4<T>[cells]          AoS, 4 conserved variables per cell, so a
 //                                    neighbour gather is one 32 B (FP64) read
 //   half-edges CSR per cell, sorted by reference edge id and side, so every
 //              per-cell sum runs in the reference's summation order:
-//              he_off[cells+1], he_nbr, he_edge (edge | side<<31), he_twin,
-//              he_bc (BC slot on the interior side of a boundary edge, else -1)
+//              he_off[cells+1], he_nbr, he_edge (edge | bc<<30 | side<<31),
+//              he_twin, he_bc (BC slot of a boundary edge's interior side),
+//              he_slot (the edge's index in its tile's edge list)
 //   edges      G4<T>(nx, ny, length, Green-Gauss weight) + G2<T>(centre)
 //   cells      area, 1/area, sqrt(area), centre
-//   increments Q4<T>[half-edges]  limited reconstruction (grad.d)*phi per
-//              half-edge, written by the gradient+limiter pass and read by the
-//              flux pass (own and twin side) and the boundary update
+//   faces      Q4<T>[half-edges]  limited linear reconstruction
+//              q + (grad q . d) phi at each half-edge, written by the
+//              gradient/limiter pass, read by the flux pass
+//   bincr      Q4<T>[ghosts]      the increment (grad q . d) phi of each
+//              boundary edge's interior side, for the boundary update
+//   tiles      per 256-cell tile: the edges it touches, each with its two
+//              half-edges and cells
 //
-// Every kernel is one thread per cell and gathers over the cell's half-edges:
-// no atomics, no colouring, deterministic, and the order of every
-// floating-point sum equals the reference's edge loop order.
+// No atomics and no colouring: every per-cell sum is a gather in the
+// reference's edge order, so results are deterministic and bit-identical to
+// the reference CPU run in FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,8 +36,10 @@ template <class T> struct alignas(4 * sizeof(T)) Q4 { T v[4]; };
 template <class T> struct alignas(2 * sizeof(T)) G2 { T v[2]; };
 
 constexpr uint32_t kSideBit = 0x80000000u;
-constexpr uint32_t kEdgeMask = 0x7fffffffu;
-constexpr int kBlock = 256;
+constexpr uint32_t kBcBit = 0x40000000u;
+constexpr uint32_t kEdgeMask = 0x3fffffffu;
+constexpr int kBlock = 256;        // cells per workgroup = cells per tile
+constexpr int kTileEdges = 1024;   // LDS edge slots per tile
 constexpr int kMaxBc = 16;
 
 // Per-run control block, advanced on the device so that steps can be queued
@@ -55,6 +63,7 @@ struct MeshView {
     const uint32_t* he_edge;
     const int32_t* he_twin;
     const int32_t* he_bc;
+    const uint16_t* he_slot;
     const Q4<T>* e_geo;   // nx, ny, len, gg weight
     const G2<T>* e_c;     // edge centre
     const T* c_area;
@@ -63,10 +72,14 @@ struct MeshView {
     const G2<T>* c_c;     // cell centre
 };
 
-template <class T>
-struct BcTable {
-    BcParam<T> p[kMaxBc];
+struct TileView {
+    int32_t n_tiles;
+    const int32_t* off;    // [n_tiles+1] into the tile edge lists
+    const int32_t* edge;   // device edge id
+    const int2* he;        // (half-edge of side 0, half-edge of side 1)
+    const int2* cell;      // (cell of side 0, cell of side 1)
 };
+
 
 struct PhysParams {
     double gamma, cfl, dt_fixed, K, phi_yt;
@@ -79,18 +92,23 @@ void launch_dt(hipStream_t s, const MeshView<T>& m, int dt_kind, const PhysParam
                const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl);
 void launch_dt_min(hipStream_t s, const double* blk_min, int nblk, const void* dt_cells,
                    int dt_is_f32, int32_t dev0, double* dt_global, const Ctrl* ctrl);
+// Gradient + limiter + face states; with dt_kind >= 0 also the time step of
+// every cell and per-block minima (first stage of a step).
 template <class T>
 void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind,
-                    bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* incr,
+                    bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr,
+                    int dt_kind, T* dt, double* blk_min,
                     Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl);
+// Flux + RK update + boundary update; tiles == nullptr selects the per-cell
+// form that computes every interior edge flux from both sides.
 template <class T>
-void launch_flux(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool joined, bool lin,
-                 const BcTable<T>& bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
-                 const Q4<T>* incr, const T* dt, int dt_stride, const double* dt_global, T alpha,
-                 double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
+void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, const PhysParams& pp,
+                 bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0,
+                 Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt, int dt_stride,
+                 T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
-                        const BcTable<T>& bcs, Q4<T>* q);
+                        const BcParam<T>* bcs, Q4<T>* q);
 void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol);
 void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,
                      const void* dt_src, int dt_is_f32, int32_t dt_index);

@@ -3,8 +3,6 @@
 // expression rounds exactly as the reference's x86-64 build does.
 #include "fvh_kernels.h"
 
-#include <float.h>
-
 namespace fvh {
 
 namespace {
@@ -45,56 +43,74 @@ __device__ inline void block_sum4(double* v) {
 template <class T>
 __device__ inline Q4<T> ld(const Q4<T>* p) { return *p; }
 
+// XCD-aware block order.  The dispatcher hands workgroup b to XCD b % 8, and
+// each XCD has its own L2; remapping hardware block b to logical block
+// (b % 8) * per + b / 8 gives every XCD one contiguous range of cells, so a
+// cell's neighbours (close in the renumbered order) sit in the same L2.
+constexpr int kXcd = 8;
+__device__ inline int xcd_block(int nblk) {
+    const int per = (nblk + kXcd - 1) / kXcd;
+    const int b = blockIdx.x;
+    const int lb = (b % kXcd) * per + b / kXcd;
+    return lb < nblk ? lb : -1;
+}
+
+template <class T>
+__device__ inline bool physical(T g, const T* q) {
+    return q[0] > T(0) && pressure(g, q) > T(0);
+}
+
 // ---------------------------------------------------------------------------
 // time step: calc_dt kinds + block minimum for the global-dt reduction
+// (first-order configurations; second order computes dt in k_gradlim)
 // ---------------------------------------------------------------------------
+template <class T, int KIND>
+__device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, const Q4<T>& qc,
+                            const Q4<T>* __restrict__ q) {
+    if (KIND == 0) return T(pp.dt_fixed);
+    const T g = T(pp.gamma);
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    if (KIND == 1) {
+        // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184)
+        T s = T(0);
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            const Q4<T> qn = ld(q + m.he_nbr[h]);
+            const bool side = (ue & kSideBit) != 0;
+            const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
+            const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+            s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+        }
+        return T(pp.cfl) * m.c_area[c] / s;
+    }
+    // edge-minimum local form (naca/main.cpp:151-182)
+    T r = T(1);
+    const T A = m.c_area[c];
+    for (int h = h0; h < h1; ++h) {
+        const Q4<T> geo = m.e_geo[m.he_edge[h] & kEdgeMask];
+        const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+        r = smin(r, T(pp.cfl) * (A / geo.v[2]) / sabs(eo));
+    }
+    return r;
+}
+
 template <class T, int KIND>
 __global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, PhysParams pp,
                                                const Q4<T>* __restrict__ q, T* __restrict__ dt,
                                                double* __restrict__ blk_min, const Ctrl* __restrict__ ctrl) {
     if (stopped(ctrl)) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int lb = xcd_block((m.n_total + kBlock - 1) / kBlock);
+    if (lb < 0) return;
+    const int c = lb * kBlock + threadIdx.x;
     double v = kInf;
     if (c < m.n_total) {
-        T r;
-        if (KIND == 0) {
-            r = T(pp.dt_fixed);
-        } else {
-            const T g = T(pp.gamma);
-            const Q4<T> qc = ld(q + c);
-            const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-            if (KIND == 1) {
-                // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184)
-                T s = T(0);
-                for (int h = h0; h < h1; ++h) {
-                    const uint32_t ue = m.he_edge[h];
-                    const Q4<T> geo = m.e_geo[ue & kEdgeMask & 0x3fffffffu];
-                    const Q4<T> qn = ld(q + m.he_nbr[h]);
-                    const bool side = (ue & kSideBit) != 0;
-                    const T* qi = side ? qn.v : qc.v;
-                    const T* qj = side ? qc.v : qn.v;
-                    const T ei = eig_sum(g, qi, geo.v[0], geo.v[1]);
-                    const T ej = eig_sum(g, qj, geo.v[0], geo.v[1]);
-                    s = s + smax(ei, ej) * geo.v[2];
-                }
-                r = T(pp.cfl) * m.c_area[c] / s;
-            } else {
-                // edge-minimum local form (naca/main.cpp:151-182)
-                r = T(1);
-                const T A = m.c_area[c];
-                for (int h = h0; h < h1; ++h) {
-                    const Q4<T> geo = m.e_geo[m.he_edge[h] & 0x3fffffffu];
-                    const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
-                    const T d = T(pp.cfl) * (A / geo.v[2]) / sabs(eo);
-                    r = smin(r, d);
-                }
-            }
-        }
+        const T r = cell_dt<T, KIND>(m, pp, c, ld(q + c), q);
         dt[c] = r;
         v = (double)r;
     }
     v = block_min(v);
-    if (threadIdx.x == 0) blk_min[blockIdx.x] = v;
+    if (threadIdx.x == 0) blk_min[lb] = v;
 }
 
 template <class T>
@@ -113,36 +129,64 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(const double* __restrict__ bl
 }
 
 // ---------------------------------------------------------------------------
-// Green-Gauss gradient + neighbour min/max + limiter + reconstruction
-// increments, one pass per cell (explicit.cpp:112-258, 307-313)
+// Green-Gauss gradient + neighbour min/max + limiter + face states, one pass
+// per cell (explicit.cpp:112-258, 307-313); optionally the time step.
 // ---------------------------------------------------------------------------
-template <class T, int PHI>
+template <class T, int PHI, int DT>
 __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp, int grad, int lim_on,
                                                     int lin, const Q4<T>* __restrict__ q,
-                                                    Q4<T>* __restrict__ incr, Q4<T>* __restrict__ dbg_gx,
-                                                    Q4<T>* __restrict__ dbg_gy, Q4<T>* __restrict__ dbg_lim,
-                                                    const Ctrl* __restrict__ ctrl) {
+                                                    Q4<T>* __restrict__ face, Q4<T>* __restrict__ bincr,
+                                                    T* __restrict__ dt, double* __restrict__ blk_min,
+                                                    Q4<T>* __restrict__ dbg_gx, Q4<T>* __restrict__ dbg_gy,
+                                                    Q4<T>* __restrict__ dbg_lim, const Ctrl* __restrict__ ctrl) {
     if (stopped(ctrl)) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= m.n_total) return;
-    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-    const Q4<T> qc = ld(q + c);
-    const G2<T> cc = m.c_c[c];
-    T gx[4] = {T(0), T(0), T(0), T(0)};
-    T gy[4] = {T(0), T(0), T(0), T(0)};
-    // limiters are reset to 1 when computed (explicit.cpp:184-186) and stay
-    // zero-initialised otherwise (explicit.cpp:673)
-    const T lim0 = lim_on ? T(1) : T(0);
-    T lim[4] = {lim0, lim0, lim0, lim0};
-    if (c < m.n_real) {
-        if (grad) {
+    const int lb = xcd_block((m.n_total + kBlock - 1) / kBlock);
+    if (lb < 0) return;
+    const int c = lb * kBlock + threadIdx.x;
+    double vmin = kInf;
+    if (c < m.n_total) {
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const Q4<T> qc = ld(q + c);
+        const G2<T> cc = m.c_c[c];
+        const bool real = c < m.n_real;
+        const T g = T(pp.gamma);
+        T gx[4] = {T(0), T(0), T(0), T(0)};
+        T gy[4] = {T(0), T(0), T(0), T(0)};
+        // limiters are reset to 1 when computed (explicit.cpp:184-186) and
+        // stay zero-initialised otherwise (explicit.cpp:673)
+        const T lim0 = lim_on ? T(1) : T(0);
+        T lim[4] = {lim0, lim0, lim0, lim0};
+        T qmn[4], qmx[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
+        T dts = (DT == 2) ? T(1) : T(0);
+        // one pass over the half-edges feeds the Green-Gauss sums (edge order,
+        // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
+        // time-step sum or minimum
+        if ((real && (grad || lim_on)) || DT) {
             for (int h = h0; h < h1; ++h) {
                 const int nb = m.he_nbr[h];
-                if (nb == c) continue;   // the reference skips i == j edges
                 const uint32_t ue = m.he_edge[h];
-                const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
-                const Q4<T> qn = ld(q + nb);
                 const bool side = (ue & kSideBit) != 0;
+                const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+                const Q4<T> qn = ld(q + nb);
+                if (DT == 1) {
+                    const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
+                    const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+                    dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+                } else if (DT == 2) {
+                    const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+                    dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
+                }
+                if (!real) continue;
+                if (lim_on) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        qmn[k] = smin(qmn[k], qn.v[k]);
+                        qmx[k] = smax(qmx[k], qn.v[k]);
+                    }
+                }
+                if (!grad || nb == c) continue;   // the reference skips i == j edges
                 const T* qi = side ? qn.v : qc.v;
                 const T* qj = side ? qc.v : qn.v;
                 const T w = geo.v[3];
@@ -159,6 +203,14 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp
                     }
                 }
             }
+        }
+        if (DT) {
+            T r = dts;
+            if (DT == 1) r = T(pp.cfl) * m.c_area[c] / dts;
+            dt[c] = r;
+            vmin = (double)r;
+        }
+        if (real && grad) {
             const T ia = m.c_inv_area[c];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -166,24 +218,13 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp
                 gy[k] = gy[k] * ia;
             }
         }
-        if (lim_on) {
-            T qmn[4], qmx[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
-            for (int h = h0; h < h1; ++h) {
-                const Q4<T> qn = ld(q + m.he_nbr[h]);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    qmn[k] = smin(qmn[k], qn.v[k]);
-                    qmx[k] = smax(qmx[k], qn.v[k]);
-                }
-            }
+        if (real && lim_on) {
             const T Ka = T(pp.K) * m.c_sqrt_area[c];
             const T K3a = Ka * Ka * Ka;
             const T yt = T(pp.phi_yt);
             const T tol = T(1e-15);
             for (int h = h0; h < h1; ++h) {
-                const G2<T> ec = m.e_c[m.he_edge[h] & 0x3fffffffu];
+                const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
                 const T dx = ec.v[0] - cc.v[0];
                 const T dy = ec.v[1] - cc.v[1];
 #pragma unroll
@@ -211,129 +252,216 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp
                 }
             }
         }
-    }
-    if (dbg_gx) {
-        Q4<T> a, b, d;
+        if (dbg_gx) {
+            Q4<T> a, b, d;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
-        dbg_gx[c] = a;
-        dbg_gy[c] = b;
-        dbg_lim[c] = d;
-    }
-    if (lin) {
-        for (int h = h0; h < h1; ++h) {
-            const G2<T> ec = m.e_c[m.he_edge[h] & 0x3fffffffu];
-            const T dx = ec.v[0] - cc.v[0];
-            const T dy = ec.v[1] - cc.v[1];
-            Q4<T> inc;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
-            incr[h] = inc;
+            for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
+            dbg_gx[c] = a;
+            dbg_gy[c] = b;
+            dbg_lim[c] = d;
         }
+        if (lin) {
+            for (int h = h0; h < h1; ++h) {
+                const uint32_t ue = m.he_edge[h];
+                const G2<T> ec = m.e_c[ue & kEdgeMask];
+                const T dx = ec.v[0] - cc.v[0];
+                const T dy = ec.v[1] - cc.v[1];
+                Q4<T> inc, fs;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
+                    fs.v[k] = qc.v[k] + inc.v[k];
+                }
+                face[h] = fs;
+                if (ue & kBcBit) bincr[m.he_nbr[h] - m.n_real] = inc;
+            }
+        }
+    }
+    if (DT) {
+        vmin = block_min(vmin);
+        if (threadIdx.x == 0) blk_min[lb] = vmin;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Edge flux gather + RK stage update + boundary update (explicit.cpp:261-417)
+// RK stage update + boundary update of one cell from its accumulated qt
+// (explicit.cpp:368-417)
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBcBit = 0x40000000u;
+template <class T, bool LIN>
+__device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
+                                   int h0, int h1, const T (&qt)[4], const Q4<T>* __restrict__ q0,
+                                   Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
+                                   const T* __restrict__ dt, int dt_stride, T alpha, bool want_rs,
+                                   double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
+    const T g = T(pp.gamma);
+    const T d = dt[(int64_t)c * dt_stride];
+    const Q4<T> qs = ld(q0 + c);
+    Q4<T> out;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
+    qout[c] = out;
+    if (dbg_qt) {
+        Q4<T> t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
+        dbg_qt[c] = t;
+    }
+    if (want_rs) {
+        const double A = (double)m.c_area[c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rs[k] = (double)qt[k] * (double)qt[k] * A;
+    }
+    // ghost cells of this cell's boundary edges
+    for (int h = h0; h < h1; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        if (!(ue & kBcBit)) continue;
+        const int nb = m.he_nbr[h];
+        const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+        T qi[4];
+        if (LIN) {
+            const Q4<T> ia = ld(bincr + (nb - m.n_real));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + ia.v[k];
+            if (pp.positivity && !physical(g, qi))
+                for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+        }
+        Q4<T> gs;
+        boundary(bcs[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
+        qout[nb] = gs;
+    }
+}
 
+// Per-cell form: every interior edge flux is computed by both of its cells.
 template <class T, bool JOINED, bool LIN>
-__global__ __launch_bounds__(kBlock) void k_flux(MeshView<T> m, PhysParams pp, BcTable<T> bcs,
-                                                 const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
-                                                 Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ incr,
-                                                 const T* __restrict__ dt, int dt_stride, T alpha,
-                                                 double* __restrict__ res_blk, Q4<T>* __restrict__ dbg_qt,
-                                                 const Ctrl* __restrict__ ctrl) {
+__global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, PhysParams pp, const BcParam<T>* __restrict__ bcs,
+                                                      const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
+                                                      Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
+                                                      const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
+                                                      int dt_stride, T alpha, double* __restrict__ res_blk,
+                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
     if (stopped(ctrl)) return;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
+    const int lb = xcd_block((m.n_real + kBlock - 1) / kBlock);
+    if (lb < 0) return;
+    const int c = lb * kBlock + threadIdx.x;
     const T g = T(pp.gamma);
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
     if (c < m.n_real) {
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-        const Q4<T> qc = ld(qin + c);
         const T A = m.c_area[c];
         T qt[4] = {T(0), T(0), T(0), T(0)};
         for (int h = h0; h < h1; ++h) {
             const uint32_t ue = m.he_edge[h];
-            const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
-            const Q4<T> qn = ld(qin + m.he_nbr[h]);
-            T a[4], b[4];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            Q4<T> a, b;
             if (LIN) {
-                const Q4<T> ia = incr[h];
-                const Q4<T> ib = incr[m.he_twin[h]];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    a[k] = qc.v[k] + ia.v[k];
-                    b[k] = qn.v[k] + ib.v[k];
-                }
+                a = ld(face + h);
+                b = ld(face + m.he_twin[h]);
                 if (pp.positivity) {
-                    if (!(a[0] > T(0)) || !(pressure(g, a) > T(0)))
-                        for (int k = 0; k < 4; ++k) a[k] = qc.v[k];
-                    if (!(b[0] > T(0)) || !(pressure(g, b) > T(0)))
-                        for (int k = 0; k < 4; ++k) b[k] = qn.v[k];
+                    if (!physical(g, a.v)) a = ld(qin + c);
+                    if (!physical(g, b.v)) b = ld(qin + m.he_nbr[h]);
                 }
             } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) { a[k] = qc.v[k]; b[k] = qn.v[k]; }
+                a = ld(qin + c);
+                b = ld(qin + m.he_nbr[h]);
             }
             const bool side = (ue & kSideBit) != 0;
-            T f[4];
-            roe_flux<T, JOINED>(g, f, side ? b : a, side ? a : b, geo.v[0], geo.v[1]);
+            T qi[4], qj[4], f[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                qi[k] = side ? b.v[k] : a.v[k];
+                qj[k] = side ? a.v[k] : b.v[k];
+            }
+            roe_flux<T, JOINED>(g, f, qi, qj, geo.v[0], geo.v[1]);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const T t = f[k] * geo.v[2] / A;
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        const T d = dt[(int64_t)c * dt_stride];
-        const Q4<T> qs = ld(q0 + c);
-        Q4<T> out;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
-        qout[c] = out;
-        if (dbg_qt) {
-            Q4<T> t;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
-            dbg_qt[c] = t;
-        }
-        if (res_blk) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) rs[k] = (double)qt[k] * (double)qt[k] * (double)A;
-        }
-        // ghost cells of this cell's boundary edges (update_bounds, :380-417)
-        for (int h = h0; h < h1; ++h) {
-            const uint32_t ue = m.he_edge[h];
-            if (!(ue & kBcBit)) continue;
-            const Q4<T> geo = m.e_geo[ue & 0x3fffffffu];
-            T qi[4];
-            if (LIN) {
-                const Q4<T> ia = incr[h];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + ia.v[k];
-                if (pp.positivity && (!(qi[0] > T(0)) || !(pressure(g, qi) > T(0))))
-                    for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
-            }
-            Q4<T> gs;
-            boundary(bcs.p[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
-            qout[m.he_nbr[h]] = gs;
-        }
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt, dt_stride, alpha,
+                            res_blk != nullptr, rs, dbg_qt);
     }
     if (res_blk) {
         block_sum4(rs);
         if (threadIdx.x == 0)
-            for (int k = 0; k < 4; ++k) res_blk[4 * blockIdx.x + k] = rs[k];
+            for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
+    }
+}
+
+// Tiled form: a workgroup owns a tile of 256 consecutive cells.  Phase 1
+// computes the flux (times edge length) of every edge the tile touches once,
+// edge-parallel, into LDS; phase 2 gathers each cell's edges from LDS in
+// reference edge order.  Edges on a tile border are computed by both tiles
+// (same inputs, same bits).
+template <class T, bool JOINED, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileView tv, PhysParams pp, const BcParam<T>* __restrict__ bcs,
+                                                      const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
+                                                      Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
+                                                      const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
+                                                      int dt_stride, T alpha, double* __restrict__ res_blk,
+                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
+    if (stopped(ctrl)) return;
+    const int lb = xcd_block(tv.n_tiles);
+    if (lb < 0) return;
+    __shared__ Q4<T> fle[kTileEdges];
+    const T g = T(pp.gamma);
+    const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
+    for (int k = e0 + (int)threadIdx.x; k < e1; k += kBlock) {
+        const Q4<T> geo = m.e_geo[tv.edge[k]];
+        Q4<T> a, b;
+        if (LIN) {
+            const int2 hh = tv.he[k];
+            a = ld(face + hh.x);
+            b = ld(face + hh.y);
+            if (pp.positivity) {
+                if (!physical(g, a.v)) a = ld(qin + tv.cell[k].x);
+                if (!physical(g, b.v)) b = ld(qin + tv.cell[k].y);
+            }
+        } else {
+            const int2 cc = tv.cell[k];
+            a = ld(qin + cc.x);
+            b = ld(qin + cc.y);
+        }
+        T f[4];
+        roe_flux<T, JOINED>(g, f, a.v, b.v, geo.v[0], geo.v[1]);
+        Q4<T> r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
+        fle[k - e0] = r;
+    }
+    __syncthreads();
+    const int c = lb * kBlock + threadIdx.x;
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < m.n_real) {
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const T A = m.c_area[c];
+        T qt[4] = {T(0), T(0), T(0), T(0)};
+        for (int h = h0; h < h1; ++h) {
+            const bool side = (m.he_edge[h] & kSideBit) != 0;
+            const Q4<T> r = fle[m.he_slot[h]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T t = r.v[k] / A;
+                qt[k] = side ? qt[k] + t : qt[k] - t;
+            }
+        }
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt, dt_stride, alpha,
+                            res_blk != nullptr, rs, dbg_qt);
+    }
+    if (res_blk) {
+        block_sum4(rs);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
     }
 }
 
 // Initial boundary update with zero gradients and limiters (explicit.cpp:712).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParams pp, int lin,
-                                                        BcTable<T> bcs, Q4<T>* __restrict__ q) {
+                                                        const BcParam<T>* __restrict__ bcs, Q4<T>* __restrict__ q) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= m.n_real) return;
     const T g = T(pp.gamma);
@@ -342,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParam
     for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
         const uint32_t ue = m.he_edge[h];
         if (!(ue & kBcBit)) continue;
-        const int e = ue & 0x3fffffffu;
+        const int e = ue & kEdgeMask;
         const Q4<T> geo = m.e_geo[e];
         T qi[4];
         if (lin) {
@@ -357,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParam
             for (int k = 0; k < 4; ++k) qi[k] = qc.v[k];
         }
         Q4<T> gs;
-        boundary(bcs.p[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
+        boundary(bcs[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
         q[m.he_nbr[h]] = gs;
     }
 }
@@ -406,6 +534,9 @@ __global__ __launch_bounds__(kBlock) void k_ctrl_end(Ctrl* ctrl, const double* _
 }
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+// hardware grid for xcd_block(): a multiple of the XCD count
+inline int xcd_grid_blocks(int nb) { return ((nb + kXcd - 1) / kXcd) * kXcd; }
+inline int xcd_grid(int n) { return xcd_grid_blocks(nblocks(n)); }
 
 }  // namespace
 
@@ -415,7 +546,7 @@ inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 template <class T>
 void launch_dt(hipStream_t s, const MeshView<T>& m, int dt_kind, const PhysParams& pp,
                const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
-    const int nb = nblocks(m.n_total);
+    const int nb = xcd_grid(m.n_total);
     if (nb == 0) return;
     if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
     else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
@@ -434,30 +565,50 @@ void launch_dt_min(hipStream_t s, const double* blk_min, int nblk, const void* d
 
 template <class T>
 void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind, bool grad,
-                    bool lim, bool lin, const Q4<T>* q, Q4<T>* incr, Q4<T>* dbg_gx, Q4<T>* dbg_gy,
-                    Q4<T>* dbg_lim, const Ctrl* ctrl) {
-    const int nb = nblocks(m.n_total);
+                    bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr, int dt_kind, T* dt,
+                    double* blk_min, Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl) {
+    const int nb = xcd_grid(m.n_total);
     if (nb == 0) return;
-#define FVH_GL(P) hipLaunchKernelGGL((k_gradlim<T, P>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)grad, \
-                                     (int)lim, (int)lin, q, incr, dbg_gx, dbg_gy, dbg_lim, ctrl)
+#define FVH_GL(P, D) hipLaunchKernelGGL((k_gradlim<T, P, D>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)grad, \
+                                        (int)lim, (int)lin, q, face, bincr, dt, blk_min, dbg_gx, dbg_gy, dbg_lim, ctrl)
+#define FVH_GLD(P)                                  \
+    do {                                            \
+        if (dt_kind == 1) FVH_GL(P, 1);             \
+        else if (dt_kind == 2) FVH_GL(P, 2);        \
+        else FVH_GL(P, 0);                          \
+    } while (0)
     switch (phi_kind) {
-        case PHI_ZERO: FVH_GL(PHI_ZERO); break;
-        case PHI_MICHALAK: FVH_GL(PHI_MICHALAK); break;
-        case PHI_VENKAT: FVH_GL(PHI_VENKAT); break;
-        default: FVH_GL(PHI_MINMOD); break;
+        case PHI_ZERO: FVH_GLD(PHI_ZERO); break;
+        case PHI_MICHALAK: FVH_GLD(PHI_MICHALAK); break;
+        case PHI_VENKAT: FVH_GLD(PHI_VENKAT); break;
+        default: FVH_GLD(PHI_MINMOD); break;
     }
+#undef FVH_GLD
 #undef FVH_GL
 }
 
 template <class T>
-void launch_flux(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool joined, bool lin,
-                 const BcTable<T>& bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, const Q4<T>* incr,
-                 const T* dt, int dt_stride, const double* /*dt_global*/, T alpha, double* res_blk,
+void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, const PhysParams& pp, bool joined,
+                 bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
+                 const Q4<T>* face, const Q4<T>* bincr, const T* dt, int dt_stride, T alpha, double* res_blk,
                  Q4<T>* dbg_qt, const Ctrl* ctrl) {
-    const int nb = nblocks(m.n_real);
-    if (nb == 0) return;
-#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, pp, bcs, qin, q0, \
-                                        qout, incr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
+    if (m.n_real == 0) return;
+    if (tiles) {
+        const int nb = xcd_grid_blocks(tiles->n_tiles);
+        const TileView tv = *tiles;
+#define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tv, pp, bcs, qin, \
+                                        q0, qout, face, bincr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
+        if (joined) {
+            if (lin) FVH_FT(true, true); else FVH_FT(true, false);
+        } else {
+            if (lin) FVH_FT(false, true); else FVH_FT(false, false);
+        }
+#undef FVH_FT
+        return;
+    }
+    const int nb = xcd_grid(m.n_real);
+#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, pp, bcs, qin, q0, \
+                                        qout, face, bincr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
     if (joined) {
         if (lin) FVH_FX(true, true); else FVH_FX(true, false);
     } else {
@@ -468,7 +619,7 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool
 
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
-                        const BcTable<T>& bcs, Q4<T>* q) {
+                        const BcParam<T>* bcs, Q4<T>* q) {
     const int nb = nblocks(m.n_real);
     if (nb == 0) return;
     hipLaunchKernelGGL((k_bounds_init<T>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q);
@@ -488,16 +639,17 @@ void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk,
                            (const double*)dt_src, dt_index);
 }
 
-#define FVH_INST(T)                                                                                         \
-    template void launch_dt<T>(hipStream_t, const MeshView<T>&, int, const PhysParams&, const Q4<T>*, T*,    \
-                               double*, const Ctrl*);                                                       \
-    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const PhysParams&, int, bool, bool,    \
-                                    bool, const Q4<T>*, Q4<T>*, Q4<T>*, Q4<T>*, Q4<T>*, const Ctrl*);       \
-    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool, bool,            \
-                                 const BcTable<T>&, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,       \
-                                 const T*, int, const double*, T, double*, Q4<T>*, const Ctrl*);            \
-    template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,           \
-                                        const BcTable<T>&, Q4<T>*);
+#define FVH_INST(T)                                                                                          \
+    template void launch_dt<T>(hipStream_t, const MeshView<T>&, int, const PhysParams&, const Q4<T>*, T*,     \
+                               double*, const Ctrl*);                                                        \
+    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const PhysParams&, int, bool, bool,     \
+                                    bool, const Q4<T>*, Q4<T>*, Q4<T>*, int, T*, double*, Q4<T>*, Q4<T>*,    \
+                                    Q4<T>*, const Ctrl*);                                                    \
+    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileView*, const PhysParams&, bool,   \
+                                 bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,  \
+                                 const Q4<T>*, const T*, int, T, double*, Q4<T>*, const Ctrl*);              \
+    template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,            \
+                                        const BcParam<T>*, Q4<T>*);
 FVH_INST(double)
 FVH_INST(float)
 

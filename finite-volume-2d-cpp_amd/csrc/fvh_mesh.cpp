@@ -536,6 +536,44 @@ std::vector<int32_t> rcm_order(const fvh_mesh& m) {
     return order;
 }
 
+std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile) {
+    const int32_t N = m.n_real;
+    std::vector<int32_t> idx(N);
+    for (int32_t i = 0; i < N; ++i) idx[i] = i;
+    if (tile < 1) tile = 1;
+    // explicit stack: (lo, hi) ranges, left part processed first
+    std::vector<std::pair<int32_t, int32_t>> stack;
+    std::vector<int32_t> out;
+    out.reserve(N);
+    stack.push_back({0, N});
+    while (!stack.empty()) {
+        auto [lo, hi] = stack.back();
+        stack.pop_back();
+        const int32_t n = hi - lo;
+        if (n <= tile) {
+            std::sort(idx.begin() + lo, idx.begin() + hi);
+            out.insert(out.end(), idx.begin() + lo, idx.begin() + hi);
+            continue;
+        }
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int32_t k = lo; k < hi; ++k) {
+            const int32_t c = idx[k];
+            x0 = std::min(x0, m.c_cx[c]); x1 = std::max(x1, m.c_cx[c]);
+            y0 = std::min(y0, m.c_cy[c]); y1 = std::max(y1, m.c_cy[c]);
+        }
+        const bool by_x = (x1 - x0) >= (y1 - y0);
+        const std::vector<double>& key = by_x ? m.c_cx : m.c_cy;
+        const int32_t tiles = (n + tile - 1) / tile;
+        const int32_t mid = lo + (tiles / 2) * tile;
+        std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int32_t a, int32_t b) {
+            return key[a] != key[b] ? key[a] < key[b] : a < b;
+        });
+        stack.push_back({mid, hi});   // right after left
+        stack.push_back({lo, mid});
+    }
+    return out;
+}
+
 }  // namespace fvh
 
 // ---------------------------------------------------------------------------
@@ -618,8 +656,8 @@ int64_t fvh_mesh_export(const fvh_mesh* m, const char* field, void* dst, int64_t
     else if (f == "boundaryEdges") iv = &m->bnd_edge;
     else if (f == "boundaryTags") iv = &m->bnd_tag;
     else if (f == "cellsIsTriangle") { tmp.assign(m->cell_tri.begin(), m->cell_tri.end()); iv = &tmp; }
-    else if (f == "devicePermutation") {
-        std::vector<int32_t> order = fvh::rcm_order(*m);
+    else if (f == "rcmPermutation" || f == "tilePermutation") {
+        std::vector<int32_t> order = f == "rcmPermutation" ? fvh::rcm_order(*m) : fvh::rcb_order(*m, 256);
         tmp.assign(m->n_cells, 0);
         for (int32_t k = 0; k < (int32_t)order.size(); ++k) tmp[order[k]] = k;
         for (int32_t g = m->n_real; g < m->n_cells; ++g) tmp[g] = g;

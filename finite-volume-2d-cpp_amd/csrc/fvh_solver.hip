@@ -17,7 +17,6 @@
 #include "fvh_kernels.h"
 
 namespace fvh {
-
 namespace {
 
 const char* kKernelNames[] = {"dt", "dt_min", "gradlim", "flux", "ctrl_begin", "ctrl_end", "bounds_init"};
@@ -35,8 +34,12 @@ enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_COUNT };
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { if (p) (void)hipFree(p); }
     int alloc(size_t n) {
+        if (p) { (void)hipFree(p); p = nullptr; }
         bytes = n;
         if (n == 0) n = 8;
         return hipMalloc(&p, n) == hipSuccess ? 0 : -1;
@@ -44,7 +47,6 @@ struct DevBuf {
 };
 
 }  // namespace
-
 }  // namespace fvh
 
 struct fvh_solver {
@@ -52,14 +54,18 @@ struct fvh_solver {
     int device = 0;
     hipStream_t stream = nullptr;
     bool f32 = false;
+    bool second = false;                  // gradient/limiter/reconstruction pass present
     int32_t n_real = 0, n_total = 0, n_edges = 0, n_he = 0, n_bnd = 0;
     int32_t dev_ref0 = 0;                 // device position of reference cell 0
     std::vector<int32_t> pos;             // reference cell -> device position
     std::vector<int32_t> inv;             // device position -> reference cell
     // device buffers
-    fvh::DevBuf he_off, he_nbr, he_edge, he_twin, he_bc;
+    fvh::DevBuf he_off, he_nbr, he_edge, he_twin, he_bc, he_slot;
     fvh::DevBuf e_geo, e_c, c_area, c_inv_area, c_sqrt_area, c_c;
-    fvh::DevBuf q[3], dt, dt_global, incr, blk_min, res_blk, ctrl;
+    fvh::DevBuf t_off, t_edge, t_he, t_cell;
+    int32_t n_tiles = 0;
+    bool use_tiles = false;
+    fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs;
     fvh::DevBuf dbg[4];
     int cur = 0;                          // index of the current state buffer
     // profiling
@@ -90,6 +96,7 @@ MeshView<T> view(fvh_solver* s) {
     m.he_edge = (const uint32_t*)s->he_edge.p;
     m.he_twin = (const int32_t*)s->he_twin.p;
     m.he_bc = (const int32_t*)s->he_bc.p;
+    m.he_slot = (const uint16_t*)s->he_slot.p;
     m.e_geo = (const Q4<T>*)s->e_geo.p;
     m.e_c = (const G2<T>*)s->e_c.p;
     m.c_area = (const T*)s->c_area.p;
@@ -97,6 +104,16 @@ MeshView<T> view(fvh_solver* s) {
     m.c_sqrt_area = (const T*)s->c_sqrt_area.p;
     m.c_c = (const G2<T>*)s->c_c.p;
     return m;
+}
+
+TileView tile_view(fvh_solver* s) {
+    TileView t;
+    t.n_tiles = s->n_tiles;
+    t.off = (const int32_t*)s->t_off.p;
+    t.edge = (const int32_t*)s->t_edge.p;
+    t.he = (const int2*)s->t_he.p;
+    t.cell = (const int2*)s->t_cell.p;
+    return t;
 }
 
 PhysParams phys(const fvh_config& c) {
@@ -111,14 +128,14 @@ PhysParams phys(const fvh_config& c) {
 }
 
 template <class T>
-BcTable<T> bc_table(const fvh_config& c) {
-    BcTable<T> t;
+std::vector<BcParam<T>> bc_table(const fvh_config& c) {
+    std::vector<BcParam<T>> t(kMaxBc);
     for (int k = 0; k < kMaxBc; ++k) {
-        t.p[k].kind = k < c.n_bc ? c.bc[k].kind : 0;
-        t.p[k].rho = T(k < c.n_bc ? c.bc[k].rho : 0);
-        t.p[k].u = T(k < c.n_bc ? c.bc[k].u : 0);
-        t.p[k].v = T(k < c.n_bc ? c.bc[k].v : 0);
-        t.p[k].p = T(k < c.n_bc ? c.bc[k].p : 0);
+        t[k].kind = k < c.n_bc ? c.bc[k].kind : 0;
+        t[k].rho = T(k < c.n_bc ? c.bc[k].rho : 0);
+        t[k].u = T(k < c.n_bc ? c.bc[k].u : 0);
+        t[k].v = T(k < c.n_bc ? c.bc[k].v : 0);
+        t[k].p = T(k < c.n_bc ? c.bc[k].p : 0);
     }
     return t;
 }
@@ -166,11 +183,15 @@ int32_t drain_events(fvh_solver* s) {
     return FVH_OK;
 }
 
+int32_t upload(DevBuf& b, const void* src, size_t bytes) {
+    if (b.alloc(bytes)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+    if (bytes) FVH_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+    return FVH_OK;
+}
+
 template <class T>
-int32_t upload_geometry(fvh_solver* s, const fvh_mesh& m, const std::vector<int32_t>& he_off,
-                        const std::vector<int32_t>& he_nbr, const std::vector<uint32_t>& he_edge,
-                        const std::vector<int32_t>& he_twin, const std::vector<int32_t>& he_bc,
-                        const std::vector<int32_t>& dev_edge_ref, const std::vector<double>& gg) {
+int32_t upload_geometry(fvh_solver* s, const fvh_mesh& m, const std::vector<int32_t>& dev_edge_ref,
+                        const std::vector<double>& gg) {
     const int32_t E = s->n_edges, N = s->n_total;
     std::vector<Q4<T>> geo(E);
     std::vector<G2<T>> ec(E);
@@ -194,45 +215,43 @@ int32_t upload_geometry(fvh_solver* s, const fvh_mesh& m, const std::vector<int3
         cc[d].v[0] = T(m.c_cx[r]);
         cc[d].v[1] = T(m.c_cy[r]);
     }
-    auto up = [&](DevBuf& b, const void* src, size_t bytes) -> int32_t {
-        if (b.alloc(bytes)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
-        if (bytes) FVH_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
-        return FVH_OK;
-    };
     int32_t rc;
-    if ((rc = up(s->he_off, he_off.data(), he_off.size() * 4))) return rc;
-    if ((rc = up(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
-    if ((rc = up(s->he_edge, he_edge.data(), he_edge.size() * 4))) return rc;
-    if ((rc = up(s->he_twin, he_twin.data(), he_twin.size() * 4))) return rc;
-    if ((rc = up(s->he_bc, he_bc.data(), he_bc.size() * 4))) return rc;
-    if ((rc = up(s->e_geo, geo.data(), geo.size() * sizeof(Q4<T>)))) return rc;
-    if ((rc = up(s->e_c, ec.data(), ec.size() * sizeof(G2<T>)))) return rc;
-    if ((rc = up(s->c_area, area.data(), N * sizeof(T)))) return rc;
-    if ((rc = up(s->c_inv_area, ia.data(), N * sizeof(T)))) return rc;
-    if ((rc = up(s->c_sqrt_area, sa.data(), N * sizeof(T)))) return rc;
-    if ((rc = up(s->c_c, cc.data(), N * sizeof(G2<T>)))) return rc;
+    const std::vector<BcParam<T>> bt = bc_table<T>(s->cfg);
+    if ((rc = upload(s->bcs, bt.data(), bt.size() * sizeof(BcParam<T>)))) return rc;
+    if ((rc = upload(s->e_geo, geo.data(), geo.size() * sizeof(Q4<T>)))) return rc;
+    if ((rc = upload(s->e_c, ec.data(), ec.size() * sizeof(G2<T>)))) return rc;
+    if ((rc = upload(s->c_area, area.data(), N * sizeof(T)))) return rc;
+    if ((rc = upload(s->c_inv_area, ia.data(), N * sizeof(T)))) return rc;
+    if ((rc = upload(s->c_sqrt_area, sa.data(), N * sizeof(T)))) return rc;
+    if ((rc = upload(s->c_c, cc.data(), N * sizeof(G2<T>)))) return rc;
     for (int k = 0; k < 3; ++k)
         if (s->q[k].alloc((size_t)N * sizeof(Q4<T>))) { set_error("hipMalloc failed"); return FVH_E_HIP; }
     if (s->dt.alloc((size_t)N * sizeof(T)) || s->dt_global.alloc(8) ||
-        s->incr.alloc((size_t)s->n_he * sizeof(Q4<T>)) ||
+        s->face.alloc((size_t)(s->second ? s->n_he : 1) * sizeof(Q4<T>)) ||
+        s->bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<T>)) ||
         s->blk_min.alloc((size_t)((N + kBlock - 1) / kBlock + 1) * 8) ||
         s->res_blk.alloc((size_t)((s->n_real + kBlock - 1) / kBlock + 1) * 32) || s->ctrl.alloc(sizeof(Ctrl))) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
     FVH_HIP(hipMemset(s->ctrl.p, 0, sizeof(Ctrl)));
-    FVH_HIP(hipMemset(s->incr.p, 0, s->incr.bytes ? s->incr.bytes : 8));
+    FVH_HIP(hipMemset(s->face.p, 0, s->face.bytes));
+    FVH_HIP(hipMemset(s->bincr.p, 0, s->bincr.bytes));
     return FVH_OK;
 }
 
 // Queue the kernels of one RK step (explicit.cpp:729-747) reading state
 // buffer s->cur.  run_mode adds the stop test and bookkeeping kernels.
+// dbg (tests): 4 device buffers for gx, gy, limiter and qt of one stage run
+// with alpha_override, leaving the current state buffer unchanged.
 template <class T>
-int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res) {
+int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res,
+                     Q4<T>* const* dbg = nullptr, const double* alpha_override = nullptr) {
     const fvh_config& c = s->cfg;
     const MeshView<T> m = view<T>(s);
+    const TileView tv = tile_view(s);
     const PhysParams pp = phys(c);
-    const BcTable<T> bcs = bc_table<T>(c);
+    const BcParam<T>* bcs = (const BcParam<T>*)s->bcs.p;
     const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
                lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
     Ctrl* ctrl = (Ctrl*)s->ctrl.p;
@@ -246,43 +265,49 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     Q4<T>* qa = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
     Q4<T>* qb = (Q4<T>*)s->q[(s->cur + 2) % 3].p;
     T* dt = (T*)s->dt.p;
-    {
+    double* blk = (double*)s->blk_min.p;
+    Q4<T>* face = (Q4<T>*)s->face.p;
+    Q4<T>* bincr = (Q4<T>*)s->bincr.p;
+    if (!s->second) {
         Bracket b(s, K_DT);
-        launch_dt<T>(st, m, c.dt_kind, pp, q0, dt, (double*)s->blk_min.p, cc);
-    }
-    if (gdt) {
-        Bracket b(s, K_DTMIN);
-        launch_dt_min(st, (const double*)s->blk_min.p, (s->n_total + kBlock - 1) / kBlock, dt, s->f32,
-                      s->dev_ref0, (double*)s->dt_global.p, cc);
+        launch_dt<T>(st, m, c.dt_kind, pp, q0, dt, blk, cc);
     }
     const T* dtp = gdt ? (const T*)s->dt_global.p : dt;
     const int dt_stride = gdt ? 0 : 1;
     const Q4<T>* qin = q0;
     Q4<T>* out = qa;
-    for (int k = 0; k < c.n_alpha; ++k) {
-        if (grad || lim || lin) {
+    const int n_stage = alpha_override ? 1 : c.n_alpha;
+    for (int k = 0; k < n_stage; ++k) {
+        if (s->second) {
+            // the first stage's pass also computes the time step from q0
             Bracket b(s, K_GRADLIM);
-            launch_gradlim<T>(st, m, pp, c.phi, grad, lim, lin, qin, (Q4<T>*)s->incr.p, nullptr, nullptr,
-                              nullptr, cc);
+            launch_gradlim<T>(st, m, pp, c.phi, grad, lim, lin, qin, face, bincr, k == 0 ? c.dt_kind : -1, dt,
+                              blk, dbg ? dbg[0] : nullptr, dbg ? dbg[1] : nullptr, dbg ? dbg[2] : nullptr, cc);
         }
-        const bool last = k == c.n_alpha - 1;
+        if (k == 0 && gdt) {
+            Bracket b(s, K_DTMIN);
+            launch_dt_min(st, blk, (s->n_total + kBlock - 1) / kBlock, dt, s->f32, s->dev_ref0,
+                          (double*)s->dt_global.p, cc);
+        }
+        const bool last = k == n_stage - 1;
         {
             Bracket b(s, K_FLUX);
-            launch_flux<T>(st, m, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs, qin, q0, out,
-                           (const Q4<T>*)s->incr.p, dtp, dt_stride, nullptr, T(c.alpha[k]),
-                           (last && need_res) ? (double*)s->res_blk.p : nullptr, nullptr, cc);
+            launch_flux<T>(st, m, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs, qin,
+                           q0, out, face, bincr, dtp, dt_stride,
+                           T(alpha_override ? *alpha_override : c.alpha[k]),
+                           (last && need_res) ? (double*)s->res_blk.p : nullptr, dbg ? dbg[3] : nullptr, cc);
         }
         qin = out;
         out = (out == qa) ? qb : qa;
     }
-    {
+    if (!dbg) {
         Bracket b(s, K_END);
         launch_ctrl_end(st, ctrl, (const double*)s->res_blk.p, (s->n_real + kBlock - 1) / kBlock, need_res,
                         gdt ? s->dt_global.p : s->dt.p, s->f32, gdt ? 0 : s->dev_ref0);
     }
     FVH_HIP(hipGetLastError());
     // the last stage wrote qin; it becomes the current state
-    s->cur = (qin == qa) ? (s->cur + 1) % 3 : (s->cur + 2) % 3;
+    if (!dbg) s->cur = (qin == qa) ? (s->cur + 1) % 3 : (s->cur + 2) % 3;
     return FVH_OK;
 }
 
@@ -313,12 +338,7 @@ int32_t get_q4(fvh_solver* s, const void* dev, double* out, int32_t count) {
 template <class T>
 int32_t phases_t(fvh_solver* s, double alpha, double* dt_out, double* gx, double* gy, double* lim,
                  double* qt, double* qk) {
-    const fvh_config& c = s->cfg;
-    const MeshView<T> m = view<T>(s);
-    const PhysParams pp = phys(c);
-    const BcTable<T> bcs = bc_table<T>(c);
-    const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim_on = c.flags & FVH_FLAG_LIMITERS,
-               lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
+    const bool gdt = s->cfg.flags & FVH_FLAG_GLOBAL_DT;
     const size_t qbytes = (size_t)s->n_total * sizeof(Q4<T>);
     for (int k = 0; k < 4; ++k) {
         if (!s->dbg[k].p && s->dbg[k].alloc(qbytes)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
@@ -328,38 +348,21 @@ int32_t phases_t(fvh_solver* s, double alpha, double* dt_out, double* gx, double
     Q4<T>* out = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
     // update_cells runs over every cell; ghost cells keep q (their qt is 0)
     FVH_HIP(hipMemcpyAsync(out, q0, qbytes, hipMemcpyDeviceToDevice, s->stream));
-    T* dt = (T*)s->dt.p;
-    launch_dt<T>(s->stream, m, c.dt_kind, pp, q0, dt, (double*)s->blk_min.p, nullptr);
-    if (gdt)
-        launch_dt_min(s->stream, (const double*)s->blk_min.p, (s->n_total + kBlock - 1) / kBlock, dt, s->f32,
-                      s->dev_ref0, (double*)s->dt_global.p, nullptr);
-    Q4<T>* dgx = (Q4<T>*)s->dbg[0].p;
-    Q4<T>* dgy = (Q4<T>*)s->dbg[1].p;
-    Q4<T>* dlim = (Q4<T>*)s->dbg[2].p;
-    Q4<T>* dqt = (Q4<T>*)s->dbg[3].p;
-    if (grad || lim_on || lin)
-        launch_gradlim<T>(s->stream, m, pp, c.phi, grad, lim_on, lin, q0, (Q4<T>*)s->incr.p, dgx, dgy, dlim,
-                          nullptr);
-    else {
-        // neither pass ran: gradients 0, limiters 0 (zero-initialised vectors)
-    }
-    launch_flux<T>(s->stream, m, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs, q0, q0, out,
-                   (const Q4<T>*)s->incr.p, gdt ? (const T*)s->dt_global.p : dt, gdt ? 0 : 1, nullptr, T(alpha),
-                   nullptr, dqt, nullptr);
-    FVH_HIP(hipGetLastError());
+    Q4<T>* dbg[4] = {(Q4<T>*)s->dbg[0].p, (Q4<T>*)s->dbg[1].p, (Q4<T>*)s->dbg[2].p, (Q4<T>*)s->dbg[3].p};
+    int32_t rc = enqueue_step<T>(s, false, nullptr, 0, dbg, &alpha);
+    if (rc) return rc;
     FVH_HIP(hipStreamSynchronize(s->stream));
     if (dt_out) {
         std::vector<T> h(s->n_total);
         T g = T(0);
-        FVH_HIP(hipMemcpy(h.data(), dt, h.size() * sizeof(T), hipMemcpyDeviceToHost));
+        FVH_HIP(hipMemcpy(h.data(), s->dt.p, h.size() * sizeof(T), hipMemcpyDeviceToHost));
         if (gdt) FVH_HIP(hipMemcpy(&g, s->dt_global.p, sizeof(T), hipMemcpyDeviceToHost));
         for (int32_t d = 0; d < s->n_total; ++d) dt_out[s->inv[d]] = (double)(gdt ? g : h[d]);
     }
-    int32_t rc = FVH_OK;
-    if (gx && (rc = get_q4<T>(s, dgx, gx, s->n_total))) return rc;
-    if (gy && (rc = get_q4<T>(s, dgy, gy, s->n_total))) return rc;
-    if (lim && (rc = get_q4<T>(s, dlim, lim, s->n_total))) return rc;
-    if (qt && (rc = get_q4<T>(s, dqt, qt, s->n_total))) return rc;
+    if (gx && (rc = get_q4<T>(s, dbg[0], gx, s->n_total))) return rc;
+    if (gy && (rc = get_q4<T>(s, dbg[1], gy, s->n_total))) return rc;
+    if (lim && (rc = get_q4<T>(s, dbg[2], lim, s->n_total))) return rc;
+    if (qt && (rc = get_q4<T>(s, dbg[3], qt, s->n_total))) return rc;
     if (qk && (rc = get_q4<T>(s, out, qk, s->n_total))) return rc;
     return FVH_OK;
 }
@@ -369,8 +372,8 @@ int32_t apply_bounds_t(fvh_solver* s) {
     const fvh_config& c = s->cfg;
     {
         Bracket b(s, K_BINIT);
-        launch_bounds_init<T>(s->stream, view<T>(s), phys(c), (c.flags & FVH_FLAG_LINEAR) != 0, bc_table<T>(c),
-                              (Q4<T>*)s->q[s->cur].p);
+        launch_bounds_init<T>(s->stream, view<T>(s), phys(c), (c.flags & FVH_FLAG_LINEAR) != 0,
+                              (const BcParam<T>*)s->bcs.p, (Q4<T>*)s->q[s->cur].p);
     }
     FVH_HIP(hipGetLastError());
     Ctrl z;
@@ -421,6 +424,49 @@ int32_t run_t(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
     return drain_events(s);
 }
 
+// Tile edge lists for the tiled flux pass.  Returns false when a tile touches
+// more edges than its LDS holds.
+bool build_tiles(fvh_solver* s, const std::vector<int32_t>& off, const std::vector<uint32_t>& he_edge,
+                 std::vector<uint16_t>& he_slot, std::vector<int32_t>& t_off, std::vector<int32_t>& t_edge,
+                 std::vector<int32_t>& t_he, std::vector<int32_t>& t_cell) {
+    const int32_t E = s->n_edges;
+    std::vector<int32_t> eh(2 * (size_t)E, -1), ecell(2 * (size_t)E, -1);
+    for (int32_t d = 0; d < s->n_total; ++d)
+        for (int32_t h = off[d]; h < off[d + 1]; ++h) {
+            const uint32_t ue = he_edge[h];
+            const int32_t e = (int32_t)(ue & kEdgeMask);
+            const int sd = (ue & kSideBit) ? 1 : 0;
+            eh[2 * (size_t)e + sd] = h;
+            ecell[2 * (size_t)e + sd] = d;
+        }
+    s->n_tiles = (s->n_real + kBlock - 1) / kBlock;
+    t_off.assign(s->n_tiles + 1, 0);
+    std::vector<int32_t> local;
+    std::vector<int32_t> slot_of(E, -1);
+    for (int32_t t = 0; t < s->n_tiles; ++t) {
+        const int32_t c0 = t * kBlock, c1 = std::min(s->n_real, c0 + kBlock);
+        local.clear();
+        for (int32_t d = c0; d < c1; ++d)
+            for (int32_t h = off[d]; h < off[d + 1]; ++h) local.push_back((int32_t)(he_edge[h] & kEdgeMask));
+        std::sort(local.begin(), local.end());
+        local.erase(std::unique(local.begin(), local.end()), local.end());
+        if ((int)local.size() > kTileEdges) return false;
+        for (size_t k = 0; k < local.size(); ++k) {
+            const int32_t e = local[k];
+            slot_of[e] = (int32_t)k;
+            t_edge.push_back(e);
+            t_he.push_back(eh[2 * (size_t)e]);
+            t_he.push_back(eh[2 * (size_t)e + 1]);
+            t_cell.push_back(ecell[2 * (size_t)e]);
+            t_cell.push_back(ecell[2 * (size_t)e + 1]);
+        }
+        for (int32_t d = c0; d < c1; ++d)
+            for (int32_t h = off[d]; h < off[d + 1]; ++h) he_slot[h] = (uint16_t)slot_of[he_edge[h] & kEdgeMask];
+        t_off[t + 1] = t_off[t] + (int32_t)local.size();
+    }
+    return true;
+}
+
 }  // namespace
 }  // namespace fvh
 
@@ -447,7 +493,8 @@ void fvh_config_default(fvh_config* c) {
     const double rk5[5] = {0.0695, 0.1602, 0.2898, 0.506, 1.};
     c->n_alpha = 5;
     for (int k = 0; k < 5; ++k) c->alpha[k] = rk5[k];
-    c->renumber = 1;
+    c->renumber = FVH_ORDER_TILES;
+    c->flux_mode = FVH_FLUX_MODE_AUTO;
 }
 
 int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** out) {
@@ -457,8 +504,9 @@ int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** o
         return FVH_E_ARG;
     }
     if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
-    if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2) {
-        set_error("unknown flux, limiter or time-step kind");
+    if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2 ||
+        c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 2) {
+        set_error("unknown flux, limiter, time-step, order or flux-mode kind");
         return FVH_E_ARG;
     }
     for (int k = 0; k < c->n_bc; ++k)
@@ -476,20 +524,23 @@ int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** o
             return FVH_E_BC;
         }
     }
+    if (m->n_edges() >= (int32_t)kEdgeMask) { set_error("too many edges"); return FVH_E_UNSUPPORTED; }
     std::unique_ptr<fvh_solver> s(new fvh_solver());
     s->cfg = *c;
     s->f32 = c->precision == FVH_F32;
+    s->second = (c->flags & (FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR)) != 0;
     if (hipGetDevice(&s->device) != hipSuccess) { set_error("no HIP device"); return FVH_E_HIP; }
     FVH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->n_real = m->n_real;
     s->n_total = m->n_cells;
     s->n_edges = m->n_edges();
     s->n_bnd = (int32_t)m->bnd_edge.size();
-    // device order: RCM over real cells, ghosts keep their order
+    // device order of the real cells; ghosts keep theirs
     s->pos.assign(s->n_total, 0);
     s->inv.assign(s->n_total, 0);
     std::vector<int32_t> order;
-    if (c->renumber) order = rcm_order(*m);
+    if (c->renumber == FVH_ORDER_RCM) order = rcm_order(*m);
+    else if (c->renumber == FVH_ORDER_TILES) order = rcb_order(*m, kBlock);
     else { order.resize(m->n_real); for (int32_t i = 0; i < m->n_real; ++i) order[i] = i; }
     for (int32_t k = 0; k < m->n_real; ++k) { s->pos[order[k]] = k; s->inv[k] = order[k]; }
     for (int32_t g = m->n_real; g < m->n_cells; ++g) { s->pos[g] = g; s->inv[g] = g; }
@@ -525,10 +576,9 @@ int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** o
         if (dev_of_edge[e] < 0) { dev_of_edge[e] = (int32_t)dev_edge_ref.size(); dev_edge_ref.push_back(e); }
         uint32_t w = (uint32_t)dev_of_edge[e];
         if (he_side[h]) w |= kSideBit;
-        if (he_bc[h] >= 0) w |= 0x40000000u;
+        if (he_bc[h] >= 0) w |= kBcBit;
         he_edge[h] = w;
     }
-    if (E >= 0x40000000) { set_error("too many edges"); return FVH_E_UNSUPPORTED; }
     // Green-Gauss weight per edge, as calc_gradients computes it (explicit.cpp:132-140)
     std::vector<double> gg(E);
     for (int32_t e = 0; e < E; ++e) {
@@ -541,15 +591,37 @@ int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** o
         const double dij = std::sqrt(dxij * dxij + dyij * dyij);
         gg[e] = dif / dij;
     }
-    int32_t rc = s->f32 ? upload_geometry<float>(s.get(), *m, off, he_nbr, he_edge, he_twin, he_bc, dev_edge_ref, gg)
-                        : upload_geometry<double>(s.get(), *m, off, he_nbr, he_edge, he_twin, he_bc, dev_edge_ref, gg);
+    // tiles for the tiled flux pass
+    std::vector<uint16_t> he_slot(s->n_he, 0);
+    std::vector<int32_t> t_off, t_edge, t_he, t_cell;
+    s->use_tiles = false;
+    if (c->flux_mode != FVH_FLUX_MODE_CELL && s->n_real > 0)
+        s->use_tiles = build_tiles(s.get(), off, he_edge, he_slot, t_off, t_edge, t_he, t_cell);
+    if (c->flux_mode == FVH_FLUX_MODE_TILE && !s->use_tiles && s->n_real > 0) {
+        set_error("a tile touches more than 1024 edges; use the per-cell flux mode");
+        return FVH_E_UNSUPPORTED;
+    }
+    int32_t rc;
+    if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
+    if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
+    if ((rc = upload(s->he_edge, he_edge.data(), he_edge.size() * 4))) return rc;
+    if ((rc = upload(s->he_twin, he_twin.data(), he_twin.size() * 4))) return rc;
+    if ((rc = upload(s->he_bc, he_bc.data(), he_bc.size() * 4))) return rc;
+    if ((rc = upload(s->he_slot, he_slot.data(), he_slot.size() * 2))) return rc;
+    if (s->use_tiles) {
+        if ((rc = upload(s->t_off, t_off.data(), t_off.size() * 4))) return rc;
+        if ((rc = upload(s->t_edge, t_edge.data(), t_edge.size() * 4))) return rc;
+        if ((rc = upload(s->t_he, t_he.data(), t_he.size() * 4))) return rc;
+        if ((rc = upload(s->t_cell, t_cell.data(), t_cell.size() * 4))) return rc;
+    }
+    rc = s->f32 ? upload_geometry<float>(s.get(), *m, dev_edge_ref, gg)
+                : upload_geometry<double>(s.get(), *m, dev_edge_ref, gg);
     if (rc) return rc;
     // algorithmic bytes per stage (state s, geometry g bytes)
     const double sb = s->f32 ? 4.0 : 8.0, gb = sb, V = 4.0;
     const double C = s->n_real, B = s->n_bnd;
     double bytes = C * (3 * V * sb + sb + gb) + E * (3 * gb + 8) + B * (2 * V * sb + 2 * gb + 4);
-    if (c->flags & (FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR))
-        bytes += C * (7 * V * sb + 2 * gb) + E * (2 * gb);
+    if (s->second) bytes += C * (7 * V * sb + 2 * gb) + E * (2 * gb);
     s->stage_bytes = bytes;
     *out = s.release();
     return FVH_OK;

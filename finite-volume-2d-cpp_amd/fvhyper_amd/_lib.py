@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("dt_kind", C.c_int32), ("dt_fixed", C.c_double), ("cfl", C.c_double),
         ("flags", C.c_uint32), ("n_alpha", C.c_int32), ("alpha", C.c_double * FVH_MAX_ALPHA),
         ("n_bc", C.c_int32), ("bc", BcSpec * FVH_MAX_BC),
-        ("renumber", C.c_int32), ("block_size", C.c_int32),
+        ("renumber", C.c_int32), ("flux_mode", C.c_int32),
     ]
 
 

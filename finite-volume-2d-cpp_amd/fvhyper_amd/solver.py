@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import check, dptr, iptr
 
 _INT_FIELDS = {"edgesNodes", "edgesCells", "cellsNodes", "cellsIsTriangle", "boundaryEdges",
-               "boundaryTags", "devicePermutation"}
+               "boundaryTags", "rcmPermutation", "tilePermutation"}
 
 
 class Mesh:
@@ -72,7 +72,11 @@ class Mesh:
             pass
 
 
-def make_config(cfg: Dict, precision: str = "f64", renumber: bool = True) -> _lib.Config:
+ORDERS = {"file": 0, "rcm": 1, "tiles": 2}
+FLUX_MODES = {"auto": 0, "cell": 1, "tile": 2}
+
+
+def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: str = "auto") -> _lib.Config:
     """fvh_config from the shared configuration dict (see oracle/oracle.py)."""
     c = _lib.Config()
     _lib.load().fvh_config_default(C.byref(c))
@@ -113,19 +117,25 @@ def make_config(cfg: Dict, precision: str = "f64", renumber: bool = True) -> _li
         c.bc[k].name = name.encode()
         c.bc[k].kind = _lib.BC[kind]
         c.bc[k].rho, c.bc[k].u, c.bc[k].v, c.bc[k].p = [float(v) for v in par[:4]]
-    c.renumber = 1 if renumber else 0
+    if renumber is True:
+        renumber = "rcm"
+    elif renumber is False:
+        renumber = "file"
+    c.renumber = ORDERS[renumber]
+    c.flux_mode = FLUX_MODES[flux_mode]
     return c
 
 
 class Solver:
     """Explicit FV solver on the current HIP device (one stream per handle)."""
 
-    def __init__(self, mesh: Mesh, cfg: Dict, precision: str = "f64", renumber: bool = True):
+    def __init__(self, mesh: Mesh, cfg: Dict, precision: str = "f64", renumber="tiles",
+                 flux_mode: str = "auto"):
         self.mesh = mesh
         self.cfg = dict(cfg)
         self.precision = precision
         L = _lib.load()
-        c = make_config(cfg, precision, renumber)
+        c = make_config(cfg, precision, renumber, flux_mode)
         h = C.c_void_p()
         check(L.fvh_solver_create(mesh._h, C.byref(c), C.byref(h)))
         self._h = h

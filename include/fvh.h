@@ -51,6 +51,14 @@ enum { FVH_BC_COPY = 0,          /* shockTube/main.cpp:158-163 */
        FVH_BC_INLET_OUTLET = 2,  /* forwardStep/main.cpp:194-261 */
        FVH_BC_FARFIELD = 3       /* naca/main.cpp:192-259 */ };
 enum { FVH_F64 = 0, FVH_F32 = 1 };
+/* Device cell order (results do not depend on it, only locality does). */
+enum { FVH_ORDER_FILE = 0,       /* reference numbering */
+       FVH_ORDER_RCM = 1,        /* reverse Cuthill-McKee */
+       FVH_ORDER_TILES = 2       /* recursive coordinate bisection into 256-cell tiles */ };
+/* Flux pass: each tile of 256 cells computes every edge flux it touches once
+ * into LDS (TILE), or each cell recomputes the flux of each of its edges
+ * (CELL).  AUTO picks TILE. */
+enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2 };
 
 /* solver:: flags of fvhyper/explicit.h:35-40 */
 #define FVH_FLAG_GRADIENTS   1u   /* do_calc_gradients */
@@ -87,8 +95,8 @@ typedef struct fvh_config {
     double alpha[FVH_MAX_ALPHA];
     int32_t n_bc;
     fvh_bc_spec bc[FVH_MAX_BC];
-    int32_t renumber;          /* 1: reverse Cuthill-McKee device order (default) */
-    int32_t block_size;        /* 0: library default */
+    int32_t renumber;          /* device cell order: FVH_ORDER_* */
+    int32_t flux_mode;         /* FVH_FLUX_MODE_* */
 } fvh_config;
 
 /* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */
@@ -137,7 +145,8 @@ int32_t fvh_mesh_counts(const fvh_mesh* m, int64_t* counts);
  * double fields: nodesX nodesY edgesLengths edgesNormalsX edgesNormalsY
  *   edgesCentersX edgesCentersY cellsAreas cellsCentersX cellsCentersY
  * int32 fields: edgesNodes(2/edge) edgesCells(2/edge) cellsNodes(4/cell)
- *   cellsIsTriangle boundaryEdges boundaryTags devicePermutation
+ *   cellsIsTriangle boundaryEdges boundaryTags rcmPermutation tilePermutation
+ *   (the permutations give each reference cell's device position)
  * Returns the number of elements (dst may be NULL to query it). */
 int64_t fvh_mesh_export(const fvh_mesh* m, const char* field, void* dst, int64_t n);
 

@@ -10,20 +10,21 @@ from helpers import CONFIGS, MESHES, bits_equal, random_state, rel_err, same_val
 pytestmark = pytest.mark.gpu
 
 
-def _pair(mesh_name, cfg, precision="f64", renumber=True):
+def _pair(mesh_name, cfg, precision="f64", renumber="tiles", flux_mode="auto"):
     m = MESHES[mesh_name]()
     om = O.Mesh(m)
     pm = Mesh.from_arrays(m)
-    s = Solver(pm, cfg, precision=precision, renumber=renumber)
+    s = Solver(pm, cfg, precision=precision, renumber=renumber, flux_mode=flux_mode)
     return m, om, pm, s
 
 
+@pytest.mark.parametrize("flux_mode", ["tile", "cell"])
 @pytest.mark.parametrize("mesh_name", sorted(MESHES))
 @pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
-def test_phases_bit_exact(mesh_name, cfg_name):
-    print("case", mesh_name, cfg_name, flush=True)
+def test_phases_bit_exact(mesh_name, cfg_name, flux_mode):
+    print("case", mesh_name, cfg_name, flux_mode, flush=True)
     cfg = CONFIGS[cfg_name]
-    m, om, pm, s = _pair(mesh_name, cfg)
+    m, om, pm, s = _pair(mesh_name, cfg, flux_mode=flux_mode)
     q = random_state(om.n_cells, seed=101)
     s.set_state(q)
     g = s.phases(alpha=0.2898)
@@ -32,13 +33,13 @@ def test_phases_bit_exact(mesh_name, cfg_name):
         assert bits_equal(g[key], o[key]), (key, np.max(np.abs(g[key] - o[key])))
 
 
-@pytest.mark.parametrize("renumber", [True, False])
+@pytest.mark.parametrize("order,flux_mode", [("tiles", "tile"), ("rcm", "tile"), ("file", "cell")])
 @pytest.mark.parametrize("mesh_name", sorted(MESHES))
 @pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
-def test_run_bit_exact(mesh_name, cfg_name, renumber):
-    print("case", mesh_name, cfg_name, renumber, flush=True)
+def test_run_bit_exact(mesh_name, cfg_name, order, flux_mode):
+    print("case", mesh_name, cfg_name, order, flux_mode, flush=True)
     cfg = CONFIGS[cfg_name]
-    m, om, pm, s = _pair(mesh_name, cfg, renumber=renumber)
+    m, om, pm, s = _pair(mesh_name, cfg, renumber=order, flux_mode=flux_mode)
     q0 = random_state(om.n_cells, seed=202, mach=0.3)
     s.set_state(q0)
     st = s.run(max_step=12, print_interval=5)
@@ -104,3 +105,34 @@ def test_forward_step_full_size_two_steps():
     qo, steps, t, R, R0, _ = om.run(cfg, q0, 2)
     assert st["step"] == steps == 2 and st["time"] == t
     assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
+
+
+def test_positivity_fallback_bit_exact():
+    """The forward-step workload at 40 cells per unit for 150 RK5 steps: the
+    positivity fallback triggers at the step corner from step ~56 on; GPU and
+    oracle must take it identically."""
+    m, cfg, ic = workloads.forward_step(n_per_unit=40)
+    assert cfg["positivity"]
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_step=150)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 150)
+    assert st["step"] == steps == 150 and st["time"] == t
+    assert np.all(np.isfinite(qo))
+    assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
+
+
+def test_reference_scheme_blows_up_identically():
+    """Without the fallback the reference scheme goes non-finite at the step
+    corner; the GPU reproduces the same trajectory up to and through it."""
+    m, cfg, ic = workloads.forward_step(n_per_unit=40, reference_compatible=True)
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_step=130)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 130)
+    assert not np.all(np.isfinite(qo))
+    assert same_values(s.get_state(), qo)
