@@ -18,6 +18,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfvh.so")
 SOURCES = ["fvh_mesh.cpp", "fvh_kernels.hip", "fvh_solver.hip"]
+DROPIN = os.path.join(HERE, "dropin")
+DROPIN_SOURCES = ["fvhyper_dropin.cpp", "probe.cpp"]
+DROPIN_LIB = os.path.join(HERE, "libfvhyper.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
@@ -32,11 +35,11 @@ def flags(arch: str):
     ]
 
 
-def compile_one(src: str, arch: str) -> str:
+def compile_one(src: str, arch: str, srcdir: str = CSRC, extra=()) -> str:
     obj = os.path.join(OUT, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + flags(arch) + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + flags(arch) + list(extra) + ["-c", os.path.join(srcdir, src), "-o", obj]
     if src.endswith(".cpp"):
-        cmd = [HIPCC] + flags(arch) + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + flags(arch) + list(extra) + ["-x", "hip", "-c", os.path.join(srcdir, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
@@ -53,6 +56,17 @@ def build(arch: str = "gfx950", jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # the fvhyper drop-in layer (C++ headers with the reference's API)
+    inc = ["-I%s" % os.path.join(DROPIN, "include")]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        dobjs = list(ex.map(lambda s: compile_one(s, arch, os.path.join(DROPIN, "src"), inc), DROPIN_SOURCES))
+    tmp = DROPIN_LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=%s" % arch, "-shared", "-fPIC", "-o", tmp] + dobjs + \
+          ["-L%s" % HERE, "-lfvh", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, DROPIN_LIB)
     return LIB
 
 

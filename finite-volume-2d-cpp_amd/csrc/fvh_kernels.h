@@ -46,6 +46,7 @@ constexpr int kMaxBc = 16;
 // without host round trips; every kernel is a no-op once `stop` is set.
 struct Ctrl {
     double time;
+    double time_prev;  // time before the last step
     double dt0;        // dt of reference cell 0 in the current step
     double R[4];
     double R0[4];

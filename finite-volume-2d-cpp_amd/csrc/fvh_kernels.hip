@@ -527,6 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_ctrl_end(Ctrl* ctrl, const double* _
         const double d0 = (double)dt_src[dt_index];
         ctrl->dt0 = d0;
         ctrl->step += 1;
+        ctrl->time_prev = ctrl->time;
         ctrl->time += d0;
         for (int k = 0; k < 4; ++k)
             if (!(ctrl->R[k] == ctrl->R[k])) ctrl->nonfinite = 1;

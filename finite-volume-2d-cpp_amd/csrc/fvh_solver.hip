@@ -391,37 +391,46 @@ int32_t read_status(fvh_solver* s, fvh_status* st) {
     st->step = h.step;
     st->stopped = h.stop;
     st->time = h.time;
+    st->time_prev = h.time_prev;
     for (int k = 0; k < 4; ++k) { st->R[k] = h.R[k]; st->R0[k] = h.R0[k]; }
     return FVH_OK;
+}
+
+// Executes up to n more steps of run() from the current status (steps are
+// queued in growing chunks; kernels queued after the stop are no-ops).
+template <class T>
+int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_status* st) {
+    int32_t rc;
+    fvh_status now;
+    if ((rc = read_status(s, &now))) return rc;
+    const uint32_t print = o->print_interval ? o->print_interval : 1;
+    const bool every = o->tolerance > 1.01e-16;
+    uint64_t step = now.step;
+    const uint64_t first = step;
+    const int cur0 = s->cur;
+    // each executed step moves the state forward by one or two buffers
+    const int delta = (s->cfg.n_alpha % 2) ? 1 : 2;
+    uint64_t chunk = 8, left = n;
+    while (!now.stopped && left > 0) {
+        const uint64_t k_n = std::min<uint64_t>(chunk, left);
+        for (uint64_t k = 0; k < k_n; ++k, ++step) {
+            const int need = (step == 0) || (step % print == 0) || every;
+            if ((rc = enqueue_step<T>(s, true, o, need))) return rc;
+        }
+        left -= k_n;
+        if ((rc = read_status(s, &now))) return rc;
+        if (chunk < 256) chunk *= 2;
+    }
+    s->cur = (int)((cur0 + (uint64_t)(now.step - first) * delta) % 3);
+    *st = now;
+    return drain_events(s);
 }
 
 template <class T>
 int32_t run_t(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
     int32_t rc = apply_bounds_t<T>(s);
     if (rc) return rc;
-    const uint32_t print = o->print_interval ? o->print_interval : 1;
-    const bool every = o->tolerance > 1.01e-16;
-    uint64_t step = 0;
-    const int cur0 = s->cur;
-    // each executed step moves the state forward by one or two buffers
-    const int delta = (s->cfg.n_alpha % 2) ? 1 : 2;
-    // steps are queued in chunks; kernels after the stop are no-ops
-    uint32_t chunk = 8;
-    for (;;) {
-        for (uint32_t k = 0; k < chunk; ++k, ++step) {
-            const int need = (step == 0) || (step % print == 0) || every;
-            if ((rc = enqueue_step<T>(s, true, o, need))) return rc;
-        }
-        fvh_status now;
-        if ((rc = read_status(s, &now))) return rc;
-        if (now.stopped) {
-            *st = now;
-            s->cur = (int)((cur0 + (uint64_t)now.step * delta) % 3);
-            break;
-        }
-        if (chunk < 256) chunk *= 2;
-    }
-    return drain_events(s);
+    return run_continue_t<T>(s, o, ~0ull, st);
 }
 
 // Tile edge lists for the tiled flux pass.  Returns false when a tile touches
@@ -655,6 +664,16 @@ int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, d
 int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
     return s->f32 ? run_t<float>(s, o, st) : run_t<double>(s, o, st);
+}
+
+int32_t fvh_solver_run_begin(fvh_solver* s) {
+    if (!s) { set_error("null argument"); return FVH_E_ARG; }
+    return s->f32 ? apply_bounds_t<float>(s) : apply_bounds_t<double>(s);
+}
+
+int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_t n, fvh_status* st) {
+    if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
+    return s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st);
 }
 
 int32_t fvh_solver_advance(fvh_solver* s, int32_t n) {

@@ -1,0 +1,305 @@
+// probe.cpp -- map the user's fvhyper plugin functions onto device kinds.
+//
+// fvhyper plugins are host function pointers (examples/*/main.cpp), which a
+// GPU cannot call.  The library implements every plugin kind the reference's
+// Euler examples use (fvh_physics.h).  Here each user function is evaluated
+// on the host on a deterministic battery of inputs and compared bit for bit
+// with the candidate device kinds (the same fvh_physics.h code, compiled for
+// the host); the time step is compared cell by cell on the real mesh and
+// initial state through the library itself.  The first exact match is used;
+// no match throws.
+#include <fvhyper/explicit.h>
+#include <fvhyper/mesh.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <stdexcept>
+#include <string>
+
+#include "../../csrc/fvh_physics.h"
+#include "dropin_internal.h"
+
+namespace fvhyper {
+namespace dropin {
+namespace {
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    double u01() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * 0x1.0p-53;
+    }
+    double in(double a, double b) { return a + (b - a) * u01(); }
+};
+
+bool same(const double* a, const double* b, int n) { return std::memcmp(a, b, sizeof(double) * n) == 0; }
+
+// x and its decimal roundings to 1..17 significant digits
+std::vector<double> nice(double x) {
+    std::set<double> s;
+    s.insert(x);
+    char buf[64];
+    for (int d = 1; d <= 17; ++d) {
+        std::snprintf(buf, sizeof buf, "%.*g", d, x);
+        s.insert(std::strtod(buf, nullptr));
+    }
+    return std::vector<double>(s.begin(), s.end());
+}
+
+void conserved(double* q, double rho, double u, double v, double p) {
+    q[0] = rho;
+    q[1] = rho * u;
+    q[2] = rho * v;
+    q[3] = p / 0.4 + 0.5 * rho * (u * u + v * v);
+}
+
+// ---- flux: Roe spelling and gamma ------------------------------------------
+struct FluxKind { int joined; double gamma; };
+
+FluxKind probe_flux() {
+    Rng r(11);
+    const int NS = 256;
+    std::vector<double> qi(4 * NS), qj(4 * NS), nn(2 * NS), fu(4 * NS);
+    const double zero[4] = {0, 0, 0, 0};
+    for (int k = 0; k < NS; ++k) {
+        conserved(&qi[4 * k], r.in(0.2, 3), r.in(-3, 3), r.in(-3, 3), r.in(0.2, 3));
+        if (k % 8 == 0) std::memcpy(&qj[4 * k], &qi[4 * k], 32);
+        else conserved(&qj[4 * k], r.in(0.2, 3), r.in(-3, 3), r.in(-3, 3), r.in(0.2, 3));
+        const double t = r.in(0, 6.283185307179586);
+        nn[2 * k] = std::cos(t);
+        nn[2 * k + 1] = std::sin(t);
+        calc_flux(&fu[4 * k], &qi[4 * k], &qj[4 * k], zero, zero, &nn[2 * k]);
+    }
+    // gamma from the consistent flux: f1 = rho u^2 + p with qi = qj, n = (1, 0)
+    double q[4] = {1.0, 0.3, 0.1, 3.0}, f[4], n[2] = {1.0, 0.0};
+    calc_flux(f, q, q, zero, zero, n);
+    const double p = f[1] - q[1] * (q[1] / q[0]);
+    const double g_est = 1.0 + p / (q[3] - 0.5 / q[0] * (q[1] * q[1] + q[2] * q[2]));
+    std::vector<double> gs = nice(g_est);
+    for (double g : {1.4, 5.0 / 3.0, 1.3, 1.2, 1.1, 1.67}) gs.push_back(g);
+    for (double g : gs) {
+        for (int joined = 0; joined < 2; ++joined) {
+            bool ok = true;
+            for (int k = 0; k < NS && ok; ++k) {
+                double fd[4];
+                if (joined) fvh::roe_flux<double, true>(g, fd, &qi[4 * k], &qj[4 * k], nn[2 * k], nn[2 * k + 1]);
+                else fvh::roe_flux<double, false>(g, fd, &qi[4 * k], &qj[4 * k], nn[2 * k], nn[2 * k + 1]);
+                ok = same(fd, &fu[4 * k], 4);
+            }
+            if (ok) return {joined, g};
+        }
+    }
+    throw std::runtime_error("calc_flux: no device flux kind reproduces it (the library implements the Roe flux "
+                             "with the entropy cure in the shockTube/naca and forwardStep spellings)");
+}
+
+// ---- limiter function --------------------------------------------------------
+struct PhiKind { int kind; double yt; };
+
+PhiKind probe_phi() {
+    std::vector<double> ys;
+    for (int k = 0; k <= 300; ++k) ys.push_back(-3.0 + 15.0 * k / 300.0);
+    for (double y : {0.0, 1.0, 2.0, 1e-3, 1e3, 1e6, -1e3, 0.5, 1.999, 2.001}) ys.push_back(y);
+    std::vector<double> fu(ys.size());
+    for (size_t k = 0; k < ys.size(); ++k) fu[k] = limiter_func(ys[k]);
+    auto matches = [&](int kind, double yt) {
+        for (size_t k = 0; k < ys.size(); ++k) {
+            double v;
+            switch (kind) {
+                case FVH_PHI_ZERO: v = fvh::phi<double, fvh::PHI_ZERO>(ys[k], yt); break;
+                case FVH_PHI_MICHALAK: v = fvh::phi<double, fvh::PHI_MICHALAK>(ys[k], yt); break;
+                case FVH_PHI_VENKATAKRISHNAN: v = fvh::phi<double, fvh::PHI_VENKAT>(ys[k], yt); break;
+                default: v = fvh::phi<double, fvh::PHI_MINMOD>(ys[k], yt); break;
+            }
+            if (!same(&v, &fu[k], 1)) return false;
+        }
+        return true;
+    };
+    if (matches(FVH_PHI_ZERO, 2.0)) return {FVH_PHI_ZERO, 2.0};
+    if (matches(FVH_PHI_MINMOD, 2.0)) return {FVH_PHI_MINMOD, 2.0};
+    if (matches(FVH_PHI_VENKATAKRISHNAN, 2.0)) return {FVH_PHI_VENKATAKRISHNAN, 2.0};
+    // Michalak: phi == 1 from y_t on; locate the threshold by bisection
+    double lo = 0.0, hi = 1e3;
+    if (limiter_func(hi) == 1.0 && limiter_func(lo) != 1.0) {
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (limiter_func(mid) == 1.0) hi = mid; else lo = mid;
+        }
+        for (double yt : nice(hi))
+            if (matches(FVH_PHI_MICHALAK, yt)) return {FVH_PHI_MICHALAK, yt};
+    }
+    throw std::runtime_error("limiter_func: no device limiter kind reproduces it (zero, Michalak, "
+                             "Venkatakrishnan, minmod)");
+}
+
+// ---- boundary functions --------------------------------------------------------
+fvh_bc_spec probe_bc(const std::string& name, void (*fn)(double*, double*, double*), double gamma) {
+    Rng r(29);
+    const int NS = 192;
+    std::vector<double> qs(4 * NS), ns(2 * NS), bu(4 * NS);
+    for (int k = 0; k < NS; ++k) {
+        const double t = r.in(0, 6.283185307179586);
+        ns[2 * k] = std::cos(t);
+        ns[2 * k + 1] = std::sin(t);
+        const double rho = r.in(0.3, 2.0), p = r.in(0.3, 2.0);
+        const double c = std::sqrt(gamma * p / rho);
+        const double mach = (k % 2) ? r.in(0.1, 0.95) : r.in(1.1, 3.0);
+        const double a = r.in(0, 6.283185307179586);
+        double q[4];
+        q[0] = rho;
+        q[1] = rho * mach * c * std::cos(a);
+        q[2] = rho * mach * c * std::sin(a);
+        q[3] = p / (gamma - 1) + 0.5 * (q[1] * q[1] + q[2] * q[2]) / rho;
+        std::memcpy(&qs[4 * k], q, 32);
+        double b[4];
+        fn(b, q, &ns[2 * k]);
+        std::memcpy(&bu[4 * k], b, 32);
+    }
+    auto matches = [&](const fvh::BcParam<double>& P) {
+        for (int k = 0; k < NS; ++k) {
+            double b[4];
+            fvh::boundary(P, gamma, b, &qs[4 * k], ns[2 * k], ns[2 * k + 1]);
+            if (!same(b, &bu[4 * k], 4)) return false;
+        }
+        return true;
+    };
+    fvh_bc_spec spec;
+    std::memset(&spec, 0, sizeof spec);
+    std::snprintf(spec.name, sizeof spec.name, "%s", name.c_str());
+    fvh::BcParam<double> P{fvh::BC_COPY, 0, 0, 0, 0};
+    if (matches(P)) { spec.kind = FVH_BC_COPY; return spec; }
+    P.kind = fvh::BC_MIRROR;
+    if (matches(P)) { spec.kind = FVH_BC_MIRROR; return spec; }
+    // characteristic kinds: a supersonic state entering through n returns the freestream
+    double q[4], n[2] = {1.0, 0.0}, bv[4];
+    conserved(q, 1.0, -2.5 * std::sqrt(gamma), 0.0, 1.0);
+    q[3] = 1.0 / (gamma - 1) + 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0];
+    fn(bv, q, n);
+    const double rho = bv[0];
+    std::vector<double> us, vs, ps;
+    for (double u : nice(bv[1] / rho)) if (rho * u == bv[1]) us.push_back(u);
+    for (double v : nice(bv[2] / rho)) if (rho * v == bv[2]) vs.push_back(v);
+    const double p_est = (gamma - 1) * (bv[3] - 0.5 / rho * (bv[1] * bv[1] + bv[2] * bv[2]));
+    for (double p : nice(p_est))
+        if (p / (gamma - 1) + 0.5 / rho * (bv[1] * bv[1] + bv[2] * bv[2]) == bv[3]) ps.push_back(p);
+    for (int kind : {fvh::BC_INLET_OUTLET, fvh::BC_FARFIELD})
+        for (double u : us)
+            for (double v : vs)
+                for (double p : ps) {
+                    P = fvh::BcParam<double>{kind, rho, u, v, p};
+                    if (matches(P)) {
+                        spec.kind = kind;
+                        spec.rho = rho; spec.u = u; spec.v = v; spec.p = p;
+                        return spec;
+                    }
+                }
+    throw std::runtime_error("boundaries::bounds[\"" + name + "\"]: no device boundary kind reproduces it "
+                             "(copy, mirror wall, characteristic inlet/outlet, farfield)");
+}
+
+// ---- time step: compared on the mesh itself through the library --------------
+std::vector<double> device_dt(mesh& m, fvh_config c, const std::vector<double>& q) {
+    c.flags &= ~FVH_FLAG_GLOBAL_DT;
+    fvh_solver* s = nullptr;
+    if (fvh_solver_create(m.device_mesh(), &c, &s) != FVH_OK) throw std::runtime_error(fvh_last_error());
+    std::vector<double> dt(m.cellsAreas.size());
+    int32_t rc = fvh_solver_set_state(s, q.data());
+    if (rc == FVH_OK) rc = fvh_solver_phases(s, 0.0, dt.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
+    fvh_solver_free(s);
+    if (rc != FVH_OK) throw std::runtime_error(fvh_last_error());
+    return dt;
+}
+
+void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial) {
+    // the state the first calc_dt sees: the initial solution after the
+    // initial boundary update (explicit.cpp:712, 729)
+    std::vector<double> q(q_initial);
+    {
+        fvh_solver* s = nullptr;
+        if (fvh_solver_create(m.device_mesh(), &c, &s) != FVH_OK) throw std::runtime_error(fvh_last_error());
+        int32_t rc = fvh_solver_set_state(s, q.data());
+        if (rc == FVH_OK) rc = fvh_solver_run_begin(s);
+        if (rc == FVH_OK) rc = fvh_solver_get_state(s, q.data());
+        fvh_solver_free(s);
+        if (rc != FVH_OK) throw std::runtime_error(fvh_last_error());
+    }
+    const size_t N = m.cellsAreas.size();
+    std::vector<double> dtv(vars * N, 0.0);
+    calc_dt(dtv, q, m);
+    std::vector<double> du(N);
+    bool all_equal = true;
+    for (size_t i = 0; i < N; ++i) {
+        du[i] = dtv[vars * i];
+        for (int k = 1; k < vars; ++k)
+            if (!same(&dtv[vars * i + k], &du[i], 1))
+                throw std::runtime_error("calc_dt: the variables of a cell get different time steps");
+        if (!same(&du[i], &du[0], 1)) all_equal = false;
+    }
+    if (all_equal) {
+        c.dt_kind = FVH_DT_FIXED;
+        c.dt_fixed = du[0];
+        if (same(device_dt(m, c, q).data(), du.data(), (int)N)) return;
+    }
+    for (int kind : {FVH_DT_SUM_CFL, FVH_DT_EDGE_MIN}) {
+        c.dt_kind = kind;
+        c.cfl = 1.0;
+        const std::vector<double> d1 = device_dt(m, c, q);
+        // the time step is linear in the CFL number away from the edge-min cap of 1
+        size_t i = 0;
+        while (i + 1 < N && !(d1[i] < 0.5 && d1[i] > 0)) ++i;
+        for (double cfl : nice(du[i] / d1[i])) {
+            c.cfl = cfl;
+            if (same(device_dt(m, c, q).data(), du.data(), (int)N)) return;
+        }
+    }
+    throw std::runtime_error("calc_dt: no device time-step kind reproduces it on this mesh (fixed, summed CFL, "
+                             "edge-minimum local)");
+}
+
+}  // namespace
+
+fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
+    if (vars != 4)
+        throw std::runtime_error("vars = " + std::to_string(vars) +
+                                 ": the device path implements the 4-variable Euler plugins");
+    if (solver::diffusive_gradients) throw std::runtime_error("diffusive_gradients: viscous fluxes are not on the device path");
+    if (solver::smooth_residuals) throw std::runtime_error("smooth_residuals: residual smoothing is not on the device path");
+    fvh_config c;
+    fvh_config_default(&c);
+    c.flags = 0;
+    if (solver::do_calc_gradients) c.flags |= FVH_FLAG_GRADIENTS;
+    if (solver::do_calc_limiters) c.flags |= FVH_FLAG_LIMITERS;
+    if (solver::linear_interpolate) c.flags |= FVH_FLAG_LINEAR;
+    if (solver::global_dt) c.flags |= FVH_FLAG_GLOBAL_DT;
+    c.K = solver::limiter_k_value;
+    const FluxKind fk = probe_flux();
+    c.flux = fk.joined ? FVH_FLUX_ROE_JOINED : FVH_FLUX_ROE_SPLIT;
+    c.gamma = fk.gamma;
+    if (solver::do_calc_limiters) {
+        const PhiKind pk = probe_phi();
+        c.phi = pk.kind;
+        c.phi_yt = pk.yt;
+    }
+    // one BC slot per physical name the mesh uses
+    std::vector<std::string> seen;
+    c.n_bc = 0;
+    for (size_t b = 0; b < m.boundaryNames.size(); ++b) {
+        const std::string& nm = m.boundaryNames[b];
+        if (std::find(seen.begin(), seen.end(), nm) != seen.end()) continue;
+        if (c.n_bc == FVH_MAX_BC) throw std::runtime_error("more than 16 boundary names");
+        seen.push_back(nm);
+        c.bc[c.n_bc++] = probe_bc(nm, m.boundaryFuncs[b], c.gamma);
+    }
+    probe_dt(m, c, q_initial);
+    return c;
+}
+
+}  // namespace dropin
+}  // namespace fvhyper

@@ -48,7 +48,7 @@ class RunOptions(C.Structure):
 
 class Status(C.Structure):
     _fields_ = [("step", C.c_uint32), ("stopped", C.c_uint32), ("time", C.c_double),
-                ("R", C.c_double * 4), ("R0", C.c_double * 4)]
+                ("time_prev", C.c_double), ("R", C.c_double * 4), ("R0", C.c_double * 4)]
 
 
 # every symbol include/fvh.h declares
@@ -57,7 +57,7 @@ EXPORTS = [
     "fvh_mesh_free", "fvh_mesh_counts", "fvh_mesh_export", "fvh_mesh_name",
     "fvh_config_default", "fvh_solver_create", "fvh_solver_free", "fvh_solver_set_state",
     "fvh_solver_get_state", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
-    "fvh_solver_advance", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
+    "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
 ]
 
@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_phases.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, dp]
     L.fvh_solver_run.argtypes = [vp, C.POINTER(RunOptions), C.POINTER(Status)]
     L.fvh_solver_advance.argtypes = [vp, C.c_int32]
+    L.fvh_solver_run_begin.argtypes = [vp]
+    L.fvh_solver_run_continue.argtypes = [vp, C.POINTER(RunOptions), C.c_uint32, C.POINTER(Status)]
     L.fvh_solver_synchronize.argtypes = [vp]
     L.fvh_solver_status.argtypes = [vp, C.POINTER(Status)]
     L.fvh_solver_profile.argtypes = [vp, C.c_int32]

@@ -182,6 +182,16 @@ class Solver:
         check(_lib.load().fvh_solver_run(self._h, C.byref(o), C.byref(st)))
         return self._status_dict(st)
 
+    def run_begin(self) -> None:
+        check(_lib.load().fvh_solver_run_begin(self._h))
+
+    def run_continue(self, n: int, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
+                     print_interval: int = 1) -> Dict:
+        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol))
+        st = _lib.Status()
+        check(_lib.load().fvh_solver_run_continue(self._h, C.byref(o), int(n), C.byref(st)))
+        return self._status_dict(st)
+
     def advance(self, n: int) -> None:
         check(_lib.load().fvh_solver_advance(self._h, int(n)))
 
@@ -196,6 +206,7 @@ class Solver:
     @staticmethod
     def _status_dict(st) -> Dict:
         return {"step": int(st.step), "stopped": bool(st.stopped), "time": float(st.time),
+                "time_prev": float(st.time_prev),
                 "R": np.array(list(st.R)), "R0": np.array(list(st.R0))}
 
     def profile(self, enable: bool) -> None:

@@ -111,6 +111,7 @@ typedef struct fvh_status {
     uint32_t step;             /* steps taken */
     uint32_t stopped;          /* 1 once the stop test of explicit.cpp:718-726 held */
     double time;               /* sum of dt[cell 0] over steps (explicit.cpp:800) */
+    double time_prev;          /* time before the last step (what the reference prints for it) */
     double R[4];               /* residual, normalised by R0 after step 0 */
     double R0[4];              /* step-0 residual (explicit.cpp:752-754) */
 } fvh_status;
@@ -179,6 +180,13 @@ int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, d
 /* The run() loop of explicit.cpp:657-817 from the current state: applies the
  * initial boundary update, then steps until the stop test holds. */
 int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st);
+
+/* run() in pieces, for callers that print or save between steps:
+ * run_begin applies the initial boundary update and resets step/time;
+ * run_continue executes up to n more steps of the same loop (stop test,
+ * residuals on print steps) and returns the status after them. */
+int32_t fvh_solver_run_begin(fvh_solver* s);
+int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_t n, fvh_status* st);
 
 /* Enqueue n full steps (dt, all RK stages, swap) without residuals or stop
  * test; returns without waiting.  Used for fixed-length timing. */

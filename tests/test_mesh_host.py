@@ -1,0 +1,255 @@
+"""Host side of libfvh.so (no GPU needed): the mesh reader and builder must
+reproduce the reference's numbering and geometry bit for bit, and the device
+renumberings must be valid, deterministic permutations."""
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fvhyper_amd import FvhError, Mesh, meshgen
+from fvhyper_amd import _lib
+from fvhyper_amd.solver import make_config
+from helpers import bits_equal
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+FIELDS = [("edgesCells", "edge_cells"), ("edgesNodes", "edge_nodes"), ("edgesLengths", "edge_len"),
+          ("edgesNormalsX", "edge_nx"), ("edgesNormalsY", "edge_ny"), ("edgesCentersX", "edge_cx"),
+          ("edgesCentersY", "edge_cy"), ("cellsAreas", "cell_area"), ("cellsCentersX", "cell_cx"),
+          ("cellsCentersY", "cell_cy"), ("boundaryEdges", "bnd_edge"), ("cellsNodes", "cell_nodes")]
+
+
+def _same(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype.kind == "f" or b.dtype.kind == "f":
+        return bits_equal(a, b)
+    return np.array_equal(a.astype(np.int64).ravel(), b.astype(np.int64).ravel())
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "mesh__*.npz"))))
+def test_product_mesh_matches_reference_fixture(path):
+    d = np.load(path)
+    m = meshgen.MeshArrays(d["x"], d["y"], d["cells"], d["bnd"], d["bnd_tag"], [str(s) for s in d["names"]])
+    pm = Mesh.from_arrays(m)
+    assert pm.n_real == int(d["ref_nReal"][0])
+    for ref_name, _ in FIELDS:
+        assert _same(pm.export(ref_name), d["ref_" + ref_name]), ref_name
+
+
+@pytest.mark.parametrize("factory", [lambda: meshgen.forward_step(n_per_unit=40),
+                                     lambda: meshgen.mixed_channel(nx=30, ny=12, tri_fraction=0.3),
+                                     lambda: meshgen.sod_tube()])
+def test_product_mesh_matches_oracle(factory, tmp_path):
+    m = factory()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    for ref_name, attr in FIELDS:
+        assert _same(pm.export(ref_name), getattr(om, attr)), ref_name
+    # the MSH 4.1 round trip gives the same mesh
+    meshgen.write_msh(str(tmp_path / "m.msh"), m)
+    pr = Mesh.read_msh(str(tmp_path / "m.msh"))
+    assert pr.names == pm.names
+    for ref_name, _ in FIELDS + [("nodesX", None), ("nodesY", None), ("boundaryTags", None)]:
+        assert _same(pr.export(ref_name), pm.export(ref_name)), ref_name
+
+
+GMSH_STYLE = """$MeshFormat\r
+4.1 0 8\r
+$EndMeshFormat\r
+$PhysicalNames\r
+3\r
+1 7 "left"\r
+1 8 "rest"\r
+2 9 "fluid"\r
+$EndPhysicalNames\r
+$Entities\r
+4 4 1 0\r
+1 0 0 0 0 \r
+2 1 0 0 0 \r
+3 1 1 0 0 \r
+4 0 1 0 0 \r
+1 0 0 0 1 0 0 1 8 2 1 -2 \r
+2 1 0 0 1 1 0 1 8 2 2 -3 \r
+3 0 1 0 1 1 0 1 8 2 3 -4 \r
+4 0 0 0 0 1 0 1 7 2 4 -1 \r
+1 0 0 0 1 1 0 1 9 4 1 2 3 4 \r
+$EndEntities\r
+$Nodes\r
+3 5 1 5\r
+0 1 0 1\r
+1\r
+0 0 0\r
+0 3 0 1\r
+3\r
+1 1 0\r
+2 1 0 3\r
+2\r
+5\r
+4\r
+1 0 0\r
+0.5 0.5 0\r
+0 1 0\r
+$EndNodes\r
+$Elements\r
+3 8 1 8\r
+1 4 1 1\r
+1 4 1\r
+1 1 1 3\r
+2 1 2\r
+3 2 3\r
+4 3 4\r
+2 1 2 4\r
+5 1 2 5\r
+6 2 3 5\r
+7 3 4 5\r
+8 4 1 5\r
+$EndElements\r
+"""
+
+
+def test_reader_accepts_gmsh_layout(tmp_path):
+    """Several node blocks (tags out of order), CRLF, trailing spaces, point
+    entities, line blocks split per curve entity."""
+    p = tmp_path / "g.msh"
+    p.write_bytes(GMSH_STYLE.encode())
+    pm = Mesh.read_msh(str(p))
+    assert pm.n_real == 4 and pm.n_bnd == 4 and pm.n_cells == 8
+    assert pm.names == ["left", "rest"]
+    # node index = file order: tags 1, 3, 2, 5, 4
+    np.testing.assert_array_equal(pm.export("nodesX"), [0, 1, 1, 0.5, 0])
+    np.testing.assert_array_equal(pm.export("nodesY"), [0, 1, 0, 0.5, 1])
+    # same mesh from arrays (0-based file-order node indices)
+    idx = {1: 0, 3: 1, 2: 2, 5: 3, 4: 4}
+    cells = np.array([[idx[1], idx[2], idx[5], -1], [idx[2], idx[3], idx[5], -1],
+                      [idx[3], idx[4], idx[5], -1], [idx[4], idx[1], idx[5], -1]], dtype=np.int32)
+    bnd = np.array([[idx[4], idx[1]], [idx[1], idx[2]], [idx[2], idx[3]], [idx[3], idx[4]]], dtype=np.int32)
+    m = meshgen.MeshArrays(pm.export("nodesX"), pm.export("nodesY"), cells, bnd,
+                           np.array([0, 1, 1, 1], dtype=np.int32), ["left", "rest"])
+    om = O.Mesh(m)
+    for ref_name, attr in FIELDS:
+        assert _same(pm.export(ref_name), getattr(om, attr)), ref_name
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(FvhError, match="cannot open"):
+        Mesh.read_msh(str(tmp_path / "missing.msh"))
+    bad = GMSH_STYLE.replace("1 4 1 1\r\n1 4 1\r\n", "1 4 1 1\r\n1 4 2\r\n")   # edge (4,2) is a diagonal? no: not an edge
+    (tmp_path / "b.msh").write_bytes(bad.encode())
+    with pytest.raises(FvhError, match="invalid edge ref"):
+        Mesh.read_msh(str(tmp_path / "b.msh"))
+    (tmp_path / "v.msh").write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
+    with pytest.raises(FvhError, match="MSH 4"):
+        Mesh.read_msh(str(tmp_path / "v.msh"))
+
+
+def test_missing_boundary_condition_is_reported_like_the_reference():
+    m = meshgen.sod_tube(nx=8, ny=2)
+    pm = Mesh.from_arrays(m)
+    L = _lib.load()
+    c = make_config({"bcs": {"wall": ("mirror",)}})
+    h = C.c_void_p()
+    rc = L.fvh_solver_create(pm._h, C.byref(c), C.byref(h))
+    assert rc == -5   # FVH_E_BC, raised before any device work
+    msg = L.fvh_last_error().decode()
+    assert msg.startswith("Physical name transmissive of tag") and msg.endswith("not in mesh")
+
+
+def _rcm_restated(om):
+    """Reverse Cuthill-McKee as documented in fvh_mesh.cpp, restated."""
+    N = om.n_real
+    adj = [[] for _ in range(N)]
+    for a, b in om.edge_cells:
+        if a != b and a < N and b < N:
+            adj[a].append(b)
+            adj[b].append(a)
+    deg = [len(x) for x in adj]
+    key = lambda v: (deg[v], v)
+    for v in range(N):
+        adj[v].sort(key=key)
+    done = [False] * N
+    order = []
+
+    def bfs(r):
+        lev = {r: 0}
+        q = [r]
+        for v in q:
+            for w in adj[v]:
+                if not done[w] and w not in lev:
+                    lev[w] = lev[v] + 1
+                    q.append(w)
+        return lev, q
+
+    while len(order) < N:
+        r = min((v for v in range(N) if not done[v]), key=key)
+        lev, q = bfs(r)
+        ecc = lev[q[-1]]
+        while True:
+            best = min((v for v in q if lev[v] == ecc), key=key)
+            lev2, q2 = bfs(best)
+            if lev2[q2[-1]] <= ecc:
+                break
+            r, lev, q, ecc = best, lev2, q2, lev2[q2[-1]]
+        start = len(order)
+        order.append(r)
+        done[r] = True
+        h = start
+        while h < len(order):
+            for w in adj[order[h]]:
+                if not done[w]:
+                    done[w] = True
+                    order.append(w)
+            h += 1
+    return order[::-1]
+
+
+@pytest.mark.parametrize("factory", [lambda: meshgen.mixed_channel(nx=10, ny=5),
+                                     lambda: meshgen.forward_step(n_per_unit=10),
+                                     lambda: meshgen.sod_tube(nx=30, ny=3)])
+def test_rcm_permutation_is_exact_and_valid(factory):
+    m = factory()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    pos = pm.export("rcmPermutation")
+    N = pm.n_real
+    assert sorted(pos[:N].tolist()) == list(range(N))
+    assert np.array_equal(pos[N:], np.arange(N, pm.n_cells))
+    order = _rcm_restated(om)
+    mine = np.empty(N, dtype=np.int64)
+    mine[pos[:N]] = np.arange(N)
+    assert mine.tolist() == order
+
+
+def test_rcm_reduces_bandwidth_of_a_shuffled_mesh():
+    m = meshgen.forward_step(n_per_unit=20)
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(m.n_cells)
+    shuffled = meshgen.MeshArrays(m.x, m.y, m.cells[perm], m.bnd, m.bnd_tag, m.names)
+    pm = Mesh.from_arrays(shuffled)
+    ec = pm.export("edgesCells").reshape(-1, 2)
+    real = (ec[:, 0] < pm.n_real) & (ec[:, 1] < pm.n_real) & (ec[:, 0] != ec[:, 1])
+    bw_before = np.max(np.abs(ec[real, 0] - ec[real, 1]))
+    pos = pm.export("rcmPermutation")
+    bw_after = np.max(np.abs(pos[ec[real, 0]] - pos[ec[real, 1]]))
+    assert bw_after * 10 < bw_before
+
+
+def test_tile_order_is_a_compact_deterministic_partition():
+    m = meshgen.forward_step(n_per_unit=40)
+    pm = Mesh.from_arrays(m)
+    pos = pm.export("tilePermutation")
+    N = pm.n_real
+    assert sorted(pos[:N].tolist()) == list(range(N))
+    assert np.array_equal(pos, Mesh.from_arrays(m).export("tilePermutation"))
+    order = np.empty(N, dtype=np.int64)
+    order[pos[:N]] = np.arange(N)
+    cx, cy = pm.export("cellsCentersX")[:N], pm.export("cellsCentersY")[:N]
+    # every full tile of 256 cells is spatially compact: its bounding box is
+    # far smaller than the domain
+    for t in range(N // 256):
+        ids = order[256 * t: 256 * (t + 1)]
+        assert (np.ptp(cx[ids]) + np.ptp(cy[ids])) < 0.6
+        assert np.all(np.diff(ids) > 0)   # ascending reference index inside a tile

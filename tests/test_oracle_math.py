@@ -1,0 +1,307 @@
+"""Pins for the oracle that do not come from the reference's outputs:
+closed forms, invariants and an independent eigen-decomposition of the Roe
+flux, each chosen so that a plausible slip (a dropped term, a sign, a swapped
+operand) fails."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fvhyper_amd import meshgen, workloads
+from helpers import random_state
+
+G = 1.4
+
+
+def prim(rho, u, v, p):
+    return np.array([rho, rho * u, rho * v, p / (G - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def phys_flux(q, n):
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    p = (G - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+    V = u * n[0] + v * n[1]
+    return np.array([rho * V, q[1] * V + p * n[0], q[2] * V + p * n[1], (q[3] + p) * V])
+
+
+def jacobian(u, v, H, n):
+    """Euler flux Jacobian dF.n/dQ at velocity (u, v) and total enthalpy H."""
+    g1 = G - 1
+    k = 0.5 * (u * u + v * v)
+    V = u * n[0] + v * n[1]
+    return np.array([
+        [0, n[0], n[1], 0],
+        [g1 * k * n[0] - u * V, V - (G - 2) * u * n[0], u * n[1] - g1 * v * n[0], g1 * n[0]],
+        [g1 * k * n[1] - v * V, v * n[0] - g1 * u * n[1], V - (G - 2) * v * n[1], g1 * n[1]],
+        [(g1 * k - H) * V, H * n[0] - g1 * u * V, H * n[1] - g1 * v * V, G * V],
+    ])
+
+
+def roe_textbook(qL, qR, n):
+    """0.5 (F_L + F_R) - 0.5 |A(Roe average)| (Q_R - Q_L), |A| by eig."""
+    sL, sR = np.sqrt(qL[0]), np.sqrt(qR[0])
+    pL = (G - 1) * (qL[3] - 0.5 * (qL[1] ** 2 + qL[2] ** 2) / qL[0])
+    pR = (G - 1) * (qR[3] - 0.5 * (qR[1] ** 2 + qR[2] ** 2) / qR[0])
+    u = (qL[1] / sL + qR[1] / sR) / (sL + sR)
+    v = (qL[2] / sL + qR[2] / sR) / (sL + sR)
+    H = ((qL[3] + pL) / sL + (qR[3] + pR) / sR) / (sL + sR)
+    A = jacobian(u, v, H, n)
+    w, R = np.linalg.eig(A)
+    absA = (R @ np.diag(np.abs(w)) @ np.linalg.inv(R)).real
+    return 0.5 * (phys_flux(qL, n) + phys_flux(qR, n)) - 0.5 * absA @ (qR - qL)
+
+
+CFG_SPLIT = dict(flux="roe_split", gamma=G)
+CFG_JOINED = dict(flux="roe_joined", gamma=G)
+
+
+def test_jacobian_matches_finite_difference():
+    q = prim(1.2, 0.3, -0.2, 0.9)
+    n = np.array([0.6, 0.8])
+    rho, u, v = q[0], q[1] / q[0], q[2] / q[0]
+    p = (G - 1) * (q[3] - 0.5 * rho * (u * u + v * v))
+    A = jacobian(u, v, (q[3] + p) / rho, n)
+    eps = 1e-6
+    J = np.zeros((4, 4))
+    for k in range(4):
+        d = np.zeros(4)
+        d[k] = eps
+        J[:, k] = (phys_flux(q + d, n) - phys_flux(q - d, n)) / (2 * eps)
+    np.testing.assert_allclose(A, J, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("cfg", [CFG_SPLIT, CFG_JOINED])
+def test_flux_consistency(cfg):
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        q = prim(rng.uniform(0.2, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.2, 2))
+        t = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(t), np.sin(t)])
+        np.testing.assert_allclose(O.flux(cfg, q, q, n), phys_flux(q, n), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("cfg", [CFG_SPLIT, CFG_JOINED])
+def test_flux_equals_textbook_roe_where_the_fix_is_inactive(cfg):
+    """The entropy cure changes only |V-c| and |V+c| when VL-c < V-c or
+    VR+c > V+c; in compressive states (VL >= V >= VR) it is plain Roe."""
+    rng = np.random.default_rng(2)
+    checked = 0
+    while checked < 60:
+        qL = prim(rng.uniform(0.3, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2))
+        qR = prim(rng.uniform(0.3, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2))
+        t = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(t), np.sin(t)])
+        sL, sR = np.sqrt(qL[0]), np.sqrt(qR[0])
+        u = (qL[1] / sL + qR[1] / sR) / (sL + sR)
+        v = (qL[2] / sL + qR[2] / sR) / (sL + sR)
+        V = u * n[0] + v * n[1]
+        VL = (qL[1] * n[0] + qL[2] * n[1]) / qL[0]
+        VR = (qR[1] * n[0] + qR[2] * n[1]) / qR[0]
+        if not (VL >= V >= VR):
+            continue
+        np.testing.assert_allclose(O.flux(cfg, qL, qR, n), roe_textbook(qL, qR, n), rtol=1e-10, atol=1e-11)
+        checked += 1
+
+
+@pytest.mark.parametrize("cfg", [CFG_SPLIT, CFG_JOINED])
+def test_flux_conservation_and_rotation(cfg):
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        qL = prim(rng.uniform(0.3, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.3, 2))
+        qR = prim(rng.uniform(0.3, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.3, 2))
+        t = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(t), np.sin(t)])
+        f = O.flux(cfg, qL, qR, n)
+        np.testing.assert_allclose(O.flux(cfg, qR, qL, -n), -f, rtol=1e-12, atol=1e-12)
+        # rotating states and normal together rotates the momentum flux
+        a = rng.uniform(0, 2 * np.pi)
+        Rm = np.array([[np.cos(a), -np.sin(a)], [np.sin(a), np.cos(a)]])
+        rot = lambda q: np.array([q[0], *(Rm @ q[1:3]), q[3]])
+        fr = O.flux(cfg, rot(qL), rot(qR), Rm @ n)
+        np.testing.assert_allclose(fr, rot(f), rtol=1e-11, atol=1e-11)
+
+
+def test_split_and_joined_differ_only_in_rounding():
+    rng = np.random.default_rng(4)
+    diff = 0
+    for _ in range(200):
+        qL = prim(rng.uniform(0.3, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.3, 2))
+        qR = prim(rng.uniform(0.3, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.3, 2))
+        n = np.array([0.6, -0.8])
+        a, b = O.flux(CFG_SPLIT, qL, qR, n), O.flux(CFG_JOINED, qL, qR, n)
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-13)
+        diff += int(not np.array_equal(a, b))
+    assert diff > 0   # the association really differs, so both spellings are needed
+
+
+def test_limiter_functions():
+    mich = dict(phi="michalak", phi_yt=2.0)
+    ys = np.linspace(-1, 3, 41)
+    for y in ys:
+        want = 1.0 if y >= 2 else y - y * y / 4
+        assert abs(O.phi(mich, y) - want) < 1e-15
+    h = 1e-6
+    assert abs((O.phi(mich, 2 - h) - O.phi(mich, 2 - 2 * h)) / h) < 1e-5     # phi'(y_t) = 0
+    ven = dict(phi="venkat")
+    assert O.phi(ven, 0.0) == 0.0
+    assert O.phi(ven, 1.0) == 0.75
+    assert abs(O.phi(ven, 1e8) - 1.0) < 1e-7
+    for y in ys:
+        assert abs(O.phi(ven, y) - (y * y + 2 * y) / (y * y + y + 2)) < 1e-15
+    mm = dict(phi="minmod")
+    for y in ys:
+        assert O.phi(mm, y) == max(0.0, min(1.0, y))
+    assert O.phi(dict(phi="zero"), 0.7) == 0.0
+
+
+def test_boundary_kinds():
+    rng = np.random.default_rng(5)
+    for _ in range(30):
+        q = prim(rng.uniform(0.3, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.3, 2))
+        t = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(t), np.sin(t)])
+        b = O.bc(("mirror",), G, q, n)
+        assert b[0] == q[0] and b[3] == q[3]
+        assert abs(np.hypot(b[1], b[2]) - np.hypot(q[1], q[2])) < 1e-12
+        assert abs((b[1] * n[0] + b[2] * n[1]) + (q[1] * n[0] + q[2] * n[1])) < 1e-12
+        assert abs((-b[1] * n[1] + b[2] * n[0]) - (-q[1] * n[1] + q[2] * n[0])) < 1e-12
+        assert np.array_equal(O.bc(("copy",), G, q, n), q)
+    free = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+    n = np.array([-1.0, 0.0])
+    qin = prim(1.0, 2.5, 0.1, 0.5)           # supersonic, entering (u.n < 0)
+    np.testing.assert_allclose(O.bc(free, G, qin, n), prim(1.4, 3.0, 0.0, 1.0), rtol=1e-15)
+    np.testing.assert_array_equal(O.bc(free, G, qin, -n), qin)     # leaving: copy
+    far = ("farfield", 1.4, 0.8, 0.0175, 1.0)
+    np.testing.assert_allclose(O.bc(far, G, qin, n), prim(1.4, 0.8, 0.0175, 1.0), rtol=1e-15)
+    # subsonic outflow keeps the interior density jump consistent with p_b = p_inf
+    qs = prim(1.2, 0.3, 0.0, 1.3)
+    b = O.bc(free, G, qs, np.array([1.0, 0.0]))
+    pb = (G - 1) * (b[3] - 0.5 * (b[1] ** 2 + b[2] ** 2) / b[0])
+    assert abs(pb - 1.0) < 1e-12
+
+
+def _quad_mesh():
+    return meshgen.mixed_channel(nx=8, ny=6, jitter=0.0, tri_fraction=0.0)
+
+
+def test_green_gauss_exact_for_linear_field_on_uniform_quads():
+    m = _quad_mesh()
+    om = O.Mesh(m)
+    a, bx, by = 0.7, 1.3, -2.1
+    q = np.zeros((om.n_cells, 4))
+    for k in range(4):
+        q[:, k] = a + (k + 1) * (bx * om.cell_cx + by * om.cell_cy)
+    cfg = dict(grad=True, bcs={"wall": ("copy",), "transmissive": ("copy",)})
+    out = om.phases(cfg, q.ravel())
+    gx = out["gx"].reshape(-1, 4)
+    gy = out["gy"].reshape(-1, 4)
+    interior = np.ones(om.n_real, dtype=bool)
+    for e in range(om.n_edges):
+        if om.edge_cells[e, 1] >= om.n_real:
+            interior[om.edge_cells[e, 0]] = False
+    assert interior.sum() > 10
+    for k in range(4):
+        np.testing.assert_allclose(gx[:om.n_real][interior, k], (k + 1) * bx, rtol=1e-12)
+        np.testing.assert_allclose(gy[:om.n_real][interior, k], (k + 1) * by, rtol=1e-12)
+
+
+def test_constant_state_is_preserved():
+    m = meshgen.sod_tube(nx=20, ny=4)
+    om = O.Mesh(m)
+    q0 = workloads.initial_state(om, workloads.uniform_ic(1.1, 0.4, -0.2, 0.9))
+    cfg = dict(flux="roe_joined", dt="sum_cfl", cfl=0.5, gdt=True, phi="venkat", grad=True, lim=True,
+               lin=True, bcs={"wall": ("copy",), "transmissive": ("copy",)})
+    out = om.phases(cfg, q0)
+    assert np.all(out["lim"][:4 * om.n_real] == 1.0)
+    assert np.max(np.abs(out["gx"])) < 1e-12
+    q, steps, t, R, R0, _ = om.run(cfg, q0, 20)
+    np.testing.assert_allclose(q, q0, rtol=1e-13, atol=1e-13)
+
+
+def test_closed_box_conserves_mass_and_energy():
+    """First order: each wall ghost is the exact mirror of the current
+    interior state, so the wall mass and energy fluxes vanish.  (In second
+    order the ghost mirrors the previous stage's reconstruction -- the stale
+    gradient of explicit.cpp:397-404 -- and mass leaks at ~1e-8.)"""
+    m = meshgen.sod_tube(nx=40, ny=4)
+    om = O.Mesh(m)
+    cfg = dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True,
+               bcs={"wall": ("mirror",), "transmissive": ("mirror",)})
+    q0 = workloads.initial_state(om, workloads.sod_ic())
+    q, *_ = om.run(cfg, q0, 40)
+    A = om.cell_area[:om.n_real]
+    for k in (0, 3):
+        before = np.sum(A * q0.reshape(-1, 4)[:om.n_real, k])
+        after = np.sum(A * q.reshape(-1, 4)[:om.n_real, k])
+        assert abs(after - before) < 1e-12 * abs(before)
+
+
+def test_residual_and_dt_closed_forms():
+    m = meshgen.forward_step(n_per_unit=10)
+    om = O.Mesh(m)
+    cfg = dict(flux="roe_joined", dt="sum_cfl", cfl=0.8, gdt=False, phi="venkat", grad=True, lim=True, lin=True,
+               bcs={"wall": ("mirror",), "inlet": ("inlet_outlet", 1.4, 3.0, 0.0, 1.0),
+                    "outlet": ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)})
+    q = random_state(om.n_cells, seed=9)
+    out = om.phases(cfg, q)
+    qt = out["qt"].reshape(-1, 4)[:om.n_real]
+    R = np.sqrt(np.sum(qt * qt * om.cell_area[:om.n_real, None], axis=0))
+    np.testing.assert_allclose(out["R"], R, rtol=1e-13)
+    Q = q.reshape(-1, 4)
+    p = (G - 1) * (Q[:, 3] - 0.5 * (Q[:, 1] ** 2 + Q[:, 2] ** 2) / Q[:, 0])
+    c = np.sqrt(G * p / Q[:, 0])
+    s = np.zeros(om.n_cells)
+    for e in range(om.n_edges):
+        i, j = om.edge_cells[e]
+        n = (om.edge_nx[e], om.edge_ny[e])
+        ei = c[i] + abs((Q[i, 1] * n[0] + Q[i, 2] * n[1]) / Q[i, 0])
+        ej = c[j] + abs((Q[j, 1] * n[0] + Q[j, 2] * n[1]) / Q[j, 0])
+        s[i] += max(ei, ej) * om.edge_len[e]
+        s[j] += max(ei, ej) * om.edge_len[e]
+    np.testing.assert_allclose(out["dt"], 0.8 * om.cell_area / s, rtol=1e-13)
+
+
+def test_mesh_geometry_invariants():
+    m = meshgen.mixed_channel(nx=10, ny=5, jitter=0.2)
+    om = O.Mesh(m)
+    assert abs(np.sum(om.cell_area[:om.n_real]) - 0.5) < 1e-13        # [0,1]x[0,0.5]
+    np.testing.assert_allclose(np.hypot(om.edge_nx, om.edge_ny), 1.0, rtol=1e-15)
+    c0 = om.edge_cells[:, 0]
+    dot = om.edge_nx * (om.edge_cx - om.cell_cx[c0]) + om.edge_ny * (om.edge_cy - om.cell_cy[c0])
+    assert np.all(dot >= 0)
+    # closed cells: sum of outward l*n is zero
+    sx = np.zeros(om.n_real)
+    sy = np.zeros(om.n_real)
+    for e in range(om.n_edges):
+        i, j = om.edge_cells[e]
+        for cell, sgn in ((i, 1.0), (j, -1.0)):
+            if cell < om.n_real:
+                sx[cell] += sgn * om.edge_len[e] * om.edge_nx[e]
+                sy[cell] += sgn * om.edge_len[e] * om.edge_ny[e]
+    assert np.max(np.abs(sx)) < 1e-15 and np.max(np.abs(sy)) < 1e-15
+    # ghost centres mirror the owner centre through the boundary edge centre
+    for b, e in enumerate(om.bnd_edge):
+        own, gh = om.edge_cells[e]
+        d = np.hypot(om.edge_cx[e] - om.cell_cx[own], om.edge_cy[e] - om.cell_cy[own])
+        assert abs(np.hypot(om.cell_cx[gh] - om.edge_cx[e], om.cell_cy[gh] - om.edge_cy[e]) - d) < 1e-15
+        assert om.cell_area[gh] == om.cell_area[own]
+
+
+def test_rk_stage_coefficients_compose_as_low_storage_scheme():
+    """With zero flux divergence except a uniform source-free state the RK
+    loop is the identity; with the alpha list (1) it is forward Euler: one
+    Euler step equals one phases() update with alpha = 1."""
+    m = meshgen.sod_tube(nx=20, ny=4)
+    om = O.Mesh(m)
+    cfg = dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, alpha=[1.0],
+               bcs={"wall": ("mirror",), "transmissive": ("copy",)})
+    q0 = workloads.initial_state(om, workloads.sod_ic())
+    ph = O.Mesh(m)
+    # run() applies the initial boundary update first; do the same here
+    qb = q0.copy()
+    import ctypes as C
+    c = O.make_config(cfg, om.names)
+    O.lib().orc_update_bounds(ph._p, C.byref(c), qb, np.zeros_like(qb), np.zeros_like(qb), np.zeros_like(qb))
+    one = ph.phases(cfg, qb, alpha=1.0)["qk"]
+    q, steps, t, *_ = om.run(cfg, q0, 1)
+    assert steps == 1
+    np.testing.assert_array_equal(q, one)
