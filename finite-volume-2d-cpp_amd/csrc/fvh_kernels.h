@@ -110,7 +110,7 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, con
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
                         const BcParam<T>* bcs, Q4<T>* q);
-void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol);
+void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var);
 void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,
                      const void* dt_src, int dt_is_f32, int32_t dt_index);
 

@@ -493,13 +493,15 @@ __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParam
 // ---------------------------------------------------------------------------
 // run() bookkeeping on the device (explicit.cpp:715-801)
 // ---------------------------------------------------------------------------
-__global__ void k_ctrl_begin(Ctrl* ctrl, uint32_t max_step, double max_time, double tol) {
+__global__ void k_ctrl_begin(Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var) {
     if (ctrl->stop) return;
     double Rmax = 0.0;
-    if (ctrl->step > 0) {
-        for (int k = 0; k < 4; ++k) Rmax = smax(ctrl->R[k], Rmax);
-    } else {
+    if (ctrl->step == 0) {
         Rmax = 1.0;
+    } else if (stop_var >= 0 && stop_var < 4) {
+        Rmax = ctrl->R[stop_var];
+    } else {
+        for (int k = 0; k < 4; ++k) Rmax = smax(ctrl->R[k], Rmax);
     }
     if ((ctrl->step >= max_step) | (ctrl->time >= max_time) | (Rmax < tol)) ctrl->stop = 1;
 }
@@ -626,8 +628,8 @@ void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& p
     hipLaunchKernelGGL((k_bounds_init<T>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q);
 }
 
-void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol) {
-    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(1), 0, s, ctrl, max_step, max_time, tol);
+void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var) {
+    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(1), 0, s, ctrl, max_step, max_time, tol, stop_var);
 }
 
 void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,

@@ -259,7 +259,7 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     hipStream_t st = s->stream;
     if (run_mode) {
         Bracket b(s, K_BEGIN);
-        launch_ctrl_begin(st, ctrl, o->max_step, o->max_time, o->tolerance);
+        launch_ctrl_begin(st, ctrl, o->max_step, o->max_time, o->tolerance, o->stop_var);
     }
     Q4<T>* q0 = (Q4<T>*)s->q[s->cur].p;
     Q4<T>* qa = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
@@ -661,8 +661,14 @@ int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, d
                   : phases_t<double>(s, alpha, dt, gx, gy, lim, qt, qk);
 }
 
+static bool bad_options(const fvh_run_options* o) {
+    if (o->stop_var < -1 || o->stop_var > 3) { set_error("stop_var must be -1 or a variable index 0..3"); return true; }
+    return false;
+}
+
 int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
+    if (bad_options(o)) return FVH_E_ARG;
     return s->f32 ? run_t<float>(s, o, st) : run_t<double>(s, o, st);
 }
 
@@ -673,12 +679,13 @@ int32_t fvh_solver_run_begin(fvh_solver* s) {
 
 int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_t n, fvh_status* st) {
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
+    if (bad_options(o)) return FVH_E_ARG;
     return s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st);
 }
 
 int32_t fvh_solver_advance(fvh_solver* s, int32_t n) {
     if (!s || n < 0) { set_error("bad argument"); return FVH_E_ARG; }
-    fvh_run_options o{1e300, 0xffffffffu, 1, 0.0};
+    fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
     for (int32_t k = 0; k < n; ++k) {
         int32_t rc = s->f32 ? enqueue_step<float>(s, false, &o, 0) : enqueue_step<double>(s, false, &o, 0);
         if (rc) return rc;

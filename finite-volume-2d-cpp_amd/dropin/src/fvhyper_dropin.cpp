@@ -157,6 +157,8 @@ void run(const std::string name, std::vector<double>& q, mpi_wrapper& pool, mesh
     o.max_step = opt.max_step;
     o.print_interval = opt.print_interval ? opt.print_interval : 1;
     o.tolerance = opt.tolerance;
+    o.stop_var = -1;
+    o.reserved = 0;
     const auto begin = std::chrono::steady_clock::now();
     auto seconds = [&]() {
         const auto end = std::chrono::steady_clock::now();

@@ -43,7 +43,8 @@ class Config(C.Structure):
 
 class RunOptions(C.Structure):
     _fields_ = [("max_time", C.c_double), ("max_step", C.c_uint32),
-                ("print_interval", C.c_uint32), ("tolerance", C.c_double)]
+                ("print_interval", C.c_uint32), ("tolerance", C.c_double),
+                ("stop_var", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Status(C.Structure):

@@ -315,3 +315,155 @@ def write_msh(path: str, m: MeshArrays) -> None:
         for blk in blocks:
             f.write(blk + "\n")
         f.write("$EndElements\n")
+
+
+# ---------------------------------------------------------------------------
+# NACA 0012 O-mesh: quad boundary band + triangulated rings to the farfield
+# ---------------------------------------------------------------------------
+
+def naca4_thickness(x, t=0.12):
+    """Half thickness of a symmetric NACA 4-digit section, closed trailing edge."""
+    return 5.0 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)
+
+
+def _zipper(a_idx, a_ang, b_idx, b_ang):
+    """Counter-clockwise triangles between an inner ring A and an outer ring B,
+    given node indices and polar angles (each ring ascending, one turn)."""
+    na, nb = a_idx.size, b_idx.size
+    # unwrap both to start near a_ang[0]
+    base = a_ang[0]
+    ua = base + np.mod(a_ang - base, 2 * np.pi)
+    ub = base + np.mod(b_ang - base, 2 * np.pi)
+    jb = int(np.argmin(ub))          # ring B starts at its first node past A's start
+    b_idx = np.roll(b_idx, -jb)
+    ub = np.roll(ub, -jb)
+    ua_next = np.append(ua[1:], ua[0] + 2 * np.pi)
+    ub_next = np.append(ub[1:], ub[0] + 2 * np.pi)
+    ev_ang = np.concatenate([ua_next, ub_next])
+    ev_src = np.concatenate([np.zeros(na, dtype=np.int8), np.ones(nb, dtype=np.int8)])
+    order = np.lexsort((ev_src, ev_ang))
+    src = ev_src[order]
+    ia = np.concatenate([[0], np.cumsum(src == 0)])[:-1]
+    ib = np.concatenate([[0], np.cumsum(src == 1)])[:-1]
+    tri = np.empty((na + nb, 4), dtype=np.int32)
+    tri[:, 3] = -1
+    isa = src == 0
+    tri[isa, 0] = a_idx[ia[isa] % na]
+    tri[isa, 1] = b_idx[ib[isa] % nb]
+    tri[isa, 2] = a_idx[(ia[isa] + 1) % na]
+    tri[~isa, 0] = a_idx[ia[~isa] % na]
+    tri[~isa, 1] = b_idx[ib[~isa] % nb]
+    tri[~isa, 2] = b_idx[(ib[~isa] + 1) % nb]
+    return tri
+
+
+
+def _naca_layout(n_surf, n_layers, first, growth, r_far, h):
+    """Node rings of the O-mesh.
+
+    Surface node k sits at parameter beta_k = 2 pi k / n_surf,
+    x = (1 + cos beta)/2, upper surface for beta < pi.  Band layer j lies at
+    wall distance d_j = first (growth^j - 1)/(growth - 1) along a direction
+    that turns from the surface normal to the straight line towards C_k, the
+    point of polar angle beta_k on the circle of radius R_b = d_L + 1/2 about
+    mid-chord; the last layer is that circle.  Outside it, concentric circles
+    at radial spacing ~h carry max(previous, 2 pi r / h) uniformly spaced
+    nodes each, the last at r_far.  Returns (band [L+1, n_surf, 2], rings)."""
+    k = np.arange(n_surf)
+    beta = 2.0 * np.pi * k / n_surf
+    xs = 0.5 * (1.0 + np.cos(beta))
+    ys = np.where(beta < np.pi, 1.0, -1.0) * naca4_thickness(xs)
+    ys[0] = 0.0
+    if n_surf % 2 == 0:
+        ys[n_surf // 2] = 0.0
+    P = np.stack([xs, ys], 1)
+    tang = np.roll(P, -1, 0) - np.roll(P, 1, 0)
+    nrm = np.stack([tang[:, 1], -tang[:, 0]], 1)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    d = first * (growth ** np.arange(n_layers + 1) - 1.0) / (growth - 1.0)
+    r_b = d[-1] + 0.5
+    centre = np.array([0.5, 0.0])
+    C = centre + r_b * np.stack([np.cos(beta), np.sin(beta)], 1)
+    band = np.empty((n_layers + 1, n_surf, 2))
+    for j in range(n_layers + 1):
+        s = d[j] / d[-1]
+        w = s * s
+        band[j] = P + (1.0 - w) * d[j] * nrm + w * s * (C - P)
+    n_rings = max(1, int(round((r_far - r_b) / h)))
+    rings = []
+    n_prev = n_surf
+    for j in range(1, n_rings + 1):
+        r = r_b + (r_far - r_b) * j / n_rings
+        n_j = max(n_prev, int(round(2.0 * np.pi * r / h)))
+        th = 2.0 * np.pi * (np.arange(n_j) + 0.5 * (j % 2)) / n_j
+        rings.append(centre + r * np.stack([np.cos(th), np.sin(th)], 1))
+        n_prev = n_j
+    return band, rings
+
+
+def _naca_count(n_surf, n_layers, first, growth, r_far, h):
+    _, rings = _naca_layout(n_surf, n_layers, first, growth, r_far, h)
+    n = [n_surf] + [r.shape[0] for r in rings]
+    return n_surf * n_layers + sum(n[i] + n[i + 1] for i in range(len(n) - 1))
+
+
+def naca0012_omesh(n_surf: int = 512, n_layers: int = 64, first: float = 1e-3, growth: float = 1.1,
+                   r_far: float = 50.0, target_cells: int = 2_000_000, h: float = None) -> MeshArrays:
+    """NACA 0012 O-mesh: the analytic 4-digit section (closed trailing edge)
+    with n_surf cosine-clustered surface points; n_layers quad layers of
+    geometric thickness (first spacing ``first``, ratio ``growth``: 64 layers
+    of 1e-3 x 1.1 reach ~4.45 chords); then triangles between concentric
+    circles out to a farfield of radius r_far, with the ring spacing h chosen
+    so that the mesh has about target_cells cells.  Boundaries ``airfoil``
+    and ``farfield``.  Cells in file order: quads layer by layer from the
+    wall, then triangles ring by ring, all counter-clockwise."""
+    if h is None:
+        lo, hi = 1e-3, 10.0
+        for _ in range(40):
+            mid = math.sqrt(lo * hi)
+            if _naca_count(n_surf, n_layers, first, growth, r_far, mid) > target_cells:
+                lo = mid
+            else:
+                hi = mid
+        h = hi
+    band, rings = _naca_layout(n_surf, n_layers, first, growth, r_far, h)
+    xy = np.concatenate([band.reshape(-1, 2)] + rings, 0)
+    L, S = n_layers, n_surf
+    jj, ss = np.meshgrid(np.arange(L), np.arange(S), indexing="ij")
+    a = jj * S + ss
+    b = jj * S + (ss + 1) % S
+    cells = [np.stack([a, a + S, b + S, b], -1).reshape(-1, 4).astype(np.int32)]
+    prev_idx = L * S + np.arange(S)
+    prev_ang = 2.0 * np.pi * np.arange(S) / S
+    start = (L + 1) * S
+    for j, r in enumerate(rings, 1):
+        n_j = r.shape[0]
+        idx = start + np.arange(n_j)
+        ang = 2.0 * np.pi * (np.arange(n_j) + 0.5 * (j % 2)) / n_j
+        cells.append(_zipper(prev_idx, prev_ang, idx, ang))
+        prev_idx, prev_ang, start = idx, ang, start + n_j
+    cells = np.concatenate(cells, 0)
+    surf = np.stack([np.arange(S), (np.arange(S) + 1) % S], 1)
+    far = np.stack([prev_idx, np.roll(prev_idx, -1)], 1)
+    bnd = np.concatenate([surf, far], 0).astype(np.int32)
+    tag = np.concatenate([np.zeros(S, dtype=np.int32), np.ones(far.shape[0], dtype=np.int32)])
+    return MeshArrays(xy[:, 0].copy(), xy[:, 1].copy(), cells, bnd, tag, ["airfoil", "farfield"])
+
+
+def naca0012_sized(total_cells: int, r_far: float = 50.0) -> MeshArrays:
+    """A coarsened NACA 0012 O-mesh of about total_cells cells: surface points
+    (a multiple of 8) and band layers scale with sqrt(total/2M) from 512 and
+    64; the first spacing and growth keep the band ~4.45 chords thick."""
+    s = math.sqrt(total_cells / 2.0e6)
+    n_surf = max(48, 8 * int(round(64 * s)))
+    n_layers = max(8, int(round(64 * s)))
+    first = 1e-3 / s
+    lo, hi = 1.0001, 3.0
+    for _ in range(60):
+        g = 0.5 * (lo + hi)
+        if first * (g ** n_layers - 1.0) / (g - 1.0) > 4.45:
+            hi = g
+        else:
+            lo = g
+    return naca0012_omesh(n_surf=n_surf, n_layers=n_layers, first=first, growth=0.5 * (lo + hi),
+                          r_far=r_far, target_cells=total_cells)

@@ -176,8 +176,10 @@ class Solver:
         return out
 
     def run(self, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
-            print_interval: int = 1) -> Dict:
-        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol))
+            print_interval: int = 1, stop_var: int = -1) -> Dict:
+        """run() of explicit.h:181; stop_var=-1 is the reference's stop test
+        (max over variables of R/R0), k >= 0 tests variable k alone."""
+        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol), int(stop_var), 0)
         st = _lib.Status()
         check(_lib.load().fvh_solver_run(self._h, C.byref(o), C.byref(st)))
         return self._status_dict(st)
@@ -186,8 +188,8 @@ class Solver:
         check(_lib.load().fvh_solver_run_begin(self._h))
 
     def run_continue(self, n: int, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
-                     print_interval: int = 1) -> Dict:
-        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol))
+                     print_interval: int = 1, stop_var: int = -1) -> Dict:
+        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol), int(stop_var), 0)
         st = _lib.Status()
         check(_lib.load().fvh_solver_run_continue(self._h, C.byref(o), int(n), C.byref(st)))
         return self._status_dict(st)

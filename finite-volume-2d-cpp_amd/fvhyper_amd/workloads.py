@@ -81,6 +81,44 @@ def forward_step(n_per_unit: int = 440, seed: int = 20240602, reference_compatib
     return m, cfg, uniform_ic(1.4, mach, 0.0, 1.0)
 
 
+def naca_freestream(mach: float = 0.8, aoa_deg: float = 1.25, reference_compatible: bool = False):
+    """(rho, u, v, p) of the airfoil freestream: rho=1.4, p=1 (so c=1 and
+    |V| = Mach), velocity at the angle of attack.  reference_compatible gives
+    the reference's hard-coded u=0.8, v=0.0175 (examples/naca/main.cpp:49-50,
+    194-199), i.e. 1.2532 degrees."""
+    if reference_compatible:
+        return 1.4, 0.8, 0.0175, 1.0
+    a = np.radians(aoa_deg)
+    return 1.4, mach * float(np.cos(a)), mach * float(np.sin(a)), 1.0
+
+
+def naca_config(mach: float = 0.8, aoa_deg: float = 1.25, reference_compatible: bool = False) -> Dict:
+    """Physics of examples/naca/main.cpp: second order (gradients, Michalak
+    limiter y_t=2, linear interpolation; :21-28, :56-65), the shockTube/naca
+    Roe spelling (:71-137), local edge-minimum time step with CFL 1.0
+    (:32, :142-183), slip wall on ``airfoil`` and the characteristic
+    ``farfield`` (:192-272), 5-stage RK."""
+    free = ("farfield",) + naca_freestream(mach, aoa_deg, reference_compatible)
+    return dict(flux="roe_split", phi="michalak", phi_yt=2.0, dt="edge_min", cfl=1.0, gdt=False,
+                grad=True, lim=True, lin=True, gamma=GAMMA, alpha=RK5,
+                bcs={"airfoil": ("mirror",), "farfield": free})
+
+
+def naca_airfoil(total_cells: int = 2_000_000, mach: float = 0.8, aoa_deg: float = 1.25,
+                 reference_compatible: bool = False) -> Tuple[meshgen.MeshArrays, Dict, Callable]:
+    """Transonic NACA 0012 to steady state: O-mesh with 512 surface points, a
+    64-layer quad band (first spacing 1e-3, growth 1.1) and triangles to a
+    farfield of radius 50 chords, ~2.0M cells at the default size (smaller
+    sizes coarsen surface points and layers with sqrt(cells)); Mach 0.8 at
+    1.25 degrees; uniform freestream initial state (naca/main.cpp:43-53).
+    Converged when the density residual has dropped 6 orders (stop_var=0,
+    tolerance 1e-6)."""
+    m = meshgen.naca0012_omesh() if total_cells == 2_000_000 else meshgen.naca0012_sized(total_cells)
+    cfg = naca_config(mach, aoa_deg, reference_compatible)
+    _, rho, u, v, p = cfg["bcs"]["farfield"]
+    return m, cfg, uniform_ic(rho, u, v, p)
+
+
 def initial_state(mesh, ic: Callable) -> np.ndarray:
     """Initial state over all cells of a built mesh (solver.Mesh or oracle Mesh)."""
     if hasattr(mesh, "export"):

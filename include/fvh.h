@@ -105,6 +105,9 @@ typedef struct fvh_run_options {
     uint32_t max_step;
     uint32_t print_interval;
     double tolerance;
+    int32_t stop_var;          /* -1: stop when max_k R[k] < tolerance, as explicit.cpp:718-723;
+                                  k in 0..3: when R[k] < tolerance (e.g. 0, the density residual) */
+    int32_t reserved;          /* 0 */
 } fvh_run_options;
 
 typedef struct fvh_status {

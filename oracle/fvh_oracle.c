@@ -594,7 +594,7 @@ void orc_calc_residuals(const orc_mesh* m, const double* qt, double* R) {
 
 /* run (explicit.cpp:657-817), single rank, no output files */
 int orc_run(const orc_mesh* m, const orc_config* cf, double* q,
-            uint32_t max_step, double max_time, double tol, uint32_t print_interval,
+            uint32_t max_step, double max_time, double tol, int32_t stop_var, uint32_t print_interval,
             uint32_t* steps_out, double* time_out, double* R_out, double* R0_out,
             double* qt_out) {
     const int64_t N = m->n_cells, n = V * N;
@@ -614,8 +614,9 @@ int orc_run(const orc_mesh* m, const orc_config* cf, double* q,
     double time = 0;
     for (;;) {
         double Rmax = 0.0;
-        if (step > 0) { for (int k = 0; k < V; ++k) Rmax = smax(R[k], Rmax); }
-        else Rmax = 1.0;
+        if (step == 0) Rmax = 1.0;
+        else if (stop_var >= 0 && stop_var < V) Rmax = R[stop_var];
+        else { for (int k = 0; k < V; ++k) Rmax = smax(R[k], Rmax); }
         if ((step >= max_step) | (time >= max_time) | (Rmax < tol)) break;
         orc_calc_dt(m, cf, q, dt);
         if (cf->gdt) orc_min_dt(m, dt);

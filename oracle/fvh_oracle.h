@@ -96,9 +96,11 @@ void orc_bc(const orc_config* c, int slot, double* b, const double* qint, const 
 
 /* run() of explicit.cpp:657-817 with the config's alpha list.  q holds the
  * initial solution over all cells on entry and the final state on exit.
+ * stop_var < 0 is the reference's stop test (max_k R[k] < tol,
+ * explicit.cpp:718-723); stop_var = k tests R[k] alone.
  * out: step count, final time, R[4] (normalised), R0[4]. */
 int orc_run(const orc_mesh* m, const orc_config* c, double* q,
-            uint32_t max_step, double max_time, double tol, uint32_t print_interval,
+            uint32_t max_step, double max_time, double tol, int32_t stop_var, uint32_t print_interval,
             uint32_t* steps_out, double* time_out, double* R_out, double* R0_out,
             double* qt_out);
 

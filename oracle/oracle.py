@@ -90,7 +90,7 @@ def lib():
         L.orc_phi.argtypes = [cfgp, C.c_double]
         L.orc_phi.restype = C.c_double
         L.orc_bc.argtypes = [cfgp, C.c_int, dp, dp, dp]
-        L.orc_run.argtypes = [mp, cfgp, dp, C.c_uint32, C.c_double, C.c_double, C.c_uint32,
+        L.orc_run.argtypes = [mp, cfgp, dp, C.c_uint32, C.c_double, C.c_double, C.c_int32, C.c_uint32,
                               C.POINTER(C.c_uint32), C.POINTER(C.c_double), dp, dp, C.c_void_p]
         L.orc_run.restype = C.c_int
         _lib = L
@@ -215,8 +215,9 @@ class Mesh:
         return out
 
     def run(self, cfg: Dict, q0: np.ndarray, max_step: int, max_time: float = 1e10,
-            tol: float = 1e-16, print_interval: int = 1):
+            tol: float = 1e-16, print_interval: int = 1, stop_var: int = -1):
         """run() of explicit.cpp:657-817 from initial state q0 (all cells).
+        stop_var=-1 is the reference's stop test; k >= 0 tests R[k] only.
         Returns (q, steps, time, R, R0, qt)."""
         L = lib()
         c = make_config(cfg, self.names)
@@ -227,7 +228,7 @@ class Mesh:
         R0 = np.zeros(4)
         qt = np.zeros_like(q)
         rc = L.orc_run(self._p, C.byref(c), q, int(max_step), float(max_time), float(tol),
-                       int(print_interval), C.byref(steps), C.byref(t), R, R0,
+                       int(stop_var), int(print_interval), C.byref(steps), C.byref(t), R, R0,
                        qt.ctypes.data_as(C.c_void_p))
         if rc != 0:
             raise MemoryError("orc_run failed")

@@ -1,6 +1,6 @@
 """Regenerate the golden fixtures from the compiled reference.
 
-    make -C oracle ref && python tests/golden/gen_fixtures.py
+    make -C oracle ref && python tests/golden/gen_fixtures.py [mesh ...]
 
 Runs oracle/_ref/fvh_ref -- the reference fvhyper library built from its own
 sources with the renamed example plugins -- on small seeded meshes written
@@ -32,7 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import oracle as O  # noqa: E402
 from fvhyper_amd import meshgen  # noqa: E402
-from helpers import random_state  # noqa: E402
+from helpers import airfoil_as_channel, airfoil_bump_state, random_state  # noqa: E402
 
 KIND_NAME = {"mirror": "wall", "copy": "transmissive", "inlet_outlet": "inlet", "farfield": "farfield"}
 
@@ -40,6 +40,7 @@ FIX_MESHES = {
     "mixed": lambda: meshgen.mixed_channel(nx=12, ny=6),
     "sod": lambda: meshgen.sod_tube(nx=24, ny=3),
     "step": lambda: meshgen.forward_step(n_per_unit=10),
+    "naca": lambda: airfoil_as_channel(meshgen.naca0012_sized(1500)),
 }
 
 FS = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
@@ -75,6 +76,15 @@ def renamed(m, cfg):
     return m2, cfg2
 
 
+def run_state(mesh_name, om):
+    """Initial state of the run fixtures: seeded random cells on the channel
+    meshes; on the airfoil mesh (where the local-dt limiter case would blow
+    up from cell-by-cell noise) the naca freestream with a smooth bump."""
+    if mesh_name != "naca":
+        return random_state(om.n_cells, seed=12, mach=0.3)
+    return airfoil_bump_state(om.cell_cx, om.cell_cy)
+
+
 def mesh_arrays(m):
     return dict(x=m.x, y=m.y, cells=m.cells, bnd=m.bnd, bnd_tag=m.bnd_tag, names=np.array(m.names))
 
@@ -83,7 +93,10 @@ def main():
     if not O.ref_available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
     work = tempfile.mkdtemp(prefix="fvh_golden_")
+    only = sys.argv[1:]              # optional: regenerate these meshes only
     for mname, fac in FIX_MESHES.items():
+        if only and mname not in only:
+            continue
         m = fac()
         meshgen.write_msh(os.path.join(work, mname + ".msh"), m)
         r = O.run_ref("mesh", work, mname)
@@ -103,7 +116,7 @@ def main():
             ph = O.run_ref("phases", work, case, flags=O.ref_flags(cfg2), alpha="0.2898", **kv)
             np.savez_compressed(os.path.join(HERE, "phases__%s.npz" % case), q=q, alpha=0.2898,
                                 **{k: v for k, v in ph.items() if k != "stdout"})
-            q0 = random_state(om.n_cells, seed=12, mach=0.3)
+            q0 = run_state(mname, om)
             q0.tofile(qpath)
             run = O.run_ref("run", work, case, flags=O.ref_flags(cfg2), max_step="10", print="1", **kv)
             last = run["stdout"].strip().splitlines()[-1].split(",")

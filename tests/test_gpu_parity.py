@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 from fvhyper_amd import Mesh, Solver, meshgen, workloads
-from helpers import CONFIGS, MESHES, bits_equal, random_state, rel_err, same_values
+from helpers import CONFIGS, MESHES, airfoil_bump_state, bits_equal, random_state, rel_err, same_values
 
 pytestmark = pytest.mark.gpu
 
@@ -40,7 +40,10 @@ def test_run_bit_exact(mesh_name, cfg_name, order, flux_mode):
     print("case", mesh_name, cfg_name, order, flux_mode, flush=True)
     cfg = CONFIGS[cfg_name]
     m, om, pm, s = _pair(mesh_name, cfg, renumber=order, flux_mode=flux_mode)
-    q0 = random_state(om.n_cells, seed=202, mach=0.3)
+    if mesh_name == "naca":
+        q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
+    else:
+        q0 = random_state(om.n_cells, seed=202, mach=0.3)
     s.set_state(q0)
     st = s.run(max_step=12, print_interval=5)
     qg = s.get_state()
@@ -136,3 +139,55 @@ def test_reference_scheme_blows_up_identically():
     qo, steps, t, R, R0, _ = om.run(cfg, q0, 130)
     assert not np.all(np.isfinite(qo))
     assert same_values(s.get_state(), qo)
+
+
+@pytest.mark.parametrize("mesh_name,cfg_name", [("sod", "first_split_sum"), ("step", "second_venkat_global"),
+                                               ("naca", "second_michalak_local")])
+def test_fp32_state_against_fp64_oracle(mesh_name, cfg_name):
+    """FP32 state and geometry, same kernels: error against the FP64 oracle
+    after 10 steps, as max_ik |a-b| / max(|a_ik|, max_i |a_.k|), below 2e-5."""
+    cfg = CONFIGS[cfg_name]
+    m, om, pm, s = _pair(mesh_name, cfg, precision="f32")
+    if mesh_name == "sod":
+        q0 = workloads.initial_state(om, workloads.sod_ic())
+    elif mesh_name == "naca":
+        q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
+    else:
+        q0 = random_state(om.n_cells, seed=303, mach=0.3)
+    s.set_state(q0)
+    st = s.run(max_step=10)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 10)
+    assert st["step"] == steps == 10
+    assert np.all(np.isfinite(qo))
+    err = rel_err(qo, s.get_state())
+    print("fp32 rel err", mesh_name, cfg_name, err)
+    assert err < 2e-5
+
+
+def test_airfoil_workload_bit_exact():
+    """The transonic NACA 0012 workload (Mach 0.8, 1.25 degrees) on a 20k-cell
+    O-mesh from the freestream, 30 steps, tile order: bit-exact."""
+    m, cfg, ic = workloads.naca_airfoil(20000)
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_step=30, print_interval=10)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 30, print_interval=10)
+    assert st["step"] == steps == 30 and np.all(np.isfinite(qo))
+    assert bits_equal(s.get_state(), qo)
+
+
+def test_airfoil_density_stop_matches():
+    """Stop when the density residual alone has dropped below the tolerance
+    (stop_var=0): same step count and state as the oracle."""
+    m, cfg, ic = workloads.naca_airfoil(3000)
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg)
+    q0 = workloads.initial_state(om, ic)
+    s.set_state(q0)
+    st = s.run(max_step=5000, tol=0.05, stop_var=0)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 5000, tol=0.05, stop_var=0)
+    assert st["stopped"] and st["step"] == steps < 5000
+    assert bits_equal(s.get_state(), qo)
+    np.testing.assert_allclose(st["R"], R, rtol=1e-10)
