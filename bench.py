@@ -1,13 +1,18 @@
-"""Benchmark: explicit second-order RK5 on the Mach-3 forward-step workload.
+"""Benchmark: explicit second-order RK5 steps of the explicit path.
 
     python bench.py [--gpus 1] [--steps 400] [--warmup 20] [--precision f64]
+                    [--workload step|airfoil] [--converge]
 
 One step = one RK5 step (5 stages: gradient+limiter, flux+update+boundary)
-plus the time-step pass, on the 975,744-cell forward-step mesh, FP64,
-inputs resident in HBM.  Prints one JSON line with the throughput in
-Mcell-stage/s, the HBM roofline of the dominant kernel (measured with HIP
-events on the solver's stream over the timed region) and a CPU baseline
-(the reference library on a bounded sample of the same workload).
+plus the time-step pass, inputs resident in HBM.  The default workload is
+the 975,744-cell Mach-3 forward step in FP64; ``--workload airfoil`` is the
+2.0M-cell transonic NACA 0012 O-mesh with local time steps, and with
+``--converge`` it runs from the freestream until the density residual has
+dropped 6 orders and reports the time to get there.  Prints one JSON line
+with the throughput in Mcell-stage/s, the HBM roofline of the dominant
+kernel (measured with HIP events on the solver's stream over the timed
+region) and a CPU baseline (the reference library on a bounded sample of
+the same workload).
 """
 from __future__ import annotations
 
@@ -39,9 +44,11 @@ def algorithmic_bytes(C, E, B, s, g, V=4):
     return {"stage": flux + gradlim, "flux": flux, "gradlim": gradlim}
 
 
-def cpu_baseline(mesh_arrays, cfg, steps: int):
+def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step"):
     """Time the reference library (oracle/_ref/fvh_ref, kind 'reference') on
-    `steps` RK5 steps of the same mesh; falls back to the C restatement."""
+    `steps` RK5 steps of the same mesh; falls back to the C restatement.
+    For the airfoil the reference's farfield has its own hard-coded
+    freestream (Mach 0.8 at 1.2532 degrees), which costs the same."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from fvhyper_amd import meshgen
@@ -50,28 +57,148 @@ def cpu_baseline(mesh_arrays, cfg, steps: int):
     if O.ref_available():
         try:
             work = tempfile.mkdtemp(prefix="fvh_cpu_")
-            meshgen.write_msh(os.path.join(work, "step.msh"), mesh_arrays)
+            meshgen.write_msh(os.path.join(work, "case.msh"), mesh_arrays)
             cfg2 = dict(cfg)
             cfg2["K"] = 7.5   # the reference library fixes K; the cost is the same
             kv = O.ref_args(cfg2)
-            kv.update(ic="fs", mach="3", max_step=str(steps), print=str(steps))
-            r = O.run_ref("run", work, "step", flags=O.ref_flags(cfg2), timeout=900, **kv)
+            if workload == "step":
+                kv.update(ic="fs", mach="3")
+            else:
+                kv.update(ic="nc")
+            kv.update(max_step=str(steps), print=str(steps))
+            r = O.run_ref("run", work, "case", flags=O.ref_flags(cfg2), timeout=900, **kv)
             last = r["stdout"].strip().splitlines()[-1].split(",")
             secs = float(last[2])
             return {"value": C * 5 * steps / secs / 1e6, "unit": "Mcell-stage/s", "cores": 1,
                     "kind": "reference",
-                    "sample": "%d RK5 steps (%d stages) of the %d-cell step workload, fvhyper library "
-                              "built from its sources (-O3), 1 rank, RealTime loop clock" % (steps, 5 * steps, C)}
+                    "sample": "%d RK5 steps (%d stages) of the %d-cell %s workload, fvhyper library "
+                              "built from its sources (-O3), 1 rank, RealTime loop clock"
+                              % (steps, 5 * steps, C, workload)}
         except Exception as e:  # pragma: no cover - reported, then the port is timed
             print("reference baseline unavailable: %s" % e, file=sys.stderr)
     om = O.Mesh(mesh_arrays)
     from fvhyper_amd import workloads
-    q0 = workloads.initial_state(om, workloads.uniform_ic(1.4, 3.0, 0.0, 1.0))
+    _, rho, u, v, p = cfg["bcs"]["inlet" if workload == "step" else "farfield"]
+    q0 = workloads.initial_state(om, workloads.uniform_ic(rho, u, v, p))
     t0 = time.perf_counter()
     om.run(cfg, q0, steps)
     secs = time.perf_counter() - t0
     return {"value": C * 5 * steps / secs / 1e6, "unit": "Mcell-stage/s", "cores": 1, "kind": "port",
-            "sample": "%d RK5 steps of the %d-cell step workload, C restatement, 1 thread" % (steps, C)}
+            "sample": "%d RK5 steps of the %d-cell %s workload, C restatement, 1 thread" % (steps, C, workload)}
+
+
+def converge(s, C, stages, tol, stop_var, chunk=2000, limit=400000):
+    """Run from the freestream until R[stop_var] < tol (the reference's
+    max-over-variables rule for stop_var=-1); returns a dict of the run."""
+    import numpy as np
+    s.run_begin()
+    s.synchronize()
+    t0 = time.perf_counter()
+    while True:
+        st = s.run_continue(chunk, max_step=limit, tol=tol, stop_var=stop_var, print_interval=100)
+        if st["stopped"] or not np.all(np.isfinite(st["R"])) or st["step"] >= limit:
+            break
+    secs = time.perf_counter() - t0
+    return {"steps": st["step"], "seconds": round(secs, 3), "converged": bool(st["stopped"]) and st["step"] < limit,
+            "tolerance": tol, "rule": "density" if stop_var == 0 else "max_over_variables",
+            "R": [float(x) for x in st["R"]], "Mcell_stage_per_s": round(C * stages * st["step"] / secs / 1e6, 2)}
+
+
+def sweep(a):
+    """Parameter sweep: a batch of coarsened airfoil instances (sizes
+    log-uniform in 5k-200k cells, Mach 0.5-0.85, angle -2..4 degrees), FP32
+    state with FP64 residual sums, every step replayed from a HIP graph;
+    against the same instances run one after another through the plain
+    single-mesh path (own solver each, eager launches, synchronised per
+    instance)."""
+    import numpy as np
+    import torch
+
+    from fvhyper_amd import Batch, Mesh, Solver, workloads
+
+    t0 = time.perf_counter()
+    items = workloads.naca_sweep(a.instances)
+    built = {}
+    meshes = []
+    for it in items:
+        key = id(it[0])
+        if key not in built:
+            built[key] = Mesh.from_arrays(it[0])
+        meshes.append(built[key])
+    cfgs = [it[1] for it in items]
+    q0 = [workloads.initial_state(m, it[2]) for m, it in zip(meshes, items)]
+    b = Batch(meshes, cfgs, precision=a.precision)
+    b.set_state(b.join(q0))
+    b.apply_bounds()
+    b.use_graphs(True)
+    setup = time.perf_counter() - t0
+    C = sum(m.n_real for m in meshes)
+    E = sum(m.n_edges for m in meshes)
+    B = sum(m.n_bnd for m in meshes)
+    stages = len(cfgs[0]["alpha"])
+    b.advance(a.warmup)
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.advance(a.steps)
+    b.synchronize()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    if not np.all(np.isfinite(b.get_state())):
+        print("non-finite batch state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+    # kernel timing (events, eager launches) over a few more steps
+    b.use_graphs(False)
+    b.profile(True)
+    b.advance(max(2, a.steps // 10))
+    b.synchronize()
+    kt = b.kernel_times()
+    b.profile(False)
+    # the same instances one by one
+    k_pm = a.per_mesh_steps or a.steps
+    solvers = []
+    for m, c, q in zip(meshes, cfgs, q0):
+        x = Solver(m, c, precision=a.precision)
+        x.set_state(q)
+        x.apply_bounds()
+        x.advance(2)
+        x.synchronize()
+        solvers.append(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in solvers:
+        x.advance(k_pm)
+        x.synchronize()
+    secs_pm = time.perf_counter() - t0
+    value = C * stages * a.steps / secs / 1e6
+    value_pm = C * stages * k_pm / secs_pm / 1e6
+    sb = 4 if a.precision == "f32" else 8
+    ab = algorithmic_bytes(C, E, B, sb, sb)
+    roof = None
+    if kt.get("flux", {}).get("count", 0):
+        avg_ms = kt["flux"]["total_ms"] / kt["flux"]["count"]
+        ach = ab["flux"] / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_flux (edge-flux gather + RK update + boundary), all instances",
+                "bytes_per_launch": ab["flux"], "avg_launch_us": round(avg_ms * 1e3, 2),
+                "kernels_us": {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}}
+    out = {
+        "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "f64" else "f32", "data": "synthetic",
+        "config": {"workload": "naca0012_sweep", "instances": len(meshes), "cells": C, "edges": E,
+                   "boundary_edges": B, "cells_range": [5000, 200000], "mach_range": [0.5, 0.85],
+                   "aoa_deg_range": [-2.0, 4.0], "order": 2, "limiter": "michalak", "dt": "local",
+                   "rk_stages": stages, "precision": a.precision, "residual_sums": "f64",
+                   "launch": "batched, hipGraph per step", "parallelism": "single_gpu"},
+        "per_mesh": {"value": round(value_pm, 2), "steps": k_pm, "seconds": round(secs_pm, 3),
+                     "launch": "one solver per instance, eager, synchronised per instance"},
+        "speedup_vs_per_mesh": round(value / value_pm, 3), "setup_s": round(setup, 1),
+        "roofline": roof, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -80,7 +207,15 @@ def main():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--n-per-unit", type=int, default=440)
+    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep"])
+    ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
+    ap.add_argument("--per-mesh-steps", type=int, default=0,
+                    help="sweep: steps of the one-by-one baseline (default: --steps)")
+    ap.add_argument("--n-per-unit", type=int, default=440, help="step mesh resolution")
+    ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
+    ap.add_argument("--converge", action="store_true",
+                    help="airfoil: also run from the freestream until the density residual drops 6 orders")
+    ap.add_argument("--stop-var", type=int, default=0, help="with --converge: 0 density, -1 max over variables")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-launch events")
@@ -98,7 +233,12 @@ def main():
     from fvhyper_amd import Mesh, Solver, workloads
 
     torch.cuda.set_device(0)
-    m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
+    if a.workload == "sweep":
+        return sweep(a)
+    if a.workload == "step":
+        m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
+    else:
+        m, cfg, ic = workloads.naca_airfoil(a.cells)
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg, precision=a.precision)
     q0 = workloads.initial_state(pm, ic)
@@ -146,17 +286,28 @@ def main():
         stage_ms = sum(v["total_ms"] for v in kt.values()) / (a.steps * stages)
         roof["stage_achieved"] = round(ab["stage"] / (stage_ms * 1e-3) / 1e9, 1)
         roof["kernels_us"] = {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}
-    cpu = None if a.no_cpu else cpu_baseline(m, cfg, a.cpu_steps)
+    conv = None
+    if a.converge:
+        s.set_state(q0)
+        conv = converge(s, C, stages, 1e-6, a.stop_var)
+    cpu = None if a.no_cpu else cpu_baseline(m, cfg, a.cpu_steps, a.workload)
+    if a.workload == "step":
+        wl = {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
+              "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
+              "cfl": cfg["cfl"], "dt": "global", "precision": a.precision, "parallelism": "single_gpu"}
+    else:
+        wl = {"workload": "naca0012_transonic", "cells": C, "edges": E, "boundary_edges": B,
+              "order": 2, "limiter": "michalak", "mach": 0.8, "aoa_deg": 1.25, "rk_stages": stages,
+              "cfl": cfg["cfl"], "dt": "local", "precision": a.precision, "parallelism": "single_gpu"}
     out = {
         "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if a.precision == "f64" else "f32", "data": "synthetic",
-        "config": {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
-                   "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
-                   "cfl": cfg["cfl"], "precision": a.precision, "parallelism": "single_gpu"},
-        "roofline": roof, "cpu_baseline": cpu,
+        "config": wl, "roofline": roof, "cpu_baseline": cpu,
     }
+    if conv is not None:
+        out["converge"] = conv
     print(json.dumps(out))
 
 

@@ -73,6 +73,25 @@ struct MeshView {
     const G2<T>* c_c;     // cell centre
 };
 
+// Tiles of at most kBlock consecutive device cells, each owned by one
+// instance (a single mesh is a batch of one instance).  The tiles of real
+// cells come first, [0, n_rt), instance by instance; then the tiles of the
+// boundary ghost cells, [n_rt, n_all).  One workgroup processes one tile.
+struct TileMap {
+    int32_t n_rt, n_all;
+    const int32_t* c0;     // [n_all+1] first device cell of each tile
+    const int32_t* inst;   // [n_all] instance owning each tile
+};
+
+// Per-instance ranges for the segmented reductions (time-step minimum,
+// residual sums) and the run bookkeeping.
+struct InstView {
+    int32_t n;
+    const int4* tiles;     // (rt0, rt1, gt0, gt1): real and ghost tile ranges
+    const int32_t* dev0;   // device position of the instance's reference cell 0
+};
+
+// Edge lists of the real-cell tiles for the tiled flux pass.
 struct TileView {
     int32_t n_tiles;
     const int32_t* off;    // [n_tiles+1] into the tile edge lists
@@ -87,31 +106,39 @@ struct PhysParams {
     int positivity;   // first-order fallback for non-physical face states
 };
 
-// Launchers (implemented in fvh_kernels.hip).  All take the solver stream.
+// Launchers (implemented in fvh_kernels.hip).  All take the solver stream;
+// ctrl is the per-instance control array (nullptr: no stop test).
 template <class T>
-void launch_dt(hipStream_t s, const MeshView<T>& m, int dt_kind, const PhysParams& pp,
+void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_kind, const PhysParams& pp,
                const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl);
-void launch_dt_min(hipStream_t s, const double* blk_min, int nblk, const void* dt_cells,
-                   int dt_is_f32, int32_t dev0, double* dt_global, const Ctrl* ctrl);
+// Per-instance minimum of the time step over real and ghost cells (min_dt).
+void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells,
+                   int dt_is_f32, void* dt_global, const Ctrl* ctrl);
 // Gradient + limiter + face states; with dt_kind >= 0 also the time step of
-// every cell and per-block minima (first stage of a step).
+// every cell and per-tile minima (first stage of a step).
 template <class T>
-void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind,
+void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
                     bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr,
                     int dt_kind, T* dt, double* blk_min,
                     Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl);
 // Flux + RK update + boundary update; tiles == nullptr selects the per-cell
-// form that computes every interior edge flux from both sides.
+// form that computes every interior edge flux from both sides.  dt_inst
+// (global time step per instance) replaces the per-cell dt when non-null.
 template <class T>
-void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, const PhysParams& pp,
-                 bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0,
-                 Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt, int dt_stride,
-                 T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
+void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles,
+                 const PhysParams& pp, bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
+                 const Q4<T>* q0, Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
+                 const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
                         const BcParam<T>* bcs, Q4<T>* q);
-void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var);
-void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,
-                     const void* dt_src, int dt_is_f32, int32_t dt_index);
+void launch_ctrl_begin(hipStream_t s, int n_inst, Ctrl* ctrl, uint32_t max_step, double max_time, double tol,
+                       int stop_var);
+void launch_ctrl_end(hipStream_t s, const InstView& iv, Ctrl* ctrl, const double* res_blk, int need_res,
+                     const void* dt_src, int dt_is_f32, int dt_is_inst);
+// FP64 conserved sums sum_i A_i q_ik over each instance's real cells.
+template <class T>
+void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* q,
+                 double* blk, double* out);
 
 }  // namespace fvh

@@ -9,7 +9,7 @@ namespace {
 
 constexpr double kInf = __builtin_inf();
 
-__device__ inline bool stopped(const Ctrl* ctrl) { return ctrl != nullptr && ctrl->stop != 0; }
+__device__ inline bool stopped(const Ctrl* ctrl, int inst) { return ctrl != nullptr && ctrl[inst].stop != 0; }
 
 // NaN-ignoring minimum over a 256-thread block (std::min(a,b) keeps a unless
 // b < a, so NaN never wins once a finite value is held).
@@ -96,15 +96,14 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
 }
 
 template <class T, int KIND>
-__global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, PhysParams pp,
+__global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, TileMap tm, PhysParams pp,
                                                const Q4<T>* __restrict__ q, T* __restrict__ dt,
                                                double* __restrict__ blk_min, const Ctrl* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const int lb = xcd_block((m.n_total + kBlock - 1) / kBlock);
-    if (lb < 0) return;
-    const int c = lb * kBlock + threadIdx.x;
+    const int lb = xcd_block(tm.n_all);
+    if (lb < 0 || stopped(ctrl, tm.inst[lb])) return;
+    const int c = tm.c0[lb] + threadIdx.x;
     double v = kInf;
-    if (c < m.n_total) {
+    if (c < tm.c0[lb + 1]) {
         const T r = cell_dt<T, KIND>(m, pp, c, ld(q + c), q);
         dt[c] = r;
         v = (double)r;
@@ -113,18 +112,22 @@ __global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, PhysParams pp,
     if (threadIdx.x == 0) blk_min[lb] = v;
 }
 
+// One workgroup per instance: minimum over its real and ghost tiles.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_dt_min(const double* __restrict__ blk_min, int nblk,
-                                                   const T* __restrict__ dt_cells, int32_t dev0,
-                                                   T* __restrict__ dt_global, const Ctrl* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
+__global__ __launch_bounds__(kBlock) void k_dt_min(InstView iv, const double* __restrict__ blk_min,
+                                                   const T* __restrict__ dt_cells, T* __restrict__ dt_global,
+                                                   const Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x;
+    if (stopped(ctrl, i)) return;
+    const int4 r = iv.tiles[i];
     double v = kInf;
-    for (int i = threadIdx.x; i < nblk; i += kBlock) v = smin(v, blk_min[i]);
+    for (int t = r.x + threadIdx.x; t < r.y; t += kBlock) v = smin(v, blk_min[t]);
+    for (int t = r.z + threadIdx.x; t < r.w; t += kBlock) v = smin(v, blk_min[t]);
     v = block_min(v);
     if (threadIdx.x == 0) {
         // min_dt starts from dt[0] (explicit.cpp:557): a NaN there survives
-        const T d0 = dt_cells[dev0];
-        dt_global[0] = (d0 != d0) ? d0 : T(v);
+        const T d0 = dt_cells[iv.dev0[i]];
+        dt_global[i] = (d0 != d0) ? d0 : T(v);
     }
 }
 
@@ -133,18 +136,17 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(const double* __restrict__ bl
 // per cell (explicit.cpp:112-258, 307-313); optionally the time step.
 // ---------------------------------------------------------------------------
 template <class T, int PHI, int DT>
-__global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, PhysParams pp, int grad, int lim_on,
+__global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad, int lim_on,
                                                     int lin, const Q4<T>* __restrict__ q,
                                                     Q4<T>* __restrict__ face, Q4<T>* __restrict__ bincr,
                                                     T* __restrict__ dt, double* __restrict__ blk_min,
                                                     Q4<T>* __restrict__ dbg_gx, Q4<T>* __restrict__ dbg_gy,
                                                     Q4<T>* __restrict__ dbg_lim, const Ctrl* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const int lb = xcd_block((m.n_total + kBlock - 1) / kBlock);
-    if (lb < 0) return;
-    const int c = lb * kBlock + threadIdx.x;
+    const int lb = xcd_block(tm.n_all);
+    if (lb < 0 || stopped(ctrl, tm.inst[lb])) return;
+    const int c = tm.c0[lb] + threadIdx.x;
     double vmin = kInf;
-    if (c < m.n_total) {
+    if (c < tm.c0[lb + 1]) {
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
         const Q4<T> qc = ld(q + c);
         const G2<T> cc = m.c_c[c];
@@ -291,10 +293,9 @@ template <class T, bool LIN>
 __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
                                    int h0, int h1, const T (&qt)[4], const Q4<T>* __restrict__ q0,
                                    Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
-                                   const T* __restrict__ dt, int dt_stride, T alpha, bool want_rs,
+                                   const T d, T alpha, bool want_rs,
                                    double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
     const T g = T(pp.gamma);
-    const T d = dt[(int64_t)c * dt_stride];
     const Q4<T> qs = ld(q0 + c);
     Q4<T> out;
 #pragma unroll
@@ -336,19 +337,22 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
 
 // Per-cell form: every interior edge flux is computed by both of its cells.
 template <class T, bool JOINED, bool LIN>
-__global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, PhysParams pp, const BcParam<T>* __restrict__ bcs,
+__global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm, PhysParams pp,
+                                                      const BcParam<T>* __restrict__ bcs,
                                                       const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
                                                       Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
-                                                      int dt_stride, T alpha, double* __restrict__ res_blk,
+                                                      const T* __restrict__ dt_inst, T alpha,
+                                                      double* __restrict__ res_blk,
                                                       Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const int lb = xcd_block((m.n_real + kBlock - 1) / kBlock);
+    const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
-    const int c = lb * kBlock + threadIdx.x;
+    const int inst = tm.inst[lb];
+    if (stopped(ctrl, inst)) return;
+    const int c = tm.c0[lb] + threadIdx.x;
     const T g = T(pp.gamma);
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
-    if (c < m.n_real) {
+    if (c < tm.c0[lb + 1]) {
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
         const T A = m.c_area[c];
         T qt[4] = {T(0), T(0), T(0), T(0)};
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, PhysParams 
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt, dt_stride, alpha,
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt_inst ? dt_inst[inst] : dt[c], alpha,
                             res_blk != nullptr, rs, dbg_qt);
     }
     if (res_blk) {
@@ -397,15 +401,18 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, PhysParams 
 // reference edge order.  Edges on a tile border are computed by both tiles
 // (same inputs, same bits).
 template <class T, bool JOINED, bool LIN>
-__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileView tv, PhysParams pp, const BcParam<T>* __restrict__ bcs,
+__global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm, TileView tv, PhysParams pp,
+                                                      const BcParam<T>* __restrict__ bcs,
                                                       const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
                                                       Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
-                                                      int dt_stride, T alpha, double* __restrict__ res_blk,
+                                                      const T* __restrict__ dt_inst, T alpha,
+                                                      double* __restrict__ res_blk,
                                                       Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
-    if (stopped(ctrl)) return;
-    const int lb = xcd_block(tv.n_tiles);
+    const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
+    const int inst = tm.inst[lb];
+    if (stopped(ctrl, inst)) return;
     __shared__ Q4<T> fle[kTileEdges];
     const T g = T(pp.gamma);
     const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
@@ -433,9 +440,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileView tv
         fle[k - e0] = r;
     }
     __syncthreads();
-    const int c = lb * kBlock + threadIdx.x;
+    const int c = tm.c0[lb] + threadIdx.x;
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
-    if (c < m.n_real) {
+    if (c < tm.c0[lb + 1]) {
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
         const T A = m.c_area[c];
         T qt[4] = {T(0), T(0), T(0), T(0)};
@@ -448,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileView tv
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt, dt_stride, alpha,
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt_inst ? dt_inst[inst] : dt[c], alpha,
                             res_blk != nullptr, rs, dbg_qt);
     }
     if (res_blk) {
@@ -493,47 +500,86 @@ __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParam
 // ---------------------------------------------------------------------------
 // run() bookkeeping on the device (explicit.cpp:715-801)
 // ---------------------------------------------------------------------------
-__global__ void k_ctrl_begin(Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var) {
-    if (ctrl->stop) return;
+__global__ void k_ctrl_begin(int n_inst, Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    Ctrl* c = ctrl + i;
+    if (c->stop) return;
     double Rmax = 0.0;
-    if (ctrl->step == 0) {
+    if (c->step == 0) {
         Rmax = 1.0;
     } else if (stop_var >= 0 && stop_var < 4) {
-        Rmax = ctrl->R[stop_var];
+        Rmax = c->R[stop_var];
     } else {
-        for (int k = 0; k < 4; ++k) Rmax = smax(ctrl->R[k], Rmax);
+        for (int k = 0; k < 4; ++k) Rmax = smax(c->R[k], Rmax);
     }
-    if ((ctrl->step >= max_step) | (ctrl->time >= max_time) | (Rmax < tol)) ctrl->stop = 1;
+    if ((c->step >= max_step) | (c->time >= max_time) | (Rmax < tol)) c->stop = 1;
 }
 
+// One workgroup per instance: residual norms from the per-tile partial sums
+// (explicit.cpp:481-552, 752-771) and the step/time bookkeeping (:800).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_ctrl_end(Ctrl* ctrl, const double* __restrict__ res_blk, int nblk,
-                                                     int need_res, const T* __restrict__ dt_src, int32_t dt_index) {
-    if (ctrl->stop) return;
+__global__ __launch_bounds__(kBlock) void k_ctrl_end(InstView iv, Ctrl* ctrl, const double* __restrict__ res_blk,
+                                                     int need_res, const T* __restrict__ dt_src, int dt_is_inst) {
+    const int i = blockIdx.x;
+    Ctrl* c = ctrl + i;
+    if (c->stop) return;
     if (need_res) {
+        const int4 r = iv.tiles[i];
         double s[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = threadIdx.x; i < nblk; i += kBlock)
-            for (int k = 0; k < 4; ++k) s[k] += res_blk[4 * i + k];
+        for (int t = r.x + threadIdx.x; t < r.y; t += kBlock)
+            for (int k = 0; k < 4; ++k) s[k] += res_blk[4 * t + k];
         block_sum4(s);
         if (threadIdx.x == 0) {
             double R[4];
             for (int k = 0; k < 4; ++k) R[k] = sqrt(s[k]);
-            if (ctrl->step == 0) {
-                for (int k = 0; k < 4; ++k) { ctrl->R0[k] = R[k]; ctrl->R[k] = R[k]; }
+            if (c->step == 0) {
+                for (int k = 0; k < 4; ++k) { c->R0[k] = R[k]; c->R[k] = R[k]; }
             } else {
-                for (int k = 0; k < 4; ++k) ctrl->R[k] = R[k] / ctrl->R0[k];
+                for (int k = 0; k < 4; ++k) c->R[k] = R[k] / c->R0[k];
             }
         }
     }
     if (threadIdx.x == 0) {
-        const double d0 = (double)dt_src[dt_index];
-        ctrl->dt0 = d0;
-        ctrl->step += 1;
-        ctrl->time_prev = ctrl->time;
-        ctrl->time += d0;
+        const double d0 = (double)dt_src[dt_is_inst ? i : iv.dev0[i]];
+        c->dt0 = d0;
+        c->step += 1;
+        c->time_prev = c->time;
+        c->time += d0;
         for (int k = 0; k < 4; ++k)
-            if (!(ctrl->R[k] == ctrl->R[k])) ctrl->nonfinite = 1;
+            if (!(c->R[k] == c->R[k])) c->nonfinite = 1;
     }
+}
+
+// Conserved sums: per real tile the FP64 partial sum_c A_c q_ck in a fixed
+// order, then one workgroup per instance adds its tiles' partials.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_sums_tile(MeshView<T> m, TileMap tm, const Q4<T>* __restrict__ q,
+                                                      double* __restrict__ blk) {
+    const int t = blockIdx.x;
+    if (t >= tm.n_rt) return;
+    const int c = tm.c0[t] + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < tm.c0[t + 1]) {
+        const Q4<T> qc = ld(q + c);
+        const double A = (double)m.c_area[c];
+        for (int k = 0; k < 4; ++k) v[k] = A * (double)qc.v[k];
+    }
+    block_sum4(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) blk[4 * t + k] = v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sums_inst(InstView iv, const double* __restrict__ blk,
+                                                      double* __restrict__ out) {
+    const int i = blockIdx.x;
+    const int4 r = iv.tiles[i];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = r.x + threadIdx.x; t < r.y; t += kBlock)
+        for (int k = 0; k < 4; ++k) s[k] += blk[4 * t + k];
+    block_sum4(s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) out[4 * i + k] = s[k];
 }
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
@@ -547,32 +593,33 @@ inline int xcd_grid(int n) { return xcd_grid_blocks(nblocks(n)); }
 // launchers
 // ---------------------------------------------------------------------------
 template <class T>
-void launch_dt(hipStream_t s, const MeshView<T>& m, int dt_kind, const PhysParams& pp,
+void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_kind, const PhysParams& pp,
                const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
-    const int nb = xcd_grid(m.n_total);
+    const int nb = xcd_grid_blocks(tm.n_all);
     if (nb == 0) return;
-    if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
-    else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
-    else hipLaunchKernelGGL((k_dt<T, 2>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, blk_min, ctrl);
+    if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else hipLaunchKernelGGL((k_dt<T, 2>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
 }
 
-void launch_dt_min(hipStream_t s, const double* blk_min, int nblk, const void* dt_cells, int dt_is_f32,
-                   int32_t dev0, double* dt_global, const Ctrl* ctrl) {
+void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells, int dt_is_f32,
+                   void* dt_global, const Ctrl* ctrl) {
+    if (iv.n == 0) return;
     if (dt_is_f32)
-        hipLaunchKernelGGL((k_dt_min<float>), dim3(1), dim3(kBlock), 0, s, blk_min, nblk,
-                           (const float*)dt_cells, dev0, (float*)dt_global, ctrl);
+        hipLaunchKernelGGL((k_dt_min<float>), dim3(iv.n), dim3(kBlock), 0, s, iv, blk_min, (const float*)dt_cells,
+                           (float*)dt_global, ctrl);
     else
-        hipLaunchKernelGGL((k_dt_min<double>), dim3(1), dim3(kBlock), 0, s, blk_min, nblk,
-                           (const double*)dt_cells, dev0, dt_global, ctrl);
+        hipLaunchKernelGGL((k_dt_min<double>), dim3(iv.n), dim3(kBlock), 0, s, iv, blk_min, (const double*)dt_cells,
+                           (double*)dt_global, ctrl);
 }
 
 template <class T>
-void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind, bool grad,
-                    bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr, int dt_kind, T* dt,
+void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
+                    bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr, int dt_kind, T* dt,
                     double* blk_min, Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl) {
-    const int nb = xcd_grid(m.n_total);
+    const int nb = xcd_grid_blocks(tm.n_all);
     if (nb == 0) return;
-#define FVH_GL(P, D) hipLaunchKernelGGL((k_gradlim<T, P, D>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)grad, \
+#define FVH_GL(P, D) hipLaunchKernelGGL((k_gradlim<T, P, D>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, (int)grad, \
                                         (int)lim, (int)lin, q, face, bincr, dt, blk_min, dbg_gx, dbg_gy, dbg_lim, ctrl)
 #define FVH_GLD(P)                                  \
     do {                                            \
@@ -591,16 +638,16 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, i
 }
 
 template <class T>
-void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, const PhysParams& pp, bool joined,
-                 bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
-                 const Q4<T>* face, const Q4<T>* bincr, const T* dt, int dt_stride, T alpha, double* res_blk,
+void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles, const PhysParams& pp,
+                 bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
+                 const Q4<T>* face, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
                  Q4<T>* dbg_qt, const Ctrl* ctrl) {
-    if (m.n_real == 0) return;
+    const int nb = xcd_grid_blocks(tm.n_rt);
+    if (nb == 0) return;
     if (tiles) {
-        const int nb = xcd_grid_blocks(tiles->n_tiles);
         const TileView tv = *tiles;
-#define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tv, pp, bcs, qin, \
-                                        q0, qout, face, bincr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
+#define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, tv, pp, bcs, \
+                                        qin, q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
         if (joined) {
             if (lin) FVH_FT(true, true); else FVH_FT(true, false);
         } else {
@@ -609,9 +656,8 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileView* tiles, con
 #undef FVH_FT
         return;
     }
-    const int nb = xcd_grid(m.n_real);
-#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, pp, bcs, qin, q0, \
-                                        qout, face, bincr, dt, dt_stride, alpha, res_blk, dbg_qt, ctrl)
+#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qin, \
+                                        q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
     if (joined) {
         if (lin) FVH_FX(true, true); else FVH_FX(true, false);
     } else {
@@ -628,31 +674,46 @@ void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& p
     hipLaunchKernelGGL((k_bounds_init<T>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q);
 }
 
-void launch_ctrl_begin(hipStream_t s, Ctrl* ctrl, uint32_t max_step, double max_time, double tol, int stop_var) {
-    hipLaunchKernelGGL(k_ctrl_begin, dim3(1), dim3(1), 0, s, ctrl, max_step, max_time, tol, stop_var);
+void launch_ctrl_begin(hipStream_t s, int n_inst, Ctrl* ctrl, uint32_t max_step, double max_time, double tol,
+                       int stop_var) {
+    if (n_inst == 0) return;
+    hipLaunchKernelGGL(k_ctrl_begin, dim3(nblocks(n_inst)), dim3(kBlock), 0, s, n_inst, ctrl, max_step, max_time,
+                       tol, stop_var);
 }
 
-void launch_ctrl_end(hipStream_t s, Ctrl* ctrl, const double* res_blk, int nblk, int need_res,
-                     const void* dt_src, int dt_is_f32, int32_t dt_index) {
+void launch_ctrl_end(hipStream_t s, const InstView& iv, Ctrl* ctrl, const double* res_blk, int need_res,
+                     const void* dt_src, int dt_is_f32, int dt_is_inst) {
+    if (iv.n == 0) return;
     if (dt_is_f32)
-        hipLaunchKernelGGL((k_ctrl_end<float>), dim3(1), dim3(kBlock), 0, s, ctrl, res_blk, nblk, need_res,
-                           (const float*)dt_src, dt_index);
+        hipLaunchKernelGGL((k_ctrl_end<float>), dim3(iv.n), dim3(kBlock), 0, s, iv, ctrl, res_blk, need_res,
+                           (const float*)dt_src, dt_is_inst);
     else
-        hipLaunchKernelGGL((k_ctrl_end<double>), dim3(1), dim3(kBlock), 0, s, ctrl, res_blk, nblk, need_res,
-                           (const double*)dt_src, dt_index);
+        hipLaunchKernelGGL((k_ctrl_end<double>), dim3(iv.n), dim3(kBlock), 0, s, iv, ctrl, res_blk, need_res,
+                           (const double*)dt_src, dt_is_inst);
+}
+
+template <class T>
+void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* q,
+                 double* blk, double* out) {
+    if (tm.n_rt == 0 || iv.n == 0) return;
+    hipLaunchKernelGGL((k_sums_tile<T>), dim3(tm.n_rt), dim3(kBlock), 0, s, m, tm, q, blk);
+    hipLaunchKernelGGL(k_sums_inst, dim3(iv.n), dim3(kBlock), 0, s, iv, (const double*)blk, out);
 }
 
 #define FVH_INST(T)                                                                                          \
-    template void launch_dt<T>(hipStream_t, const MeshView<T>&, int, const PhysParams&, const Q4<T>*, T*,     \
-                               double*, const Ctrl*);                                                        \
-    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const PhysParams&, int, bool, bool,     \
-                                    bool, const Q4<T>*, Q4<T>*, Q4<T>*, int, T*, double*, Q4<T>*, Q4<T>*,    \
-                                    Q4<T>*, const Ctrl*);                                                    \
-    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileView*, const PhysParams&, bool,   \
-                                 bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,  \
-                                 const Q4<T>*, const T*, int, T, double*, Q4<T>*, const Ctrl*);              \
+    template void launch_dt<T>(hipStream_t, const MeshView<T>&, const TileMap&, int, const PhysParams&,        \
+                               const Q4<T>*, T*, double*, const Ctrl*);                                      \
+    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&, int,   \
+                                    bool, bool, bool, const Q4<T>*, Q4<T>*, Q4<T>*, int, T*, double*, Q4<T>*, \
+                                    Q4<T>*, Q4<T>*, const Ctrl*);                                            \
+    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView*,            \
+                                 const PhysParams&, bool, bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, \
+                                 Q4<T>*, const Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,  \
+                                 const Ctrl*);                                                               \
     template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,            \
-                                        const BcParam<T>*, Q4<T>*);
+                                        const BcParam<T>*, Q4<T>*);                                          \
+    template void launch_sums<T>(hipStream_t, const MeshView<T>&, const TileMap&, const InstView&,           \
+                                 const Q4<T>*, double*, double*);
 FVH_INST(double)
 FVH_INST(float)
 

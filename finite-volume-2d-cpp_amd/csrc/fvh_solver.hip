@@ -4,6 +4,16 @@
 // functions it calls, on one GPU.  The host code here only builds the device
 // layout and queues kernels; every per-cell, per-edge and per-step operation
 // runs on the GPU.
+//
+// A solver holds one or more independent instances (meshes with their own
+// boundary tables, step counters, time, residuals and stop flags).  A single
+// mesh is a batch of one.  Device layout of a batch: the real cells of every
+// instance (instance by instance, each in its own locality order), then the
+// ghost cells of every instance; tiles of at most 256 cells never straddle
+// two instances, so every workgroup works for exactly one instance and the
+// per-instance reductions are segmented sums over tile ranges.  Batching
+// changes no arithmetic: each instance's results are bit-identical to a run
+// of that mesh alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,28 +56,46 @@ struct DevBuf {
     }
 };
 
+// One captured RK step, replayed with hipGraphLaunch.
+struct StepGraph {
+    int cur = 0, need = 0, run_mode = 0;
+    fvh_run_options o{};
+    int cur_after = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 }  // namespace
 }  // namespace fvh
 
 struct fvh_solver {
-    fvh_config cfg{};
+    fvh_config cfg{};                     // physics, the same for every instance
+    std::vector<fvh_config> icfg;         // per instance (boundary tables)
     int device = 0;
     hipStream_t stream = nullptr;
     bool f32 = false;
     bool second = false;                  // gradient/limiter/reconstruction pass present
+    int32_t n_inst = 0;
     int32_t n_real = 0, n_total = 0, n_edges = 0, n_he = 0, n_bnd = 0;
-    int32_t dev_ref0 = 0;                 // device position of reference cell 0
-    std::vector<int32_t> pos;             // reference cell -> device position
-    std::vector<int32_t> inv;             // device position -> reference cell
+    std::vector<int32_t> inv;             // device cell -> cell of the concatenated reference-order state
+    // per instance
+    std::vector<int32_t> i_ncells, i_real0, i_nreal, i_ghost0, i_nghost, i_dev0;
+    std::vector<int64_t> i_state0;        // first cell of the instance in the concatenated state
+    std::vector<int4> i_tiles;            // (rt0, rt1, gt0, gt1)
+    // tiles
+    int32_t n_rt = 0, n_all = 0;
+    fvh::DevBuf tile_c0, tile_inst, inst_tiles, inst_dev0;
     // device buffers
     fvh::DevBuf he_off, he_nbr, he_edge, he_twin, he_bc, he_slot;
     fvh::DevBuf e_geo, e_c, c_area, c_inv_area, c_sqrt_area, c_c;
     fvh::DevBuf t_off, t_edge, t_he, t_cell;
-    int32_t n_tiles = 0;
     bool use_tiles = false;
-    fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs;
+    fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
     int cur = 0;                          // index of the current state buffer
+    // graphs
+    bool use_graphs = false;
+    std::vector<fvh::StepGraph> graphs;
     // profiling
     bool profiling = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -76,7 +104,16 @@ struct fvh_solver {
     double k_ms[fvh::K_COUNT] = {};
     double stage_bytes = 0.0;
 
+    void drop_graphs() {
+        for (auto& g : graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+        }
+        graphs.clear();
+    }
     ~fvh_solver() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         for (auto& pe : pending) { event_pool.push_back(pe.second.first); event_pool.push_back(pe.second.second); }
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -106,9 +143,26 @@ MeshView<T> view(fvh_solver* s) {
     return m;
 }
 
+TileMap tile_map(fvh_solver* s) {
+    TileMap t;
+    t.n_rt = s->n_rt;
+    t.n_all = s->n_all;
+    t.c0 = (const int32_t*)s->tile_c0.p;
+    t.inst = (const int32_t*)s->tile_inst.p;
+    return t;
+}
+
+InstView inst_view(fvh_solver* s) {
+    InstView v;
+    v.n = s->n_inst;
+    v.tiles = (const int4*)s->inst_tiles.p;
+    v.dev0 = (const int32_t*)s->inst_dev0.p;
+    return v;
+}
+
 TileView tile_view(fvh_solver* s) {
     TileView t;
-    t.n_tiles = s->n_tiles;
+    t.n_tiles = s->n_rt;
     t.off = (const int32_t*)s->t_off.p;
     t.edge = (const int32_t*)s->t_edge.p;
     t.he = (const int2*)s->t_he.p;
@@ -127,15 +181,20 @@ PhysParams phys(const fvh_config& c) {
     return p;
 }
 
+// kMaxBc slots per instance, instance-major
 template <class T>
-std::vector<BcParam<T>> bc_table(const fvh_config& c) {
-    std::vector<BcParam<T>> t(kMaxBc);
-    for (int k = 0; k < kMaxBc; ++k) {
-        t[k].kind = k < c.n_bc ? c.bc[k].kind : 0;
-        t[k].rho = T(k < c.n_bc ? c.bc[k].rho : 0);
-        t[k].u = T(k < c.n_bc ? c.bc[k].u : 0);
-        t[k].v = T(k < c.n_bc ? c.bc[k].v : 0);
-        t[k].p = T(k < c.n_bc ? c.bc[k].p : 0);
+std::vector<BcParam<T>> bc_table(const std::vector<fvh_config>& cs) {
+    std::vector<BcParam<T>> t(kMaxBc * cs.size());
+    for (size_t i = 0; i < cs.size(); ++i) {
+        const fvh_config& c = cs[i];
+        for (int k = 0; k < kMaxBc; ++k) {
+            BcParam<T>& b = t[i * kMaxBc + k];
+            b.kind = k < c.n_bc ? c.bc[k].kind : 0;
+            b.rho = T(k < c.n_bc ? c.bc[k].rho : 0);
+            b.u = T(k < c.n_bc ? c.bc[k].u : 0);
+            b.v = T(k < c.n_bc ? c.bc[k].v : 0);
+            b.p = T(k < c.n_bc ? c.bc[k].p : 0);
+        }
     }
     return t;
 }
@@ -189,34 +248,37 @@ int32_t upload(DevBuf& b, const void* src, size_t bytes) {
     return FVH_OK;
 }
 
+// Host geometry in device order (doubles; converted to the state type on upload).
+struct HostGeo {
+    std::vector<double> e_nx, e_ny, e_len, e_gg, e_cx, e_cy;   // per device edge
+    std::vector<double> c_area, c_cx, c_cy;                    // per device cell
+};
+
 template <class T>
-int32_t upload_geometry(fvh_solver* s, const fvh_mesh& m, const std::vector<int32_t>& dev_edge_ref,
-                        const std::vector<double>& gg) {
+int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
     const int32_t E = s->n_edges, N = s->n_total;
     std::vector<Q4<T>> geo(E);
     std::vector<G2<T>> ec(E);
     for (int32_t d = 0; d < E; ++d) {
-        const int32_t e = dev_edge_ref[d];
-        geo[d].v[0] = T(m.e_nx[e]);
-        geo[d].v[1] = T(m.e_ny[e]);
-        geo[d].v[2] = T(m.e_len[e]);
-        geo[d].v[3] = T(gg[e]);
-        ec[d].v[0] = T(m.e_cx[e]);
-        ec[d].v[1] = T(m.e_cy[e]);
+        geo[d].v[0] = T(h.e_nx[d]);
+        geo[d].v[1] = T(h.e_ny[d]);
+        geo[d].v[2] = T(h.e_len[d]);
+        geo[d].v[3] = T(h.e_gg[d]);
+        ec[d].v[0] = T(h.e_cx[d]);
+        ec[d].v[1] = T(h.e_cy[d]);
     }
     std::vector<T> area(N), ia(N), sa(N);
     std::vector<G2<T>> cc(N);
     for (int32_t d = 0; d < N; ++d) {
-        const int32_t r = s->inv[d];
-        const double A = m.c_area[r];
+        const double A = h.c_area[d];
         area[d] = T(A);
         ia[d] = T(1. / A);          // invA of explicit.cpp:158
         sa[d] = T(std::sqrt(A));    // sqrt_area of explicit.cpp:218
-        cc[d].v[0] = T(m.c_cx[r]);
-        cc[d].v[1] = T(m.c_cy[r]);
+        cc[d].v[0] = T(h.c_cx[d]);
+        cc[d].v[1] = T(h.c_cy[d]);
     }
     int32_t rc;
-    const std::vector<BcParam<T>> bt = bc_table<T>(s->cfg);
+    const std::vector<BcParam<T>> bt = bc_table<T>(s->icfg);
     if ((rc = upload(s->bcs, bt.data(), bt.size() * sizeof(BcParam<T>)))) return rc;
     if ((rc = upload(s->e_geo, geo.data(), geo.size() * sizeof(Q4<T>)))) return rc;
     if ((rc = upload(s->e_c, ec.data(), ec.size() * sizeof(G2<T>)))) return rc;
@@ -226,15 +288,16 @@ int32_t upload_geometry(fvh_solver* s, const fvh_mesh& m, const std::vector<int3
     if ((rc = upload(s->c_c, cc.data(), N * sizeof(G2<T>)))) return rc;
     for (int k = 0; k < 3; ++k)
         if (s->q[k].alloc((size_t)N * sizeof(Q4<T>))) { set_error("hipMalloc failed"); return FVH_E_HIP; }
-    if (s->dt.alloc((size_t)N * sizeof(T)) || s->dt_global.alloc(8) ||
+    if (s->dt.alloc((size_t)N * sizeof(T)) || s->dt_global.alloc((size_t)s->n_inst * 8) ||
         s->face.alloc((size_t)(s->second ? s->n_he : 1) * sizeof(Q4<T>)) ||
         s->bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<T>)) ||
-        s->blk_min.alloc((size_t)((N + kBlock - 1) / kBlock + 1) * 8) ||
-        s->res_blk.alloc((size_t)((s->n_real + kBlock - 1) / kBlock + 1) * 32) || s->ctrl.alloc(sizeof(Ctrl))) {
+        s->blk_min.alloc((size_t)(s->n_all + 1) * 8) || s->res_blk.alloc((size_t)(s->n_rt + 1) * 32) ||
+        s->sums_blk.alloc((size_t)(s->n_rt + 1) * 32) || s->sums.alloc((size_t)s->n_inst * 32) ||
+        s->ctrl.alloc((size_t)s->n_inst * sizeof(Ctrl))) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
-    FVH_HIP(hipMemset(s->ctrl.p, 0, sizeof(Ctrl)));
+    FVH_HIP(hipMemset(s->ctrl.p, 0, s->ctrl.bytes));
     FVH_HIP(hipMemset(s->face.p, 0, s->face.bytes));
     FVH_HIP(hipMemset(s->bincr.p, 0, s->bincr.bytes));
     return FVH_OK;
@@ -249,6 +312,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
                      Q4<T>* const* dbg = nullptr, const double* alpha_override = nullptr) {
     const fvh_config& c = s->cfg;
     const MeshView<T> m = view<T>(s);
+    const TileMap tm = tile_map(s);
+    const InstView iv = inst_view(s);
     const TileView tv = tile_view(s);
     const PhysParams pp = phys(c);
     const BcParam<T>* bcs = (const BcParam<T>*)s->bcs.p;
@@ -259,21 +324,20 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     hipStream_t st = s->stream;
     if (run_mode) {
         Bracket b(s, K_BEGIN);
-        launch_ctrl_begin(st, ctrl, o->max_step, o->max_time, o->tolerance, o->stop_var);
+        launch_ctrl_begin(st, s->n_inst, ctrl, o->max_step, o->max_time, o->tolerance, o->stop_var);
     }
     Q4<T>* q0 = (Q4<T>*)s->q[s->cur].p;
     Q4<T>* qa = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
     Q4<T>* qb = (Q4<T>*)s->q[(s->cur + 2) % 3].p;
     T* dt = (T*)s->dt.p;
+    T* dtg = (T*)s->dt_global.p;
     double* blk = (double*)s->blk_min.p;
     Q4<T>* face = (Q4<T>*)s->face.p;
     Q4<T>* bincr = (Q4<T>*)s->bincr.p;
     if (!s->second) {
         Bracket b(s, K_DT);
-        launch_dt<T>(st, m, c.dt_kind, pp, q0, dt, blk, cc);
+        launch_dt<T>(st, m, tm, c.dt_kind, pp, q0, dt, blk, cc);
     }
-    const T* dtp = gdt ? (const T*)s->dt_global.p : dt;
-    const int dt_stride = gdt ? 0 : 1;
     const Q4<T>* qin = q0;
     Q4<T>* out = qa;
     const int n_stage = alpha_override ? 1 : c.n_alpha;
@@ -281,19 +345,18 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         if (s->second) {
             // the first stage's pass also computes the time step from q0
             Bracket b(s, K_GRADLIM);
-            launch_gradlim<T>(st, m, pp, c.phi, grad, lim, lin, qin, face, bincr, k == 0 ? c.dt_kind : -1, dt,
+            launch_gradlim<T>(st, m, tm, pp, c.phi, grad, lim, lin, qin, face, bincr, k == 0 ? c.dt_kind : -1, dt,
                               blk, dbg ? dbg[0] : nullptr, dbg ? dbg[1] : nullptr, dbg ? dbg[2] : nullptr, cc);
         }
         if (k == 0 && gdt) {
             Bracket b(s, K_DTMIN);
-            launch_dt_min(st, blk, (s->n_total + kBlock - 1) / kBlock, dt, s->f32, s->dev_ref0,
-                          (double*)s->dt_global.p, cc);
+            launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
         }
         const bool last = k == n_stage - 1;
         {
             Bracket b(s, K_FLUX);
-            launch_flux<T>(st, m, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs, qin,
-                           q0, out, face, bincr, dtp, dt_stride,
+            launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
+                           qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr,
                            T(alpha_override ? *alpha_override : c.alpha[k]),
                            (last && need_res) ? (double*)s->res_blk.p : nullptr, dbg ? dbg[3] : nullptr, cc);
         }
@@ -302,8 +365,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     }
     if (!dbg) {
         Bracket b(s, K_END);
-        launch_ctrl_end(st, ctrl, (const double*)s->res_blk.p, (s->n_real + kBlock - 1) / kBlock, need_res,
-                        gdt ? s->dt_global.p : s->dt.p, s->f32, gdt ? 0 : s->dev_ref0);
+        launch_ctrl_end(st, iv, ctrl, (const double*)s->res_blk.p, need_res, gdt ? s->dt_global.p : s->dt.p,
+                        s->f32, gdt ? 1 : 0);
     }
     FVH_HIP(hipGetLastError());
     // the last stage wrote qin; it becomes the current state
@@ -311,13 +374,62 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     return FVH_OK;
 }
 
+bool same_options(const fvh_run_options& a, const fvh_run_options& b) {
+    return a.max_time == b.max_time && a.max_step == b.max_step && a.tolerance == b.tolerance &&
+           a.stop_var == b.stop_var;
+}
+
+// One step, replayed from a captured graph when graphs are on (a graph per
+// state buffer, residual flag and run options; kernels take their buffers
+// and options by value, so a replay is the same launch sequence).
+template <class T>
+int32_t step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res) {
+    if (!s->use_graphs || s->profiling) return enqueue_step<T>(s, run_mode, o, need_res);
+    fvh_run_options key{};
+    if (run_mode) key = *o;
+    for (auto& g : s->graphs) {
+        if (g.cur == s->cur && g.need == need_res && g.run_mode == (int)run_mode && same_options(g.o, key)) {
+            FVH_HIP(hipGraphLaunch(g.exec, s->stream));
+            s->cur = g.cur_after;
+            return FVH_OK;
+        }
+    }
+    if (s->graphs.size() >= 24) s->drop_graphs();
+    StepGraph g;
+    g.cur = s->cur;
+    g.need = need_res;
+    g.run_mode = run_mode;
+    g.o = key;
+    FVH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int32_t rc = enqueue_step<T>(s, run_mode, o, need_res);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        s->cur = g.cur;
+        return rc;
+    }
+    if (e != hipSuccess) {
+        s->cur = g.cur;
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return FVH_E_HIP;
+    }
+    g.cur_after = s->cur;
+    g.graph = graph;
+    FVH_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    s->graphs.push_back(g);
+    FVH_HIP(hipGraphLaunch(g.exec, s->stream));
+    return FVH_OK;
+}
+
 template <class T>
 int32_t set_state_t(fvh_solver* s, const double* q) {
     std::vector<Q4<T>> h(s->n_total);
     for (int32_t d = 0; d < s->n_total; ++d) {
-        const int32_t r = s->inv[d];
-        for (int k = 0; k < 4; ++k) h[d].v[k] = T(q[4 * (int64_t)r + k]);
+        const int64_t r = s->inv[d];
+        for (int k = 0; k < 4; ++k) h[d].v[k] = T(q[4 * r + k]);
     }
+    FVH_HIP(hipStreamSynchronize(s->stream));
     for (int b = 0; b < 3; ++b)
         FVH_HIP(hipMemcpy(s->q[b].p, h.data(), h.size() * sizeof(Q4<T>), hipMemcpyHostToDevice));
     return FVH_OK;
@@ -329,8 +441,8 @@ int32_t get_q4(fvh_solver* s, const void* dev, double* out, int32_t count) {
     FVH_HIP(hipStreamSynchronize(s->stream));
     FVH_HIP(hipMemcpy(h.data(), dev, h.size() * sizeof(Q4<T>), hipMemcpyDeviceToHost));
     for (int32_t d = 0; d < count; ++d) {
-        const int32_t r = s->inv[d];
-        for (int k = 0; k < 4; ++k) out[4 * (int64_t)r + k] = (double)h[d].v[k];
+        const int64_t r = s->inv[d];
+        for (int k = 0; k < 4; ++k) out[4 * r + k] = (double)h[d].v[k];
     }
     return FVH_OK;
 }
@@ -376,37 +488,57 @@ int32_t apply_bounds_t(fvh_solver* s) {
                               (const BcParam<T>*)s->bcs.p, (Q4<T>*)s->q[s->cur].p);
     }
     FVH_HIP(hipGetLastError());
-    Ctrl z;
-    std::memset(&z, 0, sizeof z);
-    for (int k = 0; k < 4; ++k) z.R[k] = 1.0;
-    FVH_HIP(hipMemcpyAsync(s->ctrl.p, &z, sizeof z, hipMemcpyHostToDevice, s->stream));
+    std::vector<Ctrl> z(s->n_inst);
+    std::memset(z.data(), 0, z.size() * sizeof(Ctrl));
+    for (auto& c0 : z)
+        for (int k = 0; k < 4; ++k) c0.R[k] = 1.0;
+    FVH_HIP(hipMemcpyAsync(s->ctrl.p, z.data(), z.size() * sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
     FVH_HIP(hipStreamSynchronize(s->stream));
     return FVH_OK;
 }
 
-int32_t read_status(fvh_solver* s, fvh_status* st) {
-    Ctrl h;
+int32_t read_ctrl(fvh_solver* s, std::vector<Ctrl>& h) {
+    h.resize(s->n_inst);
     FVH_HIP(hipStreamSynchronize(s->stream));
-    FVH_HIP(hipMemcpy(&h, s->ctrl.p, sizeof h, hipMemcpyDeviceToHost));
+    FVH_HIP(hipMemcpy(h.data(), s->ctrl.p, h.size() * sizeof(Ctrl), hipMemcpyDeviceToHost));
+    return FVH_OK;
+}
+
+void to_status(const Ctrl& h, fvh_status* st) {
     st->step = h.step;
     st->stopped = h.stop;
     st->time = h.time;
     st->time_prev = h.time_prev;
     for (int k = 0; k < 4; ++k) { st->R[k] = h.R[k]; st->R0[k] = h.R0[k]; }
+}
+
+// Status of the whole solver: instance 0's counters and residuals, with
+// `stopped` set once every instance has stopped and `step` the largest count.
+int32_t read_status(fvh_solver* s, fvh_status* st) {
+    std::vector<Ctrl> h;
+    int32_t rc = read_ctrl(s, h);
+    if (rc) return rc;
+    to_status(h[0], st);
+    uint32_t all = 1, mx = 0;
+    for (const Ctrl& c : h) { all &= c.stop ? 1u : 0u; mx = std::max(mx, c.step); }
+    st->stopped = all;
+    st->step = mx;
     return FVH_OK;
 }
 
 // Executes up to n more steps of run() from the current status (steps are
-// queued in growing chunks; kernels queued after the stop are no-ops).
+// queued in growing chunks; kernels queued after an instance's stop are
+// no-ops for it).  Afterwards every instance's state is in buffer s->cur.
 template <class T>
 int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_status* st) {
     int32_t rc;
+    std::vector<Ctrl> before, after;
+    if ((rc = read_ctrl(s, before))) return rc;
     fvh_status now;
     if ((rc = read_status(s, &now))) return rc;
     const uint32_t print = o->print_interval ? o->print_interval : 1;
     const bool every = o->tolerance > 1.01e-16;
     uint64_t step = now.step;
-    const uint64_t first = step;
     const int cur0 = s->cur;
     // each executed step moves the state forward by one or two buffers
     const int delta = (s->cfg.n_alpha % 2) ? 1 : 2;
@@ -415,13 +547,31 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
         const uint64_t k_n = std::min<uint64_t>(chunk, left);
         for (uint64_t k = 0; k < k_n; ++k, ++step) {
             const int need = (step == 0) || (step % print == 0) || every;
-            if ((rc = enqueue_step<T>(s, true, o, need))) return rc;
+            if ((rc = fvh::step<T>(s, true, o, need))) return rc;
         }
         left -= k_n;
         if ((rc = read_status(s, &now))) return rc;
         if (chunk < 256) chunk *= 2;
     }
-    s->cur = (int)((cur0 + (uint64_t)(now.step - first) * delta) % 3);
+    if ((rc = read_ctrl(s, after))) return rc;
+    uint64_t most = 0;
+    for (int i = 0; i < s->n_inst; ++i) most = std::max<uint64_t>(most, after[i].step - before[i].step);
+    s->cur = (int)((cur0 + most * delta) % 3);
+    // an instance that stopped early holds its state in another buffer
+    for (int i = 0; i < s->n_inst; ++i) {
+        const uint64_t done = after[i].step - before[i].step;
+        const int b = (int)((cur0 + done * delta) % 3);
+        if (b == s->cur) continue;
+        const size_t qs = sizeof(Q4<T>);
+        char* dst = (char*)s->q[s->cur].p;
+        const char* src = (const char*)s->q[b].p;
+        FVH_HIP(hipMemcpyAsync(dst + (size_t)s->i_real0[i] * qs, src + (size_t)s->i_real0[i] * qs,
+                               (size_t)s->i_nreal[i] * qs, hipMemcpyDeviceToDevice, s->stream));
+        if (s->i_nghost[i])
+            FVH_HIP(hipMemcpyAsync(dst + (size_t)s->i_ghost0[i] * qs, src + (size_t)s->i_ghost0[i] * qs,
+                                   (size_t)s->i_nghost[i] * qs, hipMemcpyDeviceToDevice, s->stream));
+    }
+    FVH_HIP(hipStreamSynchronize(s->stream));
     *st = now;
     return drain_events(s);
 }
@@ -435,9 +585,9 @@ int32_t run_t(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
 
 // Tile edge lists for the tiled flux pass.  Returns false when a tile touches
 // more edges than its LDS holds.
-bool build_tiles(fvh_solver* s, const std::vector<int32_t>& off, const std::vector<uint32_t>& he_edge,
-                 std::vector<uint16_t>& he_slot, std::vector<int32_t>& t_off, std::vector<int32_t>& t_edge,
-                 std::vector<int32_t>& t_he, std::vector<int32_t>& t_cell) {
+bool build_tiles(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::vector<int32_t>& off,
+                 const std::vector<uint32_t>& he_edge, std::vector<uint16_t>& he_slot, std::vector<int32_t>& t_off,
+                 std::vector<int32_t>& t_edge, std::vector<int32_t>& t_he, std::vector<int32_t>& t_cell) {
     const int32_t E = s->n_edges;
     std::vector<int32_t> eh(2 * (size_t)E, -1), ecell(2 * (size_t)E, -1);
     for (int32_t d = 0; d < s->n_total; ++d)
@@ -448,12 +598,11 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& off, const std::vect
             eh[2 * (size_t)e + sd] = h;
             ecell[2 * (size_t)e + sd] = d;
         }
-    s->n_tiles = (s->n_real + kBlock - 1) / kBlock;
-    t_off.assign(s->n_tiles + 1, 0);
+    t_off.assign(s->n_rt + 1, 0);
     std::vector<int32_t> local;
     std::vector<int32_t> slot_of(E, -1);
-    for (int32_t t = 0; t < s->n_tiles; ++t) {
-        const int32_t c0 = t * kBlock, c1 = std::min(s->n_real, c0 + kBlock);
+    for (int32_t t = 0; t < s->n_rt; ++t) {
+        const int32_t c0 = tile_c0[t], c1 = tile_c0[t + 1];
         local.clear();
         for (int32_t d = c0; d < c1; ++d)
             for (int32_t h = off[d]; h < off[d + 1]; ++h) local.push_back((int32_t)(he_edge[h] & kEdgeMask));
@@ -474,6 +623,284 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& off, const std::vect
         t_off[t + 1] = t_off[t] + (int32_t)local.size();
     }
     return true;
+}
+
+int32_t check_config(const fvh_config* c) {
+    if (c->n_alpha < 1 || c->n_alpha > FVH_MAX_ALPHA || c->n_bc < 0 || c->n_bc > FVH_MAX_BC) {
+        set_error("config out of range (alpha or bc count)");
+        return FVH_E_ARG;
+    }
+    if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
+    if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2 ||
+        c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 2) {
+        set_error("unknown flux, limiter, time-step, order or flux-mode kind");
+        return FVH_E_ARG;
+    }
+    for (int k = 0; k < c->n_bc; ++k)
+        if (c->bc[k].kind < 0 || c->bc[k].kind > 3) { set_error("unknown boundary kind"); return FVH_E_ARG; }
+    return FVH_OK;
+}
+
+// Everything but the boundary table must agree across a batch.
+bool same_physics(const fvh_config& a, const fvh_config& b) {
+    if (a.precision != b.precision || a.flux != b.flux || a.gamma != b.gamma || a.phi != b.phi ||
+        a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
+        a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
+        a.flux_mode != b.flux_mode)
+        return false;
+    for (int k = 0; k < a.n_alpha; ++k)
+        if (a.alpha[k] != b.alpha[k]) return false;
+    return true;
+}
+
+// Build the device layout of n instances and upload it.
+int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_solver** out) {
+    if (n < 1 || !ms || !cs || !out) { set_error("null argument or empty batch"); return FVH_E_ARG; }
+    int32_t rc;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!ms[i]) { set_error("null mesh"); return FVH_E_ARG; }
+        if ((rc = check_config(&cs[i]))) return rc;
+        if (!same_physics(cs[0], cs[i])) {
+            set_error("batch instance " + std::to_string(i) + ": configs may differ only in their boundary tables");
+            return FVH_E_ARG;
+        }
+    }
+    std::unique_ptr<fvh_solver> s(new fvh_solver());
+    const fvh_config& c = cs[0];
+    s->cfg = c;
+    s->icfg.assign(cs, cs + n);
+    s->n_inst = n;
+    s->f32 = c.precision == FVH_F32;
+    s->second = (c.flags & (FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR)) != 0;
+    // physical name -> BC slot of each instance (mesh.cpp:662-667)
+    std::vector<std::vector<int32_t>> slot_of_name(n);
+    int64_t tot_real = 0, tot_cells = 0, tot_edges = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const fvh_mesh& m = *ms[i];
+        slot_of_name[i].assign(m.names.size(), -1);
+        for (size_t t = 0; t < m.names.size(); ++t)
+            for (int k = 0; k < cs[i].n_bc; ++k)
+                if (m.names[t] == std::string(cs[i].bc[k].name)) slot_of_name[i][t] = k;
+        for (size_t b = 0; b < m.bnd_tag.size(); ++b) {
+            const int32_t t = m.bnd_tag[b];
+            if (slot_of_name[i][t] < 0) {
+                set_error("Physical name " + m.names[t] + " of tag " + std::to_string(t) + " not in mesh");
+                return FVH_E_BC;
+            }
+        }
+        tot_real += m.n_real;
+        tot_cells += m.n_cells;
+        tot_edges += m.n_edges();
+    }
+    if (tot_edges >= (int64_t)kEdgeMask || tot_cells >= (int64_t)INT32_MAX / 4) {
+        set_error("too many edges or cells");
+        return FVH_E_UNSUPPORTED;
+    }
+    if (hipGetDevice(&s->device) != hipSuccess) { set_error("no HIP device"); return FVH_E_HIP; }
+    FVH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->n_real = (int32_t)tot_real;
+    s->n_total = (int32_t)tot_cells;
+    s->n_edges = (int32_t)tot_edges;
+    const int32_t N = s->n_total, E = s->n_edges;
+    s->inv.assign(N, 0);
+    s->i_ncells.resize(n); s->i_real0.resize(n); s->i_nreal.resize(n); s->i_ghost0.resize(n);
+    s->i_nghost.resize(n); s->i_dev0.resize(n); s->i_state0.resize(n); s->i_tiles.resize(n);
+    // global ids: edge_base[i] + local edge; device cell of (instance, local reference cell)
+    std::vector<int32_t> edge_base(n), cell_dev;     // cell_dev over the concatenated state
+    cell_dev.assign(N, 0);
+    std::vector<int32_t> tile_c0, tile_inst;
+    {
+        int32_t r0 = 0, g0 = s->n_real, eb = 0;
+        int64_t st0 = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const fvh_mesh& m = *ms[i];
+            std::vector<int32_t> order;
+            if (c.renumber == FVH_ORDER_RCM) order = rcm_order(m);
+            else if (c.renumber == FVH_ORDER_TILES) order = rcb_order(m, kBlock);
+            else { order.resize(m.n_real); for (int32_t k = 0; k < m.n_real; ++k) order[k] = k; }
+            s->i_ncells[i] = m.n_cells;
+            s->i_real0[i] = r0;
+            s->i_nreal[i] = m.n_real;
+            s->i_ghost0[i] = g0;
+            s->i_nghost[i] = m.n_cells - m.n_real;
+            s->i_state0[i] = st0;
+            for (int32_t k = 0; k < m.n_real; ++k) {
+                cell_dev[st0 + order[k]] = r0 + k;
+                s->inv[r0 + k] = (int32_t)(st0 + order[k]);
+            }
+            for (int32_t g = m.n_real; g < m.n_cells; ++g) {
+                const int32_t d = g0 + (g - m.n_real);
+                cell_dev[st0 + g] = d;
+                s->inv[d] = (int32_t)(st0 + g);
+            }
+            s->i_dev0[i] = m.n_cells > 0 ? cell_dev[st0] : 0;
+            edge_base[i] = eb;
+            r0 += m.n_real;
+            g0 += m.n_cells - m.n_real;
+            st0 += m.n_cells;
+            eb += m.n_edges();
+        }
+        // real tiles, instance by instance, then ghost tiles
+        for (int32_t i = 0; i < n; ++i) {
+            s->i_tiles[i].x = (int32_t)tile_inst.size();
+            for (int32_t k = 0; k < s->i_nreal[i]; k += kBlock) {
+                tile_c0.push_back(s->i_real0[i] + k);
+                tile_inst.push_back(i);
+            }
+            s->i_tiles[i].y = (int32_t)tile_inst.size();
+        }
+        s->n_rt = (int32_t)tile_inst.size();
+        for (int32_t i = 0; i < n; ++i) {
+            s->i_tiles[i].z = (int32_t)tile_inst.size();
+            for (int32_t k = 0; k < s->i_nghost[i]; k += kBlock) {
+                tile_c0.push_back(s->i_ghost0[i] + k);
+                tile_inst.push_back(i);
+            }
+            s->i_tiles[i].w = (int32_t)tile_inst.size();
+        }
+        s->n_all = (int32_t)tile_inst.size();
+        // tile t spans [c0[t], c0[t+1]): the next tile starts where this one
+        // ends because instances are packed back to back within the real and
+        // the ghost ranges
+        std::vector<int32_t> ends(s->n_all);
+        for (int32_t t = 0; t < s->n_all; ++t) {
+            const int32_t i = tile_inst[t];
+            const bool real = t < s->n_rt;
+            const int32_t lim = real ? s->i_real0[i] + s->i_nreal[i] : s->i_ghost0[i] + s->i_nghost[i];
+            ends[t] = std::min(tile_c0[t] + kBlock, lim);
+        }
+        for (int32_t t = 0; t + 1 < s->n_all; ++t)
+            if (ends[t] != tile_c0[t + 1]) { set_error("internal: tile layout"); return FVH_E_UNSUPPORTED; }
+        tile_c0.push_back(s->n_all ? ends[s->n_all - 1] : 0);
+    }
+    // half-edges in CSR by device cell, each list sorted by (reference edge, side)
+    std::vector<int32_t> off(N + 1, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        const fvh_mesh& m = *ms[i];
+        const int64_t st0 = s->i_state0[i];
+        for (int32_t e = 0; e < m.n_edges(); ++e)
+            for (int sd = 0; sd < 2; ++sd) off[cell_dev[st0 + m.edge_cells[2 * (int64_t)e + sd]] + 1] += 1;
+    }
+    for (int32_t d = 0; d < N; ++d) off[d + 1] += off[d];
+    s->n_he = off[N];
+    std::vector<int32_t> cursor(off.begin(), off.end() - 1);
+    std::vector<int32_t> he_nbr(s->n_he), he_twin(s->n_he), he_bc(s->n_he, -1), he_ref(s->n_he);
+    std::vector<uint32_t> he_side(s->n_he);
+    s->n_bnd = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const fvh_mesh& m = *ms[i];
+        const int64_t st0 = s->i_state0[i];
+        std::vector<int32_t> bc_of_edge(m.n_edges(), -1);
+        for (size_t b = 0; b < m.bnd_edge.size(); ++b)
+            bc_of_edge[m.bnd_edge[b]] = i * kMaxBc + slot_of_name[i][m.bnd_tag[b]];
+        s->n_bnd += (int32_t)m.bnd_edge.size();
+        for (int32_t e = 0; e < m.n_edges(); ++e) {
+            const int32_t d0 = cell_dev[st0 + m.edge_cells[2 * (int64_t)e]];
+            const int32_t d1 = cell_dev[st0 + m.edge_cells[2 * (int64_t)e + 1]];
+            const int32_t h0 = cursor[d0]++;
+            const int32_t h1 = cursor[d1]++;
+            const int32_t ge = edge_base[i] + e;
+            he_nbr[h0] = d1; he_nbr[h1] = d0;
+            he_twin[h0] = h1; he_twin[h1] = h0;
+            he_ref[h0] = ge; he_ref[h1] = ge;
+            he_side[h0] = 0; he_side[h1] = 1;
+            if (bc_of_edge[e] >= 0) he_bc[h0] = bc_of_edge[e];
+        }
+    }
+    // device edge numbering: first use in half-edge order; host geometry in device order
+    HostGeo hg;
+    hg.e_nx.resize(E); hg.e_ny.resize(E); hg.e_len.resize(E); hg.e_gg.resize(E); hg.e_cx.resize(E); hg.e_cy.resize(E);
+    hg.c_area.resize(N); hg.c_cx.resize(N); hg.c_cy.resize(N);
+    std::vector<int32_t> edge_inst(E);
+    for (int32_t i = 0; i < n; ++i)
+        for (int32_t e = 0; e < ms[i]->n_edges(); ++e) edge_inst[edge_base[i] + e] = i;
+    std::vector<int32_t> dev_of_edge(E, -1);
+    int32_t n_dev_edges = 0;
+    std::vector<uint32_t> he_edge(s->n_he);
+    for (int32_t h = 0; h < s->n_he; ++h) {
+        const int32_t ge = he_ref[h];
+        if (dev_of_edge[ge] < 0) {
+            const int32_t d = n_dev_edges++;
+            dev_of_edge[ge] = d;
+            const int32_t i = edge_inst[ge];
+            const fvh_mesh& m = *ms[i];
+            const int32_t e = ge - edge_base[i];
+            hg.e_nx[d] = m.e_nx[e];
+            hg.e_ny[d] = m.e_ny[e];
+            hg.e_len[d] = m.e_len[e];
+            hg.e_cx[d] = m.e_cx[e];
+            hg.e_cy[d] = m.e_cy[e];
+            // Green-Gauss weight, as calc_gradients computes it (explicit.cpp:132-140)
+            const int32_t ci = m.edge_cells[2 * (int64_t)e], cj = m.edge_cells[2 * (int64_t)e + 1];
+            const double dxif = m.e_cx[e] - m.c_cx[ci];
+            const double dyif = m.e_cy[e] - m.c_cy[ci];
+            const double dif = std::sqrt(dxif * dxif + dyif * dyif);
+            const double dxij = m.c_cx[ci] - m.c_cx[cj];
+            const double dyij = m.c_cy[ci] - m.c_cy[cj];
+            const double dij = std::sqrt(dxij * dxij + dyij * dyij);
+            hg.e_gg[d] = dif / dij;
+        }
+        uint32_t w = (uint32_t)dev_of_edge[ge];
+        if (he_side[h]) w |= kSideBit;
+        if (he_bc[h] >= 0) w |= kBcBit;
+        he_edge[h] = w;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        const fvh_mesh& m = *ms[i];
+        const int64_t st0 = s->i_state0[i];
+        for (int32_t r = 0; r < m.n_cells; ++r) {
+            const int32_t d = cell_dev[st0 + r];
+            hg.c_area[d] = m.c_area[r];
+            hg.c_cx[d] = m.c_cx[r];
+            hg.c_cy[d] = m.c_cy[r];
+        }
+    }
+    // tiles for the tiled flux pass
+    std::vector<uint16_t> he_slot(s->n_he, 0);
+    std::vector<int32_t> t_off, t_edge, t_he, t_cell;
+    s->use_tiles = false;
+    if (c.flux_mode != FVH_FLUX_MODE_CELL && s->n_real > 0)
+        s->use_tiles = build_tiles(s.get(), tile_c0, off, he_edge, he_slot, t_off, t_edge, t_he, t_cell);
+    if (c.flux_mode == FVH_FLUX_MODE_TILE && !s->use_tiles && s->n_real > 0) {
+        set_error("a tile touches more than 1024 edges; use the per-cell flux mode");
+        return FVH_E_UNSUPPORTED;
+    }
+    if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
+    if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
+    if ((rc = upload(s->he_edge, he_edge.data(), he_edge.size() * 4))) return rc;
+    if ((rc = upload(s->he_twin, he_twin.data(), he_twin.size() * 4))) return rc;
+    if ((rc = upload(s->he_bc, he_bc.data(), he_bc.size() * 4))) return rc;
+    if ((rc = upload(s->he_slot, he_slot.data(), he_slot.size() * 2))) return rc;
+    if ((rc = upload(s->tile_c0, tile_c0.data(), tile_c0.size() * 4))) return rc;
+    if ((rc = upload(s->tile_inst, tile_inst.data(), tile_inst.size() * 4))) return rc;
+    if ((rc = upload(s->inst_tiles, s->i_tiles.data(), s->i_tiles.size() * sizeof(int4)))) return rc;
+    if ((rc = upload(s->inst_dev0, s->i_dev0.data(), s->i_dev0.size() * 4))) return rc;
+    if (s->use_tiles) {
+        if ((rc = upload(s->t_off, t_off.data(), t_off.size() * 4))) return rc;
+        if ((rc = upload(s->t_edge, t_edge.data(), t_edge.size() * 4))) return rc;
+        if ((rc = upload(s->t_he, t_he.data(), t_he.size() * 4))) return rc;
+        if ((rc = upload(s->t_cell, t_cell.data(), t_cell.size() * 4))) return rc;
+    }
+    rc = s->f32 ? upload_geometry<float>(s.get(), hg) : upload_geometry<double>(s.get(), hg);
+    if (rc) return rc;
+    // algorithmic bytes per stage (state s, geometry g bytes)
+    const double sb = s->f32 ? 4.0 : 8.0, gb = sb, V = 4.0;
+    const double C = s->n_real, B = s->n_bnd;
+    double bytes = C * (3 * V * sb + sb + gb) + E * (3 * gb + 8) + B * (2 * V * sb + 2 * gb + 4);
+    if (s->second) bytes += C * (7 * V * sb + 2 * gb) + E * (2 * gb);
+    s->stage_bytes = bytes;
+    *out = s.release();
+    return FVH_OK;
+}
+
+template <class T>
+int32_t sums_t(fvh_solver* s, double* out) {
+    launch_sums<T>(s->stream, view<T>(s), tile_map(s), inst_view(s), (const Q4<T>*)s->q[s->cur].p,
+                   (double*)s->sums_blk.p, (double*)s->sums.p);
+    FVH_HIP(hipGetLastError());
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    FVH_HIP(hipMemcpy(out, s->sums.p, (size_t)s->n_inst * 32, hipMemcpyDeviceToHost));
+    return FVH_OK;
 }
 
 }  // namespace
@@ -508,131 +935,42 @@ void fvh_config_default(fvh_config* c) {
 
 int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** out) {
     if (!m || !c || !out) { set_error("null argument"); return FVH_E_ARG; }
-    if (c->n_alpha < 1 || c->n_alpha > FVH_MAX_ALPHA || c->n_bc < 0 || c->n_bc > FVH_MAX_BC) {
-        set_error("config out of range (alpha or bc count)");
-        return FVH_E_ARG;
-    }
-    if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
-    if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2 ||
-        c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 2) {
-        set_error("unknown flux, limiter, time-step, order or flux-mode kind");
-        return FVH_E_ARG;
-    }
-    for (int k = 0; k < c->n_bc; ++k)
-        if (c->bc[k].kind < 0 || c->bc[k].kind > 3) { set_error("unknown boundary kind"); return FVH_E_ARG; }
-    // physical name -> BC slot (mesh.cpp:662-667)
-    std::vector<int32_t> slot_of_name(m->names.size(), -1);
-    for (size_t t = 0; t < m->names.size(); ++t) {
-        for (int k = 0; k < c->n_bc; ++k)
-            if (m->names[t] == std::string(c->bc[k].name)) slot_of_name[t] = k;
-    }
-    for (size_t b = 0; b < m->bnd_tag.size(); ++b) {
-        const int32_t t = m->bnd_tag[b];
-        if (slot_of_name[t] < 0) {
-            set_error("Physical name " + m->names[t] + " of tag " + std::to_string(t) + " not in mesh");
-            return FVH_E_BC;
-        }
-    }
-    if (m->n_edges() >= (int32_t)kEdgeMask) { set_error("too many edges"); return FVH_E_UNSUPPORTED; }
-    std::unique_ptr<fvh_solver> s(new fvh_solver());
-    s->cfg = *c;
-    s->f32 = c->precision == FVH_F32;
-    s->second = (c->flags & (FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR)) != 0;
-    if (hipGetDevice(&s->device) != hipSuccess) { set_error("no HIP device"); return FVH_E_HIP; }
-    FVH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    s->n_real = m->n_real;
-    s->n_total = m->n_cells;
-    s->n_edges = m->n_edges();
-    s->n_bnd = (int32_t)m->bnd_edge.size();
-    // device order of the real cells; ghosts keep theirs
-    s->pos.assign(s->n_total, 0);
-    s->inv.assign(s->n_total, 0);
-    std::vector<int32_t> order;
-    if (c->renumber == FVH_ORDER_RCM) order = rcm_order(*m);
-    else if (c->renumber == FVH_ORDER_TILES) order = rcb_order(*m, kBlock);
-    else { order.resize(m->n_real); for (int32_t i = 0; i < m->n_real; ++i) order[i] = i; }
-    for (int32_t k = 0; k < m->n_real; ++k) { s->pos[order[k]] = k; s->inv[k] = order[k]; }
-    for (int32_t g = m->n_real; g < m->n_cells; ++g) { s->pos[g] = g; s->inv[g] = g; }
-    s->dev_ref0 = s->n_total > 0 ? s->pos[0] : 0;
-    // half-edges in CSR by device cell, each list sorted by (reference edge, side)
-    const int32_t E = s->n_edges, N = s->n_total;
-    std::vector<int32_t> off(N + 1, 0);
-    for (int32_t e = 0; e < E; ++e)
-        for (int sd = 0; sd < 2; ++sd) off[s->pos[m->edge_cells[2 * (int64_t)e + sd]] + 1] += 1;
-    for (int32_t d = 0; d < N; ++d) off[d + 1] += off[d];
-    s->n_he = off[N];
-    std::vector<int32_t> cursor(off.begin(), off.end() - 1);
-    std::vector<int32_t> he_nbr(s->n_he), he_twin(s->n_he), he_bc(s->n_he, -1), he_ref(s->n_he);
-    std::vector<uint32_t> he_side(s->n_he);
-    std::vector<int32_t> bc_of_edge(E, -1);
-    for (size_t b = 0; b < m->bnd_edge.size(); ++b) bc_of_edge[m->bnd_edge[b]] = slot_of_name[m->bnd_tag[b]];
-    for (int32_t e = 0; e < E; ++e) {
-        const int32_t d0 = s->pos[m->edge_cells[2 * (int64_t)e]], d1 = s->pos[m->edge_cells[2 * (int64_t)e + 1]];
-        const int32_t h0 = cursor[d0]++;
-        const int32_t h1 = cursor[d1]++;
-        he_nbr[h0] = d1; he_nbr[h1] = d0;
-        he_twin[h0] = h1; he_twin[h1] = h0;
-        he_ref[h0] = e; he_ref[h1] = e;
-        he_side[h0] = 0; he_side[h1] = 1;
-        if (bc_of_edge[e] >= 0) he_bc[h0] = bc_of_edge[e];
-    }
-    // device edge numbering: first use in half-edge order
-    std::vector<int32_t> dev_of_edge(E, -1), dev_edge_ref;
-    dev_edge_ref.reserve(E);
-    std::vector<uint32_t> he_edge(s->n_he);
-    for (int32_t h = 0; h < s->n_he; ++h) {
-        const int32_t e = he_ref[h];
-        if (dev_of_edge[e] < 0) { dev_of_edge[e] = (int32_t)dev_edge_ref.size(); dev_edge_ref.push_back(e); }
-        uint32_t w = (uint32_t)dev_of_edge[e];
-        if (he_side[h]) w |= kSideBit;
-        if (he_bc[h] >= 0) w |= kBcBit;
-        he_edge[h] = w;
-    }
-    // Green-Gauss weight per edge, as calc_gradients computes it (explicit.cpp:132-140)
-    std::vector<double> gg(E);
-    for (int32_t e = 0; e < E; ++e) {
-        const int32_t i = m->edge_cells[2 * (int64_t)e], j = m->edge_cells[2 * (int64_t)e + 1];
-        const double dxif = m->e_cx[e] - m->c_cx[i];
-        const double dyif = m->e_cy[e] - m->c_cy[i];
-        const double dif = std::sqrt(dxif * dxif + dyif * dyif);
-        const double dxij = m->c_cx[i] - m->c_cx[j];
-        const double dyij = m->c_cy[i] - m->c_cy[j];
-        const double dij = std::sqrt(dxij * dxij + dyij * dyij);
-        gg[e] = dif / dij;
-    }
-    // tiles for the tiled flux pass
-    std::vector<uint16_t> he_slot(s->n_he, 0);
-    std::vector<int32_t> t_off, t_edge, t_he, t_cell;
-    s->use_tiles = false;
-    if (c->flux_mode != FVH_FLUX_MODE_CELL && s->n_real > 0)
-        s->use_tiles = build_tiles(s.get(), off, he_edge, he_slot, t_off, t_edge, t_he, t_cell);
-    if (c->flux_mode == FVH_FLUX_MODE_TILE && !s->use_tiles && s->n_real > 0) {
-        set_error("a tile touches more than 1024 edges; use the per-cell flux mode");
-        return FVH_E_UNSUPPORTED;
-    }
-    int32_t rc;
-    if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
-    if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
-    if ((rc = upload(s->he_edge, he_edge.data(), he_edge.size() * 4))) return rc;
-    if ((rc = upload(s->he_twin, he_twin.data(), he_twin.size() * 4))) return rc;
-    if ((rc = upload(s->he_bc, he_bc.data(), he_bc.size() * 4))) return rc;
-    if ((rc = upload(s->he_slot, he_slot.data(), he_slot.size() * 2))) return rc;
-    if (s->use_tiles) {
-        if ((rc = upload(s->t_off, t_off.data(), t_off.size() * 4))) return rc;
-        if ((rc = upload(s->t_edge, t_edge.data(), t_edge.size() * 4))) return rc;
-        if ((rc = upload(s->t_he, t_he.data(), t_he.size() * 4))) return rc;
-        if ((rc = upload(s->t_cell, t_cell.data(), t_cell.size() * 4))) return rc;
-    }
-    rc = s->f32 ? upload_geometry<float>(s.get(), *m, dev_edge_ref, gg)
-                : upload_geometry<double>(s.get(), *m, dev_edge_ref, gg);
+    return create(1, &m, c, out);
+}
+
+int32_t fvh_batch_create(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs, fvh_solver** out) {
+    return create(n, meshes, configs, out);
+}
+
+int32_t fvh_batch_size(const fvh_solver* s) { return s ? s->n_inst : 0; }
+
+int32_t fvh_batch_instance(const fvh_solver* s, int32_t k, int64_t* info) {
+    if (!s || !info || k < 0 || k >= s->n_inst) { set_error("bad argument"); return FVH_E_ARG; }
+    info[0] = s->i_ncells[k];
+    info[1] = s->i_nreal[k];
+    info[2] = s->i_state0[k];
+    return FVH_OK;
+}
+
+int32_t fvh_batch_status(fvh_solver* s, int32_t k, fvh_status* st) {
+    if (!s || !st || k < 0 || k >= s->n_inst) { set_error("bad argument"); return FVH_E_ARG; }
+    std::vector<Ctrl> h;
+    int32_t rc = read_ctrl(s, h);
     if (rc) return rc;
-    // algorithmic bytes per stage (state s, geometry g bytes)
-    const double sb = s->f32 ? 4.0 : 8.0, gb = sb, V = 4.0;
-    const double C = s->n_real, B = s->n_bnd;
-    double bytes = C * (3 * V * sb + sb + gb) + E * (3 * gb + 8) + B * (2 * V * sb + 2 * gb + 4);
-    if (s->second) bytes += C * (7 * V * sb + 2 * gb) + E * (2 * gb);
-    s->stage_bytes = bytes;
-    *out = s.release();
+    to_status(h[k], st);
+    return FVH_OK;
+}
+
+int32_t fvh_batch_sums(fvh_solver* s, double* out) {
+    if (!s || !out) { set_error("null argument"); return FVH_E_ARG; }
+    return s->f32 ? sums_t<float>(s, out) : sums_t<double>(s, out);
+}
+
+int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable) {
+    if (!s) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    if (!enable) s->drop_graphs();
+    s->use_graphs = enable != 0;
     return FVH_OK;
 }
 
@@ -657,6 +995,7 @@ int32_t fvh_solver_apply_bounds(fvh_solver* s) {
 int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, double* gy, double* lim,
                           double* qt, double* qk) {
     if (!s) { set_error("null argument"); return FVH_E_ARG; }
+    if (s->n_inst != 1) { set_error("phases: single-instance solvers only"); return FVH_E_UNSUPPORTED; }
     return s->f32 ? phases_t<float>(s, alpha, dt, gx, gy, lim, qt, qk)
                   : phases_t<double>(s, alpha, dt, gx, gy, lim, qt, qk);
 }
@@ -687,7 +1026,7 @@ int32_t fvh_solver_advance(fvh_solver* s, int32_t n) {
     if (!s || n < 0) { set_error("bad argument"); return FVH_E_ARG; }
     fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
     for (int32_t k = 0; k < n; ++k) {
-        int32_t rc = s->f32 ? enqueue_step<float>(s, false, &o, 0) : enqueue_step<double>(s, false, &o, 0);
+        int32_t rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
         if (rc) return rc;
     }
     return FVH_OK;

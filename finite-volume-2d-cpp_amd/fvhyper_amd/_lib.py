@@ -60,6 +60,8 @@ EXPORTS = [
     "fvh_solver_get_state", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
     "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
+    "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
+    "fvh_batch_status", "fvh_batch_sums",
 ]
 
 _lib = None
@@ -112,6 +114,12 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_kernel_name.restype = C.c_char_p
     L.fvh_solver_stage_bytes.argtypes = [vp]
     L.fvh_solver_stage_bytes.restype = C.c_double
+    L.fvh_solver_use_graphs.argtypes = [vp, C.c_int32]
+    L.fvh_batch_create.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), C.POINTER(vp)]
+    L.fvh_batch_size.argtypes = [vp]
+    L.fvh_batch_instance.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
+    L.fvh_batch_status.argtypes = [vp, C.c_int32, C.POINTER(Status)]
+    L.fvh_batch_sums.argtypes = [vp, dp]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

@@ -402,9 +402,19 @@ def _naca_layout(n_surf, n_layers, first, growth, r_far, h):
 
 
 def _naca_count(n_surf, n_layers, first, growth, r_far, h):
-    _, rings = _naca_layout(n_surf, n_layers, first, growth, r_far, h)
-    n = [n_surf] + [r.shape[0] for r in rings]
-    return n_surf * n_layers + sum(n[i] + n[i + 1] for i in range(len(n) - 1))
+    """Cell count of the O-mesh for ring spacing h, from the ring node counts
+    alone (the same rule as _naca_layout, without coordinates)."""
+    d_last = first * (growth ** n_layers - 1.0) / (growth - 1.0)
+    r_b = d_last + 0.5
+    n_rings = max(1, int(round((r_far - r_b) / h)))
+    total = n_surf * n_layers
+    n_prev = n_surf
+    for j in range(1, n_rings + 1):
+        r = r_b + (r_far - r_b) * j / n_rings
+        n_j = max(n_prev, int(round(2.0 * np.pi * r / h)))
+        total += n_prev + n_j
+        n_prev = n_j
+    return total
 
 
 def naca0012_omesh(n_surf: int = 512, n_layers: int = 64, first: float = 1e-3, growth: float = 1.1,

@@ -139,6 +139,11 @@ class Solver:
         h = C.c_void_p()
         check(L.fvh_solver_create(mesh._h, C.byref(c), C.byref(h)))
         self._h = h
+        self._n_cells = mesh.n_cells
+
+    def use_graphs(self, enable: bool = True) -> None:
+        """Replay each RK step from a captured HIP graph."""
+        check(_lib.load().fvh_solver_use_graphs(self._h, 1 if enable else 0))
 
     def __del__(self):
         try:
@@ -150,7 +155,7 @@ class Solver:
 
     @property
     def n_state(self) -> int:
-        return 4 * self.mesh.n_cells
+        return 4 * self._n_cells
 
     def set_state(self, q: np.ndarray) -> None:
         q = np.ascontiguousarray(q, dtype=np.float64).ravel()
@@ -224,3 +229,53 @@ class Solver:
 
     def stage_bytes(self) -> float:
         return float(_lib.load().fvh_solver_stage_bytes(self._h))
+
+
+class Batch(Solver):
+    """Many independent meshes advanced together in the same launches
+    (fvh_batch_create).  ``cfgs`` is one config dict per mesh; they may
+    differ only in their boundary tables.  States are the concatenation of
+    the instances' reference-layout states; ``split``/``join`` convert."""
+
+    def __init__(self, meshes: List[Mesh], cfgs: List[Dict], precision: str = "f64", renumber="tiles",
+                 flux_mode: str = "auto"):
+        if len(meshes) != len(cfgs) or not meshes:
+            raise ValueError("one config per mesh, at least one mesh")
+        self.meshes = list(meshes)
+        self.cfg = dict(cfgs[0])
+        self.precision = precision
+        L = _lib.load()
+        n = len(meshes)
+        cs = (_lib.Config * n)(*[make_config(c, precision, renumber, flux_mode) for c in cfgs])
+        hs = (C.c_void_p * n)(*[m._h for m in meshes])
+        h = C.c_void_p()
+        check(L.fvh_batch_create(n, hs, cs, C.byref(h)))
+        self._h = h
+        self.mesh = meshes[0]
+        info = (C.c_int64 * 3)()
+        self.offsets = []
+        for k in range(n):
+            check(L.fvh_batch_instance(self._h, k, info))
+            self.offsets.append(int(info[2]))
+        self._n_cells = sum(m.n_cells for m in meshes)
+
+    def __len__(self) -> int:
+        return len(self.meshes)
+
+    def split(self, q: np.ndarray) -> List[np.ndarray]:
+        """Concatenated state -> one reference-layout state per instance."""
+        return [q[4 * o:4 * (o + m.n_cells)].copy() for o, m in zip(self.offsets, self.meshes)]
+
+    def join(self, qs: List[np.ndarray]) -> np.ndarray:
+        return np.concatenate([np.ascontiguousarray(q, dtype=np.float64).ravel() for q in qs])
+
+    def instance_status(self, k: int) -> Dict:
+        st = _lib.Status()
+        check(_lib.load().fvh_batch_status(self._h, int(k), C.byref(st)))
+        return self._status_dict(st)
+
+    def sums(self) -> np.ndarray:
+        """FP64 conserved totals sum A q per instance, shape (n, 4)."""
+        out = np.zeros(4 * len(self.meshes))
+        check(_lib.load().fvh_batch_sums(self._h, dptr(out)))
+        return out.reshape(-1, 4)

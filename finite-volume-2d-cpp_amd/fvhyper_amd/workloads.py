@@ -95,11 +95,20 @@ def naca_freestream(mach: float = 0.8, aoa_deg: float = 1.25, reference_compatib
 def naca_config(mach: float = 0.8, aoa_deg: float = 1.25, reference_compatible: bool = False) -> Dict:
     """Physics of examples/naca/main.cpp: second order (gradients, Michalak
     limiter y_t=2, linear interpolation; :21-28, :56-65), the shockTube/naca
-    Roe spelling (:71-137), local edge-minimum time step with CFL 1.0
-    (:32, :142-183), slip wall on ``airfoil`` and the characteristic
-    ``farfield`` (:192-272), 5-stage RK."""
+    Roe spelling (:71-137), slip wall on ``airfoil`` and the characteristic
+    ``farfield`` (:192-272), 5-stage RK, local time stepping.
+
+    The example's own time step (:142-183) takes its wave speed as
+    (c+|u|)n_x + (c+|v|)n_y with signed normal components; that nearly
+    vanishes for some edge directions, and on the stretched quads of the
+    boundary band it over-estimates dt by up to the aspect ratio (the run goes
+    non-finite within 60 steps from 50k cells up).  The workload therefore
+    uses the forwardStep example's summed form, dt = CFL A / sum_e
+    (|u.n|+c) l (forwardStep/main.cpp:140-185), per cell (global_dt off) with
+    CFL 2; reference_compatible restores the example's form at CFL 1.0."""
     free = ("farfield",) + naca_freestream(mach, aoa_deg, reference_compatible)
-    return dict(flux="roe_split", phi="michalak", phi_yt=2.0, dt="edge_min", cfl=1.0, gdt=False,
+    dt = dict(dt="edge_min", cfl=1.0) if reference_compatible else dict(dt="sum_cfl", cfl=2.0)
+    return dict(flux="roe_split", phi="michalak", phi_yt=2.0, gdt=False, **dt,
                 grad=True, lim=True, lin=True, gamma=GAMMA, alpha=RK5,
                 bcs={"airfoil": ("mirror",), "farfield": free})
 
@@ -117,6 +126,29 @@ def naca_airfoil(total_cells: int = 2_000_000, mach: float = 0.8, aoa_deg: float
     cfg = naca_config(mach, aoa_deg, reference_compatible)
     _, rho, u, v, p = cfg["bcs"]["farfield"]
     return m, cfg, uniform_ic(rho, u, v, p)
+
+
+def naca_sweep(n: int = 256, seed: int = 20240604, min_cells: int = 5000, max_cells: int = 200000,
+               mach_range=(0.5, 0.85), aoa_range=(-2.0, 4.0)):
+    """Parameter sweep over coarsened airfoil meshes: n instances with cell
+    counts log-uniform in [min_cells, max_cells], Mach and angle of attack
+    uniform in their ranges, all drawn from one splitmix64 stream.  Returns a
+    list of (mesh_arrays, cfg, ic, (cells, mach, aoa)); meshes of the same
+    target size are generated once and shared."""
+    u = meshgen.uniform01(seed, 3 * n).reshape(n, 3)
+    lo, hi = np.log(min_cells), np.log(max_cells)
+    cache = {}
+    out = []
+    for k in range(n):
+        cells = int(round(float(np.exp(lo + (hi - lo) * u[k, 0]))))
+        mach = mach_range[0] + (mach_range[1] - mach_range[0]) * float(u[k, 1])
+        aoa = aoa_range[0] + (aoa_range[1] - aoa_range[0]) * float(u[k, 2])
+        if cells not in cache:
+            cache[cells] = meshgen.naca0012_sized(cells)
+        cfg = naca_config(mach, aoa)
+        _, rho, uu, vv, p = cfg["bcs"]["farfield"]
+        out.append((cache[cells], cfg, uniform_ic(rho, uu, vv, p), (cells, mach, aoa)))
+    return out
 
 
 def initial_state(mesh, ic: Callable) -> np.ndarray:

@@ -209,6 +209,32 @@ const char* fvh_solver_kernel_name(int32_t k);
  * adds C(7Vs+2g) + E(2g). */
 double fvh_solver_stage_bytes(const fvh_solver* s);
 
+/* Capture each RK step (every kernel of it) in a HIP graph on first use and
+ * replay it afterwards (one graph per state buffer, residual flag and run
+ * options).  No reference counterpart: it removes per-launch host overhead
+ * on small meshes.  Results are unchanged. */
+int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable);
+
+/* ---- batches: many independent meshes advanced together (no reference
+ * counterpart; a parameter sweep in the reference is one process per mesh).
+ * The handle is a fvh_solver and every fvh_solver_* call applies to all
+ * instances at once.  Configs may differ only in their boundary tables
+ * (e.g. a freestream per instance); each instance keeps its own step count,
+ * time, residuals and stop test, and stops on its own.  States cross as the
+ * concatenation of the instances' reference-layout states, in instance
+ * order.  Each instance's results are bit-identical to a single-mesh solver
+ * on that mesh.  fvh_solver_status reports instance 0 with `stopped` set
+ * once all instances have stopped and `step` the largest step count. */
+int32_t fvh_batch_create(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs,
+                         fvh_solver** out);
+int32_t fvh_batch_size(const fvh_solver* s);
+/* info[0..2] = cells incl. ghosts, real cells, first cell in the concatenated state */
+int32_t fvh_batch_instance(const fvh_solver* s, int32_t k, int64_t* info);
+int32_t fvh_batch_status(fvh_solver* s, int32_t k, fvh_status* st);
+/* Conserved totals sum_c A_c q_ck over each instance's real cells, summed in
+ * FP64 in a fixed order (4 per instance), from the current state. */
+int32_t fvh_batch_sums(fvh_solver* s, double* out);
+
 #ifdef __cplusplus
 }
 #endif

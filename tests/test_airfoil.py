@@ -107,3 +107,18 @@ def test_naca_plugin_run_matches_oracle(tmp_path):
     q, steps, t, R, R0, _ = om.run(cfg, q0, 40, print_interval=1000)
     assert steps == 40 and np.all(np.isfinite(q))
     assert bits_equal(q, r["q"])
+
+
+def test_example_time_step_is_unstable_on_the_boundary_band():
+    """Why the workload does not use the naca example's time step: on a 50k
+    O-mesh the example's edge-minimum dt at CFL 1.0 drives the run
+    non-finite within 60 steps, while the summed form the workload uses
+    (CFL 2.0, per cell) stays finite."""
+    m, cfg, ic = workloads.naca_airfoil(50000)
+    assert cfg["dt"] == "sum_cfl" and not cfg["gdt"]
+    om = O.Mesh(m)
+    q0 = workloads.initial_state(om, ic)
+    q_ex, *_ = om.run(dict(cfg, dt="edge_min", cfl=1.0), q0, 60)
+    q_wl, *_ = om.run(cfg, q0, 60)
+    assert not np.all(np.isfinite(q_ex))
+    assert np.all(np.isfinite(q_wl))
