@@ -40,6 +40,7 @@ constexpr uint32_t kBcBit = 0x40000000u;
 constexpr uint32_t kEdgeMask = 0x3fffffffu;
 constexpr int kBlock = 256;        // cells per workgroup = cells per tile
 constexpr int kTileEdges = 1024;   // LDS edge slots per tile
+constexpr int kStageBlock = 512;   // threads of the fused stage pass (one tile per workgroup)
 constexpr int kMaxBc = 16;
 
 // Per-run control block, advanced on the device so that steps can be queued
@@ -100,6 +101,19 @@ struct TileView {
     const int2* cell;      // (cell of side 0, cell of side 1)
 };
 
+// Halo of each real tile for the fused stage pass: the tile's ring-1
+// neighbours (real cells first, then ghosts) and the ring-2 neighbours of its
+// real ring-1 cells, with every neighbour reference pre-resolved to an index
+// into the tile's local cell list [tile cells | ring 1 | ring 2].
+struct HaloView {
+    const int4* ext;          // [n_rt] (offset into cells, ring-1 real, ring-1 total, ring-2)
+    const int32_t* cells;     // ring-1 then ring-2 device cells of each tile
+    const int32_t* r1_start;  // parallel to cells: a real ring-1 cell's first entry in r1_loc
+    const uint16_t* r1_loc;   // local neighbour index per half-edge of real ring-1 cells
+    const uint16_t* he_loc;   // per half-edge of a real cell: local neighbour index in its own tile
+    const ushort2* te_loc;    // per tile edge slot: local indices of the edge's two cells
+    int32_t max_ext, max_g, max_edges;   // largest tile (LDS sizing)
+};
 
 struct PhysParams {
     double gamma, cfl, dt_fixed, K, phi_yt;
@@ -129,6 +143,17 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
                  const PhysParams& pp, bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
                  const Q4<T>* q0, Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
                  const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
+// Fused stage: gradient + limiter (tile and ring 1, from LDS), face states,
+// edge fluxes, gather, RK update and boundary update in one pass per tile.
+// dt_kind >= 0 also computes the tile cells' local time step (first stage).
+// Returns false when the tile halo does not fit the workgroup's LDS.
+template <class T>
+bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
+                  const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
+                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, T* dt,
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl);
+template <class T>
+size_t stage_lds_bytes(const HaloView& hv);
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
                         const BcParam<T>* bcs, Q4<T>* q);

@@ -24,18 +24,19 @@ __device__ inline double block_min(double v) {
     return r;
 }
 
-// Fixed-pattern sum of 4 values over a 256-thread block (deterministic).
+// Fixed-pattern sum of 4 values over an NB-thread block (deterministic).
+template <int NB = kBlock>
 __device__ inline void block_sum4(double* v) {
     for (int k = 0; k < 4; ++k)
         for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-    __shared__ double part[kBlock / 64][4];
+    __shared__ double part[NB / 64][4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < 4; ++k) part[wave][k] = v[k];
     __syncthreads();
     for (int k = 0; k < 4; ++k) {
         double s = part[0][k];
-        for (int w = 1; w < kBlock / 64; ++w) s += part[w][k];
+        for (int w = 1; w < NB / 64; ++w) s += part[w][k];
         v[k] = s;
     }
 }
@@ -132,9 +133,130 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(InstView iv, const double* __
 }
 
 // ---------------------------------------------------------------------------
-// Green-Gauss gradient + neighbour min/max + limiter + face states, one pass
-// per cell (explicit.cpp:112-258, 307-313); optionally the time step.
+// Green-Gauss gradient + neighbour min/max + limiter of one real cell
+// (explicit.cpp:112-258), optionally its time step.  nbr(h) yields the state
+// across half-edge h and whether that neighbour is the cell itself; the
+// two-pass and the fused forms differ only in where it reads from.
 // ---------------------------------------------------------------------------
+template <class T, int PHI, int DT, class Nbr>
+__device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp, int grad, int lim_on, bool real,
+                                     bool do_dt, int c, int h0, int h1, const Q4<T>& qc, const G2<T>& cc, Nbr nbr,
+                                     T (&gx)[4], T (&gy)[4], T (&lim)[4], T& dt_out) {
+    const T g = T(pp.gamma);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { gx[k] = T(0); gy[k] = T(0); }
+    // limiters are reset to 1 when computed (explicit.cpp:184-186) and
+    // stay zero-initialised otherwise (explicit.cpp:673)
+    const T lim0 = lim_on ? T(1) : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) lim[k] = lim0;
+    T qmn[4], qmx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
+    T dts = (DT == 2) ? T(1) : T(0);
+    // one pass over the half-edges feeds the Green-Gauss sums (edge order,
+    // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
+    // time-step sum or minimum
+    if ((real && (grad || lim_on)) || (DT && do_dt)) {
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const bool side = (ue & kSideBit) != 0;
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            bool self;
+            const Q4<T> qn = nbr(h, self);
+            if (DT == 1 && do_dt) {
+                const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
+                const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+                dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+            } else if (DT == 2 && do_dt) {
+                const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+                dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
+            }
+            if (!real) continue;
+            if (lim_on) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    qmn[k] = smin(qmn[k], qn.v[k]);
+                    qmx[k] = smax(qmx[k], qn.v[k]);
+                }
+            }
+            if (!grad || self) continue;   // the reference skips i == j edges
+            const T* qi = side ? qn.v : qc.v;
+            const T* qj = side ? qc.v : qn.v;
+            const T w = geo.v[3];
+            const T w1 = T(1) - w;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T f = (qi[k] * w1 + qj[k] * w) * geo.v[2];
+                if (!side) {
+                    gx[k] = gx[k] + f * geo.v[0];
+                    gy[k] = gy[k] + f * geo.v[1];
+                } else {
+                    gx[k] = gx[k] - f * geo.v[0];
+                    gy[k] = gy[k] - f * geo.v[1];
+                }
+            }
+        }
+    }
+    if (DT && do_dt) dt_out = (DT == 1) ? T(pp.cfl) * m.c_area[c] / dts : dts;
+    if (real && grad) {
+        const T ia = m.c_inv_area[c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gx[k] = gx[k] * ia;
+            gy[k] = gy[k] * ia;
+        }
+    }
+    if (real && lim_on) {
+        const T Ka = T(pp.K) * m.c_sqrt_area[c];
+        const T K3a = Ka * Ka * Ka;
+        const T yt = T(pp.phi_yt);
+        const T tol = T(1e-15);
+        for (int h = h0; h < h1; ++h) {
+            const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T dqg = gx[k] * dx + gy[k] * dy;
+                const T dmax = qmx[k] - qc.v[k];
+                const T dmin = qmn[k] - qc.v[k];
+                const T spread = (dmax - dmin) * (dmax - dmin);
+                T sig;
+                if (spread <= K3a) {
+                    sig = T(1);
+                } else if (spread <= T(2) * K3a) {
+                    const T y = (spread / K3a - T(1));
+                    sig = T(2) * y * y * y - T(3) * y * y + T(1);
+                } else {
+                    sig = T(0);
+                }
+                T l = T(1);
+                if (sig < T(1)) {
+                    if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
+                    else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+                }
+                l = sig + (T(1) - sig) * l;
+                lim[k] = smin(lim[k], l);
+            }
+        }
+    }
+}
+
+// Limited linear increment (grad q . d) phi at an edge centre (explicit.cpp:307-313).
+template <class T>
+__device__ inline Q4<T> face_inc(const T (&gx)[4], const T (&gy)[4], const T (&lim)[4], const G2<T>& ec,
+                                 const G2<T>& cc) {
+    const T dx = ec.v[0] - cc.v[0];
+    const T dy = ec.v[1] - cc.v[1];
+    Q4<T> inc;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
+    return inc;
+}
+
+// Two-pass form, pass 1: gradient + limiter + face states of every cell of
+// the tile, optionally its time step and the tile's minimum.
 template <class T, int PHI, int DT>
 __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad, int lim_on,
                                                     int lin, const Q4<T>* __restrict__ q,
@@ -151,108 +273,16 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
         const Q4<T> qc = ld(q + c);
         const G2<T> cc = m.c_c[c];
         const bool real = c < m.n_real;
-        const T g = T(pp.gamma);
-        T gx[4] = {T(0), T(0), T(0), T(0)};
-        T gy[4] = {T(0), T(0), T(0), T(0)};
-        // limiters are reset to 1 when computed (explicit.cpp:184-186) and
-        // stay zero-initialised otherwise (explicit.cpp:673)
-        const T lim0 = lim_on ? T(1) : T(0);
-        T lim[4] = {lim0, lim0, lim0, lim0};
-        T qmn[4], qmx[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
-        T dts = (DT == 2) ? T(1) : T(0);
-        // one pass over the half-edges feeds the Green-Gauss sums (edge order,
-        // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
-        // time-step sum or minimum
-        if ((real && (grad || lim_on)) || DT) {
-            for (int h = h0; h < h1; ++h) {
-                const int nb = m.he_nbr[h];
-                const uint32_t ue = m.he_edge[h];
-                const bool side = (ue & kSideBit) != 0;
-                const Q4<T> geo = m.e_geo[ue & kEdgeMask];
-                const Q4<T> qn = ld(q + nb);
-                if (DT == 1) {
-                    const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
-                    const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
-                    dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
-                } else if (DT == 2) {
-                    const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
-                    dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
-                }
-                if (!real) continue;
-                if (lim_on) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        qmn[k] = smin(qmn[k], qn.v[k]);
-                        qmx[k] = smax(qmx[k], qn.v[k]);
-                    }
-                }
-                if (!grad || nb == c) continue;   // the reference skips i == j edges
-                const T* qi = side ? qn.v : qc.v;
-                const T* qj = side ? qc.v : qn.v;
-                const T w = geo.v[3];
-                const T w1 = T(1) - w;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const T f = (qi[k] * w1 + qj[k] * w) * geo.v[2];
-                    if (!side) {
-                        gx[k] = gx[k] + f * geo.v[0];
-                        gy[k] = gy[k] + f * geo.v[1];
-                    } else {
-                        gx[k] = gx[k] - f * geo.v[0];
-                        gy[k] = gy[k] - f * geo.v[1];
-                    }
-                }
-            }
-        }
+        T gx[4], gy[4], lim[4], dtc = T(0);
+        auto nbr = [&](int h, bool& self) {
+            const int nb = m.he_nbr[h];
+            self = nb == c;
+            return ld(q + nb);
+        };
+        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, real, true, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
         if (DT) {
-            T r = dts;
-            if (DT == 1) r = T(pp.cfl) * m.c_area[c] / dts;
-            dt[c] = r;
-            vmin = (double)r;
-        }
-        if (real && grad) {
-            const T ia = m.c_inv_area[c];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                gx[k] = gx[k] * ia;
-                gy[k] = gy[k] * ia;
-            }
-        }
-        if (real && lim_on) {
-            const T Ka = T(pp.K) * m.c_sqrt_area[c];
-            const T K3a = Ka * Ka * Ka;
-            const T yt = T(pp.phi_yt);
-            const T tol = T(1e-15);
-            for (int h = h0; h < h1; ++h) {
-                const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
-                const T dx = ec.v[0] - cc.v[0];
-                const T dy = ec.v[1] - cc.v[1];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const T dqg = gx[k] * dx + gy[k] * dy;
-                    const T dmax = qmx[k] - qc.v[k];
-                    const T dmin = qmn[k] - qc.v[k];
-                    const T spread = (dmax - dmin) * (dmax - dmin);
-                    T sig;
-                    if (spread <= K3a) {
-                        sig = T(1);
-                    } else if (spread <= T(2) * K3a) {
-                        const T y = (spread / K3a - T(1));
-                        sig = T(2) * y * y * y - T(3) * y * y + T(1);
-                    } else {
-                        sig = T(0);
-                    }
-                    T l = T(1);
-                    if (sig < T(1)) {
-                        if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
-                        else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
-                    }
-                    l = sig + (T(1) - sig) * l;
-                    lim[k] = smin(lim[k], l);
-                }
-            }
+            dt[c] = dtc;
+            vmin = (double)dtc;
         }
         if (dbg_gx) {
             Q4<T> a, b, d;
@@ -265,15 +295,10 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
         if (lin) {
             for (int h = h0; h < h1; ++h) {
                 const uint32_t ue = m.he_edge[h];
-                const G2<T> ec = m.e_c[ue & kEdgeMask];
-                const T dx = ec.v[0] - cc.v[0];
-                const T dy = ec.v[1] - cc.v[1];
-                Q4<T> inc, fs;
+                const Q4<T> inc = face_inc(gx, gy, lim, m.e_c[ue & kEdgeMask], cc);
+                Q4<T> fs;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
-                    fs.v[k] = qc.v[k] + inc.v[k];
-                }
+                for (int k = 0; k < 4; ++k) fs.v[k] = qc.v[k] + inc.v[k];
                 face[h] = fs;
                 if (ue & kBcBit) bincr[m.he_nbr[h] - m.n_real] = inc;
             }
@@ -287,12 +312,14 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
 
 // ---------------------------------------------------------------------------
 // RK stage update + boundary update of one cell from its accumulated qt
-// (explicit.cpp:368-417)
+// (explicit.cpp:368-417).  inc(h) is the limited increment of the cell's
+// side of boundary half-edge h (this stage's gradient and limiter applied
+// to the just-updated state: the reference's stale-gradient boundary).
 // ---------------------------------------------------------------------------
-template <class T, bool LIN>
+template <class T, bool LIN, class Inc>
 __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
                                    int h0, int h1, const T (&qt)[4], const Q4<T>* __restrict__ q0,
-                                   Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
+                                   Q4<T>* __restrict__ qout, Inc inc_of,
                                    const T d, T alpha, bool want_rs,
                                    double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
     const T g = T(pp.gamma);
@@ -320,7 +347,7 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
         const Q4<T> geo = m.e_geo[ue & kEdgeMask];
         T qi[4];
         if (LIN) {
-            const Q4<T> ia = ld(bincr + (nb - m.n_real));
+            const Q4<T> ia = inc_of(h, nb);
 #pragma unroll
             for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + ia.v[k];
             if (pp.positivity && !physical(g, qi))
@@ -385,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt_inst ? dt_inst[inst] : dt[c], alpha,
+        auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                             res_blk != nullptr, rs, dbg_qt);
     }
     if (res_blk) {
@@ -455,11 +483,141 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, bincr, dt_inst ? dt_inst[inst] : dt[c], alpha,
+        auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
+        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                             res_blk != nullptr, rs, dbg_qt);
     }
     if (res_blk) {
         block_sum4(rs);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
+    }
+}
+
+// Fused form: one workgroup per tile does the whole stage.  Phase 0 stages
+// the states of the tile and its two neighbour rings in LDS; phase 1
+// computes gradient and limiter of the tile cells and of the real ring-1
+// cells (a ring-1 cell's gradient is recomputed by every tile that borders
+// it: same inputs, same order, same bits); phase 2 evaluates each edge the
+// tile touches once from the face states built on the fly; phase 3 gathers
+// per cell in reference edge order and finishes the stage.  No face or
+// boundary-increment buffers go through HBM.
+template <class T, int PHI, int DT, bool JOINED>
+__global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+                                                  int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
+                                                  const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
+                                                  Q4<T>* __restrict__ qout, T* __restrict__ dt,
+                                                  const T* __restrict__ dt_inst, T alpha,
+                                                  double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl) {
+    extern __shared__ __align__(32) unsigned char lds_raw[];
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    const int inst = tm.inst[lb];
+    if (stopped(ctrl, inst)) return;
+    const int c0 = tm.c0[lb], nt = tm.c0[lb + 1] - c0;
+    const int4 ex = hv.ext[lb];
+    const int n_g = nt + ex.y, n_r1 = nt + ex.z, n_ext = nt + ex.z + ex.w;
+    Q4<T>* sq = (Q4<T>*)lds_raw;
+    Q4<T>* sgx = sq + hv.max_ext;
+    Q4<T>* sgy = sgx + hv.max_g;
+    Q4<T>* slm = sgy + hv.max_g;
+    Q4<T>* fle = slm + hv.max_g;
+    const int32_t* xc = hv.cells + ex.x;
+    const T g = T(pp.gamma);
+    const T lim0 = lim_on ? T(1) : T(0);
+    // phase 0: states of tile, ring 1 and ring 2
+    for (int j = threadIdx.x; j < n_ext; j += kStageBlock) sq[j] = ld(qin + (j < nt ? c0 + j : xc[j - nt]));
+    __syncthreads();
+    // phase 1: gradients and limiters of the tile and its real ring-1 cells
+    for (int j = threadIdx.x; j < n_g; j += kStageBlock) {
+        const bool own = j < nt;
+        const int c = own ? c0 + j : xc[j - nt];
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const uint16_t* loc = own ? hv.he_loc + h0 : hv.r1_loc + hv.r1_start[ex.x + j - nt];
+        T gx[4], gy[4], lim[4], dtc = T(0);
+        auto nbr = [&](int h, bool& self) {
+            const int l = loc[h - h0];
+            self = l == j;
+            return sq[l];
+        };
+        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, sq[j], m.c_c[c], nbr, gx, gy, lim, dtc);
+        if (DT && own) dt[c] = dtc;
+        Q4<T> a, b, d;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
+        sgx[j] = a;
+        sgy[j] = b;
+        slm[j] = d;
+    }
+    __syncthreads();
+    // phase 2: each edge of the tile once, from reconstructed face states
+    const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
+    for (int k = e0 + (int)threadIdx.x; k < e1; k += kStageBlock) {
+        const int e = tv.edge[k];
+        const Q4<T> geo = m.e_geo[e];
+        const G2<T> ec = m.e_c[e];
+        const ushort2 lc = hv.te_loc[k];
+        const int2 cl = tv.cell[k];
+        Q4<T> f2[2];
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            const int l = sd ? lc.y : lc.x;
+            const G2<T> cc = m.c_c[sd ? cl.y : cl.x];
+            T gx[4], gy[4], lim[4];
+            if (l < n_g) {
+                const Q4<T> a = sgx[l], b = sgy[l], d = slm[l];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) { gx[j] = a.v[j]; gy[j] = b.v[j]; lim[j] = d.v[j]; }
+            } else {
+                // boundary ghost: zero gradient, limiter at its reset value
+#pragma unroll
+                for (int j = 0; j < 4; ++j) { gx[j] = T(0); gy[j] = T(0); lim[j] = lim0; }
+            }
+            const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
+            const Q4<T> qs = sq[l];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f2[sd].v[j] = qs.v[j] + inc.v[j];
+            if (pp.positivity && !physical(g, f2[sd].v)) f2[sd] = qs;
+        }
+        T f[4];
+        roe_flux<T, JOINED>(g, f, f2[0].v, f2[1].v, geo.v[0], geo.v[1]);
+        Q4<T> r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
+        fle[k - e0] = r;
+    }
+    __syncthreads();
+    // phase 3: ordered gather, RK update, boundary update
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    const int j = threadIdx.x;
+    if (j < nt) {
+        const int c = c0 + j;
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const T A = m.c_area[c];
+        T qt[4] = {T(0), T(0), T(0), T(0)};
+        for (int h = h0; h < h1; ++h) {
+            const bool side = (m.he_edge[h] & kSideBit) != 0;
+            const Q4<T> r = fle[m.he_slot[h]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T t = r.v[k] / A;
+                qt[k] = side ? qt[k] + t : qt[k] - t;
+            }
+        }
+        const G2<T> cc = m.c_c[c];
+        auto inc = [&](int h, int) {
+            T gx[4], gy[4], lim[4];
+            const Q4<T> a = sgx[j], b = sgy[j], d = slm[j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { gx[k] = a.v[k]; gy[k] = b.v[k]; lim[k] = d.v[k]; }
+            return face_inc(gx, gy, lim, m.e_c[m.he_edge[h] & kEdgeMask], cc);
+        };
+        finish_cell<T, true>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+                             res_blk != nullptr, rs, nullptr);
+    }
+    (void)n_r1;
+    if (res_blk) {
+        block_sum4<kStageBlock>(rs);
         if (threadIdx.x == 0)
             for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
     }
@@ -667,6 +825,52 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
 }
 
 template <class T>
+size_t stage_lds_bytes(const HaloView& hv) {
+    return (size_t)(hv.max_ext + 3 * hv.max_g + hv.max_edges) * sizeof(Q4<T>);
+}
+
+template <class T, int PHI, int DT, bool J>
+bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
+                    const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
+                    const Q4<T>* q0, Q4<T>* qout, T* dt, const T* dt_inst, T alpha, double* res_blk,
+                    const Ctrl* ctrl) {
+    const size_t lds = stage_lds_bytes<T>(hv);
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024) != hipSuccess)
+            (void)hipGetLastError();
+        attr_set = true;
+    }
+    const int nb = xcd_grid_blocks(tm.n_rt);
+    hipLaunchKernelGGL((k_stage<T, PHI, DT, J>), dim3(nb), dim3(kStageBlock), lds, s, m, tm, tv, hv, pp, (int)grad,
+                       (int)lim, bcs, qin, q0, qout, dt, dt_inst, alpha, res_blk, ctrl);
+    return true;
+}
+
+template <class T>
+bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
+                  const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
+                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, T* dt,
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl) {
+    if (tm.n_rt == 0) return true;
+    if (stage_lds_bytes<T>(hv) > 160 * 1024) return false;
+#define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, dt, dt_inst, \
+                                                  alpha, res_blk, ctrl)
+#define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
+#define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
+    switch (phi_kind) {
+        case PHI_ZERO: return FVH_STD(PHI_ZERO);
+        case PHI_MICHALAK: return FVH_STD(PHI_MICHALAK);
+        case PHI_VENKAT: return FVH_STD(PHI_VENKAT);
+        default: return FVH_STD(PHI_MINMOD);
+    }
+#undef FVH_STD
+#undef FVH_STJ
+#undef FVH_ST
+}
+
+template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
                         const BcParam<T>* bcs, Q4<T>* q) {
     const int nb = nblocks(m.n_real);
@@ -713,7 +917,12 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
     template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,            \
                                         const BcParam<T>*, Q4<T>*);                                          \
     template void launch_sums<T>(hipStream_t, const MeshView<T>&, const TileMap&, const InstView&,           \
-                                 const Q4<T>*, double*, double*);
+                                 const Q4<T>*, double*, double*);                                            \
+    template bool launch_stage<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,          \
+                                  const HaloView&, const PhysParams&, int, bool, bool, bool, int,            \
+                                  const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, T*, const T*, T,     \
+                                  double*, const Ctrl*);                                                     \
+    template size_t stage_lds_bytes<T>(const HaloView&);
 FVH_INST(double)
 FVH_INST(float)
 

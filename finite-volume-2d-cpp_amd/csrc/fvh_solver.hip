@@ -29,8 +29,8 @@
 namespace fvh {
 namespace {
 
-const char* kKernelNames[] = {"dt", "dt_min", "gradlim", "flux", "ctrl_begin", "ctrl_end", "bounds_init"};
-enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_COUNT };
+const char* kKernelNames[] = {"dt", "dt_min", "gradlim", "flux", "ctrl_begin", "ctrl_end", "bounds_init", "stage"};
+enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_STAGE, K_COUNT };
 
 #define FVH_HIP(call)                                                                       \
     do {                                                                                    \
@@ -90,6 +90,10 @@ struct fvh_solver {
     fvh::DevBuf e_geo, e_c, c_area, c_inv_area, c_sqrt_area, c_c;
     fvh::DevBuf t_off, t_edge, t_he, t_cell;
     bool use_tiles = false;
+    // fused stage pass: tile halos
+    bool fused = false;
+    fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_he_loc, x_te_loc;
+    int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
     int cur = 0;                          // index of the current state buffer
@@ -158,6 +162,20 @@ InstView inst_view(fvh_solver* s) {
     v.tiles = (const int4*)s->inst_tiles.p;
     v.dev0 = (const int32_t*)s->inst_dev0.p;
     return v;
+}
+
+HaloView halo_view(fvh_solver* s) {
+    HaloView h;
+    h.ext = (const int4*)s->x_ext.p;
+    h.cells = (const int32_t*)s->x_cells.p;
+    h.r1_start = (const int32_t*)s->x_r1_start.p;
+    h.r1_loc = (const uint16_t*)s->x_r1_loc.p;
+    h.he_loc = (const uint16_t*)s->x_he_loc.p;
+    h.te_loc = (const ushort2*)s->x_te_loc.p;
+    h.max_ext = s->x_max_ext;
+    h.max_g = s->x_max_g;
+    h.max_edges = s->x_max_edges;
+    return h;
 }
 
 TileView tile_view(fvh_solver* s) {
@@ -334,14 +352,31 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     double* blk = (double*)s->blk_min.p;
     Q4<T>* face = (Q4<T>*)s->face.p;
     Q4<T>* bincr = (Q4<T>*)s->bincr.p;
-    if (!s->second) {
+    const bool fused = s->fused && !dbg;
+    if (!s->second || (fused && gdt)) {
         Bracket b(s, K_DT);
         launch_dt<T>(st, m, tm, c.dt_kind, pp, q0, dt, blk, cc);
     }
     const Q4<T>* qin = q0;
     Q4<T>* out = qa;
     const int n_stage = alpha_override ? 1 : c.n_alpha;
-    for (int k = 0; k < n_stage; ++k) {
+    if (fused) {
+        const HaloView hv = halo_view(s);
+        if (gdt) {
+            Bracket b(s, K_DTMIN);
+            launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
+        }
+        for (int k = 0; k < n_stage; ++k) {
+            Bracket b(s, K_STAGE);
+            const bool last = k == n_stage - 1;
+            launch_stage<T>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim,
+                            (k == 0 && !gdt) ? c.dt_kind : -1, bcs, qin, q0, out, dt, gdt ? dtg : nullptr,
+                            T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc);
+            qin = out;
+            out = (out == qa) ? qb : qa;
+        }
+    }
+    for (int k = 0; k < (fused ? 0 : n_stage); ++k) {
         if (s->second) {
             // the first stage's pass also computes the time step from q0
             Bracket b(s, K_GRADLIM);
@@ -625,6 +660,79 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::
     return true;
 }
 
+// Halos of the real tiles for the fused stage pass.  Returns false when a
+// tile's local cell list does not fit 16-bit indices.
+struct HaloHost {
+    std::vector<int4> ext;
+    std::vector<int32_t> cells, r1_start;
+    std::vector<uint16_t> r1_loc, he_loc, te_loc;
+    int32_t max_ext = 0, max_g = 0, max_edges = 0;
+};
+
+bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::vector<int32_t>& off,
+                const std::vector<int32_t>& he_nbr, const std::vector<int32_t>& t_off,
+                const std::vector<int32_t>& t_cell, HaloHost& hh) {
+    const int32_t N = s->n_total;
+    std::vector<int32_t> mark(N, -1);
+    std::vector<int32_t> r1r, r1g, r2;
+    hh.ext.resize(s->n_rt);
+    hh.he_loc.assign(s->n_he, 0);
+    hh.te_loc.assign(t_cell.size(), 0);
+    for (int32_t t = 0; t < s->n_rt; ++t) {
+        const int32_t c0 = tile_c0[t], c1 = tile_c0[t + 1], nt = c1 - c0;
+        for (int32_t c = c0; c < c1; ++c) mark[c] = c - c0;
+        r1r.clear(); r1g.clear(); r2.clear();
+        for (int32_t c = c0; c < c1; ++c)
+            for (int32_t h = off[c]; h < off[c + 1]; ++h) {
+                const int32_t nb = he_nbr[h];
+                if (mark[nb] != -1) continue;
+                mark[nb] = -2;
+                (nb < s->n_real ? r1r : r1g).push_back(nb);
+            }
+        std::sort(r1r.begin(), r1r.end());
+        std::sort(r1g.begin(), r1g.end());
+        const int32_t n1r = (int32_t)r1r.size(), n1 = n1r + (int32_t)r1g.size();
+        for (int32_t k = 0; k < n1r; ++k) mark[r1r[k]] = nt + k;
+        for (size_t k = 0; k < r1g.size(); ++k) mark[r1g[k]] = nt + n1r + (int32_t)k;
+        for (int32_t c : r1r)
+            for (int32_t h = off[c]; h < off[c + 1]; ++h) {
+                const int32_t nb = he_nbr[h];
+                if (mark[nb] != -1) continue;
+                mark[nb] = -2;
+                r2.push_back(nb);
+            }
+        std::sort(r2.begin(), r2.end());
+        for (size_t k = 0; k < r2.size(); ++k) mark[r2[k]] = nt + n1 + (int32_t)k;
+        const int32_t n_ext = nt + n1 + (int32_t)r2.size();
+        if (n_ext > 65535) return false;
+        for (int32_t c = c0; c < c1; ++c)
+            for (int32_t h = off[c]; h < off[c + 1]; ++h) hh.he_loc[h] = (uint16_t)mark[he_nbr[h]];
+        const int32_t base = (int32_t)hh.cells.size();
+        hh.ext[t] = make_int4(base, n1r, n1, (int32_t)r2.size());
+        for (int32_t c : r1r) {
+            hh.cells.push_back(c);
+            hh.r1_start.push_back((int32_t)hh.r1_loc.size());
+            for (int32_t h = off[c]; h < off[c + 1]; ++h) hh.r1_loc.push_back((uint16_t)mark[he_nbr[h]]);
+        }
+        for (int32_t c : r1g) { hh.cells.push_back(c); hh.r1_start.push_back(0); }
+        for (int32_t c : r2) { hh.cells.push_back(c); hh.r1_start.push_back(0); }
+        for (int32_t k = t_off[t]; k < t_off[t + 1]; ++k) {
+            hh.te_loc[2 * (size_t)k] = (uint16_t)mark[t_cell[2 * (size_t)k]];
+            hh.te_loc[2 * (size_t)k + 1] = (uint16_t)mark[t_cell[2 * (size_t)k + 1]];
+        }
+        hh.max_ext = std::max(hh.max_ext, n_ext);
+        hh.max_g = std::max(hh.max_g, nt + n1r);
+        hh.max_edges = std::max(hh.max_edges, t_off[t + 1] - t_off[t]);
+        for (int32_t c = c0; c < c1; ++c) mark[c] = -1;
+        for (int32_t c : r1r) mark[c] = -1;
+        for (int32_t c : r1g) mark[c] = -1;
+        for (int32_t c : r2) mark[c] = -1;
+    }
+    if (hh.r1_loc.empty()) hh.r1_loc.push_back(0);
+    if (hh.cells.empty()) { hh.cells.push_back(0); hh.r1_start.push_back(0); }
+    return true;
+}
+
 int32_t check_config(const fvh_config* c) {
     if (c->n_alpha < 1 || c->n_alpha > FVH_MAX_ALPHA || c->n_bc < 0 || c->n_bc > FVH_MAX_BC) {
         set_error("config out of range (alpha or bc count)");
@@ -632,8 +740,12 @@ int32_t check_config(const fvh_config* c) {
     }
     if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
     if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2 ||
-        c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 2) {
+        c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 3) {
         set_error("unknown flux, limiter, time-step, order or flux-mode kind");
+        return FVH_E_ARG;
+    }
+    if (c->tile_cells != 0 && (c->tile_cells < 32 || c->tile_cells > kBlock)) {
+        set_error("tile_cells must be 0 or 32..256");
         return FVH_E_ARG;
     }
     for (int k = 0; k < c->n_bc; ++k)
@@ -646,7 +758,7 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
     if (a.precision != b.precision || a.flux != b.flux || a.gamma != b.gamma || a.phi != b.phi ||
         a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
         a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
-        a.flux_mode != b.flux_mode)
+        a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells)
         return false;
     for (int k = 0; k < a.n_alpha; ++k)
         if (a.alpha[k] != b.alpha[k]) return false;
@@ -702,6 +814,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     s->n_total = (int32_t)tot_cells;
     s->n_edges = (int32_t)tot_edges;
     const int32_t N = s->n_total, E = s->n_edges;
+    const int32_t tile = c.tile_cells ? c.tile_cells : kBlock;
     s->inv.assign(N, 0);
     s->i_ncells.resize(n); s->i_real0.resize(n); s->i_nreal.resize(n); s->i_ghost0.resize(n);
     s->i_nghost.resize(n); s->i_dev0.resize(n); s->i_state0.resize(n); s->i_tiles.resize(n);
@@ -716,7 +829,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             const fvh_mesh& m = *ms[i];
             std::vector<int32_t> order;
             if (c.renumber == FVH_ORDER_RCM) order = rcm_order(m);
-            else if (c.renumber == FVH_ORDER_TILES) order = rcb_order(m, kBlock);
+            else if (c.renumber == FVH_ORDER_TILES) order = rcb_order(m, tile);
             else { order.resize(m.n_real); for (int32_t k = 0; k < m.n_real; ++k) order[k] = k; }
             s->i_ncells[i] = m.n_cells;
             s->i_real0[i] = r0;
@@ -743,7 +856,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         // real tiles, instance by instance, then ghost tiles
         for (int32_t i = 0; i < n; ++i) {
             s->i_tiles[i].x = (int32_t)tile_inst.size();
-            for (int32_t k = 0; k < s->i_nreal[i]; k += kBlock) {
+            for (int32_t k = 0; k < s->i_nreal[i]; k += tile) {
                 tile_c0.push_back(s->i_real0[i] + k);
                 tile_inst.push_back(i);
             }
@@ -767,7 +880,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             const int32_t i = tile_inst[t];
             const bool real = t < s->n_rt;
             const int32_t lim = real ? s->i_real0[i] + s->i_nreal[i] : s->i_ghost0[i] + s->i_nghost[i];
-            ends[t] = std::min(tile_c0[t] + kBlock, lim);
+            ends[t] = std::min(tile_c0[t] + (real ? tile : kBlock), lim);
         }
         for (int32_t t = 0; t + 1 < s->n_all; ++t)
             if (ends[t] != tile_c0[t + 1]) { set_error("internal: tile layout"); return FVH_E_UNSUPPORTED; }
@@ -861,9 +974,36 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     s->use_tiles = false;
     if (c.flux_mode != FVH_FLUX_MODE_CELL && s->n_real > 0)
         s->use_tiles = build_tiles(s.get(), tile_c0, off, he_edge, he_slot, t_off, t_edge, t_he, t_cell);
-    if (c.flux_mode == FVH_FLUX_MODE_TILE && !s->use_tiles && s->n_real > 0) {
+    if ((c.flux_mode == FVH_FLUX_MODE_TILE || c.flux_mode == FVH_FLUX_MODE_FUSED) && !s->use_tiles &&
+        s->n_real > 0) {
         set_error("a tile touches more than 1024 edges; use the per-cell flux mode");
         return FVH_E_UNSUPPORTED;
+    }
+    // fused stage pass: linear reconstruction only (otherwise there is
+    // nothing to fuse), and only when the largest tile halo fits in LDS
+    HaloHost hh;
+    const bool lin_on = (c.flags & FVH_FLAG_LINEAR) != 0;
+    s->fused = false;
+    if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
+        lin_on && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_cell, hh)) {
+        const size_t qb = s->f32 ? 16 : 32;
+        const size_t lds = (size_t)(hh.max_ext + 3 * hh.max_g + hh.max_edges) * qb;
+        s->fused = lds <= 160 * 1024;
+    }
+    if (c.flux_mode == FVH_FLUX_MODE_FUSED && !s->fused && s->n_real > 0) {
+        set_error("fused stage pass unavailable: needs linear reconstruction and a tile halo that fits in LDS");
+        return FVH_E_UNSUPPORTED;
+    }
+    if (s->fused) {
+        s->x_max_ext = hh.max_ext;
+        s->x_max_g = hh.max_g;
+        s->x_max_edges = hh.max_edges;
+        if ((rc = upload(s->x_ext, hh.ext.data(), hh.ext.size() * sizeof(int4)))) return rc;
+        if ((rc = upload(s->x_cells, hh.cells.data(), hh.cells.size() * 4))) return rc;
+        if ((rc = upload(s->x_r1_start, hh.r1_start.data(), hh.r1_start.size() * 4))) return rc;
+        if ((rc = upload(s->x_r1_loc, hh.r1_loc.data(), hh.r1_loc.size() * 2))) return rc;
+        if ((rc = upload(s->x_he_loc, hh.he_loc.data(), hh.he_loc.size() * 2))) return rc;
+        if ((rc = upload(s->x_te_loc, hh.te_loc.data(), hh.te_loc.size() * 2))) return rc;
     }
     if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
     if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
@@ -931,6 +1071,7 @@ void fvh_config_default(fvh_config* c) {
     for (int k = 0; k < 5; ++k) c->alpha[k] = rk5[k];
     c->renumber = FVH_ORDER_TILES;
     c->flux_mode = FVH_FLUX_MODE_AUTO;
+    c->tile_cells = kBlock;
 }
 
 int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** out) {
@@ -964,6 +1105,55 @@ int32_t fvh_batch_status(fvh_solver* s, int32_t k, fvh_status* st) {
 int32_t fvh_batch_sums(fvh_solver* s, double* out) {
     if (!s || !out) { set_error("null argument"); return FVH_E_ARG; }
     return s->f32 ? sums_t<float>(s, out) : sums_t<double>(s, out);
+}
+
+int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs, const double* q,
+                     int32_t steps, int32_t* best_mode, int32_t* best_tile, double* report, int32_t max_report) {
+    if (n < 1 || !meshes || !configs || !q || steps < 1 || !best_mode || !best_tile) {
+        set_error("bad argument");
+        return FVH_E_ARG;
+    }
+    const int32_t modes[3] = {FVH_FLUX_MODE_TILE, FVH_FLUX_MODE_FUSED, FVH_FLUX_MODE_CELL};
+    const int32_t tiles[2] = {128, 256};
+    double best = 1e300;
+    int32_t rows = 0;
+    *best_mode = FVH_FLUX_MODE_TILE;
+    *best_tile = kBlock;
+    std::vector<fvh_config> cs(configs, configs + n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FVH_HIP(hipEventCreate(&e0));
+    FVH_HIP(hipEventCreate(&e1));
+    int32_t rc = FVH_OK;
+    for (int32_t mi = 0; mi < 3 && rc == FVH_OK; ++mi)
+        for (int32_t ti = 0; ti < 2 && rc == FVH_OK; ++ti) {
+            for (auto& c : cs) { c.flux_mode = modes[mi]; c.tile_cells = tiles[ti]; }
+            fvh_solver* s = nullptr;
+            double ms = -1.0;
+            if (create(n, meshes, cs.data(), &s) == FVH_OK) {
+                fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
+                rc = s->f32 ? set_state_t<float>(s, q) : set_state_t<double>(s, q);
+                if (!rc) rc = s->f32 ? apply_bounds_t<float>(s) : apply_bounds_t<double>(s);
+                for (int k = 0; k < 2 && !rc; ++k) rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
+                if (!rc && hipEventRecord(e0, s->stream) != hipSuccess) rc = FVH_E_HIP;
+                for (int k = 0; k < steps && !rc; ++k) rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
+                if (!rc && hipEventRecord(e1, s->stream) != hipSuccess) rc = FVH_E_HIP;
+                if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = FVH_E_HIP;
+                float t = 0.f;
+                if (!rc && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / steps;
+                delete s;
+            }
+            if (report && rows < max_report) {
+                report[3 * rows] = modes[mi];
+                report[3 * rows + 1] = tiles[ti];
+                report[3 * rows + 2] = ms;
+            }
+            ++rows;
+            if (ms > 0.0 && ms < best) { best = ms; *best_mode = modes[mi]; *best_tile = tiles[ti]; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return rows;
 }
 
 int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable) {

@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("dt_kind", C.c_int32), ("dt_fixed", C.c_double), ("cfl", C.c_double),
         ("flags", C.c_uint32), ("n_alpha", C.c_int32), ("alpha", C.c_double * FVH_MAX_ALPHA),
         ("n_bc", C.c_int32), ("bc", BcSpec * FVH_MAX_BC),
-        ("renumber", C.c_int32), ("flux_mode", C.c_int32),
+        ("renumber", C.c_int32), ("flux_mode", C.c_int32), ("tile_cells", C.c_int32),
     ]
 
 
@@ -61,7 +61,7 @@ EXPORTS = [
     "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
-    "fvh_batch_status", "fvh_batch_sums",
+    "fvh_batch_status", "fvh_batch_sums", "fvh_autotune",
 ]
 
 _lib = None
@@ -120,6 +120,8 @@ def load(path: str = LIB_PATH):
     L.fvh_batch_instance.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
     L.fvh_batch_status.argtypes = [vp, C.c_int32, C.POINTER(Status)]
     L.fvh_batch_sums.argtypes = [vp, dp]
+    L.fvh_autotune.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), dp, C.c_int32,
+                               C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, C.c_int32]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

@@ -73,10 +73,11 @@ class Mesh:
 
 
 ORDERS = {"file": 0, "rcm": 1, "tiles": 2}
-FLUX_MODES = {"auto": 0, "cell": 1, "tile": 2}
+FLUX_MODES = {"auto": 0, "cell": 1, "tile": 2, "fused": 3}
 
 
-def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: str = "auto") -> _lib.Config:
+def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: str = "auto",
+                tile_cells: int = 256) -> _lib.Config:
     """fvh_config from the shared configuration dict (see oracle/oracle.py)."""
     c = _lib.Config()
     _lib.load().fvh_config_default(C.byref(c))
@@ -123,6 +124,7 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
         renumber = "file"
     c.renumber = ORDERS[renumber]
     c.flux_mode = FLUX_MODES[flux_mode]
+    c.tile_cells = int(tile_cells)
     return c
 
 
@@ -130,12 +132,12 @@ class Solver:
     """Explicit FV solver on the current HIP device (one stream per handle)."""
 
     def __init__(self, mesh: Mesh, cfg: Dict, precision: str = "f64", renumber="tiles",
-                 flux_mode: str = "auto"):
+                 flux_mode: str = "auto", tile_cells: int = 256):
         self.mesh = mesh
         self.cfg = dict(cfg)
         self.precision = precision
         L = _lib.load()
-        c = make_config(cfg, precision, renumber, flux_mode)
+        c = make_config(cfg, precision, renumber, flux_mode, tile_cells)
         h = C.c_void_p()
         check(L.fvh_solver_create(mesh._h, C.byref(c), C.byref(h)))
         self._h = h
@@ -238,7 +240,7 @@ class Batch(Solver):
     the instances' reference-layout states; ``split``/``join`` convert."""
 
     def __init__(self, meshes: List[Mesh], cfgs: List[Dict], precision: str = "f64", renumber="tiles",
-                 flux_mode: str = "auto"):
+                 flux_mode: str = "auto", tile_cells: int = 256):
         if len(meshes) != len(cfgs) or not meshes:
             raise ValueError("one config per mesh, at least one mesh")
         self.meshes = list(meshes)
@@ -246,7 +248,7 @@ class Batch(Solver):
         self.precision = precision
         L = _lib.load()
         n = len(meshes)
-        cs = (_lib.Config * n)(*[make_config(c, precision, renumber, flux_mode) for c in cfgs])
+        cs = (_lib.Config * n)(*[make_config(c, precision, renumber, flux_mode, tile_cells) for c in cfgs])
         hs = (C.c_void_p * n)(*[m._h for m in meshes])
         h = C.c_void_p()
         check(L.fvh_batch_create(n, hs, cs, C.byref(h)))
@@ -279,3 +281,20 @@ class Batch(Solver):
         out = np.zeros(4 * len(self.meshes))
         check(_lib.load().fvh_batch_sums(self._h, dptr(out)))
         return out.reshape(-1, 4)
+
+
+def autotune(meshes: List[Mesh], cfgs: List[Dict], q: np.ndarray, precision: str = "f64", steps: int = 10,
+             renumber="tiles") -> Dict:
+    """Time every stage form and tile size on these meshes (fvh_autotune)
+    and return {"flux_mode", "tile_cells", "candidates": [(mode, tile, ms/step)]}."""
+    L = _lib.load()
+    n = len(meshes)
+    cs = (_lib.Config * n)(*[make_config(c, precision, renumber) for c in cfgs])
+    hs = (C.c_void_p * n)(*[m._h for m in meshes])
+    q = np.ascontiguousarray(q, dtype=np.float64).ravel()
+    bm, bt = C.c_int32(0), C.c_int32(0)
+    rep = np.zeros(3 * 16)
+    rows = check(L.fvh_autotune(n, hs, cs, dptr(q), int(steps), C.byref(bm), C.byref(bt), dptr(rep), 16))
+    names = {v: k for k, v in FLUX_MODES.items()}
+    cands = [(names[int(rep[3 * k])], int(rep[3 * k + 1]), float(rep[3 * k + 2])) for k in range(min(rows, 16))]
+    return {"flux_mode": names[bm.value], "tile_cells": bt.value, "candidates": cands}

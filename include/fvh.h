@@ -55,10 +55,15 @@ enum { FVH_F64 = 0, FVH_F32 = 1 };
 enum { FVH_ORDER_FILE = 0,       /* reference numbering */
        FVH_ORDER_RCM = 1,        /* reverse Cuthill-McKee */
        FVH_ORDER_TILES = 2       /* recursive coordinate bisection into 256-cell tiles */ };
-/* Flux pass: each tile of 256 cells computes every edge flux it touches once
- * into LDS (TILE), or each cell recomputes the flux of each of its edges
- * (CELL).  AUTO picks TILE. */
-enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2 };
+/* Stage kernels.  Second order runs a gradient/limiter pass and a flux pass
+ * per stage, with the flux pass in one of two forms: each tile of 256 cells
+ * computes every edge flux it touches once into LDS (TILE), or each cell
+ * recomputes the flux of each of its edges (CELL).  FUSED does the whole
+ * stage in one pass per tile, recomputing the gradients of the tile's
+ * neighbour ring in LDS instead of exchanging face states through HBM.
+ * AUTO picks FUSED for linear reconstruction when the halo fits in LDS,
+ * TILE otherwise.  All forms give identical results. */
+enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2, FVH_FLUX_MODE_FUSED = 3 };
 
 /* solver:: flags of fvhyper/explicit.h:35-40 */
 #define FVH_FLAG_GRADIENTS   1u   /* do_calc_gradients */
@@ -97,6 +102,7 @@ typedef struct fvh_config {
     fvh_bc_spec bc[FVH_MAX_BC];
     int32_t renumber;          /* device cell order: FVH_ORDER_* */
     int32_t flux_mode;         /* FVH_FLUX_MODE_* */
+    int32_t tile_cells;        /* cells per tile (workgroup), 32..256; 0 = 256 */
 } fvh_config;
 
 /* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */
@@ -214,6 +220,16 @@ double fvh_solver_stage_bytes(const fvh_solver* s);
  * options).  No reference counterpart: it removes per-launch host overhead
  * on small meshes.  Results are unchanged. */
 int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable);
+
+/* Autotuner (no reference counterpart): times each candidate stage form
+ * (TILE, FUSED, CELL) at each tile size (128, 256 cells) on these meshes and
+ * configs from state q (as for fvh_solver_set_state of the batch) for
+ * `steps` steps, and returns the fastest in *best_mode / *best_tile.
+ * report, if non-NULL, receives up to max_report rows (mode, tile,
+ * ms per step; -1 for a candidate that does not apply); returns the number
+ * of rows.  The configs' own flux_mode and tile_cells are ignored. */
+int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs, const double* q,
+                     int32_t steps, int32_t* best_mode, int32_t* best_tile, double* report, int32_t max_report);
 
 /* ---- batches: many independent meshes advanced together (no reference
  * counterpart; a parameter sweep in the reference is one process per mesh).

@@ -33,12 +33,15 @@ def test_phases_bit_exact(mesh_name, cfg_name, flux_mode):
         assert bits_equal(g[key], o[key]), (key, np.max(np.abs(g[key] - o[key])))
 
 
-@pytest.mark.parametrize("order,flux_mode", [("tiles", "tile"), ("rcm", "tile"), ("file", "cell")])
+@pytest.mark.parametrize("order,flux_mode", [("tiles", "tile"), ("rcm", "tile"), ("file", "cell"), ("tiles", "fused"),
+                                             ("rcm", "fused")])
 @pytest.mark.parametrize("mesh_name", sorted(MESHES))
 @pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
 def test_run_bit_exact(mesh_name, cfg_name, order, flux_mode):
     print("case", mesh_name, cfg_name, order, flux_mode, flush=True)
     cfg = CONFIGS[cfg_name]
+    if flux_mode == "fused" and not cfg.get("lin"):
+        pytest.skip("the fused stage pass exists for linear reconstruction only")
     m, om, pm, s = _pair(mesh_name, cfg, renumber=order, flux_mode=flux_mode)
     if mesh_name == "naca":
         q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
@@ -191,3 +194,44 @@ def test_airfoil_density_stop_matches():
     assert st["stopped"] and st["step"] == steps < 5000
     assert bits_equal(s.get_state(), qo)
     np.testing.assert_allclose(st["R"], R, rtol=1e-10)
+
+
+def test_fused_mode_is_refused_without_reconstruction():
+    from fvhyper_amd import FvhError
+    m = MESHES["sod"]()
+    with pytest.raises(FvhError, match="fused"):
+        Solver(Mesh.from_arrays(m), CONFIGS["first_split_sum"], flux_mode="fused")
+
+
+@pytest.mark.parametrize("flux_mode", ["tile", "fused"])
+@pytest.mark.parametrize("mesh_name,cfg_name", [("step", "second_venkat_global"), ("naca", "second_michalak_local"),
+                                               ("mixed", "second_minmod_farfield")])
+def test_tile_size_128_bit_exact(mesh_name, cfg_name, flux_mode):
+    """128-cell tiles (the autotuner's other size): same bits."""
+    print("case tile128", mesh_name, cfg_name, flux_mode, flush=True)
+    cfg = CONFIGS[cfg_name]
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg, flux_mode=flux_mode, tile_cells=128)
+    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name == "naca" else \
+        random_state(om.n_cells, seed=404, mach=0.3)
+    s.set_state(q0)
+    s.run(max_step=8)
+    qo, *_ = om.run(cfg, q0, 8)
+    assert same_values(s.get_state(), qo)
+
+
+def test_autotune_returns_a_working_choice():
+    from fvhyper_amd import autotune
+    m, cfg, ic = workloads.naca_airfoil(20000)
+    pm = Mesh.from_arrays(m)
+    q0 = workloads.initial_state(pm, ic)
+    r = autotune([pm], [cfg], q0, steps=3)
+    print("autotune", r)
+    assert r["flux_mode"] in ("tile", "fused", "cell") and r["tile_cells"] in (128, 256)
+    assert len(r["candidates"]) == 6 and all(c[2] > 0 for c in r["candidates"])
+    s = Solver(pm, cfg, flux_mode=r["flux_mode"], tile_cells=r["tile_cells"])
+    s.set_state(q0)
+    s.run(max_step=5)
+    qo, *_ = O.Mesh(m).run(cfg, q0, 5)
+    assert bits_equal(s.get_state(), qo)
