@@ -17,10 +17,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfvh.so")
-SOURCES = ["fvh_mesh.cpp", "fvh_kernels.hip", "fvh_solver.hip"]
+SOURCES = ["fvh_mesh.cpp", "fvh_kernels.hip", "fvh_adjoint.hip", "fvh_solver.hip"]
 DROPIN = os.path.join(HERE, "dropin")
 DROPIN_SOURCES = ["fvhyper_dropin.cpp", "probe.cpp"]
 DROPIN_LIB = os.path.join(HERE, "libfvhyper.so")
+CHECK_LIB = os.path.join(HERE, "libfvh_check.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 
@@ -67,6 +68,14 @@ def build(arch: str = "gfx950", jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, DROPIN_LIB)
+    # host-only checks of the physics and its reverse (CPU tests)
+    cobj = compile_one("fvh_check.cpp", arch)
+    tmp = CHECK_LIB + ".tmp"
+    r = subprocess.run([HIPCC, "--offload-arch=%s" % arch, "-shared", "-fPIC", "-o", tmp, cobj],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
+    os.replace(tmp, CHECK_LIB)
     return LIB
 
 

@@ -1,0 +1,579 @@
+// fvh_adjoint.hip -- reverse-mode (discrete adjoint) kernels of one RK stage
+// of the explicit path, and of its time step, initial boundary update and
+// boundary-pressure objective.
+//
+// A stage maps its input state qin (all cells) to its output (real cells:
+// q0 + qt dt alpha; ghosts: boundary functions of the updated interior face
+// states).  Its reverse, given the adjoint of the output, runs four passes:
+//   update  (per real cell)  boundary functions reversed into the cell and
+//                            its boundary increments; adjoint of q0, qt, dt
+//   flux    (per edge)       adjoint of qt to the edge flux, reversed
+//                            through the Roe flux into both face states
+//   gradlim (per real cell)  face states reversed into the cell state, its
+//                            gradient and its limiter; limiter and gradient
+//                            reversed; contributions to neighbours written
+//                            per half-edge
+//   gather  (per cell)       own terms plus the neighbours' half-edge
+//                            contributions, in CSR order
+// No atomics: every sum is a gather in a fixed order, so the adjoint is
+// deterministic.  Forward quantities (stage states, gradients, limiters, qt)
+// come from a replay of the step; comparisons and branches (min/max, |.|,
+// limiter ranges, positivity fallback, boundary regimes) are re-evaluated on
+// them and differentiated piecewise.
+#include "fvh_adjoint.h"
+#include "fvh_device.h"
+
+namespace fvh {
+namespace {
+
+template <class T>
+__device__ inline void q4_add(T* a, const Q4<T>& b) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = a[k] + b.v[k];
+}
+
+// ---- update pass --------------------------------------------------------
+template <class T, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm, PhysParams pp,
+                                                       const BcParam<T>* __restrict__ bcs,
+                                                       const Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ gx,
+                                                       const Q4<T>* __restrict__ gy, const Q4<T>* __restrict__ lm,
+                                                       const Q4<T>* __restrict__ qt, const T* __restrict__ dt,
+                                                       const T* __restrict__ dt_g, T alpha,
+                                                       const Q4<T>* __restrict__ lam_out, Q4<T>* __restrict__ lam_q0,
+                                                       Q4<T>* __restrict__ lam_qt, Q4<T>* __restrict__ lam_bincr,
+                                                       T* __restrict__ lam_dt, double* __restrict__ ld_blk) {
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    const int c = tm.c0[lb] + threadIdx.x;
+    const T g = T(pp.gamma);
+    double ld = 0.0;
+    if (c < tm.c0[lb + 1]) {
+        T lam[4];
+        const Q4<T> lo = lam_out[c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lam[k] = lo.v[k];
+        const Q4<T> out = qout[c];
+        const G2<T> cc = m.c_c[c];
+        for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            if (!(ue & kBcBit)) continue;
+            const int nb = m.he_nbr[h];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            T qi[4];
+            bool fallback = !LIN;
+            if (LIN) {
+                T ax[4], ay[4], al[4];
+                const Q4<T> a = gx[c], b = gy[c], d = lm[c];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) { ax[k] = a.v[k]; ay[k] = b.v[k]; al[k] = d.v[k]; }
+                const Q4<T> inc = face_inc(ax, ay, al, m.e_c[ue & kEdgeMask], cc);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + inc.v[k];
+                if (pp.positivity && !physical(g, qi)) {
+                    fallback = true;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+            }
+            const Q4<T> lg = lam_out[nb];
+            T qb[4] = {T(0), T(0), T(0), T(0)};
+            boundary_vjp(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lg.v, qb);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lam[k] = lam[k] + qb[k];
+            Q4<T> lbi;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lbi.v[k] = fallback ? T(0) : qb[k];
+            lam_bincr[nb - m.n_real] = lbi;
+        }
+        Q4<T> a0 = lam_q0[c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a0.v[k] = a0.v[k] + lam[k];
+        lam_q0[c] = a0;
+        const T d = dt_g ? dt_g[0] : dt[c];
+        const Q4<T> t = qt[c];
+        Q4<T> lq;
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lq.v[k] = lam[k] * d * alpha;
+            s = s + lam[k] * t.v[k] * alpha;
+        }
+        lam_qt[c] = lq;
+        if (dt_g) ld = (double)s;
+        else if (lam_dt) lam_dt[c] = lam_dt[c] + s;
+    }
+    if (ld_blk) {
+        double v[4] = {ld, 0.0, 0.0, 0.0};
+        block_sum4(v);
+        if (threadIdx.x == 0) ld_blk[lb] = v[0];
+    }
+}
+
+// ---- flux pass (one thread per edge) -------------------------------------
+template <class T, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams pp, int n_edges,
+                                                     const int4* __restrict__ epair, const Q4<T>* __restrict__ qin,
+                                                     const Q4<T>* __restrict__ gx, const Q4<T>* __restrict__ gy,
+                                                     const Q4<T>* __restrict__ lm, const Q4<T>* __restrict__ lam_qt,
+                                                     Q4<T>* __restrict__ lam_face) {
+    const int e = blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n_edges) return;
+    const int4 ep = epair[e];   // (half-edge side 0, half-edge side 1, cell side 0, cell side 1)
+    const Q4<T> geo = m.e_geo[e];
+    const G2<T> ec = m.e_c[e];
+    const T g = T(pp.gamma);
+    T fb[4] = {T(0), T(0), T(0), T(0)};
+    if (ep.z < m.n_real) {
+        const Q4<T> l = lam_qt[ep.z];
+        const T A = m.c_area[ep.z];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fb[k] = fb[k] - l.v[k] * geo.v[2] / A;
+    }
+    if (ep.w < m.n_real) {
+        const Q4<T> l = lam_qt[ep.w];
+        const T A = m.c_area[ep.w];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fb[k] = fb[k] + l.v[k] * geo.v[2] / A;
+    }
+    T fs[2][4];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const int c = sd ? ep.w : ep.z;
+        const Q4<T> q = qin[c];
+        if (LIN) {
+            T ax[4], ay[4], al[4];
+            const Q4<T> a = gx[c], b = gy[c], d = lm[c];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { ax[k] = a.v[k]; ay[k] = b.v[k]; al[k] = d.v[k]; }
+            const Q4<T> inc = face_inc(ax, ay, al, ec, m.c_c[c]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fs[sd][k] = q.v[k] + inc.v[k];
+            if (pp.positivity && !physical(g, fs[sd]))
+#pragma unroll
+                for (int k = 0; k < 4; ++k) fs[sd][k] = q.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) fs[sd][k] = q.v[k];
+        }
+    }
+    T qib[4] = {T(0), T(0), T(0), T(0)}, qjb[4] = {T(0), T(0), T(0), T(0)}, nb[2] = {T(0), T(0)};
+    roe_flux_vjp(g, fs[0], fs[1], geo.v[0], geo.v[1], fb, qib, qjb, nb);
+    Q4<T> a, b;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { a.v[k] = qib[k]; b.v[k] = qjb[k]; }
+    lam_face[ep.x] = a;
+    lam_face[ep.y] = b;
+}
+
+// ---- gradient/limiter pass -------------------------------------------------
+template <class T, int PHI, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad,
+                                                        int lim_on, const Q4<T>* __restrict__ qin,
+                                                        const Q4<T>* __restrict__ gxs, const Q4<T>* __restrict__ gys,
+                                                        const Q4<T>* __restrict__ lms,
+                                                        const Q4<T>* __restrict__ lam_face,
+                                                        const Q4<T>* __restrict__ lam_bincr,
+                                                        Q4<T>* __restrict__ lam_direct,
+                                                        Q4<T>* __restrict__ nb_contrib) {
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    const int c = tm.c0[lb] + threadIdx.x;
+    if (c >= tm.c0[lb + 1]) return;
+    const T g = T(pp.gamma);
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    const Q4<T> qc = qin[c];
+    const G2<T> cc = m.c_c[c];
+    T gx[4], gy[4], lim[4];
+    {
+        const Q4<T> a = gxs[c], b = gys[c], d = lms[c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { gx[k] = a.v[k]; gy[k] = b.v[k]; lim[k] = d.v[k]; }
+    }
+    T lqc[4] = {T(0), T(0), T(0), T(0)};
+    T lgx[4] = {T(0), T(0), T(0), T(0)}, lgy[4] = {T(0), T(0), T(0), T(0)}, llim[4] = {T(0), T(0), T(0), T(0)};
+    // face states q + inc (and boundary increments) reversed
+    for (int h = h0; h < h1; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        const G2<T> ec = m.e_c[ue & kEdgeMask];
+        const Q4<T> lf = lam_face[h];
+        q4_add(lqc, lf);
+        nb_contrib[h] = Q4<T>{{T(0), T(0), T(0), T(0)}};
+        if (!LIN) continue;
+        const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
+        T fs[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fs[k] = qc.v[k] + inc.v[k];
+        T li[4];
+        const bool fallback = pp.positivity && !physical(g, fs);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) li[k] = fallback ? T(0) : lf.v[k];
+        if (ue & kBcBit) {
+            const Q4<T> lb2 = lam_bincr[m.he_nbr[h] - m.n_real];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) li[k] = li[k] + lb2.v[k];
+        }
+        const T dx = ec.v[0] - cc.v[0];
+        const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lgx[k] = lgx[k] + li[k] * lim[k] * dx;
+            lgy[k] = lgy[k] + li[k] * lim[k] * dy;
+            llim[k] = llim[k] + li[k] * (gx[k] * dx + gy[k] * dy);
+        }
+    }
+    // limiter reversed (explicit.cpp:174-258)
+    if (lim_on) {
+        // neighbour extrema and the half-edge that sets each (-1: the cell itself)
+        T qmn[4], qmx[4];
+        int hmn[4], hmx[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; hmn[k] = -1; hmx[k] = -1; }
+        for (int h = h0; h < h1; ++h) {
+            const Q4<T> qn = qin[m.he_nbr[h]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (qn.v[k] < qmn[k]) { qmn[k] = qn.v[k]; hmn[k] = h; }
+                if (qmx[k] < qn.v[k]) { qmx[k] = qn.v[k]; hmx[k] = h; }
+            }
+        }
+        const T Ka = T(pp.K) * m.c_sqrt_area[c];
+        const T K3a = Ka * Ka * Ka;
+        const T yt = T(pp.phi_yt);
+        const T tol = T(1e-15);
+        for (int k = 0; k < 4; ++k) {
+            if (llim[k] == T(0)) continue;
+            // the half-edge whose value the running minimum ends on
+            T run = T(1);
+            int hs = -1;
+            for (int h = h0; h < h1; ++h) {
+                const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
+                const T dx = ec.v[0] - cc.v[0];
+                const T dy = ec.v[1] - cc.v[1];
+                const T dqg = gx[k] * dx + gy[k] * dy;
+                const T dmax = qmx[k] - qc.v[k];
+                const T dmin = qmn[k] - qc.v[k];
+                const T spread = (dmax - dmin) * (dmax - dmin);
+                T sig;
+                if (spread <= K3a) sig = T(1);
+                else if (spread <= T(2) * K3a) {
+                    const T y = (spread / K3a - T(1));
+                    sig = T(2) * y * y * y - T(3) * y * y + T(1);
+                } else sig = T(0);
+                T l = T(1);
+                if (sig < T(1)) {
+                    if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
+                    else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+                }
+                l = sig + (T(1) - sig) * l;
+                if (l < run) { run = l; hs = h; }
+            }
+            if (hs < 0) continue;
+            const G2<T> ec = m.e_c[m.he_edge[hs] & kEdgeMask];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+            const T dqg = gx[k] * dx + gy[k] * dy;
+            const T dmax = qmx[k] - qc.v[k];
+            const T dmin = qmn[k] - qc.v[k];
+            const T spread = (dmax - dmin) * (dmax - dmin);
+            T sig, dsig = T(0);
+            if (spread <= K3a) sig = T(1);
+            else if (spread <= T(2) * K3a) {
+                const T y = (spread / K3a - T(1));
+                sig = T(2) * y * y * y - T(3) * y * y + T(1);
+                dsig = (T(6) * y * y - T(6) * y) / K3a;   // d sig / d spread
+            } else sig = T(0);
+            T ph = T(1), dph = T(0), r = T(0);
+            const bool use_max = dqg > tol;
+            if (sig < T(1) && (dqg > tol || dqg < -tol)) {
+                r = (use_max ? dmax : dmin) / dqg;
+                ph = phi<T, PHI>(r, yt);
+                dph = phi_deriv<T, PHI>(r, yt);
+            }
+            const T ll = llim[k];
+            const T lsig = ll * (T(1) - ph);
+            const T lr = ll * (T(1) - sig) * dph;
+            T ldmax = T(0), ldmin = T(0), ldqg = T(0);
+            if (dph != T(0)) {
+                if (use_max) ldmax = lr / dqg; else ldmin = lr / dqg;
+                ldqg = -lr * r / dqg;
+            }
+            const T lspread = lsig * dsig;
+            ldmax = ldmax + T(2) * (dmax - dmin) * lspread;
+            ldmin = ldmin - T(2) * (dmax - dmin) * lspread;
+            lgx[k] = lgx[k] + ldqg * dx;
+            lgy[k] = lgy[k] + ldqg * dy;
+            lqc[k] = lqc[k] - ldmax - ldmin;
+            if (hmx[k] < 0) lqc[k] = lqc[k] + ldmax;
+            else { Q4<T> t = nb_contrib[hmx[k]]; t.v[k] = t.v[k] + ldmax; nb_contrib[hmx[k]] = t; }
+            if (hmn[k] < 0) lqc[k] = lqc[k] + ldmin;
+            else { Q4<T> t = nb_contrib[hmn[k]]; t.v[k] = t.v[k] + ldmin; nb_contrib[hmn[k]] = t; }
+        }
+    }
+    // Green-Gauss gradient reversed (explicit.cpp:112-171)
+    if (grad) {
+        const T ia = m.c_inv_area[c];
+        T sx[4], sy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { sx[k] = lgx[k] * ia; sy[k] = lgy[k] * ia; }
+        for (int h = h0; h < h1; ++h) {
+            const int nb = m.he_nbr[h];
+            if (nb == c) continue;
+            const uint32_t ue = m.he_edge[h];
+            const bool side = (ue & kSideBit) != 0;
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            const T w = geo.v[3];
+            const T w1 = T(1) - w;
+            Q4<T> t = nb_contrib[h];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                T lf = sx[k] * geo.v[0] + sy[k] * geo.v[1];
+                if (side) lf = -lf;
+                lf = lf * geo.v[2];
+                // f = (qi w1 + qj w) len with qi the side-0 cell
+                lqc[k] = lqc[k] + lf * (side ? w : w1);
+                t.v[k] = t.v[k] + lf * (side ? w1 : w);
+            }
+            nb_contrib[h] = t;
+        }
+    }
+    Q4<T> o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o.v[k] = lqc[k];
+    lam_direct[c] = o;
+}
+
+// ---- gather pass ---------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<T> m, const Q4<T>* __restrict__ lam_direct,
+                                                       const Q4<T>* __restrict__ lam_face,
+                                                       const Q4<T>* __restrict__ nb_contrib, Q4<T>* __restrict__ out,
+                                                       int accumulate) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    T v[4] = {T(0), T(0), T(0), T(0)};
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    if (c < m.n_real) {
+        q4_add(v, lam_direct[c]);
+    } else {
+        // a ghost's face state is the ghost state
+        for (int h = h0; h < h1; ++h) q4_add(v, lam_face[h]);
+    }
+    for (int h = h0; h < h1; ++h)
+        if (m.he_nbr[h] < m.n_real) q4_add(v, nb_contrib[m.he_twin[h]]);
+    Q4<T> o = accumulate ? out[c] : Q4<T>{{T(0), T(0), T(0), T(0)}};
+    q4_add(o.v, Q4<T>{{v[0], v[1], v[2], v[3]}});
+    out[c] = o;
+}
+
+// ---- time step reversed for one cell (the minimum of a global step) --------
+// summed form: dt = cfl A / S, S = sum_h max(e_own, e_nbr) len_h
+// edge-minimum form: dt = min(1, min_h cfl (A / len_h) / |e_h|)
+template <class T, int KIND>
+__global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q, int c,
+                              const double* __restrict__ ld_blk, int n_blk, Q4<T>* __restrict__ lam_q0) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double sacc = 0.0;
+    for (int i = 0; i < n_blk; ++i) sacc += ld_blk[i];
+    const T lam_d = T(sacc);
+    if (KIND == 0 || lam_d == T(0)) return;
+    const T g = T(pp.gamma);
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    const Q4<T> qc = q[c];
+    if (KIND == 1) {
+        T S = T(0);
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            const Q4<T> qn = q[m.he_nbr[h]];
+            const bool side = (ue & kSideBit) != 0;
+            const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
+            const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+            S = S + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+        }
+        const T dtc = T(pp.cfl) * m.c_area[c] / S;
+        const T Sb = -lam_d * dtc / S;
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            const int nb = m.he_nbr[h];
+            const Q4<T> qn = q[nb];
+            const bool side = (ue & kSideBit) != 0;
+            const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
+            const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+            // which term the forward max kept: smax(a, b) = (a < b) ? b : a,
+            // taken as smax(eo, en) on side 0 and smax(en, eo) on side 1
+            const bool own = side ? (en < eo) : !(eo < en);
+            const int who = own ? c : nb;
+            T qb[4] = {T(0), T(0), T(0), T(0)};
+            eig_sum_vjp(g, own ? qc.v : qn.v, geo.v[0], geo.v[1], Sb * geo.v[2], qb);
+            Q4<T> l = lam_q0[who];
+            for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + qb[k];
+            lam_q0[who] = l;
+        }
+    } else {
+        T r = T(1);
+        int hs = -1;
+        for (int h = h0; h < h1; ++h) {
+            const Q4<T> geo = m.e_geo[m.he_edge[h] & kEdgeMask];
+            const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+            const T val = T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo);
+            if (val < r) { r = val; hs = h; }
+        }
+        if (hs < 0) return;
+        const Q4<T> geo = m.e_geo[m.he_edge[hs] & kEdgeMask];
+        const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+        // val = K / |e|: d val / d e = -val sign(e) / |e|
+        const T eb = -lam_d * r / sabs(eo) * ((eo < T(0)) ? T(-1) : T(1));
+        T qb[4] = {T(0), T(0), T(0), T(0)};
+        eig_edge_vjp(g, qc.v, geo.v[0], geo.v[1], eb, qb);
+        Q4<T> l = lam_q0[c];
+        for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + qb[k];
+        lam_q0[c] = l;
+    }
+}
+
+// ---- initial boundary update reversed (explicit.cpp:712) -------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_bounds_init(MeshView<T> m, PhysParams pp,
+                                                            const BcParam<T>* __restrict__ bcs,
+                                                            const Q4<T>* __restrict__ q, Q4<T>* __restrict__ lam) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_real) return;
+    const T g = T(pp.gamma);
+    const Q4<T> qc = q[c];
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    bool any = false;
+    for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        if (!(ue & kBcBit)) continue;
+        const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+        const Q4<T> lg = lam[m.he_nbr[h]];
+        boundary_vjp(bcs[m.he_bc[h]], g, qc.v, geo.v[0], geo.v[1], lg.v, acc);
+        any = true;
+    }
+    if (any) {
+        Q4<T> l = lam[c];
+        q4_add(l.v, Q4<T>{{acc[0], acc[1], acc[2], acc[3]}});
+        lam[c] = l;
+    }
+}
+
+// ---- objective: J = sum_b p(ghost_b) (wx_b n_x + wy_b n_y) len_b -----------
+// obj[b] = (ghost device cell, nx, ny, len*wx... ) packed as Q4: (nx, ny, len, unused) + weights
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_objective(int n_b, PhysParams pp, const int32_t* __restrict__ ghost,
+                                                      const Q4<T>* __restrict__ geo_w, const Q4<T>* __restrict__ q,
+                                                      double* __restrict__ terms, Q4<T>* __restrict__ lam) {
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= n_b) return;
+    const Q4<T> gw = geo_w[b];   // (wx, wy, nx, ny) ... len folded into the weights
+    const int gc = ghost[b];
+    const Q4<T> qg = q[gc];
+    const T g = T(pp.gamma);
+    const T proj = gw.v[0] * gw.v[2] + gw.v[1] * gw.v[3];
+    terms[b] = (double)(pressure(g, qg.v) * proj);
+    if (lam) {
+        T l[4] = {T(0), T(0), T(0), T(0)};
+        pressure_vjp(g, qg.v, proj, l);
+        lam[gc] = Q4<T>{{l[0], l[1], l[2], l[3]}};
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_fill(Q4<T>* __restrict__ a, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = Q4<T>{{T(0), T(0), T(0), T(0)}};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <class T>
+void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
+                      bool grad, bool lim, bool lin, const BcParam<T>* bcs, int n_edges, const int4* epair,
+                      const Q4<T>* qin, const Q4<T>* qout, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lm,
+                      const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
+                      Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
+                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk) {
+    const int nrt = xcd_grid_blocks(tm.n_rt);
+    if (nrt == 0) return;
+    if (lin)
+        hipLaunchKernelGGL((k_adj_update<T, true>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, gx, gy, lm,
+                           qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt, ld_blk);
+    else
+        hipLaunchKernelGGL((k_adj_update<T, false>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, gx, gy, lm,
+                           qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt, ld_blk);
+    const int ne = nblocks(n_edges);
+    if (lin)
+        hipLaunchKernelGGL((k_adj_flux<T, true>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, qin, gx, gy, lm,
+                           (const Q4<T>*)lam_qt, lam_face);
+    else
+        hipLaunchKernelGGL((k_adj_flux<T, false>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, qin, gx, gy,
+                           lm, (const Q4<T>*)lam_qt, lam_face);
+#define FVH_AG(P, L) hipLaunchKernelGGL((k_adj_gradlim<T, P, L>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, (int)grad, \
+                                        (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face, (const Q4<T>*)lam_bincr,    \
+                                        lam_direct, nb_contrib)
+#define FVH_AGL(P) do { if (lin) FVH_AG(P, true); else FVH_AG(P, false); } while (0)
+    switch (phi_kind) {
+        case PHI_ZERO: FVH_AGL(PHI_ZERO); break;
+        case PHI_MICHALAK: FVH_AGL(PHI_MICHALAK); break;
+        case PHI_VENKAT: FVH_AGL(PHI_VENKAT); break;
+        default: FVH_AGL(PHI_MINMOD); break;
+    }
+#undef FVH_AGL
+#undef FVH_AG
+    hipLaunchKernelGGL((k_adj_gather<T>), dim3(nblocks(m.n_total)), dim3(kBlock), 0, s, m,
+                       (const Q4<T>*)lam_direct, (const Q4<T>*)lam_face, (const Q4<T>*)nb_contrib, lam_in,
+                       accumulate_in);
+}
+
+template <class T>
+void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q, int cell,
+                   const double* ld_blk, int n_blk, Q4<T>* lam_q0) {
+    if (dt_kind == 1)
+        hipLaunchKernelGGL((k_adj_dt_cell<T, 1>), dim3(1), dim3(64), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
+    else if (dt_kind == 2)
+        hipLaunchKernelGGL((k_adj_dt_cell<T, 2>), dim3(1), dim3(64), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
+}
+
+template <class T>
+void launch_adj_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs,
+                            const Q4<T>* q, Q4<T>* lam) {
+    if (m.n_real == 0) return;
+    hipLaunchKernelGGL((k_adj_bounds_init<T>), dim3(nblocks(m.n_real)), dim3(kBlock), 0, s, m, pp, bcs, q, lam);
+}
+
+template <class T>
+void launch_objective(hipStream_t s, int n_b, const PhysParams& pp, const int32_t* ghost, const Q4<T>* geo_w,
+                      const Q4<T>* q, double* terms, Q4<T>* lam) {
+    if (n_b == 0) return;
+    hipLaunchKernelGGL((k_objective<T>), dim3(nblocks(n_b)), dim3(kBlock), 0, s, n_b, pp, ghost, geo_w, q, terms, lam);
+}
+
+template <class T>
+void launch_fill(hipStream_t s, Q4<T>* a, int n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(n)), dim3(kBlock), 0, s, a, n);
+}
+
+template void launch_adj_stage<double>(hipStream_t, const MeshView<double>&, const TileMap&, const PhysParams&, int,
+                                       bool, bool, bool, const BcParam<double>*, int, const int4*,
+                                       const Q4<double>*, const Q4<double>*, const Q4<double>*, const Q4<double>*,
+                                       const Q4<double>*, const Q4<double>*, const double*, const double*, double,
+                                       const Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*,
+                                       Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*);
+template void launch_adj_dt<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int, const Q4<double>*,
+                                    int, const double*, int, Q4<double>*);
+template void launch_adj_bounds_init<double>(hipStream_t, const MeshView<double>&, const PhysParams&,
+                                             const BcParam<double>*, const Q4<double>*, Q4<double>*);
+template void launch_objective<double>(hipStream_t, int, const PhysParams&, const int32_t*, const Q4<double>*,
+                                       const Q4<double>*, double*, Q4<double>*);
+template void launch_fill<double>(hipStream_t, Q4<double>*, int);
+
+}  // namespace fvh

@@ -1,65 +1,11 @@
 // fvh_kernels.hip -- gfx950 kernels of the explicit residual-and-update path.
 // Built with -ffp-contract=off: no multiply-add is fused, so each FP64
 // expression rounds exactly as the reference's x86-64 build does.
-#include "fvh_kernels.h"
+#include "fvh_device.h"
 
 namespace fvh {
 
 namespace {
-
-constexpr double kInf = __builtin_inf();
-
-__device__ inline bool stopped(const Ctrl* ctrl, int inst) { return ctrl != nullptr && ctrl[inst].stop != 0; }
-
-// NaN-ignoring minimum over a 256-thread block (std::min(a,b) keeps a unless
-// b < a, so NaN never wins once a finite value is held).
-__device__ inline double block_min(double v) {
-    for (int off = 32; off > 0; off >>= 1) v = smin(v, __shfl_xor(v, off, 64));
-    __shared__ double part[kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) part[wave] = v;
-    __syncthreads();
-    double r = part[0];
-    for (int w = 1; w < kBlock / 64; ++w) r = smin(r, part[w]);
-    return r;
-}
-
-// Fixed-pattern sum of 4 values over an NB-thread block (deterministic).
-template <int NB = kBlock>
-__device__ inline void block_sum4(double* v) {
-    for (int k = 0; k < 4; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
-    __shared__ double part[NB / 64][4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int k = 0; k < 4; ++k) part[wave][k] = v[k];
-    __syncthreads();
-    for (int k = 0; k < 4; ++k) {
-        double s = part[0][k];
-        for (int w = 1; w < NB / 64; ++w) s += part[w][k];
-        v[k] = s;
-    }
-}
-
-template <class T>
-__device__ inline Q4<T> ld(const Q4<T>* p) { return *p; }
-
-// XCD-aware block order.  The dispatcher hands workgroup b to XCD b % 8, and
-// each XCD has its own L2; remapping hardware block b to logical block
-// (b % 8) * per + b / 8 gives every XCD one contiguous range of cells, so a
-// cell's neighbours (close in the renumbered order) sit in the same L2.
-constexpr int kXcd = 8;
-__device__ inline int xcd_block(int nblk) {
-    const int per = (nblk + kXcd - 1) / kXcd;
-    const int b = blockIdx.x;
-    const int lb = (b % kXcd) * per + b / kXcd;
-    return lb < nblk ? lb : -1;
-}
-
-template <class T>
-__device__ inline bool physical(T g, const T* q) {
-    return q[0] > T(0) && pressure(g, q) > T(0);
-}
 
 // ---------------------------------------------------------------------------
 // time step: calc_dt kinds + block minimum for the global-dt reduction
@@ -241,18 +187,6 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
             }
         }
     }
-}
-
-// Limited linear increment (grad q . d) phi at an edge centre (explicit.cpp:307-313).
-template <class T>
-__device__ inline Q4<T> face_inc(const T (&gx)[4], const T (&gy)[4], const T (&lim)[4], const G2<T>& ec,
-                                 const G2<T>& cc) {
-    const T dx = ec.v[0] - cc.v[0];
-    const T dy = ec.v[1] - cc.v[1];
-    Q4<T> inc;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) inc.v[k] = (gx[k] * dx + gy[k] * dy) * lim[k];
-    return inc;
 }
 
 // Two-pass form, pass 1: gradient + limiter + face states of every cell of
@@ -740,10 +674,6 @@ __global__ __launch_bounds__(kBlock) void k_sums_inst(InstView iv, const double*
         for (int k = 0; k < 4; ++k) out[4 * i + k] = s[k];
 }
 
-inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
-// hardware grid for xcd_block(): a multiple of the XCD count
-inline int xcd_grid_blocks(int nb) { return ((nb + kXcd - 1) / kXcd) * kXcd; }
-inline int xcd_grid(int n) { return xcd_grid_blocks(nblocks(n)); }
 
 }  // namespace
 

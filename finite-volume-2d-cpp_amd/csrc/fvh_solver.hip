@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -55,6 +56,8 @@ struct DevBuf {
         return hipMalloc(&p, n) == hipSuccess ? 0 : -1;
     }
 };
+
+constexpr int kAdjMaxStages = FVH_MAX_ALPHA;
 
 // One captured RK step, replayed with hipGraphLaunch.
 struct StepGraph {
@@ -97,6 +100,15 @@ struct fvh_solver {
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
     int cur = 0;                          // index of the current state buffer
+    // discrete adjoint: per device edge (half-edge side 0, side 1, cell side 0,
+    // cell side 1); boundary element geometry (instance 0); work buffers
+    fvh::DevBuf epair;
+    std::vector<double> b_nx, b_ny, b_len;
+    fvh::DevBuf a_qs[fvh::kAdjMaxStages + 1], a_gx[fvh::kAdjMaxStages], a_gy[fvh::kAdjMaxStages],
+        a_lm[fvh::kAdjMaxStages], a_qt[fvh::kAdjMaxStages];
+    fvh::DevBuf a_lam, a_lam_in, a_lam_q0, a_lam_qt, a_lam_bincr, a_lam_face, a_lam_direct, a_nb, a_ld, a_tmp;
+    fvh::DevBuf a_obj_ghost, a_obj_gw, a_obj_terms;
+    std::vector<std::unique_ptr<fvh::DevBuf>> a_ck;
     // graphs
     bool use_graphs = false;
     std::vector<fvh::StepGraph> graphs;
@@ -958,6 +970,23 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         if (he_bc[h] >= 0) w |= kBcBit;
         he_edge[h] = w;
     }
+    {
+        std::vector<int4> ep(E, make_int4(-1, -1, -1, -1));
+        for (int32_t d = 0; d < N; ++d)
+            for (int32_t h = off[d]; h < off[d + 1]; ++h) {
+                const uint32_t ue = he_edge[h];
+                int4& x = ep[ue & kEdgeMask];
+                if (ue & kSideBit) { x.y = h; x.w = d; } else { x.x = h; x.z = d; }
+            }
+        if ((rc = upload(s->epair, ep.data(), ep.size() * sizeof(int4)))) return rc;
+        const fvh_mesh& m0 = *ms[0];
+        for (size_t b = 0; b < m0.bnd_edge.size(); ++b) {
+            const int32_t e = m0.bnd_edge[b];
+            s->b_nx.push_back(m0.e_nx[e]);
+            s->b_ny.push_back(m0.e_ny[e]);
+            s->b_len.push_back(m0.e_len[e]);
+        }
+    }
     for (int32_t i = 0; i < n; ++i) {
         const fvh_mesh& m = *ms[i];
         const int64_t st0 = s->i_state0[i];
@@ -1040,6 +1069,230 @@ int32_t sums_t(fvh_solver* s, double* out) {
     FVH_HIP(hipGetLastError());
     FVH_HIP(hipStreamSynchronize(s->stream));
     FVH_HIP(hipMemcpy(out, s->sums.p, (size_t)s->n_inst * 32, hipMemcpyDeviceToHost));
+    return FVH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// discrete adjoint driver (FP64, single instance)
+// ---------------------------------------------------------------------------
+struct Adjoint {
+    fvh_solver* s;
+    int n_stage = 0;
+    size_t qbytes = 0;
+    int64_t fwd_steps = 0;
+    std::vector<double> dt_host;
+
+    int32_t alloc(int snaps) {
+        const size_t N = (size_t)s->n_total;
+        qbytes = N * sizeof(Q4<double>);
+        n_stage = s->cfg.n_alpha;
+        for (int k = 0; k <= n_stage; ++k)
+            if (!s->a_qs[k].p && s->a_qs[k].alloc(qbytes)) return oom();
+        for (int k = 0; k < n_stage; ++k)
+            if ((!s->a_gx[k].p && s->a_gx[k].alloc(qbytes)) || (!s->a_gy[k].p && s->a_gy[k].alloc(qbytes)) ||
+                (!s->a_lm[k].p && s->a_lm[k].alloc(qbytes)) || (!s->a_qt[k].p && s->a_qt[k].alloc(qbytes)))
+                return oom();
+        const size_t he = (size_t)std::max(s->n_he, 1) * sizeof(Q4<double>);
+        if ((!s->a_lam.p && s->a_lam.alloc(qbytes)) || (!s->a_lam_in.p && s->a_lam_in.alloc(qbytes)) ||
+            (!s->a_lam_q0.p && s->a_lam_q0.alloc(qbytes)) || (!s->a_lam_qt.p && s->a_lam_qt.alloc(qbytes)) ||
+            (!s->a_lam_bincr.p && s->a_lam_bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<double>))) ||
+            (!s->a_lam_face.p && s->a_lam_face.alloc(he)) || (!s->a_nb.p && s->a_nb.alloc(he)) ||
+            (!s->a_lam_direct.p && s->a_lam_direct.alloc(qbytes)) ||
+            (!s->a_ld.p && s->a_ld.alloc((size_t)(kAdjMaxStages * (s->n_rt + 1)) * 8)) ||
+            (!s->a_tmp.p && s->a_tmp.alloc(qbytes)))
+            return oom();
+        while ((int)s->a_ck.size() < snaps) {
+            std::unique_ptr<DevBuf> b(new DevBuf());
+            if (b->alloc(qbytes)) return oom();
+            s->a_ck.push_back(std::move(b));
+        }
+        return FVH_OK;
+    }
+    int32_t oom() {
+        set_error("hipMalloc failed (adjoint buffers)");
+        return FVH_E_HIP;
+    }
+    double work_bytes() const {
+        return (double)qbytes * (n_stage + 1 + 4 * n_stage + 7) +
+               2.0 * (double)std::max(s->n_he, 1) * sizeof(Q4<double>);
+    }
+
+    // k forward steps from state src into state dst (the fast path)
+    int32_t advance(const void* src, void* dst, int k) {
+        FVH_HIP(hipMemcpyAsync(s->q[s->cur].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
+        for (int i = 0; i < k; ++i) {
+            int32_t rc = enqueue_step<double>(s, false, &o, 0);
+            if (rc) return rc;
+        }
+        fwd_steps += k;
+        FVH_HIP(hipMemcpyAsync(dst, s->q[s->cur].p, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        return FVH_OK;
+    }
+
+    // one step from src, keeping every stage's input, gradient, limiter and qt
+    int32_t replay(const void* src) {
+        const fvh_config& c = s->cfg;
+        const MeshView<double> m = view<double>(s);
+        const TileMap tm = tile_map(s);
+        const InstView iv = inst_view(s);
+        const TileView tv = tile_view(s);
+        const PhysParams pp = phys(c);
+        const BcParam<double>* bcs = (const BcParam<double>*)s->bcs.p;
+        const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
+                   lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
+        hipStream_t st = s->stream;
+        FVH_HIP(hipMemcpyAsync(s->a_qs[0].p, src, qbytes, hipMemcpyDeviceToDevice, st));
+        double* dt = (double*)s->dt.p;
+        double* blk = (double*)s->blk_min.p;
+        if (!s->second) launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)s->a_qs[0].p, dt, blk, nullptr);
+        for (int k = 0; k < n_stage; ++k) {
+            const Q4<double>* qin = (const Q4<double>*)s->a_qs[k].p;
+            if (s->second)
+                launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, lin, qin, (Q4<double>*)s->face.p,
+                                       (Q4<double>*)s->bincr.p, k == 0 ? c.dt_kind : -1, dt, blk,
+                                       (Q4<double>*)s->a_gx[k].p, (Q4<double>*)s->a_gy[k].p,
+                                       (Q4<double>*)s->a_lm[k].p, nullptr);
+            if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, s->dt_global.p, nullptr);
+            launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
+                                qin, (const Q4<double>*)s->a_qs[0].p, (Q4<double>*)s->a_qs[k + 1].p,
+                                (const Q4<double>*)s->face.p, (const Q4<double>*)s->bincr.p, dt,
+                                gdt ? (const double*)s->dt_global.p : nullptr, c.alpha[k], nullptr,
+                                (Q4<double>*)s->a_qt[k].p, nullptr);
+        }
+        FVH_HIP(hipGetLastError());
+        fwd_steps += 1;
+        return FVH_OK;
+    }
+
+    // reverse of the replayed step: a_lam (adjoint of the step's output) ->
+    // a_lam (adjoint of its input)
+    int32_t reverse() {
+        const fvh_config& c = s->cfg;
+        const MeshView<double> m = view<double>(s);
+        const TileMap tm = tile_map(s);
+        const PhysParams pp = phys(c);
+        const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
+                   lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
+        hipStream_t st = s->stream;
+        Q4<double>* lam = (Q4<double>*)s->a_lam.p;
+        Q4<double>* lam_in = (Q4<double>*)s->a_lam_in.p;
+        Q4<double>* lam_q0 = (Q4<double>*)s->a_lam_q0.p;
+        launch_fill<double>(st, lam_q0, s->n_total);
+        for (int k = n_stage - 1; k >= 0; --k) {
+            launch_adj_stage<double>(st, m, tm, pp, c.phi, grad, lim, lin, (const BcParam<double>*)s->bcs.p,
+                                     s->n_edges, (const int4*)s->epair.p, (const Q4<double>*)s->a_qs[k].p,
+                                     (const Q4<double>*)s->a_qs[k + 1].p, (const Q4<double>*)s->a_gx[k].p,
+                                     (const Q4<double>*)s->a_gy[k].p, (const Q4<double>*)s->a_lm[k].p,
+                                     (const Q4<double>*)s->a_qt[k].p, (const double*)s->dt.p,
+                                     gdt ? (const double*)s->dt_global.p : nullptr, c.alpha[k],
+                                     k == n_stage - 1 ? lam : lam_in, lam_q0, (Q4<double>*)s->a_lam_qt.p,
+                                     (Q4<double>*)s->a_lam_bincr.p, (Q4<double>*)s->a_lam_face.p,
+                                     (Q4<double>*)s->a_lam_direct.p, (Q4<double>*)s->a_nb.p,
+                                     k == 0 ? lam_q0 : lam_in, k == 0 ? 1 : 0, nullptr,
+                                     (double*)s->a_ld.p + (size_t)k * s->n_rt);
+        }
+        if (gdt && c.dt_kind != FVH_DT_FIXED) {
+            // the cell whose step is the minimum: first in reference order (min_dt, explicit.cpp:555-564)
+            dt_host.resize(s->n_total);
+            FVH_HIP(hipMemcpyAsync(dt_host.data(), s->dt.p, dt_host.size() * 8, hipMemcpyDeviceToHost, st));
+            FVH_HIP(hipStreamSynchronize(st));
+            int32_t best = -1;
+            double bv = 0.0;
+            int32_t best_ref = INT32_MAX;
+            for (int32_t d = 0; d < s->n_total; ++d) {
+                const double v = dt_host[d];
+                if (best < 0 || v < bv || (v == bv && s->inv[d] < best_ref)) { best = d; bv = v; best_ref = s->inv[d]; }
+            }
+            if (best >= 0)
+                launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)s->a_qs[0].p, best,
+                                      (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0);
+        }
+        FVH_HIP(hipGetLastError());
+        std::swap(s->a_lam.p, s->a_lam_q0.p);
+        return FVH_OK;
+    }
+
+    // binomial checkpointing: the state of step a is in slot sa; a_lam holds
+    // the adjoint of state b; on return it holds the adjoint of state a
+    static double beta(int sn, int r) {   // C(sn + r, sn)
+        double v = 1.0;
+        for (int i = 1; i <= sn; ++i) v = v * (double)(r + i) / (double)i;
+        return v;
+    }
+    int32_t rev(int a, int b, int sa, std::vector<int>& free_slots) {
+        const int n = b - a;
+        int32_t rc;
+        if (n <= 0) return FVH_OK;
+        if (n == 1) {
+            if ((rc = replay(s->a_ck[sa]->p))) return rc;
+            return reverse();
+        }
+        if (free_slots.empty()) {
+            for (int k = b - 1; k >= a; --k) {
+                if ((rc = advance(s->a_ck[sa]->p, s->a_tmp.p, k - a))) return rc;
+                if ((rc = replay(s->a_tmp.p))) return rc;
+                if ((rc = reverse())) return rc;
+            }
+            return FVH_OK;
+        }
+        const int sn = (int)free_slots.size();
+        int r = 0;
+        while (beta(sn, r) < (double)n) ++r;
+        int mstep = n - (int)beta(sn - 1, r);
+        mstep = std::max(1, std::min(n - 1, mstep));
+        const int t = free_slots.back();
+        free_slots.pop_back();
+        if ((rc = advance(s->a_ck[sa]->p, s->a_ck[t]->p, mstep))) return rc;
+        if ((rc = rev(a + mstep, b, t, free_slots))) return rc;
+        free_slots.push_back(t);
+        return rev(a, a + mstep, sa, free_slots);
+    }
+};
+
+// Boundary-element data of the objective, uploaded per call.
+int32_t objective_setup(fvh_solver* s, const double* wx, const double* wy) {
+    const int32_t nb = s->i_nghost.empty() ? 0 : s->i_nghost[0];
+    std::vector<int32_t> ghost(nb + 1, 0);
+    std::vector<Q4<double>> gw(nb + 1);
+    for (int32_t b = 0; b < nb; ++b) {
+        ghost[b] = s->i_ghost0[0] + b;
+        gw[b].v[0] = (wx ? wx[b] : 0.0) * s->b_len[b];
+        gw[b].v[1] = (wy ? wy[b] : 0.0) * s->b_len[b];
+        gw[b].v[2] = s->b_nx[b];
+        gw[b].v[3] = s->b_ny[b];
+    }
+    int32_t rc;
+    if ((rc = upload(s->a_obj_ghost, ghost.data(), ghost.size() * 4))) return rc;
+    if ((rc = upload(s->a_obj_gw, gw.data(), gw.size() * sizeof(Q4<double>)))) return rc;
+    if (!s->a_obj_terms.p && s->a_obj_terms.alloc((size_t)(nb + 1) * 8)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+    return FVH_OK;
+}
+
+int32_t objective_eval(fvh_solver* s, const void* q, bool seed, double* J) {
+    const int32_t nb = s->i_nghost[0];
+    if (seed) launch_fill<double>(s->stream, (Q4<double>*)s->a_lam.p, s->n_total);
+    launch_objective<double>(s->stream, nb, phys(s->cfg), (const int32_t*)s->a_obj_ghost.p,
+                             (const Q4<double>*)s->a_obj_gw.p, (const Q4<double>*)q, (double*)s->a_obj_terms.p,
+                             seed ? (Q4<double>*)s->a_lam.p : nullptr);
+    FVH_HIP(hipGetLastError());
+    std::vector<double> t(nb);
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    if (nb) FVH_HIP(hipMemcpy(t.data(), s->a_obj_terms.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
+    double sum = 0.0;
+    for (int32_t b = 0; b < nb; ++b) sum += t[b];
+    *J = sum;
+    return FVH_OK;
+}
+
+int32_t adjoint_check(fvh_solver* s) {
+    if (s->f32) { set_error("the adjoint is FP64 only"); return FVH_E_UNSUPPORTED; }
+    if (s->n_inst != 1) { set_error("the adjoint takes a single-mesh solver"); return FVH_E_UNSUPPORTED; }
+    const fvh_config& c = s->cfg;
+    if (!(c.flags & FVH_FLAG_GLOBAL_DT) && c.dt_kind != FVH_DT_FIXED) {
+        set_error("the adjoint needs a global or fixed time step");
+        return FVH_E_UNSUPPORTED;
+    }
     return FVH_OK;
 }
 
@@ -1154,6 +1407,68 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     return rows;
+}
+
+int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
+                      double* J) {
+    if (!s || !q || !J || n_steps < 0) { set_error("bad argument"); return FVH_E_ARG; }
+    int32_t rc = adjoint_check(s);
+    if (rc) return rc;
+    if ((rc = objective_setup(s, wx, wy))) return rc;
+    s->cur = 0;
+    if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
+    fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
+    for (int32_t k = 0; k < n_steps; ++k)
+        if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
+    return objective_eval(s, s->q[s->cur].p, false, J);
+}
+
+int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
+                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, fvh_adjoint_stats* st) {
+    if (!s || !q || !J || !grad || n_steps < 0 || snaps < 0) { set_error("bad argument"); return FVH_E_ARG; }
+    int32_t rc = adjoint_check(s);
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    Adjoint A{s};
+    A.qbytes = (size_t)s->n_total * sizeof(Q4<double>);
+    A.n_stage = s->cfg.n_alpha;
+    if (snaps == 0) {
+        const double cap = mem_cap_bytes > 0 ? mem_cap_bytes : 8e9;
+        const double avail = cap - A.work_bytes();
+        snaps = (int32_t)std::max(1.0, std::min((double)std::max(n_steps, 1), std::floor(avail / A.qbytes)));
+    }
+    snaps = std::max(1, std::min(snaps, std::max(n_steps, 1)));
+    if ((rc = A.alloc(snaps)) || (rc = objective_setup(s, wx, wy))) return rc;
+    // initial state: bounds applied (explicit.cpp:712), kept in slot 0
+    s->cur = 0;
+    if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
+    FVH_HIP(hipMemcpyAsync(s->a_ck[0]->p, s->q[s->cur].p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+    // forward to the objective, then seed the adjoint of the final state
+    if ((rc = A.advance(s->a_ck[0]->p, s->a_tmp.p, n_steps))) return rc;
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    if ((rc = objective_eval(s, s->a_tmp.p, true, J))) return rc;
+    std::vector<int> free_slots;
+    for (int k = snaps - 1; k >= 1; --k) free_slots.push_back(k);
+    if ((rc = A.rev(0, n_steps, 0, free_slots))) return rc;
+    launch_adj_bounds_init<double>(s->stream, view<double>(s), phys(s->cfg), (const BcParam<double>*)s->bcs.p,
+                                   (const Q4<double>*)s->a_ck[0]->p, (Q4<double>*)s->a_lam.p);
+    FVH_HIP(hipGetLastError());
+    if ((rc = get_q4<double>(s, s->a_lam.p, grad, s->n_total))) return rc;
+    // ghost entries: run() overwrites them, so J does not depend on them
+    for (int32_t g = s->n_real; g < s->n_total; ++g)
+        for (int k = 0; k < 4; ++k) grad[4 * (int64_t)s->inv[g] + k] = 0.0;
+    const auto t2 = std::chrono::steady_clock::now();
+    if (st) {
+        st->steps = n_steps;
+        st->snaps = snaps;
+        st->forward_steps = A.fwd_steps;
+        st->checkpoint_bytes = (double)snaps * A.qbytes;
+        st->work_bytes = A.work_bytes();
+        st->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    }
+    return FVH_OK;
 }
 
 int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable) {

@@ -47,6 +47,12 @@ class RunOptions(C.Structure):
                 ("stop_var", C.c_int32), ("reserved", C.c_int32)]
 
 
+class AdjointStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("snaps", C.c_int32), ("forward_steps", C.c_int64),
+                ("checkpoint_bytes", C.c_double), ("work_bytes", C.c_double),
+                ("forward_ms", C.c_double), ("total_ms", C.c_double)]
+
+
 class Status(C.Structure):
     _fields_ = [("step", C.c_uint32), ("stopped", C.c_uint32), ("time", C.c_double),
                 ("time_prev", C.c_double), ("R", C.c_double * 4), ("R0", C.c_double * 4)]
@@ -61,7 +67,7 @@ EXPORTS = [
     "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
-    "fvh_batch_status", "fvh_batch_sums", "fvh_autotune",
+    "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
 ]
 
 _lib = None
@@ -120,6 +126,8 @@ def load(path: str = LIB_PATH):
     L.fvh_batch_instance.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
     L.fvh_batch_status.argtypes = [vp, C.c_int32, C.POINTER(Status)]
     L.fvh_batch_sums.argtypes = [vp, dp]
+    L.fvh_objective.argtypes = [vp, dp, C.c_int32, dp, dp, dp]
+    L.fvh_adjoint.argtypes = [vp, dp, C.c_int32, dp, dp, C.c_int32, C.c_double, dp, dp, C.POINTER(AdjointStats)]
     L.fvh_autotune.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), dp, C.c_int32,
                                C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, C.c_int32]
     for name in EXPORTS:

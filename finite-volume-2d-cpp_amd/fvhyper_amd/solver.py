@@ -232,6 +232,42 @@ class Solver:
     def stage_bytes(self) -> float:
         return float(_lib.load().fvh_solver_stage_bytes(self._h))
 
+    def _weights(self, wx, wy):
+        nb = self.mesh.n_bnd
+        out = []
+        for w in (wx, wy):
+            if w is None:
+                out.append(None)
+            else:
+                w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+                if w.size != nb:
+                    raise ValueError("one weight per boundary element (%d)" % nb)
+                out.append(w)
+        return out
+
+    def objective(self, q0: np.ndarray, n_steps: int, wx=None, wy=None) -> float:
+        """J = sum_b p(ghost_b) (wx_b n_x + wy_b n_y) len_b after n_steps
+        steps of run() from q0 (fvh_objective)."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64).ravel()
+        wx, wy = self._weights(wx, wy)
+        J = C.c_double(0.0)
+        check(_lib.load().fvh_objective(self._h, dptr(q0), int(n_steps), dptr(wx) if wx is not None else None,
+                                        dptr(wy) if wy is not None else None, C.byref(J)))
+        return float(J.value)
+
+    def adjoint(self, q0: np.ndarray, n_steps: int, wx=None, wy=None, snaps: int = 0, mem_cap_bytes: float = 8e9):
+        """(J, dJ/dq0, stats) by the discrete adjoint (fvh_adjoint)."""
+        q0 = np.ascontiguousarray(q0, dtype=np.float64).ravel()
+        wx, wy = self._weights(wx, wy)
+        J = C.c_double(0.0)
+        grad = np.zeros_like(q0)
+        st = _lib.AdjointStats()
+        check(_lib.load().fvh_adjoint(self._h, dptr(q0), int(n_steps), dptr(wx) if wx is not None else None,
+                                      dptr(wy) if wy is not None else None, int(snaps), float(mem_cap_bytes),
+                                      C.byref(J), dptr(grad), C.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in _lib.AdjointStats._fields_}
+        return float(J.value), grad, stats
+
 
 class Batch(Solver):
     """Many independent meshes advanced together in the same launches

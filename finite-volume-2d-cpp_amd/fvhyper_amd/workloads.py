@@ -81,6 +81,52 @@ def forward_step(n_per_unit: int = 440, seed: int = 20240602, reference_compatib
     return m, cfg, uniform_ic(1.4, mach, 0.0, 1.0)
 
 
+def step_face_weights(mesh) -> np.ndarray:
+    """Objective weights of the forward-step workload: 1 on the boundary
+    elements of the step's vertical face (x = 0.6, 0 <= y <= 0.2), 0 elsewhere,
+    so that J = sum p n_x len is the pressure force on that face."""
+    ex, ey = mesh.export("edgesCentersX"), mesh.export("edgesCentersY")
+    be = mesh.export("boundaryEdges")
+    on = (np.abs(ex[be] - 0.6) < 1e-9) & (ey[be] <= 0.2 + 1e-9)
+    return on.astype(np.float64)
+
+
+def bump_state(mesh, rho=1.4, u=3.0, v=0.0, p=1.0, amplitude=1e-3, n_bumps=4, seed=20240605,
+               gamma: float = GAMMA) -> np.ndarray:
+    """Uniform state with a seeded smooth perturbation of relative size
+    `amplitude`: Gaussian bumps (centres uniform in the mesh's bounding box,
+    widths 0.05-0.15, random signs) in density, the same bumps at half weight
+    in pressure, and smooth background waves sin(2 pi x/Lx + a) cos(2 pi y/Ly
+    + b) at quarter weight in pressure and in both velocity components.
+    Density and pressure must not move together: scaled by one common factor
+    they keep p/rho, and so the sound speed and every time-step wave speed,
+    uniform, and neighbouring cells then tie in the time step's max(own,
+    neighbour) -- a kink of the discrete map that central differences
+    straddle."""
+    cx, cy = mesh.export("cellsCentersX"), mesh.export("cellsCentersY")
+    nr = mesh.n_real
+    x0, x1, y0, y1 = cx[:nr].min(), cx[:nr].max(), cy[:nr].min(), cy[:nr].max()
+    lx, ly = max(x1 - x0, 1e-30), max(y1 - y0, 1e-30)
+    r = meshgen.uniform01(seed, 4 * n_bumps + 6).reshape(-1)
+    ph = 2 * np.pi * r[4 * n_bumps:]
+    r = r[:4 * n_bumps].reshape(n_bumps, 4)
+    f = np.zeros(cx.size)
+    for k in range(n_bumps):
+        bx, by = x0 + lx * r[k, 0], y0 + ly * r[k, 1]
+        w = 0.05 + 0.1 * r[k, 2]
+        sgn = 1.0 if r[k, 3] < 0.5 else -1.0
+        f += sgn * np.exp(-((cx - bx) ** 2 + (cy - by) ** 2) / (w * w))
+    X, Y = 2 * np.pi * (cx - x0) / lx, 2 * np.pi * (cy - y0) / ly
+    bg = [np.sin(X + ph[2 * i]) * np.cos(Y + ph[2 * i + 1]) for i in range(3)]
+    speed = max(np.hypot(u, v), 1.0)
+    rr = rho * (1.0 + amplitude * f)
+    pp = p * (1.0 + amplitude * (0.5 * f + 0.25 * bg[0]))
+    uu = u + amplitude * 0.25 * speed * bg[1]
+    vv = v + amplitude * 0.25 * speed * bg[2]
+    q = np.stack([rr, rr * uu, rr * vv, pp / (gamma - 1) + 0.5 * rr * (uu * uu + vv * vv)], 1)
+    return q.ravel()
+
+
 def naca_freestream(mach: float = 0.8, aoa_deg: float = 1.25, reference_compatible: bool = False):
     """(rho, u, v, p) of the airfoil freestream: rho=1.4, p=1 (so c=1 and
     |V| = Mach), velocity at the angle of attack.  reference_compatible gives

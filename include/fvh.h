@@ -231,6 +231,33 @@ int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable);
 int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs, const double* q,
                      int32_t steps, int32_t* best_mode, int32_t* best_tile, double* report, int32_t max_report);
 
+/* ---- discrete adjoint (no reference counterpart: fvhyper has no objective
+ * and no derivatives).  The objective is a boundary pressure force after
+ * n_steps steps of run() from state q (reference layout, all cells):
+ *   J = sum over boundary elements b of p(ghost_b) (wx[b] n_x,b + wy[b] n_y,b) len_b
+ * with p the ghost cell's pressure, n the boundary edge's outward normal and
+ * wx, wy one weight per boundary element (reference order; NULL = 0).
+ * fvh_adjoint returns J and dJ/dq (4 values per cell; ghost entries are 0,
+ * since run() overwrites the ghost cells before the first step), computed by
+ * reverse-mode kernels over a replay of the forward steps.  Forward states
+ * are kept at binomially placed step checkpoints: `snaps` slots (0 = as many
+ * as mem_cap_bytes allows, at most one per step).  FP64, single-mesh
+ * solvers, global or fixed time step. */
+typedef struct fvh_adjoint_stats {
+    int32_t steps;             /* steps of the objective's run */
+    int32_t snaps;             /* checkpoint slots used */
+    int64_t forward_steps;     /* forward steps executed, replays included */
+    double checkpoint_bytes;   /* device memory of the checkpoints */
+    double work_bytes;         /* device memory of one step's replay and adjoint buffers */
+    double forward_ms;         /* the plain forward run to the objective */
+    double total_ms;           /* the whole gradient evaluation */
+} fvh_adjoint_stats;
+
+int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
+                      double* J);
+int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
+                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, fvh_adjoint_stats* st);
+
 /* ---- batches: many independent meshes advanced together (no reference
  * counterpart; a parameter sweep in the reference is one process per mesh).
  * The handle is a fvh_solver and every fvh_solver_* call applies to all

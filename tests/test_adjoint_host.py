@@ -1,0 +1,158 @@
+"""The reverse (vector-Jacobian) forms of the edge flux and the boundary
+functions, on the CPU: each must be the transpose of the forward derivative
+(dot-product test against dual-number JVPs of the same code), and the dual
+JVPs must agree with central differences of the forward function."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fvhyper_amd import meshgen
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "finite-volume-2d-cpp_amd",
+                   "libfvh_check.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="libfvh_check.so not built")
+
+
+def _lib():
+    L = C.CDLL(LIB)
+    d = C.POINTER(C.c_double)
+    L.fvhc_roe.argtypes = [C.c_double, C.c_int, d, d, d, d]
+    L.fvhc_roe_vjp.argtypes = [C.c_double, d, d, d, d, d, d, d]
+    L.fvhc_roe_jvp.argtypes = [C.c_double, C.c_int, d, d, d, d, d, d, d]
+    L.fvhc_bc_vjp.argtypes = [C.c_int, d, C.c_double, d, d, d, d]
+    L.fvhc_bc.argtypes = [C.c_int, d, C.c_double, d, d, d]
+    L.fvhc_phi_deriv.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.fvhc_phi_deriv.restype = C.c_double
+    return L
+
+
+def P(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _state(u, mach=0.8):
+    rho, p = 0.5 + u[0], 0.5 + u[3]
+    c = np.sqrt(1.4 * p / rho)
+    vx, vy = (2 * u[1] - 1) * mach * c, (2 * u[2] - 1) * mach * c
+    return np.array([rho, rho * vx, rho * vy, p / 0.4 + 0.5 * rho * (vx * vx + vy * vy)])
+
+
+def _cases(n, seed, mach=0.8):
+    u = meshgen.uniform01(seed, 11 * n).reshape(n, 11)
+    for k in range(n):
+        th = 2 * np.pi * u[k, 8]
+        yield _state(u[k, :4], mach), _state(u[k, 4:8], mach), np.array([np.cos(th), np.sin(th)]), u[k, 9:]
+
+
+def roe(L, g, joined, qi, qj, n):
+    f = np.zeros(4)
+    L.fvhc_roe(g, joined, P(qi), P(qj), P(n), P(f))
+    return f
+
+
+@pytest.mark.parametrize("joined", [0, 1])
+def test_roe_flux_matches_the_oracle_flux(joined):
+    L = _lib()
+    cfg = {"flux": "roe_joined" if joined else "roe_split", "gamma": 1.4}
+    for qi, qj, n, _ in _cases(50, 901 + joined):
+        assert np.array_equal(roe(L, 1.4, joined, qi, qj, n), O.flux(cfg, qi, qj, n))
+
+
+@pytest.mark.parametrize("joined", [0, 1])
+@pytest.mark.parametrize("mach", [0.3, 0.8, 2.5])
+def test_roe_reverse_is_the_transpose_of_the_forward_derivative(joined, mach):
+    L = _lib()
+    rng = np.random.default_rng(17 + joined)
+    worst = 0.0
+    for qi, qj, n, _ in _cases(200, 333 + int(10 * mach), mach):
+        dqi, dqj, dn = rng.standard_normal(4), rng.standard_normal(4), rng.standard_normal(2)
+        fb = rng.standard_normal(4)
+        df = np.zeros(4)
+        L.fvhc_roe_jvp(1.4, joined, P(qi), P(qj), P(n), P(dqi), P(dqj), P(dn), P(df))
+        qib, qjb, nb = np.zeros(4), np.zeros(4), np.zeros(2)
+        L.fvhc_roe_vjp(1.4, P(qi), P(qj), P(n), P(fb), P(qib), P(qjb), P(nb))
+        lhs = fb @ df
+        rhs = qib @ dqi + qjb @ dqj + nb @ dn
+        scale = np.abs(fb) @ np.abs(df) + np.abs(qib) @ np.abs(dqi) + np.abs(qjb) @ np.abs(dqj) + 1e-300
+        worst = max(worst, abs(lhs - rhs) / scale)
+    assert worst < 1e-13, worst
+
+
+def test_roe_forward_derivative_matches_central_differences():
+    L = _lib()
+    rng = np.random.default_rng(5)
+    for qi, qj, n, _ in _cases(100, 777):
+        dqi, dqj = rng.standard_normal(4), rng.standard_normal(4)
+        df = np.zeros(4)
+        L.fvhc_roe_jvp(1.4, 1, P(qi), P(qj), P(n), P(dqi), P(dqj), P(np.zeros(2)), P(df))
+        h = 1e-6
+        fd = (roe(L, 1.4, 1, qi + h * dqi, qj + h * dqj, n) - roe(L, 1.4, 1, qi - h * dqi, qj - h * dqj, n)) / (2 * h)
+        np.testing.assert_allclose(df, fd, rtol=1e-5, atol=1e-6 * np.max(np.abs(df)))
+
+
+@pytest.mark.parametrize("kind,par", [(0, [0, 0, 0, 0]), (1, [0, 0, 0, 0]), (2, [1.4, 3.0, 0.0, 1.0]),
+                                      (2, [1.4, 0.5, 0.1, 1.0]), (3, [1.4, 0.8, 0.0175, 1.0])])
+def test_boundary_reverse_matches_central_differences(kind, par):
+    """b = BC(q, n): (J^T bb) . dq == bb . (b(q + h dq) - b(q - h dq)) / 2h."""
+    L = _lib()
+    rng = np.random.default_rng(kind * 10 + int(par[1] * 10))
+    for mach in (0.4, 1.6):
+        for qi, _, n, _ in _cases(40, 1234 + kind, mach):
+            dq, bb = rng.standard_normal(4), rng.standard_normal(4)
+            qb = np.zeros(4)
+            L.fvhc_bc_vjp(kind, P(par), 1.4, P(qi), P(n), P(bb), P(qb))
+
+            def b(q):
+                out = np.zeros(4)
+                L.fvhc_bc(kind, P(par), 1.4, P(q), P(n), P(out))
+                return out
+            h = 1e-7
+            fd = bb @ (b(qi + h * dq) - b(qi - h * dq)) / (2 * h)
+            assert abs(qb @ dq - fd) <= 1e-6 * (np.abs(qb) @ np.abs(dq) + 1e-12), (qb @ dq, fd)
+
+
+def test_limiter_function_derivatives():
+    L = _lib()
+    for y in np.linspace(0.05, 3.0, 37):
+        h = 1e-6
+        for kind in (1, 2):
+            cfg = {"phi": {1: "michalak", 2: "venkat"}[kind], "phi_yt": 2.0}
+            fd = (O.phi(cfg, y + h) - O.phi(cfg, y - h)) / (2 * h)
+            if kind == 1 and abs(y - 2.0) < 1e-3:
+                continue
+            assert abs(L.fvhc_phi_deriv(kind, y, 2.0) - fd) < 1e-7
+    # closed form at r = 1: ((2r+2)(r^2+r+2) - (r^2+2r)(2r+1)) / (r^2+r+2)^2 = 7/16
+    assert abs(L.fvhc_phi_deriv(2, 1.0, 2.0) - (4.0 * 4 - 3 * 3) / 16.0) < 1e-15
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_time_step_wave_speed_reverse(kind):
+    """The summed form's c + |u.n| and the edge-minimum form's
+    (c+|u|) n0 + (c+|v|) n1: reverse == transpose of the dual derivative,
+    which matches central differences of the oracle's time step."""
+    L = _lib()
+    L.fvhc_eig_vjp.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                               C.POINTER(C.c_double)]
+    L.fvhc_eig_jvp.argtypes = [C.c_int, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]
+    L.fvhc_eig_jvp.restype = C.c_double
+    rng = np.random.default_rng(kind)
+    for qi, _, n, _ in _cases(60, 4321 + kind, 0.9):
+        dq, eb = rng.standard_normal(4), float(rng.standard_normal())
+        qb = np.zeros(4)
+        L.fvhc_eig_vjp(kind, 1.4, P(qi), P(n), eb, P(qb))
+        jv = L.fvhc_eig_jvp(kind, 1.4, P(qi), P(n), P(dq))
+        assert abs(eb * jv - qb @ dq) <= 1e-13 * (abs(eb * jv) + 1e-300)
+
+        def e(q):
+            p = 0.4 * (q[3] - 0.5 / q[0] * (q[1] ** 2 + q[2] ** 2))
+            c = np.sqrt(p * 1.4 / q[0])
+            if kind == 1:
+                return c + abs(q[1] / q[0] * n[0] + q[2] / q[0] * n[1])
+            return (c + abs(q[1] / q[0])) * n[0] + (c + abs(q[2] / q[0])) * n[1]
+        h = 1e-7
+        assert abs((e(qi + h * dq) - e(qi - h * dq)) / (2 * h) - jv) < 1e-6 * (abs(jv) + 1)
