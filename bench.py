@@ -212,13 +212,100 @@ def sweep(a):
     print(json.dumps(out))
 
 
+def adjoint(a):
+    """Discrete adjoint: gradient of the step-face pressure force after
+    --adj-steps RK5 steps of the second-order forward-step solver (h = 1/220,
+    243,936 cells) from a seeded smooth perturbation of the inflow, with
+    respect to every initial cell state.  Reports the adjoint/forward cost
+    ratio, the checkpoint memory, and the agreement with central differences
+    (eps 1e-6 and 1e-8) on --adj-dirs random directions and with the
+    complex-step derivative of the CPU restatement on a smaller mesh."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from fvhyper_amd import Mesh, Solver, workloads
+
+    m, cfg, ic = workloads.forward_step(n_per_unit=220)
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg)
+    s.use_graphs(True)           # forward steps replayed from HIP graphs, in both runs
+    w = workloads.step_face_weights(pm)
+    q0 = workloads.bump_state(pm)
+    N = a.adj_steps
+    s.objective(q0, 2, wx=w)                                 # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = s.objective(q0, N, wx=w)
+    t_fwd = time.perf_counter() - t0
+    s.adjoint(q0, 2, wx=w)                                   # warm-up
+    t0 = time.perf_counter()
+    Ja, grad, st = s.adjoint(q0, N, wx=w, mem_cap_bytes=8e9)
+    t_adj = time.perf_counter() - t0
+    rng = np.random.default_rng(20240606)
+    rel6, rel8 = [], []
+    for _ in range(a.adj_dirs):
+        v = rng.standard_normal((pm.n_cells, 4)) * np.array([1.0, 3.0, 3.0, 10.0])
+        v[pm.n_real:] = 0.0
+        v = v.ravel()
+        ad = float(grad @ v)
+        for eps, out in ((1e-6, rel6), (1e-8, rel8)):
+            fd = (s.objective(q0 + eps * v, N, wx=w) - s.objective(q0 - eps * v, N, wx=w)) / (2 * eps)
+            out.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    # complex step of the CPU restatement, on a smaller mesh (CPU time)
+    cs = None
+    try:
+        import oracle as O
+        mc, cfgc, _ = workloads.forward_step(n_per_unit=40)
+        pmc = Mesh.from_arrays(mc)
+        omc = O.Mesh(mc)
+        wc = workloads.step_face_weights(pmc)
+        qc = workloads.bump_state(pmc)
+        Nc = min(N, 40)
+        _, gc, _ = Solver(pmc, cfgc).adjoint(qc, Nc, wx=wc)
+        errs = []
+        for _ in range(4):
+            v = rng.standard_normal((pmc.n_cells, 4)) * np.array([1.0, 3.0, 3.0, 10.0])
+            v[pmc.n_real:] = 0.0
+            v = v.ravel()
+            _, dJ = O.cstep_derivative(omc, cfgc, qc, v, Nc, wx=wc)
+            errs.append(abs(float(gc @ v) - dJ) / abs(dJ))
+        cs = {"cells": pmc.n_real, "steps": Nc, "directions": 4, "max_rel_err": max(errs)}
+    except Exception as e:  # pragma: no cover - reported
+        print("complex-step check skipped: %s" % e, file=sys.stderr)
+    C = pm.n_real
+    stages = len(cfg["alpha"])
+    out = {
+        "metric": "Mcell-stage/s", "value": round(C * stages * N / t_fwd / 1e6, 2), "unit": "Mcell-stage/s",
+        "n_gpus": 1, "steps": N, "warmup": 2, "ms_per_step": round(t_fwd / N * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "forward_step_adjoint", "cells": C, "steps": N, "stages": N * stages,
+                   "objective": "pressure force on the step face", "parallelism": "single_gpu"},
+        "adjoint": {"J": J, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+                    "adjoint_over_forward": round(t_adj / t_fwd, 3), "snaps": st["snaps"],
+                    "forward_steps_executed": st["forward_steps"],
+                    "checkpoint_GB": round(st["checkpoint_bytes"] / 1e9, 3),
+                    "work_GB": round(st["work_bytes"] / 1e9, 3),
+                    "fd_eps_1e-6": {"directions": len(rel6), "median_rel_err": float(np.median(rel6)),
+                                    "max_rel_err": float(np.max(rel6)),
+                                    "frac_below_1e-4": float(np.mean(np.array(rel6) < 1e-4))},
+                    "fd_eps_1e-8": {"directions": len(rel8), "median_rel_err": float(np.median(rel8)),
+                                    "max_rel_err": float(np.max(rel8))},
+                    "complex_step": cs},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep"])
+    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint"])
+    ap.add_argument("--adj-steps", type=int, default=80, help="adjoint: RK5 steps of the objective's run")
+    ap.add_argument("--adj-dirs", type=int, default=64, help="adjoint: random directions checked")
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
     ap.add_argument("--per-mesh-steps", type=int, default=0,
                     help="sweep: steps of the one-by-one baseline (default: --steps)")
@@ -246,6 +333,8 @@ def main():
     torch.cuda.set_device(0)
     if a.workload == "sweep":
         return sweep(a)
+    if a.workload == "adjoint":
+        return adjoint(a)
     if a.workload == "step":
         m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
     else:

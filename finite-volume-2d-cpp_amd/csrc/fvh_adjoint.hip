@@ -23,6 +23,8 @@
 #include "fvh_adjoint.h"
 #include "fvh_device.h"
 
+#include <algorithm>
+
 namespace fvh {
 namespace {
 
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
             bool fallback = !LIN;
             if (LIN) {
                 T ax[4], ay[4], al[4];
-                const Q4<T> a = gx[c], b = gy[c], d = lm[c];
+                const Q4<T> a = gx[c], b = gy[c], d = lm[c];   // read only for cells on a boundary
 #pragma unroll
                 for (int k = 0; k < 4; ++k) { ax[k] = a.v[k]; ay[k] = b.v[k]; al[k] = d.v[k]; }
                 const Q4<T> inc = face_inc(ax, ay, al, m.e_c[ue & kEdgeMask], cc);
@@ -244,15 +246,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
         const T K3a = Ka * Ka * Ka;
         const T yt = T(pp.phi_yt);
         const T tol = T(1e-15);
-        for (int k = 0; k < 4; ++k) {
-            if (llim[k] == T(0)) continue;
-            // the half-edge whose value the running minimum ends on
-            T run = T(1);
-            int hs = -1;
-            for (int h = h0; h < h1; ++h) {
-                const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
-                const T dx = ec.v[0] - cc.v[0];
-                const T dy = ec.v[1] - cc.v[1];
+        // the half-edge each variable's running minimum ends on (one pass, as the forward)
+        T run[4] = {T(1), T(1), T(1), T(1)};
+        int hsel[4] = {-1, -1, -1, -1};
+        const bool any = llim[0] != T(0) || llim[1] != T(0) || llim[2] != T(0) || llim[3] != T(0);
+        for (int h = h0; any && h < h1; ++h) {
+            const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
                 const T dqg = gx[k] * dx + gy[k] * dy;
                 const T dmax = qmx[k] - qc.v[k];
                 const T dmin = qmn[k] - qc.v[k];
@@ -269,8 +272,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                     else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
                 }
                 l = sig + (T(1) - sig) * l;
-                if (l < run) { run = l; hs = h; }
+                if (l < run[k]) { run[k] = l; hsel[k] = h; }
             }
+        }
+        for (int k = 0; k < 4; ++k) {
+            const int hs = hsel[k];
+            if (llim[k] == T(0)) continue;
             if (hs < 0) continue;
             const G2<T> ec = m.e_c[m.he_edge[hs] & kEdgeMask];
             const T dx = ec.v[0] - cc.v[0];
@@ -373,12 +380,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<T> m, const Q4<T
 // summed form: dt = cfl A / S, S = sum_h max(e_own, e_nbr) len_h
 // edge-minimum form: dt = min(1, min_h cfl (A / len_h) / |e_h|)
 template <class T, int KIND>
-__global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q, int c,
-                              const double* __restrict__ ld_blk, int n_blk, Q4<T>* __restrict__ lam_q0) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double sacc = 0.0;
-    for (int i = 0; i < n_blk; ++i) sacc += ld_blk[i];
-    const T lam_d = T(sacc);
+__global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q,
+                              const int32_t* __restrict__ cell, const double* __restrict__ ld_blk, int n_blk,
+                              Q4<T>* __restrict__ lam_q0) {
+    if (blockIdx.x != 0) return;
+    // sum of the per-tile partials, in a fixed pattern (deterministic)
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n_blk; i += kBlock) part[0] += ld_blk[i];
+    block_sum4(part);
+    const int c = cell[0];
+    if (threadIdx.x != 0 || c < 0) return;
+    const T lam_d = T(part[0]);
     if (KIND == 0 || lam_d == T(0)) return;
     const T g = T(pp.gamma);
     const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
@@ -434,6 +446,58 @@ __global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restr
         for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + qb[k];
         lam_q0[c] = l;
     }
+}
+
+// ---- the cell whose time step is the global minimum ------------------------
+// min_dt (explicit.cpp:555-564) keeps the first minimal entry in reference
+// order; ref[d] is the reference index of device cell d.
+__device__ inline bool better(double v, int32_t r, double bv, int32_t br) {
+    return br < 0 || v < bv || (v == bv && r < br);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_argmin_blocks(const T* __restrict__ dt, const int32_t* __restrict__ ref,
+                                                          int n, double* __restrict__ bv, int32_t* __restrict__ bref,
+                                                          int32_t* __restrict__ bdev) {
+    __shared__ double sv[kBlock];
+    __shared__ int32_t sr[kBlock], sd[kBlock];
+    double v = 0.0;
+    int32_t r = -1, d = -1;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double x = (double)dt[i];
+        if (x == x && better(x, ref[i], v, r)) { v = x; r = ref[i]; d = i; }
+    }
+    sv[threadIdx.x] = v; sr[threadIdx.x] = r; sd[threadIdx.x] = d;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            if (sr[o] >= 0 && better(sv[o], sr[o], sv[threadIdx.x], sr[threadIdx.x])) {
+                sv[threadIdx.x] = sv[o]; sr[threadIdx.x] = sr[o]; sd[threadIdx.x] = sd[o];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { bv[blockIdx.x] = sv[0]; bref[blockIdx.x] = sr[0]; bdev[blockIdx.x] = sd[0]; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_argmin_final(int nb, const double* __restrict__ bv,
+                                                         const int32_t* __restrict__ bref,
+                                                         const int32_t* __restrict__ bdev, int32_t* __restrict__ out) {
+    __shared__ double sv[kBlock];
+    __shared__ int32_t sr[kBlock], sd[kBlock];
+    const int t = threadIdx.x;
+    sv[t] = 0.0; sr[t] = -1; sd[t] = -1;
+    if (t < nb) { sv[t] = bv[t]; sr[t] = bref[t]; sd[t] = bdev[t]; }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            const int o = t + off;
+            if (sr[o] >= 0 && better(sv[o], sr[o], sv[t], sr[t])) { sv[t] = sv[o]; sr[t] = sr[o]; sd[t] = sd[o]; }
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[0] = sd[0];
 }
 
 // ---- initial boundary update reversed (explicit.cpp:712) -------------------
@@ -534,12 +598,21 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
 }
 
 template <class T>
-void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q, int cell,
-                   const double* ld_blk, int n_blk, Q4<T>* lam_q0) {
+void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, double* bv, int32_t* bref,
+                      int32_t* bdev, int32_t* out) {
+    const int nb = std::min(nblocks(n), kArgminBlocks);
+    hipLaunchKernelGGL((k_argmin_blocks<T>), dim3(nb), dim3(kBlock), 0, s, dt, ref, n, bv, bref, bdev);
+    hipLaunchKernelGGL(k_argmin_final, dim3(1), dim3(kBlock), 0, s, nb, (const double*)bv, (const int32_t*)bref,
+                       (const int32_t*)bdev, out);
+}
+
+template <class T>
+void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q,
+                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0) {
     if (dt_kind == 1)
-        hipLaunchKernelGGL((k_adj_dt_cell<T, 1>), dim3(1), dim3(64), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
+        hipLaunchKernelGGL((k_adj_dt_cell<T, 1>), dim3(1), dim3(kBlock), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
     else if (dt_kind == 2)
-        hipLaunchKernelGGL((k_adj_dt_cell<T, 2>), dim3(1), dim3(64), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
+        hipLaunchKernelGGL((k_adj_dt_cell<T, 2>), dim3(1), dim3(kBlock), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
 }
 
 template <class T>
@@ -569,7 +642,9 @@ template void launch_adj_stage<double>(hipStream_t, const MeshView<double>&, con
                                        const Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*,
                                        Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*);
 template void launch_adj_dt<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int, const Q4<double>*,
-                                    int, const double*, int, Q4<double>*);
+                                    const int32_t*, const double*, int, Q4<double>*);
+template void launch_dt_argmin<double>(hipStream_t, const double*, const int32_t*, int, double*, int32_t*, int32_t*,
+                                       int32_t*);
 template void launch_adj_bounds_init<double>(hipStream_t, const MeshView<double>&, const PhysParams&,
                                              const BcParam<double>*, const Q4<double>*, Q4<double>*);
 template void launch_objective<double>(hipStream_t, int, const PhysParams&, const int32_t*, const Q4<double>*,

@@ -171,9 +171,13 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
                       const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
                       Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
                       Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk);
+constexpr int kArgminBlocks = 256;
 template <class T>
-void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q, int cell,
-                   const double* ld_blk, int n_blk, Q4<T>* lam_q0);
+void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, double* bv, int32_t* bref,
+                      int32_t* bdev, int32_t* out);
+template <class T>
+void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q,
+                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0);
 template <class T>
 void launch_adj_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs,
                             const Q4<T>* q, Q4<T>* lam);

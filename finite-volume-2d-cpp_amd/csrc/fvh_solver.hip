@@ -108,7 +108,10 @@ struct fvh_solver {
         a_lm[fvh::kAdjMaxStages], a_qt[fvh::kAdjMaxStages];
     fvh::DevBuf a_lam, a_lam_in, a_lam_q0, a_lam_qt, a_lam_bincr, a_lam_face, a_lam_direct, a_nb, a_ld, a_tmp;
     fvh::DevBuf a_obj_ghost, a_obj_gw, a_obj_terms;
+    fvh::DevBuf a_ref, a_amin_v, a_amin_r, a_amin_d, a_amin;   // device -> reference index; argmin scratch
     std::vector<std::unique_ptr<fvh::DevBuf>> a_ck;
+    hipGraph_t a_graph = nullptr;            // one step's replay + reverse, captured once
+    hipGraphExec_t a_graph_exec = nullptr;
     // graphs
     bool use_graphs = false;
     std::vector<fvh::StepGraph> graphs;
@@ -130,6 +133,8 @@ struct fvh_solver {
     ~fvh_solver() {
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graphs();
+        if (a_graph_exec) (void)hipGraphExecDestroy(a_graph_exec);
+        if (a_graph) (void)hipGraphDestroy(a_graph);
         for (auto& pe : pending) { event_pool.push_back(pe.second.first); event_pool.push_back(pe.second.second); }
         for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1080,7 +1085,6 @@ struct Adjoint {
     int n_stage = 0;
     size_t qbytes = 0;
     int64_t fwd_steps = 0;
-    std::vector<double> dt_host;
 
     int32_t alloc(int snaps) {
         const size_t N = (size_t)s->n_total;
@@ -1099,8 +1103,12 @@ struct Adjoint {
             (!s->a_lam_face.p && s->a_lam_face.alloc(he)) || (!s->a_nb.p && s->a_nb.alloc(he)) ||
             (!s->a_lam_direct.p && s->a_lam_direct.alloc(qbytes)) ||
             (!s->a_ld.p && s->a_ld.alloc((size_t)(kAdjMaxStages * (s->n_rt + 1)) * 8)) ||
-            (!s->a_tmp.p && s->a_tmp.alloc(qbytes)))
+            (!s->a_tmp.p && s->a_tmp.alloc(qbytes)) ||
+            (!s->a_amin_v.p && s->a_amin_v.alloc(kArgminBlocks * 8)) ||
+            (!s->a_amin_r.p && s->a_amin_r.alloc(kArgminBlocks * 4)) ||
+            (!s->a_amin_d.p && s->a_amin_d.alloc(kArgminBlocks * 4)) || (!s->a_amin.p && s->a_amin.alloc(8)))
             return oom();
+        if (!s->a_ref.p && upload(s->a_ref, s->inv.data(), s->inv.size() * 4)) return oom();
         while ((int)s->a_ck.size() < snaps) {
             std::unique_ptr<DevBuf> b(new DevBuf());
             if (b->alloc(qbytes)) return oom();
@@ -1122,7 +1130,7 @@ struct Adjoint {
         FVH_HIP(hipMemcpyAsync(s->q[s->cur].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
         fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
         for (int i = 0; i < k; ++i) {
-            int32_t rc = enqueue_step<double>(s, false, &o, 0);
+            int32_t rc = fvh::step<double>(s, false, &o, 0);
             if (rc) return rc;
         }
         fwd_steps += k;
@@ -1130,8 +1138,8 @@ struct Adjoint {
         return FVH_OK;
     }
 
-    // one step from src, keeping every stage's input, gradient, limiter and qt
-    int32_t replay(const void* src) {
+    // one step from a_qs[0], keeping every stage's input, gradient, limiter and qt
+    int32_t replay_body() {
         const fvh_config& c = s->cfg;
         const MeshView<double> m = view<double>(s);
         const TileMap tm = tile_map(s);
@@ -1142,7 +1150,6 @@ struct Adjoint {
         const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
                    lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
         hipStream_t st = s->stream;
-        FVH_HIP(hipMemcpyAsync(s->a_qs[0].p, src, qbytes, hipMemcpyDeviceToDevice, st));
         double* dt = (double*)s->dt.p;
         double* blk = (double*)s->blk_min.p;
         if (!s->second) launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)s->a_qs[0].p, dt, blk, nullptr);
@@ -1161,7 +1168,6 @@ struct Adjoint {
                                 (Q4<double>*)s->a_qt[k].p, nullptr);
         }
         FVH_HIP(hipGetLastError());
-        fwd_steps += 1;
         return FVH_OK;
     }
 
@@ -1193,23 +1199,38 @@ struct Adjoint {
                                      (double*)s->a_ld.p + (size_t)k * s->n_rt);
         }
         if (gdt && c.dt_kind != FVH_DT_FIXED) {
-            // the cell whose step is the minimum: first in reference order (min_dt, explicit.cpp:555-564)
-            dt_host.resize(s->n_total);
-            FVH_HIP(hipMemcpyAsync(dt_host.data(), s->dt.p, dt_host.size() * 8, hipMemcpyDeviceToHost, st));
-            FVH_HIP(hipStreamSynchronize(st));
-            int32_t best = -1;
-            double bv = 0.0;
-            int32_t best_ref = INT32_MAX;
-            for (int32_t d = 0; d < s->n_total; ++d) {
-                const double v = dt_host[d];
-                if (best < 0 || v < bv || (v == bv && s->inv[d] < best_ref)) { best = d; bv = v; best_ref = s->inv[d]; }
-            }
-            if (best >= 0)
-                launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)s->a_qs[0].p, best,
-                                      (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0);
+            // the cell whose step is the minimum, first in reference order (min_dt, explicit.cpp:555-564)
+            launch_dt_argmin<double>(st, (const double*)s->dt.p, (const int32_t*)s->a_ref.p, s->n_total,
+                                     (double*)s->a_amin_v.p, (int32_t*)s->a_amin_r.p, (int32_t*)s->a_amin_d.p,
+                                     (int32_t*)s->a_amin.p);
+            launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)s->a_qs[0].p, (const int32_t*)s->a_amin.p,
+                                  (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0);
         }
         FVH_HIP(hipGetLastError());
-        std::swap(s->a_lam.p, s->a_lam_q0.p);
+        FVH_HIP(hipMemcpyAsync(s->a_lam.p, s->a_lam_q0.p, qbytes, hipMemcpyDeviceToDevice, st));
+        return FVH_OK;
+    }
+
+    // replay of the step starting from state src, then its reverse; the
+    // launch sequence never changes, so it is captured once as a HIP graph
+    int32_t step_back(const void* src) {
+        FVH_HIP(hipMemcpyAsync(s->a_qs[0].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        if (!s->a_graph_exec) {
+            FVH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            int32_t rc = replay_body();
+            if (!rc) rc = reverse();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(s->stream, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+            if (e != hipSuccess) {
+                set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                return FVH_E_HIP;
+            }
+            s->a_graph = g;
+            FVH_HIP(hipGraphInstantiate(&s->a_graph_exec, g, nullptr, nullptr, 0));
+        }
+        FVH_HIP(hipGraphLaunch(s->a_graph_exec, s->stream));
+        fwd_steps += 1;
         return FVH_OK;
     }
 
@@ -1224,15 +1245,11 @@ struct Adjoint {
         const int n = b - a;
         int32_t rc;
         if (n <= 0) return FVH_OK;
-        if (n == 1) {
-            if ((rc = replay(s->a_ck[sa]->p))) return rc;
-            return reverse();
-        }
+        if (n == 1) return step_back(s->a_ck[sa]->p);
         if (free_slots.empty()) {
             for (int k = b - 1; k >= a; --k) {
                 if ((rc = advance(s->a_ck[sa]->p, s->a_tmp.p, k - a))) return rc;
-                if ((rc = replay(s->a_tmp.p))) return rc;
-                if ((rc = reverse())) return rc;
+                if ((rc = step_back(s->a_tmp.p))) return rc;
             }
             return FVH_OK;
         }
@@ -1443,14 +1460,35 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     s->cur = 0;
     if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
     FVH_HIP(hipMemcpyAsync(s->a_ck[0]->p, s->q[s->cur].p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
-    // forward to the objective, then seed the adjoint of the final state
-    if ((rc = A.advance(s->a_ck[0]->p, s->a_tmp.p, n_steps))) return rc;
+    // forward to the objective, then seed the adjoint of the final state.
+    // With a slot per step the sweep keeps every step's state and each step
+    // is replayed once; otherwise the binomial schedule recomputes.
+    const bool every = snaps >= n_steps;
+    if (every) {
+        // step in the solver's own buffers, keeping a copy of each step's start
+        fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
+        for (int k = 0; k < n_steps; ++k) {
+            if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
+            FVH_HIP(hipMemcpyAsync(k + 1 < n_steps ? s->a_ck[k + 1]->p : s->a_tmp.p, s->q[s->cur].p, A.qbytes,
+                                   hipMemcpyDeviceToDevice, s->stream));
+        }
+        A.fwd_steps += n_steps;
+        if (n_steps == 0)
+            FVH_HIP(hipMemcpyAsync(s->a_tmp.p, s->a_ck[0]->p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+    } else if ((rc = A.advance(s->a_ck[0]->p, s->a_tmp.p, n_steps))) {
+        return rc;
+    }
     FVH_HIP(hipStreamSynchronize(s->stream));
     const auto t1 = std::chrono::steady_clock::now();
     if ((rc = objective_eval(s, s->a_tmp.p, true, J))) return rc;
-    std::vector<int> free_slots;
-    for (int k = snaps - 1; k >= 1; --k) free_slots.push_back(k);
-    if ((rc = A.rev(0, n_steps, 0, free_slots))) return rc;
+    if (every) {
+        for (int k = n_steps - 1; k >= 0; --k)
+            if ((rc = A.step_back(s->a_ck[k]->p))) return rc;
+    } else {
+        std::vector<int> free_slots;
+        for (int k = snaps - 1; k >= 1; --k) free_slots.push_back(k);
+        if ((rc = A.rev(0, n_steps, 0, free_slots))) return rc;
+    }
     launch_adj_bounds_init<double>(s->stream, view<double>(s), phys(s->cfg), (const BcParam<double>*)s->bcs.p,
                                    (const Q4<double>*)s->a_ck[0]->p, (Q4<double>*)s->a_lam.p);
     FVH_HIP(hipGetLastError());

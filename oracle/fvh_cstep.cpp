@@ -1,0 +1,486 @@
+// fvh_cstep.cpp -- the explicit path of fvh_oracle.c restated as a C++
+// template on the number type, for complex-step derivatives.  TEST
+// INFRASTRUCTURE ONLY (libcstep.so): the product never links or calls it.
+//
+// Instantiated on double it must reproduce fvh_oracle.c's run() bit for bit
+// (same operations, same order; a CPU test checks this), which ties it to
+// the compiled reference.  Instantiated on `cs`, a complex number whose
+// comparisons, min/max and |.| act on the real part (the usual complex-step
+// conventions), a state q0 + i h v with h = 1e-30 gives the derivative of
+// the objective along v as Im(J)/h, exact to rounding, on whatever branch
+// the real evaluation takes.  That is the check of the GPU adjoint.
+//
+// Functions follow fvh_oracle.c, which cites the reference: calc_dt kinds
+// (forwardStep/main.cpp:140-185, naca/main.cpp:142-183), min_dt
+// (explicit.cpp:555-564), calc_gradients (explicit.cpp:112-171),
+// calc_limiters (explicit.cpp:174-258), calc_time_derivatives
+// (explicit.cpp:261-364), update_cells (explicit.cpp:368-378),
+// update_bounds (explicit.cpp:380-417), run (explicit.cpp:657-817).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "fvh_oracle.h"
+
+namespace {
+
+constexpr int V = 4;
+
+struct cs {
+    double re, im;
+    cs() : re(0.0), im(0.0) {}
+    cs(double a) : re(a), im(0.0) {}
+    cs(double a, double b) : re(a), im(b) {}
+};
+inline cs operator+(cs a, cs b) { return cs(a.re + b.re, a.im + b.im); }
+inline cs operator-(cs a, cs b) { return cs(a.re - b.re, a.im - b.im); }
+inline cs operator-(cs a) { return cs(-a.re, -a.im); }
+inline cs operator*(cs a, cs b) { return cs(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+inline cs operator/(cs a, cs b) {
+    const double d = b.re * b.re + b.im * b.im;
+    return cs((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
+}
+inline cs& operator+=(cs& a, cs b) { a = a + b; return a; }
+inline cs& operator-=(cs& a, cs b) { a = a - b; return a; }
+inline cs& operator*=(cs& a, cs b) { a = a * b; return a; }
+inline bool operator<(cs a, cs b) { return a.re < b.re; }
+inline bool operator>(cs a, cs b) { return a.re > b.re; }
+inline bool operator<=(cs a, cs b) { return a.re <= b.re; }
+inline bool operator>=(cs a, cs b) { return a.re >= b.re; }
+// principal square root, imaginary part without cancellation
+inline cs sqrt(cs a) {
+    if (a.im == 0.0 && a.re >= 0.0) return cs(::sqrt(a.re), 0.0);
+    const double r = hypot(a.re, a.im);
+    if (a.re >= 0.0) {
+        const double s = ::sqrt(0.5 * (r + a.re));
+        return cs(s, a.im / (2.0 * s));
+    }
+    const double t = ::sqrt(0.5 * (r - a.re));
+    return cs(fabs(a.im) / (2.0 * t), a.im < 0.0 ? -t : t);
+}
+inline cs fabs(cs a) { return a.re < 0.0 ? -a : a; }
+inline double re(double a) { return a; }
+inline double re(cs a) { return a.re; }
+
+template <class T> inline T smin(T a, T b) { return (b < a) ? b : a; }
+template <class T> inline T smax(T a, T b) { return (a < b) ? b : a; }
+using ::fabs;
+using ::sqrt;
+
+template <class T>
+inline T pressure(double g, const T* q) {
+    return (T(g) - T(1.0)) * (q[3] - T(0.5) / q[0] * (q[1] * q[1] + q[2] * q[2]));
+}
+
+template <class T>
+void flux(const orc_config* cf, T* f, const T* qi, const T* qj, const double* n) {
+    const T g = T(cf->gamma);
+    const T n0 = T(n[0]), n1 = T(n[1]);
+    T pi = pressure(cf->gamma, qi);
+    T pj = pressure(cf->gamma, qj);
+    T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
+    T Vj = (qj[1] * n0 + qj[2] * n1) / qj[0];
+    if (cf->flux == ORC_FLUX_ROE_JOINED) {
+        f[0] = (qi[0] * Vi + qj[0] * Vj) * T(0.5);
+        f[1] = (qi[1] * Vi + pi * n0 + qj[1] * Vj + pj * n0) * T(0.5);
+        f[2] = (qi[2] * Vi + pi * n1 + qj[2] * Vj + pj * n1) * T(0.5);
+        f[3] = ((qi[3] + pi) * Vi + (qj[3] + pj) * Vj) * T(0.5);
+    } else {
+        f[0] = qi[0] * Vi;
+        f[1] = qi[1] * Vi + pi * n0;
+        f[2] = qi[2] * Vi + pi * n1;
+        f[3] = (qi[3] + pi) * Vi;
+        f[0] += qj[0] * Vj;
+        f[1] += qj[1] * Vj + pj * n0;
+        f[2] += qj[2] * Vj + pj * n1;
+        f[3] += (qj[3] + pj) * Vj;
+        for (int i = 0; i < 4; ++i) f[i] *= T(0.5);
+    }
+    const T pL = pi, pR = pj;
+    const T uL = qi[1] / qi[0];
+    const T uR = qj[1] / qj[0];
+    const T vL = qi[2] / qi[0];
+    const T vR = qj[2] / qj[0];
+    const T srhoL = sqrt(qi[0]);
+    const T srhoR = sqrt(qj[0]);
+    const T rho = srhoR * srhoL;
+    const T u = (uL * srhoL + uR * srhoR) / (srhoL + srhoR);
+    const T v = (vL * srhoL + vR * srhoR) / (srhoL + srhoR);
+    const T h = ((qi[3] + pL) / qi[0] * srhoL + (qj[3] + pR) / qj[0] * srhoR) / (srhoL + srhoR);
+    const T q2 = u * u + v * v;
+    const T c = sqrt((g - T(1.)) * (h - T(0.5) * q2));
+    const T Vn = u * n0 + v * n1;
+    const T VR = uR * n0 + vR * n1;
+    const T VL = uL * n0 + vL * n1;
+    const T lcm = fabs(smin(Vn - c, VL - c));
+    const T lc = fabs(Vn);
+    const T lcp = fabs(smax(Vn + c, VR + c));
+    const T kF1 = lcm * ((pR - pL) - rho * c * (VR - VL)) / (T(2.) * c * c);
+    const T kF234_0 = lc * ((qj[0] - qi[0]) - (pR - pL) / (c * c));
+    const T kF234_1 = lc * rho;
+    const T kF5 = lcp * ((pR - pL) + rho * c * (VR - VL)) / (T(2.) * c * c);
+    f[0] -= T(0.5) * (kF1 + kF234_0 + kF5);
+    f[1] -= T(0.5) * (kF1 * (u - c * n0) + kF234_0 * u + kF234_1 * (uR - uL - (VR - VL) * n0) + kF5 * (u + c * n0));
+    f[2] -= T(0.5) * (kF1 * (v - c * n1) + kF234_0 * v + kF234_1 * (vR - vL - (VR - VL) * n1) + kF5 * (v + c * n1));
+    f[3] -= T(0.5) * (kF1 * (h - c * Vn) + kF234_0 * q2 * T(0.5) +
+                      kF234_1 * (u * (uR - uL) + v * (vR - vL) - Vn * (VR - VL)) + kF5 * (h + c * Vn));
+}
+
+template <class T>
+T phi(const orc_config* cf, T y) {
+    switch (cf->phi) {
+    case ORC_PHI_ZERO:
+        return T(0.);
+    case ORC_PHI_MICHALAK: {
+        const T yt = T(cf->phi_yt);
+        if (y >= yt) return T(1.0);
+        const T a = T(1.0) / (yt * yt) - T(2.0) / (yt * yt * yt);
+        const T b = T(-3.0 / 2.0) * a * yt - T(0.5) / yt;
+        return a * y * y * y + b * y * y + y;
+    }
+    case ORC_PHI_VENKAT:
+        return (y * y + T(2.0) * y) / (y * y + y + T(2.0));
+    default:
+        return smax(T(0.), smin(T(1.), y));
+    }
+}
+
+template <class T>
+void bc(const orc_config* cf, int slot, T* b, const T* q, const double* nd) {
+    const int kind = cf->bc_kind[slot];
+    const double gd = cf->gamma;
+    const T g = T(gd);
+    const T n0 = T(nd[0]), n1 = T(nd[1]);
+    if (kind == ORC_BC_COPY) {
+        b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[3];
+        return;
+    }
+    if (kind == ORC_BC_MIRROR) {
+        b[0] = q[0];
+        b[1] = q[1] - T(2.0) * n0 * (n0 * q[1] + n1 * q[2]);
+        b[2] = q[2] - T(2.0) * n1 * (n0 * q[1] + n1 * q[2]);
+        b[3] = q[3];
+        return;
+    }
+    const double* P = cf->bc_param[slot];
+    T bv[4];
+    bv[0] = T(P[0]);
+    bv[1] = bv[0] * T(P[1]);
+    bv[2] = bv[0] * T(P[2]);
+    bv[3] = T(P[3]) / (g - T(1)) + T(0.5) / bv[0] * (bv[1] * bv[1] + bv[2] * bv[2]);
+    T U0 = q[1] / q[0];
+    T U1 = q[2] / q[0];
+    T u_norm = sqrt(U0 * U0 + U1 * U1);
+    T u_dot_n = U0 * n0 + U1 * n1;
+    T p = pressure(gd, q);
+    T c = sqrt(g * p / q[0]);
+    T mach = u_norm / c;
+    if (mach > T(1)) {
+        if (u_dot_n < T(0)) {
+            b[0] = bv[0]; b[1] = bv[1]; b[2] = bv[2]; b[3] = bv[3];
+        } else {
+            b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[3];
+        }
+        return;
+    }
+    T pa = pressure(gd, bv);
+    T rhoa = bv[0];
+    T ua = bv[1] / bv[0];
+    T va = bv[2] / bv[0];
+    T pd = p;
+    T rhod = q[0];
+    T ud = q[1] / q[0];
+    T rho0 = q[0];
+    T c0 = c;
+    T vd = q[2] / q[0];
+    if (u_dot_n < T(0)) {
+        if (kind == ORC_BC_INLET_OUTLET) {
+            T pb = pd;
+            b[0] = bv[0]; b[1] = bv[1]; b[2] = bv[2];
+            b[3] = pb / (g - T(1.)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+        } else {
+            T pb = T(0.5) * (pa + pd - rho0 * c0 * (n0 * (ua - ud) + n1 * (va - vd)));
+            b[0] = rhoa + (pb - pa) / (c0 * c0);
+            b[1] = b[0] * (ua - n0 * (pa - pb) / (rho0 * c0));
+            b[2] = b[1] * (va - n1 * (pa - pb) / (rho0 * c0));
+            b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+        }
+    } else {
+        T pb = pa;
+        b[0] = rhod + (pb - pd) / (c0 * c0);
+        b[1] = b[0] * (ud + n0 * (pd - pb) / (rho0 * c0));
+        b[2] = b[1] * (va + n1 * (pd - pb) / (rho0 * c0));
+        b[3] = pb / (g - T(1.)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
+    }
+}
+
+template <class T>
+void calc_dt(const orc_mesh* m, const orc_config* cf, const T* q, T* dt) {
+    const int32_t N = m->n_cells;
+    const double gd = cf->gamma;
+    const T g = T(gd);
+    if (cf->dt_kind == ORC_DT_FIXED) {
+        for (int32_t i = 0; i < N; ++i) dt[i] = T(cf->dt_fixed);
+        return;
+    }
+    if (cf->dt_kind == ORC_DT_SUM_CFL) {
+        for (int32_t i = 0; i < N; ++i) dt[i] = T(0.);
+        for (int32_t e = 0; e < m->n_edges; ++e) {
+            int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+            T le = T(m->edge_len[e]), n0 = T(m->edge_nx[e]), n1 = T(m->edge_ny[e]);
+            const T* qi = &q[V * i];
+            const T* qj = &q[V * j];
+            T ci = sqrt(pressure(gd, qi) * g / qi[0]);
+            T cj = sqrt(pressure(gd, qj) * g / qj[0]);
+            T ei = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
+            T ej = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
+            T center = smax(ei, ej);
+            dt[i] += center * le;
+            dt[j] += center * le;
+        }
+        for (int32_t i = 0; i < N; ++i) dt[i] = T(cf->cfl) * T(m->cell_area[i]) / dt[i];
+        return;
+    }
+    for (int32_t i = 0; i < N; ++i) dt[i] = T(1.0);
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+        T le = T(m->edge_len[e]), n0 = T(m->edge_nx[e]), n1 = T(m->edge_ny[e]);
+        const T* qi = &q[V * i];
+        const T* qj = &q[V * j];
+        T ci = sqrt(pressure(gd, qi) * g / qi[0]);
+        T cj = sqrt(pressure(gd, qj) * g / qj[0]);
+        T ei = (ci + fabs(qi[1] / qi[0])) * n0 + (ci + fabs(qi[2] / qi[0])) * n1;
+        T ej = (cj + fabs(qj[1] / qj[0])) * n0 + (cj + fabs(qj[2] / qj[0])) * n1;
+        T dti = T(cf->cfl) * (T(m->cell_area[i]) / le) / fabs(ei);
+        T dtj = T(cf->cfl) * (T(m->cell_area[j]) / le) / fabs(ej);
+        dt[i] = smin(dt[i], dti);
+        dt[j] = smin(dt[j], dtj);
+    }
+}
+
+template <class T>
+void min_dt(const orc_mesh* m, T* dt) {
+    T mn = dt[0];
+    for (int32_t i = 0; i < m->n_cells; ++i) mn = smin(mn, dt[i]);
+    for (int32_t i = 0; i < m->n_cells; ++i) dt[i] = mn;
+}
+
+template <class T>
+void gradients(const orc_mesh* m, const T* q, T* gx, T* gy) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    for (int64_t i = 0; i < n; ++i) { gx[i] = T(0.); gy[i] = T(0.); }
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+        T nx = T(m->edge_nx[e]), ny = T(m->edge_ny[e]), le = T(m->edge_len[e]);
+        double dxif = m->edge_cx[e] - m->cell_cx[i];
+        double dyif = m->edge_cy[e] - m->cell_cy[i];
+        double dif = ::sqrt(dxif * dxif + dyif * dyif);
+        double dxij = m->cell_cx[i] - m->cell_cx[j];
+        double dyij = m->cell_cy[i] - m->cell_cy[j];
+        double dij = ::sqrt(dxij * dxij + dyij * dyij);
+        T gf = T(dif / dij);
+        if (i != j) {
+            T f[V];
+            for (int k = 0; k < V; ++k) f[k] = (q[V * i + k] * (T(1.0) - gf) + q[V * j + k] * gf) * le;
+            for (int k = 0; k < V; ++k) {
+                gx[V * i + k] += f[k] * nx;
+                gy[V * i + k] += f[k] * ny;
+                gx[V * j + k] -= f[k] * nx;
+                gy[V * j + k] -= f[k] * ny;
+            }
+        }
+    }
+    for (int32_t i = 0; i < m->n_real; ++i) {
+        const T invA = T(1. / m->cell_area[i]);
+        for (int k = 0; k < V; ++k) { gx[V * i + k] *= invA; gy[V * i + k] *= invA; }
+    }
+    for (int32_t i = m->n_real; i < m->n_cells; ++i)
+        for (int k = 0; k < V; ++k) { gx[V * i + k] = T(0.); gy[V * i + k] = T(0.); }
+}
+
+template <class T>
+void limiters(const orc_mesh* m, const orc_config* cf, const T* q, const T* gx, const T* gy, T* lim, T* qmin,
+              T* qmax) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    for (int64_t i = 0; i < n; ++i) { lim[i] = T(1.); qmin[i] = q[i]; qmax[i] = q[i]; }
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+        for (int k = 0; k < V; ++k) {
+            qmin[V * i + k] = smin(qmin[V * i + k], q[V * j + k]);
+            qmin[V * j + k] = smin(qmin[V * j + k], q[V * i + k]);
+            qmax[V * i + k] = smax(qmax[V * i + k], q[V * j + k]);
+            qmax[V * j + k] = smax(qmax[V * j + k], q[V * i + k]);
+        }
+    }
+    const T tol = T(1e-15);
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        int32_t ids[2] = {m->edge_cells[2 * e], m->edge_cells[2 * e + 1]};
+        for (int s = 0; s < 2; ++s) {
+            int32_t id = ids[s];
+            if (id >= m->n_real) continue;
+            T dx = T(m->edge_cx[e] - m->cell_cx[id]);
+            T dy = T(m->edge_cy[e] - m->cell_cy[id]);
+            double sqrt_area = ::sqrt(m->cell_area[id]);
+            for (int k = 0; k < V; ++k) {
+                T dqg = gx[V * id + k] * dx + gy[V * id + k] * dy;
+                T dmax = qmax[V * id + k] - q[V * id + k];
+                T dmin = qmin[V * id + k] - q[V * id + k];
+                const double Ka = cf->K * sqrt_area;
+                const T K3a = T(Ka * Ka * Ka);
+                const T d2 = (dmax - dmin) * (dmax - dmin);
+                T sig;
+                if (d2 <= K3a) {
+                    sig = T(1.);
+                } else if (d2 <= T(2) * K3a) {
+                    T yy = (d2 / K3a - T(1.0));
+                    sig = T(2.0) * yy * yy * yy - T(3.0) * yy * yy + T(1.0);
+                } else {
+                    sig = T(0.);
+                }
+                T l = T(1.0);
+                if (sig < T(1.0)) {
+                    if (dqg > tol) l = phi(cf, dmax / dqg);
+                    else if (dqg < -tol) l = phi(cf, dmin / dqg);
+                    else l = T(1.0);
+                }
+                l = sig + (T(1.0) - sig) * l;
+                lim[V * id + k] = smin(lim[V * id + k], l);
+            }
+        }
+    }
+}
+
+template <class T>
+bool physical(double g, const T* q) {
+    return q[0] > T(0) && pressure(g, q) > T(0);
+}
+
+template <class T>
+void time_derivatives(const orc_mesh* m, const orc_config* cf, const T* q, const T* gx, const T* gy, const T* lim,
+                      T* qt) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    for (int64_t i = 0; i < n; ++i) qt[i] = T(0.);
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+        T le = T(m->edge_len[e]);
+        double nn[2] = {m->edge_nx[e], m->edge_ny[e]};
+        double cx = m->edge_cx[e], cy = m->edge_cy[e];
+        T di0 = T(cx - m->cell_cx[i]), di1 = T(cy - m->cell_cy[i]);
+        T dj0 = T(cx - m->cell_cx[j]), dj1 = T(cy - m->cell_cy[j]);
+        T qi[V], qj[V], f[V];
+        if (cf->lin) {
+            for (int k = 0; k < V; ++k) {
+                int64_t ki = V * (int64_t)i + k, kj = V * (int64_t)j + k;
+                qi[k] = q[ki] + (gx[ki] * di0 + gy[ki] * di1) * lim[ki];
+                qj[k] = q[kj] + (gx[kj] * dj0 + gy[kj] * dj1) * lim[kj];
+            }
+        } else {
+            for (int k = 0; k < V; ++k) { qi[k] = q[V * i + k]; qj[k] = q[V * j + k]; }
+        }
+        if (cf->lin && cf->pos) {
+            if (!physical(cf->gamma, qi))
+                for (int k = 0; k < V; ++k) qi[k] = q[V * i + k];
+            if (!physical(cf->gamma, qj))
+                for (int k = 0; k < V; ++k) qj[k] = q[V * j + k];
+        }
+        flux(cf, f, qi, qj, nn);
+        for (int k = 0; k < V; ++k) {
+            qt[V * i + k] -= f[k] * le / T(m->cell_area[i]);
+            qt[V * j + k] += f[k] * le / T(m->cell_area[j]);
+        }
+    }
+    for (int32_t i = m->n_real; i < m->n_cells; ++i)
+        for (int k = 0; k < V; ++k) qt[V * i + k] = T(0.);
+}
+
+template <class T>
+void update_bounds(const orc_mesh* m, const orc_config* cf, T* q, const T* gx, const T* gy, const T* lim) {
+    for (int32_t b = 0; b < m->n_bnd; ++b) {
+        int32_t e = m->bnd_edge[b];
+        double nn[2] = {m->edge_nx[e], m->edge_ny[e]};
+        int32_t ii = m->edge_cells[2 * e], ib = m->edge_cells[2 * e + 1];
+        T qint[V];
+        if (cf->lin) {
+            T d0 = T(m->edge_cx[e] - m->cell_cx[ii]);
+            T d1 = T(m->edge_cy[e] - m->cell_cy[ii]);
+            for (int k = 0; k < V; ++k) {
+                int64_t kk = V * (int64_t)ii + k;
+                qint[k] = q[kk] + (gx[kk] * d0 + gy[kk] * d1) * lim[kk];
+            }
+            if (cf->pos && !physical(cf->gamma, qint))
+                for (int k = 0; k < V; ++k) qint[k] = q[V * ii + k];
+        } else {
+            for (int k = 0; k < V; ++k) qint[k] = q[V * ii + k];
+        }
+        bc(cf, m->bnd_bc[b], &q[V * ib], qint, nn);
+    }
+}
+
+// run() for a fixed number of steps (explicit.cpp:657-817 without output)
+template <class T>
+void run(const orc_mesh* m, const orc_config* cf, T* q, int steps) {
+    const int64_t N = m->n_cells, n = V * N;
+    std::vector<T> qk(n), qt(n), gx(n), gy(n), lim(n), qmin(n), qmax(n), dt(N);
+    update_bounds(m, cf, q, gx.data(), gy.data(), lim.data());
+    for (int step = 0; step < steps; ++step) {
+        calc_dt(m, cf, q, dt.data());
+        if (cf->gdt) min_dt(m, dt.data());
+        for (int64_t i = 0; i < n; ++i) qk[i] = q[i];
+        for (int s = 0; s < cf->n_alpha; ++s) {
+            if (cf->grad) gradients(m, qk.data(), gx.data(), gy.data());
+            if (cf->lim) limiters(m, cf, qk.data(), gx.data(), gy.data(), lim.data(), qmin.data(), qmax.data());
+            time_derivatives(m, cf, qk.data(), gx.data(), gy.data(), lim.data(), qt.data());
+            const T a = T(cf->alpha[s]);
+            for (int64_t c = 0; c < N; ++c)
+                for (int k = 0; k < V; ++k) qk[V * c + k] = q[V * c + k] + qt[V * c + k] * dt[c] * a;
+            update_bounds(m, cf, qk.data(), gx.data(), gy.data(), lim.data());
+        }
+        for (int64_t i = 0; i < n; ++i) q[i] = qk[i];
+    }
+}
+
+// J = sum_b p(ghost_b) (wx_b n_x + wy_b n_y) len_b (the GPU objective's form)
+template <class T>
+T objective(const orc_mesh* m, const orc_config* cf, const T* q, const double* wx, const double* wy) {
+    T J = T(0.);
+    for (int32_t b = 0; b < m->n_bnd; ++b) {
+        const int32_t e = m->bnd_edge[b];
+        const int32_t ghost = m->n_real + b;
+        const double proj = (wx ? wx[b] : 0.0) * m->edge_len[e] * m->edge_nx[e] +
+                            (wy ? wy[b] : 0.0) * m->edge_len[e] * m->edge_ny[e];
+        J += pressure(cf->gamma, &q[V * (int64_t)ghost]) * T(proj);
+    }
+    return J;
+}
+
+}  // namespace
+
+extern "C" {
+
+// double instantiation: final state after `steps` steps (all cells) and J
+int cst_run_f64(const orc_mesh* m, const orc_config* cf, const double* q0, int steps, const double* wx,
+                const double* wy, double* q_out, double* J) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    std::vector<double> q(q0, q0 + n);
+    run<double>(m, cf, q.data(), steps);
+    if (q_out) memcpy(q_out, q.data(), sizeof(double) * n);
+    if (J) *J = objective<double>(m, cf, q.data(), wx, wy);
+    return 0;
+}
+
+// complex step along dir: J(q0 + i h dir) -> J (real part) and dJ = Im / h
+int cst_derivative(const orc_mesh* m, const orc_config* cf, const double* q0, const double* dir, double h,
+                   int steps, const double* wx, const double* wy, double* J, double* dJ) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    std::vector<cs> q(n);
+    for (int64_t i = 0; i < n; ++i) q[i] = cs(q0[i], h * dir[i]);
+    run<cs>(m, cf, q.data(), steps);
+    const cs j = objective<cs>(m, cf, q.data(), wx, wy);
+    *J = j.re;
+    *dJ = j.im / h;
+    return 0;
+}
+
+}  // extern "C"

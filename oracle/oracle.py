@@ -283,6 +283,58 @@ def read_dump(path: str) -> Dict[str, np.ndarray]:
     return out
 
 
+_cst = None
+
+
+def _cstep():
+    global _cst
+    if _cst is None:
+        path = os.path.join(HERE, "libcstep.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/libcstep.so is not built (run make -C oracle oracle)")
+        L = C.CDLL(path)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        L.cst_run_f64.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcConfig), dp, C.c_int, C.c_void_p, C.c_void_p,
+                                  dp, C.POINTER(C.c_double)]
+        L.cst_derivative.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcConfig), dp, dp, C.c_double, C.c_int,
+                                     C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _cst = L
+    return _cst
+
+
+def _wptr(w):
+    if w is None:
+        return None, None
+    a = np.ascontiguousarray(w, dtype=np.float64)
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+def cstep_run(mesh: "Mesh", cfg: Dict, q0: np.ndarray, steps: int, wx=None, wy=None):
+    """The templated restatement (fvh_cstep.cpp) in double: (q after `steps`
+    steps of run(), J = sum_b p(ghost_b)(wx n_x + wy n_y) len_b)."""
+    c = make_config(cfg, mesh.names)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    q = np.zeros_like(q0)
+    J = C.c_double(0.0)
+    ax, px = _wptr(wx)
+    ay, py = _wptr(wy)
+    _cstep().cst_run_f64(mesh._p, C.byref(c), q0, int(steps), px, py, q, C.byref(J))
+    return q, float(J.value)
+
+
+def cstep_derivative(mesh: "Mesh", cfg: Dict, q0: np.ndarray, direction: np.ndarray, steps: int, wx=None,
+                     wy=None, h: float = 1e-30):
+    """Complex-step derivative of J along `direction`: (J, dJ)."""
+    c = make_config(cfg, mesh.names)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    J, dJ = C.c_double(0.0), C.c_double(0.0)
+    ax, px = _wptr(wx)
+    ay, py = _wptr(wy)
+    _cstep().cst_derivative(mesh._p, C.byref(c), q0, d, float(h), int(steps), px, py, C.byref(J), C.byref(dJ))
+    return float(J.value), float(dJ.value)
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_BIN)
 

@@ -80,6 +80,17 @@ def test_checkpoint_schedule_does_not_change_the_gradient():
     assert res[0][2]["snaps"] == 1 and res[3][2]["snaps"] == 12
 
 
+def test_adjoint_with_step_graphs_is_unchanged():
+    pm, cfg, w, q0 = _case()
+    a = Solver(pm, cfg).adjoint(q0, 5, wx=w)
+    s = Solver(pm, cfg)
+    s.use_graphs(True)
+    b = s.adjoint(q0, 5, wx=w)
+    c = s.adjoint(q0, 5, wx=w, snaps=2)
+    assert a[0] == b[0] == c[0]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[1], c[1])
+
+
 def test_adjoint_refuses_what_it_does_not_cover():
     from fvhyper_amd import FvhError
     m, cfg, ic = workloads.naca_airfoil(2000)          # local time step
@@ -89,3 +100,22 @@ def test_adjoint_refuses_what_it_does_not_cover():
     pm2, cfg2, w, q0 = _case()
     with pytest.raises(FvhError, match="FP64"):
         Solver(pm2, cfg2, precision="f32").adjoint(q0, 2, wx=w)
+
+
+def test_adjoint_matches_complex_step_of_the_restatement():
+    """Against the complex-step derivative of the templated CPU restatement
+    (exact on the branch taken, like the adjoint): agreement to 1e-8, the
+    full workload (limiter, positivity fallback, global time step), 6 steps."""
+    import oracle as O
+    m, cfg, ic = workloads.forward_step(n_per_unit=20)
+    pm = Mesh.from_arrays(m)
+    om = O.Mesh(m)
+    w = workloads.step_face_weights(pm)
+    q0 = workloads.bump_state(pm)
+    s = Solver(pm, cfg)
+    J, grad, st = s.adjoint(q0, 6, wx=w)
+    for v in _directions(pm, 4, 21):
+        Jc, dJ = O.cstep_derivative(om, cfg, q0, v, 6, wx=w)
+        assert abs(Jc - J) <= 1e-12 * abs(J)
+        print("  complex step %.15e adjoint %.15e" % (dJ, grad @ v), flush=True)
+        assert abs(grad @ v - dJ) <= 1e-8 * abs(dJ)
