@@ -87,6 +87,38 @@ def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step"):
             "sample": "%d RK5 steps of the %d-cell %s workload, C restatement, 1 thread" % (steps, C, workload)}
 
 
+# rocprofv3 PMC summaries committed under profiles/: (workload, precision,
+# kernel) -> (file, row prefix).  Each file's traffic table gives HBM bytes per
+# dispatch as 2.0 x FETCH_SIZE + WRITE_SIZE (factors from a plain device copy).
+PMC_PROFILES = {
+    ("step", "f64", "stage"): ("k_stage__mach3_step_975k_fp64.md", "k_stage<double"),
+    ("step", "f32", "stage"): ("k_stage__mach3_step_975k_fp32.md", "k_stage<float"),
+    ("step", "f64", "flux"): ("k_flux_tile__mach3_step_975k_fp64.md", "k_flux_tile<double"),
+    ("airfoil", "f64", "stage"): ("k_stage__naca0012_2m_fp64.md", "k_stage<double"),
+    ("sweep", "f32", "flux"): ("k_flux_tile__naca0012_sweep256_fp32.md", "k_flux_tile<float"),
+}
+
+
+def pmc_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of this
+    workload, or None when there is none."""
+    key = PMC_PROFILES.get((workload, precision, kernel))
+    if key is None:
+        return None
+    path = os.path.join(ROOT, "profiles", key[0])
+    if not os.path.exists(path):
+        return None
+    section = False
+    for line in open(path):
+        if line.startswith("## HBM traffic"):
+            section = True
+        elif line.startswith("## "):
+            section = False
+        elif section and line.startswith("| `" + key[1]):
+            return {"bytes": round(float(line.rstrip().rstrip("|").split("|")[-1]) * 1e6), "source": "profiles/" + key[0]}
+    return None
+
+
 def converge(s, C, stages, tol, stop_var, chunk=2000, limit=400000):
     """Run from the freestream until R[stop_var] < tol (the reference's
     max-over-variables rule for stop_var=-1); returns a dict of the run."""
@@ -191,6 +223,10 @@ def sweep(a):
                            "k_flux (edge-flux gather + RK update + boundary)") + ", all instances",
                 "bytes_per_launch": nbytes, "avg_launch_us": round(avg_ms * 1e3, 2),
                 "kernels_us": {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}}
+        tr = pmc_traffic("sweep", a.precision, key) if a.instances == 256 else None
+        if tr is not None:
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_source"] = tr["source"]
     out = {
         "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
@@ -310,6 +346,8 @@ def main():
     ap.add_argument("--per-mesh-steps", type=int, default=0,
                     help="sweep: steps of the one-by-one baseline (default: --steps)")
     ap.add_argument("--n-per-unit", type=int, default=440, help="step mesh resolution")
+    ap.add_argument("--flux-mode", default="auto", choices=["auto", "fused", "tile", "cell"],
+                    help="step/airfoil: stage form (auto: fused where it fits)")
     ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
     ap.add_argument("--converge", action="store_true",
                     help="airfoil: also run from the freestream until the density residual drops 6 orders")
@@ -340,7 +378,7 @@ def main():
     else:
         m, cfg, ic = workloads.naca_airfoil(a.cells)
     pm = Mesh.from_arrays(m)
-    s = Solver(pm, cfg, precision=a.precision)
+    s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode)
     q0 = workloads.initial_state(pm, ic)
     s.set_state(q0)
     s.apply_bounds()
@@ -392,6 +430,11 @@ def main():
                 "kernel": "k_flux (edge-flux gather + RK update + boundary)",
                 "bytes_per_launch": ab["flux"], "avg_launch_us": round(avg_ms * 1e3, 2)}
     if roof is not None:
+        tr = pmc_traffic(a.workload, a.precision, "stage" if "stage" in roof["kernel"] else "flux")
+        if tr is not None and (a.workload != "step" or a.n_per_unit == 440) and (
+                a.workload != "airfoil" or a.cells == 2_000_000):
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_source"] = tr["source"]
         stage_ms = sum(v["total_ms"] for v in kt.values()) / (a.steps * stages)
         roof["stage_achieved"] = round(ab["stage"] / (stage_ms * 1e-3) / 1e9, 1)
         roof["kernels_us"] = {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}
@@ -403,11 +446,13 @@ def main():
     if a.workload == "step":
         wl = {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
               "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
-              "cfl": cfg["cfl"], "dt": "global", "precision": a.precision, "parallelism": "single_gpu"}
+              "cfl": cfg["cfl"], "dt": "global", "precision": a.precision, "stage_form": a.flux_mode,
+              "parallelism": "single_gpu"}
     else:
         wl = {"workload": "naca0012_transonic", "cells": C, "edges": E, "boundary_edges": B,
               "order": 2, "limiter": "michalak", "mach": 0.8, "aoa_deg": 1.25, "rk_stages": stages,
-              "cfl": cfg["cfl"], "dt": "local", "precision": a.precision, "parallelism": "single_gpu"}
+              "cfl": cfg["cfl"], "dt": "local", "precision": a.precision, "stage_form": a.flux_mode,
+              "parallelism": "single_gpu"}
     out = {
         "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
