@@ -252,10 +252,12 @@ def adjoint(a):
     """Discrete adjoint: gradient of the step-face pressure force after
     --adj-steps RK5 steps of the second-order forward-step solver (h = 1/220,
     243,936 cells) from a seeded smooth perturbation of the inflow, with
-    respect to every initial cell state.  Reports the adjoint/forward cost
-    ratio, the checkpoint memory, and the agreement with central differences
-    (eps 1e-6 and 1e-8) on --adj-dirs random directions and with the
-    complex-step derivative of the CPU restatement on a smaller mesh."""
+    respect to every initial cell state and, in a second timed run, also to
+    every node coordinate.  Reports the adjoint/forward cost ratios, the
+    checkpoint memory, and the agreement with central differences (eps 1e-6
+    and 1e-8) on --adj-dirs random directions and with the complex-step
+    derivative of the CPU restatement on a smaller mesh (states and interior
+    node coordinates)."""
     import numpy as np
     import torch
 
@@ -278,6 +280,10 @@ def adjoint(a):
     t0 = time.perf_counter()
     Ja, grad, st = s.adjoint(q0, N, wx=w, mem_cap_bytes=8e9)
     t_adj = time.perf_counter() - t0
+    s.adjoint(q0, 2, wx=w, nodes=True)                       # warm-up
+    t0 = time.perf_counter()
+    _, _, gxy, _ = s.adjoint(q0, N, wx=w, mem_cap_bytes=8e9, nodes=True)
+    t_adj_nodes = time.perf_counter() - t0
     rng = np.random.default_rng(20240606)
     rel6, rel8 = [], []
     for _ in range(a.adj_dirs):
@@ -306,7 +312,18 @@ def adjoint(a):
             v = v.ravel()
             _, dJ = O.cstep_derivative(omc, cfgc, qc, v, Nc, wx=wc)
             errs.append(abs(float(gc @ v) - dJ) / abs(dJ))
-        cs = {"cells": pmc.n_real, "steps": Nc, "directions": 4, "max_rel_err": max(errs)}
+        # node coordinates: interior displacements of a tenth of the spacing
+        _, _, gxyc, _ = Solver(pmc, cfgc).adjoint(qc, Nc, wx=wc, nodes=True)
+        inner = np.ones(mc.x.size, dtype=bool)
+        inner[np.asarray(mc.bnd).ravel()] = False
+        nerrs = []
+        for _ in range(2):
+            dx = np.where(inner, rng.standard_normal(mc.x.size), 0.0) * 0.1 / 40
+            dy = np.where(inner, rng.standard_normal(mc.x.size), 0.0) * 0.1 / 40
+            _, dJ = O.cstep_derivative_nodes(omc, cfgc, qc, Nc, dx=dx, dy=dy, wx=wc)
+            nerrs.append(abs(float(gxyc[:, 0] @ dx + gxyc[:, 1] @ dy) - dJ) / abs(dJ))
+        cs = {"cells": pmc.n_real, "steps": Nc, "directions": 4, "max_rel_err": max(errs),
+              "node_directions": 2, "node_max_rel_err": max(nerrs)}
     except Exception as e:  # pragma: no cover - reported
         print("complex-step check skipped: %s" % e, file=sys.stderr)
     C = pm.n_real
@@ -318,7 +335,10 @@ def adjoint(a):
         "config": {"workload": "forward_step_adjoint", "cells": C, "steps": N, "stages": N * stages,
                    "objective": "pressure force on the step face", "parallelism": "single_gpu"},
         "adjoint": {"J": J, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
-                    "adjoint_over_forward": round(t_adj / t_fwd, 3), "snaps": st["snaps"],
+                    "adjoint_over_forward": round(t_adj / t_fwd, 3),
+                    "with_node_gradient_s": round(t_adj_nodes, 4),
+                    "with_node_gradient_over_forward": round(t_adj_nodes / t_fwd, 3),
+                    "node_gradient_norm": float(np.linalg.norm(gxy)), "snaps": st["snaps"],
                     "forward_steps_executed": st["forward_steps"],
                     "checkpoint_GB": round(st["checkpoint_bytes"] / 1e9, 3),
                     "work_GB": round(st["work_bytes"] / 1e9, 3),

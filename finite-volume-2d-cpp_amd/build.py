@@ -71,7 +71,8 @@ def build(arch: str = "gfx950", jobs: int = 4) -> str:
     # host-only checks of the physics and its reverse (CPU tests)
     cobj = compile_one("fvh_check.cpp", arch)
     tmp = CHECK_LIB + ".tmp"
-    r = subprocess.run([HIPCC, "--offload-arch=%s" % arch, "-shared", "-fPIC", "-o", tmp, cobj],
+    mobj = os.path.join(OUT, "fvh_mesh.cpp.o")   # host mesh code, for the geometry reverse
+    r = subprocess.run([HIPCC, "--offload-arch=%s" % arch, "-shared", "-fPIC", "-o", tmp, cobj, mobj],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

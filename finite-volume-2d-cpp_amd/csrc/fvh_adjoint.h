@@ -221,6 +221,40 @@ FVH_HD void boundary_vjp(const BcParam<T>& P, T g, const T* q, T n0, T n1, const
     }
 }
 
+// Adjoint of the edge normal through a boundary function: nb += (db/dn)^T bb
+// (two dual-number evaluations, one per normal component).
+template <class T>
+FVH_HD void boundary_vjp_n(const BcParam<T>& P, T g, const T* q, T n0, T n1, const T* bb, T* nb) {
+    BcParam<Dual<T>> Pd;
+    Pd.kind = P.kind;
+    Pd.rho = Dual<T>(P.rho); Pd.u = Dual<T>(P.u); Pd.v = Dual<T>(P.v); Pd.p = Dual<T>(P.p);
+    for (int j = 0; j < 2; ++j) {
+        Dual<T> qd[4], bd[4];
+        for (int k = 0; k < 4; ++k) qd[k] = Dual<T>(q[k]);
+        boundary(Pd, Dual<T>(g), bd, qd, Dual<T>(n0, j == 0 ? T(1) : T(0)), Dual<T>(n1, j == 1 ? T(1) : T(0)));
+        T s = T(0);
+        for (int k = 0; k < 4; ++k) s = s + bd[k].d * bb[k];
+        nb[j] = nb[j] + s;
+    }
+}
+
+// Normal part of the reverse of eig_sum (e = c + |u.n|): nb += eb d e / d n.
+template <class T>
+FVH_HD void eig_sum_vjp_n(const T* q, T n0, T n1, T eb, T* nb) {
+    const T un = q[1] / q[0] * n0 + q[2] / q[0] * n1;
+    const T unb = (un < T(0)) ? -eb : eb;
+    nb[0] = nb[0] + unb * (q[1] / q[0]);
+    nb[1] = nb[1] + unb * (q[2] / q[0]);
+}
+
+// Normal part of the reverse of eig_edge (e = (c+|u|) n0 + (c+|v|) n1).
+template <class T>
+FVH_HD void eig_edge_vjp_n(T g, const T* q, T eb, T* nb) {
+    const T c = sqrt(pressure(g, q) * g / q[0]);
+    nb[0] = nb[0] + eb * (c + sabs(q[1] / q[0]));
+    nb[1] = nb[1] + eb * (c + sabs(q[2] / q[0]));
+}
+
 // d phi / d y of a limiter kind at y.
 template <class T, int PHI>
 FVH_HD T phi_deriv(T y, T yt) {

@@ -35,7 +35,7 @@ __device__ inline void q4_add(T* a, const Q4<T>& b) {
 }
 
 // ---- update pass --------------------------------------------------------
-template <class T, bool LIN>
+template <class T, bool LIN, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm, PhysParams pp,
                                                        const BcParam<T>* __restrict__ bcs,
                                                        const Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ gx,
@@ -44,7 +44,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
                                                        const T* __restrict__ dt_g, T alpha,
                                                        const Q4<T>* __restrict__ lam_out, Q4<T>* __restrict__ lam_q0,
                                                        Q4<T>* __restrict__ lam_qt, Q4<T>* __restrict__ lam_bincr,
-                                                       T* __restrict__ lam_dt, double* __restrict__ ld_blk) {
+                                                       T* __restrict__ lam_dt, double* __restrict__ ld_blk,
+                                                       GeoAdj<T> ga) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int c = tm.c0[lb] + threadIdx.x;
@@ -84,6 +85,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
             const Q4<T> lg = lam_out[nb];
             T qb[4] = {T(0), T(0), T(0), T(0)};
             boundary_vjp(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lg.v, qb);
+            if (GEO) {
+                // the boundary function's dependence on the edge normal
+                T nb2[2] = {T(0), T(0)};
+                boundary_vjp_n(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lg.v, nb2);
+                Q4<T> a = ga.he[h];
+                a.v[0] = a.v[0] + nb2[0];
+                a.v[1] = a.v[1] + nb2[1];
+                ga.he[h] = a;
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k) lam[k] = lam[k] + qb[k];
             Q4<T> lbi;
@@ -105,6 +115,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
             s = s + lam[k] * t.v[k] * alpha;
         }
         lam_qt[c] = lq;
+        if (GEO) {
+            // qt = (sum of signed f len) / A, so d qt / d A = -qt / A
+            const T A = m.c_area[c];
+            T sA = T(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sA = sA - lq.v[k] * t.v[k] / A;
+            Q4<T> a = ga.c[c];
+            a.v[0] = a.v[0] + sA;
+            ga.c[c] = a;
+        }
         if (dt_g) ld = (double)s;
         else if (lam_dt) lam_dt[c] = lam_dt[c] + s;
     }
@@ -116,12 +136,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
 }
 
 // ---- flux pass (one thread per edge) -------------------------------------
-template <class T, bool LIN>
+template <class T, bool LIN, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams pp, int n_edges,
                                                      const int4* __restrict__ epair, const Q4<T>* __restrict__ qin,
                                                      const Q4<T>* __restrict__ gx, const Q4<T>* __restrict__ gy,
                                                      const Q4<T>* __restrict__ lm, const Q4<T>* __restrict__ lam_qt,
-                                                     Q4<T>* __restrict__ lam_face) {
+                                                     Q4<T>* __restrict__ lam_face, GeoAdj<T> ga) {
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= n_edges) return;
     const int4 ep = epair[e];   // (half-edge side 0, half-edge side 1, cell side 0, cell side 1)
@@ -169,10 +189,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams p
     for (int k = 0; k < 4; ++k) { a.v[k] = qib[k]; b.v[k] = qjb[k]; }
     lam_face[ep.x] = a;
     lam_face[ep.y] = b;
+    if (GEO) {
+        // qt_i -= f len / A_i, qt_j += f len / A_j: the length's adjoint is
+        // sum_k f_k (lam_qt_j / A_j - lam_qt_i / A_i)_k
+        T f[4];
+        roe_flux<T, false>(g, f, fs[0], fs[1], geo.v[0], geo.v[1]);
+        T ll = T(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ll = ll + fb[k] / geo.v[2] * f[k];
+        Q4<T> ge = ga.e[e];
+        ge.v[0] = ge.v[0] + nb[0];
+        ge.v[1] = ge.v[1] + nb[1];
+        ge.v[2] = ge.v[2] + ll;
+        ga.e[e] = ge;
+    }
 }
 
 // ---- gradient/limiter pass -------------------------------------------------
-template <class T, int PHI, bool LIN>
+template <class T, int PHI, bool LIN, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad,
                                                         int lim_on, const Q4<T>* __restrict__ qin,
                                                         const Q4<T>* __restrict__ gxs, const Q4<T>* __restrict__ gys,
@@ -180,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                                                         const Q4<T>* __restrict__ lam_face,
                                                         const Q4<T>* __restrict__ lam_bincr,
                                                         Q4<T>* __restrict__ lam_direct,
-                                                        Q4<T>* __restrict__ nb_contrib) {
+                                                        Q4<T>* __restrict__ nb_contrib, GeoAdj<T> ga) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int c = tm.c0[lb] + threadIdx.x;
@@ -197,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
     }
     T lqc[4] = {T(0), T(0), T(0), T(0)};
     T lgx[4] = {T(0), T(0), T(0), T(0)}, lgy[4] = {T(0), T(0), T(0), T(0)}, llim[4] = {T(0), T(0), T(0), T(0)};
+    T lA = T(0), lcx = T(0), lcy = T(0);   // geometry adjoint of this cell (GEO)
     // face states q + inc (and boundary increments) reversed
     for (int h = h0; h < h1; ++h) {
         const uint32_t ue = m.he_edge[h];
@@ -225,6 +260,21 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
             lgx[k] = lgx[k] + li[k] * lim[k] * dx;
             lgy[k] = lgy[k] + li[k] * lim[k] * dy;
             llim[k] = llim[k] + li[k] * (gx[k] * dx + gy[k] * dy);
+        }
+        if (GEO) {
+            // inc = (gx dx + gy dy) lim with (dx, dy) = edge centre - cell centre
+            T ldx = T(0), ldy = T(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ldx = ldx + li[k] * gx[k] * lim[k];
+                ldy = ldy + li[k] * gy[k] * lim[k];
+            }
+            G2<T> a = ga.hc[h];
+            a.v[0] = a.v[0] + ldx;
+            a.v[1] = a.v[1] + ldy;
+            ga.hc[h] = a;
+            lcx = lcx - ldx;
+            lcy = lcy - ldy;
         }
     }
     // limiter reversed (explicit.cpp:174-258)
@@ -313,6 +363,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
             ldmin = ldmin - T(2) * (dmax - dmin) * lspread;
             lgx[k] = lgx[k] + ldqg * dx;
             lgy[k] = lgy[k] + ldqg * dy;
+            if (GEO) {
+                // dqg = gx dx + gy dy at the selected edge
+                G2<T> a = ga.hc[hs];
+                a.v[0] = a.v[0] + ldqg * gx[k];
+                a.v[1] = a.v[1] + ldqg * gy[k];
+                ga.hc[hs] = a;
+                lcx = lcx - ldqg * gx[k];
+                lcy = lcy - ldqg * gy[k];
+                // sig(spread / K3a) with K3a = (K sqrt(A))^3: d sig / d K3a =
+                // -dsig spread / K3a and d K3a / d A = 1.5 K3a / A
+                lA = lA - lsig * dsig * spread / K3a * (T(1.5) * K3a / m.c_area[c]);
+            }
             lqc[k] = lqc[k] - ldmax - ldmin;
             if (hmx[k] < 0) lqc[k] = lqc[k] + ldmax;
             else { Q4<T> t = nb_contrib[hmx[k]]; t.v[k] = t.v[k] + ldmax; nb_contrib[hmx[k]] = t; }
@@ -326,6 +388,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
         T sx[4], sy[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) { sx[k] = lgx[k] * ia; sy[k] = lgy[k] * ia; }
+        if (GEO) {
+            // g = (1/A) S: d g / d A = -g / A
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lA = lA - (lgx[k] * gx[k] + lgy[k] * gy[k]) / m.c_area[c];
+        }
         for (int h = h0; h < h1; ++h) {
             const int nb = m.he_nbr[h];
             if (nb == c) continue;
@@ -345,12 +412,43 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                 t.v[k] = t.v[k] + lf * (side ? w1 : w);
             }
             nb_contrib[h] = t;
+            if (GEO) {
+                // gx += +-f nx, gy += +-f ny with f = F len, F = qi (1-w) + qj w
+                const Q4<T> qn = qin[nb];
+                T lnx = T(0), lny = T(0), ll = T(0), lw = T(0);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T qi = side ? qn.v[k] : qc.v[k];
+                    const T qj = side ? qc.v[k] : qn.v[k];
+                    const T F = qi * w1 + qj * w;
+                    T lfk = sx[k] * geo.v[0] + sy[k] * geo.v[1];   // adjoint of f_k
+                    T gxs = sx[k], gys = sy[k];
+                    if (side) { lfk = -lfk; gxs = -gxs; gys = -gys; }
+                    lnx = lnx + gxs * F * geo.v[2];
+                    lny = lny + gys * F * geo.v[2];
+                    ll = ll + lfk * F;
+                    lw = lw + lfk * geo.v[2] * (qj - qi);
+                }
+                Q4<T> a = ga.he[h];
+                a.v[0] = a.v[0] + lnx;
+                a.v[1] = a.v[1] + lny;
+                a.v[2] = a.v[2] + ll;
+                a.v[3] = a.v[3] + lw;
+                ga.he[h] = a;
+            }
         }
     }
     Q4<T> o;
 #pragma unroll
     for (int k = 0; k < 4; ++k) o.v[k] = lqc[k];
     lam_direct[c] = o;
+    if (GEO) {
+        Q4<T> a = ga.c[c];
+        a.v[0] = a.v[0] + lA;
+        a.v[1] = a.v[1] + lcx;
+        a.v[2] = a.v[2] + lcy;
+        ga.c[c] = a;
+    }
 }
 
 // ---- gather pass ---------------------------------------------------------
@@ -379,10 +477,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(MeshView<T> m, const Q4<T
 // ---- time step reversed for one cell (the minimum of a global step) --------
 // summed form: dt = cfl A / S, S = sum_h max(e_own, e_nbr) len_h
 // edge-minimum form: dt = min(1, min_h cfl (A / len_h) / |e_h|)
-template <class T, int KIND>
+template <class T, int KIND, bool GEO>
 __global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q,
                               const int32_t* __restrict__ cell, const double* __restrict__ ld_blk, int n_blk,
-                              Q4<T>* __restrict__ lam_q0) {
+                              Q4<T>* __restrict__ lam_q0, GeoAdj<T> ga) {
     if (blockIdx.x != 0) return;
     // sum of the per-tile partials, in a fixed pattern (deterministic)
     double part[4] = {0.0, 0.0, 0.0, 0.0};
@@ -408,6 +506,11 @@ __global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restr
         }
         const T dtc = T(pp.cfl) * m.c_area[c] / S;
         const T Sb = -lam_d * dtc / S;
+        if (GEO) {
+            Q4<T> a = ga.c[c];
+            a.v[0] = a.v[0] + lam_d * dtc / m.c_area[c];
+            ga.c[c] = a;
+        }
         for (int h = h0; h < h1; ++h) {
             const uint32_t ue = m.he_edge[h];
             const Q4<T> geo = m.e_geo[ue & kEdgeMask];
@@ -425,6 +528,16 @@ __global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restr
             Q4<T> l = lam_q0[who];
             for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + qb[k];
             lam_q0[who] = l;
+            if (GEO) {
+                // S = sum max(e_own, e_nbr) len: the length and, through e, the normal
+                T nb2[2] = {T(0), T(0)};
+                eig_sum_vjp_n(own ? qc.v : qn.v, geo.v[0], geo.v[1], Sb * geo.v[2], nb2);
+                Q4<T> a = ga.he[h];
+                a.v[0] = a.v[0] + nb2[0];
+                a.v[1] = a.v[1] + nb2[1];
+                a.v[2] = a.v[2] + Sb * (own ? eo : en);
+                ga.he[h] = a;
+            }
         }
     } else {
         T r = T(1);
@@ -445,6 +558,19 @@ __global__ void k_adj_dt_cell(MeshView<T> m, PhysParams pp, const Q4<T>* __restr
         Q4<T> l = lam_q0[c];
         for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + qb[k];
         lam_q0[c] = l;
+        if (GEO) {
+            // r = cfl (A / len) / |e|
+            T nb2[2] = {T(0), T(0)};
+            eig_edge_vjp_n(g, qc.v, eb, nb2);
+            Q4<T> a = ga.he[hs];
+            a.v[0] = a.v[0] + nb2[0];
+            a.v[1] = a.v[1] + nb2[1];
+            a.v[2] = a.v[2] - lam_d * r / geo.v[2];
+            ga.he[hs] = a;
+            Q4<T> b = ga.c[c];
+            b.v[0] = b.v[0] + lam_d * r / m.c_area[c];
+            ga.c[c] = b;
+        }
     }
 }
 
@@ -501,10 +627,11 @@ __global__ __launch_bounds__(kBlock) void k_argmin_final(int nb, const double* _
 }
 
 // ---- initial boundary update reversed (explicit.cpp:712) -------------------
-template <class T>
+template <class T, bool GEO>
 __global__ __launch_bounds__(kBlock) void k_adj_bounds_init(MeshView<T> m, PhysParams pp,
                                                             const BcParam<T>* __restrict__ bcs,
-                                                            const Q4<T>* __restrict__ q, Q4<T>* __restrict__ lam) {
+                                                            const Q4<T>* __restrict__ q, Q4<T>* __restrict__ lam,
+                                                            GeoAdj<T> ga) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= m.n_real) return;
     const T g = T(pp.gamma);
@@ -517,6 +644,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_bounds_init(MeshView<T> m, PhysP
         const Q4<T> geo = m.e_geo[ue & kEdgeMask];
         const Q4<T> lg = lam[m.he_nbr[h]];
         boundary_vjp(bcs[m.he_bc[h]], g, qc.v, geo.v[0], geo.v[1], lg.v, acc);
+        if (GEO) {
+            T nb2[2] = {T(0), T(0)};
+            boundary_vjp_n(bcs[m.he_bc[h]], g, qc.v, geo.v[0], geo.v[1], lg.v, nb2);
+            Q4<T> a = ga.he[h];
+            a.v[0] = a.v[0] + nb2[0];
+            a.v[1] = a.v[1] + nb2[1];
+            ga.he[h] = a;
+        }
         any = true;
     }
     if (any) {
@@ -531,7 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_bounds_init(MeshView<T> m, PhysP
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_objective(int n_b, PhysParams pp, const int32_t* __restrict__ ghost,
                                                       const Q4<T>* __restrict__ geo_w, const Q4<T>* __restrict__ q,
-                                                      double* __restrict__ terms, Q4<T>* __restrict__ lam) {
+                                                      double* __restrict__ terms, Q4<T>* __restrict__ lam,
+                                                      double* __restrict__ p_out) {
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= n_b) return;
     const Q4<T> gw = geo_w[b];   // (wx, wy, nx, ny) ... len folded into the weights
@@ -539,7 +675,9 @@ __global__ __launch_bounds__(kBlock) void k_objective(int n_b, PhysParams pp, co
     const Q4<T> qg = q[gc];
     const T g = T(pp.gamma);
     const T proj = gw.v[0] * gw.v[2] + gw.v[1] * gw.v[3];
-    terms[b] = (double)(pressure(g, qg.v) * proj);
+    const T p = pressure(g, qg.v);
+    terms[b] = (double)(p * proj);
+    if (p_out) p_out[b] = (double)p;
     if (lam) {
         T l[4] = {T(0), T(0), T(0), T(0)};
         pressure_vjp(g, qg.v, proj, l);
@@ -564,26 +702,30 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
                       const Q4<T>* qin, const Q4<T>* qout, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lm,
                       const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
                       Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
-                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk) {
+                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk, const GeoAdj<T>& ga) {
     const int nrt = xcd_grid_blocks(tm.n_rt);
     if (nrt == 0) return;
-    if (lin)
-        hipLaunchKernelGGL((k_adj_update<T, true>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, gx, gy, lm,
-                           qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt, ld_blk);
-    else
-        hipLaunchKernelGGL((k_adj_update<T, false>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, gx, gy, lm,
-                           qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt, ld_blk);
+    const bool geo = ga.he != nullptr;
+#define FVH_AU(L, G) hipLaunchKernelGGL((k_adj_update<T, L, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, \
+                                        gx, gy, lm, qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt,  \
+                                        ld_blk, ga)
+    if (lin) { if (geo) FVH_AU(true, true); else FVH_AU(true, false); }
+    else { if (geo) FVH_AU(false, true); else FVH_AU(false, false); }
+#undef FVH_AU
     const int ne = nblocks(n_edges);
-    if (lin)
-        hipLaunchKernelGGL((k_adj_flux<T, true>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, qin, gx, gy, lm,
-                           (const Q4<T>*)lam_qt, lam_face);
-    else
-        hipLaunchKernelGGL((k_adj_flux<T, false>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, qin, gx, gy,
-                           lm, (const Q4<T>*)lam_qt, lam_face);
-#define FVH_AG(P, L) hipLaunchKernelGGL((k_adj_gradlim<T, P, L>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, (int)grad, \
-                                        (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face, (const Q4<T>*)lam_bincr,    \
-                                        lam_direct, nb_contrib)
-#define FVH_AGL(P) do { if (lin) FVH_AG(P, true); else FVH_AG(P, false); } while (0)
+#define FVH_AF(L, G) hipLaunchKernelGGL((k_adj_flux<T, L, G>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair,  \
+                                        qin, gx, gy, lm, (const Q4<T>*)lam_qt, lam_face, ga)
+    if (lin) { if (geo) FVH_AF(true, true); else FVH_AF(true, false); }
+    else { if (geo) FVH_AF(false, true); else FVH_AF(false, false); }
+#undef FVH_AF
+#define FVH_AG(P, L, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp,      \
+                                           (int)grad, (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face,              \
+                                           (const Q4<T>*)lam_bincr, lam_direct, nb_contrib, ga)
+#define FVH_AGL(P)                                                        \
+    do {                                                                  \
+        if (lin) { if (geo) FVH_AG(P, true, true); else FVH_AG(P, true, false); }     \
+        else { if (geo) FVH_AG(P, false, true); else FVH_AG(P, false, false); }       \
+    } while (0)
     switch (phi_kind) {
         case PHI_ZERO: FVH_AGL(PHI_ZERO); break;
         case PHI_MICHALAK: FVH_AGL(PHI_MICHALAK); break;
@@ -608,25 +750,33 @@ void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, dou
 
 template <class T>
 void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q,
-                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0) {
-    if (dt_kind == 1)
-        hipLaunchKernelGGL((k_adj_dt_cell<T, 1>), dim3(1), dim3(kBlock), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
-    else if (dt_kind == 2)
-        hipLaunchKernelGGL((k_adj_dt_cell<T, 2>), dim3(1), dim3(kBlock), 0, s, m, pp, q, cell, ld_blk, n_blk, lam_q0);
+                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0, const GeoAdj<T>& ga) {
+    const bool geo = ga.he != nullptr;
+#define FVH_AD(K, G) hipLaunchKernelGGL((k_adj_dt_cell<T, K, G>), dim3(1), dim3(kBlock), 0, s, m, pp, q, cell, ld_blk, \
+                                        n_blk, lam_q0, ga)
+    if (dt_kind == 1) { if (geo) FVH_AD(1, true); else FVH_AD(1, false); }
+    else if (dt_kind == 2) { if (geo) FVH_AD(2, true); else FVH_AD(2, false); }
+#undef FVH_AD
 }
 
 template <class T>
 void launch_adj_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs,
-                            const Q4<T>* q, Q4<T>* lam) {
+                            const Q4<T>* q, Q4<T>* lam, const GeoAdj<T>& ga) {
     if (m.n_real == 0) return;
-    hipLaunchKernelGGL((k_adj_bounds_init<T>), dim3(nblocks(m.n_real)), dim3(kBlock), 0, s, m, pp, bcs, q, lam);
+    if (ga.he)
+        hipLaunchKernelGGL((k_adj_bounds_init<T, true>), dim3(nblocks(m.n_real)), dim3(kBlock), 0, s, m, pp, bcs, q,
+                           lam, ga);
+    else
+        hipLaunchKernelGGL((k_adj_bounds_init<T, false>), dim3(nblocks(m.n_real)), dim3(kBlock), 0, s, m, pp, bcs, q,
+                           lam, ga);
 }
 
 template <class T>
 void launch_objective(hipStream_t s, int n_b, const PhysParams& pp, const int32_t* ghost, const Q4<T>* geo_w,
-                      const Q4<T>* q, double* terms, Q4<T>* lam) {
+                      const Q4<T>* q, double* terms, Q4<T>* lam, double* p_out) {
     if (n_b == 0) return;
-    hipLaunchKernelGGL((k_objective<T>), dim3(nblocks(n_b)), dim3(kBlock), 0, s, n_b, pp, ghost, geo_w, q, terms, lam);
+    hipLaunchKernelGGL((k_objective<T>), dim3(nblocks(n_b)), dim3(kBlock), 0, s, n_b, pp, ghost, geo_w, q, terms, lam,
+                       p_out);
 }
 
 template <class T>
@@ -640,15 +790,17 @@ template void launch_adj_stage<double>(hipStream_t, const MeshView<double>&, con
                                        const Q4<double>*, const Q4<double>*, const Q4<double>*, const Q4<double>*,
                                        const Q4<double>*, const Q4<double>*, const double*, const double*, double,
                                        const Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*,
-                                       Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*);
+                                       Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*,
+                                       const GeoAdj<double>&);
 template void launch_adj_dt<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int, const Q4<double>*,
-                                    const int32_t*, const double*, int, Q4<double>*);
+                                    const int32_t*, const double*, int, Q4<double>*, const GeoAdj<double>&);
 template void launch_dt_argmin<double>(hipStream_t, const double*, const int32_t*, int, double*, int32_t*, int32_t*,
                                        int32_t*);
 template void launch_adj_bounds_init<double>(hipStream_t, const MeshView<double>&, const PhysParams&,
-                                             const BcParam<double>*, const Q4<double>*, Q4<double>*);
+                                             const BcParam<double>*, const Q4<double>*, Q4<double>*,
+                                             const GeoAdj<double>&);
 template void launch_objective<double>(hipStream_t, int, const PhysParams&, const int32_t*, const Q4<double>*,
-                                       const Q4<double>*, double*, Q4<double>*);
+                                       const Q4<double>*, double*, Q4<double>*, double*);
 template void launch_fill<double>(hipStream_t, Q4<double>*, int);
 
 }  // namespace fvh

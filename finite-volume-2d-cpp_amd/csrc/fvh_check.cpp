@@ -3,10 +3,41 @@
 // and its dual-number forward derivative; the boundary functions likewise.
 // Nothing in the solver path calls these.
 #include "fvh_adjoint.h"
+#include "fvh_internal.h"
+
+#include <string>
+#include <vector>
 
 using namespace fvh;
 
 extern "C" {
+
+// Reverse of the mesh geometry (mesh_geometry_vjp) on a mesh built from
+// file-order arrays (as fvh_mesh_from_arrays takes them).  bars: per edge
+// (n_x, n_y, len, Green-Gauss weight, centre x, centre y), then per cell,
+// ghosts included, (area, centre x, centre y); gxy: 2 per node.
+int fvhc_geometry_vjp(int n_nodes, const double* x, const double* y, int n_cells, const int* cells, int n_bnd,
+                      const int* bnd, const int* tags, const double* bars, double* gxy) {
+    int nt = 0;
+    for (int b = 0; b < n_bnd; ++b) nt = tags[b] + 1 > nt ? tags[b] + 1 : nt;
+    std::vector<std::string> names;
+    for (int k = 0; k < nt; ++k) names.push_back("b" + std::to_string(k));
+    fvh_mesh m;
+    const int rc = build_mesh(m, n_nodes, x, y, n_cells, cells, n_bnd, bnd, tags, names);
+    if (rc) return rc;
+    const int E = m.n_edges(), N = m.n_cells;
+    GeoBar b;
+    b.resize(E, N);
+    std::vector<double>* ev[6] = {&b.e_nx, &b.e_ny, &b.e_len, &b.e_w, &b.e_cx, &b.e_cy};
+    std::vector<double>* cv[3] = {&b.c_area, &b.c_cx, &b.c_cy};
+    for (int e = 0; e < E; ++e)
+        for (int k = 0; k < 6; ++k) (*ev[k])[e] = bars[6 * e + k];
+    for (int c = 0; c < N; ++c)
+        for (int k = 0; k < 3; ++k) (*cv[k])[c] = bars[6 * E + 3 * c + k];
+    for (int i = 0; i < 2 * n_nodes; ++i) gxy[i] = 0.0;
+    mesh_geometry_vjp(m, b, gxy);
+    return 0;
+}
 
 void fvhc_roe(double g, int joined, const double* qi, const double* qj, const double* n, double* f) {
     if (joined) roe_flux<double, true>(g, f, qi, qj, n[0], n[1]);
@@ -56,6 +87,18 @@ double fvhc_eig_jvp(int kind, double g, const double* q, const double* n, const 
     const Dual<double> e = kind == 1 ? eig_sum(Dual<double>(g), a, Dual<double>(n[0]), Dual<double>(n[1]))
                                      : eig_edge(Dual<double>(g), a, Dual<double>(n[0]), Dual<double>(n[1]));
     return e.d;
+}
+
+// adjoints of the edge normal through a boundary function and through the
+// time-step wave speeds
+void fvhc_bc_vjp_n(int kind, const double* par, double g, const double* q, const double* n, const double* bb,
+                   double* nb) {
+    boundary_vjp_n<double>(bcp(kind, par), g, q, n[0], n[1], bb, nb);
+}
+
+void fvhc_eig_vjp_n(int kind, double g, const double* q, const double* n, double eb, double* nb) {
+    if (kind == 1) eig_sum_vjp_n<double>(q, n[0], n[1], eb, nb);
+    else eig_edge_vjp_n<double>(g, q, eb, nb);
 }
 
 double fvhc_phi_deriv(int kind, double y, double yt) {

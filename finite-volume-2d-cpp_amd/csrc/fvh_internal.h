@@ -39,6 +39,22 @@ int32_t build_mesh(fvh_mesh& m, int32_t n_nodes, const double* x, const double* 
 
 int32_t read_msh(const char* path, fvh_mesh& m);
 
+// Adjoints of the quantities derived from the node coordinates: per edge
+// (normal, length, Green-Gauss weight, centre) and per cell, ghosts included
+// (area, centre).
+struct GeoBar {
+    std::vector<double> e_nx, e_ny, e_len, e_w, e_cx, e_cy;
+    std::vector<double> c_area, c_cx, c_cy;
+    void resize(int32_t n_edges, int32_t n_cells);
+};
+
+// Reverse of the geometry build_mesh derives from the node coordinates
+// (mesh.cpp:190-260, ghost cells mesh.cpp:642-655) and of the Green-Gauss
+// weight |x_f - x_i| / |x_i - x_j| (explicit.cpp:132-140): adds dJ/dx_k and
+// dJ/dy_k to gxy[2k], gxy[2k+1].  b is consumed.  The orientation flip of a
+// normal and the sign of an area are piecewise constant and kept.
+void mesh_geometry_vjp(const fvh_mesh& m, GeoBar& b, double* gxy);
+
 // Reverse Cuthill-McKee order of the real cells: order[k] = reference cell
 // placed at device position k.  Deterministic (ties by cell index).
 std::vector<int32_t> rcm_order(const fvh_mesh& m);

@@ -161,29 +161,43 @@ void launch_ctrl_begin(hipStream_t s, int n_inst, Ctrl* ctrl, uint32_t max_step,
                        int stop_var);
 void launch_ctrl_end(hipStream_t s, const InstView& iv, Ctrl* ctrl, const double* res_blk, int need_res,
                      const void* dt_src, int dt_is_f32, int dt_is_inst);
+// Adjoint of the mesh geometry, summed over every reversed stage and step:
+// per half-edge (n_x, n_y, length, Green-Gauss weight) and (edge centre x, y),
+// written only by the thread of the half-edge's cell; per edge (n_x, n_y,
+// length, unused) from the edge pass; per cell (area, centre x, y, unused).
+// With he == nullptr no geometry adjoint is formed.
+template <class T>
+struct GeoAdj {
+    Q4<T>* he = nullptr;
+    G2<T>* hc = nullptr;
+    Q4<T>* e = nullptr;
+    Q4<T>* c = nullptr;
+};
+
 // Discrete adjoint (fvh_adjoint.hip): the reverse of one RK stage, of the
 // global time step's minimum cell, of the initial boundary update, and the
-// boundary-pressure objective with its seed.
+// boundary-pressure objective with its seed.  ga: geometry adjoint (or none).
 template <class T>
 void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
                       bool grad, bool lim, bool lin, const BcParam<T>* bcs, int n_edges, const int4* epair,
                       const Q4<T>* qin, const Q4<T>* qout, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lm,
                       const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
                       Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
-                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk);
+                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk, const GeoAdj<T>& ga);
 constexpr int kArgminBlocks = 256;
 template <class T>
 void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, double* bv, int32_t* bref,
                       int32_t* bdev, int32_t* out);
 template <class T>
 void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q,
-                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0);
+                   const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0, const GeoAdj<T>& ga);
 template <class T>
 void launch_adj_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs,
-                            const Q4<T>* q, Q4<T>* lam);
+                            const Q4<T>* q, Q4<T>* lam, const GeoAdj<T>& ga);
+// terms[b] = p_b (w.n) len per boundary element; p_out (optional) = p_b
 template <class T>
 void launch_objective(hipStream_t s, int n_b, const PhysParams& pp, const int32_t* ghost, const Q4<T>* geo_w,
-                      const Q4<T>* q, double* terms, Q4<T>* lam);
+                      const Q4<T>* q, double* terms, Q4<T>* lam, double* p_out);
 template <class T>
 void launch_fill(hipStream_t s, Q4<T>* a, int n);
 

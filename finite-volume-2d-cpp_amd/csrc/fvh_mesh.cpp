@@ -69,6 +69,101 @@ class EdgeTable {
 
 }  // namespace
 
+void GeoBar::resize(int32_t n_edges, int32_t n_cells) {
+    for (auto* v : {&e_nx, &e_ny, &e_len, &e_w, &e_cx, &e_cy}) v->assign(n_edges, 0.0);
+    for (auto* v : {&c_area, &c_cx, &c_cy}) v->assign(n_cells, 0.0);
+}
+
+void mesh_geometry_vjp(const fvh_mesh& m, GeoBar& b, double* gxy) {
+    const int32_t E = m.n_edges();
+    const double* x = m.x.data();
+    const double* y = m.y.data();
+    // Green-Gauss weight w = dif / dij of each edge (cells i, j: first and second owner)
+    for (int32_t e = 0; e < E; ++e) {
+        const double lw = b.e_w[e];
+        if (lw == 0.0) continue;
+        const int32_t i = m.edge_cells[2 * (int64_t)e], j = m.edge_cells[2 * (int64_t)e + 1];
+        const double dxif = m.e_cx[e] - m.c_cx[i], dyif = m.e_cy[e] - m.c_cy[i];
+        const double dif = std::sqrt(dxif * dxif + dyif * dyif);
+        const double dxij = m.c_cx[i] - m.c_cx[j], dyij = m.c_cy[i] - m.c_cy[j];
+        const double dij = std::sqrt(dxij * dxij + dyij * dyij);
+        if (!(dij > 0.0)) continue;
+        const double ldif = lw / dij, ldij = -lw * dif / (dij * dij);
+        if (dif > 0.0) {
+            b.e_cx[e] += ldif * dxif / dif;
+            b.e_cy[e] += ldif * dyif / dif;
+            b.c_cx[i] -= ldif * dxif / dif;
+            b.c_cy[i] -= ldif * dyif / dif;
+        }
+        b.c_cx[i] += ldij * dxij / dij;
+        b.c_cy[i] += ldij * dyij / dij;
+        b.c_cx[j] -= ldij * dxij / dij;
+        b.c_cy[j] -= ldij * dyij / dij;
+    }
+    // ghost cells: area of the owner, centre x_f + |x_f - x_owner| n
+    for (size_t k = 0; k < m.bnd_edge.size(); ++k) {
+        const int32_t g = m.n_real + (int32_t)k, e = m.bnd_edge[k];
+        const int32_t o = m.edge_cells[2 * (int64_t)e];
+        b.c_area[o] += b.c_area[g];
+        const double lgx = b.c_cx[g], lgy = b.c_cy[g];
+        if (lgx == 0.0 && lgy == 0.0) continue;
+        const double dx = m.e_cx[e] - m.c_cx[o], dy = m.e_cy[e] - m.c_cy[o];
+        const double dist = std::sqrt(dx * dx + dy * dy);
+        b.e_cx[e] += lgx;
+        b.e_cy[e] += lgy;
+        b.e_nx[e] += dist * lgx;
+        b.e_ny[e] += dist * lgy;
+        const double ldist = lgx * m.e_nx[e] + lgy * m.e_ny[e];
+        if (dist > 0.0) {
+            b.e_cx[e] += ldist * dx / dist;
+            b.e_cy[e] += ldist * dy / dist;
+            b.c_cx[o] -= ldist * dx / dist;
+            b.c_cy[o] -= ldist * dy / dist;
+        }
+    }
+    // edges: centre (x_a + x_b) / 2; t = x_b - x_a; len = |t|; n = s (-t_y, t_x) / len
+    for (int32_t e = 0; e < E; ++e) {
+        const int32_t a = m.edge_nodes[2 * (int64_t)e], bb = m.edge_nodes[2 * (int64_t)e + 1];
+        const double tx = x[bb] - x[a], ty = y[bb] - y[a];
+        const double l = m.e_len[e];
+        const double nx = m.e_nx[e], ny = m.e_ny[e];
+        // n = s u / l with u = (-t_y, t_x): lam_u = (s / l)(lam_n - (lam_n . n) n)
+        const double s = (nx * (-ty) + ny * tx) < 0.0 ? -1.0 : 1.0;
+        const double ln = b.e_nx[e] * nx + b.e_ny[e] * ny;
+        const double lux = s / l * (b.e_nx[e] - ln * nx);
+        const double luy = s / l * (b.e_ny[e] - ln * ny);
+        const double ltx = luy + b.e_len[e] * tx / l;
+        const double lty = -lux + b.e_len[e] * ty / l;
+        gxy[2 * (int64_t)bb] += ltx + 0.5 * b.e_cx[e];
+        gxy[2 * (int64_t)bb + 1] += lty + 0.5 * b.e_cy[e];
+        gxy[2 * (int64_t)a] += -ltx + 0.5 * b.e_cx[e];
+        gxy[2 * (int64_t)a + 1] += -lty + 0.5 * b.e_cy[e];
+    }
+    // real cells: centre = mean of the vertices; area 0.5 |D| per triangle
+    auto tri = [&](int32_t p, int32_t q, int32_t r, double lA) {
+        const double D = x[p] * (y[q] - y[r]) + x[q] * (y[r] - y[p]) + x[r] * (y[p] - y[q]);
+        const double lD = 0.5 * (D < 0.0 ? -1.0 : 1.0) * lA;
+        gxy[2 * (int64_t)p] += lD * (y[q] - y[r]);
+        gxy[2 * (int64_t)q] += lD * (y[r] - y[p]);
+        gxy[2 * (int64_t)r] += lD * (y[p] - y[q]);
+        gxy[2 * (int64_t)p + 1] += lD * (x[r] - x[q]);
+        gxy[2 * (int64_t)q + 1] += lD * (x[p] - x[r]);
+        gxy[2 * (int64_t)r + 1] += lD * (x[q] - x[p]);
+    };
+    for (int32_t c = 0; c < m.n_real; ++c) {
+        const int32_t* cn = &m.cell_nodes[4 * (int64_t)c];
+        const int n = m.cell_tri[c] ? 3 : 4;
+        for (int j = 0; j < n; ++j) {
+            gxy[2 * (int64_t)cn[j]] += b.c_cx[c] / (double)n;
+            gxy[2 * (int64_t)cn[j] + 1] += b.c_cy[c] / (double)n;
+        }
+        if (b.c_area[c] != 0.0) {
+            tri(cn[0], cn[1], cn[2], b.c_area[c]);
+            if (n == 4) tri(cn[0], cn[2], cn[3], b.c_area[c]);
+        }
+    }
+}
+
 int32_t build_mesh(fvh_mesh& m, int32_t n_nodes, const double* x, const double* y,
                    int32_t n_cells, const int32_t* cells, int32_t n_bnd,
                    const int32_t* bnd, const int32_t* bnd_tag,

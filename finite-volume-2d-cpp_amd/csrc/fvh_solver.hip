@@ -112,6 +112,13 @@ struct fvh_solver {
     std::vector<std::unique_ptr<fvh::DevBuf>> a_ck;
     hipGraph_t a_graph = nullptr;            // one step's replay + reverse, captured once
     hipGraphExec_t a_graph_exec = nullptr;
+    bool a_graph_geo = false;                // the captured reverse forms the geometry adjoint
+    // geometry adjoint (node-coordinate gradients): per half-edge, half-edge
+    // centre, edge, cell; host maps to the reference numbering (one instance)
+    fvh::DevBuf a_g_he, a_g_hc, a_g_e, a_g_c, a_obj_p;
+    std::vector<uint32_t> h_he_edge;         // device half-edge -> device edge | flags
+    std::vector<int32_t> h_edge_ref;         // device edge -> reference edge of its instance
+    std::unique_ptr<fvh_mesh> mesh0;         // instance 0's mesh (single-mesh solvers)
     // graphs
     bool use_graphs = false;
     std::vector<fvh::StepGraph> graphs;
@@ -969,6 +976,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             const double dyij = m.c_cy[ci] - m.c_cy[cj];
             const double dij = std::sqrt(dxij * dxij + dyij * dyij);
             hg.e_gg[d] = dif / dij;
+            s->h_edge_ref.push_back(e);
         }
         uint32_t w = (uint32_t)dev_of_edge[ge];
         if (he_side[h]) w |= kSideBit;
@@ -1042,6 +1050,10 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
     if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
     if ((rc = upload(s->he_edge, he_edge.data(), he_edge.size() * 4))) return rc;
+    if (n == 1) {
+        s->h_he_edge = he_edge;
+        s->mesh0.reset(new fvh_mesh(*ms[0]));
+    }
     if ((rc = upload(s->he_twin, he_twin.data(), he_twin.size() * 4))) return rc;
     if ((rc = upload(s->he_bc, he_bc.data(), he_bc.size() * 4))) return rc;
     if ((rc = upload(s->he_slot, he_slot.data(), he_slot.size() * 2))) return rc;
@@ -1085,6 +1097,28 @@ struct Adjoint {
     int n_stage = 0;
     size_t qbytes = 0;
     int64_t fwd_steps = 0;
+    GeoAdj<double> ga;   // geometry adjoint (he == nullptr: not formed)
+
+    // zeroed geometry-adjoint accumulators
+    int32_t alloc_geo() {
+        const size_t nh = (size_t)std::max(s->n_he, 1);
+        if ((!s->a_g_he.p && s->a_g_he.alloc(nh * sizeof(Q4<double>))) ||
+            (!s->a_g_hc.p && s->a_g_hc.alloc((nh + 1) / 2 * sizeof(Q4<double>))) ||
+            (!s->a_g_e.p && s->a_g_e.alloc((size_t)std::max(s->n_edges, 1) * sizeof(Q4<double>))) ||
+            (!s->a_g_c.p && s->a_g_c.alloc((size_t)std::max(s->n_total, 1) * sizeof(Q4<double>))) ||
+            (!s->a_obj_p.p && s->a_obj_p.alloc((size_t)(s->n_bnd + 1) * 8)))
+            return oom();
+        ga.he = (Q4<double>*)s->a_g_he.p;
+        ga.hc = (G2<double>*)s->a_g_hc.p;
+        ga.e = (Q4<double>*)s->a_g_e.p;
+        ga.c = (Q4<double>*)s->a_g_c.p;
+        launch_fill<double>(s->stream, ga.he, s->n_he);
+        launch_fill<double>(s->stream, (Q4<double>*)ga.hc, (s->n_he + 1) / 2);   // G2 pairs
+        launch_fill<double>(s->stream, ga.e, s->n_edges);
+        launch_fill<double>(s->stream, ga.c, s->n_total);
+        FVH_HIP(hipGetLastError());
+        return FVH_OK;
+    }
 
     int32_t alloc(int snaps) {
         const size_t N = (size_t)s->n_total;
@@ -1196,7 +1230,7 @@ struct Adjoint {
                                      (Q4<double>*)s->a_lam_bincr.p, (Q4<double>*)s->a_lam_face.p,
                                      (Q4<double>*)s->a_lam_direct.p, (Q4<double>*)s->a_nb.p,
                                      k == 0 ? lam_q0 : lam_in, k == 0 ? 1 : 0, nullptr,
-                                     (double*)s->a_ld.p + (size_t)k * s->n_rt);
+                                     (double*)s->a_ld.p + (size_t)k * s->n_rt, ga);
         }
         if (gdt && c.dt_kind != FVH_DT_FIXED) {
             // the cell whose step is the minimum, first in reference order (min_dt, explicit.cpp:555-564)
@@ -1204,7 +1238,7 @@ struct Adjoint {
                                      (double*)s->a_amin_v.p, (int32_t*)s->a_amin_r.p, (int32_t*)s->a_amin_d.p,
                                      (int32_t*)s->a_amin.p);
             launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)s->a_qs[0].p, (const int32_t*)s->a_amin.p,
-                                  (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0);
+                                  (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0, ga);
         }
         FVH_HIP(hipGetLastError());
         FVH_HIP(hipMemcpyAsync(s->a_lam.p, s->a_lam_q0.p, qbytes, hipMemcpyDeviceToDevice, st));
@@ -1215,6 +1249,14 @@ struct Adjoint {
     // launch sequence never changes, so it is captured once as a HIP graph
     int32_t step_back(const void* src) {
         FVH_HIP(hipMemcpyAsync(s->a_qs[0].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        if (s->a_graph_exec && s->a_graph_geo != (ga.he != nullptr)) {
+            // captured with the other geometry choice
+            FVH_HIP(hipStreamSynchronize(s->stream));
+            (void)hipGraphExecDestroy(s->a_graph_exec);
+            (void)hipGraphDestroy(s->a_graph);
+            s->a_graph_exec = nullptr;
+            s->a_graph = nullptr;
+        }
         if (!s->a_graph_exec) {
             FVH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             int32_t rc = replay_body();
@@ -1227,6 +1269,7 @@ struct Adjoint {
                 return FVH_E_HIP;
             }
             s->a_graph = g;
+            s->a_graph_geo = ga.he != nullptr;
             FVH_HIP(hipGraphInstantiate(&s->a_graph_exec, g, nullptr, nullptr, 0));
         }
         FVH_HIP(hipGraphLaunch(s->a_graph_exec, s->stream));
@@ -1286,12 +1329,12 @@ int32_t objective_setup(fvh_solver* s, const double* wx, const double* wy) {
     return FVH_OK;
 }
 
-int32_t objective_eval(fvh_solver* s, const void* q, bool seed, double* J) {
+int32_t objective_eval(fvh_solver* s, const void* q, bool seed, double* J, double* p_out = nullptr) {
     const int32_t nb = s->i_nghost[0];
     if (seed) launch_fill<double>(s->stream, (Q4<double>*)s->a_lam.p, s->n_total);
     launch_objective<double>(s->stream, nb, phys(s->cfg), (const int32_t*)s->a_obj_ghost.p,
                              (const Q4<double>*)s->a_obj_gw.p, (const Q4<double>*)q, (double*)s->a_obj_terms.p,
-                             seed ? (Q4<double>*)s->a_lam.p : nullptr);
+                             seed ? (Q4<double>*)s->a_lam.p : nullptr, p_out);
     FVH_HIP(hipGetLastError());
     std::vector<double> t(nb);
     FVH_HIP(hipStreamSynchronize(s->stream));
@@ -1299,6 +1342,57 @@ int32_t objective_eval(fvh_solver* s, const void* q, bool seed, double* J) {
     double sum = 0.0;
     for (int32_t b = 0; b < nb; ++b) sum += t[b];
     *J = sum;
+    return FVH_OK;
+}
+
+// Geometry adjoint accumulated on the device (per half-edge, edge, cell) and
+// the objective's own dependence on the boundary normals and lengths, summed
+// per reference edge and cell, then chained to the node coordinates.
+int32_t node_gradient(fvh_solver* s, const double* wx, const double* wy, double* gxy) {
+    const fvh_mesh& m = *s->mesh0;
+    const int32_t nh = s->n_he, ne = s->n_edges, nc = s->n_total, nb = (int32_t)m.bnd_edge.size();
+    std::vector<Q4<double>> he(nh), e(ne), c(nc);
+    std::vector<G2<double>> hc(nh);
+    std::vector<double> p(nb);
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    FVH_HIP(hipMemcpy(he.data(), s->a_g_he.p, (size_t)nh * sizeof(Q4<double>), hipMemcpyDeviceToHost));
+    FVH_HIP(hipMemcpy(hc.data(), s->a_g_hc.p, (size_t)nh * sizeof(G2<double>), hipMemcpyDeviceToHost));
+    FVH_HIP(hipMemcpy(e.data(), s->a_g_e.p, (size_t)ne * sizeof(Q4<double>), hipMemcpyDeviceToHost));
+    FVH_HIP(hipMemcpy(c.data(), s->a_g_c.p, (size_t)nc * sizeof(Q4<double>), hipMemcpyDeviceToHost));
+    if (nb) FVH_HIP(hipMemcpy(p.data(), s->a_obj_p.p, (size_t)nb * 8, hipMemcpyDeviceToHost));
+    GeoBar b;
+    b.resize(m.n_edges(), m.n_cells);
+    for (int32_t h = 0; h < nh; ++h) {
+        const int32_t r = s->h_edge_ref[s->h_he_edge[h] & kEdgeMask];
+        b.e_nx[r] += he[h].v[0];
+        b.e_ny[r] += he[h].v[1];
+        b.e_len[r] += he[h].v[2];
+        b.e_w[r] += he[h].v[3];
+        b.e_cx[r] += hc[h].v[0];
+        b.e_cy[r] += hc[h].v[1];
+    }
+    for (int32_t d = 0; d < ne; ++d) {
+        const int32_t r = s->h_edge_ref[d];
+        b.e_nx[r] += e[d].v[0];
+        b.e_ny[r] += e[d].v[1];
+        b.e_len[r] += e[d].v[2];
+    }
+    for (int32_t d = 0; d < nc; ++d) {
+        const int32_t r = s->inv[d];
+        b.c_area[r] += c[d].v[0];
+        b.c_cx[r] += c[d].v[1];
+        b.c_cy[r] += c[d].v[2];
+    }
+    // J = sum_b p_b (wx_b n_x + wy_b n_y) len_b
+    for (int32_t k = 0; k < nb; ++k) {
+        const int32_t r = m.bnd_edge[k];
+        const double ax = wx ? wx[k] : 0.0, ay = wy ? wy[k] : 0.0;
+        b.e_nx[r] += p[k] * ax * m.e_len[r];
+        b.e_ny[r] += p[k] * ay * m.e_len[r];
+        b.e_len[r] += p[k] * (ax * m.e_nx[r] + ay * m.e_ny[r]);
+    }
+    std::fill(gxy, gxy + 2 * m.x.size(), 0.0);
+    mesh_geometry_vjp(m, b, gxy);
     return FVH_OK;
 }
 
@@ -1441,10 +1535,12 @@ int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const dou
 }
 
 int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
-                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, fvh_adjoint_stats* st) {
+                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, double* grad_nodes,
+                    fvh_adjoint_stats* st) {
     if (!s || !q || !J || !grad || n_steps < 0 || snaps < 0) { set_error("bad argument"); return FVH_E_ARG; }
     int32_t rc = adjoint_check(s);
     if (rc) return rc;
+    if (grad_nodes && !s->mesh0) { set_error("node gradients: the solver keeps no mesh"); return FVH_E_UNSUPPORTED; }
     const auto t0 = std::chrono::steady_clock::now();
     Adjoint A{s};
     A.qbytes = (size_t)s->n_total * sizeof(Q4<double>);
@@ -1456,6 +1552,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     }
     snaps = std::max(1, std::min(snaps, std::max(n_steps, 1)));
     if ((rc = A.alloc(snaps)) || (rc = objective_setup(s, wx, wy))) return rc;
+    if (grad_nodes && (rc = A.alloc_geo())) return rc;
     // initial state: bounds applied (explicit.cpp:712), kept in slot 0
     s->cur = 0;
     if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
@@ -1480,7 +1577,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     }
     FVH_HIP(hipStreamSynchronize(s->stream));
     const auto t1 = std::chrono::steady_clock::now();
-    if ((rc = objective_eval(s, s->a_tmp.p, true, J))) return rc;
+    if ((rc = objective_eval(s, s->a_tmp.p, true, J, grad_nodes ? (double*)s->a_obj_p.p : nullptr))) return rc;
     if (every) {
         for (int k = n_steps - 1; k >= 0; --k)
             if ((rc = A.step_back(s->a_ck[k]->p))) return rc;
@@ -1490,9 +1587,10 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
         if ((rc = A.rev(0, n_steps, 0, free_slots))) return rc;
     }
     launch_adj_bounds_init<double>(s->stream, view<double>(s), phys(s->cfg), (const BcParam<double>*)s->bcs.p,
-                                   (const Q4<double>*)s->a_ck[0]->p, (Q4<double>*)s->a_lam.p);
+                                   (const Q4<double>*)s->a_ck[0]->p, (Q4<double>*)s->a_lam.p, A.ga);
     FVH_HIP(hipGetLastError());
     if ((rc = get_q4<double>(s, s->a_lam.p, grad, s->n_total))) return rc;
+    if (grad_nodes && (rc = node_gradient(s, wx, wy, grad_nodes))) return rc;
     // ghost entries: run() overwrites them, so J does not depend on them
     for (int32_t g = s->n_real; g < s->n_total; ++g)
         for (int k = 0; k < 4; ++k) grad[4 * (int64_t)s->inv[g] + k] = 0.0;

@@ -127,7 +127,8 @@ def load(path: str = LIB_PATH):
     L.fvh_batch_status.argtypes = [vp, C.c_int32, C.POINTER(Status)]
     L.fvh_batch_sums.argtypes = [vp, dp]
     L.fvh_objective.argtypes = [vp, dp, C.c_int32, dp, dp, dp]
-    L.fvh_adjoint.argtypes = [vp, dp, C.c_int32, dp, dp, C.c_int32, C.c_double, dp, dp, C.POINTER(AdjointStats)]
+    L.fvh_adjoint.argtypes = [vp, dp, C.c_int32, dp, dp, C.c_int32, C.c_double, dp, dp, dp,
+                              C.POINTER(AdjointStats)]
     L.fvh_autotune.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), dp, C.c_int32,
                                C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, C.c_int32]
     for name in EXPORTS:

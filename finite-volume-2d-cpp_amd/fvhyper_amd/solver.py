@@ -255,17 +255,23 @@ class Solver:
                                         dptr(wy) if wy is not None else None, C.byref(J)))
         return float(J.value)
 
-    def adjoint(self, q0: np.ndarray, n_steps: int, wx=None, wy=None, snaps: int = 0, mem_cap_bytes: float = 8e9):
-        """(J, dJ/dq0, stats) by the discrete adjoint (fvh_adjoint)."""
+    def adjoint(self, q0: np.ndarray, n_steps: int, wx=None, wy=None, snaps: int = 0, mem_cap_bytes: float = 8e9,
+                nodes: bool = False):
+        """(J, dJ/dq0, stats) by the discrete adjoint (fvh_adjoint); with
+        nodes=True, (J, dJ/dq0, dJ/dxy, stats) where dJ/dxy has shape
+        (n_nodes, 2): the derivative with respect to each node's x and y."""
         q0 = np.ascontiguousarray(q0, dtype=np.float64).ravel()
         wx, wy = self._weights(wx, wy)
         J = C.c_double(0.0)
         grad = np.zeros_like(q0)
+        gxy = np.zeros((self.mesh.n_nodes, 2)) if nodes else None
         st = _lib.AdjointStats()
         check(_lib.load().fvh_adjoint(self._h, dptr(q0), int(n_steps), dptr(wx) if wx is not None else None,
                                       dptr(wy) if wy is not None else None, int(snaps), float(mem_cap_bytes),
-                                      C.byref(J), dptr(grad), C.byref(st)))
+                                      C.byref(J), dptr(grad), dptr(gxy) if nodes else None, C.byref(st)))
         stats = {k: getattr(st, k) for k, _ in _lib.AdjointStats._fields_}
+        if nodes:
+            return float(J.value), grad, gxy, stats
         return float(J.value), grad, stats
 
 

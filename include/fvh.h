@@ -239,7 +239,11 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
  * wx, wy one weight per boundary element (reference order; NULL = 0).
  * fvh_adjoint returns J and dJ/dq (4 values per cell; ghost entries are 0,
  * since run() overwrites the ghost cells before the first step), computed by
- * reverse-mode kernels over a replay of the forward steps.  Forward states
+ * reverse-mode kernels over a replay of the forward steps.  With grad_nodes
+ * non-NULL it also returns dJ/d(node coordinates), (d/dx, d/dy) per node in
+ * file order: the adjoint of every geometric quantity the run uses (normals,
+ * lengths, areas, centres, Green-Gauss weights, ghost centres), chained to
+ * the nodes; boundary nodes included (moving them moves the boundary).  Forward states
  * are kept at binomially placed step checkpoints: `snaps` slots (0 = as many
  * as mem_cap_bytes allows, at most one per step).  FP64, single-mesh
  * solvers, global or fixed time step. */
@@ -256,7 +260,8 @@ typedef struct fvh_adjoint_stats {
 int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
                       double* J);
 int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,
-                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, fvh_adjoint_stats* st);
+                    int32_t snaps, double mem_cap_bytes, double* J, double* grad, double* grad_nodes,
+                    fvh_adjoint_stats* st);
 
 /* ---- batches: many independent meshes advanced together (no reference
  * counterpart; a parameter sweep in the reference is one process per mesh).

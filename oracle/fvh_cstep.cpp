@@ -75,9 +75,9 @@ inline T pressure(double g, const T* q) {
 }
 
 template <class T>
-void flux(const orc_config* cf, T* f, const T* qi, const T* qj, const double* n) {
+void flux(const orc_config* cf, T* f, const T* qi, const T* qj, const T* n) {
     const T g = T(cf->gamma);
-    const T n0 = T(n[0]), n1 = T(n[1]);
+    const T n0 = n[0], n1 = n[1];
     T pi = pressure(cf->gamma, qi);
     T pj = pressure(cf->gamma, qj);
     T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
@@ -148,11 +148,11 @@ T phi(const orc_config* cf, T y) {
 }
 
 template <class T>
-void bc(const orc_config* cf, int slot, T* b, const T* q, const double* nd) {
+void bc(const orc_config* cf, int slot, T* b, const T* q, const T* nd) {
     const int kind = cf->bc_kind[slot];
     const double gd = cf->gamma;
     const T g = T(gd);
-    const T n0 = T(nd[0]), n1 = T(nd[1]);
+    const T n0 = nd[0], n1 = nd[1];
     if (kind == ORC_BC_COPY) {
         b[0] = q[0]; b[1] = q[1]; b[2] = q[2]; b[3] = q[3];
         return;
@@ -216,8 +216,75 @@ void bc(const orc_config* cf, int slot, T* b, const T* q, const double* nd) {
     }
 }
 
+// Mesh geometry over the number type: the node coordinates may carry a
+// complex perturbation.  Same operations and order as orc_mesh_build
+// (fvh_oracle.c, after mesh.cpp:190-260 and the ghost cells of
+// mesh.cpp:623-669), so on double it reproduces the oracle's arrays.
 template <class T>
-void calc_dt(const orc_mesh* m, const orc_config* cf, const T* q, T* dt) {
+struct Geo {
+    std::vector<T> ccx, ccy, area;            // all cells, ghosts included
+    std::vector<T> ecx, ecy, enx, eny, elen;  // edges
+};
+
+template <class T>
+void geometry(const orc_mesh* m, const T* X, const T* Y, Geo<T>& G) {
+    const int32_t NC = m->n_cells, NR = m->n_real, E = m->n_edges;
+    G.ccx.assign(NC, T(0.)); G.ccy.assign(NC, T(0.)); G.area.assign(NC, T(0.));
+    G.ecx.assign(E, T(0.)); G.ecy.assign(E, T(0.)); G.enx.assign(E, T(0.)); G.eny.assign(E, T(0.));
+    G.elen.assign(E, T(0.));
+    for (int32_t c = 0; c < NR; ++c) {
+        T cx = T(0.0), cy = T(0.0);
+        const int n = m->cell_tri[c] ? 3 : 4;
+        for (int j = 0; j < n; ++j) {
+            cx += X[m->cell_nodes[4 * c + j]] / T((double)n);
+            cy += Y[m->cell_nodes[4 * c + j]] / T((double)n);
+        }
+        G.ccx[c] = cx;
+        G.ccy[c] = cy;
+    }
+    for (int32_t e = 0; e < E; ++e) {
+        const int32_t n0 = m->edge_nodes[2 * e], n1 = m->edge_nodes[2 * e + 1];
+        G.ecx[e] = (X[n1] + X[n0]) * T(0.5);
+        G.ecy[e] = (Y[n1] + Y[n0]) * T(0.5);
+        const T dex = X[n1] - X[n0];
+        const T dey = Y[n1] - Y[n0];
+        const T l = sqrt(dex * dex + dey * dey);
+        G.elen[e] = l;
+        G.enx[e] = -dey / l;
+        G.eny[e] = dex / l;
+        const int32_t c0 = m->edge_cells[2 * e];
+        const T dot = G.enx[e] * (G.ecx[e] - G.ccx[c0]) + G.eny[e] * (G.ecy[e] - G.ccy[c0]);
+        if (dot < T(0.)) {
+            G.enx[e] *= T(-1.);
+            G.eny[e] *= T(-1.);
+        }
+    }
+    for (int32_t c = 0; c < NR; ++c) {
+        const int32_t* cn = &m->cell_nodes[4 * c];
+        const T x1 = X[cn[0]], x2 = X[cn[1]], x3 = X[cn[2]];
+        const T y1 = Y[cn[0]], y2 = Y[cn[1]], y3 = Y[cn[2]];
+        if (m->cell_tri[c]) {
+            G.area[c] = T(0.5) * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2));
+        } else {
+            const T x4 = X[cn[3]], y4 = Y[cn[3]];
+            G.area[c] = T(0.5) * fabs(x1 * (y2 - y3) + x2 * (y3 - y1) + x3 * (y1 - y2)) +
+                        T(0.5) * fabs(x1 * (y3 - y4) + x3 * (y4 - y1) + x4 * (y1 - y3));
+        }
+    }
+    for (int32_t b = 0; b < m->n_bnd; ++b) {
+        const int32_t e = m->bnd_edge[b];
+        const int32_t cell = m->edge_cells[2 * e], g = NR + b;
+        const T dx = G.ecx[e] - G.ccx[cell];
+        const T dy = G.ecy[e] - G.ccy[cell];
+        const T dist = sqrt(dx * dx + dy * dy);
+        G.area[g] = G.area[cell];
+        G.ccx[g] = G.ecx[e] + dist * G.enx[e];
+        G.ccy[g] = G.ecy[e] + dist * G.eny[e];
+    }
+}
+
+template <class T>
+void calc_dt(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T* q, T* dt) {
     const int32_t N = m->n_cells;
     const double gd = cf->gamma;
     const T g = T(gd);
@@ -229,7 +296,7 @@ void calc_dt(const orc_mesh* m, const orc_config* cf, const T* q, T* dt) {
         for (int32_t i = 0; i < N; ++i) dt[i] = T(0.);
         for (int32_t e = 0; e < m->n_edges; ++e) {
             int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
-            T le = T(m->edge_len[e]), n0 = T(m->edge_nx[e]), n1 = T(m->edge_ny[e]);
+            T le = G.elen[e], n0 = G.enx[e], n1 = G.eny[e];
             const T* qi = &q[V * i];
             const T* qj = &q[V * j];
             T ci = sqrt(pressure(gd, qi) * g / qi[0]);
@@ -240,21 +307,21 @@ void calc_dt(const orc_mesh* m, const orc_config* cf, const T* q, T* dt) {
             dt[i] += center * le;
             dt[j] += center * le;
         }
-        for (int32_t i = 0; i < N; ++i) dt[i] = T(cf->cfl) * T(m->cell_area[i]) / dt[i];
+        for (int32_t i = 0; i < N; ++i) dt[i] = T(cf->cfl) * G.area[i] / dt[i];
         return;
     }
     for (int32_t i = 0; i < N; ++i) dt[i] = T(1.0);
     for (int32_t e = 0; e < m->n_edges; ++e) {
         int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
-        T le = T(m->edge_len[e]), n0 = T(m->edge_nx[e]), n1 = T(m->edge_ny[e]);
+        T le = G.elen[e], n0 = G.enx[e], n1 = G.eny[e];
         const T* qi = &q[V * i];
         const T* qj = &q[V * j];
         T ci = sqrt(pressure(gd, qi) * g / qi[0]);
         T cj = sqrt(pressure(gd, qj) * g / qj[0]);
         T ei = (ci + fabs(qi[1] / qi[0])) * n0 + (ci + fabs(qi[2] / qi[0])) * n1;
         T ej = (cj + fabs(qj[1] / qj[0])) * n0 + (cj + fabs(qj[2] / qj[0])) * n1;
-        T dti = T(cf->cfl) * (T(m->cell_area[i]) / le) / fabs(ei);
-        T dtj = T(cf->cfl) * (T(m->cell_area[j]) / le) / fabs(ej);
+        T dti = T(cf->cfl) * (G.area[i] / le) / fabs(ei);
+        T dtj = T(cf->cfl) * (G.area[j] / le) / fabs(ej);
         dt[i] = smin(dt[i], dti);
         dt[j] = smin(dt[j], dtj);
     }
@@ -268,19 +335,19 @@ void min_dt(const orc_mesh* m, T* dt) {
 }
 
 template <class T>
-void gradients(const orc_mesh* m, const T* q, T* gx, T* gy) {
+void gradients(const orc_mesh* m, const Geo<T>& G, const T* q, T* gx, T* gy) {
     const int64_t n = (int64_t)V * m->n_cells;
     for (int64_t i = 0; i < n; ++i) { gx[i] = T(0.); gy[i] = T(0.); }
     for (int32_t e = 0; e < m->n_edges; ++e) {
         int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
-        T nx = T(m->edge_nx[e]), ny = T(m->edge_ny[e]), le = T(m->edge_len[e]);
-        double dxif = m->edge_cx[e] - m->cell_cx[i];
-        double dyif = m->edge_cy[e] - m->cell_cy[i];
-        double dif = ::sqrt(dxif * dxif + dyif * dyif);
-        double dxij = m->cell_cx[i] - m->cell_cx[j];
-        double dyij = m->cell_cy[i] - m->cell_cy[j];
-        double dij = ::sqrt(dxij * dxij + dyij * dyij);
-        T gf = T(dif / dij);
+        T nx = G.enx[e], ny = G.eny[e], le = G.elen[e];
+        T dxif = G.ecx[e] - G.ccx[i];
+        T dyif = G.ecy[e] - G.ccy[i];
+        T dif = sqrt(dxif * dxif + dyif * dyif);
+        T dxij = G.ccx[i] - G.ccx[j];
+        T dyij = G.ccy[i] - G.ccy[j];
+        T dij = sqrt(dxij * dxij + dyij * dyij);
+        T gf = dif / dij;
         if (i != j) {
             T f[V];
             for (int k = 0; k < V; ++k) f[k] = (q[V * i + k] * (T(1.0) - gf) + q[V * j + k] * gf) * le;
@@ -293,7 +360,7 @@ void gradients(const orc_mesh* m, const T* q, T* gx, T* gy) {
         }
     }
     for (int32_t i = 0; i < m->n_real; ++i) {
-        const T invA = T(1. / m->cell_area[i]);
+        const T invA = T(1.) / G.area[i];
         for (int k = 0; k < V; ++k) { gx[V * i + k] *= invA; gy[V * i + k] *= invA; }
     }
     for (int32_t i = m->n_real; i < m->n_cells; ++i)
@@ -301,8 +368,8 @@ void gradients(const orc_mesh* m, const T* q, T* gx, T* gy) {
 }
 
 template <class T>
-void limiters(const orc_mesh* m, const orc_config* cf, const T* q, const T* gx, const T* gy, T* lim, T* qmin,
-              T* qmax) {
+void limiters(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T* q, const T* gx, const T* gy,
+              T* lim, T* qmin, T* qmax) {
     const int64_t n = (int64_t)V * m->n_cells;
     for (int64_t i = 0; i < n; ++i) { lim[i] = T(1.); qmin[i] = q[i]; qmax[i] = q[i]; }
     for (int32_t e = 0; e < m->n_edges; ++e) {
@@ -320,15 +387,15 @@ void limiters(const orc_mesh* m, const orc_config* cf, const T* q, const T* gx, 
         for (int s = 0; s < 2; ++s) {
             int32_t id = ids[s];
             if (id >= m->n_real) continue;
-            T dx = T(m->edge_cx[e] - m->cell_cx[id]);
-            T dy = T(m->edge_cy[e] - m->cell_cy[id]);
-            double sqrt_area = ::sqrt(m->cell_area[id]);
+            T dx = G.ecx[e] - G.ccx[id];
+            T dy = G.ecy[e] - G.ccy[id];
+            T sqrt_area = sqrt(G.area[id]);
             for (int k = 0; k < V; ++k) {
                 T dqg = gx[V * id + k] * dx + gy[V * id + k] * dy;
                 T dmax = qmax[V * id + k] - q[V * id + k];
                 T dmin = qmin[V * id + k] - q[V * id + k];
-                const double Ka = cf->K * sqrt_area;
-                const T K3a = T(Ka * Ka * Ka);
+                const T Ka = T(cf->K) * sqrt_area;
+                const T K3a = Ka * Ka * Ka;
                 const T d2 = (dmax - dmin) * (dmax - dmin);
                 T sig;
                 if (d2 <= K3a) {
@@ -358,17 +425,17 @@ bool physical(double g, const T* q) {
 }
 
 template <class T>
-void time_derivatives(const orc_mesh* m, const orc_config* cf, const T* q, const T* gx, const T* gy, const T* lim,
-                      T* qt) {
+void time_derivatives(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T* q, const T* gx,
+                      const T* gy, const T* lim, T* qt) {
     const int64_t n = (int64_t)V * m->n_cells;
     for (int64_t i = 0; i < n; ++i) qt[i] = T(0.);
     for (int32_t e = 0; e < m->n_edges; ++e) {
         int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
-        T le = T(m->edge_len[e]);
-        double nn[2] = {m->edge_nx[e], m->edge_ny[e]};
-        double cx = m->edge_cx[e], cy = m->edge_cy[e];
-        T di0 = T(cx - m->cell_cx[i]), di1 = T(cy - m->cell_cy[i]);
-        T dj0 = T(cx - m->cell_cx[j]), dj1 = T(cy - m->cell_cy[j]);
+        T le = G.elen[e];
+        T nn[2] = {G.enx[e], G.eny[e]};
+        T cx = G.ecx[e], cy = G.ecy[e];
+        T di0 = cx - G.ccx[i], di1 = cy - G.ccy[i];
+        T dj0 = cx - G.ccx[j], dj1 = cy - G.ccy[j];
         T qi[V], qj[V], f[V];
         if (cf->lin) {
             for (int k = 0; k < V; ++k) {
@@ -387,8 +454,8 @@ void time_derivatives(const orc_mesh* m, const orc_config* cf, const T* q, const
         }
         flux(cf, f, qi, qj, nn);
         for (int k = 0; k < V; ++k) {
-            qt[V * i + k] -= f[k] * le / T(m->cell_area[i]);
-            qt[V * j + k] += f[k] * le / T(m->cell_area[j]);
+            qt[V * i + k] -= f[k] * le / G.area[i];
+            qt[V * j + k] += f[k] * le / G.area[j];
         }
     }
     for (int32_t i = m->n_real; i < m->n_cells; ++i)
@@ -396,15 +463,16 @@ void time_derivatives(const orc_mesh* m, const orc_config* cf, const T* q, const
 }
 
 template <class T>
-void update_bounds(const orc_mesh* m, const orc_config* cf, T* q, const T* gx, const T* gy, const T* lim) {
+void update_bounds(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* q, const T* gx, const T* gy,
+                   const T* lim) {
     for (int32_t b = 0; b < m->n_bnd; ++b) {
         int32_t e = m->bnd_edge[b];
-        double nn[2] = {m->edge_nx[e], m->edge_ny[e]};
+        T nn[2] = {G.enx[e], G.eny[e]};
         int32_t ii = m->edge_cells[2 * e], ib = m->edge_cells[2 * e + 1];
         T qint[V];
         if (cf->lin) {
-            T d0 = T(m->edge_cx[e] - m->cell_cx[ii]);
-            T d1 = T(m->edge_cy[e] - m->cell_cy[ii]);
+            T d0 = G.ecx[e] - G.ccx[ii];
+            T d1 = G.ecy[e] - G.ccy[ii];
             for (int k = 0; k < V; ++k) {
                 int64_t kk = V * (int64_t)ii + k;
                 qint[k] = q[kk] + (gx[kk] * d0 + gy[kk] * d1) * lim[kk];
@@ -420,22 +488,22 @@ void update_bounds(const orc_mesh* m, const orc_config* cf, T* q, const T* gx, c
 
 // run() for a fixed number of steps (explicit.cpp:657-817 without output)
 template <class T>
-void run(const orc_mesh* m, const orc_config* cf, T* q, int steps) {
+void run(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* q, int steps) {
     const int64_t N = m->n_cells, n = V * N;
     std::vector<T> qk(n), qt(n), gx(n), gy(n), lim(n), qmin(n), qmax(n), dt(N);
-    update_bounds(m, cf, q, gx.data(), gy.data(), lim.data());
+    update_bounds(m, G, cf, q, gx.data(), gy.data(), lim.data());
     for (int step = 0; step < steps; ++step) {
-        calc_dt(m, cf, q, dt.data());
+        calc_dt(m, G, cf, q, dt.data());
         if (cf->gdt) min_dt(m, dt.data());
         for (int64_t i = 0; i < n; ++i) qk[i] = q[i];
         for (int s = 0; s < cf->n_alpha; ++s) {
-            if (cf->grad) gradients(m, qk.data(), gx.data(), gy.data());
-            if (cf->lim) limiters(m, cf, qk.data(), gx.data(), gy.data(), lim.data(), qmin.data(), qmax.data());
-            time_derivatives(m, cf, qk.data(), gx.data(), gy.data(), lim.data(), qt.data());
+            if (cf->grad) gradients(m, G, qk.data(), gx.data(), gy.data());
+            if (cf->lim) limiters(m, G, cf, qk.data(), gx.data(), gy.data(), lim.data(), qmin.data(), qmax.data());
+            time_derivatives(m, G, cf, qk.data(), gx.data(), gy.data(), lim.data(), qt.data());
             const T a = T(cf->alpha[s]);
             for (int64_t c = 0; c < N; ++c)
                 for (int k = 0; k < V; ++k) qk[V * c + k] = q[V * c + k] + qt[V * c + k] * dt[c] * a;
-            update_bounds(m, cf, qk.data(), gx.data(), gy.data(), lim.data());
+            update_bounds(m, G, cf, qk.data(), gx.data(), gy.data(), lim.data());
         }
         for (int64_t i = 0; i < n; ++i) q[i] = qk[i];
     }
@@ -443,14 +511,14 @@ void run(const orc_mesh* m, const orc_config* cf, T* q, int steps) {
 
 // J = sum_b p(ghost_b) (wx_b n_x + wy_b n_y) len_b (the GPU objective's form)
 template <class T>
-T objective(const orc_mesh* m, const orc_config* cf, const T* q, const double* wx, const double* wy) {
+T objective(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T* q, const double* wx,
+            const double* wy) {
     T J = T(0.);
     for (int32_t b = 0; b < m->n_bnd; ++b) {
         const int32_t e = m->bnd_edge[b];
         const int32_t ghost = m->n_real + b;
-        const double proj = (wx ? wx[b] : 0.0) * m->edge_len[e] * m->edge_nx[e] +
-                            (wy ? wy[b] : 0.0) * m->edge_len[e] * m->edge_ny[e];
-        J += pressure(cf->gamma, &q[V * (int64_t)ghost]) * T(proj);
+        const T proj = T(wx ? wx[b] : 0.0) * G.elen[e] * G.enx[e] + T(wy ? wy[b] : 0.0) * G.elen[e] * G.eny[e];
+        J += pressure(cf->gamma, &q[V * (int64_t)ghost]) * proj;
     }
     return J;
 }
@@ -459,28 +527,86 @@ T objective(const orc_mesh* m, const orc_config* cf, const T* q, const double* w
 
 extern "C" {
 
+// double instantiation of the geometry: 5 arrays over all cells (cx, cy,
+// area) then 5 over edges (cx, cy, nx, ny, len), packed in that order
+int cst_geometry_f64(const orc_mesh* m, double* out) {
+    Geo<double> G;
+    geometry<double>(m, m->x, m->y, G);
+    const int64_t NC = m->n_cells, E = m->n_edges;
+    const std::vector<double>* cells[3] = {&G.ccx, &G.ccy, &G.area};
+    const std::vector<double>* edges[5] = {&G.ecx, &G.ecy, &G.enx, &G.eny, &G.elen};
+    for (int k = 0; k < 3; ++k) memcpy(out + k * NC, cells[k]->data(), sizeof(double) * NC);
+    for (int k = 0; k < 5; ++k) memcpy(out + 3 * NC + k * E, edges[k]->data(), sizeof(double) * E);
+    return 0;
+}
+
+// complex-step derivative of the geometry along node displacements (dx, dy):
+// per edge (n_x, n_y, len, Green-Gauss weight as calc_gradients forms it,
+// centre x, centre y), then per cell, ghosts included, (area, centre x, y)
+int cst_geometry_jvp(const orc_mesh* m, const double* dx, const double* dy, double h, double* out) {
+    std::vector<cs> X(m->n_nodes), Y(m->n_nodes);
+    for (int32_t k = 0; k < m->n_nodes; ++k) {
+        X[k] = cs(m->x[k], h * dx[k]);
+        Y[k] = cs(m->y[k], h * dy[k]);
+    }
+    Geo<cs> G;
+    geometry<cs>(m, X.data(), Y.data(), G);
+    const int64_t E = m->n_edges;
+    for (int32_t e = 0; e < m->n_edges; ++e) {
+        const int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+        const cs dxif = G.ecx[e] - G.ccx[i], dyif = G.ecy[e] - G.ccy[i];
+        const cs dxij = G.ccx[i] - G.ccx[j], dyij = G.ccy[i] - G.ccy[j];
+        const cs w = sqrt(dxif * dxif + dyif * dyif) / sqrt(dxij * dxij + dyij * dyij);
+        const cs v[6] = {G.enx[e], G.eny[e], G.elen[e], w, G.ecx[e], G.ecy[e]};
+        for (int k = 0; k < 6; ++k) out[6 * e + k] = v[k].im / h;
+    }
+    for (int32_t c = 0; c < m->n_cells; ++c) {
+        out[6 * E + 3 * c] = G.area[c].im / h;
+        out[6 * E + 3 * c + 1] = G.ccx[c].im / h;
+        out[6 * E + 3 * c + 2] = G.ccy[c].im / h;
+    }
+    return 0;
+}
+
 // double instantiation: final state after `steps` steps (all cells) and J
 int cst_run_f64(const orc_mesh* m, const orc_config* cf, const double* q0, int steps, const double* wx,
                 const double* wy, double* q_out, double* J) {
     const int64_t n = (int64_t)V * m->n_cells;
+    Geo<double> G;
+    geometry<double>(m, m->x, m->y, G);
     std::vector<double> q(q0, q0 + n);
-    run<double>(m, cf, q.data(), steps);
+    run<double>(m, G, cf, q.data(), steps);
     if (q_out) memcpy(q_out, q.data(), sizeof(double) * n);
-    if (J) *J = objective<double>(m, cf, q.data(), wx, wy);
+    if (J) *J = objective<double>(m, G, cf, q.data(), wx, wy);
     return 0;
 }
 
-// complex step along dir: J(q0 + i h dir) -> J (real part) and dJ = Im / h
-int cst_derivative(const orc_mesh* m, const orc_config* cf, const double* q0, const double* dir, double h,
-                   int steps, const double* wx, const double* wy, double* J, double* dJ) {
+// complex step along (dir in the initial state, dx/dy in the node
+// coordinates; either may be NULL): J(q0 + i h dir; x + i h dx, y + i h dy)
+// -> J (real part) and dJ = Im / h
+int cst_derivative2(const orc_mesh* m, const orc_config* cf, const double* q0, const double* dir,
+                    const double* dx, const double* dy, double h, int steps, const double* wx, const double* wy,
+                    double* J, double* dJ) {
     const int64_t n = (int64_t)V * m->n_cells;
+    std::vector<cs> X(m->n_nodes), Y(m->n_nodes);
+    for (int32_t k = 0; k < m->n_nodes; ++k) {
+        X[k] = cs(m->x[k], dx ? h * dx[k] : 0.0);
+        Y[k] = cs(m->y[k], dy ? h * dy[k] : 0.0);
+    }
+    Geo<cs> G;
+    geometry<cs>(m, X.data(), Y.data(), G);
     std::vector<cs> q(n);
-    for (int64_t i = 0; i < n; ++i) q[i] = cs(q0[i], h * dir[i]);
-    run<cs>(m, cf, q.data(), steps);
-    const cs j = objective<cs>(m, cf, q.data(), wx, wy);
+    for (int64_t i = 0; i < n; ++i) q[i] = cs(q0[i], dir ? h * dir[i] : 0.0);
+    run<cs>(m, G, cf, q.data(), steps);
+    const cs j = objective<cs>(m, G, cf, q.data(), wx, wy);
     *J = j.re;
     *dJ = j.im / h;
     return 0;
+}
+
+int cst_derivative(const orc_mesh* m, const orc_config* cf, const double* q0, const double* dir, double h,
+                   int steps, const double* wx, const double* wy, double* J, double* dJ) {
+    return cst_derivative2(m, cf, q0, dir, nullptr, nullptr, h, steps, wx, wy, J, dJ);
 }
 
 }  // extern "C"

@@ -298,6 +298,11 @@ def _cstep():
                                   dp, C.POINTER(C.c_double)]
         L.cst_derivative.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcConfig), dp, dp, C.c_double, C.c_int,
                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cst_derivative2.argtypes = [C.POINTER(OrcMesh), C.POINTER(OrcConfig), dp, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.cst_geometry_f64.argtypes = [C.POINTER(OrcMesh), dp]
+        L.cst_geometry_jvp.argtypes = [C.POINTER(OrcMesh), dp, dp, C.c_double, dp]
         _cst = L
     return _cst
 
@@ -333,6 +338,49 @@ def cstep_derivative(mesh: "Mesh", cfg: Dict, q0: np.ndarray, direction: np.ndar
     ay, py = _wptr(wy)
     _cstep().cst_derivative(mesh._p, C.byref(c), q0, d, float(h), int(steps), px, py, C.byref(J), C.byref(dJ))
     return float(J.value), float(dJ.value)
+
+
+def cstep_derivative_nodes(mesh: "Mesh", cfg: Dict, q0: np.ndarray, steps: int, dx=None, dy=None,
+                           direction=None, wx=None, wy=None, h: float = 1e-30):
+    """Complex-step derivative of J along a perturbation of the node
+    coordinates (dx, dy: one value per node, reference node order) and,
+    optionally, of the initial state: (J, dJ)."""
+    c = make_config(cfg, mesh.names)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    ad, pd = _wptr(direction)
+    ax_, pdx = _wptr(dx)
+    ay_, pdy = _wptr(dy)
+    J, dJ = C.c_double(0.0), C.c_double(0.0)
+    ax, px = _wptr(wx)
+    ay, py = _wptr(wy)
+    _cstep().cst_derivative2(mesh._p, C.byref(c), q0, pd, pdx, pdy, float(h), int(steps), px, py, C.byref(J),
+                             C.byref(dJ))
+    return float(J.value), float(dJ.value)
+
+
+def cstep_geometry(mesh: "Mesh") -> Dict[str, np.ndarray]:
+    """Geometry of the templated restatement in double (cells: cx, cy, area;
+    edges: cx, cy, nx, ny, len)."""
+    p = mesh._p.contents
+    NC, E = p.n_cells, p.n_edges
+    out = np.zeros(3 * NC + 5 * E)
+    _cstep().cst_geometry_f64(mesh._p, out)
+    keys = ["cell_cx", "cell_cy", "cell_area"]
+    res = {k: out[i * NC:(i + 1) * NC] for i, k in enumerate(keys)}
+    for i, k in enumerate(["edge_cx", "edge_cy", "edge_nx", "edge_ny", "edge_len"]):
+        res[k] = out[3 * NC + i * E:3 * NC + (i + 1) * E]
+    return res
+
+
+def cstep_geometry_jvp(mesh: "Mesh", dx: np.ndarray, dy: np.ndarray, h: float = 1e-30) -> np.ndarray:
+    """Complex-step derivative of the geometry along node displacements:
+    per edge (n_x, n_y, len, Green-Gauss weight, centre x, y), then per
+    cell, ghosts included, (area, centre x, y), packed."""
+    p = mesh._p.contents
+    out = np.zeros(6 * p.n_edges + 3 * p.n_cells)
+    _cstep().cst_geometry_jvp(mesh._p, np.ascontiguousarray(dx, dtype=np.float64),
+                              np.ascontiguousarray(dy, dtype=np.float64), float(h), out)
+    return out
 
 
 def ref_available() -> bool:

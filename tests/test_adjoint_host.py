@@ -156,3 +156,81 @@ def test_time_step_wave_speed_reverse(kind):
             return (c + abs(q[1] / q[0])) * n[0] + (c + abs(q[2] / q[0])) * n[1]
         h = 1e-7
         assert abs((e(qi + h * dq) - e(qi - h * dq)) / (2 * h) - jv) < 1e-6 * (abs(jv) + 1)
+
+
+@pytest.mark.parametrize("kind,par", [(1, [0, 0, 0, 0]), (2, [1.4, 3.0, 0.0, 1.0]), (2, [1.4, 0.5, 0.1, 1.0]),
+                                      (3, [1.4, 0.8, 0.0175, 1.0])])
+def test_boundary_normal_reverse_matches_central_differences(kind, par):
+    """Adjoint of the edge normal through a boundary function (used by the
+    node-coordinate gradient): nb . dn == bb . (b(n + h dn) - b(n - h dn)) / 2h."""
+    L = _lib()
+    d = C.POINTER(C.c_double)
+    L.fvhc_bc_vjp_n.argtypes = [C.c_int, d, C.c_double, d, d, d, d]
+    rng = np.random.default_rng(100 + kind)
+    for mach in (0.4, 1.6):
+        for qi, _, n, _ in _cases(40, 2234 + kind, mach):
+            dn, bb = rng.standard_normal(2), rng.standard_normal(4)
+            nb = np.zeros(2)
+            L.fvhc_bc_vjp_n(kind, P(par), 1.4, P(qi), P(n), P(bb), P(nb))
+
+            def b(nn):
+                out = np.zeros(4)
+                L.fvhc_bc(kind, P(par), 1.4, P(qi), P(nn), P(out))
+                return out
+            h = 1e-7
+            fd = bb @ (b(n + h * dn) - b(n - h * dn)) / (2 * h)
+            noise = 1e-8 * (np.abs(bb) @ np.abs(b(n)))   # rounding of b over 2h
+            assert abs(nb @ dn - fd) <= 1e-6 * (np.abs(nb) @ np.abs(dn)) + noise, (nb @ dn, fd)
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_time_step_wave_speed_normal_reverse(kind):
+    L = _lib()
+    d = C.POINTER(C.c_double)
+    L.fvhc_eig_vjp_n.argtypes = [C.c_int, C.c_double, d, d, C.c_double, d]
+    rng = np.random.default_rng(200 + kind)
+    for qi, _, n, _ in _cases(60, 5321 + kind, 0.9):
+        dn, eb = rng.standard_normal(2), float(rng.standard_normal())
+        nb = np.zeros(2)
+        L.fvhc_eig_vjp_n(kind, 1.4, P(qi), P(n), eb, P(nb))
+
+        def e(nn):
+            p = 0.4 * (qi[3] - 0.5 / qi[0] * (qi[1] ** 2 + qi[2] ** 2))
+            c = np.sqrt(p * 1.4 / qi[0])
+            if kind == 1:
+                return c + abs(qi[1] / qi[0] * nn[0] + qi[2] / qi[0] * nn[1])
+            return (c + abs(qi[1] / qi[0])) * nn[0] + (c + abs(qi[2] / qi[0])) * nn[1]
+        h = 1e-7
+        fd = eb * (e(n + h * dn) - e(n - h * dn)) / (2 * h)
+        assert abs(nb @ dn - fd) < 1e-6 * (abs(fd) + 1)
+
+
+@pytest.mark.parametrize("mesh_name", ["mixed", "naca", "step"])
+def test_geometry_reverse_is_the_transpose_of_the_geometry_derivative(mesh_name):
+    """The host reverse of the mesh geometry (normals, lengths, Green-Gauss
+    weights, centres, areas, ghost cells -> node coordinates) against the
+    complex-step derivative of the restatement's geometry: for random
+    adjoints b and node displacements d, b . (dG/dx d) == (dG/dx^T b) . d."""
+    from helpers import MESHES
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    L = _lib()
+    d = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    L.fvhc_geometry_vjp.argtypes = [C.c_int, d, d, C.c_int, ip, C.c_int, ip, ip, d, d]
+    cells = np.ascontiguousarray(m.cells, dtype=np.int32)
+    bnd = np.ascontiguousarray(m.bnd, dtype=np.int32)
+    tags = np.ascontiguousarray(m.bnd_tag, dtype=np.int32)
+    rng = np.random.default_rng(7)
+    n = 6 * om.n_edges + 3 * om.n_cells
+    for _ in range(3):
+        bars = rng.standard_normal(n)
+        gxy = np.zeros(2 * m.x.size)
+        rc = L.fvhc_geometry_vjp(m.x.size, P(m.x), P(m.y), cells.shape[0], cells.ctypes.data_as(ip), bnd.shape[0],
+                                 bnd.ctypes.data_as(ip), tags.ctypes.data_as(ip), P(bars), P(gxy))
+        assert rc == 0
+        dx, dy = rng.standard_normal(m.x.size), rng.standard_normal(m.x.size)
+        jvp = O.cstep_geometry_jvp(om, dx, dy)
+        lhs = bars @ jvp
+        rhs = gxy[0::2] @ dx + gxy[1::2] @ dy
+        assert abs(lhs - rhs) <= 1e-10 * (np.abs(bars) @ np.abs(jvp)), (lhs, rhs)

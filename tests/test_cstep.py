@@ -41,3 +41,81 @@ def test_complex_step_matches_central_differences():
         h = 1e-6
         fd = (O.cstep_run(om, smooth, q0 + h * v, 3, wx=w)[1] - O.cstep_run(om, smooth, q0 - h * v, 3, wx=w)[1]) / (2 * h)
         assert abs(dJ - fd) <= 1e-6 * abs(dJ), (dJ, fd)
+
+
+@pytest.mark.parametrize("mesh_name", sorted(MESHES))
+def test_template_geometry_is_the_oracle(mesh_name):
+    """The restatement's geometry over the number type, in double, is the
+    oracle's mesh geometry bit for bit (centres, areas, ghost cells, edge
+    normals, lengths and centres)."""
+    om = O.Mesh(MESHES[mesh_name]())
+    g = O.cstep_geometry(om)
+    for k, v in g.items():
+        assert bits_equal(v, getattr(om, k)), k
+
+
+def interior_nodes(m):
+    on_bnd = np.zeros(m.x.size, dtype=bool)
+    on_bnd[np.asarray(m.bnd).ravel()] = True
+    return ~on_bnd
+
+
+def moved(m, eps, dx, dy):
+    import dataclasses
+    return dataclasses.replace(m, x=m.x + eps * dx, y=m.y + eps * dy)
+
+
+@pytest.mark.parametrize("second_order", [False, True])
+def test_node_complex_step_matches_central_differences(second_order):
+    """d J / d (node coordinates) along random interior directions: complex
+    step through the geometry against central differences of J over meshes
+    rebuilt with moved nodes (no limiter, so the map is smooth)."""
+    m, cfg, ic = workloads.forward_step(n_per_unit=10)
+    om = O.Mesh(m)
+    free = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+    smooth = dict(flux="roe_joined", dt="sum_cfl", cfl=0.8, gdt=True, grad=second_order, lin=second_order,
+                  bcs={"wall": ("mirror",), "inlet": free, "outlet": free})
+    from fvhyper_amd import Mesh
+    pm = Mesh.from_arrays(m)
+    q0 = workloads.bump_state(pm)
+    w = workloads.step_face_weights(pm)
+    inner = interior_nodes(m)
+    h = 0.01 / 10   # node jitter scale: a tenth of the spacing
+    rng = np.random.default_rng(18)
+    for _ in range(3):
+        dx = np.where(inner, rng.standard_normal(m.x.size), 0.0) * h
+        dy = np.where(inner, rng.standard_normal(m.x.size), 0.0) * h
+        J, dJ = O.cstep_derivative_nodes(om, smooth, q0, 3, dx=dx, dy=dy, wx=w)
+        assert J == O.cstep_run(om, smooth, q0, 3, wx=w)[1]
+        eps = 1e-5
+        Jp = O.cstep_run(O.Mesh(moved(m, eps, dx, dy)), smooth, q0, 3, wx=w)[1]
+        Jm = O.cstep_run(O.Mesh(moved(m, -eps, dx, dy)), smooth, q0, 3, wx=w)[1]
+        fd = (Jp - Jm) / (2 * eps)
+        assert abs(dJ - fd) <= 1e-6 * abs(dJ), (dJ, fd)
+        assert abs(dJ) > 0.0
+
+
+def test_node_derivative_of_geometry_closed_form():
+    """Complex step through the geometry against closed forms: a translation
+    of every node leaves J unchanged (the scheme sees only differences of
+    coordinates), and with zero steps J = sum p (w.n) len is homogeneous of
+    degree one in the coordinates, so its derivative along the coordinates
+    themselves (a dilation) is J."""
+    m, cfg, ic = workloads.forward_step(n_per_unit=10)
+    om = O.Mesh(m)
+    free = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+    smooth = dict(flux="roe_joined", dt="sum_cfl", cfl=0.8, gdt=True, grad=True, lin=True,
+                  bcs={"wall": ("mirror",), "inlet": free, "outlet": free})
+    from fvhyper_amd import Mesh
+    pm = Mesh.from_arrays(m)
+    q0 = workloads.bump_state(pm)
+    w = workloads.step_face_weights(pm)
+    ones = np.ones(m.x.size)
+    # translation: the discrete scheme depends on differences of coordinates only
+    _, d_tx = O.cstep_derivative_nodes(om, smooth, q0, 3, dx=ones, wx=w)
+    _, d_ty = O.cstep_derivative_nodes(om, smooth, q0, 3, dy=ones, wx=w)
+    J = O.cstep_run(om, smooth, q0, 0, wx=w)[1]
+    assert abs(d_tx) <= 1e-12 * abs(J) and abs(d_ty) <= 1e-12 * abs(J)
+    # zero steps: J = sum p (n.w) len is homogeneous of degree 1 in the coordinates
+    J0, d_dil = O.cstep_derivative_nodes(om, smooth, q0, 0, dx=m.x, dy=m.y, wx=w)
+    assert abs(d_dil - J0) <= 1e-13 * abs(J0)

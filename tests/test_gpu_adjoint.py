@@ -119,3 +119,90 @@ def test_adjoint_matches_complex_step_of_the_restatement():
         assert abs(Jc - J) <= 1e-12 * abs(J)
         print("  complex step %.15e adjoint %.15e" % (dJ, grad @ v), flush=True)
         assert abs(grad @ v - dJ) <= 1e-8 * abs(dJ)
+
+
+def _node_directions(m, n, seed, interior=True):
+    """Random node displacements of about a tenth of the spacing, interior
+    nodes only by default (moving a boundary node moves the boundary)."""
+    rng = np.random.default_rng(seed)
+    keep = np.ones(m.x.size, dtype=bool)
+    if interior:
+        keep[np.asarray(m.bnd).ravel()] = False
+    h = 0.1 / 20
+    return [(np.where(keep, rng.standard_normal(m.x.size), 0.0) * h,
+             np.where(keep, rng.standard_normal(m.x.size), 0.0) * h) for _ in range(n)]
+
+
+@pytest.mark.parametrize("variant", ["workload", "first_order_fixed_dt", "edge_min_dt"])
+def test_node_gradient_matches_complex_step_of_the_restatement(variant):
+    """dJ/d(node coordinates) against the complex-step derivative of the CPU
+    restatement, whose geometry is itself computed over complex node
+    coordinates: agreement to 1e-8 along random interior displacements.  In
+    the first-order variant also along a displacement of every node,
+    boundary included; with the limiter that direction is not
+    differentiable here: the boundary nodes are not jittered, so congruent
+    boundary cells in the uniform inflow tie exactly in the limiter's
+    minimum and one-sided differences disagree (a kink, not a rounding
+    effect)."""
+    import oracle as O
+    print("case node gradient", variant, flush=True)
+    m, cfg, ic = workloads.forward_step(n_per_unit=20)
+    free = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+    if variant == "first_order_fixed_dt":
+        cfg = dict(flux="roe_joined", gamma=1.4, dt="fixed", dt_fixed=1e-4, gdt=True,
+                   bcs={"wall": ("mirror",), "inlet": free, "outlet": free})
+    elif variant == "edge_min_dt":
+        cfg = dict(cfg, dt="edge_min", cfl=0.5)
+    pm = Mesh.from_arrays(m)
+    om = O.Mesh(m)
+    w = workloads.step_face_weights(pm)
+    q0 = workloads.bump_state(pm)
+    s = Solver(pm, cfg)
+    J, grad, gxy, st = s.adjoint(q0, 6, wx=w, nodes=True)
+    J2, grad2, _ = s.adjoint(q0, 6, wx=w)
+    assert J == J2 and np.array_equal(grad, grad2)   # the state gradient does not change
+    dirs = _node_directions(m, 3, 31)
+    if variant == "first_order_fixed_dt":
+        dirs += _node_directions(m, 1, 32, interior=False)
+    for dx, dy in dirs:
+        Jc, dJ = O.cstep_derivative_nodes(om, cfg, q0, 6, dx=dx, dy=dy, wx=w)
+        ad = float(gxy[:, 0] @ dx + gxy[:, 1] @ dy)
+        print("  complex step %.15e adjoint %.15e" % (dJ, ad), flush=True)
+        assert abs(Jc - J) <= 1e-12 * abs(J)
+        assert abs(ad - dJ) <= 1e-8 * abs(dJ)
+
+
+def test_node_gradient_is_translation_invariant():
+    """The scheme sees only differences of coordinates: moving every node
+    by the same vector changes nothing, so the node gradient sums to zero
+    in x and in y."""
+    print("case node gradient translation", flush=True)
+    pm, cfg, w, q0 = _case()
+    J, grad, gxy, st = Solver(pm, cfg).adjoint(q0, 4, wx=w, nodes=True)
+    scale = np.abs(gxy).sum(axis=0)
+    assert np.all(np.abs(gxy.sum(axis=0)) <= 1e-10 * scale), (gxy.sum(axis=0), scale)
+
+
+def test_node_gradient_matches_central_differences_of_moved_meshes():
+    """Central differences of J over meshes rebuilt with displaced interior
+    nodes (first-order scheme with a fixed step, where J is smooth in the
+    coordinates)."""
+    import dataclasses
+    print("case node gradient fd", flush=True)
+    m, cfg, ic = workloads.forward_step(n_per_unit=20)
+    free = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
+    cfg = dict(flux="roe_joined", gamma=1.4, dt="fixed", dt_fixed=1e-4, gdt=True,
+               bcs={"wall": ("mirror",), "inlet": free, "outlet": free})
+    pm = Mesh.from_arrays(m)
+    w = workloads.step_face_weights(pm)
+    q0 = workloads.bump_state(pm)
+    J, grad, gxy, st = Solver(pm, cfg).adjoint(q0, 6, wx=w, nodes=True)
+    eps = 1e-6
+    for dx, dy in _node_directions(m, 2, 41):
+        def J_at(t):
+            mm = Mesh.from_arrays(dataclasses.replace(m, x=m.x + t * dx, y=m.y + t * dy))
+            return Solver(mm, cfg).objective(q0, 6, wx=w)
+        fd = (J_at(eps) - J_at(-eps)) / (2 * eps)
+        ad = float(gxy[:, 0] @ dx + gxy[:, 1] @ dy)
+        print("  fd %.12e adjoint %.12e" % (fd, ad), flush=True)
+        assert abs(ad - fd) <= 1e-6 * abs(fd)
