@@ -221,6 +221,22 @@ FVH_HD void boundary_vjp(const BcParam<T>& P, T g, const T* q, T n0, T n1, const
     }
 }
 
+// One column of the boundary function's Jacobian against bb: sum_k
+// (d b_k / d x_j) bb_k with x = (q_0..q_3, n_0, n_1), by one dual-number
+// evaluation.
+template <class T>
+FVH_HD T boundary_dir(const BcParam<T>& P, T g, const T* q, T n0, T n1, int j, const T* bb) {
+    BcParam<Dual<T>> Pd;
+    Pd.kind = P.kind;
+    Pd.rho = Dual<T>(P.rho); Pd.u = Dual<T>(P.u); Pd.v = Dual<T>(P.v); Pd.p = Dual<T>(P.p);
+    Dual<T> qd[4], bd[4];
+    for (int k = 0; k < 4; ++k) qd[k] = Dual<T>(q[k], k == j ? T(1) : T(0));
+    boundary(Pd, Dual<T>(g), bd, qd, Dual<T>(n0, j == 4 ? T(1) : T(0)), Dual<T>(n1, j == 5 ? T(1) : T(0)));
+    T s = T(0);
+    for (int k = 0; k < 4; ++k) s = s + bd[k].d * bb[k];
+    return s;
+}
+
 // Adjoint of the edge normal through a boundary function: nb += (db/dn)^T bb
 // (two dual-number evaluations, one per normal component).
 template <class T>

@@ -34,72 +34,123 @@ __device__ inline void q4_add(T* a, const Q4<T>& b) {
     for (int k = 0; k < 4; ++k) a[k] = a[k] + b.v[k];
 }
 
-// ---- update pass --------------------------------------------------------
+// Adjoint of a stage's output state: an explicit array, or gathered on the
+// fly from the next stage's reverse (its per-cell direct terms, per-half-edge
+// face adjoints and per-half-edge neighbour contributions), in the order the
+// gather pass uses.
+template <class T>
+struct LamSrc {
+    const Q4<T>* lam;      // explicit, or nullptr: gathered from the three below
+    const Q4<T>* direct;
+    const Q4<T>* face;
+    const Q4<T>* nb;
+};
+
+template <class T>
+__device__ inline Q4<T> lam_of(const MeshView<T>& m, const LamSrc<T>& L, int c) {
+    if (L.lam) return L.lam[c];
+    T v[4] = {T(0), T(0), T(0), T(0)};
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    if (c < m.n_real) {
+        q4_add(v, L.direct[c]);
+    } else {
+        for (int h = h0; h < h1; ++h) q4_add(v, L.face[h]);   // a ghost's face state is the ghost state
+    }
+    for (int h = h0; h < h1; ++h)
+        if (m.he_nbr[h] < m.n_real) q4_add(v, L.nb[m.he_twin[h]]);
+    return Q4<T>{{v[0], v[1], v[2], v[3]}};
+}
+
+// ---- boundary pass (per ghost cell) -----------------------------------------
+// ghost = boundary(interior face state of the owner, n): the adjoint of the
+// ghost's output reversed into the owner's updated state (lam_bq) and, with
+// reconstruction, into the owner's face increment (lam_bincr).  kBcLanes
+// lanes per ghost, one Jacobian column each (lanes 0-3 the state, 4-5 the
+// normal when the geometry adjoint is formed), combined by shuffles.
+constexpr int kBcLanes = 8;
+
 template <class T, bool LIN, bool GEO>
-__global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm, PhysParams pp,
-                                                       const BcParam<T>* __restrict__ bcs,
-                                                       const Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ gx,
-                                                       const Q4<T>* __restrict__ gy, const Q4<T>* __restrict__ lm,
-                                                       const Q4<T>* __restrict__ qt, const T* __restrict__ dt,
-                                                       const T* __restrict__ dt_g, T alpha,
-                                                       const Q4<T>* __restrict__ lam_out, Q4<T>* __restrict__ lam_q0,
-                                                       Q4<T>* __restrict__ lam_qt, Q4<T>* __restrict__ lam_bincr,
-                                                       T* __restrict__ lam_dt, double* __restrict__ ld_blk,
-                                                       GeoAdj<T> ga) {
-    const int lb = xcd_block(tm.n_rt);
-    if (lb < 0) return;
-    const int c = tm.c0[lb] + threadIdx.x;
-    const T g = T(pp.gamma);
-    double ld = 0.0;
-    if (c < tm.c0[lb + 1]) {
-        T lam[4];
-        const Q4<T> lo = lam_out[c];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lam[k] = lo.v[k];
+__global__ __launch_bounds__(kBlock) void k_adj_bc(MeshView<T> m, PhysParams pp, const BcParam<T>* __restrict__ bcs,
+                                                   const Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ gx,
+                                                   const Q4<T>* __restrict__ gy, const Q4<T>* __restrict__ lm,
+                                                   LamSrc<T> L, Q4<T>* __restrict__ lam_bq,
+                                                   Q4<T>* __restrict__ lam_bincr, GeoAdj<T> ga) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = t % kBcLanes;
+    const int gc = m.n_real + t / kBcLanes;
+    // every lane of a group takes the same branches; a group never straddles a wavefront
+    const bool live = gc < m.n_total;
+    const int hg = live ? m.he_off[gc] : 0;
+    const bool has_edge = live && m.he_off[gc + 1] > hg;   // (a repeated boundary element leaves a ghost with no edge)
+    T col = T(0);
+    bool fallback = !LIN;
+    int h = 0;
+    if (has_edge) {
+        h = m.he_twin[hg];   // the owner's half-edge
+        const int c = m.he_nbr[hg];
+        const uint32_t ue = m.he_edge[h];
+        const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+        const T g = T(pp.gamma);
         const Q4<T> out = qout[c];
-        const G2<T> cc = m.c_c[c];
-        for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
-            const uint32_t ue = m.he_edge[h];
-            if (!(ue & kBcBit)) continue;
-            const int nb = m.he_nbr[h];
-            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
-            T qi[4];
-            bool fallback = !LIN;
-            if (LIN) {
-                T ax[4], ay[4], al[4];
-                const Q4<T> a = gx[c], b = gy[c], d = lm[c];   // read only for cells on a boundary
+        T qi[4];
+        if (LIN) {
+            T ax[4], ay[4], al[4];
+            const Q4<T> a = gx[c], b = gy[c], d = lm[c];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) { ax[k] = a.v[k]; ay[k] = b.v[k]; al[k] = d.v[k]; }
-                const Q4<T> inc = face_inc(ax, ay, al, m.e_c[ue & kEdgeMask], cc);
+            for (int k = 0; k < 4; ++k) { ax[k] = a.v[k]; ay[k] = b.v[k]; al[k] = d.v[k]; }
+            const Q4<T> inc = face_inc(ax, ay, al, m.e_c[ue & kEdgeMask], m.c_c[c]);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + inc.v[k];
-                if (pp.positivity && !physical(g, qi)) {
-                    fallback = true;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
-                }
-            } else {
+            for (int k = 0; k < 4; ++k) qi[k] = out.v[k] + inc.v[k];
+            if (pp.positivity && !physical(g, qi)) {
+                fallback = true;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
             }
-            const Q4<T> lg = lam_out[nb];
-            T qb[4] = {T(0), T(0), T(0), T(0)};
-            boundary_vjp(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lg.v, qb);
-            if (GEO) {
-                // the boundary function's dependence on the edge normal
-                T nb2[2] = {T(0), T(0)};
-                boundary_vjp_n(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lg.v, nb2);
-                Q4<T> a = ga.he[h];
-                a.v[0] = a.v[0] + nb2[0];
-                a.v[1] = a.v[1] + nb2[1];
-                ga.he[h] = a;
-            }
+        } else {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lam[k] = lam[k] + qb[k];
-            Q4<T> lbi;
+            for (int k = 0; k < 4; ++k) qi[k] = out.v[k];
+        }
+        const Q4<T> lg = lam_of(m, L, gc);
+        if (lane < 4 || (GEO && lane < 6)) col = boundary_dir(bcs[m.he_bc[h]], g, qi, geo.v[0], geo.v[1], lane, lg.v);
+    }
+    const int base = (threadIdx.x / kBcLanes) * kBcLanes;
+    T v[6];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lbi.v[k] = fallback ? T(0) : qb[k];
-            lam_bincr[nb - m.n_real] = lbi;
+    for (int j = 0; j < 6; ++j) v[j] = __shfl(col, base + j);
+    if (!live || lane != 0) return;
+    // (0 + x: the sum boundary_vjp forms for one edge)
+    const Q4<T> qb4{{T(0) + v[0], T(0) + v[1], T(0) + v[2], T(0) + v[3]}};
+    lam_bq[gc - m.n_real] = qb4;
+    lam_bincr[gc - m.n_real] = fallback ? Q4<T>{{T(0), T(0), T(0), T(0)}} : qb4;
+    if (GEO && has_edge) {
+        Q4<T> a = ga.he[h];
+        a.v[0] = a.v[0] + v[4];
+        a.v[1] = a.v[1] + v[5];
+        ga.he[h] = a;
+    }
+}
+
+// ---- update pass --------------------------------------------------------
+template <class T, bool GEO>
+__global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm, PhysParams pp,
+                                                       const Q4<T>* __restrict__ qt, const T* __restrict__ dt,
+                                                       const T* __restrict__ dt_g, T alpha, LamSrc<T> L,
+                                                       const Q4<T>* __restrict__ lam_bq, Q4<T>* __restrict__ lam_q0,
+                                                       Q4<T>* __restrict__ lam_qt, T* __restrict__ lam_dt,
+                                                       double* __restrict__ ld_blk, GeoAdj<T> ga) {
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    const int c = tm.c0[lb] + threadIdx.x;
+    double ld = 0.0;
+    if (c < tm.c0[lb + 1]) {
+        T lam[4];
+        const Q4<T> lo = lam_of(m, L, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lam[k] = lo.v[k];
+        // ghosts updated from this cell's face states (boundary pass)
+        for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
+            if (!(m.he_edge[h] & kBcBit)) continue;
+            q4_add(lam, lam_bq[m.he_nbr[h] - m.n_real]);
         }
         Q4<T> a0 = lam_q0[c];
 #pragma unroll
@@ -701,30 +752,41 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
                       bool grad, bool lim, bool lin, const BcParam<T>* bcs, int n_edges, const int4* epair,
                       const Q4<T>* qin, const Q4<T>* qout, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lm,
                       const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
-                      Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
-                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk, const GeoAdj<T>& ga) {
+                      Q4<T>* lam_qt, Q4<T>* lam_bq, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct,
+                      Q4<T>* nb_contrib, Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk,
+                      const GeoAdj<T>& ga) {
     const int nrt = xcd_grid_blocks(tm.n_rt);
     if (nrt == 0) return;
     const bool geo = ga.he != nullptr;
-#define FVH_AU(L, G) hipLaunchKernelGGL((k_adj_update<T, L, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, bcs, qout, \
-                                        gx, gy, lm, qt, dt, dt_g, alpha, lam_out, lam_q0, lam_qt, lam_bincr, lam_dt,  \
-                                        ld_blk, ga)
-    if (lin) { if (geo) FVH_AU(true, true); else FVH_AU(true, false); }
-    else { if (geo) FVH_AU(false, true); else FVH_AU(false, false); }
-#undef FVH_AU
+    // the adjoint of this stage's output: explicit, or the parts the previous call left
+    const LamSrc<T> L{lam_out, lam_direct, lam_face, nb_contrib};
+    const int n_ghost = m.n_total - m.n_real;
+    if (n_ghost > 0) {
+#define FVH_AB(L_, G) hipLaunchKernelGGL((k_adj_bc<T, L_, G>), dim3(nblocks(n_ghost * kBcLanes)), dim3(kBlock), 0, s, m, pp, \
+                                         bcs, qout, gx, gy, lm, L, lam_bq, lam_bincr, ga)
+        if (lin) { if (geo) FVH_AB(true, true); else FVH_AB(true, false); }
+        else { if (geo) FVH_AB(false, true); else FVH_AB(false, false); }
+#undef FVH_AB
+    }
+    if (geo)
+        hipLaunchKernelGGL((k_adj_update<T, true>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, qt, dt, dt_g, alpha, L,
+                           (const Q4<T>*)lam_bq, lam_q0, lam_qt, lam_dt, ld_blk, ga);
+    else
+        hipLaunchKernelGGL((k_adj_update<T, false>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, qt, dt, dt_g, alpha, L,
+                           (const Q4<T>*)lam_bq, lam_q0, lam_qt, lam_dt, ld_blk, ga);
     const int ne = nblocks(n_edges);
-#define FVH_AF(L, G) hipLaunchKernelGGL((k_adj_flux<T, L, G>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair,  \
-                                        qin, gx, gy, lm, (const Q4<T>*)lam_qt, lam_face, ga)
+#define FVH_AF(L_, G) hipLaunchKernelGGL((k_adj_flux<T, L_, G>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, \
+                                         qin, gx, gy, lm, (const Q4<T>*)lam_qt, lam_face, ga)
     if (lin) { if (geo) FVH_AF(true, true); else FVH_AF(true, false); }
     else { if (geo) FVH_AF(false, true); else FVH_AF(false, false); }
 #undef FVH_AF
-#define FVH_AG(P, L, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp,      \
-                                           (int)grad, (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face,              \
-                                           (const Q4<T>*)lam_bincr, lam_direct, nb_contrib, ga)
-#define FVH_AGL(P)                                                        \
-    do {                                                                  \
-        if (lin) { if (geo) FVH_AG(P, true, true); else FVH_AG(P, true, false); }     \
-        else { if (geo) FVH_AG(P, false, true); else FVH_AG(P, false, false); }       \
+#define FVH_AG(P, L_, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L_, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp,    \
+                                            (int)grad, (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face,            \
+                                            (const Q4<T>*)lam_bincr, lam_direct, nb_contrib, ga)
+#define FVH_AGL(P)                                                                \
+    do {                                                                          \
+        if (lin) { if (geo) FVH_AG(P, true, true); else FVH_AG(P, true, false); } \
+        else { if (geo) FVH_AG(P, false, true); else FVH_AG(P, false, false); }   \
     } while (0)
     switch (phi_kind) {
         case PHI_ZERO: FVH_AGL(PHI_ZERO); break;
@@ -734,9 +796,12 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
     }
 #undef FVH_AGL
 #undef FVH_AG
-    hipLaunchKernelGGL((k_adj_gather<T>), dim3(nblocks(m.n_total)), dim3(kBlock), 0, s, m,
-                       (const Q4<T>*)lam_direct, (const Q4<T>*)lam_face, (const Q4<T>*)nb_contrib, lam_in,
-                       accumulate_in);
+    // the adjoint of the stage input: gathered here only when asked for;
+    // otherwise the next call gathers it on the fly
+    if (lam_in)
+        hipLaunchKernelGGL((k_adj_gather<T>), dim3(nblocks(m.n_total)), dim3(kBlock), 0, s, m,
+                           (const Q4<T>*)lam_direct, (const Q4<T>*)lam_face, (const Q4<T>*)nb_contrib, lam_in,
+                           accumulate_in);
 }
 
 template <class T>
@@ -790,7 +855,7 @@ template void launch_adj_stage<double>(hipStream_t, const MeshView<double>&, con
                                        const Q4<double>*, const Q4<double>*, const Q4<double>*, const Q4<double>*,
                                        const Q4<double>*, const Q4<double>*, const double*, const double*, double,
                                        const Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*,
-                                       Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*,
+                                       Q4<double>*, Q4<double>*, Q4<double>*, Q4<double>*, int, double*, double*,
                                        const GeoAdj<double>&);
 template void launch_adj_dt<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int, const Q4<double>*,
                                     const int32_t*, const double*, int, Q4<double>*, const GeoAdj<double>&);

@@ -177,13 +177,17 @@ struct GeoAdj {
 // Discrete adjoint (fvh_adjoint.hip): the reverse of one RK stage, of the
 // global time step's minimum cell, of the initial boundary update, and the
 // boundary-pressure objective with its seed.  ga: geometry adjoint (or none).
+// lam_out == nullptr: the adjoint of the stage output is gathered from the
+// parts (lam_direct, lam_face, nb_contrib) the previous call left; lam_in ==
+// nullptr: this call's parts are left for the next call instead of gathered.
 template <class T>
 void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
                       bool grad, bool lim, bool lin, const BcParam<T>* bcs, int n_edges, const int4* epair,
                       const Q4<T>* qin, const Q4<T>* qout, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lm,
                       const Q4<T>* qt, const T* dt, const T* dt_g, T alpha, const Q4<T>* lam_out, Q4<T>* lam_q0,
-                      Q4<T>* lam_qt, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct, Q4<T>* nb_contrib,
-                      Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk, const GeoAdj<T>& ga);
+                      Q4<T>* lam_qt, Q4<T>* lam_bq, Q4<T>* lam_bincr, Q4<T>* lam_face, Q4<T>* lam_direct,
+                      Q4<T>* nb_contrib, Q4<T>* lam_in, int accumulate_in, T* lam_dt, double* ld_blk,
+                      const GeoAdj<T>& ga);
 constexpr int kArgminBlocks = 256;
 template <class T>
 void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, double* bv, int32_t* bref,

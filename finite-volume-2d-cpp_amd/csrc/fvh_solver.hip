@@ -106,10 +106,12 @@ struct fvh_solver {
     std::vector<double> b_nx, b_ny, b_len;
     fvh::DevBuf a_qs[fvh::kAdjMaxStages + 1], a_gx[fvh::kAdjMaxStages], a_gy[fvh::kAdjMaxStages],
         a_lm[fvh::kAdjMaxStages], a_qt[fvh::kAdjMaxStages];
-    fvh::DevBuf a_lam, a_lam_in, a_lam_q0, a_lam_qt, a_lam_bincr, a_lam_face, a_lam_direct, a_nb, a_ld, a_tmp;
+    fvh::DevBuf a_lam, a_lam_q0, a_lam_qt, a_lam_bq, a_lam_bincr, a_lam_face, a_lam_direct, a_nb, a_ld,
+        a_tmp;
     fvh::DevBuf a_obj_ghost, a_obj_gw, a_obj_terms;
     fvh::DevBuf a_ref, a_amin_v, a_amin_r, a_amin_d, a_amin;   // device -> reference index; argmin scratch
     std::vector<std::unique_ptr<fvh::DevBuf>> a_ck;
+    std::vector<std::unique_ptr<fvh::DevBuf>> a_tape;   // stage records of the last steps (one buffer per step)
     hipGraph_t a_graph = nullptr;            // one step's replay + reverse, captured once
     hipGraphExec_t a_graph_exec = nullptr;
     bool a_graph_geo = false;                // the captured reverse forms the geometry adjoint
@@ -1092,12 +1094,64 @@ int32_t sums_t(fvh_solver* s, double* out) {
 // ---------------------------------------------------------------------------
 // discrete adjoint driver (FP64, single instance)
 // ---------------------------------------------------------------------------
+// What the reverse of one step reads: the state entering every stage and
+// the step's output, each stage's gradient, limiter and qt, the cell time
+// steps and the global one.
+struct StepRecord {
+    void* qs[kAdjMaxStages + 1];
+    void* gx[kAdjMaxStages];
+    void* gy[kAdjMaxStages];
+    void* lm[kAdjMaxStages];
+    void* qt[kAdjMaxStages];
+    void* dt;
+    void* dtg;
+};
+
 struct Adjoint {
     fvh_solver* s;
     int n_stage = 0;
     size_t qbytes = 0;
     int64_t fwd_steps = 0;
     GeoAdj<double> ga;   // geometry adjoint (he == nullptr: not formed)
+    StepRecord work{};   // the replay buffers
+    std::vector<StepRecord> tapes;   // records kept by the forward sweep
+
+    // bytes of one kept step record
+    size_t record_bytes() const {
+        return (size_t)(n_stage + 1 + 4 * n_stage) * qbytes + (size_t)s->n_total * 8 + 256;
+    }
+    StepRecord work_record() const {
+        StepRecord r{};
+        for (int k = 0; k <= n_stage; ++k) r.qs[k] = s->a_qs[k].p;
+        for (int k = 0; k < n_stage; ++k) {
+            r.gx[k] = s->a_gx[k].p; r.gy[k] = s->a_gy[k].p; r.lm[k] = s->a_lm[k].p; r.qt[k] = s->a_qt[k].p;
+        }
+        r.dt = s->dt.p;
+        r.dtg = s->dt_global.p;
+        return r;
+    }
+    // n records carved out of one allocation each
+    int32_t alloc_tapes(int n) {
+        tapes.clear();
+        const size_t q = (qbytes + 255) / 256 * 256, d = ((size_t)s->n_total * 8 + 255) / 256 * 256;
+        for (int i = 0; i < n; ++i) {
+            if ((int)s->a_tape.size() <= i) {
+                std::unique_ptr<DevBuf> b(new DevBuf());
+                if (b->alloc(record_bytes() + (size_t)(n_stage + 1 + 4 * n_stage) * 256 + d)) return oom();
+                s->a_tape.push_back(std::move(b));
+            }
+            char* p = (char*)s->a_tape[i]->p;
+            StepRecord r{};
+            for (int k = 0; k <= n_stage; ++k) { r.qs[k] = p; p += q; }
+            for (int k = 0; k < n_stage; ++k) {
+                r.gx[k] = p; p += q; r.gy[k] = p; p += q; r.lm[k] = p; p += q; r.qt[k] = p; p += q;
+            }
+            r.dt = p; p += d;
+            r.dtg = p;
+            tapes.push_back(r);
+        }
+        return FVH_OK;
+    }
 
     // zeroed geometry-adjoint accumulators
     int32_t alloc_geo() {
@@ -1131,9 +1185,10 @@ struct Adjoint {
                 (!s->a_lm[k].p && s->a_lm[k].alloc(qbytes)) || (!s->a_qt[k].p && s->a_qt[k].alloc(qbytes)))
                 return oom();
         const size_t he = (size_t)std::max(s->n_he, 1) * sizeof(Q4<double>);
-        if ((!s->a_lam.p && s->a_lam.alloc(qbytes)) || (!s->a_lam_in.p && s->a_lam_in.alloc(qbytes)) ||
+        if ((!s->a_lam.p && s->a_lam.alloc(qbytes)) ||
             (!s->a_lam_q0.p && s->a_lam_q0.alloc(qbytes)) || (!s->a_lam_qt.p && s->a_lam_qt.alloc(qbytes)) ||
             (!s->a_lam_bincr.p && s->a_lam_bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<double>))) ||
+            (!s->a_lam_bq.p && s->a_lam_bq.alloc((size_t)(s->n_total - s->n_real + 1) * sizeof(Q4<double>))) ||
             (!s->a_lam_face.p && s->a_lam_face.alloc(he)) || (!s->a_nb.p && s->a_nb.alloc(he)) ||
             (!s->a_lam_direct.p && s->a_lam_direct.alloc(qbytes)) ||
             (!s->a_ld.p && s->a_ld.alloc((size_t)(kAdjMaxStages * (s->n_rt + 1)) * 8)) ||
@@ -1155,7 +1210,7 @@ struct Adjoint {
         return FVH_E_HIP;
     }
     double work_bytes() const {
-        return (double)qbytes * (n_stage + 1 + 4 * n_stage + 7) +
+        return (double)qbytes * (n_stage + 1 + 4 * n_stage + 6) +
                2.0 * (double)std::max(s->n_he, 1) * sizeof(Q4<double>);
     }
 
@@ -1172,8 +1227,8 @@ struct Adjoint {
         return FVH_OK;
     }
 
-    // one step from a_qs[0], keeping every stage's input, gradient, limiter and qt
-    int32_t replay_body() {
+    // one step from R.qs[0], keeping every stage's input, gradient, limiter and qt
+    int32_t replay_body(const StepRecord& R) {
         const fvh_config& c = s->cfg;
         const MeshView<double> m = view<double>(s);
         const TileMap tm = tile_map(s);
@@ -1184,30 +1239,29 @@ struct Adjoint {
         const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
                    lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
         hipStream_t st = s->stream;
-        double* dt = (double*)s->dt.p;
+        double* dt = (double*)R.dt;
         double* blk = (double*)s->blk_min.p;
-        if (!s->second) launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)s->a_qs[0].p, dt, blk, nullptr);
+        if (!s->second) launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)R.qs[0], dt, blk, nullptr);
         for (int k = 0; k < n_stage; ++k) {
-            const Q4<double>* qin = (const Q4<double>*)s->a_qs[k].p;
+            const Q4<double>* qin = (const Q4<double>*)R.qs[k];
             if (s->second)
                 launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, lin, qin, (Q4<double>*)s->face.p,
                                        (Q4<double>*)s->bincr.p, k == 0 ? c.dt_kind : -1, dt, blk,
-                                       (Q4<double>*)s->a_gx[k].p, (Q4<double>*)s->a_gy[k].p,
-                                       (Q4<double>*)s->a_lm[k].p, nullptr);
-            if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, s->dt_global.p, nullptr);
+                                       (Q4<double>*)R.gx[k], (Q4<double>*)R.gy[k], (Q4<double>*)R.lm[k], nullptr);
+            if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
             launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
-                                qin, (const Q4<double>*)s->a_qs[0].p, (Q4<double>*)s->a_qs[k + 1].p,
+                                qin, (const Q4<double>*)R.qs[0], (Q4<double>*)R.qs[k + 1],
                                 (const Q4<double>*)s->face.p, (const Q4<double>*)s->bincr.p, dt,
-                                gdt ? (const double*)s->dt_global.p : nullptr, c.alpha[k], nullptr,
-                                (Q4<double>*)s->a_qt[k].p, nullptr);
+                                gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, (Q4<double>*)R.qt[k],
+                                nullptr);
         }
         FVH_HIP(hipGetLastError());
         return FVH_OK;
     }
 
-    // reverse of the replayed step: a_lam (adjoint of the step's output) ->
+    // reverse of a recorded step: a_lam (adjoint of the step's output) ->
     // a_lam (adjoint of its input)
-    int32_t reverse() {
+    int32_t reverse(const StepRecord& R) {
         const fvh_config& c = s->cfg;
         const MeshView<double> m = view<double>(s);
         const TileMap tm = tile_map(s);
@@ -1216,28 +1270,27 @@ struct Adjoint {
                    lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
         hipStream_t st = s->stream;
         Q4<double>* lam = (Q4<double>*)s->a_lam.p;
-        Q4<double>* lam_in = (Q4<double>*)s->a_lam_in.p;
         Q4<double>* lam_q0 = (Q4<double>*)s->a_lam_q0.p;
         launch_fill<double>(st, lam_q0, s->n_total);
         for (int k = n_stage - 1; k >= 0; --k) {
             launch_adj_stage<double>(st, m, tm, pp, c.phi, grad, lim, lin, (const BcParam<double>*)s->bcs.p,
-                                     s->n_edges, (const int4*)s->epair.p, (const Q4<double>*)s->a_qs[k].p,
-                                     (const Q4<double>*)s->a_qs[k + 1].p, (const Q4<double>*)s->a_gx[k].p,
-                                     (const Q4<double>*)s->a_gy[k].p, (const Q4<double>*)s->a_lm[k].p,
-                                     (const Q4<double>*)s->a_qt[k].p, (const double*)s->dt.p,
-                                     gdt ? (const double*)s->dt_global.p : nullptr, c.alpha[k],
-                                     k == n_stage - 1 ? lam : lam_in, lam_q0, (Q4<double>*)s->a_lam_qt.p,
-                                     (Q4<double>*)s->a_lam_bincr.p, (Q4<double>*)s->a_lam_face.p,
-                                     (Q4<double>*)s->a_lam_direct.p, (Q4<double>*)s->a_nb.p,
-                                     k == 0 ? lam_q0 : lam_in, k == 0 ? 1 : 0, nullptr,
+                                     s->n_edges, (const int4*)s->epair.p, (const Q4<double>*)R.qs[k],
+                                     (const Q4<double>*)R.qs[k + 1], (const Q4<double>*)R.gx[k],
+                                     (const Q4<double>*)R.gy[k], (const Q4<double>*)R.lm[k],
+                                     (const Q4<double>*)R.qt[k], (const double*)R.dt,
+                                     gdt ? (const double*)R.dtg : nullptr, c.alpha[k],
+                                     k == n_stage - 1 ? lam : nullptr, lam_q0, (Q4<double>*)s->a_lam_qt.p,
+                                     (Q4<double>*)s->a_lam_bq.p, (Q4<double>*)s->a_lam_bincr.p,
+                                     (Q4<double>*)s->a_lam_face.p, (Q4<double>*)s->a_lam_direct.p,
+                                     (Q4<double>*)s->a_nb.p, k == 0 ? lam_q0 : nullptr, k == 0 ? 1 : 0, nullptr,
                                      (double*)s->a_ld.p + (size_t)k * s->n_rt, ga);
         }
         if (gdt && c.dt_kind != FVH_DT_FIXED) {
             // the cell whose step is the minimum, first in reference order (min_dt, explicit.cpp:555-564)
-            launch_dt_argmin<double>(st, (const double*)s->dt.p, (const int32_t*)s->a_ref.p, s->n_total,
+            launch_dt_argmin<double>(st, (const double*)R.dt, (const int32_t*)s->a_ref.p, s->n_total,
                                      (double*)s->a_amin_v.p, (int32_t*)s->a_amin_r.p, (int32_t*)s->a_amin_d.p,
                                      (int32_t*)s->a_amin.p);
-            launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)s->a_qs[0].p, (const int32_t*)s->a_amin.p,
+            launch_adj_dt<double>(st, m, pp, c.dt_kind, (const Q4<double>*)R.qs[0], (const int32_t*)s->a_amin.p,
                                   (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0, ga);
         }
         FVH_HIP(hipGetLastError());
@@ -1259,8 +1312,9 @@ struct Adjoint {
         }
         if (!s->a_graph_exec) {
             FVH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            int32_t rc = replay_body();
-            if (!rc) rc = reverse();
+            work = work_record();
+            int32_t rc = replay_body(work);
+            if (!rc) rc = reverse(work);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(s->stream, &g);
             if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1545,13 +1599,21 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     Adjoint A{s};
     A.qbytes = (size_t)s->n_total * sizeof(Q4<double>);
     A.n_stage = s->cfg.n_alpha;
+    const double cap = mem_cap_bytes > 0 ? mem_cap_bytes : 8e9;
+    const double avail = cap - A.work_bytes() - (grad_nodes ? 2.0 * A.qbytes + 3.0 * s->n_he * 32.0 : 0.0);
+    int32_t n_rec = 0;   // last steps kept whole by the forward sweep (no replay in the reverse)
     if (snaps == 0) {
-        const double cap = mem_cap_bytes > 0 ? mem_cap_bytes : 8e9;
-        const double avail = cap - A.work_bytes();
         snaps = (int32_t)std::max(1.0, std::min((double)std::max(n_steps, 1), std::floor(avail / A.qbytes)));
+        if (snaps >= n_steps && n_steps > 0) {
+            // the rest of the budget: step records instead of checkpoints, latest steps first
+            const double extra = (double)A.record_bytes() + 2048.0 * (A.n_stage + 2) - (double)A.qbytes;
+            n_rec = (int32_t)std::max(0.0, std::min((double)n_steps, std::floor((avail - (double)n_steps * A.qbytes) / extra)));
+            snaps = std::max(1, n_steps - n_rec);
+        }
     }
     snaps = std::max(1, std::min(snaps, std::max(n_steps, 1)));
     if ((rc = A.alloc(snaps)) || (rc = objective_setup(s, wx, wy))) return rc;
+    if (n_rec && (rc = A.alloc_tapes(n_rec))) return rc;
     if (grad_nodes && (rc = A.alloc_geo())) return rc;
     // initial state: bounds applied (explicit.cpp:712), kept in slot 0
     s->cur = 0;
@@ -1560,14 +1622,26 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     // forward to the objective, then seed the adjoint of the final state.
     // With a slot per step the sweep keeps every step's state and each step
     // is replayed once; otherwise the binomial schedule recomputes.
-    const bool every = snaps >= n_steps;
+    const bool every = n_rec > 0 || snaps >= n_steps;
+    const int32_t i0 = n_steps - n_rec;   // first recorded step
+    const void* final_state = s->a_tmp.p;
     if (every) {
         // step in the solver's own buffers, keeping a copy of each step's start
         fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
-        for (int k = 0; k < n_steps; ++k) {
+        for (int k = 0; k < i0; ++k) {
             if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
-            FVH_HIP(hipMemcpyAsync(k + 1 < n_steps ? s->a_ck[k + 1]->p : s->a_tmp.p, s->q[s->cur].p, A.qbytes,
-                                   hipMemcpyDeviceToDevice, s->stream));
+            void* dst = k + 1 < i0 ? s->a_ck[k + 1]->p : (n_rec ? A.tapes[0].qs[0] : s->a_tmp.p);
+            FVH_HIP(hipMemcpyAsync(dst, s->q[s->cur].p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+        }
+        if (n_rec) {
+            // the last steps recorded whole; each record's output is the next one's input
+            if (i0 == 0)
+                FVH_HIP(hipMemcpyAsync(A.tapes[0].qs[0], s->a_ck[0]->p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+            for (int k = 0; k < n_rec; ++k) {
+                if (k > 0) A.tapes[k].qs[0] = A.tapes[k - 1].qs[A.n_stage];
+                if ((rc = A.replay_body(A.tapes[k]))) return rc;
+            }
+            final_state = A.tapes[n_rec - 1].qs[A.n_stage];
         }
         A.fwd_steps += n_steps;
         if (n_steps == 0)
@@ -1577,9 +1651,11 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     }
     FVH_HIP(hipStreamSynchronize(s->stream));
     const auto t1 = std::chrono::steady_clock::now();
-    if ((rc = objective_eval(s, s->a_tmp.p, true, J, grad_nodes ? (double*)s->a_obj_p.p : nullptr))) return rc;
+    if ((rc = objective_eval(s, final_state, true, J, grad_nodes ? (double*)s->a_obj_p.p : nullptr))) return rc;
     if (every) {
-        for (int k = n_steps - 1; k >= 0; --k)
+        for (int k = n_rec - 1; k >= 0; --k)
+            if ((rc = A.reverse(A.tapes[k]))) return rc;
+        for (int k = i0 - 1; k >= 0; --k)
             if ((rc = A.step_back(s->a_ck[k]->p))) return rc;
     } else {
         std::vector<int> free_slots;
@@ -1599,7 +1675,8 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
         st->steps = n_steps;
         st->snaps = snaps;
         st->forward_steps = A.fwd_steps;
-        st->checkpoint_bytes = (double)snaps * A.qbytes;
+        st->checkpoint_bytes = (double)snaps * A.qbytes + (double)n_rec * A.record_bytes();
+        st->recorded_steps = n_rec;
         st->work_bytes = A.work_bytes();
         st->forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();

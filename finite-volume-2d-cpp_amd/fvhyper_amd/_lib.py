@@ -50,7 +50,8 @@ class RunOptions(C.Structure):
 class AdjointStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("snaps", C.c_int32), ("forward_steps", C.c_int64),
                 ("checkpoint_bytes", C.c_double), ("work_bytes", C.c_double),
-                ("forward_ms", C.c_double), ("total_ms", C.c_double)]
+                ("forward_ms", C.c_double), ("total_ms", C.c_double), ("recorded_steps", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Status(C.Structure):

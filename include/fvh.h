@@ -245,7 +245,9 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
  * lengths, areas, centres, Green-Gauss weights, ghost centres), chained to
  * the nodes; boundary nodes included (moving them moves the boundary).  Forward states
  * are kept at binomially placed step checkpoints: `snaps` slots (0 = as many
- * as mem_cap_bytes allows, at most one per step).  FP64, single-mesh
+ * as mem_cap_bytes allows, at most one per step; then the budget left over
+ * keeps the last steps' whole stage records, which the reverse reads without
+ * replaying those steps).  FP64, single-mesh
  * solvers, global or fixed time step. */
 typedef struct fvh_adjoint_stats {
     int32_t steps;             /* steps of the objective's run */
@@ -255,6 +257,8 @@ typedef struct fvh_adjoint_stats {
     double work_bytes;         /* device memory of one step's replay and adjoint buffers */
     double forward_ms;         /* the plain forward run to the objective */
     double total_ms;           /* the whole gradient evaluation */
+    int32_t recorded_steps;    /* last steps whose stage records the forward sweep kept (no replay) */
+    int32_t reserved;
 } fvh_adjoint_stats;
 
 int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const double* wx, const double* wy,

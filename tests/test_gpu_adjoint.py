@@ -80,6 +80,34 @@ def test_checkpoint_schedule_does_not_change_the_gradient():
     assert res[0][2]["snaps"] == 1 and res[3][2]["snaps"] == 12
 
 
+def test_recorded_steps_do_not_change_the_gradient():
+    """With the memory budget left after one checkpoint per step, the
+    forward sweep keeps the last steps' whole stage records and the reverse
+    reads them instead of replaying: same J and gradients, bit for bit, with
+    fewer forward steps; and with a cap that fits every record, none is
+    replayed."""
+    print("case recorded steps", flush=True)
+    pm, cfg, w, q0 = _case()
+    s = Solver(pm, cfg)
+    n = 12
+    ref = s.adjoint(q0, n, wx=w, snaps=n)
+    J0, g0, x0, st0 = s.adjoint(q0, n, wx=w, snaps=n, nodes=True)
+    assert ref[2]["recorded_steps"] == 0 and st0["recorded_steps"] == 0
+    rec = ref[2]["work_bytes"]
+    per_step = 5 * 26 * pm.n_cells * 32 / 5   # about one record: 26 state-sized arrays
+    for cap, want in ((ref[2]["work_bytes"] + n * pm.n_cells * 32 + 4.5 * per_step, "some"), (8e9, "all")):
+        J, g, x, st = s.adjoint(q0, n, wx=w, mem_cap_bytes=cap, nodes=True)
+        print("  cap %.3g: recorded %d of %d, forward steps %d" % (cap, st["recorded_steps"], n, st["forward_steps"]),
+              flush=True)
+        assert J == J0 and np.array_equal(g, g0) and np.array_equal(g, ref[1])
+        assert np.allclose(x, x0, rtol=0, atol=1e-13 * np.abs(x0).max())
+        if want == "all":
+            assert st["recorded_steps"] == n and st["forward_steps"] == n
+        else:
+            assert 0 < st["recorded_steps"] < n
+            assert st["forward_steps"] == 2 * n - st["recorded_steps"]
+
+
 def test_adjoint_with_step_graphs_is_unchanged():
     pm, cfg, w, q0 = _case()
     a = Solver(pm, cfg).adjoint(q0, 5, wx=w)
