@@ -283,13 +283,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
     T lqc[4] = {T(0), T(0), T(0), T(0)};
     T lgx[4] = {T(0), T(0), T(0), T(0)}, lgy[4] = {T(0), T(0), T(0), T(0)}, llim[4] = {T(0), T(0), T(0), T(0)};
     T lA = T(0), lcx = T(0), lcy = T(0);   // geometry adjoint of this cell (GEO)
+    // contributions to the neighbours, per half-edge (a real cell has at most
+    // kMaxHe; the host checks), stored once at the end
+    T nbc[kMaxHe][4];
+#pragma unroll
+    for (int j = 0; j < kMaxHe; ++j)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nbc[j][k] = T(0);
     // face states q + inc (and boundary increments) reversed
     for (int h = h0; h < h1; ++h) {
         const uint32_t ue = m.he_edge[h];
         const G2<T> ec = m.e_c[ue & kEdgeMask];
         const Q4<T> lf = lam_face[h];
         q4_add(lqc, lf);
-        nb_contrib[h] = Q4<T>{{T(0), T(0), T(0), T(0)}};
         if (!LIN) continue;
         const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
         T fs[4];
@@ -376,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                 if (l < run[k]) { run[k] = l; hsel[k] = h; }
             }
         }
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int hs = hsel[k];
             if (llim[k] == T(0)) continue;
@@ -428,9 +435,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
             }
             lqc[k] = lqc[k] - ldmax - ldmin;
             if (hmx[k] < 0) lqc[k] = lqc[k] + ldmax;
-            else { Q4<T> t = nb_contrib[hmx[k]]; t.v[k] = t.v[k] + ldmax; nb_contrib[hmx[k]] = t; }
             if (hmn[k] < 0) lqc[k] = lqc[k] + ldmin;
-            else { Q4<T> t = nb_contrib[hmn[k]]; t.v[k] = t.v[k] + ldmin; nb_contrib[hmn[k]] = t; }
+#pragma unroll
+            for (int j = 0; j < kMaxHe; ++j) {
+                if (h0 + j == hmx[k]) nbc[j][k] = nbc[j][k] + ldmax;
+                if (h0 + j == hmn[k]) nbc[j][k] = nbc[j][k] + ldmin;
+            }
         }
     }
     // Green-Gauss gradient reversed (explicit.cpp:112-171)
@@ -444,7 +454,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
 #pragma unroll
             for (int k = 0; k < 4; ++k) lA = lA - (lgx[k] * gx[k] + lgy[k] * gy[k]) / m.c_area[c];
         }
-        for (int h = h0; h < h1; ++h) {
+#pragma unroll
+        for (int j = 0; j < kMaxHe; ++j) {
+            const int h = h0 + j;
+            if (h >= h1) continue;
             const int nb = m.he_nbr[h];
             if (nb == c) continue;
             const uint32_t ue = m.he_edge[h];
@@ -452,7 +465,6 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
             const Q4<T> geo = m.e_geo[ue & kEdgeMask];
             const T w = geo.v[3];
             const T w1 = T(1) - w;
-            Q4<T> t = nb_contrib[h];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 T lf = sx[k] * geo.v[0] + sy[k] * geo.v[1];
@@ -460,9 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                 lf = lf * geo.v[2];
                 // f = (qi w1 + qj w) len with qi the side-0 cell
                 lqc[k] = lqc[k] + lf * (side ? w : w1);
-                t.v[k] = t.v[k] + lf * (side ? w1 : w);
+                nbc[j][k] = nbc[j][k] + lf * (side ? w1 : w);
             }
-            nb_contrib[h] = t;
             if (GEO) {
                 // gx += +-f nx, gy += +-f ny with f = F len, F = qi (1-w) + qj w
                 const Q4<T> qn = qin[nb];
@@ -489,6 +500,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
             }
         }
     }
+#pragma unroll
+    for (int j = 0; j < kMaxHe; ++j)
+        if (h0 + j < h1) nb_contrib[h0 + j] = Q4<T>{{nbc[j][0], nbc[j][1], nbc[j][2], nbc[j][3]}};
     Q4<T> o;
 #pragma unroll
     for (int k = 0; k < 4; ++k) o.v[k] = lqc[k];

@@ -38,6 +38,7 @@ template <class T> struct alignas(2 * sizeof(T)) G2 { T v[2]; };
 constexpr uint32_t kSideBit = 0x80000000u;
 constexpr uint32_t kBcBit = 0x40000000u;
 constexpr uint32_t kEdgeMask = 0x3fffffffu;
+constexpr int kMaxHe = 4;   // half-edges of a real cell the adjoint takes (triangles and quadrilaterals)
 constexpr int kBlock = 256;        // cells per workgroup = cells per tile
 constexpr int kTileEdges = 1024;   // LDS edge slots per tile
 constexpr int kStageBlock = 512;   // threads of the fused stage pass (one tile per workgroup)

@@ -121,6 +121,7 @@ struct fvh_solver {
     std::vector<uint32_t> h_he_edge;         // device half-edge -> device edge | flags
     std::vector<int32_t> h_edge_ref;         // device edge -> reference edge of its instance
     std::unique_ptr<fvh_mesh> mesh0;         // instance 0's mesh (single-mesh solvers)
+    int32_t max_real_deg = 0;                // most half-edges of any real cell
     // graphs
     bool use_graphs = false;
     std::vector<fvh::StepGraph> graphs;
@@ -922,6 +923,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     }
     for (int32_t d = 0; d < N; ++d) off[d + 1] += off[d];
     s->n_he = off[N];
+    for (int32_t d = 0; d < s->n_real; ++d) s->max_real_deg = std::max(s->max_real_deg, off[d + 1] - off[d]);
     std::vector<int32_t> cursor(off.begin(), off.end() - 1);
     std::vector<int32_t> he_nbr(s->n_he), he_twin(s->n_he), he_bc(s->n_he, -1), he_ref(s->n_he);
     std::vector<uint32_t> he_side(s->n_he);
@@ -1453,6 +1455,11 @@ int32_t node_gradient(fvh_solver* s, const double* wx, const double* wy, double*
 int32_t adjoint_check(fvh_solver* s) {
     if (s->f32) { set_error("the adjoint is FP64 only"); return FVH_E_UNSUPPORTED; }
     if (s->n_inst != 1) { set_error("the adjoint takes a single-mesh solver"); return FVH_E_UNSUPPORTED; }
+    if (s->max_real_deg > kMaxHe) {
+        set_error("the adjoint takes triangles and quadrilaterals (cells of at most " + std::to_string(kMaxHe) +
+                  " edges)");
+        return FVH_E_UNSUPPORTED;
+    }
     const fvh_config& c = s->cfg;
     if (!(c.flags & FVH_FLAG_GLOBAL_DT) && c.dt_kind != FVH_DT_FIXED) {
         set_error("the adjoint needs a global or fixed time step");
