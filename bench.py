@@ -340,6 +340,7 @@ def adjoint(a):
                     "with_node_gradient_over_forward": round(t_adj_nodes / t_fwd, 3),
                     "node_gradient_norm": float(np.linalg.norm(gxy)), "snaps": st["snaps"],
                     "forward_steps_executed": st["forward_steps"], "recorded_steps": st["recorded_steps"],
+                    "library_ms": {"forward_sweep": round(st["forward_ms"], 2), "total": round(st["total_ms"], 2)},
                     "checkpoint_GB": round(st["checkpoint_bytes"] / 1e9, 3),
                     "work_GB": round(st["work_bytes"] / 1e9, 3),
                     "fd_eps_1e-6": {"directions": len(rel6), "median_rel_err": float(np.median(rel6)),

@@ -751,6 +751,12 @@ __global__ __launch_bounds__(kBlock) void k_objective(int n_b, PhysParams pp, co
 }
 
 template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy(const Q4<T>* __restrict__ a, Q4<T>* __restrict__ b, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_fill(Q4<T>* __restrict__ a, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) a[i] = Q4<T>{{T(0), T(0), T(0), T(0)}};
@@ -859,6 +865,12 @@ void launch_objective(hipStream_t s, int n_b, const PhysParams& pp, const int32_
 }
 
 template <class T>
+void launch_copy(hipStream_t s, const void* a, void* b, int n) {
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_copy<T>), dim3(nblocks(n)), dim3(kBlock), 0, s, (const Q4<T>*)a, (Q4<T>*)b, n);
+}
+
+template <class T>
 void launch_fill(hipStream_t s, Q4<T>* a, int n) {
     if (n == 0) return;
     hipLaunchKernelGGL((k_fill<T>), dim3(nblocks(n)), dim3(kBlock), 0, s, a, n);
@@ -881,5 +893,6 @@ template void launch_adj_bounds_init<double>(hipStream_t, const MeshView<double>
 template void launch_objective<double>(hipStream_t, int, const PhysParams&, const int32_t*, const Q4<double>*,
                                        const Q4<double>*, double*, Q4<double>*, double*);
 template void launch_fill<double>(hipStream_t, Q4<double>*, int);
+template void launch_copy<double>(hipStream_t, const void*, void*, int);
 
 }  // namespace fvh

@@ -205,6 +205,9 @@ void launch_objective(hipStream_t s, int n_b, const PhysParams& pp, const int32_
                       const Q4<T>* q, double* terms, Q4<T>* lam, double* p_out);
 template <class T>
 void launch_fill(hipStream_t s, Q4<T>* a, int n);
+// n states from a to b (a kernel: cheaper to issue than the runtime's copy)
+template <class T>
+void launch_copy(hipStream_t s, const void* a, void* b, int n);
 
 // FP64 conserved sums sum_i A_i q_ik over each instance's real cells.
 template <class T>

@@ -1218,14 +1218,14 @@ struct Adjoint {
 
     // k forward steps from state src into state dst (the fast path)
     int32_t advance(const void* src, void* dst, int k) {
-        FVH_HIP(hipMemcpyAsync(s->q[s->cur].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        launch_copy<double>(s->stream, src, s->q[s->cur].p, s->n_total);
         fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
         for (int i = 0; i < k; ++i) {
             int32_t rc = fvh::step<double>(s, false, &o, 0);
             if (rc) return rc;
         }
         fwd_steps += k;
-        FVH_HIP(hipMemcpyAsync(dst, s->q[s->cur].p, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        launch_copy<double>(s->stream, s->q[s->cur].p, dst, s->n_total);
         return FVH_OK;
     }
 
@@ -1296,14 +1296,14 @@ struct Adjoint {
                                   (const double*)s->a_ld.p, n_stage * s->n_rt, lam_q0, ga);
         }
         FVH_HIP(hipGetLastError());
-        FVH_HIP(hipMemcpyAsync(s->a_lam.p, s->a_lam_q0.p, qbytes, hipMemcpyDeviceToDevice, st));
+        launch_copy<double>(st, s->a_lam_q0.p, s->a_lam.p, s->n_total);
         return FVH_OK;
     }
 
     // replay of the step starting from state src, then its reverse; the
     // launch sequence never changes, so it is captured once as a HIP graph
     int32_t step_back(const void* src) {
-        FVH_HIP(hipMemcpyAsync(s->a_qs[0].p, src, qbytes, hipMemcpyDeviceToDevice, s->stream));
+        launch_copy<double>(s->stream, src, s->a_qs[0].p, s->n_total);
         if (s->a_graph_exec && s->a_graph_geo != (ga.he != nullptr)) {
             // captured with the other geometry choice
             FVH_HIP(hipStreamSynchronize(s->stream));
@@ -1625,7 +1625,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     // initial state: bounds applied (explicit.cpp:712), kept in slot 0
     s->cur = 0;
     if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
-    FVH_HIP(hipMemcpyAsync(s->a_ck[0]->p, s->q[s->cur].p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+    launch_copy<double>(s->stream, s->q[s->cur].p, s->a_ck[0]->p, s->n_total);
     // forward to the objective, then seed the adjoint of the final state.
     // With a slot per step the sweep keeps every step's state and each step
     // is replayed once; otherwise the binomial schedule recomputes.
@@ -1638,12 +1638,12 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
         for (int k = 0; k < i0; ++k) {
             if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
             void* dst = k + 1 < i0 ? s->a_ck[k + 1]->p : (n_rec ? A.tapes[0].qs[0] : s->a_tmp.p);
-            FVH_HIP(hipMemcpyAsync(dst, s->q[s->cur].p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+            launch_copy<double>(s->stream, s->q[s->cur].p, dst, s->n_total);
         }
         if (n_rec) {
             // the last steps recorded whole; each record's output is the next one's input
             if (i0 == 0)
-                FVH_HIP(hipMemcpyAsync(A.tapes[0].qs[0], s->a_ck[0]->p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+                launch_copy<double>(s->stream, s->a_ck[0]->p, A.tapes[0].qs[0], s->n_total);
             for (int k = 0; k < n_rec; ++k) {
                 if (k > 0) A.tapes[k].qs[0] = A.tapes[k - 1].qs[A.n_stage];
                 if ((rc = A.replay_body(A.tapes[k]))) return rc;
@@ -1652,7 +1652,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
         }
         A.fwd_steps += n_steps;
         if (n_steps == 0)
-            FVH_HIP(hipMemcpyAsync(s->a_tmp.p, s->a_ck[0]->p, A.qbytes, hipMemcpyDeviceToDevice, s->stream));
+            launch_copy<double>(s->stream, s->a_ck[0]->p, s->a_tmp.p, s->n_total);
     } else if ((rc = A.advance(s->a_ck[0]->p, s->a_tmp.p, n_steps))) {
         return rc;
     }
