@@ -112,7 +112,7 @@ struct HaloView {
     const int32_t* r1_start;  // parallel to cells: a real ring-1 cell's first entry in r1_loc
     const uint16_t* r1_loc;   // local neighbour index per half-edge of real ring-1 cells
     const uint16_t* he_loc;   // per half-edge of a real cell: local neighbour index in its own tile
-    const ushort2* te_loc;    // per tile edge slot: local indices of the edge's two cells
+    const uint16_t* r1_slot;  // parallel to r1_loc: the tile's edge slot of that half-edge (0xffff: none)
     int32_t max_ext, max_g, max_edges;   // largest tile (LDS sizing)
 };
 
@@ -151,7 +151,7 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
 template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
-                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, T* dt,
+                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl);
 template <class T>
 size_t stage_lds_bytes(const HaloView& hv);

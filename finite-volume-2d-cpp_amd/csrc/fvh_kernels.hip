@@ -432,16 +432,18 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
 // the states of the tile and its two neighbour rings in LDS; phase 1
 // computes gradient and limiter of the tile cells and of the real ring-1
 // cells (a ring-1 cell's gradient is recomputed by every tile that borders
-// it: same inputs, same order, same bits); phase 2 evaluates each edge the
-// tile touches once from the face states built on the fly; phase 3 gathers
-// per cell in reference edge order and finishes the stage.  No face or
-// boundary-increment buffers go through HBM.
+// it: same inputs, same order, same bits) and writes the reconstructed face
+// state of each of their half-edges on a tile edge into that edge's slot;
+// phase 2 evaluates each tile edge once from its two face states and leaves
+// the flux in the slot; phase 3 gathers per cell in reference edge order and
+// finishes the stage.  No face state goes through HBM; the boundary
+// increments of the tile's own cells do (bincr), for phase 3.
 template <class T, int PHI, int DT, bool JOINED>
 __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
                                                   int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
                                                   const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
-                                                  Q4<T>* __restrict__ qout, T* __restrict__ dt,
-                                                  const T* __restrict__ dt_inst, T alpha,
+                                                  Q4<T>* __restrict__ qout, Q4<T>* __restrict__ bincr,
+                                                  T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
                                                   double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl) {
     extern __shared__ __align__(32) unsigned char lds_raw[];
     const int lb = xcd_block(tm.n_rt);
@@ -450,75 +452,74 @@ __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm
     if (stopped(ctrl, inst)) return;
     const int c0 = tm.c0[lb], nt = tm.c0[lb + 1] - c0;
     const int4 ex = hv.ext[lb];
-    const int n_g = nt + ex.y, n_r1 = nt + ex.z, n_ext = nt + ex.z + ex.w;
+    const int n_g = nt + ex.y, n_ext = nt + ex.z + ex.w;
     Q4<T>* sq = (Q4<T>*)lds_raw;
-    Q4<T>* sgx = sq + hv.max_ext;
-    Q4<T>* sgy = sgx + hv.max_g;
-    Q4<T>* slm = sgy + hv.max_g;
-    Q4<T>* fle = slm + hv.max_g;
+    Q4<T>* fs = sq + hv.max_ext;   // per tile edge slot: face state of side 0, side 1; then the flux in [0]
     const int32_t* xc = hv.cells + ex.x;
     const T g = T(pp.gamma);
     const T lim0 = lim_on ? T(1) : T(0);
     // phase 0: states of tile, ring 1 and ring 2
     for (int j = threadIdx.x; j < n_ext; j += kStageBlock) sq[j] = ld(qin + (j < nt ? c0 + j : xc[j - nt]));
     __syncthreads();
-    // phase 1: gradients and limiters of the tile and its real ring-1 cells
+    // phase 1: gradients and limiters of the tile and its real ring-1 cells,
+    // and their face states on the tile's edges
     for (int j = threadIdx.x; j < n_g; j += kStageBlock) {
         const bool own = j < nt;
         const int c = own ? c0 + j : xc[j - nt];
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-        const uint16_t* loc = own ? hv.he_loc + h0 : hv.r1_loc + hv.r1_start[ex.x + j - nt];
+        const int r1 = own ? 0 : hv.r1_start[ex.x + j - nt];
+        const uint16_t* loc = own ? hv.he_loc + h0 : hv.r1_loc + r1;
         T gx[4], gy[4], lim[4], dtc = T(0);
         auto nbr = [&](int h, bool& self) {
             const int l = loc[h - h0];
             self = l == j;
             return sq[l];
         };
-        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, sq[j], m.c_c[c], nbr, gx, gy, lim, dtc);
+        const Q4<T> qc = sq[j];
+        const G2<T> cc = m.c_c[c];
+        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
         if (DT && own) dt[c] = dtc;
-        Q4<T> a, b, d;
+        for (int h = h0; h < h1; ++h) {
+            const int slot = own ? (int)m.he_slot[h] : (int)hv.r1_slot[r1 + (h - h0)];
+            if (slot == 0xffff) continue;
+            const uint32_t ue = m.he_edge[h];
+            const int side = (ue & kSideBit) ? 1 : 0;
+            const G2<T> ec = m.e_c[ue & kEdgeMask];
+            const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
+            Q4<T> f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
-        sgx[j] = a;
-        sgy[j] = b;
-        slm[j] = d;
+            for (int k = 0; k < 4; ++k) f.v[k] = qc.v[k] + inc.v[k];
+            if (pp.positivity && !physical(g, f.v)) f = qc;
+            fs[2 * slot + side] = f;
+            if (own && (ue & kBcBit)) {
+                // the ghost's face: its state with a zero gradient and the reset limiter
+                const int nb = m.he_nbr[h];
+                bincr[nb - m.n_real] = inc;
+                const T z[4] = {T(0), T(0), T(0), T(0)};
+                const T l0[4] = {lim0, lim0, lim0, lim0};
+                const Q4<T> ginc = face_inc(z, z, l0, ec, m.c_c[nb]);
+                const Q4<T> qg = sq[loc[h - h0]];
+                Q4<T> fg;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) fg.v[k] = qg.v[k] + ginc.v[k];
+                if (pp.positivity && !physical(g, fg.v)) fg = qg;
+                fs[2 * slot + 1 - side] = fg;
+            }
+        }
     }
     __syncthreads();
-    // phase 2: each edge of the tile once, from reconstructed face states
+    // phase 2: each edge of the tile once, flux times length left in the slot
     const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
     for (int k = e0 + (int)threadIdx.x; k < e1; k += kStageBlock) {
-        const int e = tv.edge[k];
-        const Q4<T> geo = m.e_geo[e];
-        const G2<T> ec = m.e_c[e];
-        const ushort2 lc = hv.te_loc[k];
-        const int2 cl = tv.cell[k];
-        Q4<T> f2[2];
-#pragma unroll
-        for (int sd = 0; sd < 2; ++sd) {
-            const int l = sd ? lc.y : lc.x;
-            const G2<T> cc = m.c_c[sd ? cl.y : cl.x];
-            T gx[4], gy[4], lim[4];
-            if (l < n_g) {
-                const Q4<T> a = sgx[l], b = sgy[l], d = slm[l];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) { gx[j] = a.v[j]; gy[j] = b.v[j]; lim[j] = d.v[j]; }
-            } else {
-                // boundary ghost: zero gradient, limiter at its reset value
-#pragma unroll
-                for (int j = 0; j < 4; ++j) { gx[j] = T(0); gy[j] = T(0); lim[j] = lim0; }
-            }
-            const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
-            const Q4<T> qs = sq[l];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) f2[sd].v[j] = qs.v[j] + inc.v[j];
-            if (pp.positivity && !physical(g, f2[sd].v)) f2[sd] = qs;
-        }
+        const Q4<T> geo = m.e_geo[tv.edge[k]];
+        const int sl = k - e0;
+        const Q4<T> a = fs[2 * sl], b = fs[2 * sl + 1];
         T f[4];
-        roe_flux<T, JOINED>(g, f, f2[0].v, f2[1].v, geo.v[0], geo.v[1]);
+        roe_flux<T, JOINED>(g, f, a.v, b.v, geo.v[0], geo.v[1]);
         Q4<T> r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
-        fle[k - e0] = r;
+        fs[2 * sl] = r;
     }
     __syncthreads();
     // phase 3: ordered gather, RK update, boundary update
@@ -531,25 +532,17 @@ __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm
         T qt[4] = {T(0), T(0), T(0), T(0)};
         for (int h = h0; h < h1; ++h) {
             const bool side = (m.he_edge[h] & kSideBit) != 0;
-            const Q4<T> r = fle[m.he_slot[h]];
+            const Q4<T> r = fs[2 * (int)m.he_slot[h]];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const T t = r.v[k] / A;
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        const G2<T> cc = m.c_c[c];
-        auto inc = [&](int h, int) {
-            T gx[4], gy[4], lim[4];
-            const Q4<T> a = sgx[j], b = sgy[j], d = slm[j];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) { gx[k] = a.v[k]; gy[k] = b.v[k]; lim[k] = d.v[k]; }
-            return face_inc(gx, gy, lim, m.e_c[m.he_edge[h] & kEdgeMask], cc);
-        };
+        auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
         finish_cell<T, true>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                              res_blk != nullptr, rs, nullptr);
     }
-    (void)n_r1;
     if (res_blk) {
         block_sum4<kStageBlock>(rs);
         if (threadIdx.x == 0)
@@ -756,14 +749,14 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
 
 template <class T>
 size_t stage_lds_bytes(const HaloView& hv) {
-    return (size_t)(hv.max_ext + 3 * hv.max_g + hv.max_edges) * sizeof(Q4<T>);
+    return (size_t)(hv.max_ext + 2 * hv.max_edges) * sizeof(Q4<T>);
 }
 
 template <class T, int PHI, int DT, bool J>
 bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
-                    const Q4<T>* q0, Q4<T>* qout, T* dt, const T* dt_inst, T alpha, double* res_blk,
-                    const Ctrl* ctrl) {
+                    const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
+                    double* res_blk, const Ctrl* ctrl) {
     const size_t lds = stage_lds_bytes<T>(hv);
     static bool attr_set = false;
     if (!attr_set) {
@@ -774,19 +767,19 @@ bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
     }
     const int nb = xcd_grid_blocks(tm.n_rt);
     hipLaunchKernelGGL((k_stage<T, PHI, DT, J>), dim3(nb), dim3(kStageBlock), lds, s, m, tm, tv, hv, pp, (int)grad,
-                       (int)lim, bcs, qin, q0, qout, dt, dt_inst, alpha, res_blk, ctrl);
+                       (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl);
     return true;
 }
 
 template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
-                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, T* dt,
+                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl) {
     if (tm.n_rt == 0) return true;
     if (stage_lds_bytes<T>(hv) > 160 * 1024) return false;
-#define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, dt, dt_inst, \
-                                                  alpha, res_blk, ctrl)
+#define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt, \
+                                                  dt_inst, alpha, res_blk, ctrl)
 #define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
 #define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
     switch (phi_kind) {
@@ -850,8 +843,8 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
                                  const Q4<T>*, double*, double*);                                            \
     template bool launch_stage<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,          \
                                   const HaloView&, const PhysParams&, int, bool, bool, bool, int,            \
-                                  const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, T*, const T*, T,     \
-                                  double*, const Ctrl*);                                                     \
+                                  const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, Q4<T>*, T*,        \
+                                  const T*, T, double*, const Ctrl*);                                                     \
     template size_t stage_lds_bytes<T>(const HaloView&);
 FVH_INST(double)
 FVH_INST(float)

@@ -95,7 +95,7 @@ struct fvh_solver {
     bool use_tiles = false;
     // fused stage pass: tile halos
     bool fused = false;
-    fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_he_loc, x_te_loc;
+    fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_r1_slot, x_he_loc;
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
@@ -198,7 +198,7 @@ HaloView halo_view(fvh_solver* s) {
     h.r1_start = (const int32_t*)s->x_r1_start.p;
     h.r1_loc = (const uint16_t*)s->x_r1_loc.p;
     h.he_loc = (const uint16_t*)s->x_he_loc.p;
-    h.te_loc = (const ushort2*)s->x_te_loc.p;
+    h.r1_slot = (const uint16_t*)s->x_r1_slot.p;
     h.max_ext = s->x_max_ext;
     h.max_g = s->x_max_g;
     h.max_edges = s->x_max_edges;
@@ -397,7 +397,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
             Bracket b(s, K_STAGE);
             const bool last = k == n_stage - 1;
             launch_stage<T>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim,
-                            (k == 0 && !gdt) ? c.dt_kind : -1, bcs, qin, q0, out, dt, gdt ? dtg : nullptr,
+                            (k == 0 && !gdt) ? c.dt_kind : -1, bcs, qin, q0, out, (Q4<T>*)s->bincr.p, dt,
+                            gdt ? dtg : nullptr,
                             T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc);
             qin = out;
             out = (out == qa) ? qb : qa;
@@ -692,19 +693,19 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::
 struct HaloHost {
     std::vector<int4> ext;
     std::vector<int32_t> cells, r1_start;
-    std::vector<uint16_t> r1_loc, he_loc, te_loc;
+    std::vector<uint16_t> r1_loc, r1_slot, he_loc;
     int32_t max_ext = 0, max_g = 0, max_edges = 0;
 };
 
 bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::vector<int32_t>& off,
                 const std::vector<int32_t>& he_nbr, const std::vector<int32_t>& t_off,
-                const std::vector<int32_t>& t_cell, HaloHost& hh) {
+                const std::vector<int32_t>& t_he, HaloHost& hh) {
     const int32_t N = s->n_total;
     std::vector<int32_t> mark(N, -1);
     std::vector<int32_t> r1r, r1g, r2;
+    std::vector<int32_t> slot_of_he(s->n_he, -1);
     hh.ext.resize(s->n_rt);
     hh.he_loc.assign(s->n_he, 0);
-    hh.te_loc.assign(t_cell.size(), 0);
     for (int32_t t = 0; t < s->n_rt; ++t) {
         const int32_t c0 = tile_c0[t], c1 = tile_c0[t + 1], nt = c1 - c0;
         for (int32_t c = c0; c < c1; ++c) mark[c] = c - c0;
@@ -736,17 +737,26 @@ bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::v
             for (int32_t h = off[c]; h < off[c + 1]; ++h) hh.he_loc[h] = (uint16_t)mark[he_nbr[h]];
         const int32_t base = (int32_t)hh.cells.size();
         hh.ext[t] = make_int4(base, n1r, n1, (int32_t)r2.size());
+        // the tile's edge slot of each half-edge on one of its edges
+        for (int32_t k = t_off[t]; k < t_off[t + 1]; ++k) {
+            slot_of_he[t_he[2 * (size_t)k]] = k - t_off[t];
+            slot_of_he[t_he[2 * (size_t)k + 1]] = k - t_off[t];
+        }
         for (int32_t c : r1r) {
             hh.cells.push_back(c);
             hh.r1_start.push_back((int32_t)hh.r1_loc.size());
-            for (int32_t h = off[c]; h < off[c + 1]; ++h) hh.r1_loc.push_back((uint16_t)mark[he_nbr[h]]);
+            for (int32_t h = off[c]; h < off[c + 1]; ++h) {
+                hh.r1_loc.push_back((uint16_t)mark[he_nbr[h]]);
+                hh.r1_slot.push_back(slot_of_he[h] < 0 ? (uint16_t)0xffff : (uint16_t)slot_of_he[h]);
+            }
         }
         for (int32_t c : r1g) { hh.cells.push_back(c); hh.r1_start.push_back(0); }
         for (int32_t c : r2) { hh.cells.push_back(c); hh.r1_start.push_back(0); }
         for (int32_t k = t_off[t]; k < t_off[t + 1]; ++k) {
-            hh.te_loc[2 * (size_t)k] = (uint16_t)mark[t_cell[2 * (size_t)k]];
-            hh.te_loc[2 * (size_t)k + 1] = (uint16_t)mark[t_cell[2 * (size_t)k + 1]];
+            slot_of_he[t_he[2 * (size_t)k]] = -1;
+            slot_of_he[t_he[2 * (size_t)k + 1]] = -1;
         }
+        if (t_off[t + 1] - t_off[t] > 65535) return false;
         hh.max_ext = std::max(hh.max_ext, n_ext);
         hh.max_g = std::max(hh.max_g, nt + n1r);
         hh.max_edges = std::max(hh.max_edges, t_off[t + 1] - t_off[t]);
@@ -755,7 +765,7 @@ bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::v
         for (int32_t c : r1g) mark[c] = -1;
         for (int32_t c : r2) mark[c] = -1;
     }
-    if (hh.r1_loc.empty()) hh.r1_loc.push_back(0);
+    if (hh.r1_loc.empty()) { hh.r1_loc.push_back(0); hh.r1_slot.push_back(0xffff); }
     if (hh.cells.empty()) { hh.cells.push_back(0); hh.r1_start.push_back(0); }
     return true;
 }
@@ -1031,9 +1041,9 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     const bool lin_on = (c.flags & FVH_FLAG_LINEAR) != 0;
     s->fused = false;
     if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
-        lin_on && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_cell, hh)) {
+        lin_on && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
         const size_t qb = s->f32 ? 16 : 32;
-        const size_t lds = (size_t)(hh.max_ext + 3 * hh.max_g + hh.max_edges) * qb;
+        const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * qb;
         s->fused = lds <= 160 * 1024;
     }
     if (c.flux_mode == FVH_FLUX_MODE_FUSED && !s->fused && s->n_real > 0) {
@@ -1049,7 +1059,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         if ((rc = upload(s->x_r1_start, hh.r1_start.data(), hh.r1_start.size() * 4))) return rc;
         if ((rc = upload(s->x_r1_loc, hh.r1_loc.data(), hh.r1_loc.size() * 2))) return rc;
         if ((rc = upload(s->x_he_loc, hh.he_loc.data(), hh.he_loc.size() * 2))) return rc;
-        if ((rc = upload(s->x_te_loc, hh.te_loc.data(), hh.te_loc.size() * 2))) return rc;
+        if ((rc = upload(s->x_r1_slot, hh.r1_slot.data(), hh.r1_slot.size() * 2))) return rc;
     }
     if ((rc = upload(s->he_off, off.data(), off.size() * 4))) return rc;
     if ((rc = upload(s->he_nbr, he_nbr.data(), he_nbr.size() * 4))) return rc;
