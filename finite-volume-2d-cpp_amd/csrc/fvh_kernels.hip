@@ -158,6 +158,23 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
         const T K3a = Ka * Ka * Ka;
         const T yt = T(pp.phi_yt);
         const T tol = T(1e-15);
+        // the smoothing sigma depends on the cell and the variable only
+        // (explicit.cpp:226-238): once per variable, the same expressions
+        T sigs[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const T dmax = qmx[k] - qc.v[k];
+            const T dmin = qmn[k] - qc.v[k];
+            const T spread = (dmax - dmin) * (dmax - dmin);
+            if (spread <= K3a) {
+                sigs[k] = T(1);
+            } else if (spread <= T(2) * K3a) {
+                const T y = (spread / K3a - T(1));
+                sigs[k] = T(2) * y * y * y - T(3) * y * y + T(1);
+            } else {
+                sigs[k] = T(0);
+            }
+        }
         for (int h = h0; h < h1; ++h) {
             const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
             const T dx = ec.v[0] - cc.v[0];
@@ -167,16 +184,7 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
                 const T dqg = gx[k] * dx + gy[k] * dy;
                 const T dmax = qmx[k] - qc.v[k];
                 const T dmin = qmn[k] - qc.v[k];
-                const T spread = (dmax - dmin) * (dmax - dmin);
-                T sig;
-                if (spread <= K3a) {
-                    sig = T(1);
-                } else if (spread <= T(2) * K3a) {
-                    const T y = (spread / K3a - T(1));
-                    sig = T(2) * y * y * y - T(3) * y * y + T(1);
-                } else {
-                    sig = T(0);
-                }
+                const T sig = sigs[k];
                 T l = T(1);
                 if (sig < T(1)) {
                     if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
