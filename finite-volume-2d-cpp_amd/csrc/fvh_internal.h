@@ -38,6 +38,7 @@ int32_t build_mesh(fvh_mesh& m, int32_t n_nodes, const double* x, const double* 
                    const std::vector<std::string>& names);
 
 int32_t read_msh(const char* path, fvh_mesh& m);
+int32_t read_su2(const char* path, fvh_mesh& m);
 
 // Adjoints of the quantities derived from the node coordinates: per edge
 // (normal, length, Green-Gauss weight, centre) and per cell, ghosts included

@@ -69,6 +69,115 @@ class EdgeTable {
 
 }  // namespace
 
+// SU2 ASCII, 2-D: NDIME, NELEM (element lines "type nodes... [index]", 5 =
+// triangle, 9 = quadrilateral), NPOIN ("x y [index]"), NMARK with
+// MARKER_TAG / MARKER_ELEMS and line elements "3 a b".  Cells, nodes and
+// boundary elements keep file order; marker tags become the physical names
+// (the reference converts such files to MSH 4.1 with gmsh first).
+int32_t read_su2(const char* path, fvh_mesh& out) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { set_error(std::string("cannot open ") + path); return FVH_E_IO; }
+    std::vector<std::string> lines;
+    {
+        std::string cur;
+        int ch;
+        while ((ch = std::fgetc(f)) != EOF) {
+            if (ch == '\n') { lines.push_back(cur); cur.clear(); }
+            else if (ch != '\r') cur.push_back((char)ch);
+        }
+        if (!cur.empty()) lines.push_back(cur);
+        std::fclose(f);
+    }
+    // strip comments and blank lines
+    std::vector<std::string> t;
+    for (auto& l : lines) {
+        const size_t p = l.find('%');
+        std::string s2 = p == std::string::npos ? l : l.substr(0, p);
+        const size_t a = s2.find_first_not_of(" \t"), b = s2.find_last_not_of(" \t");
+        if (a != std::string::npos) t.push_back(s2.substr(a, b - a + 1));
+    }
+    auto fail = [&](const std::string& msg) { set_error("SU2 " + std::string(path) + ": " + msg); return FVH_E_MESH; };
+    auto is_key = [](const std::string& l, const char* k) {
+        std::string u;
+        for (char c : l) {
+            if (c == '=') break;
+            if (c != ' ' && c != '\t') u.push_back((char)std::toupper((unsigned char)c));
+        }
+        return u == k && l.find('=') != std::string::npos;
+    };
+    auto value = [](const std::string& l) {
+        std::string v = l.substr(l.find('=') + 1);
+        const size_t a = v.find_first_not_of(" \t");
+        return a == std::string::npos ? std::string() : v.substr(a);
+    };
+    auto fields = [](const std::string& l) {
+        std::vector<std::string> r;
+        size_t i = 0;
+        while (i < l.size()) {
+            while (i < l.size() && (l[i] == ' ' || l[i] == '\t')) ++i;
+            const size_t j = i;
+            while (i < l.size() && l[i] != ' ' && l[i] != '\t') ++i;
+            if (i > j) r.push_back(l.substr(j, i - j));
+        }
+        return r;
+    };
+    std::vector<double> xs, ys;
+    std::vector<int32_t> cells, bnd, tags;
+    std::vector<std::string> names;
+    size_t i = 0;
+    while (i < t.size()) {
+        const std::string& l = t[i];
+        if (is_key(l, "NDIME")) {
+            if (std::stoi(value(l)) != 2) return fail("only 2-D meshes");
+            ++i;
+        } else if (is_key(l, "NELEM")) {
+            const long n = std::stol(value(l));
+            if (n < 0 || i + 1 + (size_t)n > t.size()) return fail("truncated NELEM block");
+            for (long k = 0; k < n; ++k) {
+                const auto fs = fields(t[i + 1 + k]);
+                const int kind = fs.empty() ? -1 : std::stoi(fs[0]);
+                if (kind == 5 && fs.size() >= 4) {
+                    cells.insert(cells.end(), {std::stoi(fs[1]), std::stoi(fs[2]), std::stoi(fs[3]), -1});
+                } else if (kind == 9 && fs.size() >= 5) {
+                    cells.insert(cells.end(), {std::stoi(fs[1]), std::stoi(fs[2]), std::stoi(fs[3]), std::stoi(fs[4])});
+                } else {
+                    return fail("element type " + std::to_string(kind) + " is not a triangle or quadrilateral");
+                }
+            }
+            i += 1 + n;
+        } else if (is_key(l, "NPOIN")) {
+            const long n = std::stol(fields(value(l))[0]);
+            if (n < 0 || i + 1 + (size_t)n > t.size()) return fail("truncated NPOIN block");
+            for (long k = 0; k < n; ++k) {
+                const auto fs = fields(t[i + 1 + k]);
+                if (fs.size() < 2) return fail("bad point line");
+                xs.push_back(std::strtod(fs[0].c_str(), nullptr));
+                ys.push_back(std::strtod(fs[1].c_str(), nullptr));
+            }
+            i += 1 + n;
+        } else if (is_key(l, "MARKER_TAG")) {
+            names.push_back(value(l));
+            if (i + 1 >= t.size() || !is_key(t[i + 1], "MARKER_ELEMS")) return fail("MARKER_TAG without MARKER_ELEMS");
+            const long n = std::stol(value(t[i + 1]));
+            if (n < 0 || i + 2 + (size_t)n > t.size()) return fail("truncated marker block");
+            for (long k = 0; k < n; ++k) {
+                const auto fs = fields(t[i + 2 + k]);
+                if (fs.size() < 3 || std::stoi(fs[0]) != 3) return fail("marker element is not a line");
+                bnd.push_back(std::stoi(fs[1]));
+                bnd.push_back(std::stoi(fs[2]));
+                tags.push_back((int32_t)names.size() - 1);
+            }
+            i += 2 + n;
+        } else {
+            ++i;
+        }
+    }
+    for (int32_t v : cells)
+        if (v >= (int32_t)xs.size()) return fail("element node out of range");
+    return build_mesh(out, (int32_t)xs.size(), xs.data(), ys.data(), (int32_t)(cells.size() / 4), cells.data(),
+                      (int32_t)(bnd.size() / 2), bnd.data(), tags.data(), names);
+}
+
 void GeoBar::resize(int32_t n_edges, int32_t n_cells) {
     for (auto* v : {&e_nx, &e_ny, &e_len, &e_w, &e_cx, &e_cy}) v->assign(n_edges, 0.0);
     for (auto* v : {&c_area, &c_cx, &c_cy}) v->assign(n_cells, 0.0);
@@ -678,6 +787,20 @@ extern "C" {
 
 const char* fvh_last_error(void) { return fvh::last_error(); }
 int32_t fvh_abi_version(void) { return FVH_ABI_VERSION; }
+
+int32_t fvh_mesh_read_su2(const char* path, fvh_mesh** out) {
+    if (!path || !out) { fvh::set_error("null argument"); return FVH_E_ARG; }
+    try {
+        fvh_mesh* m = new fvh_mesh();
+        int32_t rc = fvh::read_su2(path, *m);
+        if (rc != FVH_OK) { delete m; return rc; }
+        *out = m;
+        return FVH_OK;
+    } catch (const std::exception& e) {
+        fvh::set_error(e.what());
+        return FVH_E_ARG;
+    }
+}
 
 int32_t fvh_mesh_read_msh(const char* path, fvh_mesh** out) {
     if (!path || !out) { fvh::set_error("null argument"); return FVH_E_ARG; }

@@ -61,7 +61,7 @@ class Status(C.Structure):
 
 # every symbol include/fvh.h declares
 EXPORTS = [
-    "fvh_last_error", "fvh_abi_version", "fvh_mesh_read_msh", "fvh_mesh_from_arrays",
+    "fvh_last_error", "fvh_abi_version", "fvh_mesh_read_msh", "fvh_mesh_read_su2", "fvh_mesh_from_arrays",
     "fvh_mesh_free", "fvh_mesh_counts", "fvh_mesh_export", "fvh_mesh_name",
     "fvh_config_default", "fvh_solver_create", "fvh_solver_free", "fvh_solver_set_state",
     "fvh_solver_get_state", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
@@ -92,6 +92,7 @@ def load(path: str = LIB_PATH):
     L.fvh_last_error.restype = C.c_char_p
     L.fvh_abi_version.restype = C.c_int32
     L.fvh_mesh_read_msh.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.fvh_mesh_read_su2.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.fvh_mesh_from_arrays.argtypes = [C.c_int32, dp, dp, C.c_int32, ip, C.c_int32, ip, ip, C.c_int32,
                                        C.POINTER(C.c_char_p), C.POINTER(vp)]
     L.fvh_mesh_free.argtypes = [vp]

@@ -321,6 +321,90 @@ def write_msh(path: str, m: MeshArrays) -> None:
 # NACA 0012 O-mesh: quad boundary band + triangulated rings to the farfield
 # ---------------------------------------------------------------------------
 
+# ---------------------------------------------------------------------------
+# SU2 ASCII meshes (2-D): NDIME, NELEM with element lines "type nodes... [index]"
+# (5 triangle, 9 quadrilateral), NPOIN with "x y [index]", NMARK with
+# MARKER_TAG / MARKER_ELEMS and line elements "3 a b".  Cells, nodes and
+# boundary elements keep file order; marker tags are the physical names.
+# ---------------------------------------------------------------------------
+
+def read_su2(path: str) -> MeshArrays:
+    with open(path) as f:
+        tok = [ln.split("%")[0].strip() for ln in f]
+    tok = [t for t in tok if t]
+    i = 0
+    cells, x, y, bnd, tags, names = [], [], [], [], [], []
+
+    def key(t, k):
+        return t.replace(" ", "").upper().startswith(k + "=")
+
+    def val(t):
+        return t.split("=", 1)[1].strip()
+    while i < len(tok):
+        t = tok[i]
+        if key(t, "NDIME"):
+            if int(val(t)) != 2:
+                raise ValueError("SU2: only 2-D meshes")
+            i += 1
+        elif key(t, "NELEM"):
+            n = int(val(t))
+            for k in range(n):
+                f = tok[i + 1 + k].split()
+                kind = int(f[0])
+                if kind == 5:
+                    cells.append([int(f[1]), int(f[2]), int(f[3]), -1])
+                elif kind == 9:
+                    cells.append([int(f[1]), int(f[2]), int(f[3]), int(f[4])])
+                else:
+                    raise ValueError("SU2: element type %d is not a triangle or quadrilateral" % kind)
+            i += 1 + n
+        elif key(t, "NPOIN"):
+            n = int(val(t).split()[0])
+            for k in range(n):
+                f = tok[i + 1 + k].split()
+                x.append(float(f[0]))
+                y.append(float(f[1]))
+            i += 1 + n
+        elif key(t, "NMARK"):
+            i += 1
+        elif key(t, "MARKER_TAG"):
+            names.append(val(t))
+            if not key(tok[i + 1], "MARKER_ELEMS"):
+                raise ValueError("SU2: MARKER_TAG without MARKER_ELEMS")
+            n = int(val(tok[i + 1]))
+            for k in range(n):
+                f = tok[i + 2 + k].split()
+                if int(f[0]) != 3:
+                    raise ValueError("SU2: marker element type %s is not a line" % f[0])
+                bnd.append([int(f[1]), int(f[2])])
+                tags.append(len(names) - 1)
+            i += 2 + n
+        else:
+            i += 1
+    return MeshArrays(np.array(x, dtype=np.float64), np.array(y, dtype=np.float64),
+                      np.array(cells, dtype=np.int32).reshape(-1, 4), np.array(bnd, dtype=np.int32).reshape(-1, 2),
+                      np.array(tags, dtype=np.int32), names)
+
+
+def write_su2(path: str, m: MeshArrays) -> None:
+    """The inverse of read_su2 (full precision coordinates)."""
+    lines = ["NDIME= 2", "NELEM= %d" % m.cells.shape[0]]
+    for k, c in enumerate(m.cells):
+        lines.append(("5 %d %d %d %d" % (c[0], c[1], c[2], k)) if c[3] < 0 else
+                     ("9 %d %d %d %d %d" % (c[0], c[1], c[2], c[3], k)))
+    lines.append("NPOIN= %d" % m.x.size)
+    for k in range(m.x.size):
+        lines.append("%s %s %d" % (repr(float(m.x[k])), repr(float(m.y[k])), k))
+    lines.append("NMARK= %d" % len(m.names))
+    for t, nm in enumerate(m.names):
+        sel = np.nonzero(np.asarray(m.bnd_tag) == t)[0]
+        lines.append("MARKER_TAG= %s" % nm)
+        lines.append("MARKER_ELEMS= %d" % sel.size)
+        lines += ["3 %d %d" % (m.bnd[b, 0], m.bnd[b, 1]) for b in sel]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def naca4_thickness(x, t=0.12):
     """Half thickness of a symmetric NACA 4-digit section, closed trailing edge."""
     return 5.0 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1036 * x ** 4)

@@ -41,6 +41,14 @@ class Mesh:
         return cls(h)
 
     @classmethod
+    def read_su2(cls, path: str) -> "Mesh":
+        """A 2-D SU2 ASCII mesh (fvh_mesh_read_su2); marker tags become the
+        physical names."""
+        h = C.c_void_p()
+        check(_lib.load().fvh_mesh_read_su2(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_arrays(cls, m) -> "Mesh":
         """From a meshgen.MeshArrays (file-order arrays)."""
         L = _lib.load()

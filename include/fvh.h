@@ -138,6 +138,13 @@ int32_t fvh_abi_version(void);
 /* Read a gmsh MSH 4.1 ASCII file (the subset of mesh.cpp:264-617). */
 int32_t fvh_mesh_read_msh(const char* path, fvh_mesh** out);
 
+/* A 2-D SU2 ASCII mesh (triangles and quadrilaterals, line markers), as the
+ * reference's examples/naca/*.su2; no counterpart in the reference, which
+ * converts such files to MSH 4.1 with gmsh (examples/naca/naca.geo:1-5).
+ * Cells, nodes and marker elements keep file order; marker tags are the
+ * physical names bound to boundary kinds. */
+int32_t fvh_mesh_read_su2(const char* path, fvh_mesh** out);
+
 /* Same mesh from file-order arrays: x,y per node; cells 4 node indices each
  * (0-based, the 4th -1 for a triangle); bnd 2 node indices per boundary
  * element with bnd_tag indexing names[]. */

@@ -36,12 +36,18 @@ from helpers import airfoil_as_channel, airfoil_bump_state, random_state  # noqa
 
 KIND_NAME = {"mirror": "wall", "copy": "transmissive", "inlet_outlet": "inlet", "farfield": "farfield"}
 
+# the reference's own example mesh (4,096 quadrilaterals around a NACA 0012,
+# farfield at R = 149), read from the reference tree when the fixtures are made
+REF_SU2 = "/root/reference/examples/naca/NACA_M.su2"
+
 FIX_MESHES = {
     "mixed": lambda: meshgen.mixed_channel(nx=12, ny=6),
     "sod": lambda: meshgen.sod_tube(nx=24, ny=3),
     "step": lambda: meshgen.forward_step(n_per_unit=10),
     "naca": lambda: airfoil_as_channel(meshgen.naca0012_sized(1500)),
+    "su2m": lambda: airfoil_as_channel(meshgen.read_su2(REF_SU2)),
 }
+AIRFOIL_MESHES = ("naca", "su2m")
 
 FS = ("inlet_outlet", 1.4, 3.0, 0.0, 1.0)
 NC = ("farfield", 1.4, 0.8, 0.0175, 1.0)
@@ -78,9 +84,9 @@ def renamed(m, cfg):
 
 def run_state(mesh_name, om):
     """Initial state of the run fixtures: seeded random cells on the channel
-    meshes; on the airfoil mesh (where the local-dt limiter case would blow
+    meshes; on the airfoil meshes (where the local-dt limiter case would blow
     up from cell-by-cell noise) the naca freestream with a smooth bump."""
-    if mesh_name != "naca":
+    if mesh_name not in AIRFOIL_MESHES:
         return random_state(om.n_cells, seed=12, mach=0.3)
     return airfoil_bump_state(om.cell_cx, om.cell_cy)
 

@@ -1,4 +1,6 @@
 """Shared test helpers: seeded physical states, mesh/config cases, comparisons."""
+import os
+
 import numpy as np
 
 from fvhyper_amd import meshgen
@@ -86,9 +88,18 @@ def airfoil_bump_state(cx, cy, gamma: float = 1.4) -> np.ndarray:
     return np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], 1).ravel()
 
 
+def su2_fixture_mesh():
+    """The reference's examples/naca/NACA_M.su2 (4,096 quadrilaterals), as
+    the mesh arrays the golden fixtures keep (tests/golden/mesh__su2m.npz)."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mesh__su2m.npz"))
+    return meshgen.MeshArrays(d["x"], d["y"], d["cells"], d["bnd"], d["bnd_tag"], [str(s) for s in d["names"]])
+
+
 MESHES = {
     "mixed": lambda: meshgen.mixed_channel(nx=12, ny=6),
     "sod": lambda: meshgen.sod_tube(nx=40, ny=4),
     "step": lambda: meshgen.forward_step(n_per_unit=20),
     "naca": lambda: airfoil_as_channel(meshgen.naca0012_sized(3000)),
+    "su2m": su2_fixture_mesh,
 }
+AIRFOIL_MESHES = ("naca", "su2m")

@@ -7,7 +7,7 @@ import pytest
 
 import oracle as O
 from fvhyper_amd import workloads
-from helpers import CONFIGS, MESHES, airfoil_bump_state, bits_equal, random_state
+from helpers import AIRFOIL_MESHES, CONFIGS, MESHES, airfoil_bump_state, bits_equal, random_state
 
 
 @pytest.mark.parametrize("mesh_name", sorted(MESHES))
@@ -16,7 +16,7 @@ def test_template_restatement_is_the_oracle(mesh_name, cfg_name):
     m = MESHES[mesh_name]()
     om = O.Mesh(m)
     cfg = CONFIGS[cfg_name]
-    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name == "naca" else \
+    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name in AIRFOIL_MESHES else \
         random_state(om.n_cells, seed=505, mach=0.3)
     qo, *_ = om.run(cfg, q0, 4)
     qc, _ = O.cstep_run(om, cfg, q0, 4)

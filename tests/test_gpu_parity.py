@@ -5,7 +5,7 @@ import pytest
 
 import oracle as O
 from fvhyper_amd import Mesh, Solver, meshgen, workloads
-from helpers import CONFIGS, MESHES, airfoil_bump_state, bits_equal, random_state, rel_err, same_values
+from helpers import AIRFOIL_MESHES, CONFIGS, MESHES, airfoil_bump_state, bits_equal, random_state, rel_err, same_values
 
 pytestmark = pytest.mark.gpu
 
@@ -43,7 +43,7 @@ def test_run_bit_exact(mesh_name, cfg_name, order, flux_mode):
     if flux_mode == "fused" and not cfg.get("lin"):
         pytest.skip("the fused stage pass exists for linear reconstruction only")
     m, om, pm, s = _pair(mesh_name, cfg, renumber=order, flux_mode=flux_mode)
-    if mesh_name == "naca":
+    if mesh_name in AIRFOIL_MESHES:
         q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
     else:
         q0 = random_state(om.n_cells, seed=202, mach=0.3)
@@ -153,7 +153,7 @@ def test_fp32_state_against_fp64_oracle(mesh_name, cfg_name):
     m, om, pm, s = _pair(mesh_name, cfg, precision="f32")
     if mesh_name == "sod":
         q0 = workloads.initial_state(om, workloads.sod_ic())
-    elif mesh_name == "naca":
+    elif mesh_name in AIRFOIL_MESHES:
         q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
     else:
         q0 = random_state(om.n_cells, seed=303, mach=0.3)
@@ -213,7 +213,7 @@ def test_tile_size_128_bit_exact(mesh_name, cfg_name, flux_mode):
     m = MESHES[mesh_name]()
     om = O.Mesh(m)
     s = Solver(Mesh.from_arrays(m), cfg, flux_mode=flux_mode, tile_cells=128)
-    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name == "naca" else \
+    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name in AIRFOIL_MESHES else \
         random_state(om.n_cells, seed=404, mach=0.3)
     s.set_state(q0)
     s.run(max_step=8)

@@ -253,3 +253,48 @@ def test_tile_order_is_a_compact_deterministic_partition():
         ids = order[256 * t: 256 * (t + 1)]
         assert (np.ptp(cx[ids]) + np.ptp(cy[ids])) < 0.6
         assert np.all(np.diff(ids) > 0)   # ascending reference index inside a tile
+
+
+SU2_MEMBERS = ["nodesX", "nodesY", "edgesCells", "edgesNodes", "edgesLengths", "edgesNormalsX", "edgesNormalsY",
+               "cellsAreas", "cellsCentersX", "cellsCentersY", "boundaryEdges"]
+
+
+def test_su2_reader_matches_the_fixture_arrays(tmp_path):
+    """The C ABI's SU2 reader and the Python one on an SU2 file written from
+    the reference's NACA_M.su2 mesh arrays: same file-order arrays, same
+    mesh members as building from those arrays."""
+    from helpers import su2_fixture_mesh
+    m = su2_fixture_mesh()
+    p = str(tmp_path / "naca.su2")
+    meshgen.write_su2(p, m)
+    back = meshgen.read_su2(p)
+    for k in ("x", "y", "cells", "bnd", "bnd_tag"):
+        assert np.array_equal(getattr(back, k), getattr(m, k)), k
+    assert back.names == m.names
+    a = Mesh.read_su2(p)
+    b = Mesh.from_arrays(m)
+    assert (a.n_real, a.n_cells, a.n_edges, a.n_bnd) == (4096, 4224, 8256, 128)
+    for k in SU2_MEMBERS:
+        assert bits_equal(a.export(k), b.export(k)), k
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/examples/naca"), reason="reference tree absent")
+@pytest.mark.parametrize("name", ["NACA_M.su2", "naca0012_129.su2"])
+def test_su2_readers_agree_on_the_reference_examples(name):
+    path = os.path.join("/root/reference/examples/naca", name)
+    a = Mesh.read_su2(path)
+    b = Mesh.from_arrays(meshgen.read_su2(path))
+    for k in SU2_MEMBERS:
+        assert bits_equal(a.export(k), b.export(k)), k
+
+
+def test_su2_reader_errors(tmp_path):
+    p = tmp_path / "bad.su2"
+    p.write_text("NDIME= 2\nNELEM= 1\n7 0 1 2 3 4 0\nNPOIN= 3\n0 0 0\n1 0 1\n0 1 2\nNMARK= 0\n")
+    with pytest.raises(FvhError, match="not a triangle or quadrilateral"):
+        Mesh.read_su2(str(p))
+    p.write_text("NDIME= 3\n")
+    with pytest.raises(FvhError, match="only 2-D"):
+        Mesh.read_su2(str(p))
+    with pytest.raises(FvhError, match="cannot open"):
+        Mesh.read_su2(str(tmp_path / "missing.su2"))
