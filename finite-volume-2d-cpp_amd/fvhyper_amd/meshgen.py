@@ -318,6 +318,131 @@ def write_msh(path: str, m: MeshArrays) -> None:
 
 
 # ---------------------------------------------------------------------------
+# Partitioned MSH 4.1 for several ranks, in the layout the reference reads
+# (mesh.cpp:316-340, 574-617, 803-855): rank r reads <name>_<r+1>.msh with its
+# own cells, one layer of neighbour cells marked in $GhostElements with their
+# owner partition, the boundary elements of its own cells, and the nodes of
+# all of them.  Element and node tags are the single-file writer's (global),
+# which make_comms (mesh.cpp:673-799) relies on to pair the ranks' cells.
+# ---------------------------------------------------------------------------
+
+def cell_centroids(m: MeshArrays):
+    tri = m.cells[:, 3] < 0
+    idx = np.where(m.cells < 0, m.cells[:, :1], m.cells)
+    n = np.where(tri, 3.0, 4.0)
+    cx = (m.x[idx].sum(axis=1) - np.where(tri, m.x[idx[:, 3]], 0.0)) / n
+    cy = (m.y[idx].sum(axis=1) - np.where(tri, m.y[idx[:, 3]], 0.0)) / n
+    return cx, cy
+
+
+def partition_rcb(m: MeshArrays, parts: int) -> np.ndarray:
+    """Recursive coordinate bisection of the cell centroids into `parts`
+    parts of equal size (within one cell); returns the part of each cell."""
+    cx, cy = cell_centroids(m)
+    out = np.zeros(m.n_cells, dtype=np.int32)
+
+    def rec(ids, p0, np_):
+        if np_ == 1:
+            out[ids] = p0
+            return
+        a = np_ // 2
+        c = cx[ids] if np.ptp(cx[ids]) >= np.ptp(cy[ids]) else cy[ids]
+        order = ids[np.argsort(c, kind="stable")]
+        k = (len(ids) * a) // np_
+        rec(order[:k], p0, a)
+        rec(order[k:], p0 + a, np_ - a)
+    rec(np.arange(m.n_cells), 0, parts)
+    return out
+
+
+def _cell_edge_keys(m: MeshArrays):
+    """Sorted node-pair keys of every cell side and the side's cell."""
+    tri = m.cells[:, 3] < 0
+    a = []
+    b = []
+    for i in range(4):
+        j = (i + 1) % 4
+        n0 = m.cells[:, i]
+        n1 = np.where(tri & (i == 2), m.cells[:, 0], m.cells[:, j])
+        ok = ~(tri & (i == 3))
+        a.append(np.stack([np.minimum(n0, n1)[ok], np.maximum(n0, n1)[ok]], 1))
+        b.append(np.nonzero(ok)[0])
+    keys = np.concatenate(a).astype(np.int64)
+    owner = np.concatenate(b)
+    k = keys[:, 0] * (m.x.size + 1) + keys[:, 1]
+    order = np.argsort(k, kind="stable")
+    return k[order], owner[order]
+
+
+def cell_neighbours(m: MeshArrays):
+    """Face neighbours: (cell, neighbour) pairs over shared edges."""
+    ks, os_ = _cell_edge_keys(m)
+    same = np.nonzero(ks[1:] == ks[:-1])[0]
+    return np.concatenate([os_[same], os_[same + 1]]), np.concatenate([os_[same + 1], os_[same]])
+
+
+def write_partitioned_msh(directory: str, name: str, m: MeshArrays, part: np.ndarray) -> int:
+    """Write <name>_<r+1>.msh for every part r of `part` (one per cell);
+    returns the number of parts."""
+    import os
+    P = int(part.max()) + 1
+    src, dst = cell_neighbours(m)
+    # the boundary element of each edge belongs to the cell holding both nodes
+    ks, os_ = _cell_edge_keys(m)
+    bk = np.minimum(m.bnd[:, 0], m.bnd[:, 1]).astype(np.int64) * (m.x.size + 1) + np.maximum(m.bnd[:, 0], m.bnd[:, 1])
+    pos = np.searchsorted(ks, bk)
+    if np.any(pos >= ks.size) or np.any(ks[np.minimum(pos, ks.size - 1)] != bk):
+        raise ValueError("a boundary element is not a side of any cell")
+    bnd_cell = os_[pos]
+    nb_names = len(m.names)
+    head = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", str(nb_names + 1)]
+    head += ['1 %d "%s"' % (k + 1, nm) for k, nm in enumerate(m.names)]
+    head += ['2 %d "internal"' % (nb_names + 1), "$EndPhysicalNames", "$Entities", "0 %d 1 0" % nb_names]
+    head += ["%d 0 0 0 0 0 0 1 %d 0" % (k + 1, k + 1) for k in range(nb_names)]
+    head += ["1 0 0 0 0 0 0 1 %d 0" % (nb_names + 1), "$EndEntities"]
+    for r in range(P):
+        own = np.nonzero(part == r)[0]
+        gh = np.unique(dst[(part[src] == r) & (part[dst] != r)])
+        cells = np.concatenate([own, gh])
+        bsel = np.nonzero(part[bnd_cell] == r)[0]
+        used = np.unique(np.concatenate([m.cells[cells][m.cells[cells] >= 0], m.bnd[bsel].ravel()]))
+        lines = list(head)
+        lines += ["$Nodes", "1 %d 1 %d" % (used.size, m.x.size), "2 1 0 %d" % used.size]
+        lines += [str(int(v) + 1) for v in used]
+        lines += ["%.17g %.17g 0" % (m.x[v], m.y[v]) for v in used]
+        lines.append("$EndNodes")
+        blocks = []
+        for t in range(nb_names):
+            bs = bsel[m.bnd_tag[bsel] == t]
+            if bs.size:
+                blocks.append(["1 %d 1 %d" % (t + 1, bs.size)] +
+                              ["%d %d %d" % (b + 1, m.bnd[b, 0] + 1, m.bnd[b, 1] + 1) for b in bs])
+        for want_tri in (True, False):
+            cs = cells[(m.cells[cells, 3] < 0) == want_tri]
+            if cs.size == 0:
+                continue
+            if want_tri:
+                rows = ["%d %d %d %d" % (m.n_bnd + c + 1, m.cells[c, 0] + 1, m.cells[c, 1] + 1, m.cells[c, 2] + 1)
+                        for c in cs]
+                blocks.append(["2 1 2 %d" % cs.size] + rows)
+            else:
+                rows = ["%d %d %d %d %d" % (m.n_bnd + c + 1, m.cells[c, 0] + 1, m.cells[c, 1] + 1,
+                                            m.cells[c, 2] + 1, m.cells[c, 3] + 1) for c in cs]
+                blocks.append(["2 1 3 %d" % cs.size] + rows)
+        n_el = sum(len(b_) - 1 for b_ in blocks)
+        lines += ["$Elements", "%d %d 1 %d" % (len(blocks), n_el, m.n_bnd + m.n_cells)]
+        for b_ in blocks:
+            lines += b_
+        lines.append("$EndElements")
+        lines += ["$GhostElements", str(gh.size)]
+        lines += ["%d %d 1 %d" % (m.n_bnd + c + 1, part[c] + 1, r + 1) for c in gh]
+        lines.append("$EndGhostElements")
+        with open(os.path.join(directory, "%s_%d.msh" % (name, r + 1)), "w") as f:
+            f.write("\n".join(lines) + "\n")
+    return P
+
+
+# ---------------------------------------------------------------------------
 # NACA 0012 O-mesh: quad boundary band + triangulated rings to the farfield
 # ---------------------------------------------------------------------------
 

@@ -403,6 +403,39 @@ def run_ref(mode: str, workdir: str, mesh_name: str, flags: str = "", timeout: f
     return d
 
 
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+def mpi_available() -> bool:
+    return os.path.exists(MPIRUN) and os.path.exists(REF_BIN)
+
+
+def run_ref_mpi(mode: str, workdir: str, mesh_name: str, ranks: int, flags: str = "", timeout: float = 900,
+                **kv) -> Dict:
+    """The reference harness on `ranks` MPI ranks, each reading
+    <mesh_name>_<rank+1>.msh from ``workdir``.  Returns {'stdout', 'ranks':
+    [per-rank dump dicts]} (rank 0 prints the step rows)."""
+    out = os.path.join(workdir, mesh_name + "." + mode + ".dump")
+    args = [MPIRUN, "-n", str(ranks), REF_BIN, mode, mesh_name, out] + ["%s=%s" % (k, v) for k, v in kv.items()]
+    env = dict(os.environ)
+    env["FVH_REF_FLAGS"] = flags
+    r = subprocess.run(args, cwd=workdir, env=env, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError("mpirun fvh_ref failed (%d): %s %s" % (r.returncode, r.stdout[-2000:], r.stderr[-2000:]))
+    return {"stdout": r.stdout, "ranks": [read_dump(out + "." + str(k)) for k in range(ranks)]}
+
+
+def gather_ranks(res: Dict, n_bnd: int, n_cells: int) -> np.ndarray:
+    """Each rank's own (non-ghost) cells of q, placed in the single-file cell
+    order (element tag - 1 - n_bnd); (n_cells, 4)."""
+    q = np.full((n_cells, 4), np.nan)
+    for d in res["ranks"]:
+        own = d["cellsIsGhost"] == 0
+        cells = d["cellsOriginal"].astype(np.int64)[own] - n_bnd
+        q[cells] = d["q"].reshape(-1, 4)[:own.size][own]
+    return q
+
+
 def ref_args(cfg: Dict) -> Dict[str, str]:
     """Map a config dict onto the harness's plugin selection.  Only the
     combinations the reference examples provide are expressible."""

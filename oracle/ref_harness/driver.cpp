@@ -258,7 +258,19 @@ int main(int argc, char** argv) {
     opt.print_interval = (uint)std::stoul(get("print", "1"));
     opt.verbose = get("verbose", "1") == "1";
 
-    Dump out(outpath);
+    // one dump per rank when several ranks run (<out>.<rank>)
+    Dump out(pool.size > 1 ? outpath + "." + std::to_string(pool.rank) : outpath);
+    if (pool.size > 1) {
+        // element tag - 1 of each of this rank's cells (its numbering of the
+        // partition file), and whether it is another rank's cell
+        std::vector<uint32_t> orig(m.nRealCells), ghost(m.nRealCells);
+        for (uint c = 0; c < m.nRealCells; ++c) {
+            orig[c] = m.currentToOriginalCells.at(c);
+            ghost[c] = m.cellsIsGhost[c] ? 1 : 0;
+        }
+        out.u("cellsOriginal", orig);
+        out.u("cellsIsGhost", ghost);
+    }
     if (mode == "mesh") {
         dump_mesh(out, m);
     } else if (mode == "run" || mode == "vtk") {
