@@ -119,7 +119,22 @@ PMC_PROFILES = {
     ("step", "f64", "flux"): ("k_flux_tile__mach3_step_975k_fp64.md", "k_flux_tile<double"),
     ("airfoil", "f64", "stage"): ("k_stage__naca0012_2m_fp64.md", "k_stage<double"),
     ("sweep", "f32", "flux"): ("k_flux_tile__naca0012_sweep256_fp32.md", "k_flux_tile<float"),
+    ("sweep", "f32", "stage"): ("k_stage__naca0012_sweep256_fp32.md", "k_stage<float"),
 }
+
+
+def calibrate_copy(a):
+    """A plain device copy of --calibrate-copy-mb MiB (three times), whose
+    FETCH_SIZE / WRITE_SIZE counters scale the PMC traffic of a profiled run."""
+    if not a.calibrate_copy_mb:
+        return
+    import torch
+    n = a.calibrate_copy_mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def pmc_traffic(workload, precision, kernel):
@@ -190,11 +205,13 @@ def sweep(a):
         meshes.append(built[key])
     cfgs = [it[1] for it in items]
     q0 = [workloads.initial_state(m, it[2]) for m, it in zip(meshes, items)]
-    # stage form and tile size from the autotuner, timed on every 8th instance
+    # stage form and tile size from the autotuner, timed on the whole batch: a
+    # sample of the instances fits the 256 MB Infinity Cache better than the
+    # batch does and can rank the forms the other way round
     from fvhyper_amd import autotune
-    sub = list(range(0, len(meshes), 8))
+    sub = list(range(0, len(meshes), a.tune_every))
     tuned = autotune([meshes[k] for k in sub], [cfgs[k] for k in sub], np.concatenate([q0[k] for k in sub]),
-                     precision=a.precision, steps=5)
+                     precision=a.precision, steps=3)
     b = Batch(meshes, cfgs, precision=a.precision, flux_mode=tuned["flux_mode"], tile_cells=tuned["tile_cells"])
     b.set_state(b.join(q0))
     b.apply_bounds()
@@ -238,6 +255,7 @@ def sweep(a):
         x.advance(k_pm)
         x.synchronize()
     secs_pm = time.perf_counter() - t0
+    calibrate_copy(a)
     value = C * stages * a.steps / secs / 1e6
     value_pm = C * stages * k_pm / secs_pm / 1e6
     sb = 4 if a.precision == "f32" else 8
@@ -399,6 +417,8 @@ def main():
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
     ap.add_argument("--per-mesh-steps", type=int, default=0,
                     help="sweep: steps of the one-by-one baseline (default: --steps)")
+    ap.add_argument("--tune-every", type=int, default=1,
+                    help="sweep: the autotuner times every n-th instance (1: the whole batch)")
     ap.add_argument("--n-per-unit", type=int, default=440, help="step mesh resolution")
     ap.add_argument("--flux-mode", default="auto", choices=["auto", "fused", "tile", "cell"],
                     help="step/airfoil: stage form (auto: fused where it fits)")
@@ -466,13 +486,7 @@ def main():
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(3)
 
-    if a.calibrate_copy_mb:
-        n = a.calibrate_copy_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+    calibrate_copy(a)
 
     value = C * stages * a.steps / secs / 1e6
     sb = 4 if a.precision == "f32" else 8

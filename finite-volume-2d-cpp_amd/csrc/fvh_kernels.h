@@ -143,7 +143,17 @@ template <class T>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles,
                  const PhysParams& pp, bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
                  const Q4<T>* q0, Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
-                 const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl);
+                 const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl,
+                 Q4<T>* qt_only = nullptr);
+// One sweep of implicit residual smoothing (explicit.cpp:32-66) over the qt
+// that a flux pass with qt_only stored: qt_out != nullptr stores the sweep's
+// qt; qt_out == nullptr finishes the stage from it (update, boundary update,
+// residual sums), taking the flux pass's place.
+template <class T>
+void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, bool lin,
+                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<T>* q0,
+                   Q4<T>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
+                   Q4<T>* dbg_qt, const Ctrl* ctrl);
 // Fused stage: gradient + limiter (tile and ring 1, from LDS), face states,
 // edge fluxes, gather, RK update and boundary update in one pass per tile.
 // dt_kind >= 0 also computes the tile cells' local time step (first stage).

@@ -313,7 +313,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
-                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
+                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl,
+                                                      Q4<T>* __restrict__ qt_only) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int inst = tm.inst[lb];
@@ -354,9 +355,17 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
-                            res_blk != nullptr, rs, dbg_qt);
+        if (qt_only) {
+            // residual smoothing follows: hand qt over, the update comes later
+            Q4<T> t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
+            qt_only[c] = t;
+        } else {
+            auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
+            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+                                res_blk != nullptr, rs, dbg_qt);
+        }
     }
     if (res_blk) {
         block_sum4(rs);
@@ -378,7 +387,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
-                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl) {
+                                                      Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl,
+                                                      Q4<T>* __restrict__ qt_only) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int inst = tm.inst[lb];
@@ -425,9 +435,77 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
                 qt[k] = side ? qt[k] + t : qt[k] - t;
             }
         }
-        auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-        finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
-                            res_blk != nullptr, rs, dbg_qt);
+        if (qt_only) {
+            // residual smoothing follows: hand qt over, the update comes later
+            Q4<T> t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
+            qt_only[c] = t;
+        } else {
+            auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
+            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+                                res_blk != nullptr, rs, dbg_qt);
+        }
+    }
+    if (res_blk) {
+        block_sum4(rs);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
+    }
+}
+
+// One Jacobi sweep of the implicit residual smoothing of explicit.cpp:32-66,
+//   qt_i = (qt0_i - s_i) / (1 + n_e eps),  s_i = sum over the cell's edges,
+//   in edge order, of -(qt_j eps),  eps = 0.6,  n_e = 3 or 4 edges,
+// where qt_j is the previous sweep's value (qt0 for the first) and a
+// boundary ghost contributes its qt of 0 (explicit.cpp:352-362).  With
+// qt_out the sweep's result is stored; without, it finishes the stage
+// (update, boundary update, residual sums) as the flux pass would have.
+template <class T, bool LIN>
+__global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, PhysParams pp,
+                                                   const BcParam<T>* __restrict__ bcs,
+                                                   const Q4<T>* __restrict__ qt_nb, const Q4<T>* __restrict__ qt0,
+                                                   Q4<T>* __restrict__ qt_out, const Q4<T>* __restrict__ q0,
+                                                   Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
+                                                   const T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
+                                                   double* __restrict__ res_blk, Q4<T>* __restrict__ dbg_qt,
+                                                   const Ctrl* __restrict__ ctrl) {
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    const int inst = tm.inst[lb];
+    if (stopped(ctrl, inst)) return;
+    const int c = tm.c0[lb] + threadIdx.x;
+    double rs[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < tm.c0[lb + 1]) {
+        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+        const T eps = T(0.6);
+        T sm[4] = {T(0), T(0), T(0), T(0)};
+        for (int h = h0; h < h1; ++h) {
+            Q4<T> v;
+            if (m.he_edge[h] & kBcBit) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v.v[k] = T(0);
+            } else {
+                v = ld(qt_nb + m.he_nbr[h]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sm[k] = sm[k] - v.v[k] * eps;
+        }
+        const T den = T(1) + T(h1 - h0) * eps;
+        const Q4<T> a = ld(qt0 + c);
+        T qt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qt[k] = (a.v[k] - sm[k]) / den;
+        if (qt_out) {
+            Q4<T> t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) t.v[k] = qt[k];
+            qt_out[c] = t;
+        } else {
+            auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
+            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+                                res_blk != nullptr, rs, dbg_qt);
+        }
     }
     if (res_blk) {
         block_sum4(rs);
@@ -730,13 +808,14 @@ template <class T>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles, const PhysParams& pp,
                  bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
                  const Q4<T>* face, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
-                 Q4<T>* dbg_qt, const Ctrl* ctrl) {
+                 Q4<T>* dbg_qt, const Ctrl* ctrl, Q4<T>* qt_only) {
     const int nb = xcd_grid_blocks(tm.n_rt);
     if (nb == 0) return;
     if (tiles) {
         const TileView tv = *tiles;
 #define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, tv, pp, bcs, \
-                                        qin, q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
+                                        qin, q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, \
+                                        qt_only)
         if (joined) {
             if (lin) FVH_FT(true, true); else FVH_FT(true, false);
         } else {
@@ -746,13 +825,26 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
         return;
     }
 #define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qin, \
-                                        q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
+                                        q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, qt_only)
     if (joined) {
         if (lin) FVH_FX(true, true); else FVH_FX(true, false);
     } else {
         if (lin) FVH_FX(false, true); else FVH_FX(false, false);
     }
 #undef FVH_FX
+}
+
+template <class T>
+void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, bool lin,
+                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<T>* q0,
+                   Q4<T>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
+                   Q4<T>* dbg_qt, const Ctrl* ctrl) {
+    const int nb = xcd_grid_blocks(tm.n_rt);
+    if (nb == 0) return;
+#define FVH_SM(L) hipLaunchKernelGGL((k_smooth<T, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qt_nb, qt0, \
+                                     qt_out, q0, qout, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
+    if (lin) FVH_SM(true); else FVH_SM(false);
+#undef FVH_SM
 }
 
 template <class T>
@@ -844,7 +936,11 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
     template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView*,            \
                                  const PhysParams&, bool, bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, \
                                  Q4<T>*, const Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,  \
-                                 const Ctrl*);                                                               \
+                                 const Ctrl*, Q4<T>*);                                                       \
+    template void launch_smooth<T>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&, bool, \
+                                   const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,       \
+                                   Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,             \
+                                   const Ctrl*);                                                             \
     template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,            \
                                         const BcParam<T>*, Q4<T>*);                                          \
     template void launch_sums<T>(hipStream_t, const MeshView<T>&, const TileMap&, const InstView&,           \

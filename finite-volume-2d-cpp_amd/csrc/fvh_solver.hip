@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -30,8 +31,9 @@
 namespace fvh {
 namespace {
 
-const char* kKernelNames[] = {"dt", "dt_min", "gradlim", "flux", "ctrl_begin", "ctrl_end", "bounds_init", "stage"};
-enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_STAGE, K_COUNT };
+const char* kKernelNames[] = {"dt", "dt_min", "gradlim", "flux", "ctrl_begin", "ctrl_end", "bounds_init", "stage",
+                              "smooth"};
+enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_STAGE, K_SMOOTH, K_COUNT };
 
 #define FVH_HIP(call)                                                                       \
     do {                                                                                    \
@@ -99,6 +101,7 @@ struct fvh_solver {
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
+    fvh::DevBuf qt_s[2];                     // residual smoothing: raw qt, first sweep's qt
     int cur = 0;                          // index of the current state buffer
     // discrete adjoint: per device edge (half-edge side 0, side 1, cell side 0,
     // cell side 1); boundary element geometry (instance 0); work buffers
@@ -342,6 +345,12 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
+    if (s->cfg.flags & FVH_FLAG_SMOOTH)
+        for (auto& b : s->qt_s)
+            if (b.alloc((size_t)(s->n_real + 1) * sizeof(Q4<T>))) {
+                set_error("hipMalloc failed");
+                return FVH_E_HIP;
+            }
     FVH_HIP(hipMemset(s->ctrl.p, 0, s->ctrl.bytes));
     FVH_HIP(hipMemset(s->face.p, 0, s->face.bytes));
     FVH_HIP(hipMemset(s->bincr.p, 0, s->bincr.bytes));
@@ -363,7 +372,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     const PhysParams pp = phys(c);
     const BcParam<T>* bcs = (const BcParam<T>*)s->bcs.p;
     const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
-               lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT;
+               lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT,
+               smooth = c.flags & FVH_FLAG_SMOOTH;
     Ctrl* ctrl = (Ctrl*)s->ctrl.p;
     const Ctrl* cc = run_mode ? ctrl : nullptr;
     hipStream_t st = s->stream;
@@ -416,12 +426,26 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
             launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
         }
         const bool last = k == n_stage - 1;
-        {
+        const T a = T(alpha_override ? *alpha_override : c.alpha[k]);
+        double* res = (last && need_res) ? (double*)s->res_blk.p : nullptr;
+        if (smooth) {
+            // flux pass stores qt; two smoothing sweeps, the second finishes the stage
+            Q4<T>* qs0 = (Q4<T>*)s->qt_s[0].p;
+            Q4<T>* qs1 = (Q4<T>*)s->qt_s[1].p;
+            {
+                Bracket b(s, K_FLUX);
+                launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
+                               qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr, a, nullptr, nullptr, cc, qs0);
+            }
+            Bracket b(s, K_SMOOTH);
+            launch_smooth<T>(st, m, tm, pp, lin, bcs, qs0, qs0, qs1, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
+                             nullptr, nullptr, cc);
+            launch_smooth<T>(st, m, tm, pp, lin, bcs, qs1, qs0, nullptr, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
+                             res, dbg ? dbg[3] : nullptr, cc);
+        } else {
             Bracket b(s, K_FLUX);
             launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
-                           qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr,
-                           T(alpha_override ? *alpha_override : c.alpha[k]),
-                           (last && need_res) ? (double*)s->res_blk.p : nullptr, dbg ? dbg[3] : nullptr, cc);
+                           qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr, a, res, dbg ? dbg[3] : nullptr, cc);
         }
         qin = out;
         out = (out == qa) ? qb : qa;
@@ -1040,14 +1064,16 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     HaloHost hh;
     const bool lin_on = (c.flags & FVH_FLAG_LINEAR) != 0;
     s->fused = false;
+    // residual smoothing needs every cell's qt before any update: not fusable
     if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
-        lin_on && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
+        lin_on && !(c.flags & FVH_FLAG_SMOOTH) && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
         const size_t qb = s->f32 ? 16 : 32;
         const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * qb;
         s->fused = lds <= 160 * 1024;
     }
     if (c.flux_mode == FVH_FLUX_MODE_FUSED && !s->fused && s->n_real > 0) {
-        set_error("fused stage pass unavailable: needs linear reconstruction and a tile halo that fits in LDS");
+        set_error("fused stage pass unavailable: needs linear reconstruction, no residual smoothing and a tile "
+                  "halo that fits in LDS");
         return FVH_E_UNSUPPORTED;
     }
     if (s->fused) {
@@ -1475,6 +1501,10 @@ int32_t adjoint_check(fvh_solver* s) {
         set_error("the adjoint needs a global or fixed time step");
         return FVH_E_UNSUPPORTED;
     }
+    if (c.flags & FVH_FLAG_SMOOTH) {
+        set_error("the adjoint does not take residual smoothing");
+        return FVH_E_UNSUPPORTED;
+    }
     return FVH_OK;
 }
 
@@ -1720,6 +1750,67 @@ int32_t fvh_solver_set_state(fvh_solver* s, const double* q) {
 int32_t fvh_solver_get_state(fvh_solver* s, double* q) {
     if (!s || !q) { set_error("null argument"); return FVH_E_ARG; }
     return s->f32 ? get_q4<float>(s, s->q[s->cur].p, q, s->n_total) : get_q4<double>(s, s->q[s->cur].p, q, s->n_total);
+}
+
+// Restart file, little-endian: "FVHSTAT1", int32 {version 1, precision,
+// instances, 0}, int64 value count (4 per cell, all cells), one Ctrl record
+// per instance (step, time, residuals, R0, stop flag), then the state in the
+// reference layout as doubles (exact for either precision).
+namespace {
+const char kRestartMagic[8] = {'F', 'V', 'H', 'S', 'T', 'A', 'T', '1'};
+static_assert(sizeof(fvh::Ctrl) == 104, "restart record layout");
+}  // namespace
+
+int32_t fvh_solver_save(fvh_solver* s, const char* path) {
+    if (!s || !path) { set_error("null argument"); return FVH_E_ARG; }
+    const int64_t nv = 4 * (int64_t)s->n_total;
+    std::vector<double> q((size_t)nv);
+    int32_t rc = fvh_solver_get_state(s, q.data());
+    if (rc) return rc;
+    std::vector<Ctrl> h;
+    if ((rc = read_ctrl(s, h))) return rc;
+    std::FILE* f = std::fopen(path, "wb");
+    if (!f) { set_error(std::string("cannot open ") + path + " for writing"); return FVH_E_IO; }
+    const int32_t head[4] = {1, s->f32 ? FVH_F32 : FVH_F64, s->n_inst, 0};
+    bool ok = std::fwrite(kRestartMagic, 1, 8, f) == 8 && std::fwrite(head, 4, 4, f) == 4 &&
+              std::fwrite(&nv, 8, 1, f) == 1 && std::fwrite(h.data(), sizeof(Ctrl), h.size(), f) == h.size() &&
+              std::fwrite(q.data(), 8, q.size(), f) == q.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) { set_error(std::string("write failed: ") + path); return FVH_E_IO; }
+    return FVH_OK;
+}
+
+int32_t fvh_solver_load(fvh_solver* s, const char* path) {
+    if (!s || !path) { set_error("null argument"); return FVH_E_ARG; }
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) { set_error(std::string("cannot open ") + path); return FVH_E_IO; }
+    char magic[8];
+    int32_t head[4];
+    int64_t nv = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kRestartMagic, 8) == 0 &&
+              std::fread(head, 4, 4, f) == 4 && std::fread(&nv, 8, 1, f) == 1;
+    if (!ok || head[0] != 1) {
+        std::fclose(f);
+        set_error(std::string("not a restart file: ") + path);
+        return FVH_E_FORMAT;
+    }
+    if (head[2] != s->n_inst || nv != 4 * (int64_t)s->n_total) {
+        std::fclose(f);
+        set_error("restart file holds " + std::to_string(head[2]) + " instances and " + std::to_string(nv / 4) +
+                  " cells; the solver has " + std::to_string(s->n_inst) + " and " + std::to_string(s->n_total));
+        return FVH_E_ARG;
+    }
+    std::vector<Ctrl> h((size_t)s->n_inst);
+    std::vector<double> q((size_t)nv);
+    ok = std::fread(h.data(), sizeof(Ctrl), h.size(), f) == h.size() &&
+         std::fread(q.data(), 8, q.size(), f) == q.size();
+    std::fclose(f);
+    if (!ok) { set_error(std::string("truncated restart file: ") + path); return FVH_E_FORMAT; }
+    int32_t rc = fvh_solver_set_state(s, q.data());
+    if (rc) return rc;
+    FVH_HIP(hipMemcpyAsync(s->ctrl.p, h.data(), h.size() * sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    return FVH_OK;
 }
 
 int32_t fvh_solver_apply_bounds(fvh_solver* s) {

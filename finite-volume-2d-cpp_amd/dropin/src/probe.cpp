@@ -270,7 +270,6 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
         throw std::runtime_error("vars = " + std::to_string(vars) +
                                  ": the device path implements the 4-variable Euler plugins");
     if (solver::diffusive_gradients) throw std::runtime_error("diffusive_gradients: viscous fluxes are not on the device path");
-    if (solver::smooth_residuals) throw std::runtime_error("smooth_residuals: residual smoothing is not on the device path");
     fvh_config c;
     fvh_config_default(&c);
     c.flags = 0;
@@ -278,6 +277,7 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
     if (solver::do_calc_limiters) c.flags |= FVH_FLAG_LIMITERS;
     if (solver::linear_interpolate) c.flags |= FVH_FLAG_LINEAR;
     if (solver::global_dt) c.flags |= FVH_FLAG_GLOBAL_DT;
+    if (solver::smooth_residuals) c.flags |= FVH_FLAG_SMOOTH;
     c.K = solver::limiter_k_value;
     const FluxKind fk = probe_flux();
     c.flux = fk.joined ? FVH_FLUX_ROE_JOINED : FVH_FLUX_ROE_SPLIT;

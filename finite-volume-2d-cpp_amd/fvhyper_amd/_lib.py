@@ -21,7 +21,7 @@ FLUX = {"roe_split": 0, "roe_joined": 1}
 PHI = {"zero": 0, "michalak": 1, "venkat": 2, "minmod": 3}
 DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2}
 BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3}
-FLAG_GRADIENTS, FLAG_LIMITERS, FLAG_LINEAR, FLAG_GLOBAL_DT, FLAG_POSITIVITY = 1, 2, 4, 8, 16
+FLAG_GRADIENTS, FLAG_LIMITERS, FLAG_LINEAR, FLAG_GLOBAL_DT, FLAG_POSITIVITY, FLAG_SMOOTH = 1, 2, 4, 8, 16, 32
 RK5 = [0.0695, 0.1602, 0.2898, 0.506, 1.0]
 
 
@@ -64,7 +64,7 @@ EXPORTS = [
     "fvh_last_error", "fvh_abi_version", "fvh_mesh_read_msh", "fvh_mesh_read_su2", "fvh_mesh_from_arrays",
     "fvh_mesh_free", "fvh_mesh_counts", "fvh_mesh_export", "fvh_mesh_name",
     "fvh_config_default", "fvh_solver_create", "fvh_solver_free", "fvh_solver_set_state",
-    "fvh_solver_get_state", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
+    "fvh_solver_get_state", "fvh_solver_save", "fvh_solver_load", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
     "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
@@ -108,6 +108,8 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_free.restype = None
     L.fvh_solver_set_state.argtypes = [vp, dp]
     L.fvh_solver_get_state.argtypes = [vp, dp]
+    L.fvh_solver_save.argtypes = [vp, C.c_char_p]
+    L.fvh_solver_load.argtypes = [vp, C.c_char_p]
     L.fvh_solver_apply_bounds.argtypes = [vp]
     L.fvh_solver_phases.argtypes = [vp, C.c_double, dp, dp, dp, dp, dp, dp]
     L.fvh_solver_run.argtypes = [vp, C.POINTER(RunOptions), C.POINTER(Status)]

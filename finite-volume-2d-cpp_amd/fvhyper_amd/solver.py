@@ -8,6 +8,7 @@ functions (explicit.h:88-129, here one stage of all of them at once).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Dict, List, Optional
 
 import numpy as np
@@ -109,6 +110,8 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
         flags |= _lib.FLAG_GLOBAL_DT
     if cfg.get("positivity", False):
         flags |= _lib.FLAG_POSITIVITY
+    if cfg.get("smooth", False):
+        flags |= _lib.FLAG_SMOOTH
     c.flags = flags
     alpha = list(cfg.get("alpha", _lib.RK5))
     if not 1 <= len(alpha) <= _lib.FVH_MAX_ALPHA:
@@ -180,6 +183,15 @@ class Solver:
 
     def apply_bounds(self) -> None:
         check(_lib.load().fvh_solver_apply_bounds(self._h))
+
+    def save(self, path: str) -> None:
+        """Binary restart file: every cell's state and the run bookkeeping."""
+        check(_lib.load().fvh_solver_save(self._h, os.fsencode(path)))
+
+    def load(self, path: str) -> None:
+        """Restore a restart file written by save() for the same meshes;
+        run_continue() then carries on the saved run bit for bit."""
+        check(_lib.load().fvh_solver_load(self._h, os.fsencode(path)))
 
     def phases(self, alpha: float = 0.5) -> Dict[str, np.ndarray]:
         n = self.n_state

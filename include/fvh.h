@@ -29,8 +29,8 @@ extern "C" {
 enum {
     FVH_OK = 0,
     FVH_E_ARG = -1,       /* invalid argument */
-    FVH_E_IO = -2,        /* file cannot be read */
-    FVH_E_FORMAT = -3,    /* malformed or unsupported mesh file */
+    FVH_E_IO = -2,        /* file cannot be read or written */
+    FVH_E_FORMAT = -3,    /* malformed or unsupported mesh or restart file */
     FVH_E_MESH = -4,      /* inconsistent mesh (boundary edge not in mesh, ...) */
     FVH_E_BC = -5,        /* physical name without a boundary condition */
     FVH_E_HIP = -6,       /* HIP runtime error */
@@ -70,6 +70,9 @@ enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2, F
 #define FVH_FLAG_LIMITERS    2u   /* do_calc_limiters */
 #define FVH_FLAG_LINEAR      4u   /* linear_interpolate */
 #define FVH_FLAG_GLOBAL_DT   8u   /* global_dt */
+#define FVH_FLAG_SMOOTH     32u   /* smooth_residuals: two Jacobi sweeps of implicit
+                                     residual smoothing, eps 0.6 (explicit.cpp:32-66,
+                                     743); not with the FUSED stage form */
 /* Extension without a reference counterpart: where a limited linear
  * reconstruction yields a face state with rho <= 0 or p <= 0, that face uses
  * the cell value instead (first order).  Off for reference parity. */
@@ -182,6 +185,16 @@ void fvh_solver_free(fvh_solver* s);
 /* State in the reference layout (4 doubles per cell, all cells). */
 int32_t fvh_solver_set_state(fvh_solver* s, const double* q);
 int32_t fvh_solver_get_state(fvh_solver* s, double* q);
+
+/* Binary restart (no reference counterpart: the reference writes VTU only,
+ * post.cpp:21-200).  save writes the state of every cell in the reference
+ * layout plus each instance's run bookkeeping (step, time, residuals, the
+ * step-0 residuals, stop flag); load restores both into a solver built on the
+ * same meshes, so that fvh_solver_run_continue after load reproduces the
+ * uninterrupted run bit for bit.  Format: "FVHSTAT1", int32 {1, precision,
+ * instances, 0}, int64 value count, 104-byte record per instance, doubles. */
+int32_t fvh_solver_save(fvh_solver* s, const char* path);
+int32_t fvh_solver_load(fvh_solver* s, const char* path);
 
 /* Fill ghost cells from boundary conditions with zero gradients
  * (update_bounds(q, 0, 0, 0), explicit.cpp:712) and reset step/time. */

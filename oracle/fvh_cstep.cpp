@@ -486,6 +486,28 @@ void update_bounds(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* 
     }
 }
 
+// smooth_residuals (explicit.cpp:32-66): two Jacobi sweeps, eps = 0.6
+template <class T>
+void smooth_residuals(const orc_mesh* m, T* qt) {
+    const int64_t n = (int64_t)V * m->n_cells;
+    const double epsilon = 0.6;
+    std::vector<T> qt0(qt, qt + n), sm(n);
+    for (int it = 0; it < 2; ++it) {
+        for (int64_t i = 0; i < n; ++i) sm[i] = T(0.);
+        for (int32_t e = 0; e < m->n_edges; ++e) {
+            const int64_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+            for (int k = 0; k < V; ++k) {
+                sm[V * i + k] -= qt[V * j + k] * epsilon;
+                sm[V * j + k] -= qt[V * i + k] * epsilon;
+            }
+        }
+        for (int32_t i = 0; i < m->n_real; ++i) {
+            const double ne = m->cell_tri[i] ? 3 : 4;
+            for (int k = 0; k < V; ++k) qt[V * i + k] = (qt0[V * i + k] - sm[V * i + k]) / (1. + ne * epsilon);
+        }
+    }
+}
+
 // run() for a fixed number of steps (explicit.cpp:657-817 without output)
 template <class T>
 void run(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* q, int steps) {
@@ -500,6 +522,7 @@ void run(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* q, int ste
             if (cf->grad) gradients(m, G, qk.data(), gx.data(), gy.data());
             if (cf->lim) limiters(m, G, cf, qk.data(), gx.data(), gy.data(), lim.data(), qmin.data(), qmax.data());
             time_derivatives(m, G, cf, qk.data(), gx.data(), gy.data(), lim.data(), qt.data());
+            if (cf->smooth) smooth_residuals(m, qt.data());
             const T a = T(cf->alpha[s]);
             for (int64_t c = 0; c < N; ++c)
                 for (int k = 0; k < V; ++k) qk[V * c + k] = q[V * c + k] + qt[V * c + k] * dt[c] * a;

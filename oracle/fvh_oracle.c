@@ -592,6 +592,38 @@ void orc_calc_residuals(const orc_mesh* m, const double* qt, double* R) {
     for (int k = 0; k < V; ++k) R[k] = sqrt(R[k]);
 }
 
+/* smooth_residuals (explicit.cpp:32-66): two Jacobi sweeps with eps = 0.6.
+ * Each sweep clears the smoother over every cell (:49-51), subtracts
+ * qt_j * eps from both sides of every edge in edge order (:52-59), then
+ * rewrites the real cells only (:60-65), so a boundary ghost keeps the
+ * qt = 0 that calc_time_derivatives gave it (:352-362) in both sweeps.
+ * n_e is 3 for a triangle and 4 otherwise (:61). */
+void orc_smooth_residuals(const orc_mesh* m, double* qt) {
+    const double epsilon = 0.6;
+    const int64_t n = (int64_t)V * m->n_cells;
+    double* qt0 = (double*)malloc(sizeof(double) * (size_t)n);
+    double* sm = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!qt0 || !sm) { free(qt0); free(sm); return; }
+    memcpy(qt0, qt, sizeof(double) * (size_t)n);
+    for (int it = 0; it < 2; ++it) {
+        for (int64_t i = 0; i < n; ++i) sm[i] = 0.;
+        for (int32_t e = 0; e < m->n_edges; ++e) {
+            const int64_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+            for (int k = 0; k < V; ++k) {
+                sm[V * i + k] -= qt[V * j + k] * epsilon;
+                sm[V * j + k] -= qt[V * i + k] * epsilon;
+            }
+        }
+        for (int32_t i = 0; i < m->n_real; ++i) {
+            const double ne = m->cell_tri[i] ? 3 : 4;
+            for (int k = 0; k < V; ++k)
+                qt[V * i + k] = (qt0[V * i + k] - sm[V * i + k]) / (1. + ne * epsilon);
+        }
+    }
+    free(qt0);
+    free(sm);
+}
+
 /* run (explicit.cpp:657-817), single rank, no output files */
 int orc_run(const orc_mesh* m, const orc_config* cf, double* q,
             uint32_t max_step, double max_time, double tol, int32_t stop_var, uint32_t print_interval,
@@ -625,6 +657,7 @@ int orc_run(const orc_mesh* m, const orc_config* cf, double* q,
             if (cf->grad) orc_calc_gradients(m, qk, gx, gy);
             if (cf->lim) orc_calc_limiters(m, cf, qk, gx, gy, lim, qmin, qmax);
             orc_calc_time_derivatives(m, cf, qk, gx, gy, lim, qt);
+            if (cf->smooth) orc_smooth_residuals(m, qt);   /* explicit.cpp:743 */
             orc_update_cells(N, qk, q, qt, dt, cf->alpha[s]);
             orc_update_bounds(m, cf, qk, gx, gy, lim);
         }

@@ -46,6 +46,7 @@ typedef struct {
     int n_bc;
     int bc_kind[ORC_MAX_BC];
     double bc_param[ORC_MAX_BC][4];   /* freestream rho, u, v, p */
+    int smooth;          /* solver::smooth_residuals (explicit.h:40) */
 } orc_config;
 
 typedef struct {
@@ -87,6 +88,8 @@ void orc_update_cells(int64_t n, double* q, const double* ql, const double* qt,
 void orc_update_bounds(const orc_mesh* m, const orc_config* c, double* q,
                        const double* gx, const double* gy, const double* lim);
 void orc_calc_residuals(const orc_mesh* m, const double* qt, double* R);
+/* smooth_residuals (explicit.cpp:32-66) in place on qt. */
+void orc_smooth_residuals(const orc_mesh* m, double* qt);
 
 /* Scalar plugin kinds, for known-answer tests. */
 void orc_flux(const orc_config* c, double* f, const double* qi, const double* qj,

@@ -43,6 +43,7 @@ class OrcConfig(C.Structure):
         ("K", C.c_double), ("gamma", C.c_double),
         ("n_alpha", C.c_int), ("alpha", C.c_double * 8),
         ("n_bc", C.c_int), ("bc_kind", C.c_int * 16), ("bc_param", (C.c_double * 4) * 16),
+        ("smooth", C.c_int),
     ]
 
 
@@ -86,6 +87,7 @@ def lib():
         L.orc_update_cells.argtypes = [C.c_int64, dp, dp, dp, dp, C.c_double]
         L.orc_update_bounds.argtypes = [mp, cfgp, dp, dp, dp, dp]
         L.orc_calc_residuals.argtypes = [mp, dp, dp]
+        L.orc_smooth_residuals.argtypes = [mp, dp]
         L.orc_flux.argtypes = [cfgp, dp, dp, dp, dp]
         L.orc_phi.argtypes = [cfgp, C.c_double]
         L.orc_phi.restype = C.c_double
@@ -111,6 +113,7 @@ def make_config(cfg: Dict, names: List[str]) -> OrcConfig:
     c.lin = int(bool(cfg.get("lin", False)))
     c.gdt = int(bool(cfg.get("gdt", False)))
     c.pos = int(bool(cfg.get("positivity", False)))
+    c.smooth = int(bool(cfg.get("smooth", False)))
     c.K = float(cfg.get("K", 7.5))
     c.gamma = float(cfg.get("gamma", 1.4))
     alpha = list(cfg.get("alpha", RK5))
@@ -203,6 +206,10 @@ class Mesh:
         out["lim"], out["qmin"], out["qmax"] = lim, qmin, qmax
         qt = np.zeros(n)
         L.orc_calc_time_derivatives(self._p, C.byref(c), q, gx, gy, lim, qt)
+        if c.smooth:
+            # the update and the residuals take the smoothed qt (explicit.cpp:743)
+            out["qt_raw"] = qt.copy()
+            L.orc_smooth_residuals(self._p, qt)
         out["qt"] = qt
         qk = np.zeros(n)
         L.orc_update_cells(N, qk, q, qt, dt, float(alpha))
@@ -457,4 +464,4 @@ def ref_args(cfg: Dict) -> Dict[str, str]:
 
 
 def ref_flags(cfg: Dict) -> str:
-    return ",".join(k for k in ("grad", "lim", "lin", "gdt") if cfg.get(k, False))
+    return ",".join(k for k in ("grad", "lim", "lin", "gdt", "smooth") if cfg.get(k, False))

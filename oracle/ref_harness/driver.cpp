@@ -14,7 +14,7 @@
 //   keys:  flux=st|fs|nc  dt=st|fs|nc  phi=zero|fs|nc|venk|minmod
 //          ic=st|fs|nc|file:<path>  gamma= cfl_fs= cfl_nc= mach=
 //          max_step= max_time= tol= print= alpha=
-//   solver flags come from FVH_REF_FLAGS (comma list of grad,lim,lin,gdt)
+//   solver flags come from FVH_REF_FLAGS (comma list of grad,lim,lin,gdt,smooth)
 //   because the library reads them as link-time constants.
 #include <fvhyper/mesh.h>
 #include <fvhyper/explicit.h>
@@ -82,7 +82,7 @@ namespace solver {
     extern const bool linear_interpolate = env_flag("lin");
     extern const bool diffusive_gradients = false;
     extern const bool global_dt = env_flag("gdt");
-    extern const bool smooth_residuals = false;
+    extern const bool smooth_residuals = env_flag("smooth");
 }
 
 // Venkatakrishnan's limiter function phi(r) = (r^2 + 2r)/(r^2 + r + 2).
@@ -295,7 +295,7 @@ int main(int argc, char** argv) {
         std::vector<double> q(vars * m.cellsAreas.size());
         generate_initial_solution(q, m);
         const size_t n = q.size();
-        std::vector<double> qk(n), qt(n), gx(n), gy(n), lim(n), qmin(n), qmax(n), dt(n);
+        std::vector<double> qk(n), qt(n), gx(n), gy(n), lim(n), qmin(n), qmax(n), dt(n), s0(n), s1(n);
         update_bounds(q, gx, gy, lim, m);
         uint step = 0;
         double time = 0.0;
@@ -306,6 +306,7 @@ int main(int argc, char** argv) {
             qk = q;
             for (const double& a : alpha) {
                 complete_calc_qt(qt, qk, gx, gy, qmin, qmax, lim, m, pool);
+                if (solver::smooth_residuals) smooth_residuals(qt, s0, s1, m);
                 update_cells(qk, q, qt, dt, a);
                 update_bounds(qk, gx, gy, lim, m);
             }
@@ -337,6 +338,12 @@ int main(int argc, char** argv) {
         out.d("qmin", qmin);
         out.d("qmax", qmax);
         calc_time_derivatives(qt, q, gx, gy, lim, m);
+        if (solver::smooth_residuals) {
+            // as run() (explicit.cpp:743): update and residuals see the smoothed qt
+            out.d("qt_raw", qt);
+            std::vector<double> s0(n), s1(n);
+            smooth_residuals(qt, s0, s1, m);
+        }
         out.d("qt", qt);
         update_cells(qk, q, qt, dt, a);
         out.d("qk_cells", qk);

@@ -13,6 +13,10 @@ in MSH 4.1, and stores inputs and outputs as .npz fixtures (data only):
   run__<mesh>__<cfg>.npz   initial state + q after run() (explicit.cpp:657-817)
                            for 10 steps, with the printed step/time/residuals
 
+    python tests/golden/gen_fixtures.py configs=smooth_first_split,...
+
+regenerates only the named configs' fixtures.
+
 The reference binds boundary functions by physical name (driver.cpp maps
 transmissive->copy, wall->mirror, inlet/outlet->inlet_outlet,
 farfield->farfield), so each case's mesh is written with the names of the
@@ -67,6 +71,22 @@ FIX_CONFIGS = {
                                    bcs={"wall": NC, "transmissive": ("mirror",), "inlet": NC, "outlet": NC}),
 }
 
+# implicit residual smoothing (explicit.cpp:32-66, 743), on the smaller meshes
+SMOOTH_MESHES = ("mixed", "step", "naca")
+SMOOTH_CONFIGS = {
+    "smooth_first_split": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, smooth=True,
+                               bcs={"wall": ("mirror",), "transmissive": ("copy",), "inlet": FS, "outlet": FS}),
+    "smooth_second_michalak": dict(flux="roe_split", dt="edge_min", cfl=1.0, phi="michalak", grad=True, lim=True,
+                                   lin=True, smooth=True, bcs={"wall": ("mirror",), "transmissive": NC, "inlet": NC,
+                                                               "outlet": ("copy",)}),
+}
+ALL_CONFIGS = {**FIX_CONFIGS, **SMOOTH_CONFIGS}
+
+
+def configs_of(mesh_name):
+    """Config names with fixtures on this mesh."""
+    return list(FIX_CONFIGS) + (list(SMOOTH_CONFIGS) if mesh_name in SMOOTH_MESHES else [])
+
 
 def renamed(m, cfg):
     """Mesh + config whose physical names are the harness's names for the kinds."""
@@ -99,16 +119,25 @@ def main():
     if not O.ref_available():
         raise SystemExit("build the reference harness first: make -C oracle ref")
     work = tempfile.mkdtemp(prefix="fvh_golden_")
-    only = sys.argv[1:]              # optional: regenerate these meshes only
+    # optional arguments: mesh names to regenerate, and configs=a,b to
+    # regenerate only those configs' fixtures (the mesh fixture is kept)
+    args = sys.argv[1:]
+    only_cfg = [a.split("=", 1)[1].split(",") for a in args if a.startswith("configs=")]
+    only_cfg = only_cfg[0] if only_cfg else None
+    only = [a for a in args if not a.startswith("configs=")]
     for mname, fac in FIX_MESHES.items():
         if only and mname not in only:
             continue
         m = fac()
         meshgen.write_msh(os.path.join(work, mname + ".msh"), m)
-        r = O.run_ref("mesh", work, mname)
-        np.savez_compressed(os.path.join(HERE, "mesh__%s.npz" % mname), **mesh_arrays(m),
-                            **{"ref_" + k: v for k, v in r.items() if k != "stdout"})
-        for cname, cfg in FIX_CONFIGS.items():
+        if only_cfg is None:
+            r = O.run_ref("mesh", work, mname)
+            np.savez_compressed(os.path.join(HERE, "mesh__%s.npz" % mname), **mesh_arrays(m),
+                                **{"ref_" + k: v for k, v in r.items() if k != "stdout"})
+        for cname in configs_of(mname):
+            if only_cfg is not None and cname not in only_cfg:
+                continue
+            cfg = ALL_CONFIGS[cname]
             m2, cfg2 = renamed(m, cfg)
             case = "%s__%s" % (mname, cname)
             meshgen.write_msh(os.path.join(work, case + ".msh"), m2)

@@ -68,6 +68,12 @@ CONFIGS = {
                                         "inlet": FREESTREAM_NACA, "outlet": FREESTREAM_NACA}),
     "grad_only_lin": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, grad=True, lim=False, lin=True,
                           gdt=True, bcs=BC_ALL),
+    # implicit residual smoothing (explicit.cpp:32-66, 743)
+    "smooth_first_split": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, smooth=True, bcs=BC_ALL),
+    "smooth_second_michalak": dict(flux="roe_split", dt="edge_min", cfl=1.0, phi="michalak", grad=True,
+                                   lim=True, lin=True, gdt=False, smooth=True,
+                                   bcs={"wall": ("mirror",), "transmissive": FREESTREAM_NACA,
+                                        "inlet": FREESTREAM_NACA, "outlet": ("copy",)}),
 }
 
 def airfoil_as_channel(m):

@@ -42,6 +42,8 @@ def test_run_bit_exact(mesh_name, cfg_name, order, flux_mode):
     cfg = CONFIGS[cfg_name]
     if flux_mode == "fused" and not cfg.get("lin"):
         pytest.skip("the fused stage pass exists for linear reconstruction only")
+    if flux_mode == "fused" and cfg.get("smooth"):
+        pytest.skip("residual smoothing needs every qt before any update: no fused stage pass")
     m, om, pm, s = _pair(mesh_name, cfg, renumber=order, flux_mode=flux_mode)
     if mesh_name in AIRFOIL_MESHES:
         q0 = airfoil_bump_state(om.cell_cx, om.cell_cy)
@@ -201,6 +203,79 @@ def test_fused_mode_is_refused_without_reconstruction():
     m = MESHES["sod"]()
     with pytest.raises(FvhError, match="fused"):
         Solver(Mesh.from_arrays(m), CONFIGS["first_split_sum"], flux_mode="fused")
+
+
+def test_fused_mode_is_refused_with_residual_smoothing():
+    from fvhyper_amd import FvhError
+    m = MESHES["step"]()
+    with pytest.raises(FvhError, match="smoothing"):
+        Solver(Mesh.from_arrays(m), CONFIGS["smooth_second_michalak"], flux_mode="fused")
+
+
+def test_residual_smoothing_changes_the_update():
+    """The smoothing configs really smooth: the same run without the flag
+    differs (guards against a flag that never reaches the kernels)."""
+    cfg = CONFIGS["smooth_second_michalak"]
+    plain = dict(cfg, smooth=False)
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    q0 = random_state(om.n_cells, seed=606, mach=0.3)
+    out = []
+    for c in (cfg, plain):
+        s = Solver(Mesh.from_arrays(m), c)
+        s.set_state(q0)
+        s.run(max_step=3)
+        out.append(s.get_state())
+    qo, *_ = om.run(cfg, q0, 3)
+    assert bits_equal(out[0], qo)
+    assert not bits_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("cfg_name", ["second_venkat_global", "smooth_second_michalak"])
+def test_restart_continues_the_run_bit_for_bit(tmp_path, precision, cfg_name):
+    """save() after 5 steps, load() into a fresh solver, run on: the same
+    bits, step, time and residuals as the uninterrupted 12-step run."""
+    print("case restart", precision, cfg_name, flush=True)
+    cfg = CONFIGS[cfg_name]
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    q0 = random_state(om.n_cells, seed=303, mach=0.3)
+    s = Solver(pm, cfg, precision=precision)
+    s.set_state(q0)
+    full = s.run(max_step=12, print_interval=5)
+    q_full = s.get_state()
+    s2 = Solver(pm, cfg, precision=precision)
+    s2.set_state(q0)
+    s2.run_begin()
+    mid = s2.run_continue(5, max_step=12, print_interval=5)
+    assert mid["step"] == 5
+    path = str(tmp_path / "state.fvhs")
+    s2.save(path)
+    s3 = Solver(pm, cfg, precision=precision)
+    s3.load(path)
+    assert s3.status()["step"] == 5
+    end = s3.run_continue(100, max_step=12, print_interval=5)
+    assert bits_equal(s3.get_state(), q_full)
+    for k in ("step", "stopped", "time", "time_prev", "R", "R0"):
+        assert bits_equal(np.asarray(end[k], dtype=np.float64), np.asarray(full[k], dtype=np.float64)), k
+
+
+def test_restart_rejects_another_mesh(tmp_path):
+    from fvhyper_amd import FvhError
+    cfg = CONFIGS["first_split_sum"]
+    a = Solver(Mesh.from_arrays(MESHES["sod"]()), cfg)
+    a.set_state(random_state(a.mesh.n_cells, seed=1))
+    path = str(tmp_path / "a.fvhs")
+    a.save(path)
+    b = Solver(Mesh.from_arrays(MESHES["mixed"]()), cfg)
+    with pytest.raises(FvhError, match="restart file holds"):
+        b.load(path)
+    with open(path, "r+b") as f:
+        f.write(b"XXXX")
+    with pytest.raises(FvhError, match="not a restart file"):
+        a.load(path)
 
 
 @pytest.mark.parametrize("flux_mode", ["tile", "fused"])

@@ -22,7 +22,7 @@ def _case_mesh(mesh_name, cfg_name):
     import gen_fixtures as G
     d = np.load(os.path.join(GOLDEN, "mesh__%s.npz" % mesh_name))
     m = _mesh_arrays(d)
-    return G.renamed(m, G.FIX_CONFIGS[cfg_name])
+    return G.renamed(m, G.ALL_CONFIGS[cfg_name])
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -61,8 +61,10 @@ def test_oracle_phases_match_reference(path):
     om = O.Mesh(m2)
     out = om.phases(cfg2, d["q"], alpha=float(d["alpha"]))
     assert bits_equal(out["dt"], d["dt"][::4])
-    for key in ("gx", "gy", "lim", "qmin", "qmax", "qt", "qk_cells", "qk", "R"):
+    for key in ("gx", "gy", "lim", "qmin", "qmax", "qt", "qt_raw", "qk_cells", "qk", "R"):
         if key in ("qmin", "qmax") and not cfg2.get("lim"):
+            continue
+        if key == "qt_raw" and not cfg2.get("smooth"):
             continue
         assert bits_equal(out[key], d[key]), key
 
@@ -84,7 +86,7 @@ def test_fixture_set_is_complete():
     import gen_fixtures as G
     for mn in G.FIX_MESHES:
         assert os.path.exists(os.path.join(GOLDEN, "mesh__%s.npz" % mn))
-        for cn in G.FIX_CONFIGS:
+        for cn in G.configs_of(mn):
             assert os.path.exists(os.path.join(GOLDEN, "phases__%s__%s.npz" % (mn, cn)))
             assert os.path.exists(os.path.join(GOLDEN, "run__%s__%s.npz" % (mn, cn)))
 

@@ -305,3 +305,70 @@ def test_rk_stage_coefficients_compose_as_low_storage_scheme():
     q, steps, t, *_ = om.run(cfg, q0, 1)
     assert steps == 1
     np.testing.assert_array_equal(q, one)
+
+
+def _smooth(om, qt):
+    qt = np.ascontiguousarray(qt, dtype=np.float64).copy()
+    O.lib().orc_smooth_residuals(om._p, qt)
+    return qt
+
+
+def test_residual_smoothing_is_two_jacobi_sweeps_of_the_implicit_system():
+    """Matrix form of the implicit smoothing (1 + n_e eps) r_i - eps sum_j r_j
+    = r0_i over real neighbours j (a boundary ghost's qt is 0): two Jacobi
+    sweeps from r0.  A dropped neighbour, a wrong sign or n_e off by one fails
+    at far above rounding."""
+    m = meshgen.mixed_channel(nx=12, ny=6)
+    om = O.Mesh(m)
+    nr, eps = om.n_real, 0.6
+    rng = np.random.default_rng(5)
+    r0 = np.zeros((om.n_cells, 4))
+    r0[:nr] = rng.standard_normal((nr, 4))
+    A = np.zeros((nr, nr))
+    ne = np.where(om.cell_tri[:nr] != 0, 3.0, 4.0)
+    for i, j in om.edge_cells.reshape(-1, 2):
+        if i < nr and j < nr:
+            A[i, j] += 1.0
+            A[j, i] += 1.0
+    # every cell has n_e edges, interior or boundary
+    deg = np.bincount(om.edge_cells.ravel(), minlength=om.n_cells)[:nr]
+    assert np.array_equal(deg, ne.astype(int))
+    D = (1.0 + ne * eps)[:, None]
+    r = r0[:nr].copy()
+    for _ in range(2):
+        r = (r0[:nr] + eps * (A @ r)) / D
+    out = _smooth(om, r0.ravel()).reshape(-1, 4)
+    np.testing.assert_allclose(out[:nr], r, rtol=1e-13, atol=1e-14)
+    assert np.all(out[nr:] == 0.0)
+
+
+def test_residual_smoothing_is_linear_and_keeps_interior_constants():
+    m = meshgen.forward_step(n_per_unit=10)
+    om = O.Mesh(m)
+    nr = om.n_real
+    rng = np.random.default_rng(9)
+    x = np.zeros((om.n_cells, 4))
+    y = np.zeros((om.n_cells, 4))
+    x[:nr] = rng.standard_normal((nr, 4))
+    y[:nr] = rng.standard_normal((nr, 4))
+    sx, sy = _smooth(om, x.ravel()), _smooth(om, y.ravel())
+    np.testing.assert_allclose(_smooth(om, (2.0 * x - 3.0 * y).ravel()), 2.0 * sx - 3.0 * sy, rtol=1e-12,
+                               atol=1e-12)
+    # a constant field is a fixed point except where a boundary ghost's zero
+    # reaches in two sweeps: the boundary cells and their real neighbours
+    c = np.zeros((om.n_cells, 4))
+    c[:nr] = [1.0, -2.0, 0.5, 3.0]
+    sc = _smooth(om, c.ravel()).reshape(-1, 4)
+    ec = om.edge_cells.reshape(-1, 2)
+    near = np.zeros(om.n_cells, dtype=bool)
+    near[ec[ec[:, 1] >= nr, 0]] = True
+    inner = (ec[:, 0] < nr) & (ec[:, 1] < nr)
+    grow = near.copy()
+    np.logical_or.at(grow, ec[inner, 0], near[ec[inner, 1]])
+    np.logical_or.at(grow, ec[inner, 1], near[ec[inner, 0]])
+    near = grow
+    far = ~near[:nr]
+    assert far.sum() > 50
+    np.testing.assert_allclose(sc[:nr][far], c[:nr][far], rtol=1e-14)
+    # next to a wall the ghost's zero pulls the value down
+    assert np.all(sc[:nr][near[:nr], 0] < 1.0)
