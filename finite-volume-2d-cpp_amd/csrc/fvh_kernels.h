@@ -118,7 +118,9 @@ struct HaloView {
 
 struct PhysParams {
     double gamma, cfl, dt_fixed, K, phi_yt;
+    double mu, pr, cp, r;   // laminar viscous constants (viscous flux and time steps)
     int positivity;   // first-order fallback for non-physical face states
+    int diffusive;    // face gradients for the viscous flux
 };
 
 // Launchers (implemented in fvh_kernels.hip).  All take the solver stream;
@@ -130,7 +132,9 @@ void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_ki
 void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells,
                    int dt_is_f32, void* dt_global, const Ctrl* ctrl);
 // Gradient + limiter + face states; with dt_kind >= 0 also the time step of
-// every cell and per-tile minima (first stage of a step).
+// every cell and per-tile minima (first stage of a step).  dbg_gx/dbg_gy
+// (when non-null) receive every cell's gradient, zero for ghosts; dbg_lim
+// (when non-null) its limiter.
 template <class T>
 void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
                     bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr,
@@ -139,12 +143,14 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
 // Flux + RK update + boundary update; tiles == nullptr selects the per-cell
 // form that computes every interior edge flux from both sides.  dt_inst
 // (global time step per instance) replaces the per-cell dt when non-null.
+// flux_kind: FVH_FLUX_* (0 split, 1 joined, 2 split + viscous terms; the
+// viscous kind reads the cell gradients gxc, gyc when pp.diffusive).
 template <class T>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles,
-                 const PhysParams& pp, bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
+                 const PhysParams& pp, int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
                  const Q4<T>* q0, Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
                  const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl,
-                 Q4<T>* qt_only = nullptr);
+                 Q4<T>* qt_only = nullptr, const Q4<T>* gxc = nullptr, const Q4<T>* gyc = nullptr);
 // One sweep of implicit residual smoothing (explicit.cpp:32-66) over the qt
 // that a flux pass with qt_only stored: qt_out != nullptr stores the sweep's
 // qt; qt_out == nullptr finishes the stage from it (update, boundary update,

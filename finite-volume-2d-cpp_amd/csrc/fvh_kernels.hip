@@ -31,6 +31,36 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
         }
         return T(pp.cfl) * m.c_area[c] / s;
     }
+    if (KIND == 4) {
+        // summed viscous form: cfl*A / sum_e max(eig_i + 2 eigv_i, eig_j + 2 eigv_j)*len
+        // (nacaTransient/main.cpp:243-285)
+        const T mu = T(pp.mu), pr = T(pp.pr);
+        T s = T(0);
+        for (int h = h0; h < h1; ++h) {
+            const uint32_t ue = m.he_edge[h];
+            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+            const int nb = m.he_nbr[h];
+            const Q4<T> qn = ld(q + nb);
+            const bool side = (ue & kSideBit) != 0;
+            const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]) + T(2) * eig_visc(g, mu, pr, qc.v, geo.v[2], m.c_area[c]);
+            const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]) + T(2) * eig_visc(g, mu, pr, qn.v, geo.v[2], m.c_area[nb]);
+            s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+        }
+        return T(pp.cfl) * m.c_area[c] / s;
+    }
+    if (KIND == 3) {
+        // edge-minimum viscous local form (flatPlate/main.cpp:239-277)
+        const T mu = T(pp.mu), pr = T(pp.pr);
+        const T A = m.c_area[c];
+        T r = T(1);
+        for (int h = h0; h < h1; ++h) {
+            const Q4<T> geo = m.e_geo[m.he_edge[h] & kEdgeMask];
+            const T ec = eig_abs(g, qc.v, geo.v[0], geo.v[1]) * geo.v[2];
+            const T ev = eig_visc(g, mu, pr, qc.v, geo.v[2], A);
+            r = smin(r, T(pp.cfl) * A / (ec + T(2) * ev));
+        }
+        return r;
+    }
     // edge-minimum local form (naca/main.cpp:151-182)
     T r = T(1);
     const T A = m.c_area[c];
@@ -99,7 +129,7 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
     T qmn[4], qmx[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
-    T dts = (DT == 2) ? T(1) : T(0);
+    T dts = (DT == 2 || DT == 3) ? T(1) : T(0);
     // one pass over the half-edges feeds the Green-Gauss sums (edge order,
     // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
     // time-step sum or minimum
@@ -117,6 +147,18 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
             } else if (DT == 2 && do_dt) {
                 const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
                 dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
+            } else if (DT == 3 && do_dt) {
+                const T A = m.c_area[c];
+                const T ec = eig_abs(g, qc.v, geo.v[0], geo.v[1]) * geo.v[2];
+                const T ev = eig_visc(g, T(pp.mu), T(pp.pr), qc.v, geo.v[2], A);
+                dts = smin(dts, T(pp.cfl) * A / (ec + T(2) * ev));
+            } else if (DT == 4 && do_dt) {
+                const T mu = T(pp.mu), pr = T(pp.pr);
+                const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]) +
+                             T(2) * eig_visc(g, mu, pr, qc.v, geo.v[2], m.c_area[c]);
+                const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]) +
+                             T(2) * eig_visc(g, mu, pr, qn.v, geo.v[2], m.c_area[m.he_nbr[h]]);
+                dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
             }
             if (!real) continue;
             if (lim_on) {
@@ -144,7 +186,7 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
             }
         }
     }
-    if (DT && do_dt) dt_out = (DT == 1) ? T(pp.cfl) * m.c_area[c] / dts : dts;
+    if (DT && do_dt) dt_out = (DT == 1 || DT == 4) ? T(pp.cfl) * m.c_area[c] / dts : dts;
     if (real && grad) {
         const T ia = m.c_inv_area[c];
 #pragma unroll
@@ -232,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
             for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
             dbg_gx[c] = a;
             dbg_gy[c] = b;
-            dbg_lim[c] = d;
+            if (dbg_lim) dbg_lim[c] = d;
         }
         if (lin) {
             for (int h = h0; h < h1; ++h) {
@@ -304,8 +346,29 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
     }
 }
 
+// Edge flux of kind FK (FVH_FLUX_*) from the two face states; the viscous
+// kind adds the laminar terms with the face gradient of cells ci (side 0)
+// and cj (side 1), or a zero gradient without pp.diffusive
+// (explicit.cpp:324-344).
+template <class T, int FK>
+__device__ inline void edge_flux(const MeshView<T>& m, const PhysParams& pp, T g, T* f, const T* qi, const T* qj,
+                                 T n0, T n1, int ci, int cj, const Q4<T>* __restrict__ qcell,
+                                 const Q4<T>* __restrict__ gxc, const Q4<T>* __restrict__ gyc) {
+    roe_flux<T, FK == 1>(g, f, qi, qj, n0, n1);
+    if (FK == 2) {
+        T gxv[4] = {T(0), T(0), T(0), T(0)}, gyv[4] = {T(0), T(0), T(0), T(0)};
+        if (pp.diffusive) {
+            const Q4<T> gxi = ld(gxc + ci), gyi = ld(gyc + ci), gxj = ld(gxc + cj), gyj = ld(gyc + cj);
+            const Q4<T> qa = ld(qcell + ci), qb = ld(qcell + cj);
+            const G2<T> pa = m.c_c[ci], pb = m.c_c[cj];
+            face_gradient(gxv, gyv, gxi.v, gyi.v, gxj.v, gyj.v, qa.v, qb.v, pa.v[0], pa.v[1], pb.v[0], pb.v[1]);
+        }
+        viscous_terms(g, T(pp.mu), T(pp.pr), T(pp.cp), T(pp.r), f, qi, qj, gxv, gyv, n0, n1);
+    }
+}
+
 // Per-cell form: every interior edge flux is computed by both of its cells.
-template <class T, bool JOINED, bool LIN>
+template <class T, int FK, bool LIN>
 __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm, PhysParams pp,
                                                       const BcParam<T>* __restrict__ bcs,
                                                       const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
@@ -314,7 +377,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
                                                       Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl,
-                                                      Q4<T>* __restrict__ qt_only) {
+                                                      Q4<T>* __restrict__ qt_only, const Q4<T>* __restrict__ gxc,
+                                                      const Q4<T>* __restrict__ gyc) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int inst = tm.inst[lb];
@@ -348,7 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                 qi[k] = side ? b.v[k] : a.v[k];
                 qj[k] = side ? a.v[k] : b.v[k];
             }
-            roe_flux<T, JOINED>(g, f, qi, qj, geo.v[0], geo.v[1]);
+            const int nb = m.he_nbr[h];
+            edge_flux<T, FK>(m, pp, g, f, qi, qj, geo.v[0], geo.v[1], side ? nb : c, side ? c : nb, qin, gxc, gyc);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const T t = f[k] * geo.v[2] / A;
@@ -379,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
 // edge-parallel, into LDS; phase 2 gathers each cell's edges from LDS in
 // reference edge order.  Edges on a tile border are computed by both tiles
 // (same inputs, same bits).
-template <class T, bool JOINED, bool LIN>
+template <class T, int FK, bool LIN>
 __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm, TileView tv, PhysParams pp,
                                                       const BcParam<T>* __restrict__ bcs,
                                                       const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
@@ -388,7 +453,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
                                                       Q4<T>* __restrict__ dbg_qt, const Ctrl* __restrict__ ctrl,
-                                                      Q4<T>* __restrict__ qt_only) {
+                                                      Q4<T>* __restrict__ qt_only, const Q4<T>* __restrict__ gxc,
+                                                      const Q4<T>* __restrict__ gyc) {
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
     const int inst = tm.inst[lb];
@@ -413,7 +479,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
             b = ld(qin + cc.y);
         }
         T f[4];
-        roe_flux<T, JOINED>(g, f, a.v, b.v, geo.v[0], geo.v[1]);
+        const int2 ce = tv.cell[k];
+        edge_flux<T, FK>(m, pp, g, f, a.v, b.v, geo.v[0], geo.v[1], ce.x, ce.y, qin, gxc, gyc);
         Q4<T> r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
@@ -766,6 +833,8 @@ void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_ki
     if (nb == 0) return;
     if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 3) hipLaunchKernelGGL((k_dt<T, 3>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 4) hipLaunchKernelGGL((k_dt<T, 4>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else hipLaunchKernelGGL((k_dt<T, 2>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
 }
 
@@ -792,6 +861,8 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
     do {                                            \
         if (dt_kind == 1) FVH_GL(P, 1);             \
         else if (dt_kind == 2) FVH_GL(P, 2);        \
+        else if (dt_kind == 3) FVH_GL(P, 3);        \
+        else if (dt_kind == 4) FVH_GL(P, 4);        \
         else FVH_GL(P, 0);                          \
     } while (0)
     switch (phi_kind) {
@@ -806,30 +877,35 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
 
 template <class T>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles, const PhysParams& pp,
-                 bool joined, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
+                 int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
                  const Q4<T>* face, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
-                 Q4<T>* dbg_qt, const Ctrl* ctrl, Q4<T>* qt_only) {
+                 Q4<T>* dbg_qt, const Ctrl* ctrl, Q4<T>* qt_only, const Q4<T>* gxc, const Q4<T>* gyc) {
     const int nb = xcd_grid_blocks(tm.n_rt);
     if (nb == 0) return;
     if (tiles) {
         const TileView tv = *tiles;
 #define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, tv, pp, bcs, \
                                         qin, q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, \
-                                        qt_only)
-        if (joined) {
-            if (lin) FVH_FT(true, true); else FVH_FT(true, false);
+                                        qt_only, gxc, gyc)
+        if (flux_kind == 1) {
+            if (lin) FVH_FT(1, true); else FVH_FT(1, false);
+        } else if (flux_kind == 2) {
+            if (lin) FVH_FT(2, true); else FVH_FT(2, false);
         } else {
-            if (lin) FVH_FT(false, true); else FVH_FT(false, false);
+            if (lin) FVH_FT(0, true); else FVH_FT(0, false);
         }
 #undef FVH_FT
         return;
     }
 #define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qin, \
-                                        q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, qt_only)
-    if (joined) {
-        if (lin) FVH_FX(true, true); else FVH_FX(true, false);
+                                        q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, qt_only, \
+                                        gxc, gyc)
+    if (flux_kind == 1) {
+        if (lin) FVH_FX(1, true); else FVH_FX(1, false);
+    } else if (flux_kind == 2) {
+        if (lin) FVH_FX(2, true); else FVH_FX(2, false);
     } else {
-        if (lin) FVH_FX(false, true); else FVH_FX(false, false);
+        if (lin) FVH_FX(0, true); else FVH_FX(0, false);
     }
 #undef FVH_FX
 }
@@ -934,9 +1010,9 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
                                     bool, bool, bool, const Q4<T>*, Q4<T>*, Q4<T>*, int, T*, double*, Q4<T>*, \
                                     Q4<T>*, Q4<T>*, const Ctrl*);                                            \
     template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView*,            \
-                                 const PhysParams&, bool, bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, \
+                                 const PhysParams&, int, bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*,  \
                                  Q4<T>*, const Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,  \
-                                 const Ctrl*, Q4<T>*);                                                       \
+                                 const Ctrl*, Q4<T>*, const Q4<T>*, const Q4<T>*);                                                       \
     template void launch_smooth<T>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&, bool, \
                                    const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,       \
                                    Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,             \

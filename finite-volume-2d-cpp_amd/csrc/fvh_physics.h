@@ -123,7 +123,10 @@ FVH_HD T phi(T y, T yt) {
 //  farfield characteristic:     examples/naca/main.cpp:192-259
 // Both characteristic kinds keep the reference's b[2] = b[1]*(...) spelling
 // in their subsonic branches; the freestream (rho, u, v, p) is a parameter.
-enum BcKind { BC_COPY = 0, BC_MIRROR = 1, BC_INLET_OUTLET = 2, BC_FARFIELD = 3 };
+//  no-slip wall: examples/flatPlate/main.cpp:361-366
+//  farfield, b[0] spelling: examples/flatPlate/main.cpp:286-353 (freestream a
+//  parameter; its subsonic outlet keeps the reference's v_a)
+enum BcKind { BC_COPY = 0, BC_MIRROR = 1, BC_INLET_OUTLET = 2, BC_FARFIELD = 3, BC_NOSLIP = 4, BC_FARFIELD_V = 5 };
 
 template <class T>
 struct BcParam {
@@ -142,6 +145,13 @@ FVH_HD void boundary(const BcParam<T>& P, T g, T* b, const T* q, T n0, T n1) {
         b[0] = q[0];
         b[1] = q[1] - T(2) * n0 * un;
         b[2] = q[2] - T(2) * n1 * un;
+        b[3] = q[3];
+        return;
+    }
+    if (P.kind == BC_NOSLIP) {
+        b[0] = q[0];
+        b[1] = -q[1];
+        b[2] = -q[2];
         b[3] = q[3];
         return;
     }
@@ -184,14 +194,14 @@ FVH_HD void boundary(const BcParam<T>& P, T g, T* b, const T* q, T n0, T n1) {
             const T pb = T(0.5) * (pa + pd - rho0 * c0 * (n0 * (ua - ud) + n1 * (va - vd)));
             b[0] = rhoa + (pb - pa) / (c0 * c0);
             b[1] = b[0] * (ua - n0 * (pa - pb) / (rho0 * c0));
-            b[2] = b[1] * (va - n1 * (pa - pb) / (rho0 * c0));
+            b[2] = (P.kind == BC_FARFIELD_V ? b[0] : b[1]) * (va - n1 * (pa - pb) / (rho0 * c0));
             b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
         }
     } else {
         const T pb = pa;
         b[0] = rhod + (pb - pd) / (c0 * c0);
         b[1] = b[0] * (ud + n0 * (pd - pb) / (rho0 * c0));
-        b[2] = b[1] * (va + n1 * (pd - pb) / (rho0 * c0));
+        b[2] = (P.kind == BC_FARFIELD_V ? b[0] : b[1]) * (va + n1 * (pd - pb) / (rho0 * c0));
         b[3] = pb / (g - T(1)) + T(0.5) / b[0] * (b[1] * b[1] + b[2] * b[2]);
     }
 }
@@ -208,6 +218,80 @@ template <class T>
 FVH_HD T eig_edge(T g, const T* q, T n0, T n1) {
     const T c = sqrt(pressure(g, q) * g / q[0]);
     return (c + sabs(q[1] / q[0])) * n0 + (c + sabs(q[2] / q[0])) * n1;
+}
+
+//  flatPlate local form (flatPlate/main.cpp:261-262): (c + |n0 u| + |n1 v|)
+template <class T>
+FVH_HD T eig_abs(T g, const T* q, T n0, T n1) {
+    const T c = sqrt(pressure(g, q) * g / q[0]);
+    return c + sabs(n0 * q[1] / q[0]) + sabs(n1 * q[2] / q[0]);
+}
+//  viscous spectral term of both viscous forms (flatPlate/main.cpp:264-267):
+//  max(4/(3 rho), gamma/rho) (mu/pr) len^2 / A
+template <class T>
+FVH_HD T eig_visc(T g, T mu, T pr, const T* q, T len, T A) {
+    return smax(T(4) / (T(3) * q[0]), g / q[0]) * (mu / pr) * len * len / A;
+}
+
+// gradient_for_diffusion (explicit.cpp:70-109): the mean of the two cell
+// gradients with its component along the centre line replaced by the
+// difference quotient of the cell states.
+template <class T>
+FVH_HD void face_gradient(T* gxf, T* gyf, const T* gxi, const T* gyi, const T* gxj, const T* gyj, const T* qi,
+                          const T* qj, T cix, T ciy, T cjx, T cjy) {
+    T t0 = cix - cjx;
+    T t1 = ciy - cjy;
+    const T l = sqrt(t0 * t0 + t1 * t1);
+    t0 = t0 / l;
+    t1 = t1 / l;
+    for (int k = 0; k < 4; ++k) {
+        const T gd = (qi[k] - qj[k]) / l;
+        const T gxb = (gxi[k] + gxj[k]) * T(0.5);
+        const T gyb = (gyi[k] + gyj[k]) * T(0.5);
+        const T dot = gxb * t0 + gyb * t1;
+        gxf[k] = gxb - (dot - gd) * t0;
+        gyf[k] = gyb - (dot - gd) * t1;
+    }
+}
+
+// Laminar viscous terms subtracted from the Roe flux (flatPlate/main.cpp:
+// 193-224 with calc_grad_p/t/u/v of :43-92): the stresses and heat flux of
+// the mean face state qc = (qi + qj)/2 and the conservative face gradient
+// (gx, gy).
+template <class T>
+FVH_HD void viscous_terms(T g, T mu, T pr, T cp, T r, T* f, const T* qi, const T* qj, const T* gx, const T* gy,
+                          T n0, T n1) {
+    T qc[4];
+    for (int k = 0; k < 4; ++k) qc[k] = T(0.5) * (qi[k] + qj[k]);
+    const T q00 = qc[0] * qc[0];
+    // pressure gradient (calc_grad_p)
+    T gp0 = T(2) * gx[3];
+    gp0 = gp0 - (gx[1] * qc[1] / qc[0] + qc[1] * (gx[1] * qc[0] - gx[0] * qc[1]) / q00);
+    gp0 = gp0 - (gx[2] * qc[2] / qc[0] + qc[2] * (gx[2] * qc[0] - gx[0] * qc[2]) / q00);
+    gp0 = gp0 * (T(0.5) * (g - T(1)));
+    T gp1 = T(2) * gy[3];
+    gp1 = gp1 - (gy[1] * qc[1] / qc[0] + qc[1] * (gy[1] * qc[0] - gy[0] * qc[1]) / q00);
+    gp1 = gp1 - (gy[2] * qc[2] / qc[0] + qc[2] * (gy[2] * qc[0] - gy[0] * qc[2]) / q00);
+    gp1 = gp1 * (T(0.5) * (g - T(1)));
+    // temperature gradient (calc_grad_t)
+    const T p = pressure(g, qc);
+    const T gt0 = (T(1) / r) * ((gp0 * qc[0] - gx[0] * p) / q00);
+    const T gt1 = (T(1) / r) * ((gp1 * qc[0] - gy[0] * p) / q00);
+    // velocity gradients (calc_grad_u, calc_grad_v)
+    const T gu0 = (qc[0] * gx[1] - qc[1] * gx[0]) / q00;
+    const T gu1 = (qc[0] * gy[1] - qc[1] * gy[0]) / q00;
+    const T gv0 = (qc[0] * gx[2] - qc[2] * gx[0]) / q00;
+    const T gv1 = (qc[0] * gy[2] - qc[2] * gy[0]) / q00;
+    const T div_v = gu0 + gv1;
+    const T txx = T(2) * mu * (gu0 - div_v / T(3));
+    const T tyy = T(2) * mu * (gv1 - div_v / T(3));
+    const T txy = mu * (gu1 + gv0);
+    const T k = cp * mu / pr;
+    const T ph0 = qc[1] / qc[0] * txx + qc[2] / qc[0] * txy + k * gt0;
+    const T ph1 = qc[1] / qc[0] * txy + qc[2] / qc[0] * tyy + k * gt1;
+    f[1] = f[1] - (n0 * txx + n1 * txy);
+    f[2] = f[2] - (n0 * txy + n1 * tyy);
+    f[3] = f[3] - (n0 * ph0 + n1 * ph1);
 }
 
 }  // namespace fvh

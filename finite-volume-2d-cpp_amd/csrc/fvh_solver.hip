@@ -102,6 +102,7 @@ struct fvh_solver {
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
     fvh::DevBuf qt_s[2];                     // residual smoothing: raw qt, first sweep's qt
+    fvh::DevBuf gxy[2];                      // viscous flux: cell gradients gx, gy
     int cur = 0;                          // index of the current state buffer
     // discrete adjoint: per device edge (half-edge side 0, side 1, cell side 0,
     // cell side 1); boundary element geometry (instance 0); work buffers
@@ -226,6 +227,11 @@ PhysParams phys(const fvh_config& c) {
     p.K = c.K;
     p.phi_yt = c.phi_yt;
     p.positivity = (c.flags & FVH_FLAG_POSITIVITY) ? 1 : 0;
+    p.mu = c.mu;
+    p.pr = c.pr;
+    p.cp = c.cp;
+    p.r = c.r_gas;
+    p.diffusive = (c.flags & FVH_FLAG_DIFFUSIVE) ? 1 : 0;
     return p;
 }
 
@@ -351,6 +357,15 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
                 set_error("hipMalloc failed");
                 return FVH_E_HIP;
             }
+    if (s->cfg.flux == FVH_FLUX_ROE_VISCOUS && (s->cfg.flags & FVH_FLAG_DIFFUSIVE))
+        for (auto& b : s->gxy) {
+            // zero until a gradient pass writes them (and for good without one)
+            if (b.alloc((size_t)N * sizeof(Q4<T>))) {
+                set_error("hipMalloc failed");
+                return FVH_E_HIP;
+            }
+            FVH_HIP(hipMemset(b.p, 0, b.bytes));
+        }
     FVH_HIP(hipMemset(s->ctrl.p, 0, s->ctrl.bytes));
     FVH_HIP(hipMemset(s->face.p, 0, s->face.bytes));
     FVH_HIP(hipMemset(s->bincr.p, 0, s->bincr.bytes));
@@ -389,6 +404,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     double* blk = (double*)s->blk_min.p;
     Q4<T>* face = (Q4<T>*)s->face.p;
     Q4<T>* bincr = (Q4<T>*)s->bincr.p;
+    Q4<T>* gx_c = (Q4<T>*)s->gxy[0].p;   // null unless the viscous flux takes face gradients
+    Q4<T>* gy_c = (Q4<T>*)s->gxy[1].p;
     const bool fused = s->fused && !dbg;
     if (!s->second || (fused && gdt)) {
         Bracket b(s, K_DT);
@@ -419,7 +436,7 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
             // the first stage's pass also computes the time step from q0
             Bracket b(s, K_GRADLIM);
             launch_gradlim<T>(st, m, tm, pp, c.phi, grad, lim, lin, qin, face, bincr, k == 0 ? c.dt_kind : -1, dt,
-                              blk, dbg ? dbg[0] : nullptr, dbg ? dbg[1] : nullptr, dbg ? dbg[2] : nullptr, cc);
+                              blk, dbg ? dbg[0] : gx_c, dbg ? dbg[1] : gy_c, dbg ? dbg[2] : nullptr, cc);
         }
         if (k == 0 && gdt) {
             Bracket b(s, K_DTMIN);
@@ -428,14 +445,16 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         const bool last = k == n_stage - 1;
         const T a = T(alpha_override ? *alpha_override : c.alpha[k]);
         double* res = (last && need_res) ? (double*)s->res_blk.p : nullptr;
+        const Q4<T>* gxv = dbg ? dbg[0] : gx_c;
+        const Q4<T>* gyv = dbg ? dbg[1] : gy_c;
         if (smooth) {
             // flux pass stores qt; two smoothing sweeps, the second finishes the stage
             Q4<T>* qs0 = (Q4<T>*)s->qt_s[0].p;
             Q4<T>* qs1 = (Q4<T>*)s->qt_s[1].p;
             {
                 Bracket b(s, K_FLUX);
-                launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
-                               qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr, a, nullptr, nullptr, cc, qs0);
+                launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs, qin, q0, out, face,
+                               bincr, dt, gdt ? dtg : nullptr, a, nullptr, nullptr, cc, qs0, gxv, gyv);
             }
             Bracket b(s, K_SMOOTH);
             launch_smooth<T>(st, m, tm, pp, lin, bcs, qs0, qs0, qs1, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
@@ -444,8 +463,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
                              res, dbg ? dbg[3] : nullptr, cc);
         } else {
             Bracket b(s, K_FLUX);
-            launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
-                           qin, q0, out, face, bincr, dt, gdt ? dtg : nullptr, a, res, dbg ? dbg[3] : nullptr, cc);
+            launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs, qin, q0, out, face, bincr,
+                           dt, gdt ? dtg : nullptr, a, res, dbg ? dbg[3] : nullptr, cc, nullptr, gxv, gyv);
         }
         qin = out;
         out = (out == qa) ? qb : qa;
@@ -800,7 +819,7 @@ int32_t check_config(const fvh_config* c) {
         return FVH_E_ARG;
     }
     if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
-    if (c->flux < 0 || c->flux > 1 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 2 ||
+    if (c->flux < 0 || c->flux > 2 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 4 ||
         c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 3) {
         set_error("unknown flux, limiter, time-step, order or flux-mode kind");
         return FVH_E_ARG;
@@ -810,7 +829,13 @@ int32_t check_config(const fvh_config* c) {
         return FVH_E_ARG;
     }
     for (int k = 0; k < c->n_bc; ++k)
-        if (c->bc[k].kind < 0 || c->bc[k].kind > 3) { set_error("unknown boundary kind"); return FVH_E_ARG; }
+        if (c->bc[k].kind < 0 || c->bc[k].kind > 5) { set_error("unknown boundary kind"); return FVH_E_ARG; }
+    const bool visc = c->flux == FVH_FLUX_ROE_VISCOUS || c->dt_kind == FVH_DT_EDGE_MIN_VISCOUS ||
+                      c->dt_kind == FVH_DT_SUM_VISCOUS;
+    if (visc && (!(c->pr != 0.0) || !(c->r_gas != 0.0))) {
+        set_error("viscous flux or time step: pr and r_gas must be non-zero");
+        return FVH_E_ARG;
+    }
     return FVH_OK;
 }
 
@@ -819,7 +844,8 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
     if (a.precision != b.precision || a.flux != b.flux || a.gamma != b.gamma || a.phi != b.phi ||
         a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
         a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
-        a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells)
+        a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells || a.mu != b.mu || a.pr != b.pr ||
+        a.cp != b.cp || a.r_gas != b.r_gas)
         return false;
     for (int k = 0; k < a.n_alpha; ++k)
         if (a.alpha[k] != b.alpha[k]) return false;
@@ -1064,16 +1090,18 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     HaloHost hh;
     const bool lin_on = (c.flags & FVH_FLAG_LINEAR) != 0;
     s->fused = false;
-    // residual smoothing needs every cell's qt before any update: not fusable
+    // residual smoothing needs every cell's qt before any update: not fusable;
+    // the viscous flux and time steps exist in the two-pass forms only
+    const bool fusable = !(c.flags & FVH_FLAG_SMOOTH) && c.flux != FVH_FLUX_ROE_VISCOUS && c.dt_kind <= FVH_DT_EDGE_MIN;
     if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
-        lin_on && !(c.flags & FVH_FLAG_SMOOTH) && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
+        lin_on && fusable && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
         const size_t qb = s->f32 ? 16 : 32;
         const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * qb;
         s->fused = lds <= 160 * 1024;
     }
     if (c.flux_mode == FVH_FLUX_MODE_FUSED && !s->fused && s->n_real > 0) {
-        set_error("fused stage pass unavailable: needs linear reconstruction, no residual smoothing and a tile "
-                  "halo that fits in LDS");
+        set_error("fused stage pass unavailable: needs linear reconstruction, an inviscid flux and time step, no "
+                  "residual smoothing and a tile halo that fits in LDS");
         return FVH_E_UNSUPPORTED;
     }
     if (s->fused) {
@@ -1287,7 +1315,7 @@ struct Adjoint {
                                        (Q4<double>*)s->bincr.p, k == 0 ? c.dt_kind : -1, dt, blk,
                                        (Q4<double>*)R.gx[k], (Q4<double>*)R.gy[k], (Q4<double>*)R.lm[k], nullptr);
             if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
-            launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux == FVH_FLUX_ROE_JOINED, lin, bcs,
+            launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs,
                                 qin, (const Q4<double>*)R.qs[0], (Q4<double>*)R.qs[k + 1],
                                 (const Q4<double>*)s->face.p, (const Q4<double>*)s->bincr.p, dt,
                                 gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, (Q4<double>*)R.qt[k],
@@ -1505,6 +1533,15 @@ int32_t adjoint_check(fvh_solver* s) {
         set_error("the adjoint does not take residual smoothing");
         return FVH_E_UNSUPPORTED;
     }
+    if (c.flux == FVH_FLUX_ROE_VISCOUS || c.dt_kind > FVH_DT_EDGE_MIN) {
+        set_error("the adjoint takes the inviscid fluxes and time steps");
+        return FVH_E_UNSUPPORTED;
+    }
+    for (int k = 0; k < c.n_bc; ++k)
+        if (c.bc[k].kind > FVH_BC_FARFIELD) {
+            set_error("the adjoint takes the copy, mirror, inlet/outlet and farfield boundaries");
+            return FVH_E_UNSUPPORTED;
+        }
     return FVH_OK;
 }
 
@@ -1537,6 +1574,11 @@ void fvh_config_default(fvh_config* c) {
     c->renumber = FVH_ORDER_TILES;
     c->flux_mode = FVH_FLUX_MODE_AUTO;
     c->tile_cells = kBlock;
+    // consts:: of examples/flatPlate/main.cpp:28-35
+    c->mu = 2e-5;
+    c->pr = 0.72;
+    c->cp = 1.0;
+    c->r_gas = 8.3145;
 }
 
 int32_t fvh_solver_create(const fvh_mesh* m, const fvh_config* c, fvh_solver** out) {

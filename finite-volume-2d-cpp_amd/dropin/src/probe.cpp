@@ -61,7 +61,65 @@ void conserved(double* q, double rho, double u, double v, double p) {
 }
 
 // ---- flux: Roe spelling and gamma ------------------------------------------
-struct FluxKind { int joined; double gamma; };
+struct Visc { double pr, cp, r; };
+struct FluxKind {
+    int kind;                 // FVH_FLUX_*
+    double gamma;
+    double mu;                // viscous kind only
+    std::vector<Visc> visc;   // (pr, cp, r) that reproduce the viscous terms bit for bit
+};
+
+// Laminar terms: the user's flux with a gradient, against the device kind.
+// A shear layer gives mu; heat conduction gives cp/(pr r), and every
+// (pr, cp, r) from a list of usual values and decimal roundings that
+// reproduces the flux on the battery is kept (pr is settled by calc_dt).
+void probe_viscous(FluxKind& fk, Rng& r, int NS, const std::vector<double>& qi, const std::vector<double>& qj,
+                   const std::vector<double>& nn) {
+    const double g = fk.gamma, zero[4] = {0, 0, 0, 0};
+    std::vector<double> gx(4 * NS), gy(4 * NS), fu(4 * NS);
+    for (int k = 0; k < 4 * NS; ++k) { gx[k] = r.in(-2, 2); gy[k] = r.in(-2, 2); }
+    for (int k = 0; k < NS; ++k) calc_flux(&fu[4 * k], &qi[4 * k], &qj[4 * k], &gx[4 * k], &gy[4 * k], &nn[2 * k]);
+    // shear u = a y at constant rho, p: f1 changes by -mu a through n = (0, 1)
+    double q[4], f0[4], f1[4];
+    conserved(q, 1.3, 0.4, 0.0, 0.9);
+    q[3] = 0.9 / (g - 1) + 0.5 * (q[1] * q[1]) / q[0];
+    const double a = 1.0, n[2] = {0.0, 1.0};
+    const double gys[4] = {0.0, q[0] * a, 0.0, q[1] * a};
+    calc_flux(f0, q, q, zero, zero, n);
+    calc_flux(f1, q, q, zero, gys, n);
+    const double mu_est = -(f1[1] - f0[1]) / a;
+    // heat conduction at rest: f3 changes by -(cp mu / pr) (dp/dx) / (rho r)
+    conserved(q, 1.3, 0.0, 0.0, 0.9);
+    q[3] = 0.9 / (g - 1);
+    const double b = 0.7, nx[2] = {1.0, 0.0};
+    const double gxs[4] = {0.0, 0.0, 0.0, b / (g - 1)};
+    calc_flux(f0, q, q, zero, zero, nx);
+    calc_flux(f1, q, q, gxs, zero, nx);
+    const double heat = -(f1[3] - f0[3]) * q[0] / b;   // cp mu / (pr r)
+    auto matches = [&](double mu, double pr, double cp, double rg) {
+        for (int k = 0; k < NS; ++k) {
+            double fd[4];
+            fvh::roe_flux<double, false>(g, fd, &qi[4 * k], &qj[4 * k], nn[2 * k], nn[2 * k + 1]);
+            fvh::viscous_terms<double>(g, mu, pr, cp, rg, fd, &qi[4 * k], &qj[4 * k], &gx[4 * k], &gy[4 * k],
+                                       nn[2 * k], nn[2 * k + 1]);
+            if (!same(fd, &fu[4 * k], 4)) return false;
+        }
+        return true;
+    };
+    for (double mu : nice(mu_est)) {
+        if (!(mu > 0)) continue;
+        for (double pr : {0.72, 0.71, 0.7, 0.73, 0.75, 0.9, 1.0, 0.5})
+            for (double cp : {1.0, 1004.5, 1005.0, 1004.0, 1006.0, 1004.68506, 718.0, 717.5})
+                for (double rg : nice(cp * mu / pr / heat))
+                    if (fk.visc.size() < 64 && matches(mu, pr, cp, rg)) {
+                        fk.mu = mu;
+                        fk.visc.push_back({pr, cp, rg});
+                    }
+        if (!fk.visc.empty()) return;
+    }
+    throw std::runtime_error("calc_flux: depends on the gradients, but no laminar viscous kind reproduces it (the "
+                             "library implements the viscous terms of the flatPlate/nacaTransient examples)");
+}
 
 FluxKind probe_flux() {
     Rng r(11);
@@ -93,11 +151,26 @@ FluxKind probe_flux() {
                 else fvh::roe_flux<double, false>(g, fd, &qi[4 * k], &qj[4 * k], nn[2 * k], nn[2 * k + 1]);
                 ok = same(fd, &fu[4 * k], 4);
             }
-            if (ok) return {joined, g};
+            if (!ok) continue;
+            FluxKind fk{joined ? FVH_FLUX_ROE_JOINED : FVH_FLUX_ROE_SPLIT, g, 0.0, {}};
+            // gradient dependence: the viscous kind (split Roe + laminar terms)
+            bool grad_free = true;
+            for (int k = 0; k < NS && grad_free; ++k) {
+                double gxk[4], gyk[4], fg[4];
+                for (int j = 0; j < 4; ++j) { gxk[j] = r.in(-2, 2); gyk[j] = r.in(-2, 2); }
+                calc_flux(fg, &qi[4 * k], &qj[4 * k], gxk, gyk, &nn[2 * k]);
+                grad_free = same(fg, &fu[4 * k], 4);
+            }
+            if (grad_free) return fk;
+            if (joined) continue;
+            fk.kind = FVH_FLUX_ROE_VISCOUS;
+            probe_viscous(fk, r, NS, qi, qj, nn);
+            return fk;
         }
     }
     throw std::runtime_error("calc_flux: no device flux kind reproduces it (the library implements the Roe flux "
-                             "with the entropy cure in the shockTube/naca and forwardStep spellings)");
+                             "with the entropy cure in the shockTube/naca and forwardStep spellings, and with the "
+                             "laminar terms of the flatPlate/nacaTransient examples)");
 }
 
 // ---- limiter function --------------------------------------------------------
@@ -177,6 +250,8 @@ fvh_bc_spec probe_bc(const std::string& name, void (*fn)(double*, double*, doubl
     if (matches(P)) { spec.kind = FVH_BC_COPY; return spec; }
     P.kind = fvh::BC_MIRROR;
     if (matches(P)) { spec.kind = FVH_BC_MIRROR; return spec; }
+    P.kind = fvh::BC_NOSLIP;
+    if (matches(P)) { spec.kind = FVH_BC_NOSLIP; return spec; }
     // characteristic kinds: a supersonic state entering through n returns the freestream
     double q[4], n[2] = {1.0, 0.0}, bv[4];
     conserved(q, 1.0, -2.5 * std::sqrt(gamma), 0.0, 1.0);
@@ -189,7 +264,7 @@ fvh_bc_spec probe_bc(const std::string& name, void (*fn)(double*, double*, doubl
     const double p_est = (gamma - 1) * (bv[3] - 0.5 / rho * (bv[1] * bv[1] + bv[2] * bv[2]));
     for (double p : nice(p_est))
         if (p / (gamma - 1) + 0.5 / rho * (bv[1] * bv[1] + bv[2] * bv[2]) == bv[3]) ps.push_back(p);
-    for (int kind : {fvh::BC_INLET_OUTLET, fvh::BC_FARFIELD})
+    for (int kind : {fvh::BC_INLET_OUTLET, fvh::BC_FARFIELD, fvh::BC_FARFIELD_V})
         for (double u : us)
             for (double v : vs)
                 for (double p : ps) {
@@ -201,7 +276,8 @@ fvh_bc_spec probe_bc(const std::string& name, void (*fn)(double*, double*, doubl
                     }
                 }
     throw std::runtime_error("boundaries::bounds[\"" + name + "\"]: no device boundary kind reproduces it "
-                             "(copy, mirror wall, characteristic inlet/outlet, farfield)");
+                             "(copy, mirror wall, no-slip wall, characteristic inlet/outlet, farfield in the naca "
+                             "and flatPlate spellings)");
 }
 
 // ---- time step: compared on the mesh itself through the library --------------
@@ -217,7 +293,7 @@ std::vector<double> device_dt(mesh& m, fvh_config c, const std::vector<double>& 
     return dt;
 }
 
-void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial) {
+void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial, const std::vector<Visc>& visc) {
     // the state the first calc_dt sees: the initial solution after the
     // initial boundary update (explicit.cpp:712, 729)
     std::vector<double> q(q_initial);
@@ -247,8 +323,19 @@ void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial) {
         c.dt_fixed = du[0];
         if (same(device_dt(m, c, q).data(), du.data(), (int)N)) return;
     }
-    for (int kind : {FVH_DT_SUM_CFL, FVH_DT_EDGE_MIN}) {
-        c.dt_kind = kind;
+    // the viscous kinds once per (pr, cp, r) the flux left open: pr enters
+    // the time step through mu/pr
+    std::vector<std::pair<int, Visc>> tries = {{FVH_DT_SUM_CFL, Visc{c.pr, c.cp, c.r_gas}},
+                                               {FVH_DT_EDGE_MIN, Visc{c.pr, c.cp, c.r_gas}}};
+    for (const Visc& v : visc) {
+        tries.push_back({FVH_DT_EDGE_MIN_VISCOUS, v});
+        tries.push_back({FVH_DT_SUM_VISCOUS, v});
+    }
+    for (const auto& t : tries) {
+        c.dt_kind = t.first;
+        c.pr = t.second.pr;
+        c.cp = t.second.cp;
+        c.r_gas = t.second.r;
         c.cfl = 1.0;
         const std::vector<double> d1 = device_dt(m, c, q);
         // the time step is linear in the CFL number away from the edge-min cap of 1
@@ -260,7 +347,7 @@ void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial) {
         }
     }
     throw std::runtime_error("calc_dt: no device time-step kind reproduces it on this mesh (fixed, summed CFL, "
-                             "edge-minimum local)");
+                             "edge-minimum local, and their viscous forms)");
 }
 
 }  // namespace
@@ -269,7 +356,6 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
     if (vars != 4)
         throw std::runtime_error("vars = " + std::to_string(vars) +
                                  ": the device path implements the 4-variable Euler plugins");
-    if (solver::diffusive_gradients) throw std::runtime_error("diffusive_gradients: viscous fluxes are not on the device path");
     fvh_config c;
     fvh_config_default(&c);
     c.flags = 0;
@@ -278,10 +364,17 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
     if (solver::linear_interpolate) c.flags |= FVH_FLAG_LINEAR;
     if (solver::global_dt) c.flags |= FVH_FLAG_GLOBAL_DT;
     if (solver::smooth_residuals) c.flags |= FVH_FLAG_SMOOTH;
+    if (solver::diffusive_gradients) c.flags |= FVH_FLAG_DIFFUSIVE;
     c.K = solver::limiter_k_value;
     const FluxKind fk = probe_flux();
-    c.flux = fk.joined ? FVH_FLUX_ROE_JOINED : FVH_FLUX_ROE_SPLIT;
+    c.flux = fk.kind;
     c.gamma = fk.gamma;
+    if (fk.kind == FVH_FLUX_ROE_VISCOUS) {
+        c.mu = fk.mu;
+        c.pr = fk.visc[0].pr;
+        c.cp = fk.visc[0].cp;
+        c.r_gas = fk.visc[0].r;
+    }
     if (solver::do_calc_limiters) {
         const PhiKind pk = probe_phi();
         c.phi = pk.kind;
@@ -297,7 +390,7 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
         seen.push_back(nm);
         c.bc[c.n_bc++] = probe_bc(nm, m.boundaryFuncs[b], c.gamma);
     }
-    probe_dt(m, c, q_initial);
+    probe_dt(m, c, q_initial, fk.visc);
     return c;
 }
 

@@ -17,11 +17,12 @@ FVH_MAX_ALPHA = 8
 FVH_MAX_BC = 16
 FVH_NAME_LEN = 64
 
-FLUX = {"roe_split": 0, "roe_joined": 1}
+FLUX = {"roe_split": 0, "roe_joined": 1, "roe_viscous": 2}
 PHI = {"zero": 0, "michalak": 1, "venkat": 2, "minmod": 3}
-DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2}
-BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3}
+DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2, "edge_min_viscous": 3, "sum_viscous": 4}
+BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3, "noslip": 4, "farfield_v": 5}
 FLAG_GRADIENTS, FLAG_LIMITERS, FLAG_LINEAR, FLAG_GLOBAL_DT, FLAG_POSITIVITY, FLAG_SMOOTH = 1, 2, 4, 8, 16, 32
+FLAG_DIFFUSIVE = 64
 RK5 = [0.0695, 0.1602, 0.2898, 0.506, 1.0]
 
 
@@ -38,6 +39,7 @@ class Config(C.Structure):
         ("flags", C.c_uint32), ("n_alpha", C.c_int32), ("alpha", C.c_double * FVH_MAX_ALPHA),
         ("n_bc", C.c_int32), ("bc", BcSpec * FVH_MAX_BC),
         ("renumber", C.c_int32), ("flux_mode", C.c_int32), ("tile_cells", C.c_int32),
+        ("mu", C.c_double), ("pr", C.c_double), ("cp", C.c_double), ("r_gas", C.c_double),
     ]
 
 

@@ -225,6 +225,26 @@ def forward_step(n_per_unit: int = 440, jitter: float = 0.2, seed: int = 2024060
     return _grid_mesh(nx, ny, 0.0, 0.0, h, h, keep, jitter, seed, True, name)
 
 
+def flat_plate(nx: int = 60, ny: int = 20, length: float = 1.5, height: float = 0.5, plate_x: float = 0.25,
+               jitter: float = 0.15, seed: int = 11) -> MeshArrays:
+    """Flat-plate box [0,length]x[0,height] with the group names of the
+    reference's flatPlate example: ``left``, ``right``, ``top``, and the
+    bottom split at x = plate_x into ``bot0`` (slip, upstream) and ``bot1``
+    (the plate).  Squares split into triangles along a fixed diagonal."""
+    hx, hy = length / nx, height / ny
+
+    def name(mx, my, ox, oy):
+        if ox < 0:
+            return "left"
+        if ox > 0:
+            return "right"
+        if oy > 0:
+            return "top"
+        return "bot0" if mx < plate_x else "bot1"
+
+    return _grid_mesh(nx, ny, 0.0, 0.0, hx, hy, lambda i, j: np.ones_like(i, dtype=bool), jitter, seed, False, name)
+
+
 def mixed_channel(nx: int = 12, ny: int = 6, jitter: float = 0.15, seed: int = 7,
                   tri_fraction: float = 0.5, names=("wall", "transmissive")) -> MeshArrays:
     """A small jittered channel [0,1]x[0,0.5] of quads, a seeded fraction of

@@ -112,7 +112,12 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
         flags |= _lib.FLAG_POSITIVITY
     if cfg.get("smooth", False):
         flags |= _lib.FLAG_SMOOTH
+    if cfg.get("diffusive", False):
+        flags |= _lib.FLAG_DIFFUSIVE
     c.flags = flags
+    for k in ("mu", "pr", "cp", "r_gas"):
+        if k in cfg:
+            setattr(c, k, float(cfg[k]))
     alpha = list(cfg.get("alpha", _lib.RK5))
     if not 1 <= len(alpha) <= _lib.FVH_MAX_ALPHA:
         raise ValueError("1..%d RK stages supported" % _lib.FVH_MAX_ALPHA)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define FVH_ABI_VERSION 1
+#define FVH_ABI_VERSION 2
 
 enum {
     FVH_OK = 0,
@@ -41,15 +41,26 @@ enum {
 /* Roe flux with the entropy cure; the two reference spellings differ in the
  * association of the central flux, which changes the last bits. */
 enum { FVH_FLUX_ROE_SPLIT = 0,   /* examples/shockTube/main.cpp:68-135, naca/main.cpp:71-137 */
-       FVH_FLUX_ROE_JOINED = 1   /* examples/forwardStep/main.cpp:72-135 */ };
+       FVH_FLUX_ROE_JOINED = 1,  /* examples/forwardStep/main.cpp:72-135 */
+       /* the split Roe flux plus the laminar viscous terms of
+        * examples/flatPlate/main.cpp:126-225 (= nacaTransient/main.cpp:130-229),
+        * constants mu, pr, cp, r_gas of the config; the face gradient is that of
+        * gradient_for_diffusion (explicit.cpp:70-109) with FVH_FLAG_DIFFUSIVE,
+        * zero without */
+       FVH_FLUX_ROE_VISCOUS = 2 };
 enum { FVH_PHI_ZERO = 0, FVH_PHI_MICHALAK = 1, FVH_PHI_VENKATAKRISHNAN = 2, FVH_PHI_MINMOD = 3 };
 enum { FVH_DT_FIXED = 0,         /* shockTube/main.cpp:140-147 */
        FVH_DT_SUM_CFL = 1,       /* forwardStep/main.cpp:140-185 */
-       FVH_DT_EDGE_MIN = 2       /* naca/main.cpp:142-183 */ };
+       FVH_DT_EDGE_MIN = 2,      /* naca/main.cpp:142-183 */
+       FVH_DT_EDGE_MIN_VISCOUS = 3,  /* flatPlate/main.cpp:230-278 */
+       FVH_DT_SUM_VISCOUS = 4    /* nacaTransient/main.cpp:234-286 */ };
 enum { FVH_BC_COPY = 0,          /* shockTube/main.cpp:158-163 */
        FVH_BC_MIRROR = 1,        /* forwardStep/main.cpp:262-269 */
        FVH_BC_INLET_OUTLET = 2,  /* forwardStep/main.cpp:194-261 */
-       FVH_BC_FARFIELD = 3       /* naca/main.cpp:192-259 */ };
+       FVH_BC_FARFIELD = 3,      /* naca/main.cpp:192-259 */
+       FVH_BC_NOSLIP = 4,        /* flatPlate/main.cpp:361-366: momentum reversed */
+       FVH_BC_FARFIELD_V = 5     /* flatPlate/main.cpp:286-353: the farfield with
+                                    b[2] = b[0]*(...) in both subsonic branches */ };
 enum { FVH_F64 = 0, FVH_F32 = 1 };
 /* Device cell order (results do not depend on it, only locality does). */
 enum { FVH_ORDER_FILE = 0,       /* reference numbering */
@@ -70,6 +81,8 @@ enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2, F
 #define FVH_FLAG_LIMITERS    2u   /* do_calc_limiters */
 #define FVH_FLAG_LINEAR      4u   /* linear_interpolate */
 #define FVH_FLAG_GLOBAL_DT   8u   /* global_dt */
+#define FVH_FLAG_DIFFUSIVE  64u   /* diffusive_gradients: face gradients for the
+                                     viscous flux (explicit.cpp:70-109, 324-338) */
 #define FVH_FLAG_SMOOTH     32u   /* smooth_residuals: two Jacobi sweeps of implicit
                                      residual smoothing, eps 0.6 (explicit.cpp:32-66,
                                      743); not with the FUSED stage form */
@@ -106,6 +119,10 @@ typedef struct fvh_config {
     int32_t renumber;          /* device cell order: FVH_ORDER_* */
     int32_t flux_mode;         /* FVH_FLUX_MODE_* */
     int32_t tile_cells;        /* cells per tile (workgroup), 32..256; 0 = 256 */
+    /* laminar viscous constants of FVH_FLUX_ROE_VISCOUS and the viscous time
+     * steps (consts:: of flatPlate/main.cpp:28-35): viscosity, Prandtl number,
+     * heat capacity, gas constant */
+    double mu, pr, cp, r_gas;
 } fvh_config;
 
 /* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */

@@ -226,11 +226,86 @@ static inline double pressure(double g, const double* q) {
     return (g - 1) * (q[3] - 0.5 / q[0] * (q[1] * q[1] + q[2] * q[2]));
 }
 
-/* Roe flux with the entropy cure: shockTube/main.cpp:68-135 (SPLIT: central
- * part accumulated per side, as also naca/main.cpp:71-137) and
- * forwardStep/main.cpp:72-135 (JOINED: central part in one sum). */
+/* Laminar stresses and heat flux of the viscous examples, subtracted from
+ * the Roe flux: flatPlate/main.cpp:193-224 with its helpers calc_grad_p
+ * (:43-57), calc_grad_t (:65-78), calc_grad_u (:80-85), calc_grad_v
+ * (:87-92), all on the mean face state; nacaTransient/main.cpp holds the
+ * same code. */
+static void viscous_terms(const orc_config* cf, double* f, const double* qi, const double* qj,
+                          const double* gx, const double* gy, const double* n) {
+    const double g = cf->gamma, mu = cf->mu;
+    double q[4];
+    for (int k = 0; k < 4; ++k) q[k] = 0.5 * (qi[k] + qj[k]);
+    double gp[2];
+    gp[0] = 2. * gx[3];
+    gp[0] -= gx[1] * q[1] / q[0] + q[1] * (gx[1] * q[0] - gx[0] * q[1]) / (q[0] * q[0]);
+    gp[0] -= gx[2] * q[2] / q[0] + q[2] * (gx[2] * q[0] - gx[0] * q[2]) / (q[0] * q[0]);
+    gp[0] *= 0.5 * (g - 1);
+    gp[1] = 2. * gy[3];
+    gp[1] -= gy[1] * q[1] / q[0] + q[1] * (gy[1] * q[0] - gy[0] * q[1]) / (q[0] * q[0]);
+    gp[1] -= gy[2] * q[2] / q[0] + q[2] * (gy[2] * q[0] - gy[0] * q[2]) / (q[0] * q[0]);
+    gp[1] *= 0.5 * (g - 1);
+    const double p = pressure(g, q);
+    double gt[2];
+    gt[0] = (1. / cf->r) * ((gp[0] * q[0] - gx[0] * p) / (q[0] * q[0]));
+    gt[1] = (1. / cf->r) * ((gp[1] * q[0] - gy[0] * p) / (q[0] * q[0]));
+    double gu[2], gv[2];
+    gu[0] = (q[0] * gx[1] - q[1] * gx[0]) / (q[0] * q[0]);
+    gu[1] = (q[0] * gy[1] - q[1] * gy[0]) / (q[0] * q[0]);
+    gv[0] = (q[0] * gx[2] - q[2] * gx[0]) / (q[0] * q[0]);
+    gv[1] = (q[0] * gy[2] - q[2] * gy[0]) / (q[0] * q[0]);
+    const double div_v = gu[0] + gv[1];
+    const double txx = 2. * mu * (gu[0] - div_v / 3.);
+    const double tyy = 2. * mu * (gv[1] - div_v / 3.);
+    const double txy = mu * (gu[1] + gv[0]);
+    const double k = cf->cp * mu / cf->pr;
+    double ph[2];
+    ph[0] = q[1] / q[0] * txx + q[2] / q[0] * txy + k * gt[0];
+    ph[1] = q[1] / q[0] * txy + q[2] / q[0] * tyy + k * gt[1];
+    f[1] -= n[0] * txx + n[1] * txy;
+    f[2] -= n[0] * txy + n[1] * tyy;
+    f[3] -= n[0] * ph[0] + n[1] * ph[1];
+}
+
+static void roe_flux(const orc_config* cf, double* f, const double* qi, const double* qj, const double* n);
+
 void orc_flux(const orc_config* cf, double* f, const double* qi, const double* qj,
               const double* n) {
+    const double z[4] = {0., 0., 0., 0.};
+    orc_flux_grad(cf, f, qi, qj, z, z, n);
+}
+
+void orc_flux_grad(const orc_config* cf, double* f, const double* qi, const double* qj,
+                   const double* gx, const double* gy, const double* n) {
+    roe_flux(cf, f, qi, qj, n);
+    if (cf->flux == ORC_FLUX_ROE_VISCOUS) viscous_terms(cf, f, qi, qj, gx, gy, n);
+}
+
+/* gradient_for_diffusion (explicit.cpp:70-109) */
+void orc_face_gradient(double* gx, double* gy, const double* gxi, const double* gyi,
+                       const double* gxj, const double* gyj, const double* qi, const double* qj,
+                       const double* ci, const double* cj) {
+    double t[2];
+    t[0] = ci[0] - cj[0];
+    t[1] = ci[1] - cj[1];
+    const double l = sqrt(t[0] * t[0] + t[1] * t[1]);
+    t[0] = t[0] / l;
+    t[1] = t[1] / l;
+    for (int k = 0; k < 4; ++k) {
+        const double dir = (qi[k] - qj[k]) / l;
+        const double xb = (gxi[k] + gxj[k]) * 0.5;
+        const double yb = (gyi[k] + gyj[k]) * 0.5;
+        const double dot = xb * t[0] + yb * t[1];
+        gx[k] = xb - (dot - dir) * t[0];
+        gy[k] = yb - (dot - dir) * t[1];
+    }
+}
+
+/* Roe flux with the entropy cure: shockTube/main.cpp:68-135 (SPLIT: central
+ * part accumulated per side, as also naca/main.cpp:71-137 and the viscous
+ * examples' flatPlate/main.cpp:126-191) and forwardStep/main.cpp:72-135
+ * (JOINED: central part in one sum). */
+static void roe_flux(const orc_config* cf, double* f, const double* qi, const double* qj, const double* n) {
     const double g = cf->gamma;
     double pi = pressure(g, qi);
     double pj = pressure(g, qj);
@@ -304,7 +379,9 @@ double orc_phi(const orc_config* cf, double y) {
 
 /* boundary kinds: copy (shockTube/main.cpp:158-163), mirror wall
  * (forwardStep/main.cpp:262-269), characteristic inlet/outlet
- * (forwardStep/main.cpp:194-261), farfield (naca/main.cpp:192-259). */
+ * (forwardStep/main.cpp:194-261), farfield (naca/main.cpp:192-259), no-slip
+ * wall (flatPlate/main.cpp:361-366) and the viscous examples' farfield
+ * (flatPlate/main.cpp:286-353: b[0] where naca has b[1]). */
 void orc_bc(const orc_config* cf, int slot, double* b, const double* q, const double* n) {
     const int kind = cf->bc_kind[slot];
     const double g = cf->gamma;
@@ -316,6 +393,13 @@ void orc_bc(const orc_config* cf, int slot, double* b, const double* q, const do
         b[0] = q[0];
         b[1] = q[1] - 2.0 * n[0] * (n[0] * q[1] + n[1] * q[2]);
         b[2] = q[2] - 2.0 * n[1] * (n[0] * q[1] + n[1] * q[2]);
+        b[3] = q[3];
+        return;
+    }
+    if (kind == ORC_BC_NOSLIP) {
+        b[0] = q[0];
+        b[1] = -q[1];
+        b[2] = -q[2];
         b[3] = q[3];
         return;
     }
@@ -359,14 +443,16 @@ void orc_bc(const orc_config* cf, int slot, double* b, const double* q, const do
             double pb = 0.5 * (pa + pd - rho0 * c0 * (n[0] * (ua - ud) + n[1] * (va - vd)));
             b[0] = rhoa + (pb - pa) / (c0 * c0);
             b[1] = b[0] * (ua - n[0] * (pa - pb) / (rho0 * c0));
-            b[2] = b[1] * (va - n[1] * (pa - pb) / (rho0 * c0));   /* b[1], as in the reference */
+            /* b[1] as naca/main.cpp:246, b[0] as flatPlate/main.cpp:340 */
+            b[2] = (kind == ORC_BC_FARFIELD_V ? b[0] : b[1]) * (va - n[1] * (pa - pb) / (rho0 * c0));
             b[3] = pb / (g - 1) + 0.5 / b[0] * (b[1] * b[1] + b[2] * b[2]);
         }
     } else {
         double pb = pa;
         b[0] = rhod + (pb - pd) / (c0 * c0);
         b[1] = b[0] * (ud + n[0] * (pd - pb) / (rho0 * c0));
-        b[2] = b[1] * (va + n[1] * (pd - pb) / (rho0 * c0));       /* b[1] and va, as in the reference */
+        /* va in both spellings; b[1] as naca/main.cpp:253, b[0] as flatPlate/main.cpp:347 */
+        b[2] = (kind == ORC_BC_FARFIELD_V ? b[0] : b[1]) * (va + n[1] * (pd - pb) / (rho0 * c0));
         b[3] = pb / (g - 1.) + 0.5 / b[0] * (b[1] * b[1] + b[2] * b[2]);
     }
 }
@@ -400,6 +486,39 @@ void orc_calc_dt(const orc_mesh* m, const orc_config* cf, const double* q, doubl
             dt[j] += center * le;
         }
         for (int32_t i = 0; i < N; ++i) dt[i] = cf->cfl * m->cell_area[i] / dt[i];
+        return;
+    }
+    if (cf->dt_kind == ORC_DT_EDGE_MIN_VISCOUS || cf->dt_kind == ORC_DT_SUM_VISCOUS) {
+        /* flatPlate/main.cpp:230-278 (local minimum over edges) and
+         * nacaTransient/main.cpp:234-286 (summed): convective plus C = 2
+         * times the viscous spectral radius */
+        const int sum = cf->dt_kind == ORC_DT_SUM_VISCOUS;
+        const double C = 2;
+        for (int32_t i = 0; i < N; ++i) dt[i] = sum ? 0. : 1.0;
+        for (int32_t e = 0; e < m->n_edges; ++e) {
+            int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
+            double le = m->edge_len[e], n0 = m->edge_nx[e], n1 = m->edge_ny[e];
+            const double* qi = &q[V * i];
+            const double* qj = &q[V * j];
+            double ci = sqrt(pressure(g, qi) * g / qi[0]);
+            double cj = sqrt(pressure(g, qj) * g / qj[0]);
+            double evi = smax(4. / (3. * qi[0]), g / qi[0]) * (cf->mu / cf->pr) * le * le / m->cell_area[i];
+            double evj = smax(4. / (3. * qj[0]), g / qj[0]) * (cf->mu / cf->pr) * le * le / m->cell_area[j];
+            if (sum) {
+                double eci = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
+                double ecj = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
+                double center = smax(eci + C * evi, ecj + C * evj);
+                dt[i] += center * le;
+                dt[j] += center * le;
+            } else {
+                double eci = (ci + fabs(n0 * qi[1] / qi[0]) + fabs(n1 * qi[2] / qi[0])) * le;
+                double ecj = (cj + fabs(n0 * qj[1] / qj[0]) + fabs(n1 * qj[2] / qj[0])) * le;
+                dt[i] = smin(dt[i], cf->cfl * m->cell_area[i] / (eci + C * evi));
+                dt[j] = smin(dt[j], cf->cfl * m->cell_area[j] / (ecj + C * evj));
+            }
+        }
+        if (sum)
+            for (int32_t i = 0; i < N; ++i) dt[i] = cf->cfl * m->cell_area[i] / dt[i];
         return;
     }
     for (int32_t i = 0; i < N; ++i) dt[i] = 1.0;
@@ -543,7 +662,16 @@ void orc_calc_time_derivatives(const orc_mesh* m, const orc_config* cf, const do
             if (!(qj[0] > 0) || !(pressure(cf->gamma, qj) > 0))
                 for (int k = 0; k < V; ++k) qj[k] = q[V * j + k];
         }
-        orc_flux(cf, f, qi, qj, nn);
+        /* viscous face gradient from the cell gradients and cell states
+         * (explicit.cpp:324-338); zero without diffusive_gradients */
+        double gxv[V] = {0., 0., 0., 0.}, gyv[V] = {0., 0., 0., 0.};
+        if (cf->diff) {
+            const double ci[2] = {m->cell_cx[i], m->cell_cy[i]};
+            const double cj[2] = {m->cell_cx[j], m->cell_cy[j]};
+            orc_face_gradient(gxv, gyv, &gx[V * i], &gy[V * i], &gx[V * j], &gy[V * j], &q[V * i], &q[V * j], ci,
+                              cj);
+        }
+        orc_flux_grad(cf, f, qi, qj, gxv, gyv, nn);
         for (int k = 0; k < V; ++k) {
             qt[V * i + k] -= f[k] * le / m->cell_area[i];
             qt[V * j + k] += f[k] * le / m->cell_area[j];

@@ -21,10 +21,12 @@
 extern "C" {
 #endif
 
-enum { ORC_FLUX_ROE_SPLIT = 0, ORC_FLUX_ROE_JOINED = 1 };
+enum { ORC_FLUX_ROE_SPLIT = 0, ORC_FLUX_ROE_JOINED = 1, ORC_FLUX_ROE_VISCOUS = 2 };
 enum { ORC_PHI_ZERO = 0, ORC_PHI_MICHALAK = 1, ORC_PHI_VENKAT = 2, ORC_PHI_MINMOD = 3 };
-enum { ORC_DT_FIXED = 0, ORC_DT_SUM_CFL = 1, ORC_DT_EDGE_MIN = 2 };
-enum { ORC_BC_COPY = 0, ORC_BC_MIRROR = 1, ORC_BC_INLET_OUTLET = 2, ORC_BC_FARFIELD = 3 };
+enum { ORC_DT_FIXED = 0, ORC_DT_SUM_CFL = 1, ORC_DT_EDGE_MIN = 2, ORC_DT_EDGE_MIN_VISCOUS = 3,
+       ORC_DT_SUM_VISCOUS = 4 };
+enum { ORC_BC_COPY = 0, ORC_BC_MIRROR = 1, ORC_BC_INLET_OUTLET = 2, ORC_BC_FARFIELD = 3, ORC_BC_NOSLIP = 4,
+       ORC_BC_FARFIELD_V = 5 };
 
 #define ORC_MAX_ALPHA 8
 #define ORC_MAX_BC 16
@@ -47,6 +49,8 @@ typedef struct {
     int bc_kind[ORC_MAX_BC];
     double bc_param[ORC_MAX_BC][4];   /* freestream rho, u, v, p */
     int smooth;          /* solver::smooth_residuals (explicit.h:40) */
+    double mu, pr, cp, r;   /* consts:: of the viscous examples (flatPlate/main.cpp:28-35) */
+    int diff;            /* solver::diffusive_gradients (explicit.h:38) */
 } orc_config;
 
 typedef struct {
@@ -91,9 +95,16 @@ void orc_calc_residuals(const orc_mesh* m, const double* qt, double* R);
 /* smooth_residuals (explicit.cpp:32-66) in place on qt. */
 void orc_smooth_residuals(const orc_mesh* m, double* qt);
 
-/* Scalar plugin kinds, for known-answer tests. */
+/* Scalar plugin kinds, for known-answer tests.  orc_flux passes zero
+ * gradients to the viscous kind; orc_flux_grad takes the face gradient. */
 void orc_flux(const orc_config* c, double* f, const double* qi, const double* qj,
               const double* n);
+void orc_flux_grad(const orc_config* c, double* f, const double* qi, const double* qj,
+                   const double* gx, const double* gy, const double* n);
+/* gradient_for_diffusion (explicit.cpp:70-109) */
+void orc_face_gradient(double* gx, double* gy, const double* gxi, const double* gyi,
+                       const double* gxj, const double* gyj, const double* qi, const double* qj,
+                       const double* ci, const double* cj);
 double orc_phi(const orc_config* c, double r);
 void orc_bc(const orc_config* c, int slot, double* b, const double* qint, const double* n);
 

@@ -28,10 +28,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "fvh_ref")
 
-FLUX = {"roe_split": 0, "roe_joined": 1}
+FLUX = {"roe_split": 0, "roe_joined": 1, "roe_viscous": 2}
 PHI = {"zero": 0, "michalak": 1, "venkat": 2, "minmod": 3}
-DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2}
-BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3}
+DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2, "edge_min_viscous": 3, "sum_viscous": 4}
+BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3, "noslip": 4, "farfield_v": 5}
+# consts:: of examples/flatPlate/main.cpp:28-35
+VISCOUS_DEFAULTS = {"mu": 2e-5, "pr": 0.72, "cp": 1.0, "r_gas": 8.3145}
 RK5 = [0.0695, 0.1602, 0.2898, 0.506, 1.0]
 
 
@@ -44,6 +46,8 @@ class OrcConfig(C.Structure):
         ("n_alpha", C.c_int), ("alpha", C.c_double * 8),
         ("n_bc", C.c_int), ("bc_kind", C.c_int * 16), ("bc_param", (C.c_double * 4) * 16),
         ("smooth", C.c_int),
+        ("mu", C.c_double), ("pr", C.c_double), ("cp", C.c_double), ("r", C.c_double),
+        ("diff", C.c_int),
     ]
 
 
@@ -89,6 +93,8 @@ def lib():
         L.orc_calc_residuals.argtypes = [mp, dp, dp]
         L.orc_smooth_residuals.argtypes = [mp, dp]
         L.orc_flux.argtypes = [cfgp, dp, dp, dp, dp]
+        L.orc_flux_grad.argtypes = [cfgp, dp, dp, dp, dp, dp, dp]
+        L.orc_face_gradient.argtypes = [dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
         L.orc_phi.argtypes = [cfgp, C.c_double]
         L.orc_phi.restype = C.c_double
         L.orc_bc.argtypes = [cfgp, C.c_int, dp, dp, dp]
@@ -114,6 +120,11 @@ def make_config(cfg: Dict, names: List[str]) -> OrcConfig:
     c.gdt = int(bool(cfg.get("gdt", False)))
     c.pos = int(bool(cfg.get("positivity", False)))
     c.smooth = int(bool(cfg.get("smooth", False)))
+    c.diff = int(bool(cfg.get("diffusive", False)))
+    c.mu = float(cfg.get("mu", VISCOUS_DEFAULTS["mu"]))
+    c.pr = float(cfg.get("pr", VISCOUS_DEFAULTS["pr"]))
+    c.cp = float(cfg.get("cp", VISCOUS_DEFAULTS["cp"]))
+    c.r = float(cfg.get("r_gas", VISCOUS_DEFAULTS["r_gas"]))
     c.K = float(cfg.get("K", 7.5))
     c.gamma = float(cfg.get("gamma", 1.4))
     alpha = list(cfg.get("alpha", RK5))
@@ -248,6 +259,23 @@ def flux(cfg: Dict, qi, qj, n) -> np.ndarray:
     lib().orc_flux(C.byref(c), f, np.ascontiguousarray(qi, dtype=np.float64),
                    np.ascontiguousarray(qj, dtype=np.float64), np.ascontiguousarray(n, dtype=np.float64))
     return f
+
+
+def flux_grad(cfg: Dict, qi, qj, gx, gy, n) -> np.ndarray:
+    """Edge flux with a face gradient (the viscous kind uses it)."""
+    c = make_config(dict(cfg, bcs={}), [])
+    f = np.zeros(4)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (qi, qj, gx, gy, n)]
+    lib().orc_flux_grad(C.byref(c), f, *a)
+    return f
+
+
+def face_gradient(gxi, gyi, gxj, gyj, qi, qj, ci, cj):
+    """gradient_for_diffusion (explicit.cpp:70-109): (gx, gy) of the face."""
+    gx, gy = np.zeros(4), np.zeros(4)
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (gxi, gyi, gxj, gyj, qi, qj, ci, cj)]
+    lib().orc_face_gradient(gx, gy, *a)
+    return gx, gy
 
 
 def phi(cfg: Dict, r: float) -> float:
@@ -447,8 +475,12 @@ def ref_args(cfg: Dict) -> Dict[str, str]:
     """Map a config dict onto the harness's plugin selection.  Only the
     combinations the reference examples provide are expressible."""
     kv = {}
-    kv["flux"] = {"roe_split": "st", "roe_joined": "fs"}[cfg.get("flux", "roe_split")]
-    kv["dt"] = {"fixed": "st", "sum_cfl": "fs", "edge_min": "nc"}[cfg.get("dt", "fixed")]
+    kv["flux"] = {"roe_split": "st", "roe_joined": "fs", "roe_viscous": "fp"}[cfg.get("flux", "roe_split")]
+    kv["dt"] = {"fixed": "st", "sum_cfl": "fs", "edge_min": "nc", "edge_min_viscous": "fp",
+                "sum_viscous": "nt"}[cfg.get("dt", "fixed")]
+    for k, v in VISCOUS_DEFAULTS.items():
+        if float(cfg.get(k, v)) != v:
+            raise ValueError("the reference's viscous examples fix %s = %r" % (k, v))
     phi_ = cfg.get("phi", "zero")
     kv["phi"] = {"zero": "zero", "michalak": "fs", "venkat": "venk", "minmod": "minmod"}[phi_]
     if phi_ == "michalak" and float(cfg.get("phi_yt", 2.0)) != 2.0:
@@ -460,8 +492,11 @@ def ref_args(cfg: Dict) -> Dict[str, str]:
     kv["gamma"] = repr(float(cfg.get("gamma", 1.4)))
     kv["cfl_fs"] = repr(float(cfg.get("cfl", 1.5)))
     kv["cfl_nc"] = repr(float(cfg.get("cfl", 1.0)))
+    kv["cfl_fp"] = repr(float(cfg.get("cfl", 2.0)))
+    kv["cfl_nt"] = repr(float(cfg.get("cfl", 1.5)))
     return kv
 
 
 def ref_flags(cfg: Dict) -> str:
-    return ",".join(k for k in ("grad", "lim", "lin", "gdt", "smooth") if cfg.get(k, False))
+    names = {"diffusive": "diff"}
+    return ",".join(names.get(k, k) for k in ("grad", "lim", "lin", "gdt", "smooth", "diffusive") if cfg.get(k, False))

@@ -11,10 +11,11 @@
 //          loop    the run() loop with a caller-chosen alpha list (alpha=a,b,..)
 //          phases  one stage of every phase from the state in ic=file:<path>
 //          vtk     fvhyper::run then fvhyper::writeVtk (post.cpp:21-200)
-//   keys:  flux=st|fs|nc  dt=st|fs|nc  phi=zero|fs|nc|venk|minmod
-//          ic=st|fs|nc|file:<path>  gamma= cfl_fs= cfl_nc= mach=
+//   keys:  flux=st|fs|nc|fp|nt  dt=st|fs|nc|fp|nt  phi=zero|fs|nc|venk|minmod
+//          ic=st|fs|nc|file:<path>  gamma= cfl_fs= cfl_nc= cfl_fp= cfl_nt= mach=
+//          (fp, nt: the viscous flatPlate and nacaTransient examples)
 //          max_step= max_time= tol= print= alpha=
-//   solver flags come from FVH_REF_FLAGS (comma list of grad,lim,lin,gdt,smooth)
+//   solver flags come from FVH_REF_FLAGS (comma list of grad,lim,lin,gdt,smooth,diff)
 //   because the library reads them as link-time constants.
 #include <fvhyper/mesh.h>
 #include <fvhyper/explicit.h>
@@ -44,6 +45,13 @@ namespace fvhyper {
 FVH_DECLARE_PLUGIN(st)
 FVH_DECLARE_PLUGIN(fs)
 FVH_DECLARE_PLUGIN(nc)
+FVH_DECLARE_PLUGIN(fp)
+FVH_DECLARE_PLUGIN(nt)
+namespace fp_consts { extern double cfl; }
+namespace nt_consts { extern double cfl; extern double mach; }
+namespace fp_boundaries { void wall(double*, double*, double*);
+                          void farfield(double*, double*, double*); }
+namespace nt_boundaries { void farfield(double*, double*, double*); }
 double bg_limiter_func(const double&);
 namespace fs_consts { extern double cfl; extern double mach; }
 namespace nc_consts { extern double cfl; }
@@ -80,7 +88,7 @@ namespace solver {
     extern const bool do_calc_gradients = env_flag("grad");
     extern const bool do_calc_limiters = env_flag("lim");
     extern const bool linear_interpolate = env_flag("lin");
-    extern const bool diffusive_gradients = false;
+    extern const bool diffusive_gradients = env_flag("diff");
     extern const bool global_dt = env_flag("gdt");
     extern const bool smooth_residuals = env_flag("smooth");
 }
@@ -104,12 +112,16 @@ void calc_flux(double* f, const double* qi, const double* qj,
                const double* gx, const double* gy, const double* n) {
     if (g_flux == "st") st_calc_flux(f, qi, qj, gx, gy, n);
     else if (g_flux == "fs") fs_calc_flux(f, qi, qj, gx, gy, n);
+    else if (g_flux == "fp") fp_calc_flux(f, qi, qj, gx, gy, n);
+    else if (g_flux == "nt") nt_calc_flux(f, qi, qj, gx, gy, n);
     else nc_calc_flux(f, qi, qj, gx, gy, n);
 }
 
 void calc_dt(std::vector<double>& dt, const std::vector<double>& q, mesh& m) {
     if (g_dt == "st") st_calc_dt(dt, q, m);
     else if (g_dt == "fs") fs_calc_dt(dt, q, m);
+    else if (g_dt == "fp") fp_calc_dt(dt, q, m);
+    else if (g_dt == "nt") nt_calc_dt(dt, q, m);
     else nc_calc_dt(dt, q, m);
 }
 
@@ -137,6 +149,9 @@ namespace boundaries {
         {"outlet", fs_boundaries::inlet_outlet},
         {"airfoil", nc_boundaries::wall},
         {"farfield", nc_boundaries::farfield},
+        {"noslip", fp_boundaries::wall},
+        {"farfield_fp", fp_boundaries::farfield},
+        {"farfield_nt", nt_boundaries::farfield},
     };
 }
 
@@ -242,9 +257,12 @@ int main(int argc, char** argv) {
     if (kv.count("gamma")) {
         double g = std::stod(kv["gamma"]);
         st_consts::gamma = g; fs_consts::gamma = g; nc_consts::gamma = g;
+        fp_consts::gamma = g; nt_consts::gamma = g;
     }
     if (kv.count("cfl_fs")) fs_consts::cfl = std::stod(kv["cfl_fs"]);
     if (kv.count("cfl_nc")) nc_consts::cfl = std::stod(kv["cfl_nc"]);
+    if (kv.count("cfl_fp")) fp_consts::cfl = std::stod(kv["cfl_fp"]);
+    if (kv.count("cfl_nt")) nt_consts::cfl = std::stod(kv["cfl_nt"]);
     if (kv.count("mach")) fs_consts::mach = std::stod(kv["mach"]);
 
     mpi_wrapper pool;

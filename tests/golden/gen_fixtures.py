@@ -38,7 +38,17 @@ import oracle as O  # noqa: E402
 from fvhyper_amd import meshgen  # noqa: E402
 from helpers import airfoil_as_channel, airfoil_bump_state, random_state  # noqa: E402
 
-KIND_NAME = {"mirror": "wall", "copy": "transmissive", "inlet_outlet": "inlet", "farfield": "farfield"}
+KIND_NAME = {"mirror": "wall", "copy": "transmissive", "inlet_outlet": "inlet", "farfield": "farfield",
+             "noslip": "noslip"}
+# the viscous examples' farfields differ in their hard-coded freestream
+FARFIELD_V_NAME = {(1.4, 0.2, 0.0, 1.0): "farfield_fp", (1.4, 0.8, 0.0, 1.0): "farfield_nt"}
+
+
+def kind_name(spec):
+    """The harness's boundary name for a (kind, freestream...) spec."""
+    if spec[0] == "farfield_v":
+        return FARFIELD_V_NAME[tuple(float(v) for v in spec[1:])]
+    return KIND_NAME[spec[0]]
 
 # the reference's own example mesh (4,096 quadrilaterals around a NACA 0012,
 # farfield at R = 149), read from the reference tree when the fixtures are made
@@ -80,25 +90,38 @@ SMOOTH_CONFIGS = {
                                    lin=True, smooth=True, bcs={"wall": ("mirror",), "transmissive": NC, "inlet": NC,
                                                                "outlet": ("copy",)}),
 }
-ALL_CONFIGS = {**FIX_CONFIGS, **SMOOTH_CONFIGS}
+# the viscous examples (flatPlate, nacaTransient): split Roe flux plus laminar
+# terms with diffusive face gradients, their time steps and boundaries
+VISC_MESHES = ("mixed", "step", "naca")
+FP_FF = ("farfield_v", 1.4, 0.2, 0.0, 1.0)
+NT_FF = ("farfield_v", 1.4, 0.8, 0.0, 1.0)
+VISC_CONFIGS = {
+    "visc_flatplate": dict(flux="roe_viscous", dt="edge_min_viscous", cfl=2.0, phi="michalak", grad=True, lim=True,
+                           lin=True, diffusive=True,
+                           bcs={"wall": ("noslip",), "transmissive": ("mirror",), "inlet": FP_FF, "outlet": FP_FF}),
+    "visc_transient": dict(flux="roe_viscous", dt="sum_viscous", cfl=1.5, grad=True, diffusive=True, gdt=True,
+                           bcs={"wall": ("noslip",), "transmissive": NT_FF, "inlet": NT_FF, "outlet": NT_FF}),
+}
+ALL_CONFIGS = {**FIX_CONFIGS, **SMOOTH_CONFIGS, **VISC_CONFIGS}
 
 
 def configs_of(mesh_name):
     """Config names with fixtures on this mesh."""
-    return list(FIX_CONFIGS) + (list(SMOOTH_CONFIGS) if mesh_name in SMOOTH_MESHES else [])
+    return (list(FIX_CONFIGS) + (list(SMOOTH_CONFIGS) if mesh_name in SMOOTH_MESHES else []) +
+            (list(VISC_CONFIGS) if mesh_name in VISC_MESHES else []))
 
 
 def renamed(m, cfg):
     """Mesh + config whose physical names are the harness's names for the kinds."""
     names, tagmap = [], []
     for nm in m.names:
-        kn = KIND_NAME[cfg["bcs"][nm][0]]
+        kn = kind_name(cfg["bcs"][nm])
         if kn not in names:
             names.append(kn)
         tagmap.append(names.index(kn))
     m2 = meshgen.MeshArrays(m.x, m.y, m.cells, m.bnd, np.asarray(tagmap, dtype=np.int32)[m.bnd_tag], names)
     cfg2 = dict(cfg)
-    cfg2["bcs"] = {KIND_NAME[v[0]]: v for v in cfg["bcs"].values()}
+    cfg2["bcs"] = {kind_name(v): v for v in cfg["bcs"].values()}
     return m2, cfg2
 
 

@@ -53,6 +53,11 @@ def rel_err(a, b) -> float:
 BC_ALL = {"wall": ("mirror",), "transmissive": ("copy",), "inlet": FREESTREAM_STEP,
           "outlet": FREESTREAM_STEP}
 
+FREESTREAM_VISC_FP = ("farfield_v", 1.4, 0.2, 0.0, 1.0)
+FREESTREAM_VISC_NT = ("farfield_v", 1.4, 0.8, 0.0, 1.0)
+BC_VISC_FP = {"wall": ("noslip",), "transmissive": ("mirror",), "inlet": FREESTREAM_VISC_FP,
+              "outlet": FREESTREAM_VISC_FP}
+
 CONFIGS = {
     "first_split_sum": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, bcs=BC_ALL),
     "first_joined_fixed": dict(flux="roe_joined", dt="fixed", dt_fixed=2e-5, gdt=True, bcs=BC_ALL),
@@ -74,6 +79,16 @@ CONFIGS = {
                                    lim=True, lin=True, gdt=False, smooth=True,
                                    bcs={"wall": ("mirror",), "transmissive": FREESTREAM_NACA,
                                         "inlet": FREESTREAM_NACA, "outlet": ("copy",)}),
+    # the viscous examples (flatPlate, nacaTransient): split Roe flux plus the
+    # laminar terms on diffusive face gradients, their time steps and
+    # boundaries; visc_strong raises mu 1000-fold so the terms matter
+    "visc_flatplate": dict(flux="roe_viscous", dt="edge_min_viscous", cfl=2.0, phi="michalak", grad=True, lim=True,
+                           lin=True, diffusive=True, bcs=BC_VISC_FP),
+    "visc_strong": dict(flux="roe_viscous", dt="edge_min_viscous", cfl=1.0, phi="venkat", grad=True, lim=True,
+                        lin=True, diffusive=True, mu=0.02, pr=0.7, cp=1.5, r_gas=2.0, bcs=BC_VISC_FP),
+    "visc_transient": dict(flux="roe_viscous", dt="sum_viscous", cfl=1.5, grad=True, diffusive=True, gdt=True,
+                           bcs={"wall": ("noslip",), "transmissive": FREESTREAM_VISC_NT, "inlet": FREESTREAM_VISC_NT,
+                                "outlet": FREESTREAM_VISC_NT}),
 }
 
 def airfoil_as_channel(m):

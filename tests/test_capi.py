@@ -29,12 +29,16 @@ def test_every_declared_symbol_is_exported():
 
 def test_abi_version_and_defaults():
     L = _lib.load()
-    assert L.fvh_abi_version() == 1
+    assert L.fvh_abi_version() == 2
     c = _lib.Config()
     L.fvh_config_default(C.byref(c))
     assert c.n_alpha == 5 and list(c.alpha)[:5] == [0.0695, 0.1602, 0.2898, 0.506, 1.0]
     assert c.K == 7.5 and c.gamma == 1.4 and c.phi_yt == 2.0 and c.dt_fixed == 2e-5
     assert c.renumber == 2 and c.flux_mode == 0
+    # the laminar constants of the flat-plate example
+    assert (c.mu, c.pr, c.cp, c.r_gas) == (2e-5, 0.72, 1.0, 8.3145)
+    import ctypes
+    assert ctypes.sizeof(_lib.Config) == 1856   # sizeof(fvh_config) in C
 
 
 def test_mesh_entry_points_without_gpu():

@@ -13,6 +13,8 @@ from helpers import AIRFOIL_MESHES, CONFIGS, MESHES, airfoil_bump_state, bits_eq
 @pytest.mark.parametrize("mesh_name", sorted(MESHES))
 @pytest.mark.parametrize("cfg_name", sorted(CONFIGS))
 def test_template_restatement_is_the_oracle(mesh_name, cfg_name):
+    if CONFIGS[cfg_name].get("flux") == "roe_viscous":
+        pytest.skip("the complex-step restatement serves the adjoint, which takes the inviscid fluxes")
     m = MESHES[mesh_name]()
     om = O.Mesh(m)
     cfg = CONFIGS[cfg_name]

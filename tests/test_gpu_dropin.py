@@ -1,5 +1,5 @@
 """Drop-in parity: the reference's own example programs
-(examples/{shockTube,forwardStep,naca}/main.cpp), compiled unchanged once
+(examples/{shockTube,forwardStep,naca,flatPlate,nacaTransient}/main.cpp), compiled unchanged once
 against the reference library and once against this repository's fvhyper
 headers + libfvhyper.so (oracle/Makefile target `dropin`), run on the same
 meshes.  The printed step/time columns and every VTU file they write must be
@@ -29,6 +29,12 @@ CASES = {
     "forwardStep": ("step", lambda: meshgen.forward_step(n_per_unit=10)),
     "naca": ("naca", lambda: meshgen.MeshArrays(*(lambda m: (m.x, m.y, m.cells, m.bnd, m.bnd_tag,
                                                              ["airfoil", "farfield"]))(meshgen.sod_tube(nx=24, ny=4)))),
+    # the viscous examples: laminar flux on diffusive face gradients, viscous
+    # time steps, no-slip wall and their farfield (30,000 and 15,000 steps)
+    "flatPlate": ("square", lambda: meshgen.flat_plate(nx=20, ny=6)),
+    "nacaTransient": ("naca0012", lambda: (lambda m: meshgen.MeshArrays(
+        m.x, m.y, m.cells, m.bnd, m.bnd_tag, ["wall" if n == "airfoil" else n for n in m.names]))(
+        meshgen.naca0012_sized(800))),
 }
 
 

@@ -2,6 +2,8 @@
 closed forms, invariants and an independent eigen-decomposition of the Roe
 flux, each chosen so that a plausible slip (a dropped term, a sign, a swapped
 operand) fails."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -372,3 +374,116 @@ def test_residual_smoothing_is_linear_and_keeps_interior_constants():
     np.testing.assert_allclose(sc[:nr][far], c[:nr][far], rtol=1e-14)
     # next to a wall the ghost's zero pulls the value down
     assert np.all(sc[:nr][near[:nr], 0] < 1.0)
+
+
+VISC = dict(flux="roe_viscous", gamma=G, mu=0.03, pr=0.72, cp=2.5, r_gas=8.3145)
+
+
+def test_face_gradient_keeps_the_mean_across_and_the_difference_quotient_along():
+    """gradient_for_diffusion: along the centre line the face gradient is the
+    difference quotient of the cell states, across it the mean of the cell
+    gradients."""
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        gxi, gyi, gxj, gyj, qi, qj = rng.standard_normal((6, 4))
+        ci, cj = rng.standard_normal(2), rng.standard_normal(2)
+        gx, gy = O.face_gradient(gxi, gyi, gxj, gyj, qi, qj, ci, cj)
+        t = (ci - cj) / np.linalg.norm(ci - cj)
+        along = gx * t[0] + gy * t[1]
+        np.testing.assert_allclose(along, (qi - qj) / np.linalg.norm(ci - cj), rtol=1e-12, atol=1e-12)
+        across = -gx * t[1] + gy * t[0]
+        np.testing.assert_allclose(across, 0.5 * (-(gxi + gxj) * t[1] + (gyi + gyj) * t[0]), rtol=1e-12, atol=1e-12)
+    # a linear field is reproduced exactly
+    b = rng.standard_normal((2, 4))
+    ci, cj = np.array([0.3, -0.1]), np.array([-0.2, 0.4])
+    q0 = rng.standard_normal(4)
+    gx, gy = O.face_gradient(b[0], b[1], b[0], b[1], q0 + b[0] * ci[0] + b[1] * ci[1],
+                             q0 + b[0] * cj[0] + b[1] * cj[1], ci, cj)
+    np.testing.assert_allclose(gx, b[0], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(gy, b[1], rtol=1e-12, atol=1e-12)
+
+
+def test_viscous_flux_of_a_shear_layer_and_of_heat_conduction():
+    """Closed forms of the laminar terms: a shear u = a y at constant rho and
+    p gives tau_xy = mu a (momentum) and u tau_xy (energy) through a face of
+    normal y; a still gas with a pressure gradient at constant rho conducts
+    k dT/dx = (cp mu / pr) (dp/dx) / (rho r)."""
+    rho, u, v, p, a = 1.3, 0.4, 0.0, 0.9, 2.0
+    q = prim(rho, u, v, p)
+    zero = np.zeros(4)
+    # conservative gradient of the shear layer: d(rho u)/dy = rho a, d(rho E)/dy = rho u a
+    gy = np.array([0.0, rho * a, 0.0, rho * u * a])
+    n = np.array([0.0, 1.0])
+    dv = O.flux_grad(VISC, q, q, zero, gy, n) - O.flux_grad(VISC, q, q, zero, zero, n)
+    mu = VISC["mu"]
+    np.testing.assert_allclose(dv, [0.0, -mu * a, 0.0, -u * mu * a], rtol=1e-13, atol=1e-16)
+    # inviscid part: the viscous kind with a zero gradient is the split Roe flux
+    assert np.array_equal(O.flux_grad(VISC, q, q, zero, zero, n), O.flux(CFG_SPLIT, q, q, n))
+    # heat conduction: u = v = 0, dp/dx = b, rho constant -> d(rho E)/dx = b/(g-1)
+    b = 0.7
+    q = prim(rho, 0.0, 0.0, p)
+    gx = np.array([0.0, 0.0, 0.0, b / (G - 1)])
+    n = np.array([1.0, 0.0])
+    dv = O.flux_grad(VISC, q, q, gx, zero, n) - O.flux_grad(VISC, q, q, zero, zero, n)
+    k = VISC["cp"] * mu / VISC["pr"]
+    np.testing.assert_allclose(dv, [0.0, 0.0, 0.0, -k * b / (rho * VISC["r_gas"])], rtol=1e-13, atol=1e-16)
+    # bulk compression: u = c x gives tau_xx = 2 mu (c - c/3) = 4 mu c / 3
+    c = 0.5
+    q = prim(rho, 0.0, 0.0, p)
+    gx = np.array([0.0, rho * c, 0.0, 0.0])
+    dv = O.flux_grad(VISC, q, q, gx, zero, n) - O.flux_grad(VISC, q, q, zero, zero, n)
+    np.testing.assert_allclose(dv[1], -4.0 * mu * c / 3.0, rtol=1e-13)
+
+
+def test_viscous_time_steps_reduce_to_the_inviscid_forms_without_viscosity():
+    """mu -> 0: the summed viscous step is the summed CFL step; the viscous
+    edge-minimum step is cfl*A / ((c + |u n0| + |v n1|) len) per edge."""
+    m = meshgen.mixed_channel(nx=8, ny=4)
+    om = O.Mesh(m)
+    q = random_state(om.n_cells, seed=21, mach=0.4)
+    base = dict(gamma=G, cfl=0.7, bcs={})
+    dt_sum = np.zeros(om.n_cells)
+    dt_vs = np.zeros(om.n_cells)
+    c1 = O.make_config(dict(base, dt="sum_cfl"), [])
+    c2 = O.make_config(dict(base, dt="sum_viscous", mu=0.0), [])
+    O.lib().orc_calc_dt(om._p, C.byref(c1), q, dt_sum)
+    O.lib().orc_calc_dt(om._p, C.byref(c2), q, dt_vs)
+    assert np.array_equal(dt_sum, dt_vs)
+    # viscous edge-minimum by brute force, with viscosity
+    mu, pr = 0.05, 0.72
+    c3 = O.make_config(dict(base, dt="edge_min_viscous", mu=mu, pr=pr), [])
+    dt3 = np.zeros(om.n_cells)
+    O.lib().orc_calc_dt(om._p, C.byref(c3), q, dt3)
+    Q = q.reshape(-1, 4)
+    want = np.ones(om.n_cells)
+    ec = om.edge_cells.reshape(-1, 2)
+    for e, (i, j) in enumerate(ec):
+        le, nx, ny = om.edge_len[e], om.edge_nx[e], om.edge_ny[e]
+        for cidx in (i, j):
+            rho, mx, my, E = Q[cidx]
+            pp = (G - 1) * (E - 0.5 * (mx * mx + my * my) / rho)
+            cs = np.sqrt(G * pp / rho)
+            conv = (cs + abs(nx * mx / rho) + abs(ny * my / rho)) * le
+            visc = max(4.0 / (3.0 * rho), G / rho) * mu / pr * le * le / om.cell_area[cidx]
+            want[cidx] = min(want[cidx], 0.7 * om.cell_area[cidx] / (conv + 2.0 * visc))
+    np.testing.assert_allclose(dt3, want, rtol=1e-13)
+
+
+def test_noslip_and_viscous_farfield_boundaries():
+    q = prim(1.2, 0.3, -0.2, 0.8)
+    n = np.array([0.6, 0.8])
+    b = O.bc(("noslip",), G, q, n)
+    assert np.array_equal(b, [q[0], -q[1], -q[2], q[3]])
+    # the b[0] farfield differs from the naca spelling in the v momentum only
+    ff = (1.4, 0.2, 0.0, 1.0)
+    for qq, nn in ((prim(1.1, 0.1, 0.05, 0.95), np.array([-1.0, 0.0])),
+                   (prim(1.1, 0.1, 0.05, 0.95), np.array([0.0, 1.0]))):
+        a = O.bc(("farfield_v",) + ff, G, qq, nn)
+        c = O.bc(("farfield",) + ff, G, qq, nn)
+        assert np.array_equal(a[[0, 1]], c[[0, 1]])
+        # b[2] = b[0] * w there, b[1] * w in the naca spelling
+        np.testing.assert_allclose(a[2] * c[1], c[2] * a[0], rtol=1e-14)
+        # and the same boundary pressure
+        pa = (G - 1) * (a[3] - 0.5 / a[0] * (a[1] ** 2 + a[2] ** 2))
+        pc = (G - 1) * (c[3] - 0.5 / c[0] * (c[1] ** 2 + c[2] ** 2))
+        np.testing.assert_allclose(pa, pc, rtol=1e-13)
