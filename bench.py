@@ -411,7 +411,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default=None, choices=["f64", "f32"],
                     help="default: f64 (f32 for the sweep, whose state is FP32)")
-    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint"])
+    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint", "plate"])
+    ap.add_argument("--smooth", action="store_true", help="plate: with residual smoothing")
     ap.add_argument("--adj-steps", type=int, default=80, help="adjoint: RK5 steps of the objective's run")
     ap.add_argument("--adj-dirs", type=int, default=64, help="adjoint: random directions checked")
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
@@ -451,6 +452,8 @@ def main():
         return adjoint(a)
     if a.workload == "step":
         m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
+    elif a.workload == "plate":
+        m, cfg, ic = workloads.flat_plate(smooth=a.smooth)
     else:
         m, cfg, ic = workloads.naca_airfoil(a.cells)
     pm = Mesh.from_arrays(m)
@@ -500,6 +503,29 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_stage (fused gradient + limiter + flux + RK update + boundary, one launch per stage)",
                 "bytes_per_launch": ab["stage"], "avg_launch_us": round(avg_ms * 1e3, 2)}
+    elif kt and a.workload == "plate":
+        # viscous two-pass stage: the gradient pass also writes gx, gy; the
+        # flux pass reads them and the cell centres (face gradients)
+        V = 4
+        H = 2 * E - B
+        per = {"gradlim": ab["gradlim"] + C * 2 * V * sb,
+               "flux": ab["flux"] + C * (2 * V * sb + 2 * sb)}
+        if a.smooth:
+            # two sweeps: qt in and out, the half-edge lists; the second also
+            # finishes the stage (q0 read, q written, dt, area)
+            per["smooth"] = (C * (2 * V * sb + 4) + H * 8 + C * (4 * V * sb + 4 + 2 * sb) + H * 8) // 2
+        tot = {k: kt[k]["total_ms"] for k in per if kt.get(k, {}).get("count", 0)}
+        key = max(tot, key=tot.get)
+        avg_ms = kt[key]["total_ms"] / kt[key]["count"]
+        ach = per[key] / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_" + key + " (viscous two-pass stage" + (", residual smoothing" if a.smooth else "") + ")",
+                "bytes_per_launch": per[key], "avg_launch_us": round(avg_ms * 1e3, 2),
+                "others": {k: {"bytes_per_launch": per[k],
+                               "avg_launch_us": round(kt[k]["total_ms"] / kt[k]["count"] * 1e3, 2),
+                               "frac": round(per[k] / (kt[k]["total_ms"] / kt[k]["count"] * 1e-3) / 1e9
+                                             / HBM_PEAK_GBS, 4)} for k in tot if k != key}}
     elif kt and kt.get("flux", {}).get("count", 0):
         avg_ms = kt["flux"]["total_ms"] / kt["flux"]["count"]
         ach = ab["flux"] / (avg_ms * 1e-3) / 1e9
@@ -509,7 +535,8 @@ def main():
                 "bytes_per_launch": ab["flux"], "avg_launch_us": round(avg_ms * 1e3, 2)}
     if roof is not None:
         roof["events_window_steps"] = max(20, a.steps // 10)
-        tr = pmc_traffic(a.workload, a.precision, "stage" if "stage" in roof["kernel"] else "flux")
+        tr = pmc_traffic(a.workload, a.precision, "stage" if "stage" in roof["kernel"] else "flux") \
+            if a.workload != "plate" else None
         if tr is not None and (a.workload != "step" or a.n_per_unit == 440) and (
                 a.workload != "airfoil" or a.cells == 2_000_000):
             roof["traffic"] = tr["bytes"]
@@ -521,8 +548,13 @@ def main():
     if a.converge:
         s.set_state(q0)
         conv = converge(s, C, stages, 1e-6, a.stop_var)
-    cpu = None if a.no_cpu else cpu_baseline(m, cfg, a.cpu_steps, a.workload)
-    if a.workload == "step":
+    cpu = None if (a.no_cpu or a.workload == "plate") else cpu_baseline(m, cfg, a.cpu_steps, a.workload)
+    if a.workload == "plate":
+        wl = {"workload": "laminar_flat_plate", "cells": C, "edges": E, "boundary_edges": B, "order": 2,
+              "limiter": "michalak", "flux": "roe + laminar viscous (diffusive face gradients)",
+              "residual_smoothing": a.smooth, "rk_stages": stages, "cfl": cfg["cfl"], "dt": "local viscous",
+              "precision": a.precision, "stage_form": "tile", "parallelism": "single_gpu"}
+    elif a.workload == "step":
         wl = {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
               "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
               "cfl": cfg["cfl"], "dt": "global", "precision": a.precision, "stage_form": a.flux_mode,

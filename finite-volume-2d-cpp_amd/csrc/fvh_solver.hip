@@ -456,9 +456,12 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
                 launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs, qin, q0, out, face,
                                bincr, dt, gdt ? dtg : nullptr, a, nullptr, nullptr, cc, qs0, gxv, gyv);
             }
+            {
+                Bracket b(s, K_SMOOTH);
+                launch_smooth<T>(st, m, tm, pp, lin, bcs, qs0, qs0, qs1, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
+                                 nullptr, nullptr, cc);
+            }
             Bracket b(s, K_SMOOTH);
-            launch_smooth<T>(st, m, tm, pp, lin, bcs, qs0, qs0, qs1, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
-                             nullptr, nullptr, cc);
             launch_smooth<T>(st, m, tm, pp, lin, bcs, qs1, qs0, nullptr, q0, out, bincr, dt, gdt ? dtg : nullptr, a,
                              res, dbg ? dbg[3] : nullptr, cc);
         } else {

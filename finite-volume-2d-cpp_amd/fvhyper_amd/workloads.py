@@ -174,6 +174,23 @@ def naca_airfoil(total_cells: int = 2_000_000, mach: float = 0.8, aoa_deg: float
     return m, cfg, uniform_ic(rho, u, v, p)
 
 
+def flat_plate(nx: int = 1000, ny: int = 500, smooth: bool = False) -> Tuple[meshgen.MeshArrays, Dict, Callable]:
+    """Laminar flat plate with the physics of examples/flatPlate/main.cpp:
+    split Roe flux plus laminar terms on diffusive face gradients (mu 2e-5,
+    Pr 0.72, cp 1, R 8.3145), Michalak limiter, viscous edge-minimum local
+    dt at CFL 2, no-slip plate (bot1), slip wall upstream (bot0), farfield
+    (rho 1.4, u 0.2, p 1) elsewhere; uniform freestream initial state
+    (flatPlate/main.cpp:98-109).  nx x ny squares of [0,1.5]x[0,0.5], two
+    triangles each: 1,000,000 cells at the default size.  ``smooth`` adds
+    the residual smoothing of explicit.cpp:32-66."""
+    m = meshgen.flat_plate(nx=nx, ny=ny)
+    ff = ("farfield_v", 1.4, 0.2, 0.0, 1.0)
+    cfg = dict(flux="roe_viscous", dt="edge_min_viscous", cfl=2.0, phi="michalak", grad=True, lim=True, lin=True,
+               diffusive=True, gdt=False, smooth=smooth, mu=2e-5, pr=0.72, cp=1.0, r_gas=8.3145, alpha=RK5,
+               bcs={"top": ff, "left": ff, "right": ff, "bot0": ("mirror",), "bot1": ("noslip",)})
+    return m, cfg, uniform_ic(1.4, 0.2, 0.0, 1.0)
+
+
 def naca_sweep(n: int = 256, seed: int = 20240604, min_cells: int = 5000, max_cells: int = 200000,
                mach_range=(0.5, 0.85), aoa_range=(-2.0, 4.0)):
     """Parameter sweep over coarsened airfoil meshes: n instances with cell
