@@ -123,6 +123,15 @@ PMC_PROFILES = {
 }
 
 
+def achievable(roof):
+    """Beside the published peak, the bandwidth an in-repo device copy
+    kernel reaches on this GPU (fvh_stream_copy: 1 GB, read + write)."""
+    from fvhyper_amd import stream_copy_gbs
+    g = stream_copy_gbs(1024.0, 20)
+    roof["peak_achievable"] = round(g, 1)
+    roof["frac_achievable"] = round(roof["achieved"] / g, 4)
+
+
 def calibrate_copy(a):
     """A plain device copy of --calibrate-copy-mb MiB (three times), whose
     FETCH_SIZE / WRITE_SIZE counters scale the PMC traffic of a profiled run."""
@@ -272,6 +281,7 @@ def sweep(a):
                            "k_flux (edge-flux gather + RK update + boundary)") + ", all instances",
                 "bytes_per_launch": nbytes, "avg_launch_us": round(avg_ms * 1e3, 2),
                 "kernels_us": {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}}
+        achievable(roof)
         tr = pmc_traffic("sweep", a.precision, key) if a.instances == 256 else None
         if tr is not None:
             roof["traffic"] = tr["bytes"]
@@ -534,6 +544,7 @@ def main():
                 "kernel": "k_flux (edge-flux gather + RK update + boundary)",
                 "bytes_per_launch": ab["flux"], "avg_launch_us": round(avg_ms * 1e3, 2)}
     if roof is not None:
+        achievable(roof)
         roof["events_window_steps"] = max(20, a.steps // 10)
         tr = pmc_traffic(a.workload, a.precision, "stage" if "stage" in roof["kernel"] else "flux") \
             if a.workload != "plate" else None

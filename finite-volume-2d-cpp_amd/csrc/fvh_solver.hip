@@ -1556,6 +1556,25 @@ int32_t adjoint_check(fvh_solver* s) {
 // ---------------------------------------------------------------------------
 using namespace fvh;
 
+namespace fvh {
+namespace {
+typedef float v4f __attribute__((ext_vector_type(4)));
+// grid-stride copy, 16 B per lane per iteration; NT: non-temporal loads and
+// stores (the data is touched once)
+template <bool NT>
+__global__ void k_stream_copy(const v4f* __restrict__ a, v4f* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (NT) {
+            const v4f v = __builtin_nontemporal_load(a + i);
+            __builtin_nontemporal_store(v, b + i);
+        } else {
+            b[i] = a[i];
+        }
+    }
+}
+}  // namespace
+}  // namespace fvh
+
 extern "C" {
 
 void fvh_config_default(fvh_config* c) {
@@ -1891,6 +1910,70 @@ int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
     if (bad_options(o)) return FVH_E_ARG;
     return s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st);
+}
+
+int32_t fvh_solver_run_log(fvh_solver* s, const fvh_run_options* o, fvh_row_cb cb, void* user, fvh_status* st) {
+    if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
+    if (bad_options(o)) return FVH_E_ARG;
+    int32_t rc = fvh_solver_run_begin(s);
+    if (rc) return rc;
+    const uint32_t print = o->print_interval ? o->print_interval : 1;
+    uint32_t done = 0;
+    for (;;) {
+        // to the next step the reference prints: step 0, then every print-th
+        const uint32_t n = done == 0 ? 1 : ((done - 1) / print + 1) * print + 1 - done;
+        if ((rc = s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st))) return rc;
+        if (st->step <= done) break;
+        done = st->step;
+        const uint32_t step = done - 1;
+        if (cb && (step == 0 || step % print == 0)) cb(step, st, user);
+        if (st->stopped) break;
+    }
+    return FVH_OK;
+}
+
+int32_t fvh_stream_copy(double mbytes, int32_t reps, double* gbs) {
+    if (!(mbytes >= 1.0) || reps < 1 || !gbs) { set_error("bad argument"); return FVH_E_ARG; }
+    using fvh::v4f;
+    const size_t n = (size_t)(mbytes * 1e6) / sizeof(v4f);
+    DevBuf a, b;
+    if (a.alloc(n * sizeof(v4f)) || b.alloc(n * sizeof(v4f))) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+    FVH_HIP(hipMemset(a.p, 0, a.bytes));
+    hipStream_t st = nullptr;
+    FVH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int32_t rc = FVH_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = FVH_E_HIP;
+    // the best of a few launch shapes: 8, 32 and 128 workgroups of 256 per CU
+    // (256 CUs) or one lane per element, plain and non-temporal
+    double best = 0.0;
+    const size_t shapes[4] = {256 * 8, 256 * 32, 256 * 128, (n + 255) / 256};
+    for (int nt = 0; nt < 2 && !rc; ++nt)
+        for (size_t g : shapes) {
+            const int blocks = (int)std::min<size_t>(std::max<size_t>(g, 1), (n + 255) / 256);
+            auto launch = [&]() {
+                if (nt) hipLaunchKernelGGL(k_stream_copy<true>, dim3(blocks), dim3(256), 0, st, (const v4f*)a.p,
+                                           (v4f*)b.p, n);
+                else hipLaunchKernelGGL(k_stream_copy<false>, dim3(blocks), dim3(256), 0, st, (const v4f*)a.p,
+                                        (v4f*)b.p, n);
+            };
+            float ms = 0.f;
+            launch();
+            if (hipEventRecord(e0, st) != hipSuccess) { rc = FVH_E_HIP; break; }
+            for (int r = 0; r < reps; ++r) launch();
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess) {
+                rc = FVH_E_HIP;
+                break;
+            }
+            best = std::max(best, 2.0 * (double)(n * sizeof(v4f)) * reps / (ms * 1e-3) / 1e9);
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (rc) { set_error("stream copy failed"); return rc; }
+    *gbs = best;
+    return FVH_OK;
 }
 
 int32_t fvh_solver_advance(fvh_solver* s, int32_t n) {

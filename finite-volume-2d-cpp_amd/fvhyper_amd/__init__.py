@@ -6,6 +6,7 @@ generators and workload definitions.
 """
 from . import meshgen, workloads
 from ._lib import FvhError, load
-from .solver import Batch, Mesh, Solver, autotune, make_config
+from .solver import Batch, Mesh, Solver, autotune, make_config, stream_copy_gbs
 
-__all__ = ["meshgen", "workloads", "Mesh", "Solver", "Batch", "autotune", "make_config", "load", "FvhError"]
+__all__ = ["meshgen", "workloads", "Mesh", "Solver", "Batch", "autotune", "make_config", "stream_copy_gbs", "load",
+           "FvhError"]

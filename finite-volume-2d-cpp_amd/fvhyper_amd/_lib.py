@@ -61,13 +61,16 @@ class Status(C.Structure):
                 ("time_prev", C.c_double), ("R", C.c_double * 4), ("R0", C.c_double * 4)]
 
 
+ROW_CB = C.CFUNCTYPE(None, C.c_uint32, C.POINTER(Status), C.c_void_p)
+
 # every symbol include/fvh.h declares
 EXPORTS = [
     "fvh_last_error", "fvh_abi_version", "fvh_mesh_read_msh", "fvh_mesh_read_su2", "fvh_mesh_from_arrays",
     "fvh_mesh_free", "fvh_mesh_counts", "fvh_mesh_export", "fvh_mesh_name",
     "fvh_config_default", "fvh_solver_create", "fvh_solver_free", "fvh_solver_set_state",
     "fvh_solver_get_state", "fvh_solver_save", "fvh_solver_load", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
-    "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
+    "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_run_log",
+    "fvh_stream_copy", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
     "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
@@ -118,6 +121,8 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_advance.argtypes = [vp, C.c_int32]
     L.fvh_solver_run_begin.argtypes = [vp]
     L.fvh_solver_run_continue.argtypes = [vp, C.POINTER(RunOptions), C.c_uint32, C.POINTER(Status)]
+    L.fvh_solver_run_log.argtypes = [vp, C.POINTER(RunOptions), ROW_CB, vp, C.POINTER(Status)]
+    L.fvh_stream_copy.argtypes = [C.c_double, C.c_int32, dp]
     L.fvh_solver_synchronize.argtypes = [vp]
     L.fvh_solver_status.argtypes = [vp, C.POINTER(Status)]
     L.fvh_solver_profile.argtypes = [vp, C.c_int32]

@@ -226,6 +226,21 @@ class Solver:
         check(_lib.load().fvh_solver_run_continue(self._h, C.byref(o), int(n), C.byref(st)))
         return self._status_dict(st)
 
+    def run_log(self, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
+                print_interval: int = 1, stop_var: int = -1):
+        """run() with the reference's printed rows: returns (status, rows), a
+        row (step, time before the step, R[4]) after step 0 and after every
+        print_interval-th step (explicit.cpp:752-785)."""
+        o = _lib.RunOptions(float(max_time), int(max_step), int(print_interval), float(tol), int(stop_var), 0)
+        st = _lib.Status()
+        rows = []
+
+        def row(step, p, _user):
+            rows.append((int(step), float(p.contents.time_prev), list(p.contents.R)))
+        cb = _lib.ROW_CB(row)
+        check(_lib.load().fvh_solver_run_log(self._h, C.byref(o), cb, None, C.byref(st)))
+        return self._status_dict(st), rows
+
     def advance(self, n: int) -> None:
         check(_lib.load().fvh_solver_advance(self._h, int(n)))
 
@@ -348,6 +363,13 @@ class Batch(Solver):
         out = np.zeros(4 * len(self.meshes))
         check(_lib.load().fvh_batch_sums(self._h, dptr(out)))
         return out.reshape(-1, 4)
+
+
+def stream_copy_gbs(mbytes: float = 1024.0, reps: int = 20) -> float:
+    """Achievable HBM bandwidth of a plain device copy (read + write GB/s)."""
+    g = C.c_double(0.0)
+    check(_lib.load().fvh_stream_copy(float(mbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 def autotune(meshes: List[Mesh], cfgs: List[Dict], q: np.ndarray, precision: str = "f64", steps: int = 10,

@@ -234,6 +234,20 @@ int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st);
 int32_t fvh_solver_run_begin(fvh_solver* s);
 int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_t n, fvh_status* st);
 
+/* run() with its printed rows as a callback (explicit.cpp:752-785): applies
+ * the initial boundary update, steps until the stop test holds, and calls
+ * cb(step, status, user) after step 0 and after every step whose index is a
+ * multiple of print_interval, with that step's residuals (R0 after step 0).
+ * cb may be NULL. */
+typedef void (*fvh_row_cb)(uint32_t step, const fvh_status* st, void* user);
+int32_t fvh_solver_run_log(fvh_solver* s, const fvh_run_options* o, fvh_row_cb cb, void* user, fvh_status* st);
+
+/* Measurement helper without a reference counterpart: the HBM bandwidth a
+ * plain device-to-device copy of mbytes MB achieves on the current device,
+ * read plus write bytes per second over reps launches (GB/s), the best of
+ * a few launch shapes with plain and non-temporal accesses. */
+int32_t fvh_stream_copy(double mbytes, int32_t reps, double* gbs);
+
 /* Enqueue n full steps (dt, all RK stages, swap) without residuals or stop
  * test; returns without waiting.  Used for fixed-length timing. */
 int32_t fvh_solver_advance(fvh_solver* s, int32_t n);

@@ -149,3 +149,28 @@ def test_graph_replay_is_bit_exact(batched):
     e.advance(5)
     g.synchronize()
     assert bits_equal(g.get_state(), e.get_state())
+
+
+@pytest.mark.parametrize("extra", ["smooth", "viscous"])
+def test_batch_with_smoothing_or_viscous_terms_matches_oracle(extra):
+    """Residual smoothing and the viscous flux in a batch: each instance's
+    sweeps and face gradients stay inside it (bit-identical to the oracle)."""
+    print("case batch", extra, flush=True)
+    items = _sweep(3, seed=13)
+    cfgs = []
+    for it in items:
+        c = dict(it[1])
+        if extra == "smooth":
+            c["smooth"] = True
+        else:
+            c.update(flux="roe_viscous", diffusive=True, mu=0.01, pr=0.72, cp=1.0, r_gas=2.0)
+        cfgs.append(c)
+    meshes = [Mesh.from_arrays(it[0]) for it in items]
+    b = Batch(meshes, cfgs)
+    q0 = _states(meshes, items)
+    b.set_state(b.join(q0))
+    b.run(max_step=6)
+    qb = b.split(b.get_state())
+    for k, it in enumerate(items):
+        qo, *_ = O.Mesh(it[0]).run(cfgs[k], q0[k], 6)
+        assert bits_equal(qb[k], qo), k

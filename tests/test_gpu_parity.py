@@ -250,6 +250,34 @@ def test_viscous_fp32_against_fp64_oracle(precision):
     assert err < 2e-5
 
 
+def test_run_log_rows_follow_the_print_interval():
+    """run_log: a row after step 0 (R = R0) and after every 4th step, and
+    the same end state as run()."""
+    cfg = CONFIGS["second_venkat_global"]
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    pm = Mesh.from_arrays(m)
+    q0 = random_state(om.n_cells, seed=808, mach=0.3)
+    a = Solver(pm, cfg)
+    a.set_state(q0)
+    st, rows = a.run_log(max_step=13, print_interval=4)
+    assert [r[0] for r in rows] == [0, 4, 8, 12]
+    assert bits_equal(rows[0][2], st["R0"])
+    b = Solver(pm, cfg)
+    b.set_state(q0)
+    st2 = b.run(max_step=13, print_interval=4)
+    assert bits_equal(a.get_state(), b.get_state()) and st["step"] == st2["step"] == 13
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 13, print_interval=4)
+    assert bits_equal(a.get_state(), qo)
+
+
+def test_stream_copy_reaches_hbm_rates():
+    from fvhyper_amd import stream_copy_gbs
+    g = stream_copy_gbs(1024.0, 3)
+    print("stream copy GB/s", g)
+    assert 1000.0 < g < 12000.0
+
+
 def test_residual_smoothing_changes_the_update():
     """The smoothing configs really smooth: the same run without the flag
     differs (guards against a flag that never reaches the kernels)."""
