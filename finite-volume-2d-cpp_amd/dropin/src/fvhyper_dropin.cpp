@@ -13,6 +13,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <stdexcept>
@@ -200,8 +201,21 @@ void run(const std::string name, std::vector<double>& q, mpi_wrapper& pool, mesh
 // ---------------------------------------------------------------------------
 // writeVtk (post.cpp:21-200): same names, layout and std::to_string numbers
 // ---------------------------------------------------------------------------
+bool vtk_full_precision = false;
+
 void writeVtk(const std::string name, std::vector<double>& q, mesh& m, int rank, int world_size,
               const std::string time) {
+    // full precision (an extension, off by default): %.17g numbers and
+    // Float64 arrays, for post-processing that needs the exact state
+    const char* fp = std::getenv("FVH_VTK_FULL_PRECISION");
+    const bool full = vtk_full_precision || (fp && fp[0] == '1');
+    auto num = [full](double x) {
+        if (!full) return std::to_string(x);
+        char buf[40];
+        std::snprintf(buf, sizeof buf, "%.17g", x);
+        return std::string(buf);
+    };
+    const std::string ftype = full ? "Float64" : "Float32";
     const std::string dash_time = time.empty() ? "" : "_" + time;
     std::string filename = (world_size > 1) ? name + "_" + std::to_string(rank) + dash_time + ".vtu"
                                             : name + dash_time + ".vtu";
@@ -215,9 +229,9 @@ void writeVtk(const std::string name, std::vector<double>& q, mesh& m, int rank,
     s += "    <Piece NumberOfPoints=\"" + std::to_string(m.nodesX.size()) + "\" NumberOfCells=\"" +
          std::to_string(N) + "\">\n";
     s += "      <Points>\n";
-    s += "        <DataArray type=\"Float32\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
+    s += "        <DataArray type=\"" + ftype + "\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
     for (size_t i = 0; i < m.nodesX.size(); ++i)
-        s += "          " + std::to_string(m.nodesX[i]) + " " + std::to_string(m.nodesY[i]) + " 0.0\n";
+        s += "          " + num(m.nodesX[i]) + " " + num(m.nodesY[i]) + " 0.0\n";
     s += "        </DataArray>\n";
     s += "      </Points>\n";
     s += "      <Cells>\n";
@@ -249,26 +263,26 @@ void writeVtk(const std::string name, std::vector<double>& q, mesh& m, int rank,
     for (uint c = 0; c < N; ++c) s += "          " + std::to_string(rank) + "\n";
     s += "        </DataArray>\n";
     for (size_t k = 0; k < V; ++k) {
-        s += "        <DataArray type=\"Float32\" Name=\"" + var_names[k] + "\" Format=\"ascii\">\n";
-        for (uint c = 0; c < N; ++c) s += "          " + std::to_string(q[V * c + k]) + "\n";
+        s += "        <DataArray type=\"" + ftype + "\" Name=\"" + var_names[k] + "\" Format=\"ascii\">\n";
+        for (uint c = 0; c < N; ++c) s += "          " + num(q[V * c + k]) + "\n";
         s += "        </DataArray>\n";
     }
     for (auto& kv : post::extra_scalars) {
-        s += "        <DataArray type=\"Float32\" Name=\"" + kv.first + "\" Format=\"ascii\">\n";
+        s += "        <DataArray type=\"" + ftype + "\" Name=\"" + kv.first + "\" Format=\"ascii\">\n";
         for (uint c = 0; c < N; ++c) {
             double out[1];
             kv.second(out, &q[V * c]);
-            s += "          " + std::to_string(out[0]) + "\n";
+            s += "          " + num(out[0]) + "\n";
         }
         s += "        </DataArray>\n";
     }
     for (auto& kv : post::extra_vectors) {
-        s += "        <DataArray type=\"Float32\" Name=\"" + kv.first +
+        s += "        <DataArray type=\"" + ftype + "\" Name=\"" + kv.first +
              "\" NumberOfComponents=\"3\" Format=\"ascii\">\n";
         for (uint c = 0; c < N; ++c) {
             double out[2];
             kv.second(out, &q[V * c]);
-            s += "          " + std::to_string(out[0]) + " " + std::to_string(out[1]) + " 0.0\n";
+            s += "          " + num(out[0]) + " " + num(out[1]) + " 0.0\n";
         }
         s += "        </DataArray>\n";
     }

@@ -71,3 +71,41 @@ def test_reference_example_runs_unchanged_on_the_gpu(example, tmp_path):
     assert sorted(files_ref) == sorted(files_gpu) and files_ref
     for k in files_ref:
         assert files_ref[k] == files_gpu[k], k
+
+
+def _vtu_arrays(text):
+    """name -> list of number strings of every DataArray of a VTU file."""
+    import re
+    out = {}
+    for m in re.finditer(r'<DataArray type="(\w+)"(?: Name="(\w+)")?[^>]*>(.*?)</DataArray>', text, re.S):
+        out[m.group(2) or "points"] = (m.group(1), m.group(3).split())
+    return out
+
+
+def test_full_precision_vtu_rounds_to_the_reference_text(tmp_path):
+    """FVH_VTK_FULL_PRECISION=1: the same files with Float64 arrays whose
+    %.17g numbers print, through std::to_string's %f, as the default text."""
+    dropin_bin = os.path.join(REF, "dropin_shockTube")
+    if not os.path.exists(dropin_bin):
+        pytest.skip("example binaries not built (make -C oracle dropin needs the reference tree)")
+    mesh_name, fac = CASES["shockTube"]
+    m = fac()
+    _, plain = _run(dropin_bin, str(tmp_path / "plain"), mesh_name, m)
+    os.environ["FVH_VTK_FULL_PRECISION"] = "1"
+    try:
+        _, full = _run(dropin_bin, str(tmp_path / "full"), mesh_name, m)
+    finally:
+        del os.environ["FVH_VTK_FULL_PRECISION"]
+    assert sorted(plain) == sorted(full) and plain
+    for k in plain:
+        a, b = _vtu_arrays(plain[k].decode()), _vtu_arrays(full[k].decode())
+        assert sorted(a) == sorted(b)
+        for name in a:
+            (ta, va), (tb, vb) = a[name], b[name]
+            if ta == "Int32" or name == "rank":
+                assert va == vb, name
+                continue
+            assert ta == "Float32" and tb == "Float64", name
+            assert len(va) == len(vb)
+            for x, y in zip(va, vb):
+                assert x == y or x == "%f" % float(y), (name, x, y)
