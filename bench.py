@@ -422,7 +422,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["f64", "f32"],
                     help="default: f64 (f32 for the sweep, whose state is FP32)")
     ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint", "plate"])
-    ap.add_argument("--smooth", action="store_true", help="plate: with residual smoothing")
+    ap.add_argument("--smooth", action="store_true", help="plate, airfoil: with residual smoothing")
+    ap.add_argument("--cfl", type=float, default=0.0, help="airfoil: CFL number instead of the workload's 2")
     ap.add_argument("--adj-steps", type=int, default=80, help="adjoint: RK5 steps of the objective's run")
     ap.add_argument("--adj-dirs", type=int, default=64, help="adjoint: random directions checked")
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
@@ -466,6 +467,10 @@ def main():
         m, cfg, ic = workloads.flat_plate(smooth=a.smooth)
     else:
         m, cfg, ic = workloads.naca_airfoil(a.cells)
+        if a.smooth:
+            cfg["smooth"] = True
+        if a.cfl > 0:
+            cfg["cfl"] = a.cfl
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode)
     q0 = workloads.initial_state(pm, ic)
@@ -574,7 +579,7 @@ def main():
         wl = {"workload": "naca0012_transonic", "cells": C, "edges": E, "boundary_edges": B,
               "order": 2, "limiter": "michalak", "mach": 0.8, "aoa_deg": 1.25, "rk_stages": stages,
               "cfl": cfg["cfl"], "dt": "local", "precision": a.precision, "stage_form": a.flux_mode,
-              "parallelism": "single_gpu"}
+              "residual_smoothing": bool(cfg.get("smooth")), "parallelism": "single_gpu"}
     out = {
         "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
