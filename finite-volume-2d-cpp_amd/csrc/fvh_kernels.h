@@ -41,7 +41,7 @@ constexpr uint32_t kEdgeMask = 0x3fffffffu;
 constexpr int kMaxHe = 4;   // half-edges of a real cell the adjoint takes (triangles and quadrilaterals)
 constexpr int kBlock = 256;        // cells per workgroup = cells per tile
 constexpr int kTileEdges = 1024;   // LDS edge slots per tile
-constexpr int kStageBlock = 512;   // threads of the fused stage pass (one tile per workgroup)
+constexpr int kStageBlock = 512;   // largest workgroup of the fused stage pass (one tile per workgroup)
 constexpr int kMaxBc = 16;
 
 // Per-run control block, advanced on the device so that steps can be queued
@@ -163,12 +163,14 @@ void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const
 // Fused stage: gradient + limiter (tile and ring 1, from LDS), face states,
 // edge fluxes, gather, RK update and boundary update in one pass per tile.
 // dt_kind >= 0 also computes the tile cells' local time step (first stage).
+// threads: workgroup size, 256 or 512; 0 picks 256 when four workgroups'
+// halos fit a CU's LDS, else 512.
 // Returns false when the tile halo does not fit the workgroup's LDS.
 template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
-                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl);
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0);
 template <class T>
 size_t stage_lds_bytes(const HaloView& hv);
 template <class T>

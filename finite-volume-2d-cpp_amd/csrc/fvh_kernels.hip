@@ -591,13 +591,14 @@ __global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, Ph
 // the flux in the slot; phase 3 gathers per cell in reference edge order and
 // finishes the stage.  No face state goes through HBM; the boundary
 // increments of the tile's own cells do (bincr), for phase 3.
-template <class T, int PHI, int DT, bool JOINED>
-__global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+template <class T, int PHI, int DT, bool JOINED, int NB>
+__global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
                                                   int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
                                                   const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
                                                   Q4<T>* __restrict__ qout, Q4<T>* __restrict__ bincr,
                                                   T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
                                                   double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl) {
+    static_assert(NB >= kBlock, "phase 3 gives each tile cell one lane");
     extern __shared__ __align__(32) unsigned char lds_raw[];
     const int lb = xcd_block(tm.n_rt);
     if (lb < 0) return;
@@ -612,11 +613,11 @@ __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm
     const T g = T(pp.gamma);
     const T lim0 = lim_on ? T(1) : T(0);
     // phase 0: states of tile, ring 1 and ring 2
-    for (int j = threadIdx.x; j < n_ext; j += kStageBlock) sq[j] = ld(qin + (j < nt ? c0 + j : xc[j - nt]));
+    for (int j = threadIdx.x; j < n_ext; j += NB) sq[j] = ld(qin + (j < nt ? c0 + j : xc[j - nt]));
     __syncthreads();
     // phase 1: gradients and limiters of the tile and its real ring-1 cells,
     // and their face states on the tile's edges
-    for (int j = threadIdx.x; j < n_g; j += kStageBlock) {
+    for (int j = threadIdx.x; j < n_g; j += NB) {
         const bool own = j < nt;
         const int c = own ? c0 + j : xc[j - nt];
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
@@ -663,7 +664,7 @@ __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm
     __syncthreads();
     // phase 2: each edge of the tile once, flux times length left in the slot
     const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
-    for (int k = e0 + (int)threadIdx.x; k < e1; k += kStageBlock) {
+    for (int k = e0 + (int)threadIdx.x; k < e1; k += NB) {
         const Q4<T> geo = m.e_geo[tv.edge[k]];
         const int sl = k - e0;
         const Q4<T> a = fs[2 * sl], b = fs[2 * sl + 1];
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(kStageBlock) void k_stage(MeshView<T> m, TileMap tm
                              res_blk != nullptr, rs, nullptr);
     }
     if (res_blk) {
-        block_sum4<kStageBlock>(rs);
+        block_sum4<NB>(rs);
         if (threadIdx.x == 0)
             for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
     }
@@ -928,22 +929,40 @@ size_t stage_lds_bytes(const HaloView& hv) {
     return (size_t)(hv.max_ext + 2 * hv.max_edges) * sizeof(Q4<T>);
 }
 
-template <class T, int PHI, int DT, bool J>
-bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
-                    const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
-                    const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
-                    double* res_blk, const Ctrl* ctrl) {
-    const size_t lds = stage_lds_bytes<T>(hv);
+template <class T, int PHI, int DT, bool J, int NB>
+void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
+                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
+                     const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
+                     double* res_blk, const Ctrl* ctrl, size_t lds) {
     static bool attr_set = false;
     if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 160 * 1024) != hipSuccess)
             (void)hipGetLastError();
         attr_set = true;
     }
     const int nb = xcd_grid_blocks(tm.n_rt);
-    hipLaunchKernelGGL((k_stage<T, PHI, DT, J>), dim3(nb), dim3(kStageBlock), lds, s, m, tm, tv, hv, pp, (int)grad,
+    hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
                        (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl);
+}
+
+// Workgroup size: a tile per workgroup either way.  With 256 lanes the
+// phases loop over a tile's cells and edges, and four workgroups share a
+// CU when their LDS fits (state registers allow 16 waves per CU): every
+// wave has work in every phase.  Tiles whose halo needs more than a
+// quarter of the CU's LDS run with 512 lanes, two workgroups per CU.
+template <class T, int PHI, int DT, bool J>
+bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
+                    const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
+                    const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
+                    double* res_blk, const Ctrl* ctrl, int threads) {
+    const size_t lds = stage_lds_bytes<T>(hv);
+    if (threads == kStageBlock / 2 || (threads == 0 && lds + 1024 <= 160 * 1024 / 4))
+        launch_stage_nb<T, PHI, DT, J, kStageBlock / 2>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
+                                                        dt, dt_inst, alpha, res_blk, ctrl, lds);
+    else
+        launch_stage_nb<T, PHI, DT, J, kStageBlock>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt,
+                                                    dt_inst, alpha, res_blk, ctrl, lds);
     return true;
 }
 
@@ -951,11 +970,11 @@ template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
-                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl) {
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads) {
     if (tm.n_rt == 0) return true;
     if (stage_lds_bytes<T>(hv) > 160 * 1024) return false;
 #define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt, \
-                                                  dt_inst, alpha, res_blk, ctrl)
+                                                  dt_inst, alpha, res_blk, ctrl, threads)
 #define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
 #define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
     switch (phi_kind) {
@@ -1024,7 +1043,7 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
     template bool launch_stage<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,          \
                                   const HaloView&, const PhysParams&, int, bool, bool, bool, int,            \
                                   const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, Q4<T>*, T*,        \
-                                  const T*, T, double*, const Ctrl*);                                                     \
+                                  const T*, T, double*, const Ctrl*, int);                                                     \
     template size_t stage_lds_bytes<T>(const HaloView&);
 FVH_INST(double)
 FVH_INST(float)

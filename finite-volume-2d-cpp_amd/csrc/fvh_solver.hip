@@ -426,7 +426,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
             launch_stage<T>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim,
                             (k == 0 && !gdt) ? c.dt_kind : -1, bcs, qin, q0, out, (Q4<T>*)s->bincr.p, dt,
                             gdt ? dtg : nullptr,
-                            T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc);
+                            T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc,
+                            c.stage_threads);
             qin = out;
             out = (out == qa) ? qb : qa;
         }
@@ -827,6 +828,10 @@ int32_t check_config(const fvh_config* c) {
         set_error("unknown flux, limiter, time-step, order or flux-mode kind");
         return FVH_E_ARG;
     }
+    if (c->stage_threads != 0 && c->stage_threads != kStageBlock / 2 && c->stage_threads != kStageBlock) {
+        set_error("stage_threads must be 0, 256 or 512");
+        return FVH_E_ARG;
+    }
     if (c->tile_cells != 0 && (c->tile_cells < 32 || c->tile_cells > kBlock)) {
         set_error("tile_cells must be 0 or 32..256");
         return FVH_E_ARG;
@@ -848,7 +853,7 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
         a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
         a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
         a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells || a.mu != b.mu || a.pr != b.pr ||
-        a.cp != b.cp || a.r_gas != b.r_gas)
+        a.cp != b.cp || a.r_gas != b.r_gas || a.stage_threads != b.stage_threads)
         return false;
     for (int k = 0; k < a.n_alpha; ++k)
         if (a.alpha[k] != b.alpha[k]) return false;

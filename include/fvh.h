@@ -123,6 +123,10 @@ typedef struct fvh_config {
      * steps (consts:: of flatPlate/main.cpp:28-35): viscosity, Prandtl number,
      * heat capacity, gas constant */
     double mu, pr, cp, r_gas;
+    /* workgroup size of the fused stage pass: 0 = by the tile halo's LDS
+     * (256 lanes when four workgroups fit a CU, else 512), or 256, 512 */
+    int32_t stage_threads;
+    int32_t reserved;          /* 0 */
 } fvh_config;
 
 /* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */

@@ -363,6 +363,24 @@ def test_tile_size_128_bit_exact(mesh_name, cfg_name, flux_mode):
     assert same_values(s.get_state(), qo)
 
 
+@pytest.mark.parametrize("threads", [256, 512])
+@pytest.mark.parametrize("mesh_name,cfg_name", [("step", "second_venkat_global"), ("su2m", "second_michalak_local"),
+                                               ("mixed", "grad_only_lin")])
+def test_fused_workgroup_sizes_bit_exact(mesh_name, cfg_name, threads):
+    """The fused stage pass with 256 and with 512 lanes per tile: same bits."""
+    print("case stage threads", mesh_name, cfg_name, threads, flush=True)
+    cfg = dict(CONFIGS[cfg_name], stage_threads=threads)
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    s = Solver(Mesh.from_arrays(m), cfg, flux_mode="fused")
+    q0 = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name in AIRFOIL_MESHES else \
+        random_state(om.n_cells, seed=505, mach=0.3)
+    s.set_state(q0)
+    s.run(max_step=8)
+    qo, *_ = om.run(CONFIGS[cfg_name], q0, 8)
+    assert same_values(s.get_state(), qo)
+
+
 def test_autotune_returns_a_working_choice():
     from fvhyper_amd import autotune
     m, cfg, ic = workloads.naca_airfoil(20000)
