@@ -435,6 +435,7 @@ def main():
     ap.add_argument("--flux-mode", default="auto", choices=["auto", "fused", "tile", "cell"],
                     help="step/airfoil: stage form (auto: fused where it fits)")
     ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
+    ap.add_argument("--tile-cells", type=int, default=256, help="step/airfoil: cells per tile (32..256)")
     ap.add_argument("--converge", action="store_true",
                     help="airfoil: also run from the freestream until the density residual drops 6 orders")
     ap.add_argument("--stop-var", type=int, default=0, help="with --converge: 0 density, -1 max over variables")
@@ -472,7 +473,7 @@ def main():
         if a.cfl > 0:
             cfg["cfl"] = a.cfl
     pm = Mesh.from_arrays(m)
-    s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode)
+    s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode, tile_cells=a.tile_cells)
     q0 = workloads.initial_state(pm, ic)
     s.set_state(q0)
     s.apply_bounds()

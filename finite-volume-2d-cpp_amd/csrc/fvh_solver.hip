@@ -1648,7 +1648,7 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
         return FVH_E_ARG;
     }
     const int32_t modes[3] = {FVH_FLUX_MODE_TILE, FVH_FLUX_MODE_FUSED, FVH_FLUX_MODE_CELL};
-    const int32_t tiles[2] = {128, 256};
+    const int32_t tiles[3] = {128, 192, 256};
     double best = 1e300;
     int32_t rows = 0;
     *best_mode = FVH_FLUX_MODE_TILE;
@@ -1659,7 +1659,7 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
     FVH_HIP(hipEventCreate(&e1));
     int32_t rc = FVH_OK;
     for (int32_t mi = 0; mi < 3 && rc == FVH_OK; ++mi)
-        for (int32_t ti = 0; ti < 2 && rc == FVH_OK; ++ti) {
+        for (int32_t ti = 0; ti < 3 && rc == FVH_OK; ++ti) {
             for (auto& c : cs) { c.flux_mode = modes[mi]; c.tile_cells = tiles[ti]; }
             fvh_solver* s = nullptr;
             double ms = -1.0;

@@ -277,7 +277,7 @@ double fvh_solver_stage_bytes(const fvh_solver* s);
 int32_t fvh_solver_use_graphs(fvh_solver* s, int32_t enable);
 
 /* Autotuner (no reference counterpart): times each candidate stage form
- * (TILE, FUSED, CELL) at each tile size (128, 256 cells) on these meshes and
+ * (TILE, FUSED, CELL) at each tile size (128, 192, 256 cells) on these meshes and
  * configs from state q (as for fvh_solver_set_state of the batch) for
  * `steps` steps, and returns the fastest in *best_mode / *best_tile.
  * report, if non-NULL, receives up to max_report rows (mode, tile,

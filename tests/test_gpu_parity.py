@@ -388,8 +388,8 @@ def test_autotune_returns_a_working_choice():
     q0 = workloads.initial_state(pm, ic)
     r = autotune([pm], [cfg], q0, steps=3)
     print("autotune", r)
-    assert r["flux_mode"] in ("tile", "fused", "cell") and r["tile_cells"] in (128, 256)
-    assert len(r["candidates"]) == 6 and all(c[2] > 0 for c in r["candidates"])
+    assert r["flux_mode"] in ("tile", "fused", "cell") and r["tile_cells"] in (128, 192, 256)
+    assert len(r["candidates"]) == 9 and all(c[2] > 0 for c in r["candidates"])
     s = Solver(pm, cfg, flux_mode=r["flux_mode"], tile_cells=r["tile_cells"])
     s.set_state(q0)
     s.run(max_step=5)
