@@ -18,11 +18,13 @@ namespace fvh {
 template <class T>
 FVH_HD void pressure_vjp(T g, const T* q, T pb, T* qb) {
     const T gm = g - T(1);
-    const T k = q[1] * q[1] + q[2] * q[2];
-    qb[0] = qb[0] + pb * gm * T(0.5) * k / (q[0] * q[0]);
-    qb[1] = qb[1] - pb * gm * q[1] / q[0];
-    qb[2] = qb[2] - pb * gm * q[2] / q[0];
-    qb[3] = qb[3] + pb * gm;
+    const T r = T(1) / q[0];
+    const T u = q[1] * r, v = q[2] * r;
+    const T pg = pb * gm;
+    qb[0] = qb[0] + pg * T(0.5) * (u * u + v * v);
+    qb[1] = qb[1] - pg * u;
+    qb[2] = qb[2] - pg * v;
+    qb[3] = qb[3] + pg;
 }
 
 // Reverse of roe_flux (both spellings have the same derivative): given fb
@@ -30,24 +32,30 @@ FVH_HD void pressure_vjp(T g, const T* q, T pb, T* qb) {
 template <class T>
 FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb, T* qib, T* qjb, T* nb) {
     // ---- forward recomputation (the values roe_flux forms) ----
+    // The derivative is not required to round like roe_flux, so repeated
+    // divisions become multiplications by a reciprocal.  The sign tests
+    // below can only change where the rounding straddles a tie, where the
+    // flux has no derivative anyway.
+    const T r_i = T(1) / qi[0], r_j = T(1) / qj[0];
     const T pi = pressure(g, qi);
     const T pj = pressure(g, qj);
-    const T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
-    const T Vj = (qj[1] * n0 + qj[2] * n1) / qj[0];
-    const T uL = qi[1] / qi[0], uR = qj[1] / qj[0];
-    const T vL = qi[2] / qi[0], vR = qj[2] / qj[0];
+    const T uL = qi[1] * r_i, uR = qj[1] * r_j;
+    const T vL = qi[2] * r_i, vR = qj[2] * r_j;
+    const T Vi = uL * n0 + vL * n1;
+    const T Vj = uR * n0 + vR * n1;
     const T sL = sqrt(qi[0]), sR = sqrt(qj[0]);
     const T rho = sR * sL;
     const T den = sL + sR;
-    const T u = (uL * sL + uR * sR) / den;
-    const T v = (vL * sL + vR * sR) / den;
-    const T hL = (qi[3] + pi) / qi[0], hR = (qj[3] + pj) / qj[0];
-    const T h = (hL * sL + hR * sR) / den;
+    const T r_d = T(1) / den;
+    const T u = (uL * sL + uR * sR) * r_d;
+    const T v = (vL * sL + vR * sR) * r_d;
+    const T hL = (qi[3] + pi) * r_i, hR = (qj[3] + pj) * r_j;
+    const T h = (hL * sL + hR * sR) * r_d;
     const T q2 = u * u + v * v;
     const T c = sqrt((g - T(1)) * (h - T(0.5) * q2));
     const T V = u * n0 + v * n1;
-    const T VR = uR * n0 + vR * n1;
-    const T VL = uL * n0 + vL * n1;
+    const T VR = Vj;
+    const T VL = Vi;
     const T am = V - c, bm = VL - c;
     const bool m_is_b = bm < am;
     const T mm = m_is_b ? bm : am;
@@ -60,13 +68,15 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     const T dp = pj - pi;
     const T dV = VR - VL;
     const T rc = rho * c;
+    const T r_cc = T(1) / (c * c);
+    const T r_c2 = T(0.5) * r_cc;
     const T c2x2 = T(2) * c * c;
     const T A1 = dp - rc * dV, A5 = dp + rc * dV;
-    const T kF1 = lcm * A1 / c2x2;
-    const T B0 = (qj[0] - qi[0]) - dp / (c * c);
+    const T kF1 = lcm * A1 * r_c2;
+    const T B0 = (qj[0] - qi[0]) - dp * r_cc;
     const T kF0 = lc * B0;
     const T kFr = lc * rho;
-    const T kF5 = lcp * A5 / c2x2;
+    const T kF5 = lcp * A5 * r_c2;
     // ---- reverse ----
     // f = F - 0.5 D, F = 0.5 (Fi + Fj)
     T Db[4], Fib[4];
@@ -95,11 +105,11 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     T vRb = kFr * (Db[2] + Db[3] * v);
     T vLb = -vRb;
     // kF1 = lcm A1 / c2x2 ; kF5 = lcp A5 / c2x2
-    T lcmb = kF1b * A1 / c2x2;
-    const T A1b = kF1b * lcm / c2x2;
-    T lcpb = kF5b * A5 / c2x2;
-    const T A5b = kF5b * lcp / c2x2;
-    T c2x2b = -(kF1b * kF1 + kF5b * kF5) / c2x2;
+    T lcmb = kF1b * A1 * r_c2;
+    const T A1b = kF1b * lcm * r_c2;
+    T lcpb = kF5b * A5 * r_c2;
+    const T A5b = kF5b * lcp * r_c2;
+    T c2x2b = -(kF1b * kF1 + kF5b * kF5) * r_c2;
     T dpb = A1b + A5b;
     T rcb = (A5b - A1b) * dV;
     dVb = dVb + (A5b - A1b) * rc;
@@ -108,8 +118,8 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     const T B0b = kF0b * lc;
     T rhob = kFrb * lc;
     T qi0b = -B0b, qj0b = B0b;
-    dpb = dpb - B0b / (c * c);
-    cb = cb + B0b * T(2) * dp / (c * c * c);
+    dpb = dpb - B0b * r_cc;
+    cb = cb + B0b * T(2) * dp * r_cc / c;
     cb = cb + c2x2b * T(4) * c;
     rhob = rhob + rcb * c;
     cb = cb + rcb * rho;
@@ -132,23 +142,24 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     ub = ub + q2b * T(2) * u;
     vb = vb + q2b * T(2) * v;
     // h, u, v: Roe averages
-    T sLb = hb * (hL - h) / den + ub * (uL - u) / den + vb * (vL - v) / den + rhob * sR;
-    T sRb = hb * (hR - h) / den + ub * (uR - u) / den + vb * (vR - v) / den + rhob * sL;
-    T hLb = hb * sL / den, hRb = hb * sR / den;
-    uLb = uLb + ub * sL / den; uRb = uRb + ub * sR / den;
-    vLb = vLb + vb * sL / den; vRb = vRb + vb * sR / den;
+    T sLb = (hb * (hL - h) + ub * (uL - u) + vb * (vL - v)) * r_d + rhob * sR;
+    T sRb = (hb * (hR - h) + ub * (uR - u) + vb * (vR - v)) * r_d + rhob * sL;
+    const T wL = sL * r_d, wR = sR * r_d;
+    T hLb = hb * wL, hRb = hb * wR;
+    uLb = uLb + ub * wL; uRb = uRb + ub * wR;
+    vLb = vLb + vb * wL; vRb = vRb + vb * wR;
     qi0b = qi0b + sLb / (T(2) * sL);
     qj0b = qj0b + sRb / (T(2) * sR);
     // hL = (qi3 + pi)/qi0
-    T qi3b = hLb / qi[0], qj3b = hRb / qj[0];
-    pib = pib + hLb / qi[0];
-    pjb = pjb + hRb / qj[0];
-    qi0b = qi0b - hLb * hL / qi[0];
-    qj0b = qj0b - hRb * hR / qj[0];
-    T qi1b = uLb / qi[0], qi2b = vLb / qi[0];
-    T qj1b = uRb / qj[0], qj2b = vRb / qj[0];
-    qi0b = qi0b - (uLb * uL + vLb * vL) / qi[0];
-    qj0b = qj0b - (uRb * uR + vRb * vR) / qj[0];
+    T qi3b = hLb * r_i, qj3b = hRb * r_j;
+    pib = pib + hLb * r_i;
+    pjb = pjb + hRb * r_j;
+    qi0b = qi0b - hLb * hL * r_i;
+    qj0b = qj0b - hRb * hR * r_j;
+    T qi1b = uLb * r_i, qi2b = vLb * r_i;
+    T qj1b = uRb * r_j, qj2b = vRb * r_j;
+    qi0b = qi0b - (uLb * uL + vLb * vL) * r_i;
+    qj0b = qj0b - (uRb * uR + vRb * vR) * r_j;
     // central flux, side i and side j
     T Vib = Fib[0] * qi[0] + Fib[1] * qi[1] + Fib[2] * qi[2] + Fib[3] * (qi[3] + pi);
     T Vjb = Fib[0] * qj[0] + Fib[1] * qj[1] + Fib[2] * qj[2] + Fib[3] * (qj[3] + pj);
@@ -161,10 +172,10 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     n0b = n0b + Fib[1] * (pi + pj);
     n1b = n1b + Fib[2] * (pi + pj);
     // Vi = (qi1 n0 + qi2 n1)/qi0
-    qi1b = qi1b + Vib * n0 / qi[0]; qi2b = qi2b + Vib * n1 / qi[0]; qi0b = qi0b - Vib * Vi / qi[0];
-    n0b = n0b + Vib * qi[1] / qi[0] + Vjb * qj[1] / qj[0];
-    n1b = n1b + Vib * qi[2] / qi[0] + Vjb * qj[2] / qj[0];
-    qj1b = qj1b + Vjb * n0 / qj[0]; qj2b = qj2b + Vjb * n1 / qj[0]; qj0b = qj0b - Vjb * Vj / qj[0];
+    qi1b = qi1b + Vib * n0 * r_i; qi2b = qi2b + Vib * n1 * r_i; qi0b = qi0b - Vib * Vi * r_i;
+    n0b = n0b + Vib * uL + Vjb * uR;
+    n1b = n1b + Vib * vL + Vjb * vR;
+    qj1b = qj1b + Vjb * n0 * r_j; qj2b = qj2b + Vjb * n1 * r_j; qj0b = qj0b - Vjb * Vj * r_j;
     qib[0] = qib[0] + qi0b; qib[1] = qib[1] + qi1b; qib[2] = qib[2] + qi2b; qib[3] = qib[3] + qi3b;
     qjb[0] = qjb[0] + qj0b; qjb[1] = qjb[1] + qj1b; qjb[2] = qjb[2] + qj2b; qjb[3] = qjb[3] + qj3b;
     pressure_vjp(g, qi, pib, qib);

@@ -202,15 +202,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams p
     T fb[4] = {T(0), T(0), T(0), T(0)};
     if (ep.z < m.n_real) {
         const Q4<T> l = lam_qt[ep.z];
-        const T A = m.c_area[ep.z];
+        const T w = geo.v[2] * m.c_inv_area[ep.z];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) fb[k] = fb[k] - l.v[k] * geo.v[2] / A;
+        for (int k = 0; k < 4; ++k) fb[k] = fb[k] - l.v[k] * w;
     }
     if (ep.w < m.n_real) {
         const Q4<T> l = lam_qt[ep.w];
-        const T A = m.c_area[ep.w];
+        const T w = geo.v[2] * m.c_inv_area[ep.w];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) fb[k] = fb[k] + l.v[k] * geo.v[2] / A;
+        for (int k = 0; k < 4; ++k) fb[k] = fb[k] + l.v[k] * w;
     }
     T fs[2][4];
 #pragma unroll
