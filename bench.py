@@ -435,7 +435,8 @@ def main():
     ap.add_argument("--flux-mode", default="auto", choices=["auto", "fused", "tile", "cell"],
                     help="step/airfoil: stage form (auto: fused where it fits)")
     ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
-    ap.add_argument("--tile-cells", type=int, default=256, help="step/airfoil: cells per tile (32..256)")
+    ap.add_argument("--tile-cells", type=int, default=0,
+                    help="step/airfoil/plate: cells per tile (32..256; 0: the library's automatic choice)")
     ap.add_argument("--converge", action="store_true",
                     help="airfoil: also run from the freestream until the density residual drops 6 orders")
     ap.add_argument("--stop-var", type=int, default=0, help="with --converge: 0 density, -1 max over variables")
@@ -588,6 +589,7 @@ def main():
         "dtype": "f64" if a.precision == "f64" else "f32", "data": "synthetic",
         "config": wl, "roofline": roof, "cpu_baseline": cpu,
     }
+    wl["tile_cells"] = s.tile_cells
     if conv is not None:
         out["converge"] = conv
     print(json.dumps(out))

@@ -173,6 +173,12 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0);
 template <class T>
 size_t stage_lds_bytes(const HaloView& hv);
+// Workgroup size the fused stage pass takes for a halo of lds bytes when
+// asked for `threads` (0: 256 lanes when four workgroups fit a CU's LDS).
+inline int stage_lanes(size_t lds, int threads) {
+    if (threads) return threads;
+    return lds + 1024 <= 160 * 1024 / 4 ? kStageBlock / 2 : kStageBlock;
+}
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
                         const BcParam<T>* bcs, Q4<T>* q);

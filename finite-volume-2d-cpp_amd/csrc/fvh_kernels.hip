@@ -957,7 +957,7 @@ bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
                     const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
                     double* res_blk, const Ctrl* ctrl, int threads) {
     const size_t lds = stage_lds_bytes<T>(hv);
-    if (threads == kStageBlock / 2 || (threads == 0 && lds + 1024 <= 160 * 1024 / 4))
+    if (stage_lanes(lds, threads) == kStageBlock / 2)
         launch_stage_nb<T, PHI, DT, J, kStageBlock / 2>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
                                                         dt, dt_inst, alpha, res_blk, ctrl, lds);
     else

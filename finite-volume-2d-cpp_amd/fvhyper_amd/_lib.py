@@ -72,7 +72,7 @@ EXPORTS = [
     "fvh_solver_get_state", "fvh_solver_save", "fvh_solver_load", "fvh_solver_apply_bounds", "fvh_solver_phases", "fvh_solver_run",
     "fvh_solver_advance", "fvh_solver_run_begin", "fvh_solver_run_continue", "fvh_solver_run_log",
     "fvh_stream_copy", "fvh_solver_synchronize", "fvh_solver_status", "fvh_solver_profile",
-    "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes",
+    "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes", "fvh_solver_tile_cells",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
     "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
 ]
@@ -132,6 +132,7 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_kernel_name.restype = C.c_char_p
     L.fvh_solver_stage_bytes.argtypes = [vp]
     L.fvh_solver_stage_bytes.restype = C.c_double
+    L.fvh_solver_tile_cells.argtypes = [vp]
     L.fvh_solver_use_graphs.argtypes = [vp, C.c_int32]
     L.fvh_batch_create.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), C.POINTER(vp)]
     L.fvh_batch_size.argtypes = [vp]

@@ -86,7 +86,7 @@ FLUX_MODES = {"auto": 0, "cell": 1, "tile": 2, "fused": 3}
 
 
 def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: str = "auto",
-                tile_cells: int = 256) -> _lib.Config:
+                tile_cells: int = 0) -> _lib.Config:
     """fvh_config from the shared configuration dict (see oracle/oracle.py)."""
     c = _lib.Config()
     _lib.load().fvh_config_default(C.byref(c))
@@ -149,7 +149,7 @@ class Solver:
     """Explicit FV solver on the current HIP device (one stream per handle)."""
 
     def __init__(self, mesh: Mesh, cfg: Dict, precision: str = "f64", renumber="tiles",
-                 flux_mode: str = "auto", tile_cells: int = 256):
+                 flux_mode: str = "auto", tile_cells: int = 0):
         self.mesh = mesh
         self.cfg = dict(cfg)
         self.precision = precision
@@ -273,6 +273,11 @@ class Solver:
     def stage_bytes(self) -> float:
         return float(_lib.load().fvh_solver_stage_bytes(self._h))
 
+    @property
+    def tile_cells(self) -> int:
+        """Cells per tile in use (the automatic choice resolved)."""
+        return check(_lib.load().fvh_solver_tile_cells(self._h))
+
     def _weights(self, wx, wy):
         nb = self.mesh.n_bnd
         out = []
@@ -323,7 +328,7 @@ class Batch(Solver):
     the instances' reference-layout states; ``split``/``join`` convert."""
 
     def __init__(self, meshes: List[Mesh], cfgs: List[Dict], precision: str = "f64", renumber="tiles",
-                 flux_mode: str = "auto", tile_cells: int = 256):
+                 flux_mode: str = "auto", tile_cells: int = 0):
         if len(meshes) != len(cfgs) or not meshes:
             raise ValueError("one config per mesh, at least one mesh")
         self.meshes = list(meshes)

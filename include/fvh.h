@@ -118,7 +118,9 @@ typedef struct fvh_config {
     fvh_bc_spec bc[FVH_MAX_BC];
     int32_t renumber;          /* device cell order: FVH_ORDER_* */
     int32_t flux_mode;         /* FVH_FLUX_MODE_* */
-    int32_t tile_cells;        /* cells per tile (workgroup), 32..256; 0 = 256 */
+    int32_t tile_cells;        /* cells per tile (workgroup), 32..256; 0 = automatic (the default): 256, or
+                                  fewer when the fused stage pass runs 256-lane workgroups and a tile with
+                                  its first neighbour ring would outnumber the lanes */
     /* laminar viscous constants of FVH_FLUX_ROE_VISCOUS and the viscous time
      * steps (consts:: of flatPlate/main.cpp:28-35): viscosity, Prandtl number,
      * heat capacity, gas constant */
@@ -269,6 +271,10 @@ const char* fvh_solver_kernel_name(int32_t k);
  * numerator): first order C(3Vs+s+g) + E(3g+8) + B(2Vs+2g+4); second order
  * adds C(7Vs+2g) + E(2g). */
 double fvh_solver_stage_bytes(const fvh_solver* s);
+
+/* Cells per tile the solver uses (resolves tile_cells = 0).  No reference
+ * counterpart: a tuning query. */
+int32_t fvh_solver_tile_cells(const fvh_solver* s);
 
 /* Capture each RK step (every kernel of it) in a HIP graph on first use and
  * replay it afterwards (one graph per state buffer, residual flag and run

@@ -381,6 +381,29 @@ def test_fused_workgroup_sizes_bit_exact(mesh_name, cfg_name, threads):
     assert same_values(s.get_state(), qo)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_automatic_tile_size(precision):
+    """tile_cells 0 (the default): on the forward-step mesh a 256-cell tile
+    with its ring-1 cells outnumbers the fused pass's 256 lanes, so the
+    library shrinks the tiles; the state keeps its bits (FP64: also against
+    the oracle), and an explicit size is kept as given."""
+    print("case automatic tile size", precision, flush=True)
+    m, cfg, ic = workloads.forward_step(n_per_unit=100)
+    pm = Mesh.from_arrays(m)
+    auto = Solver(pm, cfg, precision=precision)
+    fixed = Solver(pm, cfg, precision=precision, tile_cells=256)
+    assert 128 <= auto.tile_cells < 256 and auto.tile_cells % 16 == 0
+    assert fixed.tile_cells == 256
+    q0 = workloads.bump_state(pm)
+    for s in (auto, fixed):
+        s.set_state(q0)
+        s.run(max_step=6)
+    assert bits_equal(auto.get_state(), fixed.get_state())
+    if precision == "f64":
+        qo, *_ = O.Mesh(m).run(cfg, q0, 6)
+        assert bits_equal(auto.get_state(), qo)
+
+
 def test_autotune_returns_a_working_choice():
     from fvhyper_amd import autotune
     m, cfg, ic = workloads.naca_airfoil(20000)
