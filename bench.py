@@ -435,6 +435,8 @@ def main():
     ap.add_argument("--flux-mode", default="auto", choices=["auto", "fused", "tile", "cell"],
                     help="step/airfoil: stage form (auto: fused where it fits)")
     ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
+    ap.add_argument("--stage-threads", type=int, default=0, choices=[0, 256, 512],
+                    help="fused stage pass workgroup size (0: the library's choice)")
     ap.add_argument("--tile-cells", type=int, default=0,
                     help="step/airfoil/plate: cells per tile (32..256; 0: the library's automatic choice)")
     ap.add_argument("--converge", action="store_true",
@@ -473,6 +475,8 @@ def main():
             cfg["smooth"] = True
         if a.cfl > 0:
             cfg["cfl"] = a.cfl
+    if a.stage_threads:
+        cfg["stage_threads"] = a.stage_threads
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode, tile_cells=a.tile_cells)
     q0 = workloads.initial_state(pm, ic)

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams p
 
 // ---- gradient/limiter pass -------------------------------------------------
 template <class T, int PHI, bool LIN, bool GEO>
-__global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad,
+__global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, PhysParams pp, int grad,
                                                         int lim_on, const Q4<T>* __restrict__ qin,
                                                         const Q4<T>* __restrict__ gxs, const Q4<T>* __restrict__ gys,
                                                         const Q4<T>* __restrict__ lms,
@@ -266,10 +266,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, TileMap t
                                                         const Q4<T>* __restrict__ lam_bincr,
                                                         Q4<T>* __restrict__ lam_direct,
                                                         Q4<T>* __restrict__ nb_contrib, GeoAdj<T> ga) {
-    const int lb = xcd_block(tm.n_rt);
+    // one lane per real cell over the whole range (nothing here is per
+    // tile, and tiles of fewer than kBlock cells would leave lanes idle)
+    const int lb = xcd_block((m.n_real + kBlock - 1) / kBlock);
     if (lb < 0) return;
-    const int c = tm.c0[lb] + threadIdx.x;
-    if (c >= tm.c0[lb + 1]) return;
+    const int c = lb * kBlock + threadIdx.x;
+    if (c >= m.n_real) return;
     const T g = T(pp.gamma);
     const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
     const Q4<T> qc = qin[c];
@@ -800,7 +802,8 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
     if (lin) { if (geo) FVH_AF(true, true); else FVH_AF(true, false); }
     else { if (geo) FVH_AF(false, true); else FVH_AF(false, false); }
 #undef FVH_AF
-#define FVH_AG(P, L_, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L_, G>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp,    \
+#define FVH_AG(P, L_, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L_, G>), dim3(xcd_grid(m.n_real)), dim3(kBlock), 0, s, \
+                                            m, pp,                                                                   \
                                             (int)grad, (int)lim, qin, gx, gy, lm, (const Q4<T>*)lam_face,            \
                                             (const Q4<T>*)lam_bincr, lam_direct, nb_contrib, ga)
 #define FVH_AGL(P)                                                                \

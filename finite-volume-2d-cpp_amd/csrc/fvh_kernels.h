@@ -114,6 +114,7 @@ struct HaloView {
     const uint16_t* he_loc;   // per half-edge of a real cell: local neighbour index in its own tile
     const uint16_t* r1_slot;  // parallel to r1_loc: the tile's edge slot of that half-edge (0xffff: none)
     int32_t max_ext, max_g, max_edges;   // largest tile (LDS sizing)
+    int32_t mean_g;                      // tile plus real ring-1 cells, averaged over tiles
 };
 
 struct PhysParams {
@@ -160,6 +161,17 @@ void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const
                    const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<T>* q0,
                    Q4<T>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
                    Q4<T>* dbg_qt, const Ctrl* ctrl);
+// Optional outputs of a fused stage for the adjoint's replays: every cell's
+// gradient and limiter (ghosts: zero gradient, reset limiter) and every real
+// cell's qt, as the two-pass form stores them.  All null: none.
+template <class T>
+struct StageRecord {
+    Q4<T>* gx = nullptr;
+    Q4<T>* gy = nullptr;
+    Q4<T>* lm = nullptr;
+    Q4<T>* qt = nullptr;
+};
+
 // Fused stage: gradient + limiter (tile and ring 1, from LDS), face states,
 // edge fluxes, gather, RK update and boundary update in one pass per tile.
 // dt_kind >= 0 also computes the tile cells' local time step (first stage).
@@ -170,14 +182,19 @@ template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
-                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0);
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0,
+                  const StageRecord<T>& rec = StageRecord<T>());
 template <class T>
 size_t stage_lds_bytes(const HaloView& hv);
 // Workgroup size the fused stage pass takes for a halo of lds bytes when
-// asked for `threads` (0: 256 lanes when four workgroups fit a CU's LDS).
-inline int stage_lanes(size_t lds, int threads) {
+// asked for `threads`.  0: 256 lanes when four workgroups fit a CU's LDS, or
+// when three fit and an average tile with its ring-1 cells (mean_g) fits the
+// lanes, so that the gradient phase takes one round; 512 otherwise.
+inline int stage_lanes(size_t lds, int threads, int mean_g) {
     if (threads) return threads;
-    return lds + 1024 <= 160 * 1024 / 4 ? kStageBlock / 2 : kStageBlock;
+    if (lds + 1024 <= 160 * 1024 / 4) return kStageBlock / 2;
+    if (lds + 1024 <= 160 * 1024 / 3 && mean_g <= kStageBlock / 2) return kStageBlock / 2;
+    return kStageBlock;
 }
 template <class T>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,

@@ -597,7 +597,8 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
                                                   const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
                                                   Q4<T>* __restrict__ qout, Q4<T>* __restrict__ bincr,
                                                   T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
-                                                  double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl) {
+                                                  double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
+                                                  StageRecord<T> rec) {
     static_assert(NB >= kBlock, "phase 3 gives each tile cell one lane");
     extern __shared__ __align__(32) unsigned char lds_raw[];
     const int lb = xcd_block(tm.n_rt);
@@ -633,6 +634,14 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         const G2<T> cc = m.c_c[c];
         grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
         if (DT && own) dt[c] = dtc;
+        if (rec.gx && own) {
+            Q4<T> a, b, d;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { a.v[k] = gx[k]; b.v[k] = gy[k]; d.v[k] = lim[k]; }
+            rec.gx[c] = a;
+            rec.gy[c] = b;
+            rec.lm[c] = d;
+        }
         for (int h = h0; h < h1; ++h) {
             const int slot = own ? (int)m.he_slot[h] : (int)hv.r1_slot[r1 + (h - h0)];
             if (slot == 0xffff) continue;
@@ -651,6 +660,12 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
                 bincr[nb - m.n_real] = inc;
                 const T z[4] = {T(0), T(0), T(0), T(0)};
                 const T l0[4] = {lim0, lim0, lim0, lim0};
+                if (rec.gx) {
+                    // the ghost's gradient and limiter as the two-pass form stores them
+                    rec.gx[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
+                    rec.gy[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
+                    rec.lm[nb] = Q4<T>{{l0[0], l0[1], l0[2], l0[3]}};
+                }
                 const Q4<T> ginc = face_inc(z, z, l0, ec, m.c_c[nb]);
                 const Q4<T> qg = sq[loc[h - h0]];
                 Q4<T> fg;
@@ -695,7 +710,7 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         }
         auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
         finish_cell<T, true>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
-                             res_blk != nullptr, rs, nullptr);
+                             res_blk != nullptr, rs, rec.qt);
     }
     if (res_blk) {
         block_sum4<NB>(rs);
@@ -933,7 +948,7 @@ template <class T, int PHI, int DT, bool J, int NB>
 void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                      const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
                      const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
-                     double* res_blk, const Ctrl* ctrl, size_t lds) {
+                     double* res_blk, const Ctrl* ctrl, size_t lds, const StageRecord<T>& rec) {
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -943,26 +958,28 @@ void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, con
     }
     const int nb = xcd_grid_blocks(tm.n_rt);
     hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
-                       (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl);
+                       (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl, rec);
 }
 
 // Workgroup size: a tile per workgroup either way.  With 256 lanes the
 // phases loop over a tile's cells and edges, and four workgroups share a
 // CU when their LDS fits (state registers allow 16 waves per CU): every
-// wave has work in every phase.  Tiles whose halo needs more than a
-// quarter of the CU's LDS run with 512 lanes, two workgroups per CU.
+// wave has work in every phase.  Three workgroups per CU still beat two
+// of 512 lanes when a tile and its ring fit one round of the gradient
+// phase; larger halos run with 512 lanes, two workgroups per CU
+// (stage_lanes).
 template <class T, int PHI, int DT, bool J>
 bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
                     const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
-                    double* res_blk, const Ctrl* ctrl, int threads) {
+                    double* res_blk, const Ctrl* ctrl, int threads, const StageRecord<T>& rec) {
     const size_t lds = stage_lds_bytes<T>(hv);
-    if (stage_lanes(lds, threads) == kStageBlock / 2)
+    if (stage_lanes(lds, threads, hv.mean_g) == kStageBlock / 2)
         launch_stage_nb<T, PHI, DT, J, kStageBlock / 2>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
-                                                        dt, dt_inst, alpha, res_blk, ctrl, lds);
+                                                        dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
     else
         launch_stage_nb<T, PHI, DT, J, kStageBlock>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt,
-                                                    dt_inst, alpha, res_blk, ctrl, lds);
+                                                    dt_inst, alpha, res_blk, ctrl, lds, rec);
     return true;
 }
 
@@ -970,11 +987,12 @@ template <class T>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
-                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads) {
+                  const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads,
+                  const StageRecord<T>& rec) {
     if (tm.n_rt == 0) return true;
     if (stage_lds_bytes<T>(hv) > 160 * 1024) return false;
 #define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt, \
-                                                  dt_inst, alpha, res_blk, ctrl, threads)
+                                                  dt_inst, alpha, res_blk, ctrl, threads, rec)
 #define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
 #define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
     switch (phi_kind) {
@@ -1043,7 +1061,7 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
     template bool launch_stage<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,          \
                                   const HaloView&, const PhysParams&, int, bool, bool, bool, int,            \
                                   const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, Q4<T>*, T*,        \
-                                  const T*, T, double*, const Ctrl*, int);                                                     \
+                                  const T*, T, double*, const Ctrl*, int, const StageRecord<T>&);            \
     template size_t stage_lds_bytes<T>(const HaloView&);
 FVH_INST(double)
 FVH_INST(float)

@@ -98,7 +98,7 @@ struct fvh_solver {
     // fused stage pass: tile halos
     bool fused = false;
     fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_r1_slot, x_he_loc;
-    int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0;
+    int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
     fvh::DevBuf qt_s[2];                     // residual smoothing: raw qt, first sweep's qt
@@ -205,6 +205,7 @@ HaloView halo_view(fvh_solver* s) {
     h.r1_slot = (const uint16_t*)s->x_r1_slot.p;
     h.max_ext = s->x_max_ext;
     h.max_g = s->x_max_g;
+    h.mean_g = s->x_mean_g;
     h.max_edges = s->x_max_edges;
     return h;
 }
@@ -407,7 +408,9 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     Q4<T>* gx_c = (Q4<T>*)s->gxy[0].p;   // null unless the viscous flux takes face gradients
     Q4<T>* gy_c = (Q4<T>*)s->gxy[1].p;
     const bool fused = s->fused && !dbg;
-    if (!s->second || (fused && gdt)) {
+    // the time-step kernel: first order, a fixed step (the second-order
+    // passes compute only the CFL kinds) and the fused form's global step
+    if (!s->second || c.dt_kind == FVH_DT_FIXED || (fused && gdt)) {
         Bracket b(s, K_DT);
         launch_dt<T>(st, m, tm, c.dt_kind, pp, q0, dt, blk, cc);
     }
@@ -1109,18 +1112,25 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * qb;
         s->fused = lds <= 160 * 1024;
     }
-    // Automatic tile size (tile_cells 0): tiles of 256 cells first.  When the
-    // fused pass then runs 256-lane workgroups and a tile with its real
-    // ring-1 cells outnumbers the lanes, its gradient phase takes two rounds;
-    // tiles shrunk until the average tile and ring fit one round do the same
-    // work in fewer instructions per lane (the 975,744-cell step mesh: 208
-    // cells per tile, 12% faster in FP64 and FP32).
-    if (c.tile_cells == 0 && s->fused && c.renumber == FVH_ORDER_TILES && s->n_real > 0) {
+    // Automatic tile size (tile_cells 0): tiles of 256 cells first.  When an
+    // average tile with its real ring-1 cells outnumbers 256 lanes, the
+    // gradient phase of a 256-lane workgroup takes two rounds; tiles shrunk
+    // until it fits one round do the same work in fewer instructions per
+    // lane.  They are rebuilt when the fused pass runs 256-lane workgroups,
+    // or when it runs 512 and the smaller halos would let three 256-lane
+    // workgroups share a CU (the 975,744-cell step mesh: 208 cells per tile,
+    // 12% faster; the 2 M-cell airfoil: 208 cells, 256 lanes, 5% faster).
+    if (c.tile_cells == 0 && s->fused && c.renumber == FVH_ORDER_TILES && s->n_real > 0 && s->n_rt > 0 &&
+        c.stage_threads != kStageBlock) {
         const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * (s->f32 ? 16 : 32);
-        const int lanes = stage_lanes(lds, c.stage_threads);
+        const int32_t mean_g = (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt);
+        const int half = kStageBlock / 2;
+        const int lanes = stage_lanes(lds, c.stage_threads, mean_g);
         const double per_cell = (double)hh.sum_g / (double)s->n_real;
-        const int32_t t = std::max<int32_t>(128, (int32_t)(lanes / per_cell) / 16 * 16);
-        if (lanes == kStageBlock / 2 && hh.max_g > lanes && t < kBlock) {
+        const int32_t t = std::max<int32_t>(128, (int32_t)(half / per_cell) / 16 * 16);
+        // halo bytes scale slightly slower than the cell count: 5% margin
+        const bool three_fit = (double)lds * t / kBlock * 1.05 + 1024 <= 160.0 * 1024 / 3;
+        if (t < kBlock && mean_g > half && (lanes == half || (c.stage_threads == 0 && three_fit))) {
             std::vector<fvh_config> shrunk(cs, cs + n);
             for (fvh_config& x : shrunk) x.tile_cells = t;
             return create(n, ms, shrunk.data(), out);
@@ -1134,6 +1144,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     if (s->fused) {
         s->x_max_ext = hh.max_ext;
         s->x_max_g = hh.max_g;
+        s->x_mean_g = s->n_rt ? (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt) : 0;
         s->x_max_edges = hh.max_edges;
         if ((rc = upload(s->x_ext, hh.ext.data(), hh.ext.size() * sizeof(int4)))) return rc;
         if ((rc = upload(s->x_cells, hh.cells.data(), hh.cells.size() * 4))) return rc;
@@ -1334,7 +1345,33 @@ struct Adjoint {
         hipStream_t st = s->stream;
         double* dt = (double*)R.dt;
         double* blk = (double*)s->blk_min.p;
-        if (!s->second) launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)R.qs[0], dt, blk, nullptr);
+        // the time-step kernel fills the record's own dt (a fixed step too:
+        // the gradient pass computes only the CFL kinds)
+        if (!s->second || s->fused || c.dt_kind == FVH_DT_FIXED)
+            launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)R.qs[0], dt, blk, nullptr);
+        if (s->fused) {
+            // the fused stage pass, storing what the two-pass form would (every
+            // cell's time step comes from the time-step kernel above)
+            const HaloView hv = halo_view(s);
+            if (gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
+            for (int k = 0; k < n_stage; ++k) {
+                StageRecord<double> rec;
+                rec.gx = (Q4<double>*)R.gx[k];
+                rec.gy = (Q4<double>*)R.gy[k];
+                rec.lm = (Q4<double>*)R.lm[k];
+                rec.qt = (Q4<double>*)R.qt[k];
+                if (!launch_stage<double>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim, -1,
+                                          bcs, (const Q4<double>*)R.qs[k], (const Q4<double>*)R.qs[0],
+                                          (Q4<double>*)R.qs[k + 1], (Q4<double>*)s->bincr.p, dt,
+                                          gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, nullptr,
+                                          c.stage_threads, rec)) {
+                    set_error("fused stage pass: tile halo does not fit LDS");
+                    return FVH_E_UNSUPPORTED;
+                }
+            }
+            FVH_HIP(hipGetLastError());
+            return FVH_OK;
+        }
         for (int k = 0; k < n_stage; ++k) {
             const Q4<double>* qin = (const Q4<double>*)R.qs[k];
             if (s->second)

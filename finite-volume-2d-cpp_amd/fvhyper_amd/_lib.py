@@ -91,6 +91,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FvhError("libfvh.so is not built at %s (run finite-volume-2d-cpp_amd/build.py)" % path)
+    # torch first, when installed: with libfvh.so loaded before it, torch's
+    # own HIP initialisation later finds no GPU ("No HIP GPUs are
+    # available"), which breaks callers that mix both (fvhyper_amd.autograd)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)

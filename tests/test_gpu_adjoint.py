@@ -1,7 +1,7 @@
 """The discrete adjoint on the GPU: the gradient of the step-face pressure
 force after N RK5 steps of the second-order forward-step solver with respect
 to the initial state, against central differences of the forward solver, and
-independent of the checkpoint schedule."""
+independent of the checkpoint schedule; and its torch.autograd face."""
 import numpy as np
 import pytest
 
@@ -119,6 +119,28 @@ def test_adjoint_with_step_graphs_is_unchanged():
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[1], c[1])
 
 
+@pytest.mark.parametrize("cfg_over", [{}, {"gdt": False, "dt": "fixed", "dt_fixed": 2e-4}])
+def test_fused_and_two_pass_replays_agree(cfg_over):
+    """The reverse sweep replays steps with the fused stage pass (storing
+    gradients, limiters and qt) when the solver runs fused, with the
+    two-pass kernels otherwise: the same records, so the same gradients
+    bit for bit, with global and with fixed time steps, for every
+    checkpoint schedule.  (Same tiles in both: the global time step's
+    adjoint sums per-tile partials.)"""
+    print("case fused vs two-pass replays", cfg_over, flush=True)
+    pm, cfg, w, q0 = _case()
+    cfg = dict(cfg, **cfg_over)
+    fused = Solver(pm, cfg, flux_mode="fused", tile_cells=256)
+    tile = Solver(pm, cfg, flux_mode="tile", tile_cells=256)
+    J = fused.objective(q0, 7, wx=w)
+    assert J == tile.objective(q0, 7, wx=w)
+    for kw in ({}, {"snaps": 3}, {"snaps": 7}):
+        a = fused.adjoint(q0, 7, wx=w, nodes=True, **kw)
+        b = tile.adjoint(q0, 7, wx=w, nodes=True, **kw)
+        assert a[0] == b[0] == J
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
 def test_adjoint_refuses_what_it_does_not_cover():
     from fvhyper_amd import FvhError
     m, cfg, ic = workloads.naca_airfoil(2000)          # local time step
@@ -234,3 +256,33 @@ def test_node_gradient_matches_central_differences_of_moved_meshes():
         ad = float(gxy[:, 0] @ dx + gxy[:, 1] @ dy)
         print("  fd %.12e adjoint %.12e" % (fd, ad), flush=True)
         assert abs(ad - fd) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_autograd_function_carries_the_adjoint(device):
+    """flow_objective inside a torch graph: J and the chain-ruled gradients
+    (through a state parameterised by a scalar, and to the node
+    coordinates) equal what Solver.adjoint returns; without grad it runs the
+    plain objective."""
+    import torch
+    from fvhyper_amd.autograd import flow_objective
+    print("case autograd function", device, flush=True)
+    pm, cfg, w, q0 = _case(n_per_unit=20)
+    s = Solver(pm, cfg)
+    J0, g0, gxy0, _ = s.adjoint(q0, 6, wx=w, nodes=True)
+    v = _directions(pm, 1, seed=77)[0]
+    t = torch.zeros((), dtype=torch.float64, device=device, requires_grad=True)
+    q = torch.from_numpy(q0).to(device) + t * torch.from_numpy(v).to(device)
+    xy = torch.from_numpy(np.stack([pm.export("nodesX"), pm.export("nodesY")], axis=1)).to(device)
+    xy.requires_grad_(True)
+    J = flow_objective(s, q, 6, wx=w, xy=xy)
+    (2.0 * J).backward()
+    assert J.item() == J0
+    assert t.grad.device.type == device and xy.grad.device.type == device
+    assert abs(t.grad.item() - 2.0 * float(g0 @ v)) <= 1e-12 * abs(2.0 * float(g0 @ v))
+    assert np.array_equal(xy.grad.cpu().numpy(), 2.0 * gxy0)
+    with torch.no_grad():
+        J2 = flow_objective(s, torch.from_numpy(q0).to(device), 6, wx=w)
+    assert J2.item() == s.objective(q0, 6, wx=w)
+    with pytest.raises(ValueError):
+        flow_objective(s, q.detach(), 6, wx=w, xy=xy.detach() + 1.0)

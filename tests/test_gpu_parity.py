@@ -381,6 +381,25 @@ def test_fused_workgroup_sizes_bit_exact(mesh_name, cfg_name, threads):
     assert same_values(s.get_state(), qo)
 
 
+@pytest.mark.parametrize("gdt", [True, False])
+@pytest.mark.parametrize("flux_mode", ["fused", "tile"])
+def test_second_order_fixed_step(flux_mode, gdt):
+    """A fixed time step in a second-order run, with and without the global
+    minimum: the time-step kernel fills every cell's step (the gradient pass
+    computes only the CFL kinds); the oracle's bits after 8 steps."""
+    print("case second order fixed step", flux_mode, gdt, flush=True)
+    cfg = dict(CONFIGS["second_venkat_global"], dt="fixed", dt_fixed=1e-4, gdt=gdt)
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    q0 = random_state(om.n_cells, seed=909, mach=0.3)
+    s = Solver(Mesh.from_arrays(m), cfg, flux_mode=flux_mode)
+    s.set_state(q0)
+    s.run(max_step=8)
+    qo, *_ = om.run(cfg, q0, 8)
+    assert not bits_equal(qo, q0)
+    assert bits_equal(s.get_state(), qo)
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_automatic_tile_size(precision):
     """tile_cells 0 (the default): on the forward-step mesh a 256-cell tile
