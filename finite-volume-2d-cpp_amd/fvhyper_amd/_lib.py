@@ -84,6 +84,34 @@ class FvhError(RuntimeError):
     pass
 
 
+def _share_hip_runtime_with_torch():
+    """Make libfvh.so bind the HIP runtime torch ships, when torch is there.
+
+    libfvh.so needs libamdhip64.so.7 (found through its run path in the ROCm
+    install).  torch bundles its own libamdhip64 with the same soname.  Two
+    copies of the runtime in one process leave the one loaded second without
+    devices ("No HIP GPUs are available"), so whichever library comes first
+    decides.  Loading torch's copy by path, with global symbols, before
+    libfvh.so makes both use it in either import order, without the cost of
+    importing torch.  Without torch, libfvh.so takes the ROCm runtime."""
+    import importlib.util
+    import sys
+    if "torch" in sys.modules:
+        return   # torch's runtime is already in the process
+    try:
+        spec = importlib.util.find_spec("torch")
+    except Exception:   # a broken torch install: libfvh.so takes the ROCm runtime
+        return
+    for d in (spec.submodule_search_locations or []) if spec else []:
+        p = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(p):
+            try:
+                C.CDLL(p, mode=C.RTLD_GLOBAL)
+            except OSError:
+                pass
+            return
+
+
 def load(path: str = LIB_PATH):
     """Load libfvh.so and declare its signatures."""
     global _lib
@@ -91,13 +119,7 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise FvhError("libfvh.so is not built at %s (run finite-volume-2d-cpp_amd/build.py)" % path)
-    # torch first, when installed: with libfvh.so loaded before it, torch's
-    # own HIP initialisation later finds no GPU ("No HIP GPUs are
-    # available"), which breaks callers that mix both (fvhyper_amd.autograd)
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    _share_hip_runtime_with_torch()
     L = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)

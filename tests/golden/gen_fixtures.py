@@ -102,13 +102,27 @@ VISC_CONFIGS = {
     "visc_transient": dict(flux="roe_viscous", dt="sum_viscous", cfl=1.5, grad=True, diffusive=True, gdt=True,
                            bcs={"wall": ("noslip",), "transmissive": NT_FF, "inlet": NT_FF, "outlet": NT_FF}),
 }
-ALL_CONFIGS = {**FIX_CONFIGS, **SMOOTH_CONFIGS, **VISC_CONFIGS}
+# a fixed time step in a second-order run, with and without the global
+# minimum: the second-order passes compute only the CFL kinds of dt, so the
+# fixed step comes from the time-step kernel in the library
+FIXED2_MESHES = ("mixed", "step", "naca")
+FIXED2_CONFIGS = {
+    "second_venkat_fixed": dict(flux="roe_joined", dt="fixed", dt_fixed=2e-5, phi="venkat", grad=True, lim=True,
+                                lin=True, gdt=True,
+                                bcs={"wall": ("mirror",), "transmissive": ("copy",), "inlet": FS, "outlet": FS}),
+    "second_michalak_fixed_local": dict(flux="roe_split", dt="fixed", dt_fixed=2e-5, phi="michalak", grad=True,
+                                        lim=True, lin=True, gdt=False,
+                                        bcs={"wall": ("mirror",), "transmissive": NC, "inlet": NC,
+                                             "outlet": ("copy",)}),
+}
+ALL_CONFIGS = {**FIX_CONFIGS, **SMOOTH_CONFIGS, **VISC_CONFIGS, **FIXED2_CONFIGS}
 
 
 def configs_of(mesh_name):
     """Config names with fixtures on this mesh."""
     return (list(FIX_CONFIGS) + (list(SMOOTH_CONFIGS) if mesh_name in SMOOTH_MESHES else []) +
-            (list(VISC_CONFIGS) if mesh_name in VISC_MESHES else []))
+            (list(VISC_CONFIGS) if mesh_name in VISC_MESHES else []) +
+            (list(FIXED2_CONFIGS) if mesh_name in FIXED2_MESHES else []))
 
 
 def renamed(m, cfg):

@@ -73,6 +73,14 @@ CONFIGS = {
                                         "inlet": FREESTREAM_NACA, "outlet": FREESTREAM_NACA}),
     "grad_only_lin": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, grad=True, lim=False, lin=True,
                           gdt=True, bcs=BC_ALL),
+    # a fixed step in second order, global and local (golden fixtures from the
+    # compiled reference: second_venkat_fixed, second_michalak_fixed_local)
+    "second_venkat_fixed": dict(flux="roe_joined", dt="fixed", dt_fixed=2e-5, phi="venkat", grad=True, lim=True,
+                                lin=True, gdt=True, bcs=BC_ALL),
+    "second_michalak_fixed_local": dict(flux="roe_split", dt="fixed", dt_fixed=2e-5, phi="michalak", grad=True,
+                                        lim=True, lin=True, gdt=False,
+                                        bcs={"wall": ("mirror",), "transmissive": FREESTREAM_NACA,
+                                             "inlet": FREESTREAM_NACA, "outlet": ("copy",)}),
     # implicit residual smoothing (explicit.cpp:32-66, 743)
     "smooth_first_split": dict(flux="roe_split", dt="sum_cfl", cfl=0.5, gdt=True, smooth=True, bcs=BC_ALL),
     "smooth_second_michalak": dict(flux="roe_split", dt="edge_min", cfl=1.0, phi="michalak", grad=True,

@@ -103,18 +103,75 @@ def test_sod_full_size_explicit_euler():
     assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
 
 
-def test_forward_step_full_size_two_steps():
-    """The 975,744-cell Mach-3 step workload, two RK5 steps, bit-exact."""
+@pytest.mark.timeout(900)
+def test_forward_step_full_size_sixty_steps():
+    """The 975,744-cell Mach-3 step workload, 60 RK5 steps (300 stages),
+    bit-exact.  The positivity fallback has taken over faces at the step
+    corner by then: the oracle without it differs."""
     m, cfg, ic = workloads.forward_step()
+    assert cfg["positivity"]
     om = O.Mesh(m)
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg)
     q0 = workloads.initial_state(om, ic)
     s.set_state(q0)
-    st = s.run(max_step=2)
-    qo, steps, t, R, R0, _ = om.run(cfg, q0, 2)
-    assert st["step"] == steps == 2 and st["time"] == t
-    assert bits_equal(s.get_state(), qo), rel_err(qo, s.get_state())
+    st = s.run(max_step=60)
+    qg = s.get_state()
+    print("gpu done", flush=True)
+    qo, steps, t, R, R0, _ = om.run(cfg, q0, 60)
+    print("oracle done", flush=True)
+    assert st["step"] == steps == 60 and st["time"] == t
+    assert np.all(np.isfinite(qo))
+    assert bits_equal(qg, qo), rel_err(qo, qg)
+    qn, *_ = om.run(dict(cfg, positivity=False), q0, 60)
+    assert not same_values(qn, qo), "the fallback never triggered"
+
+
+def _face_states(om, q, gx, gy, lim):
+    """Reconstructed face state of each edge side (side 0, side 1)."""
+    q4, gx4, gy4, l4 = (a.reshape(-1, 4) for a in (q, gx, gy, lim))
+    out = []
+    for side in (0, 1):
+        c = om.edge_cells[:, side]
+        dx = (om.edge_cx - om.cell_cx[c])[:, None]
+        dy = (om.edge_cy - om.cell_cy[c])[:, None]
+        out.append(q4[c] + (gx4[c] * dx + gy4[c] * dy) * l4[c])
+    return out
+
+
+@pytest.mark.parametrize("flux_mode", ["tile", "cell", "fused"])
+def test_positivity_fallback_phases_bit_exact(flux_mode):
+    """A state with strong density and pressure contrasts and a limiter that
+    never acts (K = 1e4, so sigma = 1 and the reconstruction is the plain
+    linear one): a third of the faces reconstruct to rho <= 0 or p <= 0 and
+    take the cell value.  GPU phases (two-pass forms) and a 3-step run (every
+    form) equal the oracle bit for bit."""
+    print("case positivity phases", flux_mode, flush=True)
+    cfg = dict(CONFIGS["second_venkat_global"], positivity=True, K=1e4)
+    m = MESHES["mixed"]()
+    om = O.Mesh(m)
+    u = meshgen.uniform01(4242, 2 * om.n_cells).reshape(-1, 2)
+    rho = 0.02 + 2.0 * u[:, 0] ** 3
+    p = 0.02 + 2.0 * u[:, 1] ** 3
+    q0 = np.stack([rho, 0.1 * rho, -0.05 * rho, p / 0.4 + 0.5 * rho * 0.0125], 1).ravel()
+    o = om.phases(cfg, q0, alpha=0.2898)
+    faces = _face_states(om, q0, o["gx"], o["gy"], o["lim"])
+    bad = 0
+    for f in faces:
+        pf = 0.4 * (f[:, 3] - 0.5 * (f[:, 1] ** 2 + f[:, 2] ** 2) / f[:, 0])
+        bad += int(np.sum(~((f[:, 0] > 0) & (pf > 0))))
+    print("non-physical faces", bad, flush=True)
+    assert bad > 0
+    s = Solver(Mesh.from_arrays(m), cfg, flux_mode=flux_mode)
+    s.set_state(q0)
+    if flux_mode != "fused":
+        g = s.phases(alpha=0.2898)
+        for key in ("dt", "gx", "gy", "lim", "qt", "qk"):
+            assert bits_equal(g[key], o[key]), key
+    s.set_state(q0)
+    s.run(max_step=3)
+    qo, *_ = om.run(cfg, q0, 3)
+    assert same_values(s.get_state(), qo)
 
 
 def test_positivity_fallback_bit_exact():

@@ -487,3 +487,195 @@ def test_noslip_and_viscous_farfield_boundaries():
         pa = (G - 1) * (a[3] - 0.5 / a[0] * (a[1] ** 2 + a[2] ** 2))
         pc = (G - 1) * (c[3] - 0.5 / c[0] * (c[1] ** 2 + c[2] ** 2))
         np.testing.assert_allclose(pa, pc, rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# Positivity fallback (an extension without a reference counterpart): where a
+# limited linear reconstruction gives a face state with rho <= 0 or p <= 0,
+# that face takes the cell value.  These pins build inputs in which exactly
+# one face is non-physical and check the result against what the rule means,
+# with the flux taken from the textbook Roe flux above where the two sides
+# agree, not from the oracle's own fallback code.
+# ---------------------------------------------------------------------------
+
+POS_CFG = dict(flux="roe_joined", lin=True, grad=True, lim=True, phi="venkat", gamma=G,
+               bcs={"wall": ("mirror",), "transmissive": ("copy",)})
+
+
+def _phase_qt(om, cfg, q, gx, gy, lim):
+    L = O.lib()
+    c = O.make_config(cfg, om.names)
+    qt = np.zeros(4 * om.n_cells)
+    L.orc_calc_time_derivatives(om._p, C.byref(c), q, gx, gy, lim, qt)
+    return qt
+
+
+def _phase_bounds(om, cfg, q, gx, gy, lim):
+    L = O.lib()
+    c = O.make_config(cfg, om.names)
+    out = q.copy()
+    L.orc_update_bounds(om._p, C.byref(c), out, gx, gy, lim)
+    return out
+
+
+def _edges_of(om, cell):
+    return [e for e in range(om.n_edges) if cell in (om.edge_cells[e, 0], om.edge_cells[e, 1])]
+
+
+def _pressure(q):
+    return (G - 1) * (q[3] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0])
+
+
+def _one_bad_face(om, q0, var, amount, want_boundary, side=0):
+    """A cell i on side `side` of one of its edges e, and a gradient of
+    variable `var` in i only (limiter 1 everywhere) that takes face e of i
+    down by `amount` while every other face in the mesh stays physical.
+    Returns (i, e, gx, gy, lim)."""
+    n = 4 * om.n_cells
+    for i in range(om.n_real):
+        edges = _edges_of(om, i)
+        bnd = [e for e in edges if om.edge_cells[e, 1] >= om.n_real]
+        if want_boundary != bool(bnd):
+            continue
+        for e in (bnd if want_boundary else edges):
+            if om.edge_cells[e, side] != i:
+                continue
+            d = np.array([om.edge_cx[e] - om.cell_cx[i], om.edge_cy[e] - om.cell_cy[i]])
+            g = -amount * d / (d @ d)
+            ok = True
+            for f in edges:
+                if f == e:
+                    continue
+                df = np.array([om.edge_cx[f] - om.cell_cx[i], om.edge_cy[f] - om.cell_cy[i]])
+                face = q0[4 * i:4 * i + 4].copy()
+                face[var] += g @ df
+                if not (face[0] > 0.2 and _pressure(face) > 0.2):
+                    ok = False
+            if not ok:
+                continue
+            gx, gy = np.zeros(n), np.zeros(n)
+            gx[4 * i + var], gy[4 * i + var] = g
+            return i, e, gx, gy, np.ones(n)
+    raise AssertionError("no suitable cell")
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("var,amount", [(0, 1.05), (3, 2.6)])
+def test_positivity_fallback_replaces_one_interior_face(var, amount, side):
+    """rho <= 0 (var 0) or p <= 0 (var 3, the energy taken below the kinetic
+    part) on exactly one interior face: the fallback changes qt in the two
+    cells of that edge only; with that face at the cell value the edge sees
+    two equal states, so its flux is the physical flux (the textbook Roe flux
+    of equal states), cell j's qt is then exactly that of the uniform state,
+    cell i's qt is the edge sum of Roe fluxes with that one face replaced,
+    and the total sum A qt (interior edges cancel) is unchanged."""
+    m = meshgen.mixed_channel(nx=10, ny=5)
+    om = O.Mesh(m)
+    q0 = np.tile(prim(1.0, 0.3, 0.1, 1.0), om.n_cells)
+    i, e, gx, gy, lim = _one_bad_face(om, q0, var, amount, want_boundary=False, side=side)
+    j = om.edge_cells[e, 1 - side]
+    assert j < om.n_real
+    d = np.array([om.edge_cx[e] - om.cell_cx[i], om.edge_cy[e] - om.cell_cy[i]])
+    bad = q0[4 * i:4 * i + 4] + np.array([gx[4 * i + k] * d[0] + gy[4 * i + k] * d[1] for k in range(4)])
+    assert not (bad[0] > 0 and _pressure(bad) > 0)
+    z = np.zeros(4 * om.n_cells)
+    qt_uniform = _phase_qt(om, dict(POS_CFG, positivity=True), q0, z, z, lim)
+    on = _phase_qt(om, dict(POS_CFG, positivity=True), q0, gx, gy, lim)
+    with np.errstate(all="ignore"):
+        off = _phase_qt(om, dict(POS_CFG, positivity=False), q0, gx, gy, lim)
+    assert np.all(np.isfinite(on))
+    # only cells i and j see the fallback
+    other = np.ones(om.n_cells, dtype=bool)
+    other[[i, j]] = False
+    on4, off4 = on.reshape(-1, 4), off.reshape(-1, 4)
+    assert np.array_equal(on4[other].view(np.uint64), off4[other].view(np.uint64))
+    assert not np.array_equal(on4[i], off4[i]) and not np.array_equal(on4[j], off4[j])
+    # edge e carries the physical flux of the uniform state: j is untouched
+    assert np.array_equal(on4[j].view(np.uint64), qt_uniform.reshape(-1, 4)[j].view(np.uint64))
+    # cell i: the edge sum with face e at the cell value, flux of the other
+    # faces from the oracle's Roe flux (pinned above and by the fixtures)
+    exp = np.zeros(4)
+    for f in _edges_of(om, i):
+        n = np.array([om.edge_nx[f], om.edge_ny[f]])
+        side = 0 if om.edge_cells[f, 0] == i else 1
+        df = np.array([om.edge_cx[f] - om.cell_cx[i], om.edge_cy[f] - om.cell_cy[i]])
+        face = q0[4 * i:4 * i + 4] + np.array([gx[4 * i + k] * df[0] + gy[4 * i + k] * df[1] for k in range(4)])
+        other_side = q0[4 * i:4 * i + 4]   # every neighbour holds the uniform state, zero gradient
+        if f == e:
+            flux = roe_textbook(q0[4 * i:4 * i + 4], other_side, n)
+            np.testing.assert_allclose(flux, phys_flux(q0[4 * i:4 * i + 4], n), rtol=1e-14, atol=1e-14)
+        else:
+            flux = O.flux(POS_CFG, face, other_side, n) if side == 0 else O.flux(POS_CFG, other_side, face, n)
+        exp += (-1 if side == 0 else 1) * flux * om.edge_len[f] / om.cell_area[i]
+    np.testing.assert_allclose(on4[i], exp, rtol=1e-12, atol=1e-12)
+    # conservation: the fallback flux is single-valued on the edge
+    A = om.cell_area[:om.n_real, None]
+    tot = np.sum(A * on4[:om.n_real], axis=0)
+    tot_u = np.sum(A * qt_uniform.reshape(-1, 4)[:om.n_real], axis=0)
+    # only cell i's other faces differ from the uniform run; their fluxes
+    # cancel between i and its neighbours too
+    np.testing.assert_allclose(tot, tot_u, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("var,amount", [(0, 1.05), (3, 2.6)])
+def test_positivity_fallback_on_a_boundary_face(var, amount):
+    """A non-physical reconstruction on a boundary face: update_bounds gives
+    the ghost the boundary function of the cell value, exactly as the
+    pinned boundary function evaluates it, and leaves every other ghost as
+    the unguarded run does."""
+    m = meshgen.mixed_channel(nx=10, ny=5)
+    om = O.Mesh(m)
+    q0 = np.tile(prim(1.0, 0.3, 0.1, 1.0), om.n_cells)
+    i, e, gx, gy, lim = _one_bad_face(om, q0, var, amount, want_boundary=True)
+    ghost = om.edge_cells[e, 1]
+    on = _phase_bounds(om, dict(POS_CFG, positivity=True), q0, gx, gy, lim).reshape(-1, 4)
+    with np.errstate(all="ignore"):
+        off = _phase_bounds(om, dict(POS_CFG, positivity=False), q0, gx, gy, lim).reshape(-1, 4)
+    b = list(om.bnd_edge).index(e)
+    name = m.names[m.bnd_tag[b]]
+    n = np.array([om.edge_nx[e], om.edge_ny[e]])
+    want = O.bc(POS_CFG["bcs"][name], G, q0[4 * i:4 * i + 4], n)
+    assert np.array_equal(on[ghost].view(np.uint64), want.view(np.uint64))
+    assert not np.array_equal(on[ghost], off[ghost])
+    rest = np.ones(om.n_cells, dtype=bool)
+    rest[ghost] = False
+    assert np.array_equal(on[rest].view(np.uint64), off[rest].view(np.uint64))
+    # the flux pass takes the same face: qt of i with the fallback equals
+    # qt with that face's increment removed (only face e changes)
+    qt_on = _phase_qt(om, dict(POS_CFG, positivity=True), on.ravel(), gx, gy, lim).reshape(-1, 4)
+    assert np.all(np.isfinite(qt_on))
+
+
+@pytest.mark.parametrize("mesh_name", ["mixed", "step", "naca"])
+def test_positivity_fallback_is_inert_on_physical_faces(mesh_name):
+    """With every reconstructed face physical the flag changes no bit: one
+    stage of every phase, and 5-step runs that stay physical."""
+    from helpers import MESHES, CONFIGS, airfoil_bump_state
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    E = om.edge_cells
+    runs = 0
+    for cfg_name in ("second_venkat_global", "second_michalak_local", "second_minmod_farfield"):
+        cfg = CONFIGS[cfg_name]
+        q = airfoil_bump_state(om.cell_cx, om.cell_cy) if mesh_name == "naca" else \
+            random_state(om.n_cells, seed=11, mach=0.3)
+        a = om.phases(dict(cfg, positivity=False), q, alpha=0.5)
+        b = om.phases(dict(cfg, positivity=True), q, alpha=0.5)
+        # every face of the stage is physical (otherwise the check is void)
+        gx, gy, lim = (a[k].reshape(-1, 4) for k in ("gx", "gy", "lim"))
+        q4 = q.reshape(-1, 4)
+        for side in (0, 1):
+            c = E[:, side]
+            dx = (om.edge_cx - om.cell_cx[c])[:, None]
+            dy = (om.edge_cy - om.cell_cy[c])[:, None]
+            face = q4[c] + (gx[c] * dx + gy[c] * dy) * lim[c]
+            p = (G - 1) * (face[:, 3] - 0.5 * (face[:, 1] ** 2 + face[:, 2] ** 2) / face[:, 0])
+            assert np.all(face[:, 0] > 0) and np.all(p > 0), cfg_name
+        for key in ("qt", "qk"):
+            assert np.array_equal(a[key].view(np.uint64), b[key].view(np.uint64)), (cfg_name, key)
+        qa, *_ = om.run(dict(cfg, positivity=False), q, 5)
+        if np.all(np.isfinite(qa)):
+            qb, *_ = om.run(dict(cfg, positivity=True), q, 5)
+            assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64)), cfg_name
+            runs += 1
+    assert runs >= 1
