@@ -419,7 +419,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default=None, choices=["f64", "f32"],
+    ap.add_argument("--precision", default=None, choices=["f64", "f32", "f32_f64"],
                     help="default: f64 (f32 for the sweep, whose state is FP32)")
     ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint", "plate"])
     ap.add_argument("--smooth", action="store_true", help="plate, airfoil: with residual smoothing")

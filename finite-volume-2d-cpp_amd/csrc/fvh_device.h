@@ -44,6 +44,25 @@ __device__ inline void block_sum4(double* v) {
 template <class T>
 __device__ inline Q4<T> ld(const Q4<T>* p) { return *p; }
 
+// State loads and stores between the storage type S of the cell states and
+// the arithmetic type T (identical except in the FP32-state, FP64-arithmetic
+// mode, where a load widens exactly and a store rounds to nearest).
+template <class T, class S>
+__device__ inline Q4<T> ldq(const Q4<S>* p) {
+    const Q4<S> a = *p;
+    Q4<T> r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.v[k] = T(a.v[k]);
+    return r;
+}
+template <class T, class S>
+__device__ inline void stq(Q4<S>* p, const Q4<T>& a) {
+    Q4<S> r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.v[k] = S(a.v[k]);
+    *p = r;
+}
+
 // XCD-aware block order.  The dispatcher hands workgroup b to XCD b % 8, and
 // each XCD has its own L2; remapping hardware block b to logical block
 // (b % 8) * per + b / 8 gives every XCD one contiguous range of cells, so a

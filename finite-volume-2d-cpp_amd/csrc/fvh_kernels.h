@@ -125,10 +125,13 @@ struct PhysParams {
 };
 
 // Launchers (implemented in fvh_kernels.hip).  All take the solver stream;
-// ctrl is the per-instance control array (nullptr: no stop test).
-template <class T>
+// ctrl is the per-instance control array (nullptr: no stop test).  T is the
+// arithmetic type (and that of the geometry and of every intermediate
+// array), S the storage type of the cell states: (double, double) and
+// (float, float), and (double, float) for FP32 states with FP64 arithmetic.
+template <class T, class S>
 void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_kind, const PhysParams& pp,
-               const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl);
+               const Q4<S>* q, T* dt, double* blk_min, const Ctrl* ctrl);
 // Per-instance minimum of the time step over real and ghost cells (min_dt).
 void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells,
                    int dt_is_f32, void* dt_global, const Ctrl* ctrl);
@@ -136,9 +139,9 @@ void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, con
 // every cell and per-tile minima (first stage of a step).  dbg_gx/dbg_gy
 // (when non-null) receive every cell's gradient, zero for ghosts; dbg_lim
 // (when non-null) its limiter.
-template <class T>
+template <class T, class S>
 void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
-                    bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr,
+                    bool grad, bool lim, bool lin, const Q4<S>* q, Q4<T>* face, Q4<T>* bincr,
                     int dt_kind, T* dt, double* blk_min,
                     Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl);
 // Flux + RK update + boundary update; tiles == nullptr selects the per-cell
@@ -146,20 +149,20 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
 // (global time step per instance) replaces the per-cell dt when non-null.
 // flux_kind: FVH_FLUX_* (0 split, 1 joined, 2 split + viscous terms; the
 // viscous kind reads the cell gradients gxc, gyc when pp.diffusive).
-template <class T>
+template <class T, class S>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles,
-                 const PhysParams& pp, int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<T>* qin,
-                 const Q4<T>* q0, Q4<T>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
+                 const PhysParams& pp, int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<S>* qin,
+                 const Q4<S>* q0, Q4<S>* qout, const Q4<T>* face, const Q4<T>* bincr, const T* dt,
                  const T* dt_inst, T alpha, double* res_blk, Q4<T>* dbg_qt, const Ctrl* ctrl,
                  Q4<T>* qt_only = nullptr, const Q4<T>* gxc = nullptr, const Q4<T>* gyc = nullptr);
 // One sweep of implicit residual smoothing (explicit.cpp:32-66) over the qt
 // that a flux pass with qt_only stored: qt_out != nullptr stores the sweep's
 // qt; qt_out == nullptr finishes the stage from it (update, boundary update,
 // residual sums), taking the flux pass's place.
-template <class T>
+template <class T, class S>
 void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, bool lin,
-                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<T>* q0,
-                   Q4<T>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
+                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<S>* q0,
+                   Q4<S>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
                    Q4<T>* dbg_qt, const Ctrl* ctrl);
 // Optional outputs of a fused stage for the adjoint's replays: every cell's
 // gradient and limiter (ghosts: zero gradient, reset limiter) and every real
@@ -178,13 +181,20 @@ struct StageRecord {
 // threads: workgroup size, 256 or 512; 0 picks 256 when four workgroups'
 // halos fit a CU's LDS, else 512.
 // Returns false when the tile halo does not fit the workgroup's LDS.
-template <class T>
+template <class T, class S>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
-                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
+                  const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0,
                   const StageRecord<T>& rec = StageRecord<T>());
-template <class T>
+// LDS of the fused stage pass: the states of a tile's local cell list in the
+// storage type, then two face states per tile edge in the arithmetic type
+// (32-byte aligned).
+template <class S>
+__host__ __device__ inline size_t stage_lds_states(const HaloView& hv) {
+    return ((size_t)hv.max_ext * sizeof(Q4<S>) + 31) / 32 * 32;
+}
+template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv);
 // Workgroup size the fused stage pass takes for a halo of lds bytes when
 // asked for `threads`.  0: 256 lanes when four workgroups fit a CU's LDS, or
@@ -196,9 +206,31 @@ inline int stage_lanes(size_t lds, int threads, int mean_g) {
     if (lds + 1024 <= 160 * 1024 / 3 && mean_g <= kStageBlock / 2) return kStageBlock / 2;
     return kStageBlock;
 }
-template <class T>
+// Boundary update of the ghost cells from the real cells' states, with
+// given gradients and limiters (null: zero ones, the initial update).
+template <class T, class S>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
-                        const BcParam<T>* bcs, Q4<T>* q);
+                        const BcParam<T>* bcs, Q4<S>* q, const Q4<T>* gx = nullptr, const Q4<T>* gy = nullptr,
+                        const Q4<T>* lim = nullptr);
+// The phases of explicit.cpp one at a time on given arrays (device order):
+// limiters, neighbour minima and maxima from given gradients; face states
+// from given gradients and limiters (for launch_flux with qt_only);
+// update_cells over n values; min_dt over n values in place (part: at least
+// array_min_blocks(n) doubles); residual norms per instance from a given qt.
+template <class T>
+void launch_phase_limiters(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind, const Q4<T>* q,
+                           const Q4<T>* gx, const Q4<T>* gy, Q4<T>* lim, Q4<T>* qmin, Q4<T>* qmax);
+template <class T>
+void launch_phase_faces(hipStream_t s, const MeshView<T>& m, const Q4<T>* q, const Q4<T>* gx, const Q4<T>* gy,
+                        const Q4<T>* lim, Q4<T>* face);
+void launch_update_cells(hipStream_t s, double* q, const double* ql, const double* qt, const double* dt, double v,
+                         int64_t n);
+constexpr int kArrayMinBlocks = 1024;
+int array_min_blocks(int64_t n);
+void launch_array_min(hipStream_t s, double* a, int64_t n, double* part);
+template <class T>
+void launch_residuals(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* qt,
+                      double* blk, double* out);
 void launch_ctrl_begin(hipStream_t s, int n_inst, Ctrl* ctrl, uint32_t max_step, double max_time, double tol,
                        int stop_var);
 void launch_ctrl_end(hipStream_t s, const InstView& iv, Ctrl* ctrl, const double* res_blk, int need_res,
@@ -251,8 +283,8 @@ template <class T>
 void launch_copy(hipStream_t s, const void* a, void* b, int n);
 
 // FP64 conserved sums sum_i A_i q_ik over each instance's real cells.
-template <class T>
-void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* q,
+template <class T, class S>
+void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<S>* q,
                  double* blk, double* out);
 
 }  // namespace fvh

@@ -11,9 +11,9 @@ namespace {
 // time step: calc_dt kinds + block minimum for the global-dt reduction
 // (first-order configurations; second order computes dt in k_gradlim)
 // ---------------------------------------------------------------------------
-template <class T, int KIND>
+template <class T, int KIND, class S>
 __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, const Q4<T>& qc,
-                            const Q4<T>* __restrict__ q) {
+                            const Q4<S>* __restrict__ q) {
     if (KIND == 0) return T(pp.dt_fixed);
     const T g = T(pp.gamma);
     const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
@@ -23,7 +23,7 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
         for (int h = h0; h < h1; ++h) {
             const uint32_t ue = m.he_edge[h];
             const Q4<T> geo = m.e_geo[ue & kEdgeMask];
-            const Q4<T> qn = ld(q + m.he_nbr[h]);
+            const Q4<T> qn = ldq<T>(q + m.he_nbr[h]);
             const bool side = (ue & kSideBit) != 0;
             const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
             const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
@@ -40,7 +40,7 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
             const uint32_t ue = m.he_edge[h];
             const Q4<T> geo = m.e_geo[ue & kEdgeMask];
             const int nb = m.he_nbr[h];
-            const Q4<T> qn = ld(q + nb);
+            const Q4<T> qn = ldq<T>(q + nb);
             const bool side = (ue & kSideBit) != 0;
             const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]) + T(2) * eig_visc(g, mu, pr, qc.v, geo.v[2], m.c_area[c]);
             const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]) + T(2) * eig_visc(g, mu, pr, qn.v, geo.v[2], m.c_area[nb]);
@@ -72,16 +72,16 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
     return r;
 }
 
-template <class T, int KIND>
+template <class T, int KIND, class S>
 __global__ __launch_bounds__(kBlock) void k_dt(MeshView<T> m, TileMap tm, PhysParams pp,
-                                               const Q4<T>* __restrict__ q, T* __restrict__ dt,
+                                               const Q4<S>* __restrict__ q, T* __restrict__ dt,
                                                double* __restrict__ blk_min, const Ctrl* __restrict__ ctrl) {
     const int lb = xcd_block(tm.n_all);
     if (lb < 0 || stopped(ctrl, tm.inst[lb])) return;
     const int c = tm.c0[lb] + threadIdx.x;
     double v = kInf;
     if (c < tm.c0[lb + 1]) {
-        const T r = cell_dt<T, KIND>(m, pp, c, ld(q + c), q);
+        const T r = cell_dt<T, KIND>(m, pp, c, ldq<T>(q + c), q);
         dt[c] = r;
         v = (double)r;
     }
@@ -105,6 +105,55 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(InstView iv, const double* __
         // min_dt starts from dt[0] (explicit.cpp:557): a NaN there survives
         const T d0 = dt_cells[iv.dev0[i]];
         dt_global[i] = (d0 != d0) ? d0 : T(v);
+    }
+}
+
+// Limiter of one real cell from its neighbour min/max and its gradient
+// (explicit.cpp:207-257): lim[k] = min over the cell's half-edges of
+// sigma + (1 - sigma) phi(r), starting from the reset value in lim.
+template <class T, int PHI>
+__device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, int c, int h0, int h1, const Q4<T>& qc,
+                                  const G2<T>& cc, const T (&qmn)[4], const T (&qmx)[4], const T (&gx)[4],
+                                  const T (&gy)[4], T (&lim)[4]) {
+    const T Ka = T(pp.K) * m.c_sqrt_area[c];
+    const T K3a = Ka * Ka * Ka;
+    const T yt = T(pp.phi_yt);
+    const T tol = T(1e-15);
+    // the smoothing sigma depends on the cell and the variable only
+    // (explicit.cpp:226-238): once per variable, the same expressions
+    T sigs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const T dmax = qmx[k] - qc.v[k];
+        const T dmin = qmn[k] - qc.v[k];
+        const T spread = (dmax - dmin) * (dmax - dmin);
+        if (spread <= K3a) {
+            sigs[k] = T(1);
+        } else if (spread <= T(2) * K3a) {
+            const T y = (spread / K3a - T(1));
+            sigs[k] = T(2) * y * y * y - T(3) * y * y + T(1);
+        } else {
+            sigs[k] = T(0);
+        }
+    }
+    for (int h = h0; h < h1; ++h) {
+        const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
+        const T dx = ec.v[0] - cc.v[0];
+        const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const T dqg = gx[k] * dx + gy[k] * dy;
+            const T dmax = qmx[k] - qc.v[k];
+            const T dmin = qmn[k] - qc.v[k];
+            const T sig = sigs[k];
+            T l = T(1);
+            if (sig < T(1)) {
+                if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
+                else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+            }
+            l = sig + (T(1) - sig) * l;
+            lim[k] = smin(lim[k], l);
+        }
     }
 }
 
@@ -195,55 +244,14 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
             gy[k] = gy[k] * ia;
         }
     }
-    if (real && lim_on) {
-        const T Ka = T(pp.K) * m.c_sqrt_area[c];
-        const T K3a = Ka * Ka * Ka;
-        const T yt = T(pp.phi_yt);
-        const T tol = T(1e-15);
-        // the smoothing sigma depends on the cell and the variable only
-        // (explicit.cpp:226-238): once per variable, the same expressions
-        T sigs[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const T dmax = qmx[k] - qc.v[k];
-            const T dmin = qmn[k] - qc.v[k];
-            const T spread = (dmax - dmin) * (dmax - dmin);
-            if (spread <= K3a) {
-                sigs[k] = T(1);
-            } else if (spread <= T(2) * K3a) {
-                const T y = (spread / K3a - T(1));
-                sigs[k] = T(2) * y * y * y - T(3) * y * y + T(1);
-            } else {
-                sigs[k] = T(0);
-            }
-        }
-        for (int h = h0; h < h1; ++h) {
-            const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
-            const T dx = ec.v[0] - cc.v[0];
-            const T dy = ec.v[1] - cc.v[1];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T dqg = gx[k] * dx + gy[k] * dy;
-                const T dmax = qmx[k] - qc.v[k];
-                const T dmin = qmn[k] - qc.v[k];
-                const T sig = sigs[k];
-                T l = T(1);
-                if (sig < T(1)) {
-                    if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
-                    else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
-                }
-                l = sig + (T(1) - sig) * l;
-                lim[k] = smin(lim[k], l);
-            }
-        }
-    }
+    if (real && lim_on) limit_cell<T, PHI>(m, pp, c, h0, h1, qc, cc, qmn, qmx, gx, gy, lim);
 }
 
 // Two-pass form, pass 1: gradient + limiter + face states of every cell of
 // the tile, optionally its time step and the tile's minimum.
-template <class T, int PHI, int DT>
+template <class T, int PHI, int DT, class S>
 __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, PhysParams pp, int grad, int lim_on,
-                                                    int lin, const Q4<T>* __restrict__ q,
+                                                    int lin, const Q4<S>* __restrict__ q,
                                                     Q4<T>* __restrict__ face, Q4<T>* __restrict__ bincr,
                                                     T* __restrict__ dt, double* __restrict__ blk_min,
                                                     Q4<T>* __restrict__ dbg_gx, Q4<T>* __restrict__ dbg_gy,
@@ -254,14 +262,14 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
     double vmin = kInf;
     if (c < tm.c0[lb + 1]) {
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-        const Q4<T> qc = ld(q + c);
+        const Q4<T> qc = ldq<T>(q + c);
         const G2<T> cc = m.c_c[c];
         const bool real = c < m.n_real;
         T gx[4], gy[4], lim[4], dtc = T(0);
         auto nbr = [&](int h, bool& self) {
             const int nb = m.he_nbr[h];
             self = nb == c;
-            return ld(q + nb);
+            return ldq<T>(q + nb);
         };
         grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, real, true, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
         if (DT) {
@@ -300,18 +308,18 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
 // side of boundary half-edge h (this stage's gradient and limiter applied
 // to the just-updated state: the reference's stale-gradient boundary).
 // ---------------------------------------------------------------------------
-template <class T, bool LIN, class Inc>
+template <class T, bool LIN, class S, class Inc>
 __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
-                                   int h0, int h1, const T (&qt)[4], const Q4<T>* __restrict__ q0,
-                                   Q4<T>* __restrict__ qout, Inc inc_of,
+                                   int h0, int h1, const T (&qt)[4], const Q4<S>* __restrict__ q0,
+                                   Q4<S>* __restrict__ qout, Inc inc_of,
                                    const T d, T alpha, bool want_rs,
                                    double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
     const T g = T(pp.gamma);
-    const Q4<T> qs = ld(q0 + c);
+    const Q4<T> qs = ldq<T>(q0 + c);
     Q4<T> out;
 #pragma unroll
     for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
-    qout[c] = out;
+    stq(qout + c, out);
     if (dbg_qt) {
         Q4<T> t;
 #pragma unroll
@@ -342,7 +350,7 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
         }
         Q4<T> gs;
         boundary(bcs[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
-        qout[nb] = gs;
+        stq(qout + nb, gs);
     }
 }
 
@@ -350,16 +358,16 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
 // kind adds the laminar terms with the face gradient of cells ci (side 0)
 // and cj (side 1), or a zero gradient without pp.diffusive
 // (explicit.cpp:324-344).
-template <class T, int FK>
+template <class T, int FK, class S>
 __device__ inline void edge_flux(const MeshView<T>& m, const PhysParams& pp, T g, T* f, const T* qi, const T* qj,
-                                 T n0, T n1, int ci, int cj, const Q4<T>* __restrict__ qcell,
+                                 T n0, T n1, int ci, int cj, const Q4<S>* __restrict__ qcell,
                                  const Q4<T>* __restrict__ gxc, const Q4<T>* __restrict__ gyc) {
     roe_flux<T, FK == 1>(g, f, qi, qj, n0, n1);
     if (FK == 2) {
         T gxv[4] = {T(0), T(0), T(0), T(0)}, gyv[4] = {T(0), T(0), T(0), T(0)};
         if (pp.diffusive) {
             const Q4<T> gxi = ld(gxc + ci), gyi = ld(gyc + ci), gxj = ld(gxc + cj), gyj = ld(gyc + cj);
-            const Q4<T> qa = ld(qcell + ci), qb = ld(qcell + cj);
+            const Q4<T> qa = ldq<T>(qcell + ci), qb = ldq<T>(qcell + cj);
             const G2<T> pa = m.c_c[ci], pb = m.c_c[cj];
             face_gradient(gxv, gyv, gxi.v, gyi.v, gxj.v, gyj.v, qa.v, qb.v, pa.v[0], pa.v[1], pb.v[0], pb.v[1]);
         }
@@ -368,11 +376,11 @@ __device__ inline void edge_flux(const MeshView<T>& m, const PhysParams& pp, T g
 }
 
 // Per-cell form: every interior edge flux is computed by both of its cells.
-template <class T, int FK, bool LIN>
+template <class T, int FK, bool LIN, class S>
 __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm, PhysParams pp,
                                                       const BcParam<T>* __restrict__ bcs,
-                                                      const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
-                                                      Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
+                                                      const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
+                                                      Q4<S>* __restrict__ qout, const Q4<T>* __restrict__ face,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
@@ -398,12 +406,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
                 a = ld(face + h);
                 b = ld(face + m.he_twin[h]);
                 if (pp.positivity) {
-                    if (!physical(g, a.v)) a = ld(qin + c);
-                    if (!physical(g, b.v)) b = ld(qin + m.he_nbr[h]);
+                    if (!physical(g, a.v)) a = ldq<T>(qin + c);
+                    if (!physical(g, b.v)) b = ldq<T>(qin + m.he_nbr[h]);
                 }
             } else {
-                a = ld(qin + c);
-                b = ld(qin + m.he_nbr[h]);
+                a = ldq<T>(qin + c);
+                b = ldq<T>(qin + m.he_nbr[h]);
             }
             const bool side = (ue & kSideBit) != 0;
             T qi[4], qj[4], f[4];
@@ -444,11 +452,11 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
 // edge-parallel, into LDS; phase 2 gathers each cell's edges from LDS in
 // reference edge order.  Edges on a tile border are computed by both tiles
 // (same inputs, same bits).
-template <class T, int FK, bool LIN>
+template <class T, int FK, bool LIN, class S>
 __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm, TileView tv, PhysParams pp,
                                                       const BcParam<T>* __restrict__ bcs,
-                                                      const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
-                                                      Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ face,
+                                                      const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
+                                                      Q4<S>* __restrict__ qout, const Q4<T>* __restrict__ face,
                                                       const Q4<T>* __restrict__ bincr, const T* __restrict__ dt,
                                                       const T* __restrict__ dt_inst, T alpha,
                                                       double* __restrict__ res_blk,
@@ -470,13 +478,13 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
             a = ld(face + hh.x);
             b = ld(face + hh.y);
             if (pp.positivity) {
-                if (!physical(g, a.v)) a = ld(qin + tv.cell[k].x);
-                if (!physical(g, b.v)) b = ld(qin + tv.cell[k].y);
+                if (!physical(g, a.v)) a = ldq<T>(qin + tv.cell[k].x);
+                if (!physical(g, b.v)) b = ldq<T>(qin + tv.cell[k].y);
             }
         } else {
             const int2 cc = tv.cell[k];
-            a = ld(qin + cc.x);
-            b = ld(qin + cc.y);
+            a = ldq<T>(qin + cc.x);
+            b = ldq<T>(qin + cc.y);
         }
         T f[4];
         const int2 ce = tv.cell[k];
@@ -528,12 +536,12 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
 // boundary ghost contributes its qt of 0 (explicit.cpp:352-362).  With
 // qt_out the sweep's result is stored; without, it finishes the stage
 // (update, boundary update, residual sums) as the flux pass would have.
-template <class T, bool LIN>
+template <class T, bool LIN, class S>
 __global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, PhysParams pp,
                                                    const BcParam<T>* __restrict__ bcs,
                                                    const Q4<T>* __restrict__ qt_nb, const Q4<T>* __restrict__ qt0,
-                                                   Q4<T>* __restrict__ qt_out, const Q4<T>* __restrict__ q0,
-                                                   Q4<T>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
+                                                   Q4<T>* __restrict__ qt_out, const Q4<S>* __restrict__ q0,
+                                                   Q4<S>* __restrict__ qout, const Q4<T>* __restrict__ bincr,
                                                    const T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
                                                    double* __restrict__ res_blk, Q4<T>* __restrict__ dbg_qt,
                                                    const Ctrl* __restrict__ ctrl) {
@@ -591,11 +599,11 @@ __global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, Ph
 // the flux in the slot; phase 3 gathers per cell in reference edge order and
 // finishes the stage.  No face state goes through HBM; the boundary
 // increments of the tile's own cells do (bincr), for phase 3.
-template <class T, int PHI, int DT, bool JOINED, int NB>
+template <class T, int PHI, int DT, bool JOINED, int NB, class S>
 __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
                                                   int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
-                                                  const Q4<T>* __restrict__ qin, const Q4<T>* __restrict__ q0,
-                                                  Q4<T>* __restrict__ qout, Q4<T>* __restrict__ bincr,
+                                                  const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
+                                                  Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
                                                   T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
                                                   double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
                                                   StageRecord<T> rec) {
@@ -608,13 +616,14 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
     const int c0 = tm.c0[lb], nt = tm.c0[lb + 1] - c0;
     const int4 ex = hv.ext[lb];
     const int n_g = nt + ex.y, n_ext = nt + ex.z + ex.w;
-    Q4<T>* sq = (Q4<T>*)lds_raw;
-    Q4<T>* fs = sq + hv.max_ext;   // per tile edge slot: face state of side 0, side 1; then the flux in [0]
+    // states in the storage type, face states and fluxes in the arithmetic type
+    Q4<S>* sq = (Q4<S>*)lds_raw;
+    Q4<T>* fs = (Q4<T>*)(lds_raw + stage_lds_states<S>(hv));   // per tile edge slot: face state of side 0, side 1; then the flux in [0]
     const int32_t* xc = hv.cells + ex.x;
     const T g = T(pp.gamma);
     const T lim0 = lim_on ? T(1) : T(0);
     // phase 0: states of tile, ring 1 and ring 2
-    for (int j = threadIdx.x; j < n_ext; j += NB) sq[j] = ld(qin + (j < nt ? c0 + j : xc[j - nt]));
+    for (int j = threadIdx.x; j < n_ext; j += NB) sq[j] = qin[j < nt ? c0 + j : xc[j - nt]];
     __syncthreads();
     // phase 1: gradients and limiters of the tile and its real ring-1 cells,
     // and their face states on the tile's edges
@@ -628,9 +637,9 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         auto nbr = [&](int h, bool& self) {
             const int l = loc[h - h0];
             self = l == j;
-            return sq[l];
+            return ldq<T>(sq + l);
         };
-        const Q4<T> qc = sq[j];
+        const Q4<T> qc = ldq<T>(sq + j);
         const G2<T> cc = m.c_c[c];
         grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
         if (DT && own) dt[c] = dtc;
@@ -667,7 +676,7 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
                     rec.lm[nb] = Q4<T>{{l0[0], l0[1], l0[2], l0[3]}};
                 }
                 const Q4<T> ginc = face_inc(z, z, l0, ec, m.c_c[nb]);
-                const Q4<T> qg = sq[loc[h - h0]];
+                const Q4<T> qg = ldq<T>(sq + loc[h - h0]);
                 Q4<T> fg;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) fg.v[k] = qg.v[k] + ginc.v[k];
@@ -719,14 +728,19 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
     }
 }
 
-// Initial boundary update with zero gradients and limiters (explicit.cpp:712).
-template <class T>
+// Boundary update from the cells' current states (update_bounds,
+// explicit.cpp:380-417): with gradients and limiters given (gxa, gya, lima),
+// or with zero ones when they are null, as the initial update of run() does
+// (explicit.cpp:712).
+template <class T, class S>
 __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParams pp, int lin,
-                                                        const BcParam<T>* __restrict__ bcs, Q4<T>* __restrict__ q) {
+                                                        const BcParam<T>* __restrict__ bcs, Q4<S>* __restrict__ q,
+                                                        const Q4<T>* __restrict__ gxa, const Q4<T>* __restrict__ gya,
+                                                        const Q4<T>* __restrict__ lima) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= m.n_real) return;
     const T g = T(pp.gamma);
-    const Q4<T> qc = ld(q + c);
+    const Q4<T> qc = ldq<T>(q + c);
     const G2<T> cc = m.c_c[c];
     for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
         const uint32_t ue = m.he_edge[h];
@@ -739,16 +753,152 @@ __global__ __launch_bounds__(kBlock) void k_bounds_init(MeshView<T> m, PhysParam
             const T dx = ec.v[0] - cc.v[0];
             const T dy = ec.v[1] - cc.v[1];
             const T zero = T(0);
+            const Q4<T> gxc = gxa ? ld(gxa + c) : Q4<T>{{zero, zero, zero, zero}};
+            const Q4<T> gyc = gya ? ld(gya + c) : Q4<T>{{zero, zero, zero, zero}};
+            const Q4<T> lc = lima ? ld(lima + c) : Q4<T>{{zero, zero, zero, zero}};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) qi[k] = qc.v[k] + (zero * dx + zero * dy) * zero;
+            for (int k = 0; k < 4; ++k) qi[k] = qc.v[k] + (gxc.v[k] * dx + gyc.v[k] * dy) * lc.v[k];
+            if (pp.positivity && !physical(g, qi))
+                for (int k = 0; k < 4; ++k) qi[k] = qc.v[k];
         } else {
 #pragma unroll
             for (int k = 0; k < 4; ++k) qi[k] = qc.v[k];
         }
         Q4<T> gs;
         boundary(bcs[m.he_bc[h]], g, gs.v, qi, geo.v[0], geo.v[1]);
-        q[m.he_nbr[h]] = gs;
+        stq(q + m.he_nbr[h], gs);
     }
+}
+
+// ---------------------------------------------------------------------------
+// The phases one at a time on given arrays (the reference's per-phase
+// functions), in device cell order.
+// ---------------------------------------------------------------------------
+
+// calc_limiters (explicit.cpp:174-258) from given gradients: qmin and qmax of
+// every cell over its face neighbours, ghost cells included (:188-204), in
+// the reference's edge order; the limiter of every real cell; ghost cells
+// keep the reset value 1 (:184-186).
+template <class T, int PHI>
+__global__ __launch_bounds__(kBlock) void k_limiters(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q,
+                                                     const Q4<T>* __restrict__ gxa, const Q4<T>* __restrict__ gya,
+                                                     Q4<T>* __restrict__ lim_out, Q4<T>* __restrict__ qmin_out,
+                                                     Q4<T>* __restrict__ qmax_out) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    const Q4<T> qc = ld(q + c);
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    T qmn[4], qmx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
+    for (int h = h0; h < h1; ++h) {
+        const Q4<T> qn = ld(q + m.he_nbr[h]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            qmn[k] = smin(qmn[k], qn.v[k]);
+            qmx[k] = smax(qmx[k], qn.v[k]);
+        }
+    }
+    T lim[4] = {T(1), T(1), T(1), T(1)};
+    if (c < m.n_real) {
+        const Q4<T> a = ld(gxa + c), b = ld(gya + c);
+        T gx[4], gy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { gx[k] = a.v[k]; gy[k] = b.v[k]; }
+        limit_cell<T, PHI>(m, pp, c, h0, h1, qc, m.c_c[c], qmn, qmx, gx, gy, lim);
+    }
+    lim_out[c] = Q4<T>{{lim[0], lim[1], lim[2], lim[3]}};
+    if (qmin_out) qmin_out[c] = Q4<T>{{qmn[0], qmn[1], qmn[2], qmn[3]}};
+    if (qmax_out) qmax_out[c] = Q4<T>{{qmx[0], qmx[1], qmx[2], qmx[3]}};
+}
+
+// Reconstructed face state of every half-edge of every cell from given
+// gradients and limiters, q + (gx dx + gy dy) lim (explicit.cpp:307-313),
+// for the flux pass.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_faces(MeshView<T> m, const Q4<T>* __restrict__ q,
+                                                  const Q4<T>* __restrict__ gxa, const Q4<T>* __restrict__ gya,
+                                                  const Q4<T>* __restrict__ lima, Q4<T>* __restrict__ face) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    const Q4<T> qc = ld(q + c), a = ld(gxa + c), b = ld(gya + c), l = ld(lima + c);
+    const G2<T> cc = m.c_c[c];
+    T gx[4], gy[4], lim[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { gx[k] = a.v[k]; gy[k] = b.v[k]; lim[k] = l.v[k]; }
+    for (int h = m.he_off[c]; h < m.he_off[c + 1]; ++h) {
+        const Q4<T> inc = face_inc(gx, gy, lim, m.e_c[m.he_edge[h] & kEdgeMask], cc);
+        Q4<T> fs;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fs.v[k] = qc.v[k] + inc.v[k];
+        face[h] = fs;
+    }
+}
+
+// update_cells (explicit.cpp:368-378) over n values: q = ql + qt dt v.
+__global__ __launch_bounds__(kBlock) void k_update_cells(double* __restrict__ q, const double* __restrict__ ql,
+                                                         const double* __restrict__ qt,
+                                                         const double* __restrict__ dt, double v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) q[i] = ql[i] + qt[i] * dt[i] * v;
+}
+
+// min_dt (explicit.cpp:555-564) over n values: per-block minima (a NaN never
+// wins, as std::min(min, dt[i]) keeps min), then the minimum of the blocks,
+// NaN if dt[0] is NaN (the reference starts from dt[0]), written to every
+// value.
+__global__ __launch_bounds__(kBlock) void k_array_min(const double* __restrict__ a, int64_t n,
+                                                      double* __restrict__ part) {
+    double v = kInf;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        v = smin(v, a[i]);
+    v = block_min(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_array_min_fill(double* __restrict__ a, int64_t n,
+                                                           const double* __restrict__ part, int n_part) {
+    __shared__ double r;
+    double v = kInf;
+    for (int i = threadIdx.x; i < n_part; i += kBlock) v = smin(v, part[i]);
+    v = block_min(v);
+    if (threadIdx.x == 0) r = (a[0] != a[0]) ? a[0] : v;
+    __syncthreads();
+    const double w = r;
+    // every block reads r before any value changes: one block fills all
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) a[i] = w;
+}
+
+// calc_residuals (explicit.cpp:481-552) from a given qt: per real tile the
+// sum of qt^2 A in a fixed order, then per instance the square root of the
+// sum over its tiles.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_res_tile(MeshView<T> m, TileMap tm, const Q4<T>* __restrict__ qt,
+                                                     double* __restrict__ blk) {
+    const int t = blockIdx.x;
+    if (t >= tm.n_rt) return;
+    const int c = tm.c0[t] + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c < tm.c0[t + 1]) {
+        const Q4<T> a = ld(qt + c);
+        const double A = (double)m.c_area[c];
+        for (int k = 0; k < 4; ++k) v[k] = (double)a.v[k] * (double)a.v[k] * A;
+    }
+    block_sum4(v);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) blk[4 * t + k] = v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_res_inst(InstView iv, const double* __restrict__ blk,
+                                                     double* __restrict__ out) {
+    const int i = blockIdx.x;
+    const int4 r = iv.tiles[i];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = r.x + threadIdx.x; t < r.y; t += kBlock)
+        for (int k = 0; k < 4; ++k) s[k] += blk[4 * t + k];
+    block_sum4(s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) out[4 * i + k] = sqrt(s[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,15 +957,15 @@ __global__ __launch_bounds__(kBlock) void k_ctrl_end(InstView iv, Ctrl* ctrl, co
 
 // Conserved sums: per real tile the FP64 partial sum_c A_c q_ck in a fixed
 // order, then one workgroup per instance adds its tiles' partials.
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_sums_tile(MeshView<T> m, TileMap tm, const Q4<T>* __restrict__ q,
+template <class T, class S>
+__global__ __launch_bounds__(kBlock) void k_sums_tile(MeshView<T> m, TileMap tm, const Q4<S>* __restrict__ q,
                                                       double* __restrict__ blk) {
     const int t = blockIdx.x;
     if (t >= tm.n_rt) return;
     const int c = tm.c0[t] + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (c < tm.c0[t + 1]) {
-        const Q4<T> qc = ld(q + c);
+        const Q4<S> qc = q[c];
         const double A = (double)m.c_area[c];
         for (int k = 0; k < 4; ++k) v[k] = A * (double)qc.v[k];
     }
@@ -842,16 +992,16 @@ __global__ __launch_bounds__(kBlock) void k_sums_inst(InstView iv, const double*
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, class S>
 void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_kind, const PhysParams& pp,
-               const Q4<T>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
+               const Q4<S>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
     const int nb = xcd_grid_blocks(tm.n_all);
     if (nb == 0) return;
-    if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
-    else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
-    else if (dt_kind == 3) hipLaunchKernelGGL((k_dt<T, 3>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
-    else if (dt_kind == 4) hipLaunchKernelGGL((k_dt<T, 4>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
-    else hipLaunchKernelGGL((k_dt<T, 2>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 3) hipLaunchKernelGGL((k_dt<T, 3, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else if (dt_kind == 4) hipLaunchKernelGGL((k_dt<T, 4, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+    else hipLaunchKernelGGL((k_dt<T, 2, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
 }
 
 void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells, int dt_is_f32,
@@ -865,13 +1015,13 @@ void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, con
                            (double*)dt_global, ctrl);
 }
 
-template <class T>
+template <class T, class S>
 void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, int phi_kind,
-                    bool grad, bool lim, bool lin, const Q4<T>* q, Q4<T>* face, Q4<T>* bincr, int dt_kind, T* dt,
+                    bool grad, bool lim, bool lin, const Q4<S>* q, Q4<T>* face, Q4<T>* bincr, int dt_kind, T* dt,
                     double* blk_min, Q4<T>* dbg_gx, Q4<T>* dbg_gy, Q4<T>* dbg_lim, const Ctrl* ctrl) {
     const int nb = xcd_grid_blocks(tm.n_all);
     if (nb == 0) return;
-#define FVH_GL(P, D) hipLaunchKernelGGL((k_gradlim<T, P, D>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, (int)grad, \
+#define FVH_GL(P, D) hipLaunchKernelGGL((k_gradlim<T, P, D, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, (int)grad, \
                                         (int)lim, (int)lin, q, face, bincr, dt, blk_min, dbg_gx, dbg_gy, dbg_lim, ctrl)
 #define FVH_GLD(P)                                  \
     do {                                            \
@@ -891,16 +1041,16 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
 #undef FVH_GL
 }
 
-template <class T>
+template <class T, class S>
 void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView* tiles, const PhysParams& pp,
-                 int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout,
+                 int flux_kind, bool lin, const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout,
                  const Q4<T>* face, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
                  Q4<T>* dbg_qt, const Ctrl* ctrl, Q4<T>* qt_only, const Q4<T>* gxc, const Q4<T>* gyc) {
     const int nb = xcd_grid_blocks(tm.n_rt);
     if (nb == 0) return;
     if (tiles) {
         const TileView tv = *tiles;
-#define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, tv, pp, bcs, \
+#define FVH_FT(J, L) hipLaunchKernelGGL((k_flux_tile<T, J, L, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, tv, pp, bcs, \
                                         qin, q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, \
                                         qt_only, gxc, gyc)
         if (flux_kind == 1) {
@@ -913,7 +1063,7 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
 #undef FVH_FT
         return;
     }
-#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qin, \
+#define FVH_FX(J, L) hipLaunchKernelGGL((k_flux_cell<T, J, L, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qin, \
                                         q0, qout, face, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl, qt_only, \
                                         gxc, gyc)
     if (flux_kind == 1) {
@@ -926,38 +1076,38 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
 #undef FVH_FX
 }
 
-template <class T>
+template <class T, class S>
 void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const PhysParams& pp, bool lin,
-                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<T>* q0,
-                   Q4<T>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
+                   const BcParam<T>* bcs, const Q4<T>* qt_nb, const Q4<T>* qt0, Q4<T>* qt_out, const Q4<S>* q0,
+                   Q4<S>* qout, const Q4<T>* bincr, const T* dt, const T* dt_inst, T alpha, double* res_blk,
                    Q4<T>* dbg_qt, const Ctrl* ctrl) {
     const int nb = xcd_grid_blocks(tm.n_rt);
     if (nb == 0) return;
-#define FVH_SM(L) hipLaunchKernelGGL((k_smooth<T, L>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qt_nb, qt0, \
+#define FVH_SM(L) hipLaunchKernelGGL((k_smooth<T, L, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, bcs, qt_nb, qt0, \
                                      qt_out, q0, qout, bincr, dt, dt_inst, alpha, res_blk, dbg_qt, ctrl)
     if (lin) FVH_SM(true); else FVH_SM(false);
 #undef FVH_SM
 }
 
-template <class T>
+template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv) {
-    return (size_t)(hv.max_ext + 2 * hv.max_edges) * sizeof(Q4<T>);
+    return stage_lds_states<S>(hv) + (size_t)(2 * hv.max_edges) * sizeof(Q4<T>);
 }
 
-template <class T, int PHI, int DT, bool J, int NB>
+template <class T, int PHI, int DT, bool J, int NB, class S>
 void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
-                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
-                     const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
+                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<S>* qin,
+                     const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
                      double* res_blk, const Ctrl* ctrl, size_t lds, const StageRecord<T>& rec) {
     static bool attr_set = false;
     if (!attr_set) {
-        if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J, NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J, NB, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 160 * 1024) != hipSuccess)
             (void)hipGetLastError();
         attr_set = true;
     }
     const int nb = xcd_grid_blocks(tm.n_rt);
-    hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
+    hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB, S>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
                        (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl, rec);
 }
 
@@ -968,31 +1118,31 @@ void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, con
 // of 512 lanes when a tile and its ring fit one round of the gradient
 // phase; larger halos run with 512 lanes, two workgroups per CU
 // (stage_lanes).
-template <class T, int PHI, int DT, bool J>
+template <class T, int PHI, int DT, bool J, class S>
 bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
-                    const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<T>* qin,
-                    const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
+                    const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<S>* qin,
+                    const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
                     double* res_blk, const Ctrl* ctrl, int threads, const StageRecord<T>& rec) {
-    const size_t lds = stage_lds_bytes<T>(hv);
+    const size_t lds = stage_lds_bytes<T, S>(hv);
     if (stage_lanes(lds, threads, hv.mean_g) == kStageBlock / 2)
-        launch_stage_nb<T, PHI, DT, J, kStageBlock / 2>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
-                                                        dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
+        launch_stage_nb<T, PHI, DT, J, kStageBlock / 2, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout,
+                                                           bincr, dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
     else
-        launch_stage_nb<T, PHI, DT, J, kStageBlock>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt,
-                                                    dt_inst, alpha, res_blk, ctrl, lds, rec);
+        launch_stage_nb<T, PHI, DT, J, kStageBlock, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
+                                                       dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
     return true;
 }
 
-template <class T>
+template <class T, class S>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
-                  const BcParam<T>* bcs, const Q4<T>* qin, const Q4<T>* q0, Q4<T>* qout, Q4<T>* bincr, T* dt,
+                  const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads,
                   const StageRecord<T>& rec) {
     if (tm.n_rt == 0) return true;
-    if (stage_lds_bytes<T>(hv) > 160 * 1024) return false;
-#define FVH_ST(P, D, J) launch_stage_k<T, P, D, J>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, dt, \
-                                                  dt_inst, alpha, res_blk, ctrl, threads, rec)
+    if (stage_lds_bytes<T, S>(hv) > 160 * 1024) return false;
+#define FVH_ST(P, D, J) launch_stage_k<T, P, D, J, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, \
+                                                     dt, dt_inst, alpha, res_blk, ctrl, threads, rec)
 #define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
 #define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
     switch (phi_kind) {
@@ -1006,12 +1156,62 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
 #undef FVH_ST
 }
 
-template <class T>
+template <class T, class S>
 void launch_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, bool lin,
-                        const BcParam<T>* bcs, Q4<T>* q) {
+                        const BcParam<T>* bcs, Q4<S>* q, const Q4<T>* gx, const Q4<T>* gy, const Q4<T>* lim) {
     const int nb = nblocks(m.n_real);
     if (nb == 0) return;
-    hipLaunchKernelGGL((k_bounds_init<T>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q);
+    hipLaunchKernelGGL((k_bounds_init<T, S>), dim3(nb), dim3(kBlock), 0, s, m, pp, (int)lin, bcs, q, gx, gy, lim);
+}
+
+template <class T>
+void launch_phase_limiters(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int phi_kind, const Q4<T>* q,
+                           const Q4<T>* gx, const Q4<T>* gy, Q4<T>* lim, Q4<T>* qmin, Q4<T>* qmax) {
+    const int nb = nblocks(m.n_total);
+    if (nb == 0) return;
+#define FVH_LM(P) hipLaunchKernelGGL((k_limiters<T, P>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, gx, gy, lim, qmin, qmax)
+    switch (phi_kind) {
+        case PHI_ZERO: FVH_LM(PHI_ZERO); break;
+        case PHI_MICHALAK: FVH_LM(PHI_MICHALAK); break;
+        case PHI_VENKAT: FVH_LM(PHI_VENKAT); break;
+        default: FVH_LM(PHI_MINMOD); break;
+    }
+#undef FVH_LM
+}
+
+template <class T>
+void launch_phase_faces(hipStream_t s, const MeshView<T>& m, const Q4<T>* q, const Q4<T>* gx, const Q4<T>* gy,
+                        const Q4<T>* lim, Q4<T>* face) {
+    const int nb = nblocks(m.n_total);
+    if (nb == 0) return;
+    hipLaunchKernelGGL((k_faces<T>), dim3(nb), dim3(kBlock), 0, s, m, q, gx, gy, lim, face);
+}
+
+void launch_update_cells(hipStream_t s, double* q, const double* ql, const double* qt, const double* dt, double v,
+                         int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_update_cells, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, q, ql, qt,
+                       dt, v, n);
+}
+
+int array_min_blocks(int64_t n) {
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    return (int)(nb < kArrayMinBlocks ? (nb < 1 ? 1 : nb) : kArrayMinBlocks);
+}
+
+void launch_array_min(hipStream_t s, double* a, int64_t n, double* part) {
+    if (n <= 0) return;
+    const int nb = array_min_blocks(n);
+    hipLaunchKernelGGL(k_array_min, dim3(nb), dim3(kBlock), 0, s, (const double*)a, n, part);
+    hipLaunchKernelGGL(k_array_min_fill, dim3(1), dim3(kBlock), 0, s, a, n, (const double*)part, nb);
+}
+
+template <class T>
+void launch_residuals(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* qt,
+                      double* blk, double* out) {
+    if (tm.n_rt == 0 || iv.n == 0) return;
+    hipLaunchKernelGGL((k_res_tile<T>), dim3(tm.n_rt), dim3(kBlock), 0, s, m, tm, qt, blk);
+    hipLaunchKernelGGL(k_res_inst, dim3(iv.n), dim3(kBlock), 0, s, iv, (const double*)blk, out);
 }
 
 void launch_ctrl_begin(hipStream_t s, int n_inst, Ctrl* ctrl, uint32_t max_step, double max_time, double tol,
@@ -1032,38 +1232,47 @@ void launch_ctrl_end(hipStream_t s, const InstView& iv, Ctrl* ctrl, const double
                            (const double*)dt_src, dt_is_inst);
 }
 
-template <class T>
-void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<T>* q,
+template <class T, class S>
+void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const InstView& iv, const Q4<S>* q,
                  double* blk, double* out) {
     if (tm.n_rt == 0 || iv.n == 0) return;
-    hipLaunchKernelGGL((k_sums_tile<T>), dim3(tm.n_rt), dim3(kBlock), 0, s, m, tm, q, blk);
+    hipLaunchKernelGGL((k_sums_tile<T, S>), dim3(tm.n_rt), dim3(kBlock), 0, s, m, tm, q, blk);
     hipLaunchKernelGGL(k_sums_inst, dim3(iv.n), dim3(kBlock), 0, s, iv, (const double*)blk, out);
 }
 
-#define FVH_INST(T)                                                                                          \
-    template void launch_dt<T>(hipStream_t, const MeshView<T>&, const TileMap&, int, const PhysParams&,        \
-                               const Q4<T>*, T*, double*, const Ctrl*);                                      \
-    template void launch_gradlim<T>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&, int,   \
-                                    bool, bool, bool, const Q4<T>*, Q4<T>*, Q4<T>*, int, T*, double*, Q4<T>*, \
-                                    Q4<T>*, Q4<T>*, const Ctrl*);                                            \
-    template void launch_flux<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView*,            \
-                                 const PhysParams&, int, bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*,  \
-                                 Q4<T>*, const Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,  \
-                                 const Ctrl*, Q4<T>*, const Q4<T>*, const Q4<T>*);                                                       \
-    template void launch_smooth<T>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&, bool, \
-                                   const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, const Q4<T>*,       \
-                                   Q4<T>*, const Q4<T>*, const T*, const T*, T, double*, Q4<T>*,             \
-                                   const Ctrl*);                                                             \
-    template void launch_bounds_init<T>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,            \
-                                        const BcParam<T>*, Q4<T>*);                                          \
-    template void launch_sums<T>(hipStream_t, const MeshView<T>&, const TileMap&, const InstView&,           \
-                                 const Q4<T>*, double*, double*);                                            \
-    template bool launch_stage<T>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,          \
-                                  const HaloView&, const PhysParams&, int, bool, bool, bool, int,            \
-                                  const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*, Q4<T>*, T*,        \
-                                  const T*, T, double*, const Ctrl*, int, const StageRecord<T>&);            \
-    template size_t stage_lds_bytes<T>(const HaloView&);
-FVH_INST(double)
-FVH_INST(float)
+#define FVH_INST(T, S)                                                                                       \
+    template void launch_dt<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, int, const PhysParams&,     \
+                                  const Q4<S>*, T*, double*, const Ctrl*);                                   \
+    template void launch_gradlim<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&,     \
+                                       int, bool, bool, bool, const Q4<S>*, Q4<T>*, Q4<T>*, int, T*, double*, \
+                                       Q4<T>*, Q4<T>*, Q4<T>*, const Ctrl*);                                 \
+    template void launch_flux<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView*,         \
+                                    const PhysParams&, int, bool, const BcParam<T>*, const Q4<S>*,            \
+                                    const Q4<S>*, Q4<S>*, const Q4<T>*, const Q4<T>*, const T*, const T*, T,  \
+                                    double*, Q4<T>*, const Ctrl*, Q4<T>*, const Q4<T>*, const Q4<T>*);        \
+    template void launch_smooth<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const PhysParams&,     \
+                                      bool, const BcParam<T>*, const Q4<T>*, const Q4<T>*, Q4<T>*,            \
+                                      const Q4<S>*, Q4<S>*, const Q4<T>*, const T*, const T*, T, double*,     \
+                                      Q4<T>*, const Ctrl*);                                                  \
+    template void launch_bounds_init<T, S>(hipStream_t, const MeshView<T>&, const PhysParams&, bool,         \
+                                           const BcParam<T>*, Q4<S>*, const Q4<T>*, const Q4<T>*,            \
+                                           const Q4<T>*);                                                    \
+    template void launch_sums<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const InstView&,        \
+                                    const Q4<S>*, double*, double*);                                         \
+    template bool launch_stage<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,       \
+                                     const HaloView&, const PhysParams&, int, bool, bool, bool, int,         \
+                                     const BcParam<T>*, const Q4<S>*, const Q4<S>*, Q4<S>*, Q4<T>*, T*,     \
+                                     const T*, T, double*, const Ctrl*, int, const StageRecord<T>&);         \
+    template size_t stage_lds_bytes<T, S>(const HaloView&);
+FVH_INST(double, double)
+FVH_INST(float, float)
+FVH_INST(double, float)
+template void launch_phase_limiters<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int,
+                                            const Q4<double>*, const Q4<double>*, const Q4<double>*, Q4<double>*,
+                                            Q4<double>*, Q4<double>*);
+template void launch_phase_faces<double>(hipStream_t, const MeshView<double>&, const Q4<double>*, const Q4<double>*,
+                                         const Q4<double>*, const Q4<double>*, Q4<double>*);
+template void launch_residuals<double>(hipStream_t, const MeshView<double>&, const TileMap&, const InstView&,
+                                       const Q4<double>*, double*, double*);
 
 }  // namespace fvh

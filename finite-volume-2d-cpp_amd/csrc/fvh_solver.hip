@@ -44,6 +44,12 @@ enum { K_DT = 0, K_DTMIN, K_GRADLIM, K_FLUX, K_BEGIN, K_END, K_BINIT, K_STAGE, K
         }                                                                                   \
     } while (0)
 
+// Calls F<T, S>(args...) for the solver's arithmetic type T and state
+// storage type S.
+#define FVH_DISPATCH(s, F, ...)                                                     \
+    ((s)->mixed ? F<double, float>(__VA_ARGS__)                                     \
+                : (s)->f32 ? F<float, float>(__VA_ARGS__) : F<double, double>(__VA_ARGS__))
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -78,7 +84,8 @@ struct fvh_solver {
     std::vector<fvh_config> icfg;         // per instance (boundary tables)
     int device = 0;
     hipStream_t stream = nullptr;
-    bool f32 = false;
+    bool f32 = false;                     // FP32 arithmetic, geometry and states
+    bool mixed = false;                   // FP32 states, FP64 arithmetic and geometry
     bool second = false;                  // gradient/limiter/reconstruction pass present
     int32_t n_inst = 0;
     int32_t n_real = 0, n_total = 0, n_edges = 0, n_he = 0, n_bnd = 0;
@@ -101,6 +108,7 @@ struct fvh_solver {
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf dbg[4];
+    fvh::DevBuf ph[6], ph_part;              // per-phase entry points: inputs and outputs, min_dt partials
     fvh::DevBuf qt_s[2];                     // residual smoothing: raw qt, first sweep's qt
     fvh::DevBuf gxy[2];                      // viscous flux: cell gradients gx, gy
     int cur = 0;                          // index of the current state buffer
@@ -309,7 +317,7 @@ struct HostGeo {
     std::vector<double> c_area, c_cx, c_cy;                    // per device cell
 };
 
-template <class T>
+template <class T, class S>
 int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
     const int32_t E = s->n_edges, N = s->n_total;
     std::vector<Q4<T>> geo(E);
@@ -342,7 +350,7 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
     if ((rc = upload(s->c_sqrt_area, sa.data(), N * sizeof(T)))) return rc;
     if ((rc = upload(s->c_c, cc.data(), N * sizeof(G2<T>)))) return rc;
     for (int k = 0; k < 3; ++k)
-        if (s->q[k].alloc((size_t)N * sizeof(Q4<T>))) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+        if (s->q[k].alloc((size_t)N * sizeof(Q4<S>))) { set_error("hipMalloc failed"); return FVH_E_HIP; }
     if (s->dt.alloc((size_t)N * sizeof(T)) || s->dt_global.alloc((size_t)s->n_inst * 8) ||
         s->face.alloc((size_t)(s->second ? s->n_he : 1) * sizeof(Q4<T>)) ||
         s->bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<T>)) ||
@@ -377,7 +385,7 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
 // buffer s->cur.  run_mode adds the stop test and bookkeeping kernels.
 // dbg (tests): 4 device buffers for gx, gy, limiter and qt of one stage run
 // with alpha_override, leaving the current state buffer unchanged.
-template <class T>
+template <class T, class S>
 int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res,
                      Q4<T>* const* dbg = nullptr, const double* alpha_override = nullptr) {
     const fvh_config& c = s->cfg;
@@ -397,9 +405,9 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         Bracket b(s, K_BEGIN);
         launch_ctrl_begin(st, s->n_inst, ctrl, o->max_step, o->max_time, o->tolerance, o->stop_var);
     }
-    Q4<T>* q0 = (Q4<T>*)s->q[s->cur].p;
-    Q4<T>* qa = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
-    Q4<T>* qb = (Q4<T>*)s->q[(s->cur + 2) % 3].p;
+    Q4<S>* q0 = (Q4<S>*)s->q[s->cur].p;
+    Q4<S>* qa = (Q4<S>*)s->q[(s->cur + 1) % 3].p;
+    Q4<S>* qb = (Q4<S>*)s->q[(s->cur + 2) % 3].p;
     T* dt = (T*)s->dt.p;
     T* dtg = (T*)s->dt_global.p;
     double* blk = (double*)s->blk_min.p;
@@ -414,8 +422,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         Bracket b(s, K_DT);
         launch_dt<T>(st, m, tm, c.dt_kind, pp, q0, dt, blk, cc);
     }
-    const Q4<T>* qin = q0;
-    Q4<T>* out = qa;
+    const Q4<S>* qin = q0;
+    Q4<S>* out = qa;
     const int n_stage = alpha_override ? 1 : c.n_alpha;
     if (fused) {
         const HaloView hv = halo_view(s);
@@ -495,9 +503,9 @@ bool same_options(const fvh_run_options& a, const fvh_run_options& b) {
 // One step, replayed from a captured graph when graphs are on (a graph per
 // state buffer, residual flag and run options; kernels take their buffers
 // and options by value, so a replay is the same launch sequence).
-template <class T>
+template <class T, class S>
 int32_t step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res) {
-    if (!s->use_graphs || s->profiling) return enqueue_step<T>(s, run_mode, o, need_res);
+    if (!s->use_graphs || s->profiling) return enqueue_step<T, S>(s, run_mode, o, need_res);
     fvh_run_options key{};
     if (run_mode) key = *o;
     for (auto& g : s->graphs) {
@@ -514,7 +522,7 @@ int32_t step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_re
     g.run_mode = run_mode;
     g.o = key;
     FVH_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-    int32_t rc = enqueue_step<T>(s, run_mode, o, need_res);
+    int32_t rc = enqueue_step<T, S>(s, run_mode, o, need_res);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(s->stream, &graph);
     if (rc) {
@@ -535,16 +543,16 @@ int32_t step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_re
     return FVH_OK;
 }
 
-template <class T>
+template <class T, class S>
 int32_t set_state_t(fvh_solver* s, const double* q) {
-    std::vector<Q4<T>> h(s->n_total);
+    std::vector<Q4<S>> h(s->n_total);
     for (int32_t d = 0; d < s->n_total; ++d) {
         const int64_t r = s->inv[d];
-        for (int k = 0; k < 4; ++k) h[d].v[k] = T(q[4 * r + k]);
+        for (int k = 0; k < 4; ++k) h[d].v[k] = S(q[4 * r + k]);
     }
     FVH_HIP(hipStreamSynchronize(s->stream));
     for (int b = 0; b < 3; ++b)
-        FVH_HIP(hipMemcpy(s->q[b].p, h.data(), h.size() * sizeof(Q4<T>), hipMemcpyHostToDevice));
+        FVH_HIP(hipMemcpy(s->q[b].p, h.data(), h.size() * sizeof(Q4<S>), hipMemcpyHostToDevice));
     return FVH_OK;
 }
 
@@ -560,7 +568,7 @@ int32_t get_q4(fvh_solver* s, const void* dev, double* out, int32_t count) {
     return FVH_OK;
 }
 
-template <class T>
+template <class T, class S>
 int32_t phases_t(fvh_solver* s, double alpha, double* dt_out, double* gx, double* gy, double* lim,
                  double* qt, double* qk) {
     const bool gdt = s->cfg.flags & FVH_FLAG_GLOBAL_DT;
@@ -569,12 +577,12 @@ int32_t phases_t(fvh_solver* s, double alpha, double* dt_out, double* gx, double
         if (!s->dbg[k].p && s->dbg[k].alloc(qbytes)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
         FVH_HIP(hipMemsetAsync(s->dbg[k].p, 0, qbytes, s->stream));
     }
-    Q4<T>* q0 = (Q4<T>*)s->q[s->cur].p;
-    Q4<T>* out = (Q4<T>*)s->q[(s->cur + 1) % 3].p;
+    Q4<S>* q0 = (Q4<S>*)s->q[s->cur].p;
+    Q4<S>* out = (Q4<S>*)s->q[(s->cur + 1) % 3].p;
     // update_cells runs over every cell; ghost cells keep q (their qt is 0)
-    FVH_HIP(hipMemcpyAsync(out, q0, qbytes, hipMemcpyDeviceToDevice, s->stream));
+    FVH_HIP(hipMemcpyAsync(out, q0, (size_t)s->n_total * sizeof(Q4<S>), hipMemcpyDeviceToDevice, s->stream));
     Q4<T>* dbg[4] = {(Q4<T>*)s->dbg[0].p, (Q4<T>*)s->dbg[1].p, (Q4<T>*)s->dbg[2].p, (Q4<T>*)s->dbg[3].p};
-    int32_t rc = enqueue_step<T>(s, false, nullptr, 0, dbg, &alpha);
+    int32_t rc = enqueue_step<T, S>(s, false, nullptr, 0, dbg, &alpha);
     if (rc) return rc;
     FVH_HIP(hipStreamSynchronize(s->stream));
     if (dt_out) {
@@ -588,17 +596,17 @@ int32_t phases_t(fvh_solver* s, double alpha, double* dt_out, double* gx, double
     if (gy && (rc = get_q4<T>(s, dbg[1], gy, s->n_total))) return rc;
     if (lim && (rc = get_q4<T>(s, dbg[2], lim, s->n_total))) return rc;
     if (qt && (rc = get_q4<T>(s, dbg[3], qt, s->n_total))) return rc;
-    if (qk && (rc = get_q4<T>(s, out, qk, s->n_total))) return rc;
+    if (qk && (rc = get_q4<S>(s, out, qk, s->n_total))) return rc;
     return FVH_OK;
 }
 
-template <class T>
+template <class T, class S>
 int32_t apply_bounds_t(fvh_solver* s) {
     const fvh_config& c = s->cfg;
     {
         Bracket b(s, K_BINIT);
         launch_bounds_init<T>(s->stream, view<T>(s), phys(c), (c.flags & FVH_FLAG_LINEAR) != 0,
-                              (const BcParam<T>*)s->bcs.p, (Q4<T>*)s->q[s->cur].p);
+                              (const BcParam<T>*)s->bcs.p, (Q4<S>*)s->q[s->cur].p);
     }
     FVH_HIP(hipGetLastError());
     std::vector<Ctrl> z(s->n_inst);
@@ -642,7 +650,7 @@ int32_t read_status(fvh_solver* s, fvh_status* st) {
 // Executes up to n more steps of run() from the current status (steps are
 // queued in growing chunks; kernels queued after an instance's stop are
 // no-ops for it).  Afterwards every instance's state is in buffer s->cur.
-template <class T>
+template <class T, class S>
 int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_status* st) {
     int32_t rc;
     std::vector<Ctrl> before, after;
@@ -660,7 +668,7 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
         const uint64_t k_n = std::min<uint64_t>(chunk, left);
         for (uint64_t k = 0; k < k_n; ++k, ++step) {
             const int need = (step == 0) || (step % print == 0) || every;
-            if ((rc = fvh::step<T>(s, true, o, need))) return rc;
+            if ((rc = fvh::step<T, S>(s, true, o, need))) return rc;
         }
         left -= k_n;
         if ((rc = read_status(s, &now))) return rc;
@@ -675,7 +683,7 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
         const uint64_t done = after[i].step - before[i].step;
         const int b = (int)((cur0 + done * delta) % 3);
         if (b == s->cur) continue;
-        const size_t qs = sizeof(Q4<T>);
+        const size_t qs = sizeof(Q4<S>);
         char* dst = (char*)s->q[s->cur].p;
         const char* src = (const char*)s->q[b].p;
         FVH_HIP(hipMemcpyAsync(dst + (size_t)s->i_real0[i] * qs, src + (size_t)s->i_real0[i] * qs,
@@ -689,11 +697,11 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
     return drain_events(s);
 }
 
-template <class T>
+template <class T, class S>
 int32_t run_t(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
-    int32_t rc = apply_bounds_t<T>(s);
+    int32_t rc = apply_bounds_t<T, S>(s);
     if (rc) return rc;
-    return run_continue_t<T>(s, o, ~0ull, st);
+    return run_continue_t<T, S>(s, o, ~0ull, st);
 }
 
 // Tile edge lists for the tiled flux pass.  Returns false when a tile touches
@@ -822,12 +830,24 @@ bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::v
     return true;
 }
 
+// LDS bytes of the fused stage pass for a tile halo (stage_lds_bytes).
+size_t halo_lds_bytes(const fvh_solver* s, const HaloHost& hh) {
+    HaloView hv{};
+    hv.max_ext = hh.max_ext;
+    hv.max_edges = hh.max_edges;
+    if (s->mixed) return stage_lds_bytes<double, float>(hv);
+    return s->f32 ? stage_lds_bytes<float, float>(hv) : stage_lds_bytes<double, double>(hv);
+}
+
 int32_t check_config(const fvh_config* c) {
     if (c->n_alpha < 1 || c->n_alpha > FVH_MAX_ALPHA || c->n_bc < 0 || c->n_bc > FVH_MAX_BC) {
         set_error("config out of range (alpha or bc count)");
         return FVH_E_ARG;
     }
-    if (c->precision != FVH_F64 && c->precision != FVH_F32) { set_error("unknown precision"); return FVH_E_ARG; }
+    if (c->precision != FVH_F64 && c->precision != FVH_F32 && c->precision != FVH_F32_F64) {
+        set_error("unknown precision");
+        return FVH_E_ARG;
+    }
     if (c->flux < 0 || c->flux > 2 || c->phi < 0 || c->phi > 3 || c->dt_kind < 0 || c->dt_kind > 4 ||
         c->renumber < 0 || c->renumber > 2 || c->flux_mode < 0 || c->flux_mode > 3) {
         set_error("unknown flux, limiter, time-step, order or flux-mode kind");
@@ -883,6 +903,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     s->icfg.assign(cs, cs + n);
     s->n_inst = n;
     s->f32 = c.precision == FVH_F32;
+    s->mixed = c.precision == FVH_F32_F64;
     s->second = (c.flags & (FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR)) != 0;
     // physical name -> BC slot of each instance (mesh.cpp:662-667)
     std::vector<std::vector<int32_t>> slot_of_name(n);
@@ -1108,9 +1129,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     const bool fusable = !(c.flags & FVH_FLAG_SMOOTH) && c.flux != FVH_FLUX_ROE_VISCOUS && c.dt_kind <= FVH_DT_EDGE_MIN;
     if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
         lin_on && fusable && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
-        const size_t qb = s->f32 ? 16 : 32;
-        const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * qb;
-        s->fused = lds <= 160 * 1024;
+        s->fused = halo_lds_bytes(s.get(), hh) <= 160 * 1024;
     }
     // Automatic tile size (tile_cells 0): tiles of 256 cells first.  When an
     // average tile with its real ring-1 cells outnumbers 256 lanes, the
@@ -1122,7 +1141,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     // 12% faster; the 2 M-cell airfoil: 208 cells, 256 lanes, 5% faster).
     if (c.tile_cells == 0 && s->fused && c.renumber == FVH_ORDER_TILES && s->n_real > 0 && s->n_rt > 0 &&
         c.stage_threads != kStageBlock) {
-        const size_t lds = (size_t)(hh.max_ext + 2 * hh.max_edges) * (s->f32 ? 16 : 32);
+        const size_t lds = halo_lds_bytes(s.get(), hh);
         const int32_t mean_g = (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt);
         const int half = kStageBlock / 2;
         const int lanes = stage_lanes(lds, c.stage_threads, mean_g);
@@ -1173,10 +1192,10 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         if ((rc = upload(s->t_he, t_he.data(), t_he.size() * 4))) return rc;
         if ((rc = upload(s->t_cell, t_cell.data(), t_cell.size() * 4))) return rc;
     }
-    rc = s->f32 ? upload_geometry<float>(s.get(), hg) : upload_geometry<double>(s.get(), hg);
+    rc = FVH_DISPATCH(s, upload_geometry, s.get(), hg);
     if (rc) return rc;
     // algorithmic bytes per stage (state s, geometry g bytes)
-    const double sb = s->f32 ? 4.0 : 8.0, gb = sb, V = 4.0;
+    const double sb = (s->f32 || s->mixed) ? 4.0 : 8.0, gb = s->f32 ? 4.0 : 8.0, V = 4.0;
     const double C = s->n_real, B = s->n_bnd;
     double bytes = C * (3 * V * sb + sb + gb) + E * (3 * gb + 8) + B * (2 * V * sb + 2 * gb + 4);
     if (s->second) bytes += C * (7 * V * sb + 2 * gb) + E * (2 * gb);
@@ -1185,9 +1204,9 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     return FVH_OK;
 }
 
-template <class T>
+template <class T, class S>
 int32_t sums_t(fvh_solver* s, double* out) {
-    launch_sums<T>(s->stream, view<T>(s), tile_map(s), inst_view(s), (const Q4<T>*)s->q[s->cur].p,
+    launch_sums<T>(s->stream, view<T>(s), tile_map(s), inst_view(s), (const Q4<S>*)s->q[s->cur].p,
                    (double*)s->sums_blk.p, (double*)s->sums.p);
     FVH_HIP(hipGetLastError());
     FVH_HIP(hipStreamSynchronize(s->stream));
@@ -1323,7 +1342,7 @@ struct Adjoint {
         launch_copy<double>(s->stream, src, s->q[s->cur].p, s->n_total);
         fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
         for (int i = 0; i < k; ++i) {
-            int32_t rc = fvh::step<double>(s, false, &o, 0);
+            int32_t rc = fvh::step<double, double>(s, false, &o, 0);
             if (rc) return rc;
         }
         fwd_steps += k;
@@ -1581,7 +1600,7 @@ int32_t node_gradient(fvh_solver* s, const double* wx, const double* wy, double*
 }
 
 int32_t adjoint_check(fvh_solver* s) {
-    if (s->f32) { set_error("the adjoint is FP64 only"); return FVH_E_UNSUPPORTED; }
+    if (s->f32 || s->mixed) { set_error("the adjoint is FP64 only"); return FVH_E_UNSUPPORTED; }
     if (s->n_inst != 1) { set_error("the adjoint takes a single-mesh solver"); return FVH_E_UNSUPPORTED; }
     if (s->max_real_deg > kMaxHe) {
         set_error("the adjoint takes triangles and quadrilaterals (cells of at most " + std::to_string(kMaxHe) +
@@ -1608,6 +1627,49 @@ int32_t adjoint_check(fvh_solver* s) {
         }
     return FVH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// The phases one at a time on caller arrays (FP64, single instance)
+// ---------------------------------------------------------------------------
+int32_t phase_setup(fvh_solver* s) {
+    if (s->f32 || s->mixed) { set_error("the per-phase functions take FP64 solvers"); return FVH_E_UNSUPPORTED; }
+    if (s->n_inst != 1) { set_error("the per-phase functions take single-instance solvers"); return FVH_E_UNSUPPORTED; }
+    const size_t qb = (size_t)s->n_total * sizeof(Q4<double>);
+    for (auto& b : s->ph)
+        if (!b.p && b.alloc(qb)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+    if (!s->ph_part.p && s->ph_part.alloc((size_t)kArrayMinBlocks * 8)) {
+        set_error("hipMalloc failed");
+        return FVH_E_HIP;
+    }
+    return FVH_OK;
+}
+
+// host array in the reference layout (4 per cell, all cells) -> device order
+int32_t put_q4(fvh_solver* s, const double* host, void* dev) {
+    std::vector<Q4<double>> h(s->n_total);
+    for (int32_t d = 0; d < s->n_total; ++d) {
+        const int64_t r = s->inv[d];
+        for (int k = 0; k < 4; ++k) h[d].v[k] = host[4 * r + k];
+    }
+    FVH_HIP(hipMemcpyAsync(dev, h.data(), h.size() * sizeof(Q4<double>), hipMemcpyHostToDevice, s->stream));
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    return FVH_OK;
+}
+
+int32_t put_flat(fvh_solver* s, const double* host, void* dev, int64_t n) {
+    FVH_HIP(hipMemcpyAsync(dev, host, (size_t)n * 8, hipMemcpyHostToDevice, s->stream));
+    return FVH_OK;
+}
+
+int32_t get_flat(fvh_solver* s, const void* dev, double* host, int64_t n) {
+    FVH_HIP(hipMemcpyAsync(host, dev, (size_t)n * 8, hipMemcpyDeviceToHost, s->stream));
+    FVH_HIP(hipStreamSynchronize(s->stream));
+    return FVH_OK;
+}
+
+#define FVH_TRY(x) do { int32_t rc_ = (x); if (rc_) return rc_; } while (0)
+
+Q4<double>* ph_q4(fvh_solver* s, int k) { return (Q4<double>*)s->ph[k].p; }
 
 }  // namespace
 }  // namespace fvh
@@ -1699,7 +1761,7 @@ int32_t fvh_batch_status(fvh_solver* s, int32_t k, fvh_status* st) {
 
 int32_t fvh_batch_sums(fvh_solver* s, double* out) {
     if (!s || !out) { set_error("null argument"); return FVH_E_ARG; }
-    return s->f32 ? sums_t<float>(s, out) : sums_t<double>(s, out);
+    return FVH_DISPATCH(s, sums_t, s, out);
 }
 
 int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config* configs, const double* q,
@@ -1726,11 +1788,11 @@ int32_t fvh_autotune(int32_t n, const fvh_mesh* const* meshes, const fvh_config*
             double ms = -1.0;
             if (create(n, meshes, cs.data(), &s) == FVH_OK) {
                 fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
-                rc = s->f32 ? set_state_t<float>(s, q) : set_state_t<double>(s, q);
-                if (!rc) rc = s->f32 ? apply_bounds_t<float>(s) : apply_bounds_t<double>(s);
-                for (int k = 0; k < 2 && !rc; ++k) rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
+                rc = FVH_DISPATCH(s, set_state_t, s, q);
+                if (!rc) rc = FVH_DISPATCH(s, apply_bounds_t, s);
+                for (int k = 0; k < 2 && !rc; ++k) rc = FVH_DISPATCH(s, fvh::step, s, false, &o, 0);
                 if (!rc && hipEventRecord(e0, s->stream) != hipSuccess) rc = FVH_E_HIP;
-                for (int k = 0; k < steps && !rc; ++k) rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
+                for (int k = 0; k < steps && !rc; ++k) rc = FVH_DISPATCH(s, fvh::step, s, false, &o, 0);
                 if (!rc && hipEventRecord(e1, s->stream) != hipSuccess) rc = FVH_E_HIP;
                 if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = FVH_E_HIP;
                 float t = 0.f;
@@ -1758,10 +1820,10 @@ int32_t fvh_objective(fvh_solver* s, const double* q, int32_t n_steps, const dou
     if (rc) return rc;
     if ((rc = objective_setup(s, wx, wy))) return rc;
     s->cur = 0;
-    if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
+    if ((rc = set_state_t<double, double>(s, q)) || (rc = apply_bounds_t<double, double>(s))) return rc;
     fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
     for (int32_t k = 0; k < n_steps; ++k)
-        if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
+        if ((rc = fvh::step<double, double>(s, false, &o, 0))) return rc;
     return objective_eval(s, s->q[s->cur].p, false, J);
 }
 
@@ -1794,7 +1856,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
     if (grad_nodes && (rc = A.alloc_geo())) return rc;
     // initial state: bounds applied (explicit.cpp:712), kept in slot 0
     s->cur = 0;
-    if ((rc = set_state_t<double>(s, q)) || (rc = apply_bounds_t<double>(s))) return rc;
+    if ((rc = set_state_t<double, double>(s, q)) || (rc = apply_bounds_t<double, double>(s))) return rc;
     launch_copy<double>(s->stream, s->q[s->cur].p, s->a_ck[0]->p, s->n_total);
     // forward to the objective, then seed the adjoint of the final state.
     // With a slot per step the sweep keeps every step's state and each step
@@ -1806,7 +1868,7 @@ int32_t fvh_adjoint(fvh_solver* s, const double* q, int32_t n_steps, const doubl
         // step in the solver's own buffers, keeping a copy of each step's start
         fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
         for (int k = 0; k < i0; ++k) {
-            if ((rc = fvh::step<double>(s, false, &o, 0))) return rc;
+            if ((rc = fvh::step<double, double>(s, false, &o, 0))) return rc;
             void* dst = k + 1 < i0 ? s->a_ck[k + 1]->p : (n_rec ? A.tapes[0].qs[0] : s->a_tmp.p);
             launch_copy<double>(s->stream, s->q[s->cur].p, dst, s->n_total);
         }
@@ -1874,12 +1936,13 @@ void fvh_solver_free(fvh_solver* s) { delete s; }
 int32_t fvh_solver_set_state(fvh_solver* s, const double* q) {
     if (!s || !q) { set_error("null argument"); return FVH_E_ARG; }
     s->cur = 0;
-    return s->f32 ? set_state_t<float>(s, q) : set_state_t<double>(s, q);
+    return FVH_DISPATCH(s, set_state_t, s, q);
 }
 
 int32_t fvh_solver_get_state(fvh_solver* s, double* q) {
     if (!s || !q) { set_error("null argument"); return FVH_E_ARG; }
-    return s->f32 ? get_q4<float>(s, s->q[s->cur].p, q, s->n_total) : get_q4<double>(s, s->q[s->cur].p, q, s->n_total);
+    return (s->f32 || s->mixed) ? get_q4<float>(s, s->q[s->cur].p, q, s->n_total)
+                                : get_q4<double>(s, s->q[s->cur].p, q, s->n_total);
 }
 
 // Restart file, little-endian: "FVHSTAT1", int32 {version 1, precision,
@@ -1901,7 +1964,7 @@ int32_t fvh_solver_save(fvh_solver* s, const char* path) {
     if ((rc = read_ctrl(s, h))) return rc;
     std::FILE* f = std::fopen(path, "wb");
     if (!f) { set_error(std::string("cannot open ") + path + " for writing"); return FVH_E_IO; }
-    const int32_t head[4] = {1, s->f32 ? FVH_F32 : FVH_F64, s->n_inst, 0};
+    const int32_t head[4] = {1, s->cfg.precision, s->n_inst, 0};
     bool ok = std::fwrite(kRestartMagic, 1, 8, f) == 8 && std::fwrite(head, 4, 4, f) == 4 &&
               std::fwrite(&nv, 8, 1, f) == 1 && std::fwrite(h.data(), sizeof(Ctrl), h.size(), f) == h.size() &&
               std::fwrite(q.data(), 8, q.size(), f) == q.size();
@@ -1945,15 +2008,130 @@ int32_t fvh_solver_load(fvh_solver* s, const char* path) {
 
 int32_t fvh_solver_apply_bounds(fvh_solver* s) {
     if (!s) { set_error("null argument"); return FVH_E_ARG; }
-    return s->f32 ? apply_bounds_t<float>(s) : apply_bounds_t<double>(s);
+    return FVH_DISPATCH(s, apply_bounds_t, s);
 }
 
 int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, double* gy, double* lim,
                           double* qt, double* qk) {
     if (!s) { set_error("null argument"); return FVH_E_ARG; }
     if (s->n_inst != 1) { set_error("phases: single-instance solvers only"); return FVH_E_UNSUPPORTED; }
-    return s->f32 ? phases_t<float>(s, alpha, dt, gx, gy, lim, qt, qk)
-                  : phases_t<double>(s, alpha, dt, gx, gy, lim, qt, qk);
+    return FVH_DISPATCH(s, phases_t, s, alpha, dt, gx, gy, lim, qt, qk);
+}
+
+int32_t fvh_phase_gradients(fvh_solver* s, const double* q, double* gx, double* gy) {
+    if (!s || !q || !gx || !gy) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    FVH_TRY(put_q4(s, q, s->ph[0].p));
+    launch_gradlim<double>(s->stream, view<double>(s), tile_map(s), phys(s->cfg), s->cfg.phi, true, false, false,
+                           ph_q4(s, 0), (Q4<double>*)s->face.p, (Q4<double>*)s->bincr.p, -1, (double*)s->dt.p,
+                           (double*)s->blk_min.p, ph_q4(s, 1), ph_q4(s, 2), nullptr, nullptr);
+    FVH_HIP(hipGetLastError());
+    FVH_TRY(get_q4<double>(s, s->ph[1].p, gx, s->n_total));
+    return get_q4<double>(s, s->ph[2].p, gy, s->n_total);
+}
+
+int32_t fvh_phase_limiters(fvh_solver* s, const double* q, const double* gx, const double* gy, double* lim,
+                           double* qmin, double* qmax) {
+    if (!s || !q || !gx || !gy || !lim) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    FVH_TRY(put_q4(s, q, s->ph[0].p));
+    FVH_TRY(put_q4(s, gx, s->ph[1].p));
+    FVH_TRY(put_q4(s, gy, s->ph[2].p));
+    launch_phase_limiters<double>(s->stream, view<double>(s), phys(s->cfg), s->cfg.phi, ph_q4(s, 0), ph_q4(s, 1),
+                                  ph_q4(s, 2), ph_q4(s, 3), qmin ? ph_q4(s, 4) : nullptr,
+                                  qmax ? ph_q4(s, 5) : nullptr);
+    FVH_HIP(hipGetLastError());
+    FVH_TRY(get_q4<double>(s, s->ph[3].p, lim, s->n_total));
+    if (qmin) FVH_TRY(get_q4<double>(s, s->ph[4].p, qmin, s->n_total));
+    if (qmax) FVH_TRY(get_q4<double>(s, s->ph[5].p, qmax, s->n_total));
+    return FVH_OK;
+}
+
+int32_t fvh_phase_time_derivatives(fvh_solver* s, const double* q, const double* gx, const double* gy,
+                                   const double* lim, double* qt) {
+    if (!s || !q || !gx || !gy || !lim || !qt) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    const fvh_config& c = s->cfg;
+    const bool lin = c.flags & FVH_FLAG_LINEAR;
+    if (lin && !s->second) { set_error("internal: no face buffer"); return FVH_E_UNSUPPORTED; }
+    FVH_TRY(put_q4(s, q, s->ph[0].p));
+    FVH_TRY(put_q4(s, gx, s->ph[1].p));
+    FVH_TRY(put_q4(s, gy, s->ph[2].p));
+    FVH_TRY(put_q4(s, lim, s->ph[3].p));
+    const MeshView<double> m = view<double>(s);
+    if (lin) launch_phase_faces<double>(s->stream, m, ph_q4(s, 0), ph_q4(s, 1), ph_q4(s, 2), ph_q4(s, 3),
+                                        (Q4<double>*)s->face.p);
+    // ghost cells keep qt = 0 (explicit.cpp:352-362)
+    FVH_HIP(hipMemsetAsync(s->ph[4].p, 0, s->ph[4].bytes, s->stream));
+    const TileView tv = tile_view(s);
+    launch_flux<double>(s->stream, m, tile_map(s), s->use_tiles ? &tv : nullptr, phys(c), c.flux, lin,
+                        (const BcParam<double>*)s->bcs.p, ph_q4(s, 0), ph_q4(s, 0), ph_q4(s, 5),
+                        (const Q4<double>*)s->face.p, (const Q4<double>*)s->bincr.p, (const double*)s->dt.p, nullptr,
+                        0.0, nullptr, nullptr, nullptr, ph_q4(s, 4), ph_q4(s, 1), ph_q4(s, 2));
+    FVH_HIP(hipGetLastError());
+    return get_q4<double>(s, s->ph[4].p, qt, s->n_total);
+}
+
+int32_t fvh_phase_update_cells(fvh_solver* s, double* q, const double* ql, const double* qt, const double* dt,
+                               double v, int64_t n) {
+    if (!q || !ql || !qt || !dt || n < 0) { set_error("bad argument"); return FVH_E_ARG; }
+    if (!s) {
+        // no mesh: temporary device buffers on the current device's null stream
+        DevBuf b[4];
+        for (auto& x : b)
+            if (x.alloc((size_t)n * 8)) { set_error("hipMalloc failed"); return FVH_E_HIP; }
+        FVH_HIP(hipMemcpy(b[1].p, ql, (size_t)n * 8, hipMemcpyHostToDevice));
+        FVH_HIP(hipMemcpy(b[2].p, qt, (size_t)n * 8, hipMemcpyHostToDevice));
+        FVH_HIP(hipMemcpy(b[3].p, dt, (size_t)n * 8, hipMemcpyHostToDevice));
+        launch_update_cells(nullptr, (double*)b[0].p, (const double*)b[1].p, (const double*)b[2].p,
+                            (const double*)b[3].p, v, n);
+        FVH_HIP(hipGetLastError());
+        FVH_HIP(hipMemcpy(q, b[0].p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        return FVH_OK;
+    }
+    FVH_TRY(phase_setup(s));
+    if (n > 4 * (int64_t)s->n_total) { set_error("update_cells: more values than 4 per cell"); return FVH_E_ARG; }
+    FVH_TRY(put_flat(s, ql, s->ph[1].p, n));
+    FVH_TRY(put_flat(s, qt, s->ph[2].p, n));
+    FVH_TRY(put_flat(s, dt, s->ph[3].p, n));
+    launch_update_cells(s->stream, (double*)s->ph[0].p, (const double*)s->ph[1].p, (const double*)s->ph[2].p,
+                        (const double*)s->ph[3].p, v, n);
+    FVH_HIP(hipGetLastError());
+    return get_flat(s, s->ph[0].p, q, n);
+}
+
+int32_t fvh_phase_update_bounds(fvh_solver* s, double* q, const double* gx, const double* gy, const double* lim) {
+    if (!s || !q || !gx || !gy || !lim) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    FVH_TRY(put_q4(s, q, s->ph[0].p));
+    FVH_TRY(put_q4(s, gx, s->ph[1].p));
+    FVH_TRY(put_q4(s, gy, s->ph[2].p));
+    FVH_TRY(put_q4(s, lim, s->ph[3].p));
+    launch_bounds_init<double>(s->stream, view<double>(s), phys(s->cfg), (s->cfg.flags & FVH_FLAG_LINEAR) != 0,
+                               (const BcParam<double>*)s->bcs.p, ph_q4(s, 0), ph_q4(s, 1), ph_q4(s, 2),
+                               ph_q4(s, 3));
+    FVH_HIP(hipGetLastError());
+    return get_q4<double>(s, s->ph[0].p, q, s->n_total);
+}
+
+int32_t fvh_phase_min_dt(fvh_solver* s, double* dt, int64_t n) {
+    if (!s || !dt || n < 1) { set_error("bad argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    if (n > 4 * (int64_t)s->n_total) { set_error("min_dt: more values than 4 per cell"); return FVH_E_ARG; }
+    FVH_TRY(put_flat(s, dt, s->ph[0].p, n));
+    launch_array_min(s->stream, (double*)s->ph[0].p, n, (double*)s->ph_part.p);
+    FVH_HIP(hipGetLastError());
+    return get_flat(s, s->ph[0].p, dt, n);
+}
+
+int32_t fvh_phase_residuals(fvh_solver* s, const double* qt, double* R) {
+    if (!s || !qt || !R) { set_error("null argument"); return FVH_E_ARG; }
+    FVH_TRY(phase_setup(s));
+    FVH_TRY(put_q4(s, qt, s->ph[0].p));
+    launch_residuals<double>(s->stream, view<double>(s), tile_map(s), inst_view(s), ph_q4(s, 0),
+                             (double*)s->sums_blk.p, (double*)s->sums.p);
+    FVH_HIP(hipGetLastError());
+    return get_flat(s, s->sums.p, R, 4);
 }
 
 static bool bad_options(const fvh_run_options* o) {
@@ -1964,18 +2142,18 @@ static bool bad_options(const fvh_run_options* o) {
 int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st) {
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
     if (bad_options(o)) return FVH_E_ARG;
-    return s->f32 ? run_t<float>(s, o, st) : run_t<double>(s, o, st);
+    return FVH_DISPATCH(s, run_t, s, o, st);
 }
 
 int32_t fvh_solver_run_begin(fvh_solver* s) {
     if (!s) { set_error("null argument"); return FVH_E_ARG; }
-    return s->f32 ? apply_bounds_t<float>(s) : apply_bounds_t<double>(s);
+    return FVH_DISPATCH(s, apply_bounds_t, s);
 }
 
 int32_t fvh_solver_run_continue(fvh_solver* s, const fvh_run_options* o, uint32_t n, fvh_status* st) {
     if (!s || !o || !st) { set_error("null argument"); return FVH_E_ARG; }
     if (bad_options(o)) return FVH_E_ARG;
-    return s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st);
+    return FVH_DISPATCH(s, run_continue_t, s, o, n, st);
 }
 
 int32_t fvh_solver_run_log(fvh_solver* s, const fvh_run_options* o, fvh_row_cb cb, void* user, fvh_status* st) {
@@ -1988,7 +2166,7 @@ int32_t fvh_solver_run_log(fvh_solver* s, const fvh_run_options* o, fvh_row_cb c
     for (;;) {
         // to the next step the reference prints: step 0, then every print-th
         const uint32_t n = done == 0 ? 1 : ((done - 1) / print + 1) * print + 1 - done;
-        if ((rc = s->f32 ? run_continue_t<float>(s, o, n, st) : run_continue_t<double>(s, o, n, st))) return rc;
+        if ((rc = FVH_DISPATCH(s, run_continue_t, s, o, n, st))) return rc;
         if (st->step <= done) break;
         done = st->step;
         const uint32_t step = done - 1;
@@ -2046,7 +2224,7 @@ int32_t fvh_solver_advance(fvh_solver* s, int32_t n) {
     if (!s || n < 0) { set_error("bad argument"); return FVH_E_ARG; }
     fvh_run_options o{1e300, 0xffffffffu, 1, 0.0, -1, 0};
     for (int32_t k = 0; k < n; ++k) {
-        int32_t rc = s->f32 ? fvh::step<float>(s, false, &o, 0) : fvh::step<double>(s, false, &o, 0);
+        int32_t rc = FVH_DISPATCH(s, fvh::step, s, false, &o, 0);
         if (rc) return rc;
     }
     return FVH_OK;

@@ -42,16 +42,41 @@ void calc_flux(double* f, const double* qi, const double* qj, const double* gx, 
 
 void calc_dt(std::vector<double>& dt, const std::vector<double>& q, mesh& m);
 
-// One stage of each phase, on the GPU (reference: explicit.cpp:112-417).
+// The phases of one stage, each on the GPU through libfvh (reference:
+// explicit.h:88-149, bodies explicit.cpp:112-417, 481-564, 625-653).  Every
+// call copies its vectors to the device, runs the phase there and copies the
+// results back: for programs that compose their own loop; run() keeps the
+// state on the device instead.  The plugin kinds come from probing the
+// user's plugin functions once per mesh, as run() does.  Output vectors are
+// resized to vars values per cell when they are shorter.
+//
+// The reference's header and its source differ for two of them: explicit.h
+// declares calc_gradients with a const mesh& and calc_limiters without qmin
+// and qmax, while explicit.cpp defines them with mesh& and with qmin, qmax.
+// Both forms are defined here.
 void calc_gradients(std::vector<double>& gx, std::vector<double>& gy, const std::vector<double>& q,
                     const mesh& m);
+void calc_gradients(std::vector<double>& gx, std::vector<double>& gy, const std::vector<double>& q, mesh& m);
 void calc_limiters(std::vector<double>& limiters, const std::vector<double>& q, const std::vector<double>& gx,
                    const std::vector<double>& gy, const mesh& m);
+void calc_limiters(std::vector<double>& limiters, std::vector<double>& qmin, std::vector<double>& qmax,
+                   const std::vector<double>& q, const std::vector<double>& gx, const std::vector<double>& gy,
+                   mesh& m);
 void calc_time_derivatives(std::vector<double>& qt, const std::vector<double>& q, const std::vector<double>& gx,
                            const std::vector<double>& gy, const std::vector<double>& limiters, mesh& m);
 void update_cells(std::vector<double>& q, std::vector<double>& ql, const std::vector<double>& qt,
                   const std::vector<double>& dt, const double v);
+void update_bounds(std::vector<double>& q, std::vector<double>& gx, std::vector<double>& gy,
+                   std::vector<double>& limiters, mesh& m);
+// one device holds the whole mesh: no halo to exchange (explicit.cpp:420-477)
+void update_comms(std::vector<double>& q, mesh& m);
+void calc_residuals(double* R, std::vector<double>& qt, mesh& m, mpi_wrapper& pool);
 void min_dt(std::vector<double>& dt, mesh& m);
+// one rank: the local minimum is the global one (explicit.cpp:567-621)
+void validate_dt(std::vector<double>& dt, mpi_wrapper& pool);
+void complete_calc_qt(std::vector<double>& qt, std::vector<double>& q, std::vector<double>& gx,
+                      std::vector<double>& gy, std::vector<double>& qmin, std::vector<double>& qmax,
+                      std::vector<double>& limiters, mesh& m, mpi_wrapper& pool);
 
 struct solverOptions {
     double max_time = 1e10;

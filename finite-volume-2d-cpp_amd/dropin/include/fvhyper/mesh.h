@@ -22,6 +22,7 @@
 #include <fvhyper/parallel.h>
 
 struct fvh_mesh;
+struct fvh_solver;
 
 namespace fvhyper {
 
@@ -103,9 +104,13 @@ public:
 
     // libfvh mesh handle (owned)
     ::fvh_mesh* device_mesh() const { return handle_; }
+    // libfvh solver of the per-phase functions (calc_gradients, ...), built on
+    // first use from the probed plugin kinds (owned)
+    ::fvh_solver* phase_solver() const;
 
 private:
     ::fvh_mesh* handle_;
+    mutable ::fvh_solver* phase_;
 };
 
 }  // namespace fvhyper

@@ -17,5 +17,9 @@ namespace dropin {
 // naming the function when no device kind matches.
 fvh_config device_config(mesh& m, const std::vector<double>& q_initial);
 
+// The same without the time step (fixed, unused): the configuration of the
+// per-phase functions, which take the time step from the caller.
+fvh_config phase_config(mesh& m);
+
 }  // namespace dropin
 }  // namespace fvhyper

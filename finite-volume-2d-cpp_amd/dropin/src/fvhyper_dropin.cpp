@@ -37,8 +37,20 @@ int mpi_wrapper::exit() { return 0; }
 // ---------------------------------------------------------------------------
 // mesh
 // ---------------------------------------------------------------------------
-mesh::mesh() : nRealCells(0), handle_(nullptr) {}
-mesh::~mesh() { if (handle_) fvh_mesh_free(handle_); }
+mesh::mesh() : nRealCells(0), handle_(nullptr), phase_(nullptr) {}
+mesh::~mesh() {
+    if (phase_) fvh_solver_free(phase_);
+    if (handle_) fvh_mesh_free(handle_);
+}
+
+fvh_solver* mesh::phase_solver() const {
+    if (!phase_) {
+        if (!handle_) throw std::runtime_error("mesh not read");
+        fvh_config c = dropin::phase_config(const_cast<mesh&>(*this));
+        if (fvh_solver_create(handle_, &c, &phase_) != FVH_OK) throw std::runtime_error(fvh_last_error());
+    }
+    return phase_;
+}
 
 namespace {
 
@@ -66,6 +78,7 @@ void mesh::read_file(std::string name, mpi_wrapper& pool) {
     const int32_t rc = fvh_mesh_read_msh(filename.c_str(), &h);
     if (rc == FVH_E_MESH) throw std::invalid_argument(fvh_last_error());
     if (rc != FVH_OK) throw std::runtime_error(fvh_last_error());
+    if (phase_) { fvh_solver_free(phase_); phase_ = nullptr; }
     if (handle_) fvh_mesh_free(handle_);
     handle_ = h;
     export_field(h, "nodesX", nodesX);
@@ -196,6 +209,114 @@ void run(const std::string name, std::vector<double>& q, mpi_wrapper& pool, mesh
     }
     check(fvh_solver_get_state(s, q.data()));
     if (opt.verbose & (pool.rank == 0)) print_row(st.step, st.time, seconds(), st.R, false);
+}
+
+// ---------------------------------------------------------------------------
+// the phases one at a time (explicit.cpp:112-417, 481-564, 625-653)
+// ---------------------------------------------------------------------------
+namespace {
+
+void check_rc(int32_t rc) {
+    if (rc != FVH_OK) throw std::runtime_error(fvh_last_error());
+}
+
+// vars values per cell over all cells
+size_t state_size(const mesh& m) { return (size_t)vars * m.cellsAreas.size(); }
+
+void need(const std::vector<double>& v, const mesh& m, const char* what) {
+    if (v.size() < state_size(m))
+        throw std::invalid_argument(std::string(what) + ": fewer than vars values per cell");
+}
+
+void fit(std::vector<double>& v, const mesh& m) {
+    if (v.size() < state_size(m)) v.resize(state_size(m));
+}
+
+}  // namespace
+
+void calc_gradients(std::vector<double>& gx, std::vector<double>& gy, const std::vector<double>& q, mesh& m) {
+    need(q, m, "calc_gradients");
+    fit(gx, m);
+    fit(gy, m);
+    check_rc(fvh_phase_gradients(m.phase_solver(), q.data(), gx.data(), gy.data()));
+}
+
+void calc_gradients(std::vector<double>& gx, std::vector<double>& gy, const std::vector<double>& q,
+                    const mesh& m) {
+    calc_gradients(gx, gy, q, const_cast<mesh&>(m));
+}
+
+void calc_limiters(std::vector<double>& limiters, std::vector<double>& qmin, std::vector<double>& qmax,
+                   const std::vector<double>& q, const std::vector<double>& gx, const std::vector<double>& gy,
+                   mesh& m) {
+    need(q, m, "calc_limiters");
+    need(gx, m, "calc_limiters");
+    need(gy, m, "calc_limiters");
+    fit(limiters, m);
+    fit(qmin, m);
+    fit(qmax, m);
+    check_rc(fvh_phase_limiters(m.phase_solver(), q.data(), gx.data(), gy.data(), limiters.data(), qmin.data(),
+                                qmax.data()));
+}
+
+void calc_limiters(std::vector<double>& limiters, const std::vector<double>& q, const std::vector<double>& gx,
+                   const std::vector<double>& gy, const mesh& m) {
+    need(q, m, "calc_limiters");
+    need(gx, m, "calc_limiters");
+    need(gy, m, "calc_limiters");
+    fit(limiters, m);
+    check_rc(fvh_phase_limiters(const_cast<mesh&>(m).phase_solver(), q.data(), gx.data(), gy.data(),
+                                limiters.data(), nullptr, nullptr));
+}
+
+void calc_time_derivatives(std::vector<double>& qt, const std::vector<double>& q, const std::vector<double>& gx,
+                           const std::vector<double>& gy, const std::vector<double>& limiters, mesh& m) {
+    need(q, m, "calc_time_derivatives");
+    need(gx, m, "calc_time_derivatives");
+    need(gy, m, "calc_time_derivatives");
+    need(limiters, m, "calc_time_derivatives");
+    fit(qt, m);
+    check_rc(fvh_phase_time_derivatives(m.phase_solver(), q.data(), gx.data(), gy.data(), limiters.data(),
+                                        qt.data()));
+}
+
+void update_cells(std::vector<double>& q, std::vector<double>& ql, const std::vector<double>& qt,
+                  const std::vector<double>& dt, const double v) {
+    if (ql.size() < q.size() || qt.size() < q.size() || dt.size() < q.size())
+        throw std::invalid_argument("update_cells: ql, qt and dt need as many values as q");
+    check_rc(fvh_phase_update_cells(nullptr, q.data(), ql.data(), qt.data(), dt.data(), v, (int64_t)q.size()));
+}
+
+void update_bounds(std::vector<double>& q, std::vector<double>& gx, std::vector<double>& gy,
+                   std::vector<double>& limiters, mesh& m) {
+    need(q, m, "update_bounds");
+    need(gx, m, "update_bounds");
+    need(gy, m, "update_bounds");
+    need(limiters, m, "update_bounds");
+    check_rc(fvh_phase_update_bounds(m.phase_solver(), q.data(), gx.data(), gy.data(), limiters.data()));
+}
+
+void update_comms(std::vector<double>&, mesh&) {}
+
+void calc_residuals(double* R, std::vector<double>& qt, mesh& m, mpi_wrapper&) {
+    need(qt, m, "calc_residuals");
+    check_rc(fvh_phase_residuals(m.phase_solver(), qt.data(), R));
+}
+
+void min_dt(std::vector<double>& dt, mesh& m) {
+    if (dt.empty()) return;
+    check_rc(fvh_phase_min_dt(m.phase_solver(), dt.data(), (int64_t)dt.size()));
+}
+
+void validate_dt(std::vector<double>&, mpi_wrapper&) {}
+
+void complete_calc_qt(std::vector<double>& qt, std::vector<double>& q, std::vector<double>& gx,
+                      std::vector<double>& gy, std::vector<double>& qmin, std::vector<double>& qmax,
+                      std::vector<double>& limiters, mesh& m, mpi_wrapper& pool) {
+    if (solver::do_calc_gradients) calc_gradients(gx, gy, q, m);
+    if (solver::do_calc_limiters) calc_limiters(limiters, qmin, qmax, q, gx, gy, m);
+    (void)pool;
+    calc_time_derivatives(qt, q, gx, gy, limiters, m);
 }
 
 // ---------------------------------------------------------------------------

@@ -352,7 +352,9 @@ void probe_dt(mesh& m, fvh_config& c, const std::vector<double>& q_initial, cons
 
 }  // namespace
 
-fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
+namespace {
+
+fvh_config config_without_dt(mesh& m, FluxKind& fk) {
     if (vars != 4)
         throw std::runtime_error("vars = " + std::to_string(vars) +
                                  ": the device path implements the 4-variable Euler plugins");
@@ -366,7 +368,7 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
     if (solver::smooth_residuals) c.flags |= FVH_FLAG_SMOOTH;
     if (solver::diffusive_gradients) c.flags |= FVH_FLAG_DIFFUSIVE;
     c.K = solver::limiter_k_value;
-    const FluxKind fk = probe_flux();
+    fk = probe_flux();
     c.flux = fk.kind;
     c.gamma = fk.gamma;
     if (fk.kind == FVH_FLUX_ROE_VISCOUS) {
@@ -390,6 +392,19 @@ fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
         seen.push_back(nm);
         c.bc[c.n_bc++] = probe_bc(nm, m.boundaryFuncs[b], c.gamma);
     }
+    return c;
+}
+
+}  // namespace
+
+fvh_config phase_config(mesh& m) {
+    FluxKind fk;
+    return config_without_dt(m, fk);
+}
+
+fvh_config device_config(mesh& m, const std::vector<double>& q_initial) {
+    FluxKind fk;
+    fvh_config c = config_without_dt(m, fk);
     probe_dt(m, c, q_initial, fk.visc);
     return c;
 }

@@ -75,6 +75,8 @@ EXPORTS = [
     "fvh_solver_kernel_times", "fvh_solver_kernel_name", "fvh_solver_stage_bytes", "fvh_solver_tile_cells",
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
     "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
+    "fvh_phase_gradients", "fvh_phase_limiters", "fvh_phase_time_derivatives", "fvh_phase_update_cells",
+    "fvh_phase_update_bounds", "fvh_phase_min_dt", "fvh_phase_residuals",
 ]
 
 _lib = None
@@ -173,6 +175,13 @@ def load(path: str = LIB_PATH):
                               C.POINTER(AdjointStats)]
     L.fvh_autotune.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), dp, C.c_int32,
                                C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, C.c_int32]
+    L.fvh_phase_gradients.argtypes = [vp, dp, dp, dp]
+    L.fvh_phase_limiters.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fvh_phase_time_derivatives.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.fvh_phase_update_cells.argtypes = [vp, dp, dp, dp, dp, C.c_double, C.c_int64]
+    L.fvh_phase_update_bounds.argtypes = [vp, dp, dp, dp, dp]
+    L.fvh_phase_min_dt.argtypes = [vp, dp, C.c_int64]
+    L.fvh_phase_residuals.argtypes = [vp, dp, dp]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

@@ -82,6 +82,9 @@ class Mesh:
 
 
 ORDERS = {"file": 0, "rcm": 1, "tiles": 2}
+# f64: FP64 throughout (bit-identical to the reference); f32: FP32 states,
+# geometry and arithmetic; f32_f64: FP32 states, FP64 geometry and arithmetic
+PRECISION = {"f64": 0, "f32": 1, "f32_f64": 2}
 FLUX_MODES = {"auto": 0, "cell": 1, "tile": 2, "fused": 3}
 
 
@@ -90,7 +93,7 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
     """fvh_config from the shared configuration dict (see oracle/oracle.py)."""
     c = _lib.Config()
     _lib.load().fvh_config_default(C.byref(c))
-    c.precision = {"f64": 0, "f32": 1}[precision]
+    c.precision = PRECISION[precision]
     c.flux = _lib.FLUX[cfg.get("flux", "roe_split")]
     c.gamma = float(cfg.get("gamma", 1.4))
     c.phi = _lib.PHI[cfg.get("phi", "zero")]
@@ -207,6 +210,59 @@ class Solver:
                                             dptr(out["gy"]), dptr(out["lim"]), dptr(out["qt"]),
                                             dptr(out["qk"])))
         return out
+
+    # -- the phases of explicit.cpp one at a time on given arrays (FP64) --
+    def _q(self, a) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size != self.n_state:
+            raise ValueError("array holds %d values, mesh needs %d" % (a.size, self.n_state))
+        return a
+
+    def phase_gradients(self, q):
+        """calc_gradients (explicit.cpp:112-171): (gx, gy)."""
+        gx, gy = np.zeros(self.n_state), np.zeros(self.n_state)
+        check(_lib.load().fvh_phase_gradients(self._h, dptr(self._q(q)), dptr(gx), dptr(gy)))
+        return gx, gy
+
+    def phase_limiters(self, q, gx, gy):
+        """calc_limiters (explicit.cpp:174-258): (limiters, qmin, qmax)."""
+        lim, qmin, qmax = (np.zeros(self.n_state) for _ in range(3))
+        check(_lib.load().fvh_phase_limiters(self._h, dptr(self._q(q)), dptr(self._q(gx)), dptr(self._q(gy)),
+                                             dptr(lim), dptr(qmin), dptr(qmax)))
+        return lim, qmin, qmax
+
+    def phase_time_derivatives(self, q, gx, gy, lim):
+        """calc_time_derivatives (explicit.cpp:261-364): qt."""
+        qt = np.zeros(self.n_state)
+        check(_lib.load().fvh_phase_time_derivatives(self._h, dptr(self._q(q)), dptr(self._q(gx)),
+                                                     dptr(self._q(gy)), dptr(self._q(lim)), dptr(qt)))
+        return qt
+
+    def phase_update_cells(self, ql, qt, dt, v: float):
+        """update_cells (explicit.cpp:368-378): ql + qt dt v, dt one value per entry."""
+        ql, qt, dt = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (ql, qt, dt))
+        q = np.zeros(ql.size)
+        check(_lib.load().fvh_phase_update_cells(self._h, dptr(q), dptr(ql), dptr(qt), dptr(dt), float(v), ql.size))
+        return q
+
+    def phase_update_bounds(self, q, gx, gy, lim):
+        """update_bounds (explicit.cpp:380-417): q with its ghost cells updated."""
+        out = self._q(q).copy()
+        check(_lib.load().fvh_phase_update_bounds(self._h, dptr(out), dptr(self._q(gx)), dptr(self._q(gy)),
+                                                  dptr(self._q(lim))))
+        return out
+
+    def phase_min_dt(self, dt):
+        """min_dt (explicit.cpp:555-564): every value set to the minimum."""
+        out = np.ascontiguousarray(dt, dtype=np.float64).ravel().copy()
+        check(_lib.load().fvh_phase_min_dt(self._h, dptr(out), out.size))
+        return out
+
+    def phase_residuals(self, qt):
+        """calc_residuals (explicit.cpp:481-552): R[4]."""
+        R = np.zeros(4)
+        check(_lib.load().fvh_phase_residuals(self._h, dptr(self._q(qt)), dptr(R)))
+        return R
 
     def run(self, max_step: int = 100000000, max_time: float = 1e10, tol: float = 1e-16,
             print_interval: int = 1, stop_var: int = -1) -> Dict:

@@ -61,7 +61,10 @@ enum { FVH_BC_COPY = 0,          /* shockTube/main.cpp:158-163 */
        FVH_BC_NOSLIP = 4,        /* flatPlate/main.cpp:361-366: momentum reversed */
        FVH_BC_FARFIELD_V = 5     /* flatPlate/main.cpp:286-353: the farfield with
                                     b[2] = b[0]*(...) in both subsonic branches */ };
-enum { FVH_F64 = 0, FVH_F32 = 1 };
+/* Precision: FP64 throughout (reference-exact); FP32 states, geometry and
+ * arithmetic; FP32 states with FP64 arithmetic and FP64 geometry (each stage
+ * rounds the stored state to FP32, nothing else is rounded to FP32). */
+enum { FVH_F64 = 0, FVH_F32 = 1, FVH_F32_F64 = 2 };
 /* Device cell order (results do not depend on it, only locality does). */
 enum { FVH_ORDER_FILE = 0,       /* reference numbering */
        FVH_ORDER_RCM = 1,        /* reverse Cuthill-McKee */
@@ -102,7 +105,7 @@ typedef struct fvh_bc_spec {
 } fvh_bc_spec;
 
 typedef struct fvh_config {
-    int32_t precision;         /* FVH_F64 (reference-exact) or FVH_F32 */
+    int32_t precision;         /* FVH_F64 (reference-exact), FVH_F32 or FVH_F32_F64 */
     int32_t flux;              /* FVH_FLUX_* */
     double gamma;
     int32_t phi;               /* FVH_PHI_* (limiter_func) */
@@ -228,6 +231,45 @@ int32_t fvh_solver_apply_bounds(fvh_solver* s);
  * dt: 1 value per cell; the others 4 per cell.  The state is not changed. */
 int32_t fvh_solver_phases(fvh_solver* s, double alpha, double* dt, double* gx, double* gy,
                           double* lim, double* qt, double* qk);
+
+/* ---- the phases of explicit.cpp one at a time (explicit.h:88-149), for
+ * callers that compose their own loop.  Every array is in the reference
+ * layout (4 values per cell over all cells: real cells, then ghosts); each
+ * call copies its inputs to the device, runs the phase there and copies its
+ * outputs back, so it costs PCIe transfers and is meant for composing, not
+ * for speed.  FP64, single-mesh solvers; the solver's own state is not
+ * touched.  The solver's config supplies the plugin kinds: flux, limiter
+ * function, boundary kinds and the solver:: flags. */
+
+/* calc_gradients (explicit.cpp:112-171): Green-Gauss gradients of the real
+ * cells, 0 on the ghost cells. */
+int32_t fvh_phase_gradients(fvh_solver* s, const double* q, double* gx, double* gy);
+/* calc_limiters (explicit.cpp:174-258) from given gradients: the limiter of
+ * every real cell, 1 on the ghost cells; qmin, qmax (may be NULL) the
+ * neighbour minima and maxima of every cell. */
+int32_t fvh_phase_limiters(fvh_solver* s, const double* q, const double* gx, const double* gy, double* lim,
+                           double* qmin, double* qmax);
+/* calc_time_derivatives (explicit.cpp:261-364) from given gradients and
+ * limiters: qt of the real cells, 0 on the ghost cells. */
+int32_t fvh_phase_time_derivatives(fvh_solver* s, const double* q, const double* gx, const double* gy,
+                                   const double* lim, double* qt);
+/* update_cells (explicit.cpp:368-378): q[i] = ql[i] + qt[i] dt[i] v over n
+ * values (n at most 4 per cell).  s may be NULL (the reference's
+ * update_cells takes no mesh): the call then uses temporary device buffers
+ * on the current device. */
+int32_t fvh_phase_update_cells(fvh_solver* s, double* q, const double* ql, const double* qt, const double* dt,
+                               double v, int64_t n);
+/* update_bounds (explicit.cpp:380-417): the ghost cells of q from the
+ * boundary functions of the interior face states (given gradients and
+ * limiters with linear_interpolate); the real cells are unchanged. */
+int32_t fvh_phase_update_bounds(fvh_solver* s, double* q, const double* gx, const double* gy, const double* lim);
+/* min_dt (explicit.cpp:555-564): every one of the n values becomes their
+ * minimum (NaN if dt[0] is NaN, other NaNs ignored, as std::min from dt[0]). */
+int32_t fvh_phase_min_dt(fvh_solver* s, double* dt, int64_t n);
+/* calc_residuals (explicit.cpp:481-552), one rank: R[k] = sqrt(sum over the
+ * real cells of qt^2 A), summed in a fixed tree order (not ascending cell
+ * order: equal to ~1e-15 relative). */
+int32_t fvh_phase_residuals(fvh_solver* s, const double* qt, double* R);
 
 /* The run() loop of explicit.cpp:657-817 from the current state: applies the
  * initial boundary update, then steps until the stop test holds. */

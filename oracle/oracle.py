@@ -253,6 +253,45 @@ class Mesh:
         return q, int(steps.value), float(t.value), R, R0, qt
 
 
+    def run_f32_state(self, cfg: Dict, q0: np.ndarray, max_step: int):
+        """run() of explicit.cpp:657-817 with the cell states stored in FP32:
+        every state that run() writes (the initial state, the initial
+        boundary update, each stage's updated cells and ghost cells) is
+        rounded to the nearest float, and everything else -- gradients,
+        limiters, fluxes, time steps, the update itself -- is computed in
+        FP64 from the rounded states, as orc_run computes it.  The library's
+        FP32-state, FP64-arithmetic mode must give these bits.  Returns the
+        state after max_step steps (no stop test)."""
+        L = lib()
+        c = make_config(cfg, self.names)
+        N = self.n_cells
+        n = 4 * N
+        f32 = lambda a: a.astype(np.float32).astype(np.float64)
+        q = f32(np.ascontiguousarray(q0, dtype=np.float64))
+        gx, gy, lim, qmin, qmax, qt = (np.zeros(n) for _ in range(6))
+        dt, qk = np.zeros(N), np.zeros(n)
+        L.orc_update_bounds(self._p, C.byref(c), q, gx, gy, lim)
+        q = f32(q)
+        for _ in range(max_step):
+            L.orc_calc_dt(self._p, C.byref(c), q, dt)
+            if c.gdt:
+                L.orc_min_dt(self._p, dt)
+            qk[:] = q
+            for a in list(cfg.get("alpha", RK5)):
+                if c.grad:
+                    L.orc_calc_gradients(self._p, qk, gx, gy)
+                if c.lim:
+                    L.orc_calc_limiters(self._p, C.byref(c), qk, gx, gy, lim, qmin, qmax)
+                L.orc_calc_time_derivatives(self._p, C.byref(c), qk, gx, gy, lim, qt)
+                if c.smooth:
+                    L.orc_smooth_residuals(self._p, qt)
+                L.orc_update_cells(N, qk, q, qt, dt, float(a))
+                L.orc_update_bounds(self._p, C.byref(c), qk, gx, gy, lim)
+                qk[:] = f32(qk)
+            q[:] = qk
+        return q
+
+
 def flux(cfg: Dict, qi, qj, n) -> np.ndarray:
     c = make_config(dict(cfg, bcs={}), [])
     f = np.zeros(4)

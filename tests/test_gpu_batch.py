@@ -20,7 +20,7 @@ def _states(meshes, items):
     return [workloads.initial_state(m, it[2]) for m, it in zip(meshes, items)]
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32_f64"])
 def test_batch_equals_single_solvers(precision):
     """6 airfoil instances (different sizes, Mach, angle): 15 steps in one
     batch == 15 steps of each mesh alone, bit for bit."""
@@ -56,6 +56,32 @@ def test_batch_instance_matches_oracle():
         om = O.Mesh(it[0])
         qo, *_ = om.run(it[1], q0[k], 8)
         assert bits_equal(qb[k], qo), k
+
+
+@pytest.mark.parametrize("precision", ["f32_f64", "f32"])
+def test_fp32_batch_instances_against_fp64_oracle(precision):
+    """The sweep's precisions, instance by instance against the FP64
+    oracle after 25 steps (the default bench's step count): FP32 states
+    with FP64 arithmetic within 1e-6 (and bit for bit the oracle with
+    FP32-rounded stored states), FP32 arithmetic within 4e-6."""
+    print("case fp32 batch", precision, flush=True)
+    items = _sweep(4, lo=3000, hi=9000, seed=13)
+    meshes = [Mesh.from_arrays(it[0]) for it in items]
+    b = Batch(meshes, [it[1] for it in items], precision=precision)
+    q0 = _states(meshes, items)
+    b.set_state(b.join(q0))
+    b.run(max_step=25)
+    qb = b.split(b.get_state())
+    for k, it in enumerate(items):
+        om = O.Mesh(it[0])
+        qo, *_ = om.run(it[1], q0[k], 25)
+        err = rel_err(qo, qb[k])
+        print("instance", k, precision, "rel err", err, flush=True)
+        if precision == "f32_f64":
+            assert bits_equal(qb[k], om.run_f32_state(it[1], q0[k], 25)), k
+            assert err < 1e-6, (k, err)
+        else:
+            assert err < 4e-6, (k, err)
 
 
 def test_batch_instances_stop_on_their_own():

@@ -679,3 +679,29 @@ def test_positivity_fallback_is_inert_on_physical_faces(mesh_name):
             assert np.array_equal(qa.view(np.uint64), qb.view(np.uint64)), cfg_name
             runs += 1
     assert runs >= 1
+
+
+def test_f32_state_restatement_rounds_only_the_states():
+    """The FP32-state run of the oracle: its result is FP32-representable,
+    close to the FP64 run (within the state's rounding, 1e-6) but not equal
+    to it; and a uniform flow with transmissive boundaries, FP32-representable
+    and far from zero in every variable, whose FP64 updates are rounding
+    noise of 1e-16, stays exactly that state."""
+    from helpers import CONFIGS, MESHES, rel_err
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    cfg = CONFIGS["second_venkat_global"]
+    q0 = random_state(om.n_cells, seed=303, mach=0.3)
+    a = om.run_f32_state(cfg, q0, 4)
+    r, *_ = om.run(cfg, q0, 4)
+    assert np.array_equal(a, a.astype(np.float32).astype(np.float64))
+    e = rel_err(r, a)
+    assert 0 < e < 1e-6
+    box = meshgen.sod_tube(nx=20, ny=4)
+    ob = O.Mesh(box)
+    cb = dict(flux="roe_joined", dt="sum_cfl", cfl=0.5, gdt=True, phi="venkat", grad=True, lim=True, lin=True,
+              bcs={"wall": ("copy",), "transmissive": ("copy",)})
+    qc = np.tile([1.0, 0.5, 0.25, 1.40625], ob.n_cells)   # p = 0.5 up to the rounding of gamma - 1
+    qr = ob.run(cb, qc, 3)[0]
+    assert not np.array_equal(qr, qc) and np.max(np.abs(qr - qc)) < 1e-14
+    assert np.array_equal(ob.run_f32_state(cb, qc, 3), qc)
