@@ -9,10 +9,12 @@
 // arithmetic changes the last bits.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <math.h>
 
+// Host and device by default; a host-only translation unit (a plain C++
+// compiler, no HIP headers) defines FVH_HD as `inline` before including.
 #ifndef FVH_HD
+#include <hip/hip_runtime.h>
 #define FVH_HD __host__ __device__ inline
 #endif
 

@@ -109,3 +109,54 @@ def test_full_precision_vtu_rounds_to_the_reference_text(tmp_path):
             assert len(va) == len(vb)
             for x, y in zip(va, vb):
                 assert x == y or x == "%f" % float(y), (name, x, y)
+
+
+PHASES_BIN = os.path.join(ROOT, "tests", "dropin_phases", "dropin_phases")
+
+
+@pytest.mark.parametrize("mesh_name", ["step", "mixed"])
+def test_user_program_calling_the_phase_functions(mesh_name, tmp_path):
+    """A user program against the drop-in headers that calls the phases of
+    one stage itself (tests/dropin_phases/main.cpp, built against
+    libfvhyper.so): calc_gradients and calc_limiters in both the header's and
+    the source's forms, calc_time_derivatives, min_dt, update_cells,
+    update_bounds, calc_residuals and complete_calc_qt, each against the
+    oracle's phase on the same state."""
+    import ctypes as C
+
+    import oracle as O
+    from helpers import CONFIGS, MESHES, bits_equal, random_state
+    if not os.path.exists(PHASES_BIN):
+        pytest.skip("tests/dropin_phases is not built (make -C tests/dropin_phases)")
+    print("case dropin phases", mesh_name, flush=True)
+    m = MESHES[mesh_name]()
+    om = O.Mesh(m)
+    cfg = CONFIGS["second_venkat_global"]
+    q = random_state(om.n_cells, seed=515, mach=0.3)
+    meshgen.write_msh(str(tmp_path / "case.msh"), m)
+    q.tofile(str(tmp_path / "q.f64"))
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([PHASES_BIN, "case", str(tmp_path / "q.f64"), str(out)], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    got = {p[:-4]: np.fromfile(str(out / p)) for p in os.listdir(str(out))}
+    o = om.phases(cfg, q, alpha=0.506)
+    for a, b in (("gx", "gx"), ("gy", "gy"), ("gx_const", "gx"), ("gy_const", "gy"), ("lim", "lim"),
+                 ("lim_header", "lim"), ("qmin", "qmin"), ("qmax", "qmax"), ("qt", "qt"), ("qt_complete", "qt"),
+                 ("lim_complete", "lim")):
+        assert bits_equal(got[a], o[b]), a
+    # min_dt over the program's own time step
+    assert np.all(got["dt"] == np.min(got["dt_local"]))
+    # update_cells with that step, then update_bounds with this stage's
+    # gradients and limiters, as the oracle's phases do them
+    L = O.lib()
+    c = O.make_config(cfg, om.names)
+    qk = np.zeros_like(q)
+    L.orc_update_cells(om.n_cells, qk, q, o["qt"], np.ascontiguousarray(got["dt"][::4]), 0.506)
+    assert bits_equal(got["qk_cells"], qk)
+    L.orc_update_bounds(om._p, C.byref(c), qk, o["gx"], o["gy"], o["lim"])
+    assert bits_equal(got["qk"], qk)
+    R = np.zeros(4)
+    L.orc_calc_residuals(om._p, o["qt"], R)
+    np.testing.assert_allclose(got["R"], R, rtol=1e-12, atol=0)
