@@ -1,22 +1,31 @@
-"""Benchmark: explicit second-order RK5 steps of the explicit path.
+"""Benchmark of the explicit path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 400] [--warmup 20] [--precision f64]
-                    [--workload step|airfoil] [--converge]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
+    python bench.py --workload step|airfoil|sweep|adjoint|plate [...]
 
-One step = one RK5 step (5 stages: gradient+limiter, flux+update+boundary)
-plus the time-step pass, inputs resident in HBM.  The default workload is
-the 975,744-cell Mach-3 forward step in FP64; ``--workload airfoil`` is the
-2.0M-cell transonic NACA 0012 O-mesh with local time steps, and with
-``--converge`` it runs from the freestream until the density residual has
-dropped 6 orders and reports the time to get there.  Prints one JSON line
-with the throughput in Mcell-stage/s, the HBM roofline of the dominant
-kernel (measured with HIP events on the solver's stream over the timed
-region) and a CPU baseline (the reference library on a bounded sample of
-the same workload).
+With no --workload, one JSON line carries the metric's three parts:
+  * top level: the Mach-3 forward step (975,744 cells, second order,
+    Venkatakrishnan, RK5, global dt) in FP64, bit-identical to the reference
+    -- Mcell-stage/s over --steps RK5 steps replayed from HIP graphs, the HBM
+    roofline of the fused stage kernel, and the reference library itself on
+    the host's CPU share as the CPU baseline;
+  * "sweep": 256 airfoil instances (5k-200k cells, Mach 0.5-0.85, angle
+    -2..4 degrees) with FP32 states (FP64 arithmetic, residual and sums) in
+    one launch per stage, against the same instances one by one through the
+    per-mesh path, the error against the FP64 oracle on a sample of the
+    instances, and 256 single-rank reference processes as the CPU baseline;
+  * "adjoint": dJ/dq0 of the step-face pressure force after 80 RK5 steps
+    (400 stages) at 243,936 cells, its cost over the forward run, central
+    differences on 64 directions and the complex-step derivative of the CPU
+    restatement on 16 directions at that size (computed by CPU worker
+    processes while the GPU runs the rest).
+One step = one RK5 step (5 stages) plus the time-step pass; inputs are
+resident in HBM when the timed region starts.
 """
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import re
@@ -29,10 +38,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "finite-volume-2d-cpp_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# a GPU box lends one GPU 16 of its host's cores, whatever the host has
+CPU_SHARE = 16
+
+
+def cpu_info():
+    """The host CPU and the cores this run may use (its affinity, at most the
+    16-core share of one GPU)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:   # pragma: no cover - not Linux
+        aff = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff, "ranks": max(1, min(CPU_SHARE, aff))}
 
 
 def algorithmic_bytes(C, E, B, s, g, V=4):
-    """Per-stage algorithmic bytes split between the two stage kernels.
+    """Per-stage algorithmic bytes split between the two stage kernels
+    (s: state bytes, g: geometry bytes).
 
     Whole stage (second order): B2 = B1 + C(7Vs + 2g) + E(2g), with
     B1 = C(3Vs + s + g) + E(3g + 8) + B(2Vs + 2g + 4).  The flux kernel owns
@@ -44,15 +74,30 @@ def algorithmic_bytes(C, E, B, s, g, V=4):
     return {"stage": flux + gradlim, "flux": flux, "gradlim": gradlim}
 
 
-def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step"):
-    """Time the reference library (oracle/_ref/fvh_ref, kind 'reference') on
-    `steps` RK5 steps of the same mesh; falls back to the C restatement.
-    For the airfoil the reference's farfield has its own hard-coded
-    freestream (Mach 0.8 at 1.2532 degrees), which costs the same."""
+def state_bytes(precision):
+    """(state bytes, geometry bytes) of a precision mode."""
+    return {"f64": (8, 8), "f32": (4, 4), "f32_f64": (4, 8)}[precision]
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from fvhyper_amd import meshgen
+    return O
 
+
+def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step", ranks: int = 0):
+    """Time the reference library (oracle/_ref/fvh_ref, kind 'reference') on
+    `steps` RK5 steps of the same mesh, on one rank and on `ranks` MPI ranks
+    (the mesh partitioned by recursive bisection with one ghost layer, as the
+    reference reads partitioned meshes); falls back to the C restatement on
+    one thread.  The timing is the reference's own RealTime column (steady
+    clock around the loop).  For the airfoil the reference's farfield has its
+    own hard-coded freestream (Mach 0.8 at 1.2532 degrees), which costs the
+    same."""
+    O = _oracle()
+    from fvhyper_amd import meshgen
+    info = cpu_info()
+    ranks = ranks or info["ranks"]
     C = mesh_arrays.n_cells
     if O.ref_available():
         try:
@@ -60,39 +105,31 @@ def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step"):
             meshgen.write_msh(os.path.join(work, "case.msh"), mesh_arrays)
             cfg2 = dict(cfg)
             cfg2["K"] = 7.5   # the reference library fixes K; the cost is the same
+            cfg2.pop("positivity", None)
             kv = O.ref_args(cfg2)
-            if workload == "step":
-                kv.update(ic="fs", mach="3")
-            else:
-                kv.update(ic="nc")
+            kv.update(ic="fs", mach="3") if workload == "step" else kv.update(ic="nc")
             kv.update(max_step=str(steps), print=str(steps))
             r = O.run_ref("run", work, "case", flags=O.ref_flags(cfg2), timeout=900, **kv)
-            last = r["stdout"].strip().splitlines()[-1].split(",")
-            secs = float(last[2])
-            one = {"value": C * 5 * steps / secs / 1e6, "unit": "Mcell-stage/s", "cores": 1,
-                   "kind": "reference",
-                   "sample": "%d RK5 steps (%d stages) of the %d-cell %s workload, fvhyper library "
-                             "built from its sources (-O3), 1 rank, RealTime loop clock"
-                             % (steps, 5 * steps, C, workload)}
-            ranks = min(16, os.cpu_count() or 1)
+            secs = float(r["stdout"].strip().splitlines()[-1].split(",")[2])
+            one = {"value": C * 5 * steps / secs / 1e6, "unit": "Mcell-stage/s", "cores": 1, "kind": "reference",
+                   "cpu": info["model"],
+                   "sample": "%d RK5 steps (%d stages) of the %d-cell %s workload, fvhyper library built from its "
+                             "sources (-O3), 1 rank, RealTime loop clock" % (steps, 5 * steps, C, workload)}
             if ranks < 2 or not O.mpi_available():
                 return one
-            # the same library on every core of the host: the mesh partitioned by
-            # recursive bisection, one ghost layer per rank, as the reference reads
-            # it (mesh.cpp:803-855), run under mpirun
             try:
                 part = meshgen.partition_rcb(mesh_arrays, ranks)
                 meshgen.write_partitioned_msh(work, "case", mesh_arrays, part)
-                steps_p = min(steps * ranks, 100)
+                steps_p = max(steps, min(steps * ranks, 96))
                 kv.update(max_step=str(steps_p), print=str(steps_p))
                 rp = O.run_ref_mpi("run", work, "case", ranks, flags=O.ref_flags(cfg2), timeout=900, **kv)
                 secs_p = float(rp["stdout"].strip().splitlines()[-1].split(",")[2])
                 return {"value": C * 5 * steps_p / secs_p / 1e6, "unit": "Mcell-stage/s", "cores": ranks,
-                        "kind": "reference",
+                        "kind": "reference", "cpu": info["model"], "host_cores": info["nproc"],
                         "sample": "%d RK5 steps (%d stages) of the %d-cell %s workload, fvhyper library built "
-                                  "from its sources (-O3), %d MPI ranks (mpirun) on a recursive-bisection "
-                                  "partition with one ghost layer, RealTime loop clock of rank 0"
-                                  % (steps_p, 5 * steps_p, C, workload, ranks),
+                                  "from its sources (-O3), %d MPI ranks (mpirun, the CPU share one GPU has on "
+                                  "the host) on a recursive-bisection partition with one ghost layer, RealTime "
+                                  "loop clock of rank 0" % (steps_p, 5 * steps_p, C, workload, ranks),
                         "single_rank": {"value": one["value"], "sample": one["sample"]}}
             except Exception as e:  # pragma: no cover - reported, the single rank stands
                 print("multi-rank reference baseline unavailable: %s" % e, file=sys.stderr)
@@ -107,7 +144,77 @@ def cpu_baseline(mesh_arrays, cfg, steps: int, workload: str = "step"):
     om.run(cfg, q0, steps)
     secs = time.perf_counter() - t0
     return {"value": C * 5 * steps / secs / 1e6, "unit": "Mcell-stage/s", "cores": 1, "kind": "port",
+            "cpu": info["model"],
             "sample": "%d RK5 steps of the %d-cell %s workload, C restatement, 1 thread" % (steps, C, workload)}
+
+
+def sweep_mesh_names(items):
+    """File name of each instance's mesh (one file per distinct mesh)."""
+    return ["m%06d" % it[0].cells.shape[0] for it in items]
+
+
+def sweep_write_meshes(items, work, part=0, parts=1):
+    """The distinct instance meshes of a sweep as MSH 4.1 in `work` (the
+    physical names the reference's naca plugin binds: airfoil, farfield);
+    this call writes every parts-th of them, from the part-th."""
+    from fvhyper_amd import meshgen
+    seen = []
+    for it, nm in zip(items, sweep_mesh_names(items)):
+        if nm not in seen:
+            seen.append(nm)
+            if (len(seen) - 1) % parts == part:
+                meshgen.write_msh(os.path.join(work, nm + ".msh"), it[0])
+    return sweep_mesh_names(items)
+
+
+def mesh_writer_worker(a):
+    """CPU only: write this worker's share of the sweep meshes."""
+    from fvhyper_amd import workloads
+    part, parts = (int(x) for x in a.mesh_writer.split("/"))
+    sweep_write_meshes(workloads.naca_sweep(a.instances), a.mesh_dir, part, parts)
+
+
+def start_mesh_writers(a, work, procs):
+    out = []
+    for k in range(procs):
+        cmd = ["nice", "-n", "19", sys.executable, os.path.abspath(__file__), "--mesh-writer", "%d/%d" % (k, procs),
+               "--mesh-dir", work, "--instances", str(a.instances)]
+        out.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    return out
+
+
+def sweep_cpu_baseline(items, files, work, steps: int, ranks: int):
+    """The sweep on the CPU as the reference runs a sweep: one single-rank
+    reference process per instance (the naca plugin's farfield has a
+    hard-coded freestream, which costs the same for every Mach and angle),
+    `ranks` processes at a time.  Each process reports its RealTime loop
+    time; the aggregate rate is ranks x (total cell-stages / total loop
+    time)."""
+    O = _oracle()
+    info = cpu_info()
+    if not O.ref_available():
+        return None
+    cfg2 = dict(items[0][1])
+    cfg2["K"] = 7.5
+    kv = O.ref_args(cfg2)
+    kv.update(ic="nc", max_step=str(steps), print=str(steps))
+    flags = O.ref_flags(cfg2)
+
+    def one(k):
+        r = O.run_ref("run", work, files[k], flags=flags, timeout=900, **kv)
+        return float(r["stdout"].strip().splitlines()[-1].split(",")[2])
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=ranks) as ex:
+        secs = list(ex.map(one, range(len(items))))
+    wall = time.perf_counter() - t0
+    cells = [it[0].cells.shape[0] for it in items]
+    return {"value": ranks * sum(cells) * 5 * steps / sum(secs) / 1e6, "unit": "Mcell-stage/s", "cores": ranks,
+            "kind": "reference", "cpu": info["model"], "host_cores": info["nproc"],
+            "sample": "the %d instances, %d RK5 steps each, one single-rank fvhyper reference process per instance "
+                      "(-O3), %d processes at a time; rate = processes x cell-stages / summed RealTime loop time"
+                      % (len(items), steps, ranks),
+            "wall_s": round(wall, 2)}
 
 
 # rocprofv3 PMC summaries committed under profiles/: (workload, precision,
@@ -191,17 +298,55 @@ def converge(s, C, stages, tol, stop_var, chunk=2000, limit=400000):
             "R": [float(x) for x in st["R"]], "Mcell_stage_per_s": round(C * stages * st["step"] / secs / 1e6, 2)}
 
 
-def sweep(a):
-    """Parameter sweep: a batch of coarsened airfoil instances (sizes
-    log-uniform in 5k-200k cells, Mach 0.5-0.85, angle -2..4 degrees), FP32
-    state with FP64 residual sums, every step replayed from a HIP graph;
-    against the same instances run one after another through the plain
-    single-mesh path (own solver each, eager launches, synchronised per
-    instance)."""
+def rel_err(a, b) -> float:
+    """max_{i,k} |a-b| / max(|a_ik|, max_i |a_.k|): the error measure of the
+    FP32 paths against FP64 (a per-variable infinity-norm floor)."""
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64).reshape(-1, 4)
+    b = np.asarray(b, dtype=np.float64).reshape(-1, 4)
+    scale = np.maximum(np.abs(a), np.max(np.abs(a), axis=0, keepdims=True))
+    scale = np.where(scale == 0, 1.0, scale)
+    return float(np.max(np.abs(a - b) / scale)) if a.size else 0.0
+
+
+def stage_roofline(kt, nbytes, key="stage", kernel="k_stage (fused gradient + limiter + flux + RK update + "
+                   "boundary, one launch per stage)"):
+    """roofline object of the kernel `key` from event timings kt."""
+    if not kt.get(key, {}).get("count", 0):
+        return None
+    avg_ms = kt[key]["total_ms"] / kt[key]["count"]
+    ach = nbytes / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel, "bytes_per_launch": nbytes,
+            "avg_launch_us": round(avg_ms * 1e3, 2),
+            "kernels_us": {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}}
+
+
+def event_times(s, steps):
+    """Kernel durations: HIP events around every launch on the solver's
+    stream over `steps` more steps (eager launches)."""
+    s.use_graphs(False)
+    s.profile(True)
+    s.advance(steps)
+    s.synchronize()
+    kt = s.kernel_times()
+    s.profile(False)
+    return kt
+
+
+def sweep(a, with_cpu=True):
+    """Parameter sweep: 256 coarsened airfoil instances (sizes log-uniform in
+    5k-200k cells, Mach 0.5-0.85, angle -2..4 degrees), FP32 states with FP64
+    arithmetic and residual sums by default, every step of the batch
+    replayed from a HIP graph, stage form and tile size from the autotuner
+    timed on the whole batch; against the same instances one by one through
+    the per-mesh path (the library's default form, each step replayed from
+    its graph, one instance after the other).  Returns the result dict."""
     import numpy as np
     import torch
 
-    from fvhyper_amd import Batch, Mesh, Solver, workloads
+    O = _oracle()
+    from fvhyper_amd import Batch, Mesh, Solver, autotune, workloads
 
     t0 = time.perf_counter()
     items = workloads.naca_sweep(a.instances)
@@ -217,10 +362,11 @@ def sweep(a):
     # stage form and tile size from the autotuner, timed on the whole batch: a
     # sample of the instances fits the 256 MB Infinity Cache better than the
     # batch does and can rank the forms the other way round
-    from fvhyper_amd import autotune
+    t_tune = time.perf_counter()
     sub = list(range(0, len(meshes), a.tune_every))
     tuned = autotune([meshes[k] for k in sub], [cfgs[k] for k in sub], np.concatenate([q0[k] for k in sub]),
-                     precision=a.precision, steps=3)
+                     precision=a.precision, steps=2)
+    t_tune = time.perf_counter() - t_tune
     b = Batch(meshes, cfgs, precision=a.precision, flux_mode=tuned["flux_mode"], tile_cells=tuned["tile_cells"])
     b.set_state(b.join(q0))
     b.apply_bounds()
@@ -238,73 +384,157 @@ def sweep(a):
     b.synchronize()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
-    if not np.all(np.isfinite(b.get_state())):
+    ev_steps = max(2, a.steps // 10)
+    kt = event_times(b, ev_steps)
+    qb = b.split(b.get_state())
+    if not all(np.all(np.isfinite(x)) for x in qb):
         print("non-finite batch state after the timed steps", file=sys.stderr)
         sys.exit(3)
-    # kernel timing (events, eager launches) over a few more steps
-    b.use_graphs(False)
-    b.profile(True)
-    b.advance(max(2, a.steps // 10))
-    b.synchronize()
-    kt = b.kernel_times()
-    b.profile(False)
-    # the same instances one by one
+    # error against the FP64 oracle after every step the batch took, on the
+    # 8 smallest instances and the median one
+    total = a.warmup + a.steps + ev_steps
+    order = sorted(range(len(items)), key=lambda k: meshes[k].n_real)
+    sample = order[:8] + [order[len(order) // 2]]
+    errs = []
+    for k in sample:
+        qo, *_ = O.Mesh(items[k][0]).run(cfgs[k], q0[k], total)
+        errs.append(rel_err(qo, qb[k]))
+    # the same instances one by one: the library's default per-mesh path
     k_pm = a.per_mesh_steps or a.steps
-    solvers = []
-    for m, c, q in zip(meshes, cfgs, q0):
-        x = Solver(m, c, precision=a.precision, flux_mode="tile")
-        x.set_state(q)
-        x.apply_bounds()
-        x.advance(2)
-        x.synchronize()
-        solvers.append(x)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for x in solvers:
-        x.advance(k_pm)
-        x.synchronize()
-    secs_pm = time.perf_counter() - t0
-    calibrate_copy(a)
+
+    def per_mesh(flux_mode, graphs, sync_each):
+        solvers = []
+        for m, c, q in zip(meshes, cfgs, q0):
+            x = Solver(m, c, precision=a.precision, flux_mode=flux_mode)
+            x.set_state(q)
+            x.apply_bounds()
+            x.use_graphs(graphs)
+            x.advance(2)
+            x.synchronize()
+            solvers.append(x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for x in solvers:
+            x.advance(k_pm)
+            if sync_each:
+                x.synchronize()
+        for x in solvers:
+            x.synchronize()
+        dt = time.perf_counter() - t
+        del solvers
+        return C * stages * k_pm / dt / 1e6, dt
+
+    v_pm, s_pm = per_mesh("auto", True, True)
+    v_old, s_old = per_mesh("tile", False, True)
     value = C * stages * a.steps / secs / 1e6
-    value_pm = C * stages * k_pm / secs_pm / 1e6
-    sb = 4 if a.precision == "f32" else 8
-    ab = algorithmic_bytes(C, E, B, sb, sb)
-    roof = None
+    sb, gb = state_bytes(a.precision)
+    ab = algorithmic_bytes(C, E, B, sb, gb)
     key = "stage" if kt.get("stage", {}).get("count", 0) else "flux"
-    if kt.get(key, {}).get("count", 0):
-        avg_ms = kt[key]["total_ms"] / kt[key]["count"]
-        nbytes = ab["stage"] if key == "stage" else ab["flux"]
-        ach = nbytes / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": ("k_stage (fused stage pass)" if key == "stage" else
-                           "k_flux (edge-flux gather + RK update + boundary)") + ", all instances",
-                "bytes_per_launch": nbytes, "avg_launch_us": round(avg_ms * 1e3, 2),
-                "kernels_us": {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}}
+    roof = stage_roofline(kt, ab["stage"] if key == "stage" else ab["flux"], key,
+                          ("k_stage (fused stage pass)" if key == "stage" else
+                           "k_flux (edge-flux gather + RK update + boundary)") + ", all instances")
+    if roof is not None:
         achievable(roof)
         tr = pmc_traffic("sweep", a.precision, key) if a.instances == 256 else None
         if tr is not None:
             roof["traffic"] = tr["bytes"]
             roof["traffic_source"] = tr["source"]
     out = {
-        "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == "f64" else "f32", "data": "synthetic",
+        "value": round(value, 2), "unit": "Mcell-stage/s", "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 4),
+        "dtype": "f32" if a.precision == "f32" else "f64",
         "config": {"workload": "naca0012_sweep", "instances": len(meshes), "cells": C, "edges": E,
                    "boundary_edges": B, "cells_range": [5000, 200000], "mach_range": [0.5, 0.85],
                    "aoa_deg_range": [-2.0, 4.0], "order": 2, "limiter": "michalak", "dt": "local",
-                   "rk_stages": stages, "precision": a.precision, "residual_sums": "f64",
-                   "launch": "batched, hipGraph per step", "parallelism": "single_gpu",
+                   "rk_stages": stages, "precision": a.precision,
+                   "state": "f32" if a.precision != "f64" else "f64",
+                   "arithmetic": "f32" if a.precision == "f32" else "f64", "residual_sums": "f64",
+                   "launch": "batched (one launch per stage for all instances), hipGraph per step",
                    "autotuned": {"flux_mode": tuned["flux_mode"], "tile_cells": tuned["tile_cells"],
+                                 "seconds": round(t_tune, 1),
                                  "candidates_ms_per_step": [list(c) for c in tuned["candidates"]]}},
-        "per_mesh": {"value": round(value_pm, 2), "steps": k_pm, "seconds": round(secs_pm, 3),
-                     "launch": "one solver per instance (two-pass tiled stage kernels), eager, "
-                               "synchronised per instance"},
-        "speedup_vs_per_mesh": round(value / value_pm, 3), "setup_s": round(setup, 1),
-        "roofline": roof, "cpu_baseline": None,
+        "per_mesh": {"value": round(v_pm, 2), "steps": k_pm, "seconds": round(s_pm, 3),
+                     "launch": "one solver per instance, the library's default form (fused stage pass, "
+                               "automatic tiles), each step replayed from its HIP graph, instances one after "
+                               "the other (synchronised per instance)"},
+        "per_mesh_two_pass_eager": {"value": round(v_old, 2), "seconds": round(s_old, 3),
+                                    "launch": "one solver per instance, two-pass tiled stage kernels, eager "
+                                              "launches, synchronised per instance"},
+        "speedup_vs_per_mesh": round(value / v_pm, 3),
+        "speedup_vs_per_mesh_two_pass_eager": round(value / v_old, 3),
+        "rel_err_vs_fp64": {"max": float(max(errs)), "instances": len(sample), "steps": total,
+                            "cells": [int(meshes[k].n_real) for k in sample],
+                            "measure": "max_ik |a-b| / max(|a_ik|, max_i |a_.k|) against the FP64 oracle"},
+        "setup_s": round(setup, 1), "roofline": roof, "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    if with_cpu and not a.no_cpu:
+        info = cpu_info()
+        work = tempfile.mkdtemp(prefix="fvh_sweep_")
+        files = sweep_write_meshes(items, work)
+        out["cpu_baseline"] = sweep_cpu_baseline(items, files, work, a.sweep_cpu_steps, info["ranks"])
+    return out
+
+
+# ---------------------------------------------------------------------------
+# complex step at the adjoint's size, in CPU worker processes
+# ---------------------------------------------------------------------------
+ADJ_N_PER_UNIT = 220          # 243,936 cells
+ADJ_DIR_SEED = 20240606
+ADJ_SCALE = (1.0, 3.0, 3.0, 10.0)
+
+
+def adj_direction(n_cells, n_real, k):
+    """Direction k of the adjoint checks: seeded normal values scaled per
+    variable, zero on the ghost cells (run() overwrites them)."""
+    import numpy as np
+    v = np.random.default_rng(ADJ_DIR_SEED + k).standard_normal((n_cells, 4)) * np.array(ADJ_SCALE)
+    v[n_real:] = 0.0
+    return v.ravel()
+
+
+def cstep_worker(a):
+    """CPU only (no HIP call): the complex-step derivative of J along the
+    directions in --cstep-worker on the adjoint workload, from the state and
+    weights the parent wrote to --cstep-dir; prints one JSON line per
+    direction."""
+    import numpy as np
+    O = _oracle()
+    from fvhyper_amd import workloads
+    m, cfg, _ = workloads.forward_step(n_per_unit=ADJ_N_PER_UNIT)
+    om = O.Mesh(m)
+    q0 = np.load(os.path.join(a.cstep_dir, "q0.npy"))
+    w = np.load(os.path.join(a.cstep_dir, "w.npy"))
+    for k in [int(x) for x in a.cstep_worker.split(",")]:
+        t0 = time.perf_counter()
+        J, dJ = O.cstep_derivative(om, cfg, q0, adj_direction(om.n_cells, om.n_real, k), a.adj_steps, wx=w)
+        print(json.dumps({"dir": k, "J": J, "dJ": dJ, "seconds": time.perf_counter() - t0}), flush=True)
+
+
+def start_cstep_workers(a, q0, w, n_dirs, procs):
+    """Spawn `procs` CPU worker processes (lowest priority) for n_dirs
+    directions; returns the handles."""
+    import numpy as np
+    d = tempfile.mkdtemp(prefix="fvh_cstep_")
+    np.save(os.path.join(d, "q0.npy"), q0)
+    np.save(os.path.join(d, "w.npy"), w)
+    groups = [list(range(i, n_dirs, procs)) for i in range(min(procs, n_dirs))]
+    out = []
+    for g in groups:
+        cmd = ["nice", "-n", "19", sys.executable, os.path.abspath(__file__), "--cstep-worker",
+               ",".join(str(k) for k in g), "--cstep-dir", d, "--adj-steps", str(a.adj_steps)]
+        out.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+    return out
+
+
+def collect_cstep_workers(procs, timeout=900):
+    res = []
+    for p in procs:
+        o, e = p.communicate(timeout=timeout)
+        if p.returncode != 0:
+            print("complex-step worker failed: %s" % e[-1000:], file=sys.stderr)
+            continue
+        res += [json.loads(l) for l in o.strip().splitlines() if l.startswith("{")]
+    return sorted(res, key=lambda r: r["dir"])
 
 
 def adjoint(a):
@@ -313,17 +543,15 @@ def adjoint(a):
     243,936 cells) from a seeded smooth perturbation of the inflow, with
     respect to every initial cell state and, in a second timed run, also to
     every node coordinate.  Reports the adjoint/forward cost ratios, the
-    checkpoint memory, and the agreement with central differences (eps 1e-6
-    and 1e-8) on --adj-dirs random directions and with the complex-step
-    derivative of the CPU restatement on a smaller mesh (states and interior
-    node coordinates)."""
+    checkpoint memory and central differences (eps 1e-6 and 1e-8) on
+    --adj-dirs directions.  Returns the result dict (with the gradient, for
+    the complex-step check of the CPU workers)."""
     import numpy as np
     import torch
 
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from fvhyper_amd import Mesh, Solver, workloads
 
-    m, cfg, ic = workloads.forward_step(n_per_unit=220)
+    m, cfg, ic = workloads.forward_step(n_per_unit=ADJ_N_PER_UNIT)
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg)
     s.use_graphs(True)           # forward steps replayed from HIP graphs, in both runs
@@ -343,89 +571,249 @@ def adjoint(a):
     t0 = time.perf_counter()
     _, _, gxy, _ = s.adjoint(q0, N, wx=w, mem_cap_bytes=8e9, nodes=True)
     t_adj_nodes = time.perf_counter() - t0
-    rng = np.random.default_rng(20240606)
     rel6, rel8 = [], []
-    for _ in range(a.adj_dirs):
-        v = rng.standard_normal((pm.n_cells, 4)) * np.array([1.0, 3.0, 3.0, 10.0])
-        v[pm.n_real:] = 0.0
-        v = v.ravel()
+    for k in range(a.adj_dirs):
+        v = adj_direction(pm.n_cells, pm.n_real, 1000 + k)
         ad = float(grad @ v)
         for eps, out in ((1e-6, rel6), (1e-8, rel8)):
             fd = (s.objective(q0 + eps * v, N, wx=w) - s.objective(q0 - eps * v, N, wx=w)) / (2 * eps)
             out.append(abs(ad - fd) / max(abs(fd), 1e-300))
-    # complex step of the CPU restatement, on a smaller mesh (CPU time)
     cs = None
-    try:
-        import oracle as O
-        mc, cfgc, _ = workloads.forward_step(n_per_unit=40)
-        pmc = Mesh.from_arrays(mc)
-        omc = O.Mesh(mc)
-        wc = workloads.step_face_weights(pmc)
-        qc = workloads.bump_state(pmc)
-        Nc = min(N, 40)
-        _, gc, _ = Solver(pmc, cfgc).adjoint(qc, Nc, wx=wc)
-        errs = []
-        for _ in range(4):
-            v = rng.standard_normal((pmc.n_cells, 4)) * np.array([1.0, 3.0, 3.0, 10.0])
-            v[pmc.n_real:] = 0.0
-            v = v.ravel()
-            _, dJ = O.cstep_derivative(omc, cfgc, qc, v, Nc, wx=wc)
-            errs.append(abs(float(gc @ v) - dJ) / abs(dJ))
-        # node coordinates: interior displacements of a tenth of the spacing
-        _, _, gxyc, _ = Solver(pmc, cfgc).adjoint(qc, Nc, wx=wc, nodes=True)
-        inner = np.ones(mc.x.size, dtype=bool)
-        inner[np.asarray(mc.bnd).ravel()] = False
-        nerrs = []
-        for _ in range(2):
-            dx = np.where(inner, rng.standard_normal(mc.x.size), 0.0) * 0.1 / 40
-            dy = np.where(inner, rng.standard_normal(mc.x.size), 0.0) * 0.1 / 40
-            _, dJ = O.cstep_derivative_nodes(omc, cfgc, qc, Nc, dx=dx, dy=dy, wx=wc)
-            nerrs.append(abs(float(gxyc[:, 0] @ dx + gxyc[:, 1] @ dy) - dJ) / abs(dJ))
-        cs = {"cells": pmc.n_real, "steps": Nc, "directions": 4, "max_rel_err": max(errs),
-              "node_directions": 2, "node_max_rel_err": max(nerrs)}
-    except Exception as e:  # pragma: no cover - reported
-        print("complex-step check skipped: %s" % e, file=sys.stderr)
     C = pm.n_real
     stages = len(cfg["alpha"])
-    out = {
-        "metric": "Mcell-stage/s", "value": round(C * stages * N / t_fwd / 1e6, 2), "unit": "Mcell-stage/s",
-        "n_gpus": 1, "steps": N, "warmup": 2, "ms_per_step": round(t_fwd / N * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+    return {
+        "value": round(C * stages * N / t_fwd / 1e6, 2), "unit": "Mcell-stage/s", "steps": N,
+        "ms_per_step": round(t_fwd / N * 1e3, 4), "dtype": "f64",
         "config": {"workload": "forward_step_adjoint", "cells": C, "steps": N, "stages": N * stages,
-                   "objective": "pressure force on the step face", "parallelism": "single_gpu"},
-        "adjoint": {"J": J, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
-                    "adjoint_over_forward": round(t_adj / t_fwd, 3),
-                    "with_node_gradient_s": round(t_adj_nodes, 4),
-                    "with_node_gradient_over_forward": round(t_adj_nodes / t_fwd, 3),
-                    "node_gradient_norm": float(np.linalg.norm(gxy)), "snaps": st["snaps"],
-                    "forward_steps_executed": st["forward_steps"], "recorded_steps": st["recorded_steps"],
-                    "library_ms": {"forward_sweep": round(st["forward_ms"], 2), "total": round(st["total_ms"], 2)},
-                    "checkpoint_GB": round(st["checkpoint_bytes"] / 1e9, 3),
-                    "work_GB": round(st["work_bytes"] / 1e9, 3),
-                    "fd_eps_1e-6": None if not rel6 else {
-                        "directions": len(rel6), "median_rel_err": float(np.median(rel6)),
-                        "max_rel_err": float(np.max(rel6)), "frac_below_1e-4": float(np.mean(np.array(rel6) < 1e-4))},
-                    "fd_eps_1e-8": None if not rel8 else {
-                        "directions": len(rel8), "median_rel_err": float(np.median(rel8)),
-                        "max_rel_err": float(np.max(rel8))},
-                    "complex_step": cs},
-        "roofline": None, "cpu_baseline": None,
+                   "objective": "pressure force on the step face", "checkpoint_cap_GB": 8.0},
+        "J": J, "forward_s": round(t_fwd, 4), "adjoint_s": round(t_adj, 4),
+        "adjoint_over_forward": round(t_adj / t_fwd, 3),
+        "with_node_gradient_s": round(t_adj_nodes, 4),
+        "with_node_gradient_over_forward": round(t_adj_nodes / t_fwd, 3),
+        "node_gradient_norm": float(np.linalg.norm(gxy)), "snaps": st["snaps"],
+        "forward_steps_executed": st["forward_steps"], "recorded_steps": st["recorded_steps"],
+        "library_ms": {"forward_sweep": round(st["forward_ms"], 2), "total": round(st["total_ms"], 2)},
+        "checkpoint_GB": round(st["checkpoint_bytes"] / 1e9, 3), "work_GB": round(st["work_bytes"] / 1e9, 3),
+        "fd_eps_1e-6": None if not rel6 else {
+            "directions": len(rel6), "median_rel_err": float(np.median(rel6)),
+            "max_rel_err": float(np.max(rel6)), "frac_below_1e-4": float(np.mean(np.array(rel6) < 1e-4))},
+        "fd_eps_1e-8": None if not rel8 else {
+            "directions": len(rel8), "median_rel_err": float(np.median(rel8)), "max_rel_err": float(np.max(rel8))},
+        "complex_step": cs, "cpu_baseline": None, "_mesh": m, "_cfg": cfg, "_grad": grad,
+        "_n_cells": pm.n_cells, "_n_real": pm.n_real,
     }
+
+
+def step_workload(a, precision):
+    """The Mach-3 forward step (or another single-mesh workload): the timed
+    region, kernel durations and roofline.  Returns (result dict, mesh
+    arrays, cfg)."""
+    import numpy as np
+    import torch
+
+    from fvhyper_amd import Mesh, Solver, workloads
+
+    if a.workload == "plate":
+        m, cfg, ic = workloads.flat_plate(smooth=a.smooth)
+    elif a.workload == "airfoil":
+        m, cfg, ic = workloads.naca_airfoil(a.cells)
+        if a.smooth:
+            cfg["smooth"] = True
+        if a.cfl > 0:
+            cfg["cfl"] = a.cfl
+    else:
+        m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
+    if a.stage_threads:
+        cfg["stage_threads"] = a.stage_threads
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg, precision=precision, flux_mode=a.flux_mode, tile_cells=a.tile_cells)
+    q0 = workloads.initial_state(pm, ic)
+    s.set_state(q0)
+    s.apply_bounds()
+    C, E, B = pm.n_real, pm.n_edges, pm.n_bnd
+    stages = len(cfg["alpha"])
+    # timed region: the steps replayed from HIP graphs, no per-launch events
+    s.use_graphs(True)
+    s.advance(a.warmup)
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.advance(a.steps)
+    s.synchronize()
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    # kernel durations: HIP events around every launch on the solver's
+    # stream, over a further window of the same run (eager launches)
+    ev_steps = max(20, a.steps // 10)
+    kt = {} if a.no_events else event_times(s, ev_steps)
+    q = s.get_state()
+    if not np.all(np.isfinite(q)):
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+    calibrate_copy(a)
+    value = C * stages * a.steps / secs / 1e6
+    sb, gb = state_bytes(precision)
+    ab = algorithmic_bytes(C, E, B, sb, gb)
+    roof = None
+    if kt and kt.get("stage", {}).get("count", 0):
+        roof = stage_roofline(kt, ab["stage"])
+    elif kt and a.workload == "plate":
+        # viscous two-pass stage: the gradient pass also writes gx, gy; the
+        # flux pass reads them and the cell centres (face gradients)
+        V = 4
+        H = 2 * E - B
+        per = {"gradlim": ab["gradlim"] + C * 2 * V * sb, "flux": ab["flux"] + C * (2 * V * sb + 2 * gb)}
+        if a.smooth:
+            per["smooth"] = (C * (2 * V * sb + 4) + H * 8 + C * (4 * V * sb + 4 + 2 * sb) + H * 8) // 2
+        tot = {k: kt[k]["total_ms"] for k in per if kt.get(k, {}).get("count", 0)}
+        key = max(tot, key=tot.get)
+        roof = stage_roofline(kt, per[key], key, "k_" + key + " (viscous two-pass stage" +
+                              (", residual smoothing" if a.smooth else "") + ")")
+    elif kt and kt.get("flux", {}).get("count", 0):
+        roof = stage_roofline(kt, ab["flux"], "flux", "k_flux (edge-flux gather + RK update + boundary)")
+    if roof is not None:
+        achievable(roof)
+        roof["events_window_steps"] = ev_steps
+        tr = pmc_traffic(a.workload, precision, "stage" if "stage" in roof["kernel"] else "flux") \
+            if a.workload != "plate" else None
+        if tr is not None and (a.workload != "step" or a.n_per_unit == 440) and (
+                a.workload != "airfoil" or a.cells == 2_000_000):
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_source"] = tr["source"]
+            roof["frac_traffic"] = round(tr["bytes"] / (roof["avg_launch_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        stage_ms = sum(v["total_ms"] for v in kt.values()) / (ev_steps * stages)
+        roof["stage_achieved"] = round(ab["stage"] / (stage_ms * 1e-3) / 1e9, 1)
+    conv = None
+    if a.converge:
+        s.set_state(q0)
+        conv = converge(s, C, stages, 1e-6, a.stop_var)
+    if a.workload == "plate":
+        wl = {"workload": "laminar_flat_plate", "cells": C, "edges": E, "boundary_edges": B, "order": 2,
+              "limiter": "michalak", "flux": "roe + laminar viscous (diffusive face gradients)",
+              "residual_smoothing": a.smooth, "rk_stages": stages, "cfl": cfg["cfl"], "dt": "local viscous",
+              "precision": precision, "stage_form": "tile"}
+    elif a.workload == "airfoil":
+        wl = {"workload": "naca0012_transonic", "cells": C, "edges": E, "boundary_edges": B,
+              "order": 2, "limiter": "michalak", "mach": 0.8, "aoa_deg": 1.25, "rk_stages": stages,
+              "cfl": cfg["cfl"], "dt": "local", "precision": precision, "stage_form": a.flux_mode,
+              "residual_smoothing": bool(cfg.get("smooth"))}
+    else:
+        wl = {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
+              "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
+              "cfl": cfg["cfl"], "dt": "global", "precision": precision, "stage_form": a.flux_mode}
+    wl["parallelism"] = "single_gpu"
+    wl["tile_cells"] = s.tile_cells
+    wl["stage_layout"] = s.stage_info
+    out = {"value": round(value, 2), "ms_per_step": round(secs / a.steps * 1e3, 4),
+           "dtype": "f32" if precision == "f32" else "f64", "config": wl, "roofline": roof}
+    if precision == "f64":
+        out["parity"] = "FP64: bit-identical to the oracle, itself bit-identical to the reference (GPU tests)"
+    if conv is not None:
+        out["converge"] = conv
+    return out, m, cfg
+
+
+def line(result, a):
+    """The top-level JSON fields around a workload result."""
+    out = {"metric": "Mcell-stage/s", "value": result["value"], "unit": "Mcell-stage/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": result["ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": result["dtype"], "data": "synthetic"}
+    for k, v in result.items():
+        if k not in out and not k.startswith("_"):
+            out[k] = v
+    return out
+
+
+def headline(a):
+    """The default line: the step workload at top level, the sweep and the
+    adjoint as sub-objects, CPU baselines after the GPU work (the complex-step
+    workers run during it)."""
+    import numpy as np
+    import torch
+
+    from fvhyper_amd import Mesh, workloads
+    t_start = time.perf_counter()
+    info = cpu_info()
+    # the CPU workers (complex step, sweep meshes for the reference runs)
+    # start before any HIP call of this process
+    sweep_dir = tempfile.mkdtemp(prefix="fvh_sweep_")
+    writers = [] if a.no_cpu else start_mesh_writers(a, sweep_dir, 4)
+    workers = []
+    if a.cstep_dirs > 0:
+        ma, _, _ = workloads.forward_step(n_per_unit=ADJ_N_PER_UNIT)
+        pma = Mesh.from_arrays(ma)
+        workers = start_cstep_workers(a, workloads.bump_state(pma), workloads.step_face_weights(pma), a.cstep_dirs,
+                                      min(a.cstep_dirs, info["ranks"]))
+    torch.cuda.set_device(0)
+    step_res, m, cfg = step_workload(a, "f64")
+    sw = sweep(argparse.Namespace(**dict(vars(a), precision=a.sweep_precision)), with_cpu=False)
+    adj = adjoint(a)
+    t_gpu = time.perf_counter() - t_start
+    # the workers' results, then the CPU baselines on the freed cores
+    cs = collect_cstep_workers(workers) if workers else []
+    if cs:
+        adj["complex_step"] = adjoint_complex_step_check(adj, cs)
+    if not a.no_cpu:
+        step_res["cpu_baseline"] = cpu_baseline(m, cfg, a.cpu_steps, "step", info["ranks"])
+        items = workloads.naca_sweep(a.instances)
+        for p in writers:
+            _, e = p.communicate(timeout=900)
+            if p.returncode != 0:
+                raise RuntimeError("sweep mesh writer failed: %s" % e[-1000:])
+        sw["cpu_baseline"] = sweep_cpu_baseline(items, sweep_mesh_names(items), sweep_dir, a.sweep_cpu_steps,
+                                                info["ranks"])
+        adj["cpu_baseline"] = cpu_baseline(adj["_mesh"], adj["_cfg"], a.cpu_steps, "step", info["ranks"])
+        if adj["cpu_baseline"] is not None:
+            adj["cpu_baseline"]["note"] = ("the reference's forward solve of the adjoint's mesh (the reference "
+                                           "has no adjoint); one complex-step direction costs cpu_s_per_direction")
+    out = line(step_res, a)
+    out["sweep"] = sw
+    out["adjoint"] = {k: v for k, v in adj.items() if not k.startswith("_")}
+    out["headline"] = {"batched_mcell_stage_s": sw["value"], "per_mesh_mcell_stage_s": sw["per_mesh"]["value"],
+                       "batched_vs_per_mesh": sw["speedup_vs_per_mesh"],
+                       "sweep_rel_err_vs_fp64": sw["rel_err_vs_fp64"]["max"],
+                       "adjoint_over_forward": adj["adjoint_over_forward"],
+                       "adjoint_complex_step_max_rel_err": (adj["complex_step"] or {}).get("max_rel_err"),
+                       "cpu": info["model"], "cpu_cores_used": info["ranks"]}
+    out["wall_s"] = {"gpu_part": round(t_gpu, 1), "total": round(time.perf_counter() - t_start, 1)}
     print(json.dumps(out))
+
+
+def adjoint_complex_step_check(adj, cs):
+    """Compare the complex-step derivatives of the CPU workers with the
+    adjoint gradient along the same directions."""
+    import numpy as np
+    grad, n_cells, n_real = adj["_grad"], adj["_n_cells"], adj["_n_real"]
+    errs = [abs(float(grad @ adj_direction(n_cells, n_real, r["dir"])) - r["dJ"]) / abs(r["dJ"]) for r in cs]
+    return {"cells": n_real, "steps": adj["steps"], "directions": len(errs), "max_rel_err": float(max(errs)),
+            "median_rel_err": float(np.median(errs)),
+            "J_cpu_minus_gpu": float(max(abs(r["J"] - adj["J"]) for r in cs)),
+            "cpu_s_per_direction": round(float(np.median([r["seconds"] for r in cs])), 1),
+            "where": "CPU worker processes, the templated restatement on complex numbers (h = 1e-30), run while "
+                     "the GPU measured the other workloads"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default=None, choices=["f64", "f32", "f32_f64"],
-                    help="default: f64 (f32 for the sweep, whose state is FP32)")
-    ap.add_argument("--workload", default="step", choices=["step", "airfoil", "sweep", "adjoint", "plate"])
+                    help="single workloads: default f64 (f32_f64 for the sweep: FP32 states, FP64 arithmetic)")
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "step", "airfoil", "sweep", "adjoint", "plate"])
+    ap.add_argument("--sweep-precision", default="f32_f64", choices=["f64", "f32", "f32_f64"],
+                    help="headline: precision of the sweep")
     ap.add_argument("--smooth", action="store_true", help="plate, airfoil: with residual smoothing")
     ap.add_argument("--cfl", type=float, default=0.0, help="airfoil: CFL number instead of the workload's 2")
     ap.add_argument("--adj-steps", type=int, default=80, help="adjoint: RK5 steps of the objective's run")
-    ap.add_argument("--adj-dirs", type=int, default=64, help="adjoint: random directions checked")
+    ap.add_argument("--adj-dirs", type=int, default=64, help="adjoint: random directions of the central differences")
+    ap.add_argument("--cstep-dirs", type=int, default=16,
+                    help="headline: directions of the complex-step check at the adjoint's size (CPU workers)")
+    ap.add_argument("--cstep-worker", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cstep-dir", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--mesh-writer", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--mesh-dir", default="", help=argparse.SUPPRESS)
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
     ap.add_argument("--per-mesh-steps", type=int, default=0,
                     help="sweep: steps of the one-by-one baseline (default: --steps)")
@@ -443,160 +831,51 @@ def main():
                     help="airfoil: also run from the freestream until the density residual drops 6 orders")
     ap.add_argument("--stop-var", type=int, default=0, help="with --converge: 0 density, -1 max over variables")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--sweep-cpu-steps", type=int, default=2, help="steps of each reference process of the sweep")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-launch events")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run a plain torch copy of this size (for rocprof counter scaling)")
     a = ap.parse_args()
+    if a.cstep_worker:
+        return cstep_worker(a)
+    if a.mesh_writer:
+        return mesh_writer_worker(a)
     if a.precision is None:
-        a.precision = "f32" if a.workload == "sweep" else "f64"
+        a.precision = "f32_f64" if a.workload == "sweep" else "f64"
     if a.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (each GPU box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "headline":
+        return headline(a)
+    if a.workload == "adjoint":
+        workers = []
+        if a.cstep_dirs > 0:
+            from fvhyper_amd import Mesh, workloads
+            ma, _, _ = workloads.forward_step(n_per_unit=ADJ_N_PER_UNIT)
+            pma = Mesh.from_arrays(ma)
+            workers = start_cstep_workers(a, workloads.bump_state(pma), workloads.step_face_weights(pma),
+                                          a.cstep_dirs, min(a.cstep_dirs, cpu_info()["ranks"]))
+        import torch
+        torch.cuda.set_device(0)
+        r = adjoint(a)
+        cs = collect_cstep_workers(workers) if workers else []
+        if cs:
+            r["complex_step"] = adjoint_complex_step_check(r, cs)
+        print(json.dumps(line({k: v for k, v in r.items() if not k.startswith("_")}, a)))
+        return
 
-    import numpy as np
     import torch
-
-    from fvhyper_amd import Mesh, Solver, workloads
-
     torch.cuda.set_device(0)
     if a.workload == "sweep":
-        return sweep(a)
-    if a.workload == "adjoint":
-        return adjoint(a)
-    if a.workload == "step":
-        m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
-    elif a.workload == "plate":
-        m, cfg, ic = workloads.flat_plate(smooth=a.smooth)
-    else:
-        m, cfg, ic = workloads.naca_airfoil(a.cells)
-        if a.smooth:
-            cfg["smooth"] = True
-        if a.cfl > 0:
-            cfg["cfl"] = a.cfl
-    if a.stage_threads:
-        cfg["stage_threads"] = a.stage_threads
-    pm = Mesh.from_arrays(m)
-    s = Solver(pm, cfg, precision=a.precision, flux_mode=a.flux_mode, tile_cells=a.tile_cells)
-    q0 = workloads.initial_state(pm, ic)
-    s.set_state(q0)
-    s.apply_bounds()
-    C, E, B = pm.n_real, pm.n_edges, pm.n_bnd
-    stages = len(cfg["alpha"])
-
-    # timed region: the steps replayed from HIP graphs, no per-launch events
-    s.use_graphs(True)
-    s.advance(a.warmup)
-    s.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s.advance(a.steps)
-    s.synchronize()
-    torch.cuda.synchronize()
-    secs = time.perf_counter() - t0
-    # kernel durations: HIP events around every launch on the solver's
-    # stream, over a further window of the same run (eager launches)
-    kt = {}
-    if not a.no_events:
-        s.use_graphs(False)
-        s.profile(True)
-        s.advance(max(20, a.steps // 10))
-        s.synchronize()
-        kt = s.kernel_times()
-        s.profile(False)
-    q = s.get_state()
-    if not np.all(np.isfinite(q)):
-        print("non-finite state after the timed steps", file=sys.stderr)
-        sys.exit(3)
-
-    calibrate_copy(a)
-
-    value = C * stages * a.steps / secs / 1e6
-    sb = 4 if a.precision == "f32" else 8
-    ab = algorithmic_bytes(C, E, B, sb, sb)
-    roof = None
-    if kt and kt.get("stage", {}).get("count", 0):
-        # fused form: one launch is one whole stage, priced at the stage's bytes
-        avg_ms = kt["stage"]["total_ms"] / kt["stage"]["count"]
-        ach = ab["stage"] / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_stage (fused gradient + limiter + flux + RK update + boundary, one launch per stage)",
-                "bytes_per_launch": ab["stage"], "avg_launch_us": round(avg_ms * 1e3, 2)}
-    elif kt and a.workload == "plate":
-        # viscous two-pass stage: the gradient pass also writes gx, gy; the
-        # flux pass reads them and the cell centres (face gradients)
-        V = 4
-        H = 2 * E - B
-        per = {"gradlim": ab["gradlim"] + C * 2 * V * sb,
-               "flux": ab["flux"] + C * (2 * V * sb + 2 * sb)}
-        if a.smooth:
-            # two sweeps: qt in and out, the half-edge lists; the second also
-            # finishes the stage (q0 read, q written, dt, area)
-            per["smooth"] = (C * (2 * V * sb + 4) + H * 8 + C * (4 * V * sb + 4 + 2 * sb) + H * 8) // 2
-        tot = {k: kt[k]["total_ms"] for k in per if kt.get(k, {}).get("count", 0)}
-        key = max(tot, key=tot.get)
-        avg_ms = kt[key]["total_ms"] / kt[key]["count"]
-        ach = per[key] / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_" + key + " (viscous two-pass stage" + (", residual smoothing" if a.smooth else "") + ")",
-                "bytes_per_launch": per[key], "avg_launch_us": round(avg_ms * 1e3, 2),
-                "others": {k: {"bytes_per_launch": per[k],
-                               "avg_launch_us": round(kt[k]["total_ms"] / kt[k]["count"] * 1e3, 2),
-                               "frac": round(per[k] / (kt[k]["total_ms"] / kt[k]["count"] * 1e-3) / 1e9
-                                             / HBM_PEAK_GBS, 4)} for k in tot if k != key}}
-    elif kt and kt.get("flux", {}).get("count", 0):
-        avg_ms = kt["flux"]["total_ms"] / kt["flux"]["count"]
-        ach = ab["flux"] / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_flux (edge-flux gather + RK update + boundary)",
-                "bytes_per_launch": ab["flux"], "avg_launch_us": round(avg_ms * 1e3, 2)}
-    if roof is not None:
-        achievable(roof)
-        roof["events_window_steps"] = max(20, a.steps // 10)
-        tr = pmc_traffic(a.workload, a.precision, "stage" if "stage" in roof["kernel"] else "flux") \
-            if a.workload != "plate" else None
-        if tr is not None and (a.workload != "step" or a.n_per_unit == 440) and (
-                a.workload != "airfoil" or a.cells == 2_000_000):
-            roof["traffic"] = tr["bytes"]
-            roof["traffic_source"] = tr["source"]
-        stage_ms = sum(v["total_ms"] for v in kt.values()) / (roof["events_window_steps"] * stages)
-        roof["stage_achieved"] = round(ab["stage"] / (stage_ms * 1e-3) / 1e9, 1)
-        roof["kernels_us"] = {k: round(v["total_ms"] / v["count"] * 1e3, 2) for k, v in kt.items() if v["count"]}
-    conv = None
-    if a.converge:
-        s.set_state(q0)
-        conv = converge(s, C, stages, 1e-6, a.stop_var)
-    cpu = None if (a.no_cpu or a.workload == "plate") else cpu_baseline(m, cfg, a.cpu_steps, a.workload)
-    if a.workload == "plate":
-        wl = {"workload": "laminar_flat_plate", "cells": C, "edges": E, "boundary_edges": B, "order": 2,
-              "limiter": "michalak", "flux": "roe + laminar viscous (diffusive face gradients)",
-              "residual_smoothing": a.smooth, "rk_stages": stages, "cfl": cfg["cfl"], "dt": "local viscous",
-              "precision": a.precision, "stage_form": "tile", "parallelism": "single_gpu"}
-    elif a.workload == "step":
-        wl = {"workload": "mach3_forward_step", "cells": C, "edges": E, "boundary_edges": B,
-              "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
-              "cfl": cfg["cfl"], "dt": "global", "precision": a.precision, "stage_form": a.flux_mode,
-              "parallelism": "single_gpu"}
-    else:
-        wl = {"workload": "naca0012_transonic", "cells": C, "edges": E, "boundary_edges": B,
-              "order": 2, "limiter": "michalak", "mach": 0.8, "aoa_deg": 1.25, "rk_stages": stages,
-              "cfl": cfg["cfl"], "dt": "local", "precision": a.precision, "stage_form": a.flux_mode,
-              "residual_smoothing": bool(cfg.get("smooth")), "parallelism": "single_gpu"}
-    out = {
-        "metric": "Mcell-stage/s", "value": round(value, 2), "unit": "Mcell-stage/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(secs / a.steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.precision == "f64" else "f32", "data": "synthetic",
-        "config": wl, "roofline": roof, "cpu_baseline": cpu,
-    }
-    wl["tile_cells"] = s.tile_cells
-    if conv is not None:
-        out["converge"] = conv
-    print(json.dumps(out))
+        r = sweep(a)
+        print(json.dumps(line(r, a)))
+        return
+    r, m, cfg = step_workload(a, a.precision)
+    r["cpu_baseline"] = None if (a.no_cpu or a.workload == "plate") else \
+        cpu_baseline(m, cfg, a.cpu_steps, a.workload, cpu_info()["ranks"])
+    print(json.dumps(line(r, a)))
 
 
 if __name__ == "__main__":

@@ -1737,6 +1737,24 @@ int32_t fvh_batch_create(int32_t n, const fvh_mesh* const* meshes, const fvh_con
 
 int32_t fvh_batch_size(const fvh_solver* s) { return s ? s->n_inst : 0; }
 
+int32_t fvh_solver_stage_info(const fvh_solver* s, int64_t* info) {
+    if (!s || !info) { set_error("null argument"); return FVH_E_ARG; }
+    for (int k = 0; k < 8; ++k) info[k] = 0;
+    info[0] = s->fused ? 1 : 0;
+    info[1] = s->n_rt;
+    info[6] = s->cfg.tile_cells ? s->cfg.tile_cells : kBlock;
+    if (!s->fused) return FVH_OK;
+    HaloView hv = halo_view(const_cast<fvh_solver*>(s));
+    const size_t lds = s->mixed ? stage_lds_bytes<double, float>(hv)
+                                : s->f32 ? stage_lds_bytes<float, float>(hv) : stage_lds_bytes<double, double>(hv);
+    info[2] = (int64_t)lds;
+    info[3] = stage_lanes(lds, s->cfg.stage_threads, s->x_mean_g);
+    info[4] = s->x_max_ext;
+    info[5] = s->x_max_edges;
+    info[7] = s->x_mean_g;
+    return FVH_OK;
+}
+
 int32_t fvh_solver_tile_cells(const fvh_solver* s) {
     if (!s) { set_error("null solver"); return FVH_E_ARG; }
     return s->cfg.tile_cells ? s->cfg.tile_cells : kBlock;

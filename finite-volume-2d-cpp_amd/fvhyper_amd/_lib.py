@@ -76,7 +76,7 @@ EXPORTS = [
     "fvh_solver_use_graphs", "fvh_batch_create", "fvh_batch_size", "fvh_batch_instance",
     "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
     "fvh_phase_gradients", "fvh_phase_limiters", "fvh_phase_time_derivatives", "fvh_phase_update_cells",
-    "fvh_phase_update_bounds", "fvh_phase_min_dt", "fvh_phase_residuals",
+    "fvh_phase_update_bounds", "fvh_phase_min_dt", "fvh_phase_residuals", "fvh_solver_stage_info",
 ]
 
 _lib = None
@@ -164,6 +164,7 @@ def load(path: str = LIB_PATH):
     L.fvh_solver_stage_bytes.argtypes = [vp]
     L.fvh_solver_stage_bytes.restype = C.c_double
     L.fvh_solver_tile_cells.argtypes = [vp]
+    L.fvh_solver_stage_info.argtypes = [vp, C.POINTER(C.c_int64)]
     L.fvh_solver_use_graphs.argtypes = [vp, C.c_int32]
     L.fvh_batch_create.argtypes = [C.c_int32, C.POINTER(vp), C.POINTER(Config), C.POINTER(vp)]
     L.fvh_batch_size.argtypes = [vp]

@@ -163,6 +163,15 @@ class Solver:
         self._h = h
         self._n_cells = mesh.n_cells
 
+    @property
+    def stage_info(self) -> Dict[str, int]:
+        """Layout of the fused stage pass (fvh_solver_stage_info)."""
+        a = (C.c_int64 * 8)()
+        check(_lib.load().fvh_solver_stage_info(self._h, a))
+        keys = ["fused", "tiles", "lds_bytes", "lanes", "max_local_cells", "max_tile_edges", "tile_cells",
+                "mean_tile_ring1_cells"]
+        return {k: int(v) for k, v in zip(keys, a)}
+
     def use_graphs(self, enable: bool = True) -> None:
         """Replay each RK step from a captured HIP graph."""
         check(_lib.load().fvh_solver_use_graphs(self._h, 1 if enable else 0))

@@ -318,6 +318,13 @@ double fvh_solver_stage_bytes(const fvh_solver* s);
  * counterpart: a tuning query. */
 int32_t fvh_solver_tile_cells(const fvh_solver* s);
 
+/* Layout of the fused stage pass (no reference counterpart: a tuning
+ * query).  info[0..7] = fused pass in use (0/1), real tiles, LDS bytes per
+ * workgroup, lanes per workgroup, largest local cell list (tile and two
+ * neighbour rings), most edges of a tile, cells per tile, mean tile plus
+ * real ring-1 cells. */
+int32_t fvh_solver_stage_info(const fvh_solver* s, int64_t* info);
+
 /* Capture each RK step (every kernel of it) in a HIP graph on first use and
  * replay it afterwards (one graph per state buffer, residual flag and run
  * options).  No reference counterpart: it removes per-launch host overhead
