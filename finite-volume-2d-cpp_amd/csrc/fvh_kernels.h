@@ -226,6 +226,7 @@ void launch_phase_faces(hipStream_t s, const MeshView<T>& m, const Q4<T>* q, con
 void launch_update_cells(hipStream_t s, double* q, const double* ql, const double* qt, const double* dt, double v,
                          int64_t n);
 constexpr int kArrayMinBlocks = 1024;
+constexpr int64_t kMinDtDeviceMax = (int64_t)1 << 31;   // values fvh_min_dt_device accepts
 int array_min_blocks(int64_t n);
 void launch_array_min(hipStream_t s, double* a, int64_t n, double* part);
 template <class T>

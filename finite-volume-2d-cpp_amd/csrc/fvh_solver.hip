@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1629,6 +1630,35 @@ int32_t adjoint_check(fvh_solver* s) {
 }
 
 // ---------------------------------------------------------------------------
+// Scratch memory of the entry point that allocates nothing
+// (fvh_min_dt_device): a file-scope device array, one copy per GPU, whose
+// address each device resolves once.  Every call on a GPU shares its copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kArrayMinBlocks];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int32_t scratch_address(double** out) {
+    int dev = 0;
+    FVH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range"); return FVH_E_HIP; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return FVH_E_HIP;
+        }
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FVH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // The phases one at a time on caller arrays (FP64, single instance)
 // ---------------------------------------------------------------------------
 int32_t phase_setup(fvh_solver* s) {
@@ -2140,6 +2170,22 @@ int32_t fvh_phase_min_dt(fvh_solver* s, double* dt, int64_t n) {
     launch_array_min(s->stream, (double*)s->ph[0].p, n, (double*)s->ph_part.p);
     FVH_HIP(hipGetLastError());
     return get_flat(s, s->ph[0].p, dt, n);
+}
+
+int32_t fvh_min_dt_device(double* dt, int64_t n, void* stream) {
+    if (!dt || n < 1) { set_error("bad argument"); return FVH_E_ARG; }
+    if (n > kMinDtDeviceMax) {
+        set_error("min_dt on a device array: at most " + std::to_string(kMinDtDeviceMax) + " values, got " +
+                  std::to_string(n));
+        return FVH_E_ARG;
+    }
+    double* part = nullptr;
+    int32_t rc = scratch_address(&part);
+    if (rc) return rc;
+    // every block overwrites its own slot: nothing to clear beforehand
+    launch_array_min((hipStream_t)stream, dt, n, part);
+    FVH_HIP(hipGetLastError());
+    return FVH_OK;
 }
 
 int32_t fvh_phase_residuals(fvh_solver* s, const double* qt, double* R) {

@@ -77,6 +77,7 @@ EXPORTS = [
     "fvh_batch_status", "fvh_batch_sums", "fvh_autotune", "fvh_objective", "fvh_adjoint",
     "fvh_phase_gradients", "fvh_phase_limiters", "fvh_phase_time_derivatives", "fvh_phase_update_cells",
     "fvh_phase_update_bounds", "fvh_phase_min_dt", "fvh_phase_residuals", "fvh_solver_stage_info",
+    "fvh_min_dt_device",
 ]
 
 _lib = None
@@ -183,6 +184,7 @@ def load(path: str = LIB_PATH):
     L.fvh_phase_update_bounds.argtypes = [vp, dp, dp, dp, dp]
     L.fvh_phase_min_dt.argtypes = [vp, dp, C.c_int64]
     L.fvh_phase_residuals.argtypes = [vp, dp, dp]
+    L.fvh_min_dt_device.argtypes = [vp, C.c_int64, vp]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

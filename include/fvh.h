@@ -266,6 +266,17 @@ int32_t fvh_phase_update_bounds(fvh_solver* s, double* q, const double* gx, cons
 /* min_dt (explicit.cpp:555-564): every one of the n values becomes their
  * minimum (NaN if dt[0] is NaN, other NaNs ignored, as std::min from dt[0]). */
 int32_t fvh_phase_min_dt(fvh_solver* s, double* dt, int64_t n);
+/* The same minimum on a device array, in place, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  dt: a device
+ * pointer to n doubles on the current device (at most 2^31 values);
+ * stream: a hipStream_t (NULL: the null stream); returns once the kernels
+ * are queued.  The per-block minima go to a file-scope device array of 1024
+ * doubles (8 KiB on each GPU the library runs on, held until the process
+ * ends) shared by every call on that GPU: calls that overlap in time on one
+ * GPU race. */
+int32_t fvh_min_dt_device(double* dt, int64_t n, void* stream);
+
 /* calc_residuals (explicit.cpp:481-552), one rank: R[k] = sqrt(sum over the
  * real cells of qt^2 A), summed in a fixed tree order (not ascending cell
  * order: equal to ~1e-15 relative). */
