@@ -122,6 +122,9 @@ struct PhysParams {
     double mu, pr, cp, r;   // laminar viscous constants (viscous flux and time steps)
     int positivity;   // first-order fallback for non-physical face states
     int diffusive;    // face gradients for the viscous flux
+    // per instance: set to 1 by any stage update that leaves a real cell with
+    // rho <= 0, p <= 0 or a non-finite value (null: no check)
+    uint32_t* unphysical;
 };
 
 // Launchers (implemented in fvh_kernels.hip).  All take the solver stream;

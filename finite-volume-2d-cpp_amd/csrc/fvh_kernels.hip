@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
 // ---------------------------------------------------------------------------
 template <class T, bool LIN, class S, class Inc>
 __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
-                                   int h0, int h1, const T (&qt)[4], const Q4<S>* __restrict__ q0,
+                                   int inst, int h0, int h1, const T (&qt)[4], const Q4<S>* __restrict__ q0,
                                    Q4<S>* __restrict__ qout, Inc inc_of,
                                    const T d, T alpha, bool want_rs,
                                    double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
@@ -320,6 +320,10 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
 #pragma unroll
     for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
     stq(qout + c, out);
+    // rho > 0 and p > 0, i.e. 2 rho E > |rho u|^2 for rho > 0 (false for NaN):
+    // a read-only check, the state is stored as computed
+    if (pp.unphysical && !(out.v[0] > T(0) && T(2) * out.v[0] * out.v[3] > out.v[1] * out.v[1] + out.v[2] * out.v[2]))
+        pp.unphysical[inst] = 1u;
     if (dbg_qt) {
         Q4<T> t;
 #pragma unroll
@@ -436,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_cell(MeshView<T> m, TileMap tm,
             qt_only[c] = t;
         } else {
             auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+            finish_cell<T, LIN>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                                 res_blk != nullptr, rs, dbg_qt);
         }
     }
@@ -518,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_tile(MeshView<T> m, TileMap tm,
             qt_only[c] = t;
         } else {
             auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+            finish_cell<T, LIN>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                                 res_blk != nullptr, rs, dbg_qt);
         }
     }
@@ -578,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, Ph
             qt_out[c] = t;
         } else {
             auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-            finish_cell<T, LIN>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+            finish_cell<T, LIN>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                                 res_blk != nullptr, rs, dbg_qt);
         }
     }
@@ -718,7 +722,7 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
             }
         }
         auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-        finish_cell<T, true>(m, pp, bcs, c, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
+        finish_cell<T, true>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
                              res_blk != nullptr, rs, rec.qt);
     }
     if (res_blk) {

@@ -642,6 +642,9 @@ int32_t read_msh(const char* path, fvh_mesh& out) {
             }
         }
     }
+    // a file without nodes or cells (empty, or not a mesh) is refused rather
+    // than read as an empty mesh
+    if (xs.empty() || cells.empty()) return fail("no nodes or no 2-D cells: not a mesh");
     // physical names of the boundary groups, in order of first use
     std::vector<std::string> names;
     std::map<int64_t, int32_t> slot_of_phys;

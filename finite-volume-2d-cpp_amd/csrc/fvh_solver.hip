@@ -108,6 +108,7 @@ struct fvh_solver {
     fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_r1_slot, x_he_loc;
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
+    fvh::DevBuf unphysical;                  // per instance: a stage left a non-physical cell state
     fvh::DevBuf dbg[4];
     fvh::DevBuf ph[6], ph_part;              // per-phase entry points: inputs and outputs, min_dt partials
     fvh::DevBuf qt_s[2];                     // residual smoothing: raw qt, first sweep's qt
@@ -242,6 +243,7 @@ PhysParams phys(const fvh_config& c) {
     p.cp = c.cp;
     p.r = c.r_gas;
     p.diffusive = (c.flags & FVH_FLAG_DIFFUSIVE) ? 1 : 0;
+    p.unphysical = nullptr;
     return p;
 }
 
@@ -357,7 +359,7 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
         s->bincr.alloc((size_t)(s->n_bnd + 1) * sizeof(Q4<T>)) ||
         s->blk_min.alloc((size_t)(s->n_all + 1) * 8) || s->res_blk.alloc((size_t)(s->n_rt + 1) * 32) ||
         s->sums_blk.alloc((size_t)(s->n_rt + 1) * 32) || s->sums.alloc((size_t)s->n_inst * 32) ||
-        s->ctrl.alloc((size_t)s->n_inst * sizeof(Ctrl))) {
+        s->ctrl.alloc((size_t)s->n_inst * sizeof(Ctrl)) || s->unphysical.alloc((size_t)s->n_inst * 4)) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
@@ -377,6 +379,7 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
             FVH_HIP(hipMemset(b.p, 0, b.bytes));
         }
     FVH_HIP(hipMemset(s->ctrl.p, 0, s->ctrl.bytes));
+    FVH_HIP(hipMemset(s->unphysical.p, 0, s->unphysical.bytes));
     FVH_HIP(hipMemset(s->face.p, 0, s->face.bytes));
     FVH_HIP(hipMemset(s->bincr.p, 0, s->bincr.bytes));
     return FVH_OK;
@@ -394,7 +397,8 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     const TileMap tm = tile_map(s);
     const InstView iv = inst_view(s);
     const TileView tv = tile_view(s);
-    const PhysParams pp = phys(c);
+    PhysParams pp = phys(c);
+    pp.unphysical = (uint32_t*)s->unphysical.p;
     const BcParam<T>* bcs = (const BcParam<T>*)s->bcs.p;
     const bool grad = c.flags & FVH_FLAG_GRADIENTS, lim = c.flags & FVH_FLAG_LIMITERS,
                lin = c.flags & FVH_FLAG_LINEAR, gdt = c.flags & FVH_FLAG_GLOBAL_DT,
@@ -615,6 +619,7 @@ int32_t apply_bounds_t(fvh_solver* s) {
     for (auto& c0 : z)
         for (int k = 0; k < 4; ++k) c0.R[k] = 1.0;
     FVH_HIP(hipMemcpyAsync(s->ctrl.p, z.data(), z.size() * sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    FVH_HIP(hipMemsetAsync(s->unphysical.p, 0, s->unphysical.bytes, s->stream));
     FVH_HIP(hipStreamSynchronize(s->stream));
     return FVH_OK;
 }
@@ -648,6 +653,30 @@ int32_t read_status(fvh_solver* s, fvh_status* st) {
     return FVH_OK;
 }
 
+// Failure detection: instances whose stage updates left a real cell with
+// rho <= 0, p <= 0 or a non-finite value, or whose residuals are not finite.
+// Fills msg (empty when all is well).
+int32_t check_state(fvh_solver* s, std::string& msg) {
+    std::vector<uint32_t> f(s->n_inst);
+    std::vector<Ctrl> h;
+    int32_t rc = read_ctrl(s, h);
+    if (rc) return rc;
+    FVH_HIP(hipMemcpy(f.data(), s->unphysical.p, f.size() * 4, hipMemcpyDeviceToHost));
+    msg.clear();
+    int shown = 0, n_bad = 0;
+    for (int i = 0; i < s->n_inst; ++i) {
+        if (!f[i] && !h[i].nonfinite) continue;
+        ++n_bad;
+        if (shown++ < 4)
+            msg += (msg.empty() ? "" : "; ") + std::string("instance ") + std::to_string(i) + " by step " +
+                   std::to_string(h[i].step) + (f[i] ? ": a cell state with rho <= 0, p <= 0 or not finite"
+                                                     : ": non-finite residual");
+    }
+    if (n_bad > shown) msg += "; and " + std::to_string(n_bad - shown) + " more instances";
+    if (n_bad) msg = "non-physical state: " + msg;
+    return FVH_OK;
+}
+
 // Executes up to n more steps of run() from the current status (steps are
 // queued in growing chunks; kernels queued after an instance's stop are
 // no-ops for it).  Afterwards every instance's state is in buffer s->cur.
@@ -665,6 +694,7 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
     // each executed step moves the state forward by one or two buffers
     const int delta = (s->cfg.n_alpha % 2) ? 1 : 2;
     uint64_t chunk = 8, left = n;
+    std::string bad;
     while (!now.stopped && left > 0) {
         const uint64_t k_n = std::min<uint64_t>(chunk, left);
         for (uint64_t k = 0; k < k_n; ++k, ++step) {
@@ -673,6 +703,8 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
         }
         left -= k_n;
         if ((rc = read_status(s, &now))) return rc;
+        if ((rc = check_state(s, bad))) return rc;
+        if (!bad.empty()) break;
         if (chunk < 256) chunk *= 2;
     }
     if ((rc = read_ctrl(s, after))) return rc;
@@ -695,7 +727,12 @@ int32_t run_continue_t(fvh_solver* s, const fvh_run_options* o, uint64_t n, fvh_
     }
     FVH_HIP(hipStreamSynchronize(s->stream));
     *st = now;
-    return drain_events(s);
+    if ((rc = drain_events(s))) return rc;
+    if (!bad.empty()) {
+        set_error(bad);
+        return FVH_E_STATE;
+    }
+    return FVH_OK;
 }
 
 template <class T, class S>
@@ -2050,6 +2087,7 @@ int32_t fvh_solver_load(fvh_solver* s, const char* path) {
     int32_t rc = fvh_solver_set_state(s, q.data());
     if (rc) return rc;
     FVH_HIP(hipMemcpyAsync(s->ctrl.p, h.data(), h.size() * sizeof(Ctrl), hipMemcpyHostToDevice, s->stream));
+    FVH_HIP(hipMemsetAsync(s->unphysical.p, 0, s->unphysical.bytes, s->stream));
     FVH_HIP(hipStreamSynchronize(s->stream));
     return FVH_OK;
 }

@@ -283,7 +283,13 @@ int32_t fvh_min_dt_device(double* dt, int64_t n, void* stream);
 int32_t fvh_phase_residuals(fvh_solver* s, const double* qt, double* R);
 
 /* The run() loop of explicit.cpp:657-817 from the current state: applies the
- * initial boundary update, then steps until the stop test holds. */
+ * initial boundary update, then steps until the stop test holds.
+ * Failure detection (no reference counterpart: the reference runs on with
+ * NaN): every stage update checks the real cells' states, and when one has
+ * rho <= 0, p <= 0 or a non-finite value, or a residual is not finite, the
+ * run stops after the group of steps under way (at most 256) and returns
+ * FVH_E_STATE naming the instance and step; the status and the state are
+ * those at that point.  Applies to run, run_continue and run_log. */
 int32_t fvh_solver_run(fvh_solver* s, const fvh_run_options* o, fvh_status* st);
 
 /* run() in pieces, for callers that print or save between steps:

@@ -200,16 +200,60 @@ def test_positivity_fallback_bit_exact():
 
 def test_reference_scheme_blows_up_identically():
     """Without the fallback the reference scheme goes non-finite at the step
-    corner; the GPU reproduces the same trajectory up to and through it."""
+    corner; the GPU reproduces the same trajectory up to and through it, and
+    the run stops with the non-physical-state error once a stage leaves a
+    cell with rho <= 0, p <= 0 or NaN."""
+    from fvhyper_amd import FvhError
     m, cfg, ic = workloads.forward_step(n_per_unit=40, reference_compatible=True)
     om = O.Mesh(m)
     s = Solver(Mesh.from_arrays(m), cfg)
     q0 = workloads.initial_state(om, ic)
     s.set_state(q0)
-    st = s.run(max_step=130)
-    qo, steps, t, R, R0, _ = om.run(cfg, q0, 130)
-    assert not np.all(np.isfinite(qo))
-    assert same_values(s.get_state(), qo)
+    with pytest.raises(FvhError, match="non-physical state: instance 0 by step"):
+        s.run(max_step=130)
+    st = s.status()
+    assert 0 < st["step"] <= 130
+    qo, *_ = om.run(cfg, q0, st["step"])
+    q = s.get_state()
+    assert not np.all(np.isfinite(qo)) or np.any(qo.reshape(-1, 4)[:om.n_real, 0] <= 0) or \
+        np.any(0.4 * (qo[3::4] - 0.5 * (qo[1::4] ** 2 + qo[2::4] ** 2) / qo[0::4])[:om.n_real] <= 0)
+    assert same_values(q, qo)
+
+
+def test_negative_density_stops_the_run():
+    """A negative density injected through set_state: the first stage leaves
+    it non-physical, the run returns the non-physical-state error after its
+    first group of steps, and the GPU keeps working (a new run on a physical
+    state is bit-exact)."""
+    from fvhyper_amd import FvhError
+    cfg = CONFIGS["second_venkat_global"]
+    m = MESHES["step"]()
+    om = O.Mesh(m)
+    q0 = random_state(om.n_cells, seed=91, mach=0.3)
+    bad = q0.copy()
+    bad[4 * 7] = -0.5
+    s = Solver(Mesh.from_arrays(m), cfg)
+    s.set_state(bad)
+    with pytest.raises(FvhError, match="non-physical state: instance 0 by step"):
+        s.run(max_step=50)
+    assert s.status()["step"] <= 8
+    s.set_state(q0)
+    st = s.run(max_step=6)
+    qo, *_ = om.run(cfg, q0, 6)
+    assert st["step"] == 6 and bits_equal(s.get_state(), qo)
+
+
+def test_batch_names_the_failing_instance():
+    from fvhyper_amd import Batch, FvhError
+    items = workloads.naca_sweep(3, seed=11, min_cells=1500, max_cells=4000)
+    meshes = [Mesh.from_arrays(it[0]) for it in items]
+    b = Batch(meshes, [it[1] for it in items])
+    q0 = [workloads.initial_state(mm, it[2]) for mm, it in zip(meshes, items)]
+    q0[1] = q0[1].copy()
+    q0[1][4 * 3 + 3] = -1.0   # negative energy: p < 0
+    b.set_state(b.join(q0))
+    with pytest.raises(FvhError, match="instance 1 by step"):
+        b.run(max_step=20)
 
 
 # FP32 arithmetic (states, geometry and every operation in FP32): error
