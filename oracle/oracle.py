@@ -253,6 +253,42 @@ class Mesh:
         return q, int(steps.value), float(t.value), R, R0, qt
 
 
+    def first_unphysical_step(self, cfg: Dict, q0: np.ndarray, max_step: int):
+        """run() of explicit.cpp:657-817 stage by stage (the same phases as
+        orc_run), returning the first step in which a stage update leaves a
+        real cell with rho <= 0, p <= 0 or a non-finite value (what the
+        library's failure detection flags), or None within max_step steps."""
+        L = lib()
+        c = make_config(cfg, self.names)
+        N = self.n_cells
+        n = 4 * N
+        q = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        gx, gy, lim, qmin, qmax, qt = (np.zeros(n) for _ in range(6))
+        dt, qk = np.zeros(N), np.zeros(n)
+        L.orc_update_bounds(self._p, C.byref(c), q, gx, gy, lim)
+        for step in range(max_step):
+            L.orc_calc_dt(self._p, C.byref(c), q, dt)
+            if c.gdt:
+                L.orc_min_dt(self._p, dt)
+            qk[:] = q
+            for a in list(cfg.get("alpha", RK5)):
+                if c.grad:
+                    L.orc_calc_gradients(self._p, qk, gx, gy)
+                if c.lim:
+                    L.orc_calc_limiters(self._p, C.byref(c), qk, gx, gy, lim, qmin, qmax)
+                L.orc_calc_time_derivatives(self._p, C.byref(c), qk, gx, gy, lim, qt)
+                if c.smooth:
+                    L.orc_smooth_residuals(self._p, qt)
+                L.orc_update_cells(N, qk, q, qt, dt, float(a))
+                r = qk.reshape(-1, 4)[:self.n_real]
+                with np.errstate(all="ignore"):
+                    ok = (r[:, 0] > 0) & (2 * r[:, 0] * r[:, 3] > r[:, 1] ** 2 + r[:, 2] ** 2)
+                if not ok.all():
+                    return step
+                L.orc_update_bounds(self._p, C.byref(c), qk, gx, gy, lim)
+            q[:] = qk
+        return None
+
     def run_f32_state(self, cfg: Dict, q0: np.ndarray, max_step: int):
         """run() of explicit.cpp:657-817 with the cell states stored in FP32:
         every state that run() writes (the initial state, the initial
