@@ -68,6 +68,8 @@ __device__ inline void stq(Q4<S>* p, const Q4<T>& a) {
 // (b % 8) * per + b / 8 gives every XCD one contiguous range of cells, so a
 // cell's neighbours (close in the renumbered order) sit in the same L2.
 constexpr int kXcd = 8;
+// half-edges of one cell whose loads the per-cell loops issue together
+constexpr int kHeChunk = 3;
 __device__ inline int xcd_block(int nblk) {
     const int per = (nblk + kXcd - 1) / kXcd;
     const int b = blockIdx.x;

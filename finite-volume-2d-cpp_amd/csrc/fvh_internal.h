@@ -61,8 +61,13 @@ void mesh_geometry_vjp(const fvh_mesh& m, GeoBar& b, double* gxy);
 std::vector<int32_t> rcm_order(const fvh_mesh& m);
 
 // Recursive coordinate bisection of the real cells' centres into tiles of
-// `tile` cells (all full but the last), tiles in recursion order, cells in
-// ascending reference index within a tile.  order[k] as for rcm_order.
-std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile);
+// `tile` cells (all full but the last; cuts at multiples of tile along the
+// longer side of the centroids' bounding box), tiles in recursion order,
+// cells in ascending reference index within a tile.  order[k] as for
+// rcm_order.  A tile whose index (in the order without splits) is marked in
+// `split` is bisected once more, into two tiles; leaf_sizes receives the
+// size of every tile, in order.
+std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile, const std::vector<uint8_t>* split = nullptr,
+                               std::vector<int32_t>* leaf_sizes = nullptr);
 
 }  // namespace fvh

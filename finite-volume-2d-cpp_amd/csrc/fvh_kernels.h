@@ -113,8 +113,9 @@ struct HaloView {
     const uint16_t* r1_loc;   // local neighbour index per half-edge of real ring-1 cells
     const uint16_t* he_loc;   // per half-edge of a real cell: local neighbour index in its own tile
     const uint16_t* r1_slot;  // parallel to r1_loc: the tile's edge slot of that half-edge (0xffff: none)
-    int32_t max_ext, max_g, max_edges;   // largest tile (LDS sizing)
+    int32_t max_ext, max_g, max_edges;   // largest tile lists
     int32_t mean_g;                      // tile plus real ring-1 cells, averaged over tiles
+    int32_t lds_bytes;                   // largest tile's LDS (stage_lds_tile); 0: from the maxima above
 };
 
 struct PhysParams {
@@ -190,12 +191,17 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
                   const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0,
                   const StageRecord<T>& rec = StageRecord<T>());
-// LDS of the fused stage pass: the states of a tile's local cell list in the
-// storage type, then two face states per tile edge in the arithmetic type
-// (32-byte aligned).
+// LDS of the fused stage pass for a tile of n_ext local cells and n_edges
+// edges: the states of its local cell list in the storage type, then, from
+// the next 32-byte boundary, two face states per tile edge in the
+// arithmetic type.  A launch takes the largest tile's (stage_lds_bytes).
 template <class S>
-__host__ __device__ inline size_t stage_lds_states(const HaloView& hv) {
-    return ((size_t)hv.max_ext * sizeof(Q4<S>) + 31) / 32 * 32;
+__host__ __device__ inline size_t stage_lds_states(int32_t n_ext) {
+    return ((size_t)n_ext * sizeof(Q4<S>) + 31) / 32 * 32;
+}
+template <class T, class S>
+__host__ __device__ inline size_t stage_lds_tile(int32_t n_ext, int32_t n_edges) {
+    return stage_lds_states<S>(n_ext) + (size_t)(2 * n_edges) * sizeof(Q4<T>);
 }
 template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv);

@@ -19,15 +19,34 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
     const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
     if (KIND == 1) {
         // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184)
+        const EigCell<T> eo_c = eig_cell(g, qc.v);
         T s = T(0);
-        for (int h = h0; h < h1; ++h) {
-            const uint32_t ue = m.he_edge[h];
-            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
-            const Q4<T> qn = ldq<T>(q + m.he_nbr[h]);
-            const bool side = (ue & kSideBit) != 0;
-            const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
-            const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
-            s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+        for (int hb = h0; hb < h1; hb += kHeChunk) {
+            // a chunk of half-edges: records, geometry and neighbour states
+            // loaded together (indices clamped to the cell's last half-edge)
+            uint32_t uev[kHeChunk];
+            int nbv[kHeChunk];
+            Q4<T> geov[kHeChunk], qnv[kHeChunk];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(hb + i, h1 - 1);
+                uev[i] = m.he_edge[hh];
+                nbv[i] = m.he_nbr[hh];
+            }
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                geov[i] = m.e_geo[uev[i] & kEdgeMask];
+                qnv[i] = ldq<T>(q + nbv[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                if (hb + i >= h1) break;
+                const Q4<T>& geo = geov[i];
+                const bool side = (uev[i] & kSideBit) != 0;
+                const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
+                const T en = eig_sum(g, qnv[i].v, geo.v[0], geo.v[1]);
+                s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+            }
         }
         return T(pp.cfl) * m.c_area[c] / s;
     }
@@ -136,23 +155,32 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
             sigs[k] = T(0);
         }
     }
-    for (int h = h0; h < h1; ++h) {
-        const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
-        const T dx = ec.v[0] - cc.v[0];
-        const T dy = ec.v[1] - cc.v[1];
+    for (int hb = h0; hb < h1; hb += kHeChunk) {
+        // edge centres of a chunk of half-edges loaded together (indices
+        // clamped to the cell's last half-edge)
+        G2<T> ecv[kHeChunk];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const T dqg = gx[k] * dx + gy[k] * dy;
-            const T dmax = qmx[k] - qc.v[k];
-            const T dmin = qmn[k] - qc.v[k];
-            const T sig = sigs[k];
-            T l = T(1);
-            if (sig < T(1)) {
-                if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
-                else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+        for (int i = 0; i < kHeChunk; ++i) ecv[i] = m.e_c[m.he_edge[min(hb + i, h1 - 1)] & kEdgeMask];
+#pragma unroll
+        for (int i = 0; i < kHeChunk; ++i) {
+            if (hb + i >= h1) break;
+            const G2<T> ec = ecv[i];
+            const T dx = ec.v[0] - cc.v[0];
+            const T dy = ec.v[1] - cc.v[1];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const T dqg = gx[k] * dx + gy[k] * dy;
+                const T dmax = qmx[k] - qc.v[k];
+                const T dmin = qmn[k] - qc.v[k];
+                const T sig = sigs[k];
+                T l = T(1);
+                if (sig < T(1)) {
+                    if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
+                    else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+                }
+                l = sig + (T(1) - sig) * l;
+                lim[k] = smin(lim[k], l);
             }
-            l = sig + (T(1) - sig) * l;
-            lim[k] = smin(lim[k], l);
         }
     }
 }
@@ -179,58 +207,75 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
 #pragma unroll
     for (int k = 0; k < 4; ++k) { qmn[k] = qc.v[k]; qmx[k] = qc.v[k]; }
     T dts = (DT == 2 || DT == 3) ? T(1) : T(0);
+    // the cell's own wave-speed terms of the summed time step, once
+    EigCell<T> eo_c{T(0), T(0), T(0)};
+    if (DT == 1 && do_dt) eo_c = eig_cell(g, qc.v);
     // one pass over the half-edges feeds the Green-Gauss sums (edge order,
     // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
     // time-step sum or minimum
     if ((real && (grad || lim_on)) || (DT && do_dt)) {
-        for (int h = h0; h < h1; ++h) {
-            const uint32_t ue = m.he_edge[h];
-            const bool side = (ue & kSideBit) != 0;
-            const Q4<T> geo = m.e_geo[ue & kEdgeMask];
-            bool self;
-            const Q4<T> qn = nbr(h, self);
-            if (DT == 1 && do_dt) {
-                const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]);
-                const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
-                dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
-            } else if (DT == 2 && do_dt) {
-                const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
-                dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
-            } else if (DT == 3 && do_dt) {
-                const T A = m.c_area[c];
-                const T ec = eig_abs(g, qc.v, geo.v[0], geo.v[1]) * geo.v[2];
-                const T ev = eig_visc(g, T(pp.mu), T(pp.pr), qc.v, geo.v[2], A);
-                dts = smin(dts, T(pp.cfl) * A / (ec + T(2) * ev));
-            } else if (DT == 4 && do_dt) {
-                const T mu = T(pp.mu), pr = T(pp.pr);
-                const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]) +
-                             T(2) * eig_visc(g, mu, pr, qc.v, geo.v[2], m.c_area[c]);
-                const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]) +
-                             T(2) * eig_visc(g, mu, pr, qn.v, geo.v[2], m.c_area[m.he_nbr[h]]);
-                dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
-            }
-            if (!real) continue;
-            if (lim_on) {
+        for (int hb = h0; hb < h1; hb += kHeChunk) {
+            // edge records and geometry of a chunk of half-edges loaded
+            // together (indices clamped to the cell's last half-edge), so
+            // that their latencies overlap
+            uint32_t uev[kHeChunk];
+            Q4<T> geov[kHeChunk];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) uev[i] = m.he_edge[min(hb + i, h1 - 1)];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) geov[i] = m.e_geo[uev[i] & kEdgeMask];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int h = hb + i;
+                if (h >= h1) break;
+                const uint32_t ue = uev[i];
+                const Q4<T> geo = geov[i];
+                const bool side = (ue & kSideBit) != 0;
+                bool self;
+                const Q4<T> qn = nbr(h, self);
+                if (DT == 1 && do_dt) {
+                    const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
+                    const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+                    dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+                } else if (DT == 2 && do_dt) {
+                    const T eo = eig_edge(g, qc.v, geo.v[0], geo.v[1]);
+                    dts = smin(dts, T(pp.cfl) * (m.c_area[c] / geo.v[2]) / sabs(eo));
+                } else if (DT == 3 && do_dt) {
+                    const T A = m.c_area[c];
+                    const T ec = eig_abs(g, qc.v, geo.v[0], geo.v[1]) * geo.v[2];
+                    const T ev = eig_visc(g, T(pp.mu), T(pp.pr), qc.v, geo.v[2], A);
+                    dts = smin(dts, T(pp.cfl) * A / (ec + T(2) * ev));
+                } else if (DT == 4 && do_dt) {
+                    const T mu = T(pp.mu), pr = T(pp.pr);
+                    const T eo = eig_sum(g, qc.v, geo.v[0], geo.v[1]) +
+                                 T(2) * eig_visc(g, mu, pr, qc.v, geo.v[2], m.c_area[c]);
+                    const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]) +
+                                 T(2) * eig_visc(g, mu, pr, qn.v, geo.v[2], m.c_area[m.he_nbr[h]]);
+                    dts = dts + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+                }
+                if (!real) continue;
+                if (lim_on) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        qmn[k] = smin(qmn[k], qn.v[k]);
+                        qmx[k] = smax(qmx[k], qn.v[k]);
+                    }
+                }
+                if (!grad || self) continue;   // the reference skips i == j edges
+                const T* qi = side ? qn.v : qc.v;
+                const T* qj = side ? qc.v : qn.v;
+                const T w = geo.v[3];
+                const T w1 = T(1) - w;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    qmn[k] = smin(qmn[k], qn.v[k]);
-                    qmx[k] = smax(qmx[k], qn.v[k]);
-                }
-            }
-            if (!grad || self) continue;   // the reference skips i == j edges
-            const T* qi = side ? qn.v : qc.v;
-            const T* qj = side ? qc.v : qn.v;
-            const T w = geo.v[3];
-            const T w1 = T(1) - w;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T f = (qi[k] * w1 + qj[k] * w) * geo.v[2];
-                if (!side) {
-                    gx[k] = gx[k] + f * geo.v[0];
-                    gy[k] = gy[k] + f * geo.v[1];
-                } else {
-                    gx[k] = gx[k] - f * geo.v[0];
-                    gy[k] = gy[k] - f * geo.v[1];
+                    const T f = (qi[k] * w1 + qj[k] * w) * geo.v[2];
+                    if (!side) {
+                        gx[k] = gx[k] + f * geo.v[0];
+                        gy[k] = gy[k] + f * geo.v[1];
+                    } else {
+                        gx[k] = gx[k] - f * geo.v[0];
+                        gy[k] = gy[k] - f * geo.v[1];
+                    }
                 }
             }
         }
@@ -622,7 +667,7 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
     const int n_g = nt + ex.y, n_ext = nt + ex.z + ex.w;
     // states in the storage type, face states and fluxes in the arithmetic type
     Q4<S>* sq = (Q4<S>*)lds_raw;
-    Q4<T>* fs = (Q4<T>*)(lds_raw + stage_lds_states<S>(hv));   // per tile edge slot: face state of side 0, side 1; then the flux in [0]
+    Q4<T>* fs = (Q4<T>*)(lds_raw + stage_lds_states<S>(n_ext));   // per tile edge slot: face state of side 0, side 1; then the flux in [0]
     const int32_t* xc = hv.cells + ex.x;
     const T g = T(pp.gamma);
     const T lim0 = lim_on ? T(1) : T(0);
@@ -655,37 +700,53 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
             rec.gy[c] = b;
             rec.lm[c] = d;
         }
-        for (int h = h0; h < h1; ++h) {
-            const int slot = own ? (int)m.he_slot[h] : (int)hv.r1_slot[r1 + (h - h0)];
-            if (slot == 0xffff) continue;
-            const uint32_t ue = m.he_edge[h];
-            const int side = (ue & kSideBit) ? 1 : 0;
-            const G2<T> ec = m.e_c[ue & kEdgeMask];
-            const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
-            Q4<T> f;
+        for (int hb = h0; hb < h1; hb += kHeChunk) {
+            int slotv[kHeChunk];
+            uint32_t uev[kHeChunk];
+            G2<T> ecv[kHeChunk];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) f.v[k] = qc.v[k] + inc.v[k];
-            if (pp.positivity && !physical(g, f.v)) f = qc;
-            fs[2 * slot + side] = f;
-            if (own && (ue & kBcBit)) {
-                // the ghost's face: its state with a zero gradient and the reset limiter
-                const int nb = m.he_nbr[h];
-                bincr[nb - m.n_real] = inc;
-                const T z[4] = {T(0), T(0), T(0), T(0)};
-                const T l0[4] = {lim0, lim0, lim0, lim0};
-                if (rec.gx) {
-                    // the ghost's gradient and limiter as the two-pass form stores them
-                    rec.gx[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
-                    rec.gy[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
-                    rec.lm[nb] = Q4<T>{{l0[0], l0[1], l0[2], l0[3]}};
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(hb + i, h1 - 1);
+                slotv[i] = own ? (int)m.he_slot[hh] : (int)hv.r1_slot[r1 + (hh - h0)];
+                uev[i] = m.he_edge[hh];
+            }
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) ecv[i] = m.e_c[uev[i] & kEdgeMask];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int h = hb + i;
+                if (h >= h1) break;
+                const int slot = slotv[i];
+                const uint32_t ue = uev[i];
+                const G2<T> ec = ecv[i];
+                if (slot == 0xffff) continue;
+                const int side = (ue & kSideBit) ? 1 : 0;
+                const Q4<T> inc = face_inc(gx, gy, lim, ec, cc);
+                Q4<T> f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) f.v[k] = qc.v[k] + inc.v[k];
+                if (pp.positivity && !physical(g, f.v)) f = qc;
+                fs[2 * slot + side] = f;
+                if (own && (ue & kBcBit)) {
+                    // the ghost's face: its state with a zero gradient and the reset limiter
+                    const int nb = m.he_nbr[h];
+                    bincr[nb - m.n_real] = inc;
+                    const T z[4] = {T(0), T(0), T(0), T(0)};
+                    const T l0[4] = {lim0, lim0, lim0, lim0};
+                    if (rec.gx) {
+                        // the ghost's gradient and limiter as the two-pass form stores them
+                        rec.gx[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
+                        rec.gy[nb] = Q4<T>{{z[0], z[1], z[2], z[3]}};
+                        rec.lm[nb] = Q4<T>{{l0[0], l0[1], l0[2], l0[3]}};
+                    }
+                    const Q4<T> ginc = face_inc(z, z, l0, ec, m.c_c[nb]);
+                    const Q4<T> qg = ldq<T>(sq + loc[h - h0]);
+                    Q4<T> fg;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) fg.v[k] = qg.v[k] + ginc.v[k];
+                    if (pp.positivity && !physical(g, fg.v)) fg = qg;
+                    fs[2 * slot + 1 - side] = fg;
                 }
-                const Q4<T> ginc = face_inc(z, z, l0, ec, m.c_c[nb]);
-                const Q4<T> qg = ldq<T>(sq + loc[h - h0]);
-                Q4<T> fg;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) fg.v[k] = qg.v[k] + ginc.v[k];
-                if (pp.positivity && !physical(g, fg.v)) fg = qg;
-                fs[2 * slot + 1 - side] = fg;
             }
         }
     }
@@ -712,13 +773,25 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
         const T A = m.c_area[c];
         T qt[4] = {T(0), T(0), T(0), T(0)};
-        for (int h = h0; h < h1; ++h) {
-            const bool side = (m.he_edge[h] & kSideBit) != 0;
-            const Q4<T> r = fs[2 * (int)m.he_slot[h]];
+        for (int hb = h0; hb < h1; hb += kHeChunk) {
+            uint32_t uev[kHeChunk];
+            int slotv[kHeChunk];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const T t = r.v[k] / A;
-                qt[k] = side ? qt[k] + t : qt[k] - t;
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(hb + i, h1 - 1);
+                uev[i] = m.he_edge[hh];
+                slotv[i] = m.he_slot[hh];
+            }
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                if (hb + i >= h1) break;
+                const bool side = (uev[i] & kSideBit) != 0;
+                const Q4<T> r = fs[2 * slotv[i]];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const T t = r.v[k] / A;
+                    qt[k] = side ? qt[k] + t : qt[k] - t;
+                }
             }
         }
         auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
@@ -1095,7 +1168,7 @@ void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const
 
 template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv) {
-    return stage_lds_states<S>(hv) + (size_t)(2 * hv.max_edges) * sizeof(Q4<T>);
+    return hv.lds_bytes > 0 ? (size_t)hv.lds_bytes : stage_lds_tile<T, S>(hv.max_ext, hv.max_edges);
 }
 
 template <class T, int PHI, int DT, bool J, int NB, class S>

@@ -743,38 +743,62 @@ std::vector<int32_t> rcm_order(const fvh_mesh& m) {
     return order;
 }
 
-std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile) {
+std::vector<int32_t> rcb_order(const fvh_mesh& m, int32_t tile, const std::vector<uint8_t>* split,
+                               std::vector<int32_t>* leaf_sizes) {
     const int32_t N = m.n_real;
     std::vector<int32_t> idx(N);
     for (int32_t i = 0; i < N; ++i) idx[i] = i;
     if (tile < 1) tile = 1;
-    // explicit stack: (lo, hi) ranges, left part processed first
-    std::vector<std::pair<int32_t, int32_t>> stack;
-    std::vector<int32_t> out;
-    out.reserve(N);
-    stack.push_back({0, N});
-    while (!stack.empty()) {
-        auto [lo, hi] = stack.back();
-        stack.pop_back();
-        const int32_t n = hi - lo;
-        if (n <= tile) {
-            std::sort(idx.begin() + lo, idx.begin() + hi);
-            out.insert(out.end(), idx.begin() + lo, idx.begin() + hi);
-            continue;
-        }
+    if (leaf_sizes) leaf_sizes->clear();
+    // the longer side of the centroids' bounding box of idx[lo, hi)
+    auto by_x_of = [&](int32_t lo, int32_t hi) {
         double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
         for (int32_t k = lo; k < hi; ++k) {
             const int32_t c = idx[k];
             x0 = std::min(x0, m.c_cx[c]); x1 = std::max(x1, m.c_cx[c]);
             y0 = std::min(y0, m.c_cy[c]); y1 = std::max(y1, m.c_cy[c]);
         }
-        const bool by_x = (x1 - x0) >= (y1 - y0);
-        const std::vector<double>& key = by_x ? m.c_cx : m.c_cy;
-        const int32_t tiles = (n + tile - 1) / tile;
-        const int32_t mid = lo + (tiles / 2) * tile;
+        return (x1 - x0) >= (y1 - y0);
+    };
+    // idx[lo, hi) split at mid: the cells below mid along the longer side first
+    auto partition = [&](int32_t lo, int32_t mid, int32_t hi) {
+        const std::vector<double>& key = by_x_of(lo, hi) ? m.c_cx : m.c_cy;
         std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int32_t a, int32_t b) {
             return key[a] != key[b] ? key[a] < key[b] : a < b;
         });
+    };
+    // explicit stack: (lo, hi) ranges, left part processed first
+    std::vector<std::pair<int32_t, int32_t>> stack;
+    std::vector<int32_t> out;
+    out.reserve(N);
+    stack.push_back({0, N});
+    int32_t leaf = 0;
+    auto emit = [&](int32_t lo, int32_t hi) {
+        std::sort(idx.begin() + lo, idx.begin() + hi);
+        out.insert(out.end(), idx.begin() + lo, idx.begin() + hi);
+        if (leaf_sizes) leaf_sizes->push_back(hi - lo);
+    };
+    while (!stack.empty()) {
+        auto [lo, hi] = stack.back();
+        stack.pop_back();
+        const int32_t n = hi - lo;
+        if (n <= tile) {
+            // a leaf marked in split is bisected once more, into two leaves
+            const bool bisect = split && leaf < (int32_t)split->size() && (*split)[leaf] && n >= 2;
+            ++leaf;
+            if (bisect) {
+                const int32_t mid = lo + n / 2;
+                partition(lo, mid, hi);
+                emit(lo, mid);
+                emit(mid, hi);
+            } else {
+                emit(lo, hi);
+            }
+            continue;
+        }
+        const int32_t tiles = (n + tile - 1) / tile;
+        const int32_t mid = lo + (tiles / 2) * tile;
+        partition(lo, mid, hi);
         stack.push_back({mid, hi});   // right after left
         stack.push_back({lo, mid});
     }

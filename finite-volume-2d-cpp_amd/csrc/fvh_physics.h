@@ -210,10 +210,24 @@ FVH_HD void boundary(const BcParam<T>& P, T g, T* b, const T* q, T n0, T n1) {
 
 // Per-edge wave-speed terms of the calc_dt kinds.
 //  summed CFL (forwardStep/main.cpp:167-171): c + |u.n|
+// eig_cell is the part that depends on the state only (sound speed and
+// velocity), so a cell's is computed once for all its edges:
+// eig_sum(eig_cell(g, q), n0, n1) is eig_sum(g, q, n0, n1) bit for bit.
+template <class T>
+struct EigCell {
+    T c, u, v;
+};
+template <class T>
+FVH_HD EigCell<T> eig_cell(T g, const T* q) {
+    return EigCell<T>{sqrt(pressure(g, q) * g / q[0]), q[1] / q[0], q[2] / q[0]};
+}
+template <class T>
+FVH_HD T eig_sum(const EigCell<T>& e, T n0, T n1) {
+    return e.c + sabs(e.u * n0 + e.v * n1);
+}
 template <class T>
 FVH_HD T eig_sum(T g, const T* q, T n0, T n1) {
-    const T c = sqrt(pressure(g, q) * g / q[0]);
-    return c + sabs(q[1] / q[0] * n0 + q[2] / q[0] * n1);
+    return eig_sum(eig_cell(g, q), n0, n1);
 }
 //  edge-minimum local (naca/main.cpp:169-173): (c+|u|) n0 + (c+|v|) n1
 template <class T>

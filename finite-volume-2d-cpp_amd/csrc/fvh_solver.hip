@@ -106,7 +106,7 @@ struct fvh_solver {
     // fused stage pass: tile halos
     bool fused = false;
     fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_r1_slot, x_he_loc;
-    int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0;
+    int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0, x_lds = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf unphysical;                  // per instance: a stage left a non-physical cell state
     fvh::DevBuf dbg[4];
@@ -217,6 +217,7 @@ HaloView halo_view(fvh_solver* s) {
     h.max_g = s->x_max_g;
     h.mean_g = s->x_mean_g;
     h.max_edges = s->x_max_edges;
+    h.lds_bytes = s->x_lds;
     return h;
 }
 
@@ -784,6 +785,13 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::
     return true;
 }
 
+// LDS bytes of the fused stage pass for a tile with n_ext local cells and
+// n_edges edges, in the solver's precision.
+size_t halo_lds_bytes(const fvh_solver* s, int32_t n_ext, int32_t n_edges) {
+    if (s->mixed) return stage_lds_tile<double, float>(n_ext, n_edges);
+    return s->f32 ? stage_lds_tile<float, float>(n_ext, n_edges) : stage_lds_tile<double, double>(n_ext, n_edges);
+}
+
 // Halos of the real tiles for the fused stage pass.  Returns false when a
 // tile's local cell list does not fit 16-bit indices.
 struct HaloHost {
@@ -791,6 +799,7 @@ struct HaloHost {
     std::vector<int32_t> cells, r1_start;
     std::vector<uint16_t> r1_loc, r1_slot, he_loc;
     int32_t max_ext = 0, max_g = 0, max_edges = 0;
+    size_t max_lds = 0;  // the largest tile's LDS bytes
     int64_t sum_g = 0;   // tile cells plus real ring-1 cells, over all tiles
 };
 
@@ -858,6 +867,7 @@ bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::v
         hh.max_g = std::max(hh.max_g, nt + n1r);
         hh.sum_g += nt + n1r;
         hh.max_edges = std::max(hh.max_edges, t_off[t + 1] - t_off[t]);
+        hh.max_lds = std::max(hh.max_lds, halo_lds_bytes(s, n_ext, t_off[t + 1] - t_off[t]));
         for (int32_t c = c0; c < c1; ++c) mark[c] = -1;
         for (int32_t c : r1r) mark[c] = -1;
         for (int32_t c : r1g) mark[c] = -1;
@@ -868,14 +878,8 @@ bool build_halo(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::v
     return true;
 }
 
-// LDS bytes of the fused stage pass for a tile halo (stage_lds_bytes).
-size_t halo_lds_bytes(const fvh_solver* s, const HaloHost& hh) {
-    HaloView hv{};
-    hv.max_ext = hh.max_ext;
-    hv.max_edges = hh.max_edges;
-    if (s->mixed) return stage_lds_bytes<double, float>(hv);
-    return s->f32 ? stage_lds_bytes<float, float>(hv) : stage_lds_bytes<double, double>(hv);
-}
+// LDS bytes of the fused stage pass: its largest tile's.
+size_t halo_lds_bytes(const HaloHost& hh) { return hh.max_lds; }
 
 int32_t check_config(const fvh_config* c) {
     if (c->n_alpha < 1 || c->n_alpha > FVH_MAX_ALPHA || c->n_bc < 0 || c->n_bc > FVH_MAX_BC) {
@@ -923,8 +927,10 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
     return true;
 }
 
-// Build the device layout of n instances and upload it.
-int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_solver** out) {
+// Build the device layout of n instances and upload it.  splits marks, per
+// instance, the RCB tiles to bisect once more (null: none).
+int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_solver** out,
+               const std::vector<std::vector<uint8_t>>* splits = nullptr) {
     if (n < 1 || !ms || !cs || !out) { set_error("null argument or empty batch"); return FVH_E_ARG; }
     int32_t rc;
     for (int32_t i = 0; i < n; ++i) {
@@ -981,6 +987,8 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     std::vector<int32_t> edge_base(n), cell_dev;     // cell_dev over the concatenated state
     cell_dev.assign(N, 0);
     std::vector<int32_t> tile_c0, tile_inst;
+    // tile sizes of the real cells per instance (empty: tiles of `tile` cells)
+    std::vector<std::vector<int32_t>> leaves(n);
     {
         int32_t r0 = 0, g0 = s->n_real, eb = 0;
         int64_t st0 = 0;
@@ -988,7 +996,8 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             const fvh_mesh& m = *ms[i];
             std::vector<int32_t> order;
             if (c.renumber == FVH_ORDER_RCM) order = rcm_order(m);
-            else if (c.renumber == FVH_ORDER_TILES) order = rcb_order(m, tile);
+            else if (c.renumber == FVH_ORDER_TILES)
+                order = rcb_order(m, tile, splits ? &(*splits)[i] : nullptr, &leaves[i]);
             else { order.resize(m.n_real); for (int32_t k = 0; k < m.n_real; ++k) order[k] = k; }
             s->i_ncells[i] = m.n_cells;
             s->i_real0[i] = r0;
@@ -1013,12 +1022,19 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             eb += m.n_edges();
         }
         // real tiles, instance by instance, then ghost tiles
+        std::vector<int32_t> sizes;
         for (int32_t i = 0; i < n; ++i) {
             s->i_tiles[i].x = (int32_t)tile_inst.size();
-            for (int32_t k = 0; k < s->i_nreal[i]; k += tile) {
+            if (leaves[i].empty())
+                for (int32_t k = 0; k < s->i_nreal[i]; k += tile) leaves[i].push_back(std::min(tile, s->i_nreal[i] - k));
+            int32_t k = 0;
+            for (const int32_t sz : leaves[i]) {
                 tile_c0.push_back(s->i_real0[i] + k);
                 tile_inst.push_back(i);
+                sizes.push_back(sz);
+                k += sz;
             }
+            if (k != s->i_nreal[i]) { set_error("internal: tile sizes"); return FVH_E_UNSUPPORTED; }
             s->i_tiles[i].y = (int32_t)tile_inst.size();
         }
         s->n_rt = (int32_t)tile_inst.size();
@@ -1027,6 +1043,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             for (int32_t k = 0; k < s->i_nghost[i]; k += kBlock) {
                 tile_c0.push_back(s->i_ghost0[i] + k);
                 tile_inst.push_back(i);
+                sizes.push_back(std::min(kBlock, s->i_nghost[i] - k));
             }
             s->i_tiles[i].w = (int32_t)tile_inst.size();
         }
@@ -1035,12 +1052,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         // ends because instances are packed back to back within the real and
         // the ghost ranges
         std::vector<int32_t> ends(s->n_all);
-        for (int32_t t = 0; t < s->n_all; ++t) {
-            const int32_t i = tile_inst[t];
-            const bool real = t < s->n_rt;
-            const int32_t lim = real ? s->i_real0[i] + s->i_nreal[i] : s->i_ghost0[i] + s->i_nghost[i];
-            ends[t] = std::min(tile_c0[t] + (real ? tile : kBlock), lim);
-        }
+        for (int32_t t = 0; t < s->n_all; ++t) ends[t] = tile_c0[t] + sizes[t];
         for (int32_t t = 0; t + 1 < s->n_all; ++t)
             if (ends[t] != tile_c0[t + 1]) { set_error("internal: tile layout"); return FVH_E_UNSUPPORTED; }
         tile_c0.push_back(s->n_all ? ends[s->n_all - 1] : 0);
@@ -1167,7 +1179,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     const bool fusable = !(c.flags & FVH_FLAG_SMOOTH) && c.flux != FVH_FLUX_ROE_VISCOUS && c.dt_kind <= FVH_DT_EDGE_MIN;
     if ((c.flux_mode == FVH_FLUX_MODE_FUSED || c.flux_mode == FVH_FLUX_MODE_AUTO) && s->use_tiles && s->second &&
         lin_on && fusable && build_halo(s.get(), tile_c0, off, he_nbr, t_off, t_he, hh)) {
-        s->fused = halo_lds_bytes(s.get(), hh) <= 160 * 1024;
+        s->fused = halo_lds_bytes(hh) <= 160 * 1024;
     }
     // Automatic tile size (tile_cells 0): tiles of 256 cells first.  When an
     // average tile with its real ring-1 cells outnumbers 256 lanes, the
@@ -1179,7 +1191,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
     // 12% faster; the 2 M-cell airfoil: 208 cells, 256 lanes, 5% faster).
     if (c.tile_cells == 0 && s->fused && c.renumber == FVH_ORDER_TILES && s->n_real > 0 && s->n_rt > 0 &&
         c.stage_threads != kStageBlock) {
-        const size_t lds = halo_lds_bytes(s.get(), hh);
+        const size_t lds = halo_lds_bytes(hh);
         const int32_t mean_g = (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt);
         const int half = kStageBlock / 2;
         const int lanes = stage_lanes(lds, c.stage_threads, mean_g);
@@ -1193,6 +1205,30 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
             return create(n, ms, shrunk.data(), out);
         }
     }
+    // Oversized tiles: when a few tiles' halos set an LDS size that lets one
+    // 256-lane workgroup fewer share a CU than the others would need, those
+    // tiles are bisected once more (the 2 M-cell airfoil: 74 of 9,616 tiles
+    // above 40 KB, up to 51 KB, so three workgroups per CU instead of four).
+    if (s->fused && c.renumber == FVH_ORDER_TILES && !splits && s->n_rt > 0) {
+        const size_t lds = halo_lds_bytes(hh);
+        const int32_t mean_g = (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt);
+        const int per_cu = (int)((160 * 1024) / (lds + 1024));
+        if (stage_lanes(lds, c.stage_threads, mean_g) == kStageBlock / 2 && per_cu < 4) {
+            const size_t target = (160 * 1024) / (per_cu + 1) - 1024;
+            std::vector<std::vector<uint8_t>> mark(n);
+            int32_t over = 0;
+            for (int32_t t = 0; t < s->n_rt; ++t) {
+                const int32_t i = tile_inst[t];
+                const int4 x = hh.ext[t];
+                const int32_t n_ext = (tile_c0[t + 1] - tile_c0[t]) + x.z + x.w;
+                if (halo_lds_bytes(s.get(), n_ext, t_off[t + 1] - t_off[t]) <= target) continue;
+                if (mark[i].empty()) mark[i].assign(s->i_tiles[i].y - s->i_tiles[i].x, 0);
+                mark[i][t - s->i_tiles[i].x] = 1;
+                ++over;
+            }
+            if (over > 0 && over * 100 <= 3 * s->n_rt) return create(n, ms, cs, out, &mark);
+        }
+    }
     if (c.flux_mode == FVH_FLUX_MODE_FUSED && !s->fused && s->n_real > 0) {
         set_error("fused stage pass unavailable: needs linear reconstruction, an inviscid flux and time step, no "
                   "residual smoothing and a tile halo that fits in LDS");
@@ -1203,6 +1239,7 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         s->x_max_g = hh.max_g;
         s->x_mean_g = s->n_rt ? (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt) : 0;
         s->x_max_edges = hh.max_edges;
+        s->x_lds = (int32_t)hh.max_lds;
         if ((rc = upload(s->x_ext, hh.ext.data(), hh.ext.size() * sizeof(int4)))) return rc;
         if ((rc = upload(s->x_cells, hh.cells.data(), hh.cells.size() * 4))) return rc;
         if ((rc = upload(s->x_r1_start, hh.r1_start.data(), hh.r1_start.size() * 4))) return rc;

@@ -603,6 +603,28 @@ def test_automatic_tile_size(precision):
         assert bits_equal(auto.get_state(), qo)
 
 
+def test_oversized_tiles_are_bisected():
+    """200,000-cell airfoil with the default layout: 208-cell tiles, of which
+    a few in the quad band have halos needing more than 40 KB of LDS (three
+    256-lane workgroups per CU); the library bisects those tiles, so every
+    tile fits four workgroups per CU, and the state keeps the oracle's bits."""
+    print("case oversized tiles", flush=True)
+    m, cfg, ic = workloads.naca_airfoil(200_000)
+    pm = Mesh.from_arrays(m)
+    s = Solver(pm, cfg)
+    info = s.stage_info
+    print("layout", info, flush=True)
+    assert info["fused"] == 1 and info["lanes"] == 256 and info["tile_cells"] == 208
+    assert info["lds_bytes"] + 1024 <= 160 * 1024 // 4
+    full = -(-pm.n_real // info["tile_cells"])
+    assert full < info["tiles"] <= full + full * 3 // 100
+    q0 = workloads.bump_state(pm, u=0.8, v=0.0175)
+    s.set_state(q0)
+    s.run(max_step=3)
+    qo, *_ = O.Mesh(m).run(cfg, q0, 3)
+    assert bits_equal(s.get_state(), qo)
+
+
 def test_autotune_returns_a_working_choice():
     from fvhyper_amd import autotune
     m, cfg, ic = workloads.naca_airfoil(20000)
