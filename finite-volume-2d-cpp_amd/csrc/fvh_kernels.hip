@@ -187,14 +187,15 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
 
 // ---------------------------------------------------------------------------
 // Green-Gauss gradient + neighbour min/max + limiter of one real cell
-// (explicit.cpp:112-258), optionally its time step.  nbr(h) yields the state
-// across half-edge h and whether that neighbour is the cell itself; the
-// two-pass and the fused forms differ only in where it reads from.
+// (explicit.cpp:112-258), optionally its time step.  nidx(h) yields where
+// the neighbour across half-edge h is read from, nget(i, self) its state
+// and whether it is the cell itself; the two-pass and the fused forms differ
+// only in where they read from.
 // ---------------------------------------------------------------------------
-template <class T, int PHI, int DT, class Nbr>
+template <class T, int PHI, int DT, class NIdx, class NGet>
 __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp, int grad, int lim_on, bool real,
-                                     bool do_dt, int c, int h0, int h1, const Q4<T>& qc, const G2<T>& cc, Nbr nbr,
-                                     T (&gx)[4], T (&gy)[4], T (&lim)[4], T& dt_out) {
+                                     bool do_dt, int c, int h0, int h1, const Q4<T>& qc, const G2<T>& cc, NIdx nidx,
+                                     NGet nget, T (&gx)[4], T (&gy)[4], T (&lim)[4], T& dt_out) {
     const T g = T(pp.gamma);
 #pragma unroll
     for (int k = 0; k < 4; ++k) { gx[k] = T(0); gy[k] = T(0); }
@@ -219,9 +220,14 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
             // together (indices clamped to the cell's last half-edge), so
             // that their latencies overlap
             uint32_t uev[kHeChunk];
+            int nbv[kHeChunk];
             Q4<T> geov[kHeChunk];
 #pragma unroll
-            for (int i = 0; i < kHeChunk; ++i) uev[i] = m.he_edge[min(hb + i, h1 - 1)];
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(hb + i, h1 - 1);
+                uev[i] = m.he_edge[hh];
+                nbv[i] = nidx(hh);
+            }
 #pragma unroll
             for (int i = 0; i < kHeChunk; ++i) geov[i] = m.e_geo[uev[i] & kEdgeMask];
 #pragma unroll
@@ -232,7 +238,7 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
                 const Q4<T> geo = geov[i];
                 const bool side = (ue & kSideBit) != 0;
                 bool self;
-                const Q4<T> qn = nbr(h, self);
+                const Q4<T> qn = nget(nbv[i], self);
                 if (DT == 1 && do_dt) {
                     const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
                     const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
@@ -311,12 +317,12 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
         const G2<T> cc = m.c_c[c];
         const bool real = c < m.n_real;
         T gx[4], gy[4], lim[4], dtc = T(0);
-        auto nbr = [&](int h, bool& self) {
-            const int nb = m.he_nbr[h];
+        auto nidx = [&](int h) { return m.he_nbr[h]; };
+        auto nget = [&](int nb, bool& self) {
             self = nb == c;
             return ldq<T>(q + nb);
         };
-        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, real, true, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
+        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, real, true, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim, dtc);
         if (DT) {
             dt[c] = dtc;
             vmin = (double)dtc;
@@ -683,14 +689,14 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         const int r1 = own ? 0 : hv.r1_start[ex.x + j - nt];
         const uint16_t* loc = own ? hv.he_loc + h0 : hv.r1_loc + r1;
         T gx[4], gy[4], lim[4], dtc = T(0);
-        auto nbr = [&](int h, bool& self) {
-            const int l = loc[h - h0];
+        auto nidx = [&](int h) { return (int)loc[h - h0]; };
+        auto nget = [&](int l, bool& self) {
             self = l == j;
             return ldq<T>(sq + l);
         };
         const Q4<T> qc = ldq<T>(sq + j);
         const G2<T> cc = m.c_c[c];
-        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nbr, gx, gy, lim, dtc);
+        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim, dtc);
         if (DT && own) dt[c] = dtc;
         if (rec.gx && own) {
             Q4<T> a, b, d;
