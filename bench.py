@@ -34,6 +34,7 @@ import sys
 import tempfile
 import time
 
+T_START = time.perf_counter()
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "finite-volume-2d-cpp_amd"))
 
@@ -222,12 +223,14 @@ def sweep_cpu_baseline(items, files, work, steps: int, ranks: int):
 # dispatch as 2.0 x FETCH_SIZE + WRITE_SIZE (factors from a plain device copy).
 PMC_PROFILES = {
     ("step", "f64", "stage"): ("k_stage__mach3_step_975k_fp64.md", "k_stage<double"),
-    ("step", "f32", "stage"): ("k_stage__mach3_step_975k_fp32.md", "k_stage<float"),
-    ("step", "f64", "flux"): ("k_flux_tile__mach3_step_975k_fp64.md", "k_flux_tile<double"),
     ("airfoil", "f64", "stage"): ("k_stage__naca0012_2m_fp64.md", "k_stage<double"),
-    ("sweep", "f32", "flux"): ("k_flux_tile__naca0012_sweep256_fp32.md", "k_flux_tile<float"),
-    ("sweep", "f32", "stage"): ("k_stage__naca0012_sweep256_fp32.md", "k_stage<float"),
+    ("sweep", "f32_f64", "stage"): ("k_stage__naca0012_sweep256_f32_f64.md", "k_stage<double"),
 }
+
+
+def note(msg):
+    """A progress line on stderr (the JSON line stays alone on stdout)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
 
 
 def achievable(roof):
@@ -367,6 +370,7 @@ def sweep(a, with_cpu=True):
     tuned = autotune([meshes[k] for k in sub], [cfgs[k] for k in sub], np.concatenate([q0[k] for k in sub]),
                      precision=a.precision, steps=2)
     t_tune = time.perf_counter() - t_tune
+    note("sweep: autotuned %s in %.1f s" % ({k: tuned[k] for k in ("flux_mode", "tile_cells")}, t_tune))
     b = Batch(meshes, cfgs, precision=a.precision, flux_mode=tuned["flux_mode"], tile_cells=tuned["tile_cells"])
     b.set_state(b.join(q0))
     b.apply_bounds()
@@ -384,6 +388,7 @@ def sweep(a, with_cpu=True):
     b.synchronize()
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
+    note("sweep: %d batched steps in %.3f s" % (a.steps, secs))
     ev_steps = max(2, a.steps // 10)
     kt = event_times(b, ev_steps)
     qb = b.split(b.get_state())
@@ -399,6 +404,7 @@ def sweep(a, with_cpu=True):
     for k in sample:
         qo, *_ = O.Mesh(items[k][0]).run(cfgs[k], q0[k], total)
         errs.append(rel_err(qo, qb[k]))
+    note("sweep: oracle check of %d instances done" % len(sample))
     # the same instances one by one: the library's default per-mesh path
     k_pm = a.per_mesh_steps or a.steps
 
@@ -425,7 +431,9 @@ def sweep(a, with_cpu=True):
         return C * stages * k_pm / dt / 1e6, dt
 
     v_pm, s_pm = per_mesh("auto", True, True)
+    note("sweep: per-mesh leg done")
     v_old, s_old = per_mesh("tile", False, True)
+    note("sweep: per-mesh two-pass leg done")
     value = C * stages * a.steps / secs / 1e6
     sb, gb = state_bytes(a.precision)
     ab = algorithmic_bytes(C, E, B, sb, gb)
