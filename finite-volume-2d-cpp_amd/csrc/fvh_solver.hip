@@ -1294,7 +1294,10 @@ int32_t sums_t(fvh_solver* s, double* out) {
 // ---------------------------------------------------------------------------
 // What the reverse of one step reads: the state entering every stage and
 // the step's output, each stage's gradient, limiter and qt, the cell time
-// steps and the global one.
+// steps and the global one.  A kept record (`light`) holds only the states,
+// qt and the time steps; its gradients and limiters point to the work
+// buffers and are recomputed from the stage inputs by the reverse (the
+// gradient pass writes the bits the fused stage pass computes).
 struct StepRecord {
     void* qs[kAdjMaxStages + 1];
     void* gx[kAdjMaxStages];
@@ -1303,6 +1306,7 @@ struct StepRecord {
     void* qt[kAdjMaxStages];
     void* dt;
     void* dtg;
+    bool light;
 };
 
 struct Adjoint {
@@ -1314,9 +1318,9 @@ struct Adjoint {
     StepRecord work{};   // the replay buffers
     std::vector<StepRecord> tapes;   // records kept by the forward sweep
 
-    // bytes of one kept step record
+    // bytes of one kept step record: the stage states, qt, the time steps
     size_t record_bytes() const {
-        return (size_t)(n_stage + 1 + 4 * n_stage) * qbytes + (size_t)s->n_total * 8 + 256;
+        return (size_t)(n_stage + 1 + n_stage) * qbytes + (size_t)s->n_total * 8 + 256;
     }
     StepRecord work_record() const {
         StepRecord r{};
@@ -1333,17 +1337,18 @@ struct Adjoint {
         tapes.clear();
         const size_t q = (qbytes + 255) / 256 * 256, d = ((size_t)s->n_total * 8 + 255) / 256 * 256;
         for (int i = 0; i < n; ++i) {
-            if ((int)s->a_tape.size() <= i) {
+            const size_t bytes = record_bytes() + (size_t)(2 * n_stage + 1) * 256 + d;
+            if ((int)s->a_tape.size() <= i || s->a_tape[i]->bytes < bytes) {
                 std::unique_ptr<DevBuf> b(new DevBuf());
-                if (b->alloc(record_bytes() + (size_t)(n_stage + 1 + 4 * n_stage) * 256 + d)) return oom();
-                s->a_tape.push_back(std::move(b));
+                if (b->alloc(bytes)) return oom();
+                if ((int)s->a_tape.size() <= i) s->a_tape.push_back(std::move(b));
+                else s->a_tape[i] = std::move(b);
             }
             char* p = (char*)s->a_tape[i]->p;
-            StepRecord r{};
+            StepRecord r = work_record();
+            r.light = true;
             for (int k = 0; k <= n_stage; ++k) { r.qs[k] = p; p += q; }
-            for (int k = 0; k < n_stage; ++k) {
-                r.gx[k] = p; p += q; r.gy[k] = p; p += q; r.lm[k] = p; p += q; r.qt[k] = p; p += q;
-            }
+            for (int k = 0; k < n_stage; ++k) { r.qt[k] = p; p += q; }
             r.dt = p; p += d;
             r.dtg = p;
             tapes.push_back(r);
@@ -1450,9 +1455,11 @@ struct Adjoint {
             if (gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
             for (int k = 0; k < n_stage; ++k) {
                 StageRecord<double> rec;
-                rec.gx = (Q4<double>*)R.gx[k];
-                rec.gy = (Q4<double>*)R.gy[k];
-                rec.lm = (Q4<double>*)R.lm[k];
+                if (!R.light) {
+                    rec.gx = (Q4<double>*)R.gx[k];
+                    rec.gy = (Q4<double>*)R.gy[k];
+                    rec.lm = (Q4<double>*)R.lm[k];
+                }
                 rec.qt = (Q4<double>*)R.qt[k];
                 if (!launch_stage<double>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim, -1,
                                           bcs, (const Q4<double>*)R.qs[k], (const Q4<double>*)R.qs[0],
@@ -1471,7 +1478,9 @@ struct Adjoint {
             if (s->second)
                 launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, lin, qin, (Q4<double>*)s->face.p,
                                        (Q4<double>*)s->bincr.p, k == 0 ? c.dt_kind : -1, dt, blk,
-                                       (Q4<double>*)R.gx[k], (Q4<double>*)R.gy[k], (Q4<double>*)R.lm[k], nullptr);
+                                       R.light ? nullptr : (Q4<double>*)R.gx[k],
+                                       R.light ? nullptr : (Q4<double>*)R.gy[k],
+                                       R.light ? nullptr : (Q4<double>*)R.lm[k], nullptr);
             if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
             launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs,
                                 qin, (const Q4<double>*)R.qs[0], (Q4<double>*)R.qs[k + 1],
@@ -1497,6 +1506,12 @@ struct Adjoint {
         Q4<double>* lam_q0 = (Q4<double>*)s->a_lam_q0.p;
         launch_fill<double>(st, lam_q0, s->n_total);
         for (int k = n_stage - 1; k >= 0; --k) {
+            if (R.light && s->second)
+                // the stage's gradients and limiters from its input, as the
+                // forward computed them (cells and ghosts; no face states)
+                launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, false, (const Q4<double>*)R.qs[k], nullptr,
+                                       nullptr, -1, nullptr, nullptr, (Q4<double>*)R.gx[k], (Q4<double>*)R.gy[k],
+                                       (Q4<double>*)R.lm[k], nullptr);
             launch_adj_stage<double>(st, m, tm, pp, c.phi, grad, lim, lin, (const BcParam<double>*)s->bcs.p,
                                      s->n_edges, (const int4*)s->epair.p, (const Q4<double>*)R.qs[k],
                                      (const Q4<double>*)R.qs[k + 1], (const Q4<double>*)R.gx[k],

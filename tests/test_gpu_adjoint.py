@@ -82,10 +82,11 @@ def test_checkpoint_schedule_does_not_change_the_gradient():
 
 def test_recorded_steps_do_not_change_the_gradient():
     """With the memory budget left after one checkpoint per step, the
-    forward sweep keeps the last steps' whole stage records and the reverse
-    reads them instead of replaying: same J and gradients, bit for bit, with
-    fewer forward steps; and with a cap that fits every record, none is
-    replayed."""
+    forward sweep keeps the last steps' stage records (stage states, qt and
+    time steps; the reverse recomputes gradients and limiters from them) and
+    the reverse reads them instead of replaying: same J and gradients, bit
+    for bit, with fewer forward steps; and with a cap that fits every
+    record, none is replayed."""
     print("case recorded steps", flush=True)
     pm, cfg, w, q0 = _case()
     s = Solver(pm, cfg)
@@ -94,7 +95,7 @@ def test_recorded_steps_do_not_change_the_gradient():
     J0, g0, x0, st0 = s.adjoint(q0, n, wx=w, snaps=n, nodes=True)
     assert ref[2]["recorded_steps"] == 0 and st0["recorded_steps"] == 0
     rec = ref[2]["work_bytes"]
-    per_step = 5 * 26 * pm.n_cells * 32 / 5   # about one record: 26 state-sized arrays
+    per_step = 11 * pm.n_cells * 32   # about one record: 11 state-sized arrays (6 states, 5 qt)
     for cap, want in ((ref[2]["work_bytes"] + n * pm.n_cells * 32 + 4.5 * per_step, "some"), (8e9, "all")):
         J, g, x, st = s.adjoint(q0, n, wx=w, mem_cap_bytes=cap, nodes=True)
         print("  cap %.3g: recorded %d of %d, forward steps %d" % (cap, st["recorded_steps"], n, st["forward_steps"]),
