@@ -6,7 +6,9 @@
 // branch the forward evaluation takes: the min/max of the entropy fix and of
 // the limiter, |.|, the limiter's sigma ranges and the boundary regimes are
 // differentiated piecewise, exactly as a complex-step or dual-number
-// evaluation of the same code differentiates them (no smoothing).
+// evaluation of the same code differentiates them.  The SM forms are the
+// derivatives of the smoothed kinks (FVH_FLAG_SMOOTH_KINKS: |.|_w, min_w,
+// max_w of fvh_physics.h).
 #pragma once
 
 #include "fvh_dual.h"
@@ -28,9 +30,11 @@ FVH_HD void pressure_vjp(T g, const T* q, T pb, T* qb) {
 }
 
 // Reverse of roe_flux (both spellings have the same derivative): given fb
-// (adjoint of f[4]) add to qib, qjb (4 each) and nb[2].
-template <class T>
-FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb, T* qib, T* qjb, T* nb) {
+// (adjoint of f[4]) add to qib, qjb (4 each) and nb[2].  SM: the smoothed
+// entropy cure of width w (roe_flux<T, J, true>).
+template <class T, bool SM = false>
+FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb, T* qib, T* qjb, T* nb,
+                         T w = T(0)) {
     // ---- forward recomputation (the values roe_flux forms) ----
     // The derivative is not required to round like roe_flux, so repeated
     // divisions become multiplications by a reciprocal.  The sign tests
@@ -58,13 +62,13 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     const T VL = Vi;
     const T am = V - c, bm = VL - c;
     const bool m_is_b = bm < am;
-    const T mm = m_is_b ? bm : am;
-    const T lcm = sabs(mm);
-    const T lc = sabs(V);
+    const T mm = SM ? kmin(am, bm, w) : (m_is_b ? bm : am);
+    const T lcm = SM ? kabs(mm, w) : sabs(mm);
+    const T lc = SM ? kabs(V, w) : sabs(V);
     const T ap = V + c, bp = VR + c;
     const bool p_is_b = ap < bp;
-    const T MM = p_is_b ? bp : ap;
-    const T lcp = sabs(MM);
+    const T MM = SM ? kmax(ap, bp, w) : (p_is_b ? bp : ap);
+    const T lcp = SM ? kabs(MM, w) : sabs(MM);
     const T dp = pj - pi;
     const T dV = VR - VL;
     const T rc = rho * c;
@@ -126,11 +130,25 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     T pjb = dpb, pib = -dpb;
     T VRb = dVb, VLb = -dVb;
     // lcm = |min(V - c, VL - c)|, lc = |V|, lcp = |max(V + c, VR + c)|
-    const T mmb = (mm < T(0)) ? -lcmb : lcmb;
-    if (m_is_b) { VLb = VLb + mmb; cb = cb - mmb; } else { Vb = Vb + mmb; cb = cb - mmb; }
-    Vb = Vb + ((V < T(0)) ? -lcb : lcb);
-    const T MMb = (MM < T(0)) ? -lcpb : lcpb;
-    if (p_is_b) { VRb = VRb + MMb; cb = cb + MMb; } else { Vb = Vb + MMb; cb = cb + MMb; }
+    if (SM) {
+        const T mmb = lcmb * kabs_d(mm, w);
+        const T dm = kabs_d(am - bm, w);
+        Vb = Vb + mmb * T(0.5) * (T(1) - dm);
+        VLb = VLb + mmb * T(0.5) * (T(1) + dm);
+        cb = cb - mmb;
+        Vb = Vb + lcb * kabs_d(V, w);
+        const T MMb = lcpb * kabs_d(MM, w);
+        const T dp2 = kabs_d(ap - bp, w);
+        Vb = Vb + MMb * T(0.5) * (T(1) + dp2);
+        VRb = VRb + MMb * T(0.5) * (T(1) - dp2);
+        cb = cb + MMb;
+    } else {
+        const T mmb = (mm < T(0)) ? -lcmb : lcmb;
+        if (m_is_b) { VLb = VLb + mmb; cb = cb - mmb; } else { Vb = Vb + mmb; cb = cb - mmb; }
+        Vb = Vb + ((V < T(0)) ? -lcb : lcb);
+        const T MMb = (MM < T(0)) ? -lcpb : lcpb;
+        if (p_is_b) { VRb = VRb + MMb; cb = cb + MMb; } else { Vb = Vb + MMb; cb = cb + MMb; }
+    }
     // V, VR, VL
     ub = ub + Vb * n0; vb = vb + Vb * n1; n0b = n0b + Vb * u; n1b = n1b + Vb * v;
     uRb = uRb + VRb * n0; vRb = vRb + VRb * n1; n0b = n0b + VRb * uR; n1b = n1b + VRb * vR;
@@ -184,15 +202,16 @@ FVH_HD void roe_flux_vjp(T g, const T* qi, const T* qj, T n0, T n1, const T* fb,
     nb[1] = nb[1] + n1b;
 }
 
-// Reverse of eig_sum: e = c + |u.n|, c = sqrt(p g / rho).
-template <class T>
-FVH_HD void eig_sum_vjp(T g, const T* q, T n0, T n1, T eb, T* qb) {
+// Reverse of eig_sum: e = c + |u.n|, c = sqrt(p g / rho); SM: of eig_sum_k
+// (|u.n|_w).
+template <class T, bool SM = false>
+FVH_HD void eig_sum_vjp(T g, const T* q, T n0, T n1, T eb, T* qb, T w = T(0)) {
     const T p = pressure(g, q);
     const T X = p * g / q[0];
     const T c = sqrt(X);
     const T un = q[1] / q[0] * n0 + q[2] / q[0] * n1;
     const T Xb = eb / (T(2) * c);
-    const T unb = (un < T(0)) ? -eb : eb;
+    const T unb = SM ? eb * kabs_d(un, w) : ((un < T(0)) ? -eb : eb);
     pressure_vjp(g, q, Xb * g / q[0], qb);
     qb[0] = qb[0] - Xb * X / q[0] - unb * un / q[0];
     qb[1] = qb[1] + unb * n0 / q[0];
@@ -265,11 +284,12 @@ FVH_HD void boundary_vjp_n(const BcParam<T>& P, T g, const T* q, T n0, T n1, con
     }
 }
 
-// Normal part of the reverse of eig_sum (e = c + |u.n|): nb += eb d e / d n.
-template <class T>
-FVH_HD void eig_sum_vjp_n(const T* q, T n0, T n1, T eb, T* nb) {
+// Normal part of the reverse of eig_sum (e = c + |u.n|): nb += eb d e / d n;
+// SM: of eig_sum_k.
+template <class T, bool SM = false>
+FVH_HD void eig_sum_vjp_n(const T* q, T n0, T n1, T eb, T* nb, T w = T(0)) {
     const T un = q[1] / q[0] * n0 + q[2] / q[0] * n1;
-    const T unb = (un < T(0)) ? -eb : eb;
+    const T unb = SM ? eb * kabs_d(un, w) : ((un < T(0)) ? -eb : eb);
     nb[0] = nb[0] + unb * (q[1] / q[0]);
     nb[1] = nb[1] + unb * (q[2] / q[0]);
 }

@@ -187,7 +187,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_update(MeshView<T> m, TileMap tm
 }
 
 // ---- flux pass (one thread per edge) -------------------------------------
-template <class T, bool LIN, bool GEO>
+// SM: the smoothed entropy cure (FVH_FLAG_SMOOTH_KINKS)
+template <class T, bool LIN, bool GEO, bool SM>
 __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams pp, int n_edges,
                                                      const int4* __restrict__ epair, const Q4<T>* __restrict__ qin,
                                                      const Q4<T>* __restrict__ gx, const Q4<T>* __restrict__ gy,
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams p
         }
     }
     T qib[4] = {T(0), T(0), T(0), T(0)}, qjb[4] = {T(0), T(0), T(0), T(0)}, nb[2] = {T(0), T(0)};
-    roe_flux_vjp(g, fs[0], fs[1], geo.v[0], geo.v[1], fb, qib, qjb, nb);
+    roe_flux_vjp<T, SM>(g, fs[0], fs[1], geo.v[0], geo.v[1], fb, qib, qjb, nb, T(pp.kwv));
     Q4<T> a, b;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { a.v[k] = qib[k]; b.v[k] = qjb[k]; }
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_flux(MeshView<T> m, PhysParams p
         // qt_i -= f len / A_i, qt_j += f len / A_j: the length's adjoint is
         // sum_k f_k (lam_qt_j / A_j - lam_qt_i / A_i)_k
         T f[4];
-        roe_flux<T, false>(g, f, fs[0], fs[1], geo.v[0], geo.v[1]);
+        roe_flux<T, false, SM>(g, f, fs[0], fs[1], geo.v[0], geo.v[1], T(pp.kwv));
         T ll = T(0);
 #pragma unroll
         for (int k = 0; k < 4; ++k) ll = ll + fb[k] / geo.v[2] * f[k];
@@ -337,7 +338,127 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, PhysParam
         }
     }
     // limiter reversed (explicit.cpp:174-258)
-    if (lim_on) {
+    if (lim_on && PHI == PHI_VENKAT_S) {
+        // smoothed kinks: every step of the min_w / max_w chains over the
+        // neighbours, of the blend on the sign of dqg and of the running
+        // min_w over the half-edges has a derivative; each is reversed
+        const int nh = h1 - h0;
+        const T K3a = [&] { const T Ka = T(pp.K) * m.c_sqrt_area[c]; return Ka * Ka * Ka; }();
+        const T wl = T(pp.kwl);
+        T qn[kMaxHe][4];
+        T dxs[kMaxHe], dys[kMaxHe];
+#pragma unroll
+        for (int j = 0; j < kMaxHe; ++j) {
+            const int h = min(h0 + j, h1 - 1);
+            const Q4<T> a = qin[m.he_nbr[h]];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qn[j][k] = a.v[k];
+            const G2<T> ec = m.e_c[m.he_edge[h] & kEdgeMask];
+            dxs[j] = ec.v[0] - cc.v[0];
+            dys[j] = ec.v[1] - cc.v[1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const T w = T(pp.kw[k]);
+            // forward: the extrema chains a (min_w) and b (max_w)
+            T a[kMaxHe + 1], b[kMaxHe + 1];
+            a[0] = qc.v[k];
+            b[0] = qc.v[k];
+#pragma unroll
+            for (int j = 0; j < kMaxHe; ++j) {
+                a[j + 1] = j < nh ? kmin(a[j], qn[j][k], w) : a[j];
+                b[j + 1] = j < nh ? kmax(b[j], qn[j][k], w) : b[j];
+            }
+            const T dmax = b[nh] - qc.v[k];
+            const T dmin = a[nh] - qc.v[k];
+            const T spread = (dmax - dmin) * (dmax - dmin);
+            T sig, dsig = T(0);
+            if (spread <= K3a) sig = T(1);
+            else if (spread <= T(2) * K3a) {
+                const T y = (spread / K3a - T(1));
+                sig = T(2) * y * y * y - T(3) * y * y + T(1);
+                dsig = (T(6) * y * y - T(6) * y) / K3a;   // d sig / d spread
+            } else sig = T(0);
+            // forward: per half-edge limiter values and the running minimum
+            T lv[kMaxHe], Lv[kMaxHe], run[kMaxHe + 1];
+            run[0] = T(1);
+#pragma unroll
+            for (int j = 0; j < kMaxHe; ++j) {
+                const T dqg = gx[k] * dxs[j] + gy[k] * dys[j];
+                T l = T(1);
+                if (sig < T(1)) {
+                    const T t = dqg / sqrt(dqg * dqg + w * w);
+                    const T d = T(0.5) * (dmax + dmin) + T(0.5) * (dmax - dmin) * t;
+                    l = venkat_dq(d, dqg, w * w);
+                }
+                lv[j] = l;
+                Lv[j] = sig + (T(1) - sig) * l;
+                run[j + 1] = j < nh ? kmin(run[j], Lv[j], wl) : run[j];
+            }
+            // reverse of the running minimum
+            T rb = llim[k];
+            T Lb[kMaxHe];
+#pragma unroll
+            for (int j = kMaxHe - 1; j >= 0; --j) {
+                Lb[j] = T(0);
+                if (j >= nh) continue;
+                const T dk = kabs_d(run[j] - Lv[j], wl);
+                Lb[j] = rb * T(0.5) * (T(1) + dk);
+                rb = rb * T(0.5) * (T(1) - dk);
+            }
+            // reverse of L = sig + (1 - sig) l and of l(d, dqg)
+            T sigb = T(0), dmaxb = T(0), dminb = T(0);
+#pragma unroll
+            for (int j = 0; j < kMaxHe; ++j) {
+                if (j >= nh) continue;
+                sigb = sigb + Lb[j] * (T(1) - lv[j]);
+                if (!(sig < T(1))) continue;
+                const T lb = Lb[j] * (T(1) - sig);
+                const T dqg = gx[k] * dxs[j] + gy[k] * dys[j];
+                const T s2 = dqg * dqg + w * w;
+                const T t = dqg / sqrt(s2);
+                const T d = T(0.5) * (dmax + dmin) + T(0.5) * (dmax - dmin) * t;
+                const T den = d * d + d * dqg + T(2) * dqg * dqg + w * w;
+                const T num = d * d + T(2) * d * dqg + w * w;
+                const T db = lb * ((T(2) * d + T(2) * dqg) * den - num * (T(2) * d + dqg)) / (den * den);
+                T qgb = lb * (T(2) * d * den - num * (d + T(4) * dqg)) / (den * den);
+                dmaxb = dmaxb + db * T(0.5) * (T(1) + t);
+                dminb = dminb + db * T(0.5) * (T(1) - t);
+                const T tb = db * T(0.5) * (dmax - dmin);
+                qgb = qgb + tb * w * w / (s2 * sqrt(s2));
+                lgx[k] = lgx[k] + qgb * dxs[j];
+                lgy[k] = lgy[k] + qgb * dys[j];
+                if (GEO) {
+                    // dqg = gx dx + gy dy, (dx, dy) = edge centre - cell centre
+                    G2<T> e2 = ga.hc[h0 + j];
+                    e2.v[0] = e2.v[0] + qgb * gx[k];
+                    e2.v[1] = e2.v[1] + qgb * gy[k];
+                    ga.hc[h0 + j] = e2;
+                    lcx = lcx - qgb * gx[k];
+                    lcy = lcy - qgb * gy[k];
+                }
+            }
+            // sigma(spread / K3a)
+            const T lspread = sigb * dsig;
+            if (GEO) lA = lA - sigb * dsig * spread / K3a * (T(1.5) * K3a / m.c_area[c]);
+            dmaxb = dmaxb + T(2) * (dmax - dmin) * lspread;
+            dminb = dminb - T(2) * (dmax - dmin) * lspread;
+            lqc[k] = lqc[k] - dmaxb - dminb;
+            // reverse of the extrema chains
+            T ab = dminb, bb = dmaxb;
+#pragma unroll
+            for (int j = kMaxHe - 1; j >= 0; --j) {
+                if (j >= nh) continue;
+                const T da = kabs_d(a[j] - qn[j][k], w);
+                nbc[j][k] = nbc[j][k] + ab * T(0.5) * (T(1) + da);
+                ab = ab * T(0.5) * (T(1) - da);
+                const T dbx = kabs_d(b[j] - qn[j][k], w);
+                nbc[j][k] = nbc[j][k] + bb * T(0.5) * (T(1) - dbx);
+                bb = bb * T(0.5) * (T(1) + dbx);
+            }
+            lqc[k] = lqc[k] + ab + bb;
+        }
+    } else if (lim_on) {
         // neighbour extrema and the half-edge that sets each (-1: the cell itself)
         T qmn[4], qmx[4];
         int hmn[4], hmx[4];
@@ -766,6 +887,121 @@ __global__ __launch_bounds__(kBlock) void k_fill(Q4<T>* __restrict__ a, int n) {
 
 }  // namespace
 
+namespace {
+
+// ---- smoothed global time step (FVH_FLAG_SMOOTH_KINKS) ------------------
+// dt = (sum_i dt_i^-p)^(-1/p): d dt / d dt_i = (dt / dt_i)^(p+1).  Every
+// cell's summed-CFL step (eig_sum_k, max_w) is reversed: into its own state
+// (lam_q0, its own slot) and, per half-edge, into the neighbour's (nb_dt,
+// gathered by k_adj_dt_gather).  lam_dt: the total adjoint of dt.
+__global__ void k_adj_ld_sum(const double* __restrict__ ld_blk, int n_blk, double* __restrict__ out) {
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < n_blk; i += kBlock) part[0] += ld_blk[i];
+    block_sum4(part);
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+template <class T, bool GEO>
+__global__ __launch_bounds__(kBlock) void k_adj_dt_soft(MeshView<T> m, PhysParams pp, const Q4<T>* __restrict__ q,
+                                                        const T* __restrict__ dt, const T* __restrict__ dt_g,
+                                                        const double* __restrict__ lam_dt,
+                                                        Q4<T>* __restrict__ lam_q0, Q4<T>* __restrict__ nb_dt,
+                                                        GeoAdj<T> ga) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    const T g = T(pp.gamma), wv = T(pp.kwv);
+    T y = dt_g[0] / dt[c];
+    T wgt = y;
+    for (int i = 0; i < 7; ++i) wgt = wgt * wgt;   // y^128 (p = 128)
+    const T lam_c = T(lam_dt[0]) * wgt * y;
+    const Q4<T> qc = q[c];
+    T S = T(0);
+    for (int h = h0; h < h1; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+        const Q4<T> qn = q[m.he_nbr[h]];
+        const bool side = (ue & kSideBit) != 0;
+        const T eo = eig_sum_k(g, qc.v, geo.v[0], geo.v[1], wv);
+        const T en = eig_sum_k(g, qn.v, geo.v[0], geo.v[1], wv);
+        S = S + (side ? kmax(en, eo, wv) : kmax(eo, en, wv)) * geo.v[2];
+    }
+    // dt_c = cfl A / S
+    const T dtc = T(pp.cfl) * m.c_area[c] / S;
+    const T Sb = -lam_c * dtc / S;
+    if (GEO) {
+        Q4<T> a = ga.c[c];
+        a.v[0] = a.v[0] + lam_c * dtc / m.c_area[c];
+        ga.c[c] = a;
+    }
+    T own[4] = {T(0), T(0), T(0), T(0)};
+    for (int h = h0; h < h1; ++h) {
+        const uint32_t ue = m.he_edge[h];
+        const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+        const Q4<T> qn = q[m.he_nbr[h]];
+        const T eo = eig_sum_k(g, qc.v, geo.v[0], geo.v[1], wv);
+        const T en = eig_sum_k(g, qn.v, geo.v[0], geo.v[1], wv);
+        // max_w is symmetric: d/d eo = (1 + d)/2, d/d en = (1 - d)/2 with d = |.|_w'(eo - en)
+        const T dk = kabs_d(eo - en, wv);
+        const T eob = Sb * geo.v[2] * T(0.5) * (T(1) + dk);
+        const T enb = Sb * geo.v[2] * T(0.5) * (T(1) - dk);
+        eig_sum_vjp<T, true>(g, qc.v, geo.v[0], geo.v[1], eob, own, wv);
+        T nq[4] = {T(0), T(0), T(0), T(0)};
+        eig_sum_vjp<T, true>(g, qn.v, geo.v[0], geo.v[1], enb, nq, wv);
+        nb_dt[h] = Q4<T>{{nq[0], nq[1], nq[2], nq[3]}};
+        if (GEO) {
+            // S = sum max_w(e_own, e_nbr) len: the length and, through e, the normal
+            T nb2[2] = {T(0), T(0)};
+            eig_sum_vjp_n<T, true>(qc.v, geo.v[0], geo.v[1], eob, nb2, wv);
+            eig_sum_vjp_n<T, true>(qn.v, geo.v[0], geo.v[1], enb, nb2, wv);
+            Q4<T> a = ga.he[h];
+            a.v[0] = a.v[0] + nb2[0];
+            a.v[1] = a.v[1] + nb2[1];
+            a.v[2] = a.v[2] + Sb * kmax(eo, en, wv);
+            ga.he[h] = a;
+        }
+    }
+    Q4<T> l = lam_q0[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + own[k];
+    lam_q0[c] = l;
+}
+
+// lam_q0 of each cell += the neighbours' time-step contributions to it, in
+// its half-edge order (the neighbour's half-edge back to the cell is the twin)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_adj_dt_gather(MeshView<T> m, const Q4<T>* __restrict__ nb_dt,
+                                                          Q4<T>* __restrict__ lam_q0) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.n_total) return;
+    const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
+    Q4<T> l = lam_q0[c];
+    for (int h = h0; h < h1; ++h) {
+        const Q4<T> a = nb_dt[m.he_twin[h]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) l.v[k] = l.v[k] + a.v[k];
+    }
+    lam_q0[c] = l;
+}
+
+}  // namespace
+
+template <class T>
+void launch_adj_dt_soft(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const Q4<T>* q, const T* dt,
+                        const T* dt_g, const double* ld_blk, int n_blk, double* lam_dt, Q4<T>* lam_q0, Q4<T>* nb_dt,
+                        const GeoAdj<T>& ga) {
+    if (m.n_total == 0) return;
+    hipLaunchKernelGGL(k_adj_ld_sum, dim3(1), dim3(kBlock), 0, s, ld_blk, n_blk, lam_dt);
+    const int nb = nblocks(m.n_total);
+    if (ga.he)
+        hipLaunchKernelGGL((k_adj_dt_soft<T, true>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, dt_g,
+                           (const double*)lam_dt, lam_q0, nb_dt, ga);
+    else
+        hipLaunchKernelGGL((k_adj_dt_soft<T, false>), dim3(nb), dim3(kBlock), 0, s, m, pp, q, dt, dt_g,
+                           (const double*)lam_dt, lam_q0, nb_dt, ga);
+    hipLaunchKernelGGL((k_adj_dt_gather<T>), dim3(nb), dim3(kBlock), 0, s, m, (const Q4<T>*)nb_dt, lam_q0);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -797,10 +1033,16 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
         hipLaunchKernelGGL((k_adj_update<T, false>), dim3(nrt), dim3(kBlock), 0, s, m, tm, pp, qt, dt, dt_g, alpha, L,
                            (const Q4<T>*)lam_bq, lam_q0, lam_qt, lam_dt, ld_blk, ga);
     const int ne = nblocks(n_edges);
-#define FVH_AF(L_, G) hipLaunchKernelGGL((k_adj_flux<T, L_, G>), dim3(ne), dim3(kBlock), 0, s, m, pp, n_edges, epair, \
-                                         qin, gx, gy, lm, (const Q4<T>*)lam_qt, lam_face, ga)
-    if (lin) { if (geo) FVH_AF(true, true); else FVH_AF(true, false); }
-    else { if (geo) FVH_AF(false, true); else FVH_AF(false, false); }
+#define FVH_AF(L_, G, SM_) hipLaunchKernelGGL((k_adj_flux<T, L_, G, SM_>), dim3(ne), dim3(kBlock), 0, s, m, pp, \
+                                              n_edges, epair, qin, gx, gy, lm, (const Q4<T>*)lam_qt, lam_face, ga)
+    if (phi_kind == PHI_VENKAT_S) {
+        // smoothed kinks: second order only
+        if (geo) FVH_AF(true, true, true); else FVH_AF(true, false, true);
+    } else if (lin) {
+        if (geo) FVH_AF(true, true, false); else FVH_AF(true, false, false);
+    } else {
+        if (geo) FVH_AF(false, true, false); else FVH_AF(false, false, false);
+    }
 #undef FVH_AF
 #define FVH_AG(P, L_, G) hipLaunchKernelGGL((k_adj_gradlim<T, P, L_, G>), dim3(xcd_grid(m.n_real)), dim3(kBlock), 0, s, \
                                             m, pp,                                                                   \
@@ -815,6 +1057,9 @@ void launch_adj_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, co
         case PHI_ZERO: FVH_AGL(PHI_ZERO); break;
         case PHI_MICHALAK: FVH_AGL(PHI_MICHALAK); break;
         case PHI_VENKAT: FVH_AGL(PHI_VENKAT); break;
+        case PHI_VENKAT_S:
+            if (geo) FVH_AG(PHI_VENKAT_S, true, true); else FVH_AG(PHI_VENKAT_S, true, false);
+            break;
         default: FVH_AGL(PHI_MINMOD); break;
     }
 #undef FVH_AGL
@@ -888,6 +1133,9 @@ template void launch_adj_stage<double>(hipStream_t, const MeshView<double>&, con
                                        const GeoAdj<double>&);
 template void launch_adj_dt<double>(hipStream_t, const MeshView<double>&, const PhysParams&, int, const Q4<double>*,
                                     const int32_t*, const double*, int, Q4<double>*, const GeoAdj<double>&);
+template void launch_adj_dt_soft<double>(hipStream_t, const MeshView<double>&, const PhysParams&, const Q4<double>*,
+                                         const double*, const double*, const double*, int, double*, Q4<double>*,
+                                         Q4<double>*, const GeoAdj<double>&);
 template void launch_dt_argmin<double>(hipStream_t, const double*, const int32_t*, int, double*, int32_t*, int32_t*,
                                        int32_t*);
 template void launch_adj_bounds_init<double>(hipStream_t, const MeshView<double>&, const PhysParams&,

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fvh_physics.h"
 
 namespace fvh {
@@ -123,6 +125,11 @@ struct PhysParams {
     double mu, pr, cp, r;   // laminar viscous constants (viscous flux and time steps)
     int positivity;   // first-order fallback for non-physical face states
     int diffusive;    // face gradients for the viscous flux
+    // widths of the smoothed kinks (FVH_FLAG_SMOOTH_KINKS): per conserved
+    // variable (limiter extrema and the sign blend of the gradient
+    // projection), wave speeds (entropy cure, summed-CFL time step), and the
+    // limiter's running minimum (dimensionless)
+    double kw[4], kwv, kwl;
     // per instance: set to 1 by any stage update that leaves a real cell with
     // rho <= 0, p <= 0 or a non-finite value (null: no check)
     uint32_t* unphysical;
@@ -139,6 +146,15 @@ void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_ki
 // Per-instance minimum of the time step over real and ghost cells (min_dt).
 void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells,
                    int dt_is_f32, void* dt_global, const Ctrl* ctrl);
+// Smoothed kinks: replaces each instance's minimum in dt_global (from
+// launch_dt_min) by the p-norm (sum_i dt_i^-128)^(-1/128) of its cells'
+// steps (FP64; part: two words per tile).
+void launch_dt_soft(hipStream_t s, const TileMap& tm, const InstView& iv, const double* dt_cells, double* dt_global,
+                    uint64_t* part, const Ctrl* ctrl);
+// The kernels with smoothed kinks (internal kinds PHI_VENKAT_S, flux 3 and 4,
+// time step 5) exist for FP64 states and arithmetic only.
+template <class T, class S>
+constexpr bool kSmoothable = std::is_same<T, double>::value && std::is_same<S, double>::value;
 // Gradient + limiter + face states; with dt_kind >= 0 also the time step of
 // every cell and per-tile minima (first stage of a step).  dbg_gx/dbg_gy
 // (when non-null) receive every cell's gradient, zero for ghosts; dbg_lim
@@ -279,6 +295,13 @@ void launch_dt_argmin(hipStream_t s, const T* dt, const int32_t* ref, int n, dou
 template <class T>
 void launch_adj_dt(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, int dt_kind, const Q4<T>* q,
                    const int32_t* cell, const double* ld_blk, int n_blk, Q4<T>* lam_q0, const GeoAdj<T>& ga);
+// Smoothed kinks: the reverse of the p-norm global time step over every
+// cell (ld_blk: the per-tile partials of its adjoint, summed into lam_dt;
+// nb_dt: one state per half-edge of scratch).
+template <class T>
+void launch_adj_dt_soft(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const Q4<T>* q, const T* dt,
+                        const T* dt_g, const double* ld_blk, int n_blk, double* lam_dt, Q4<T>* lam_q0, Q4<T>* nb_dt,
+                        const GeoAdj<T>& ga);
 template <class T>
 void launch_adj_bounds_init(hipStream_t s, const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs,
                             const Q4<T>* q, Q4<T>* lam, const GeoAdj<T>& ga);

@@ -17,9 +17,11 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
     if (KIND == 0) return T(pp.dt_fixed);
     const T g = T(pp.gamma);
     const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-    if (KIND == 1) {
-        // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184)
+    if (KIND == 1 || KIND == 5) {
+        // summed form: cfl*A / sum_e max(eig_i, eig_j)*len  (forwardStep/main.cpp:149-184);
+        // KIND 5: its smoothed form, |.|_w and max_w (FVH_FLAG_SMOOTH_KINKS)
         const EigCell<T> eo_c = eig_cell(g, qc.v);
+        const T wv = T(pp.kwv);
         T s = T(0);
         for (int hb = h0; hb < h1; hb += kHeChunk) {
             // a chunk of half-edges: records, geometry and neighbour states
@@ -43,9 +45,15 @@ __device__ inline T cell_dt(const MeshView<T>& m, const PhysParams& pp, int c, c
                 if (hb + i >= h1) break;
                 const Q4<T>& geo = geov[i];
                 const bool side = (uev[i] & kSideBit) != 0;
-                const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
-                const T en = eig_sum(g, qnv[i].v, geo.v[0], geo.v[1]);
-                s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+                if (KIND == 5) {
+                    const T eo = eig_sum_k(eo_c, geo.v[0], geo.v[1], wv);
+                    const T en = eig_sum_k(g, qnv[i].v, geo.v[0], geo.v[1], wv);
+                    s = s + (side ? kmax(en, eo, wv) : kmax(eo, en, wv)) * geo.v[2];
+                } else {
+                    const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
+                    const T en = eig_sum(g, qnv[i].v, geo.v[0], geo.v[1]);
+                    s = s + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+                }
             }
         }
         return T(pp.cfl) * m.c_area[c] / s;
@@ -127,9 +135,75 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(InstView iv, const double* __
     }
 }
 
+// Smoothed global time step (FVH_FLAG_SMOOTH_KINKS): the p-norm
+// dt = (sum_i dt_i^-p)^(-1/p), p = 2^kSoftPow2, over an instance's cells
+// (ghosts included, as min_dt), evaluated as m / S^(1/p) with m the minimum
+// and S = sum_i (m / dt_i)^p.  Each term lies in [0, 1] and is added in
+// 64.64 fixed point, so S does not depend on the summation order; S^(1/p) is
+// kSoftPow2 square roots.  A NaN term (a NaN time step anywhere) makes the
+// step NaN.
+constexpr int kSoftPow2 = 7;
+__device__ inline unsigned __int128 soft_term(double m, double d) {
+    double y = m / d;
+    if (!(y >= 0.0 && y <= 1.0)) return (unsigned __int128)1 << 100;
+    for (int i = 0; i < kSoftPow2; ++i) y = y * y;
+    if (y >= 1.0) return (unsigned __int128)1 << 64;
+    return (unsigned __int128)(uint64_t)(y * 18446744073709551616.0);
+}
+
+__device__ inline unsigned __int128 block_sum_u128(unsigned __int128 v) {
+    __shared__ unsigned __int128 red[kBlock];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// per tile: the fixed-point sum of its cells' terms (two words per tile)
+__global__ __launch_bounds__(kBlock) void k_dt_soft_tile(TileMap tm, const double* __restrict__ dt,
+                                                         const double* __restrict__ dt_min,
+                                                         uint64_t* __restrict__ part, const Ctrl* __restrict__ ctrl) {
+    const int lb = xcd_block(tm.n_all);
+    if (lb < 0 || stopped(ctrl, tm.inst[lb])) return;
+    const int c = tm.c0[lb] + threadIdx.x;
+    const unsigned __int128 v = c < tm.c0[lb + 1] ? soft_term(dt_min[tm.inst[lb]], dt[c]) : 0;
+    const unsigned __int128 t = block_sum_u128(v);
+    if (threadIdx.x == 0) {
+        part[2 * lb] = (uint64_t)t;
+        part[2 * lb + 1] = (uint64_t)(t >> 64);
+    }
+}
+
+// per instance: S over its tiles, then dt = m / S^(1/p) in place of m
+__global__ __launch_bounds__(kBlock) void k_dt_soft_inst(InstView iv, const uint64_t* __restrict__ part,
+                                                         double* __restrict__ dt_global,
+                                                         const Ctrl* __restrict__ ctrl) {
+    const int i = blockIdx.x;
+    if (stopped(ctrl, i)) return;
+    const int4 r = iv.tiles[i];
+    unsigned __int128 v = 0;
+    for (int t = r.x + threadIdx.x; t < r.y; t += kBlock)
+        v += ((unsigned __int128)part[2 * t + 1] << 64) | part[2 * t];
+    for (int t = r.z + threadIdx.x; t < r.w; t += kBlock)
+        v += ((unsigned __int128)part[2 * t + 1] << 64) | part[2 * t];
+    const unsigned __int128 S = block_sum_u128(v);
+    if (threadIdx.x == 0) {
+        double root = (double)(uint64_t)(S >> 64) + (double)(uint64_t)S * 0x1p-64;
+        for (int k = 0; k < kSoftPow2; ++k) root = sqrt(root);
+        const double m = dt_global[i];
+        dt_global[i] = (S >> 100) != 0 ? __builtin_nan("") : m / root;
+    }
+}
+
 // Limiter of one real cell from its neighbour min/max and its gradient
 // (explicit.cpp:207-257): lim[k] = min over the cell's half-edges of
 // sigma + (1 - sigma) phi(r), starting from the reset value in lim.
+// PHI_VENKAT_S (smoothed kinks): r's numerator blends dmax and dmin by
+// t = dqg / sqrt(dqg^2 + w^2) instead of choosing by the sign of dqg,
+// phi = venkat_dq regularised by w^2, and the running minimum is min_w.
 template <class T, int PHI>
 __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, int c, int h0, int h1, const Q4<T>& qc,
                                   const G2<T>& cc, const T (&qmn)[4], const T (&qmx)[4], const T (&gx)[4],
@@ -175,11 +249,21 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
                 const T sig = sigs[k];
                 T l = T(1);
                 if (sig < T(1)) {
-                    if (dqg > tol) l = phi<T, PHI>(dmax / dqg, yt);
-                    else if (dqg < -tol) l = phi<T, PHI>(dmin / dqg, yt);
+                    if (PHI == PHI_VENKAT_S) {
+                        // smoothed kinks: the extremum by a blend on the sign of
+                        // dqg, phi(d / dqg) without the quotient
+                        const T w = T(pp.kw[k]);
+                        const T t = dqg / sqrt(dqg * dqg + w * w);
+                        const T d = T(0.5) * (dmax + dmin) + T(0.5) * (dmax - dmin) * t;
+                        l = venkat_dq(d, dqg, w * w);
+                    } else if (dqg > tol) {
+                        l = phi<T, PHI>(dmax / dqg, yt);
+                    } else if (dqg < -tol) {
+                        l = phi<T, PHI>(dmin / dqg, yt);
+                    }
                 }
                 l = sig + (T(1) - sig) * l;
-                lim[k] = smin(lim[k], l);
+                lim[k] = PHI == PHI_VENKAT_S ? kmin(lim[k], l, T(pp.kwl)) : smin(lim[k], l);
             }
         }
     }
@@ -263,8 +347,13 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
                 if (lim_on) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        qmn[k] = smin(qmn[k], qn.v[k]);
-                        qmx[k] = smax(qmx[k], qn.v[k]);
+                        if (PHI == PHI_VENKAT_S) {
+                            qmn[k] = kmin(qmn[k], qn.v[k], T(pp.kw[k]));
+                            qmx[k] = kmax(qmx[k], qn.v[k], T(pp.kw[k]));
+                        } else {
+                            qmn[k] = smin(qmn[k], qn.v[k]);
+                            qmx[k] = smax(qmx[k], qn.v[k]);
+                        }
                     }
                 }
                 if (!grad || self) continue;   // the reference skips i == j edges
@@ -409,7 +498,8 @@ __device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, c
     }
 }
 
-// Edge flux of kind FK (FVH_FLUX_*) from the two face states; the viscous
+// Edge flux of kind FK (FVH_FLUX_*, or 3 and 4: the split and joined Roe
+// fluxes with smoothed kinks) from the two face states; the viscous
 // kind adds the laminar terms with the face gradient of cells ci (side 0)
 // and cj (side 1), or a zero gradient without pp.diffusive
 // (explicit.cpp:324-344).
@@ -417,7 +507,8 @@ template <class T, int FK, class S>
 __device__ inline void edge_flux(const MeshView<T>& m, const PhysParams& pp, T g, T* f, const T* qi, const T* qj,
                                  T n0, T n1, int ci, int cj, const Q4<S>* __restrict__ qcell,
                                  const Q4<T>* __restrict__ gxc, const Q4<T>* __restrict__ gyc) {
-    roe_flux<T, FK == 1>(g, f, qi, qj, n0, n1);
+    // FK 3, 4: the split and joined spellings with the smoothed entropy cure
+    roe_flux<T, FK == 1 || FK == 4, (FK >= 3)>(g, f, qi, qj, n0, n1, T(pp.kwv));
     if (FK == 2) {
         T gxv[4] = {T(0), T(0), T(0), T(0)}, gyv[4] = {T(0), T(0), T(0), T(0)};
         if (pp.diffusive) {
@@ -764,7 +855,7 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         const int sl = k - e0;
         const Q4<T> a = fs[2 * sl], b = fs[2 * sl + 1];
         T f[4];
-        roe_flux<T, JOINED>(g, f, a.v, b.v, geo.v[0], geo.v[1]);
+        roe_flux<T, JOINED, PHI == PHI_VENKAT_S>(g, f, a.v, b.v, geo.v[0], geo.v[1], T(pp.kwv));
         Q4<T> r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
@@ -1080,11 +1171,24 @@ void launch_dt(hipStream_t s, const MeshView<T>& m, const TileMap& tm, int dt_ki
                const Q4<S>* q, T* dt, double* blk_min, const Ctrl* ctrl) {
     const int nb = xcd_grid_blocks(tm.n_all);
     if (nb == 0) return;
+    if (dt_kind == 5) {
+        if constexpr (kSmoothable<T, S>)
+            hipLaunchKernelGGL((k_dt<T, 5, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+        return;
+    }
     if (dt_kind == 0) hipLaunchKernelGGL((k_dt<T, 0, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else if (dt_kind == 1) hipLaunchKernelGGL((k_dt<T, 1, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else if (dt_kind == 3) hipLaunchKernelGGL((k_dt<T, 3, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else if (dt_kind == 4) hipLaunchKernelGGL((k_dt<T, 4, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
     else hipLaunchKernelGGL((k_dt<T, 2, S>), dim3(nb), dim3(kBlock), 0, s, m, tm, pp, q, dt, blk_min, ctrl);
+}
+
+void launch_dt_soft(hipStream_t s, const TileMap& tm, const InstView& iv, const double* dt_cells, double* dt_global,
+                    uint64_t* part, const Ctrl* ctrl) {
+    if (iv.n == 0 || tm.n_all == 0) return;
+    hipLaunchKernelGGL(k_dt_soft_tile, dim3(xcd_grid_blocks(tm.n_all)), dim3(kBlock), 0, s, tm, dt_cells,
+                       (const double*)dt_global, part, ctrl);
+    hipLaunchKernelGGL(k_dt_soft_inst, dim3(iv.n), dim3(kBlock), 0, s, iv, (const uint64_t*)part, dt_global, ctrl);
 }
 
 void launch_dt_min(hipStream_t s, const InstView& iv, const double* blk_min, const void* dt_cells, int dt_is_f32,
@@ -1118,6 +1222,10 @@ void launch_gradlim(hipStream_t s, const MeshView<T>& m, const TileMap& tm, cons
         case PHI_ZERO: FVH_GLD(PHI_ZERO); break;
         case PHI_MICHALAK: FVH_GLD(PHI_MICHALAK); break;
         case PHI_VENKAT: FVH_GLD(PHI_VENKAT); break;
+        case PHI_VENKAT_S:
+            // smoothed kinks: FP64 only, the time step from the time-step kernel
+            if constexpr (kSmoothable<T, S>) FVH_GL(PHI_VENKAT_S, 0);
+            break;
         default: FVH_GLD(PHI_MINMOD); break;
     }
 #undef FVH_GLD
@@ -1140,6 +1248,11 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
             if (lin) FVH_FT(1, true); else FVH_FT(1, false);
         } else if (flux_kind == 2) {
             if (lin) FVH_FT(2, true); else FVH_FT(2, false);
+        } else if (flux_kind == 3 || flux_kind == 4) {
+            if constexpr (kSmoothable<T, S>) {
+                if (flux_kind == 3) { if (lin) FVH_FT(3, true); else FVH_FT(3, false); }
+                else { if (lin) FVH_FT(4, true); else FVH_FT(4, false); }
+            }
         } else {
             if (lin) FVH_FT(0, true); else FVH_FT(0, false);
         }
@@ -1153,6 +1266,11 @@ void launch_flux(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const T
         if (lin) FVH_FX(1, true); else FVH_FX(1, false);
     } else if (flux_kind == 2) {
         if (lin) FVH_FX(2, true); else FVH_FX(2, false);
+    } else if (flux_kind == 3 || flux_kind == 4) {
+        if constexpr (kSmoothable<T, S>) {
+            if (flux_kind == 3) { if (lin) FVH_FX(3, true); else FVH_FX(3, false); }
+            else { if (lin) FVH_FX(4, true); else FVH_FX(4, false); }
+        }
     } else {
         if (lin) FVH_FX(0, true); else FVH_FX(0, false);
     }
@@ -1232,6 +1350,10 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
         case PHI_ZERO: return FVH_STD(PHI_ZERO);
         case PHI_MICHALAK: return FVH_STD(PHI_MICHALAK);
         case PHI_VENKAT: return FVH_STD(PHI_VENKAT);
+        case PHI_VENKAT_S:
+            // smoothed kinks: FP64 only, the time step from the time-step kernel
+            if constexpr (kSmoothable<T, S>) return FVH_STJ(PHI_VENKAT_S, 0);
+            return false;
         default: return FVH_STD(PHI_MINMOD);
     }
 #undef FVH_STD

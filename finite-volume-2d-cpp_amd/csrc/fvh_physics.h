@@ -28,6 +28,31 @@ template <class T> FVH_HD T sabs(T a) { return fabs(a); }
 
 template <class T> struct Vec4 { T v[4]; };
 
+// Smoothed kinks (FVH_FLAG_SMOOTH_KINKS; an extension with no reference
+// counterpart, for derivative checks): with a width w > 0,
+//   |x|_w = x^2 / sqrt(x^2 + w^2),  min_w(a, b) = (a + b - |a - b|_w) / 2,
+//   max_w(a, b) = (a + b + |a - b|_w) / 2.
+// Infinitely differentiable; |0|_w = 0 and min_w(a, a) = max_w(a, a) = a
+// exactly, and |x|_w -> |x|, min_w -> min, max_w -> max for |x|, |a - b| >> w.
+// min_w and max_w are symmetric bit for bit ((b-a)^2 rounds as (a-b)^2).
+template <class T> FVH_HD T kabs(T x, T w) { return x * x / sqrt(x * x + w * w); }
+template <class T> FVH_HD T kmin(T a, T b, T w) { return T(0.5) * (a + b - kabs(a - b, w)); }
+template <class T> FVH_HD T kmax(T a, T b, T w) { return T(0.5) * (a + b + kabs(a - b, w)); }
+// d|x|_w/dx = x (x^2 + 2 w^2) / (x^2 + w^2)^(3/2), in (-1, 1); the partial
+// derivatives of min_w(a, b) are (1 - d)/2 and (1 + d)/2 with d = kabs_d(a - b),
+// those of max_w(a, b) (1 + d)/2 and (1 - d)/2.
+template <class T> FVH_HD T kabs_d(T x, T w) {
+    const T s2 = x * x + w * w;
+    return x * (x * x + T(2) * w * w) / (s2 * sqrt(s2));
+}
+// Venkatakrishnan's phi(d / q) = (r^2 + 2r) / (r^2 + r + 2) with r = d / q,
+// written without the quotient and regularised by e2 > 0 as in his original
+// form: (d^2 + 2 d q + e2) / (d^2 + d q + 2 q^2 + e2), smooth everywhere
+// (value 1 at d = q = 0, where the quotient form is singular).
+template <class T> FVH_HD T venkat_dq(T d, T q, T e2) {
+    return (d * d + T(2) * d * q + e2) / (d * d + d * q + T(2) * q * q + e2);
+}
+
 // Pressure of an ideal gas, calc_p (examples/forwardStep/main.cpp:37-39).
 template <class T>
 FVH_HD T pressure(T g, const T* q) {
@@ -38,8 +63,10 @@ FVH_HD T pressure(T g, const T* q) {
 //  joined=false: examples/shockTube/main.cpp:68-135 (also naca/main.cpp:71-137)
 //  joined=true : examples/forwardStep/main.cpp:72-135
 // The two differ only in how the central flux is associated.
-template <class T, bool JOINED>
-FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1) {
+// SM: the smoothed entropy cure, |.| -> |.|_w, min/max -> min_w/max_w with
+// the velocity width w (FVH_FLAG_SMOOTH_KINKS).
+template <class T, bool JOINED, bool SM = false>
+FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1, T w = T(0)) {
     const T pi = pressure(g, qi);
     const T pj = pressure(g, qj);
     const T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
@@ -80,9 +107,9 @@ FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1) {
     const T V = u * n0 + v * n1;
     const T VR = uR * n0 + vR * n1;
     const T VL = uL * n0 + vL * n1;
-    const T lcm = sabs(smin(V - c, VL - c));
-    const T lc = sabs(V);
-    const T lcp = sabs(smax(V + c, VR + c));
+    const T lcm = SM ? kabs(kmin(V - c, VL - c, w), w) : sabs(smin(V - c, VL - c));
+    const T lc = SM ? kabs(V, w) : sabs(V);
+    const T lcp = SM ? kabs(kmax(V + c, VR + c, w), w) : sabs(smax(V + c, VR + c));
     const T dp = pR - pL;
     const T dV = VR - VL;
     const T rc = rho * c;
@@ -102,7 +129,10 @@ FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1) {
 //  Michalak:  examples/forwardStep/main.cpp:57-66 (y_t a parameter, 2 there)
 //  minmod:    examples/burgers/main.cpp:51-53
 //  Venkatakrishnan: (r^2 + 2r)/(r^2 + r + 2); no reference example.
-enum PhiKind { PHI_ZERO = 0, PHI_MICHALAK = 1, PHI_VENKAT = 2, PHI_MINMOD = 3 };
+//  PHI_VENKAT_S: the Venkatakrishnan limiter with smoothed kinks (library
+//  internal kind of FVH_FLAG_SMOOTH_KINKS; the limiter function itself is
+//  venkat_dq, see limit_cell).
+enum PhiKind { PHI_ZERO = 0, PHI_MICHALAK = 1, PHI_VENKAT = 2, PHI_MINMOD = 3, PHI_VENKAT_S = 4 };
 
 template <class T, int PHI>
 FVH_HD T phi(T y, T yt) {
@@ -113,7 +143,7 @@ FVH_HD T phi(T y, T yt) {
         const T b = T(-1.5) * a * yt - T(0.5) / yt;
         return a * y * y * y + b * y * y + y;
     }
-    if (PHI == PHI_VENKAT) return (y * y + T(2) * y) / (y * y + y + T(2));
+    if (PHI == PHI_VENKAT || PHI == PHI_VENKAT_S) return (y * y + T(2) * y) / (y * y + y + T(2));
     return smax(T(0), smin(T(1), y));
 }
 
@@ -228,6 +258,15 @@ FVH_HD T eig_sum(const EigCell<T>& e, T n0, T n1) {
 template <class T>
 FVH_HD T eig_sum(T g, const T* q, T n0, T n1) {
     return eig_sum(eig_cell(g, q), n0, n1);
+}
+// the smoothed form (FVH_FLAG_SMOOTH_KINKS): c + |u.n|_w
+template <class T>
+FVH_HD T eig_sum_k(const EigCell<T>& e, T n0, T n1, T w) {
+    return e.c + kabs(e.u * n0 + e.v * n1, w);
+}
+template <class T>
+FVH_HD T eig_sum_k(T g, const T* q, T n0, T n1, T w) {
+    return eig_sum_k(eig_cell(g, q), n0, n1, w);
 }
 //  edge-minimum local (naca/main.cpp:169-173): (c+|u|) n0 + (c+|v|) n1
 template <class T>

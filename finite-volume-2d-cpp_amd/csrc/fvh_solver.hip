@@ -108,6 +108,7 @@ struct fvh_solver {
     fvh::DevBuf x_ext, x_cells, x_r1_start, x_r1_loc, x_r1_slot, x_he_loc;
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0, x_lds = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
+    fvh::DevBuf soft_part;   // smoothed kinks: fixed-point partial sums of the p-norm time step, per tile
     fvh::DevBuf unphysical;                  // per instance: a stage left a non-physical cell state
     fvh::DevBuf dbg[4];
     fvh::DevBuf ph[6], ph_part;              // per-phase entry points: inputs and outputs, min_dt partials
@@ -231,6 +232,13 @@ TileView tile_view(fvh_solver* s) {
     return t;
 }
 
+// Kernel kinds of a configuration: with smoothed kinks, the limiter, flux and
+// summed-CFL time-step kinds of the smoothed kernels.
+inline bool kinks(const fvh_config& c) { return (c.flags & FVH_FLAG_SMOOTH_KINKS) != 0; }
+inline int kphi(const fvh_config& c) { return kinks(c) ? (int)PHI_VENKAT_S : c.phi; }
+inline int kflux(const fvh_config& c) { return kinks(c) ? c.flux + 3 : c.flux; }
+inline int kdt(const fvh_config& c) { return kinks(c) && c.dt_kind == FVH_DT_SUM_CFL ? 5 : c.dt_kind; }
+
 PhysParams phys(const fvh_config& c) {
     PhysParams p;
     p.gamma = c.gamma;
@@ -245,6 +253,23 @@ PhysParams phys(const fvh_config& c) {
     p.r = c.r_gas;
     p.diffusive = (c.flags & FVH_FLAG_DIFFUSIVE) ? 1 : 0;
     p.unphysical = nullptr;
+    // widths of the smoothed kinks: kink_width times kink_scale, a zero scale
+    // from the first characteristic boundary's freestream (or 1)
+    double sc[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+    for (int k = 0; k < c.n_bc; ++k) {
+        const fvh_bc_spec& b = c.bc[k];
+        if (b.kind != FVH_BC_INLET_OUTLET && b.kind != FVH_BC_FARFIELD && b.kind != FVH_BC_FARFIELD_V) continue;
+        const double speed = std::sqrt(b.u * b.u + b.v * b.v);
+        sc[0] = b.rho;
+        sc[1] = sc[2] = b.rho * (speed + std::sqrt(c.gamma * b.p / b.rho));
+        sc[3] = b.p / (c.gamma - 1.0) + 0.5 * b.rho * speed * speed;
+        sc[4] = speed + std::sqrt(c.gamma * b.p / b.rho);
+        break;
+    }
+    const double w = c.kink_width > 0.0 ? c.kink_width : 1e-2;
+    for (int k = 0; k < 4; ++k) p.kw[k] = w * (c.kink_scale[k] > 0.0 ? c.kink_scale[k] : sc[k]);
+    p.kwv = w * (c.kink_scale[4] > 0.0 ? c.kink_scale[4] : sc[4]);
+    p.kwl = w;
     return p;
 }
 
@@ -364,6 +389,10 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
+    if ((s->cfg.flags & FVH_FLAG_SMOOTH_KINKS) && s->soft_part.alloc((size_t)(s->n_all + 1) * 16)) {
+        set_error("hipMalloc failed");
+        return FVH_E_HIP;
+    }
     if (s->cfg.flags & FVH_FLAG_SMOOTH)
         for (auto& b : s->qt_s)
             if (b.alloc((size_t)(s->n_real + 1) * sizeof(Q4<T>))) {
@@ -390,6 +419,16 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
 // buffer s->cur.  run_mode adds the stop test and bookkeeping kernels.
 // dbg (tests): 4 device buffers for gx, gy, limiter and qt of one stage run
 // with alpha_override, leaving the current state buffer unchanged.
+// The global time step of each instance from the per-cell steps: min_dt,
+// or with smoothed kinks and a CFL step its p-norm form.
+template <class T>
+void global_dt(fvh_solver* s, hipStream_t st, const TileMap& tm, const InstView& iv, const double* blk, T* dt,
+               T* dtg, const Ctrl* cc) {
+    launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
+    if (kinks(s->cfg) && s->cfg.dt_kind != FVH_DT_FIXED)
+        launch_dt_soft(st, tm, iv, (const double*)dt, (double*)dtg, (uint64_t*)s->soft_part.p, cc);
+}
+
 template <class T, class S>
 int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int need_res,
                      Q4<T>* const* dbg = nullptr, const double* alpha_override = nullptr) {
@@ -422,11 +461,13 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
     Q4<T>* gx_c = (Q4<T>*)s->gxy[0].p;   // null unless the viscous flux takes face gradients
     Q4<T>* gy_c = (Q4<T>*)s->gxy[1].p;
     const bool fused = s->fused && !dbg;
+    const bool kk = kinks(c);
     // the time-step kernel: first order, a fixed step (the second-order
-    // passes compute only the CFL kinds) and the fused form's global step
-    if (!s->second || c.dt_kind == FVH_DT_FIXED || (fused && gdt)) {
+    // passes compute only the CFL kinds), the fused form's global step and
+    // every step with smoothed kinks
+    if (!s->second || c.dt_kind == FVH_DT_FIXED || (fused && gdt) || kk) {
         Bracket b(s, K_DT);
-        launch_dt<T>(st, m, tm, c.dt_kind, pp, q0, dt, blk, cc);
+        launch_dt<T>(st, m, tm, kdt(c), pp, q0, dt, blk, cc);
     }
     const Q4<S>* qin = q0;
     Q4<S>* out = qa;
@@ -435,13 +476,13 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         const HaloView hv = halo_view(s);
         if (gdt) {
             Bracket b(s, K_DTMIN);
-            launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
+            global_dt(s, st, tm, iv, blk, dt, dtg, cc);
         }
         for (int k = 0; k < n_stage; ++k) {
             Bracket b(s, K_STAGE);
             const bool last = k == n_stage - 1;
-            launch_stage<T>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim,
-                            (k == 0 && !gdt) ? c.dt_kind : -1, bcs, qin, q0, out, (Q4<T>*)s->bincr.p, dt,
+            launch_stage<T>(st, m, tm, tv, hv, pp, kphi(c), c.flux == FVH_FLUX_ROE_JOINED, grad, lim,
+                            (k == 0 && !gdt && !kk) ? c.dt_kind : -1, bcs, qin, q0, out, (Q4<T>*)s->bincr.p, dt,
                             gdt ? dtg : nullptr,
                             T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc,
                             c.stage_threads);
@@ -453,12 +494,13 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
         if (s->second) {
             // the first stage's pass also computes the time step from q0
             Bracket b(s, K_GRADLIM);
-            launch_gradlim<T>(st, m, tm, pp, c.phi, grad, lim, lin, qin, face, bincr, k == 0 ? c.dt_kind : -1, dt,
-                              blk, dbg ? dbg[0] : gx_c, dbg ? dbg[1] : gy_c, dbg ? dbg[2] : nullptr, cc);
+            launch_gradlim<T>(st, m, tm, pp, kphi(c), grad, lim, lin, qin, face, bincr,
+                              (k == 0 && !kk) ? c.dt_kind : -1, dt, blk, dbg ? dbg[0] : gx_c, dbg ? dbg[1] : gy_c,
+                              dbg ? dbg[2] : nullptr, cc);
         }
         if (k == 0 && gdt) {
             Bracket b(s, K_DTMIN);
-            launch_dt_min(st, iv, blk, dt, s->f32, dtg, cc);
+            global_dt(s, st, tm, iv, blk, dt, dtg, cc);
         }
         const bool last = k == n_stage - 1;
         const T a = T(alpha_override ? *alpha_override : c.alpha[k]);
@@ -471,7 +513,7 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
             Q4<T>* qs1 = (Q4<T>*)s->qt_s[1].p;
             {
                 Bracket b(s, K_FLUX);
-                launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs, qin, q0, out, face,
+                launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, kflux(c), lin, bcs, qin, q0, out, face,
                                bincr, dt, gdt ? dtg : nullptr, a, nullptr, nullptr, cc, qs0, gxv, gyv);
             }
             {
@@ -484,7 +526,7 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
                              res, dbg ? dbg[3] : nullptr, cc);
         } else {
             Bracket b(s, K_FLUX);
-            launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs, qin, q0, out, face, bincr,
+            launch_flux<T>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, kflux(c), lin, bcs, qin, q0, out, face, bincr,
                            dt, gdt ? dtg : nullptr, a, res, dbg ? dbg[3] : nullptr, cc, nullptr, gxv, gyv);
         }
         qin = out;
@@ -911,8 +953,27 @@ int32_t check_config(const fvh_config* c) {
         set_error("viscous flux or time step: pr and r_gas must be non-zero");
         return FVH_E_ARG;
     }
+    if (c->flags & FVH_FLAG_SMOOTH_KINKS) {
+        const uint32_t second = FVH_FLAG_GRADIENTS | FVH_FLAG_LIMITERS | FVH_FLAG_LINEAR;
+        if (c->precision != FVH_F64 || c->phi != FVH_PHI_VENKATAKRISHNAN || (c->flags & second) != second ||
+            c->flux == FVH_FLUX_ROE_VISCOUS || (c->dt_kind != FVH_DT_SUM_CFL && c->dt_kind != FVH_DT_FIXED)) {
+            set_error("smoothed kinks take FP64, the Venkatakrishnan limiter with gradients, limiters and linear "
+                      "reconstruction, an inviscid Roe flux and the summed-CFL or fixed time step");
+            return FVH_E_UNSUPPORTED;
+        }
+        if (!(c->kink_width >= 0.0) || !(c->kink_width < 1.0)) {
+            set_error("kink_width must lie in [0, 1)");
+            return FVH_E_ARG;
+        }
+        for (int k = 0; k < 5; ++k)
+            if (!(c->kink_scale[k] >= 0.0) || !std::isfinite(c->kink_scale[k])) {
+                set_error("kink_scale must be finite and non-negative");
+                return FVH_E_ARG;
+            }
+    }
     return FVH_OK;
 }
+
 
 // Everything but the boundary table must agree across a batch.
 bool same_physics(const fvh_config& a, const fvh_config& b) {
@@ -920,8 +981,10 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
         a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
         a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
         a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells || a.mu != b.mu || a.pr != b.pr ||
-        a.cp != b.cp || a.r_gas != b.r_gas || a.stage_threads != b.stage_threads)
+        a.cp != b.cp || a.r_gas != b.r_gas || a.stage_threads != b.stage_threads || a.kink_width != b.kink_width)
         return false;
+    for (int k = 0; k < 5; ++k)
+        if (a.kink_scale[k] != b.kink_scale[k]) return false;
     for (int k = 0; k < a.n_alpha; ++k)
         if (a.alpha[k] != b.alpha[k]) return false;
     return true;
@@ -1444,15 +1507,16 @@ struct Adjoint {
         hipStream_t st = s->stream;
         double* dt = (double*)R.dt;
         double* blk = (double*)s->blk_min.p;
+        const bool kk = kinks(c);
         // the time-step kernel fills the record's own dt (a fixed step too:
         // the gradient pass computes only the CFL kinds)
-        if (!s->second || s->fused || c.dt_kind == FVH_DT_FIXED)
-            launch_dt<double>(st, m, tm, c.dt_kind, pp, (const Q4<double>*)R.qs[0], dt, blk, nullptr);
+        if (!s->second || s->fused || c.dt_kind == FVH_DT_FIXED || kk)
+            launch_dt<double>(st, m, tm, kdt(c), pp, (const Q4<double>*)R.qs[0], dt, blk, nullptr);
         if (s->fused) {
             // the fused stage pass, storing what the two-pass form would (every
             // cell's time step comes from the time-step kernel above)
             const HaloView hv = halo_view(s);
-            if (gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
+            if (gdt) global_dt(s, st, tm, iv, blk, dt, (double*)R.dtg, nullptr);
             for (int k = 0; k < n_stage; ++k) {
                 StageRecord<double> rec;
                 if (!R.light) {
@@ -1461,7 +1525,7 @@ struct Adjoint {
                     rec.lm = (Q4<double>*)R.lm[k];
                 }
                 rec.qt = (Q4<double>*)R.qt[k];
-                if (!launch_stage<double>(st, m, tm, tv, hv, pp, c.phi, c.flux == FVH_FLUX_ROE_JOINED, grad, lim, -1,
+                if (!launch_stage<double>(st, m, tm, tv, hv, pp, kphi(c), c.flux == FVH_FLUX_ROE_JOINED, grad, lim, -1,
                                           bcs, (const Q4<double>*)R.qs[k], (const Q4<double>*)R.qs[0],
                                           (Q4<double>*)R.qs[k + 1], (Q4<double>*)s->bincr.p, dt,
                                           gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, nullptr,
@@ -1476,13 +1540,13 @@ struct Adjoint {
         for (int k = 0; k < n_stage; ++k) {
             const Q4<double>* qin = (const Q4<double>*)R.qs[k];
             if (s->second)
-                launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, lin, qin, (Q4<double>*)s->face.p,
-                                       (Q4<double>*)s->bincr.p, k == 0 ? c.dt_kind : -1, dt, blk,
+                launch_gradlim<double>(st, m, tm, pp, kphi(c), grad, lim, lin, qin, (Q4<double>*)s->face.p,
+                                       (Q4<double>*)s->bincr.p, (k == 0 && !kk) ? c.dt_kind : -1, dt, blk,
                                        R.light ? nullptr : (Q4<double>*)R.gx[k],
                                        R.light ? nullptr : (Q4<double>*)R.gy[k],
                                        R.light ? nullptr : (Q4<double>*)R.lm[k], nullptr);
-            if (k == 0 && gdt) launch_dt_min(st, iv, blk, dt, 0, R.dtg, nullptr);
-            launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, c.flux, lin, bcs,
+            if (k == 0 && gdt) global_dt(s, st, tm, iv, blk, dt, (double*)R.dtg, nullptr);
+            launch_flux<double>(st, m, tm, s->use_tiles ? &tv : nullptr, pp, kflux(c), lin, bcs,
                                 qin, (const Q4<double>*)R.qs[0], (Q4<double>*)R.qs[k + 1],
                                 (const Q4<double>*)s->face.p, (const Q4<double>*)s->bincr.p, dt,
                                 gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, (Q4<double>*)R.qt[k],
@@ -1509,10 +1573,10 @@ struct Adjoint {
             if (R.light && s->second)
                 // the stage's gradients and limiters from its input, as the
                 // forward computed them (cells and ghosts; no face states)
-                launch_gradlim<double>(st, m, tm, pp, c.phi, grad, lim, false, (const Q4<double>*)R.qs[k], nullptr,
+                launch_gradlim<double>(st, m, tm, pp, kphi(c), grad, lim, false, (const Q4<double>*)R.qs[k], nullptr,
                                        nullptr, -1, nullptr, nullptr, (Q4<double>*)R.gx[k], (Q4<double>*)R.gy[k],
                                        (Q4<double>*)R.lm[k], nullptr);
-            launch_adj_stage<double>(st, m, tm, pp, c.phi, grad, lim, lin, (const BcParam<double>*)s->bcs.p,
+            launch_adj_stage<double>(st, m, tm, pp, kphi(c), grad, lim, lin, (const BcParam<double>*)s->bcs.p,
                                      s->n_edges, (const int4*)s->epair.p, (const Q4<double>*)R.qs[k],
                                      (const Q4<double>*)R.qs[k + 1], (const Q4<double>*)R.gx[k],
                                      (const Q4<double>*)R.gy[k], (const Q4<double>*)R.lm[k],
@@ -1524,7 +1588,13 @@ struct Adjoint {
                                      (Q4<double>*)s->a_nb.p, k == 0 ? lam_q0 : nullptr, k == 0 ? 1 : 0, nullptr,
                                      (double*)s->a_ld.p + (size_t)k * s->n_rt, ga);
         }
-        if (gdt && c.dt_kind != FVH_DT_FIXED) {
+        if (gdt && c.dt_kind != FVH_DT_FIXED && kinks(c)) {
+            // the p-norm step: every cell's step reversed (a_nb is free once
+            // the stage reverses above have been gathered)
+            launch_adj_dt_soft<double>(st, m, pp, (const Q4<double>*)R.qs[0], (const double*)R.dt,
+                                       (const double*)R.dtg, (const double*)s->a_ld.p, n_stage * s->n_rt,
+                                       (double*)s->a_amin_v.p, lam_q0, (Q4<double>*)s->a_nb.p, ga);
+        } else if (gdt && c.dt_kind != FVH_DT_FIXED) {
             // the cell whose step is the minimum, first in reference order (min_dt, explicit.cpp:555-564)
             launch_dt_argmin<double>(st, (const double*)R.dt, (const int32_t*)s->a_ref.p, s->n_total,
                                      (double*)s->a_amin_v.p, (int32_t*)s->a_amin_r.p, (int32_t*)s->a_amin_d.p,
@@ -1753,6 +1823,10 @@ int32_t scratch_address(double** out) {
 int32_t phase_setup(fvh_solver* s) {
     if (s->f32 || s->mixed) { set_error("the per-phase functions take FP64 solvers"); return FVH_E_UNSUPPORTED; }
     if (s->n_inst != 1) { set_error("the per-phase functions take single-instance solvers"); return FVH_E_UNSUPPORTED; }
+    if (kinks(s->cfg)) {
+        set_error("the per-phase functions are the reference's phases; smoothed kinks run through run/advance");
+        return FVH_E_UNSUPPORTED;
+    }
     const size_t qb = (size_t)s->n_total * sizeof(Q4<double>);
     for (auto& b : s->ph)
         if (!b.p && b.alloc(qb)) { set_error("hipMalloc failed"); return FVH_E_HIP; }

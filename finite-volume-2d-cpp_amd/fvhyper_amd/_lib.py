@@ -23,6 +23,7 @@ DT = {"fixed": 0, "sum_cfl": 1, "edge_min": 2, "edge_min_viscous": 3, "sum_visco
 BC = {"copy": 0, "mirror": 1, "inlet_outlet": 2, "farfield": 3, "noslip": 4, "farfield_v": 5}
 FLAG_GRADIENTS, FLAG_LIMITERS, FLAG_LINEAR, FLAG_GLOBAL_DT, FLAG_POSITIVITY, FLAG_SMOOTH = 1, 2, 4, 8, 16, 32
 FLAG_DIFFUSIVE = 64
+FLAG_SMOOTH_KINKS = 128
 RK5 = [0.0695, 0.1602, 0.2898, 0.506, 1.0]
 
 
@@ -41,6 +42,7 @@ class Config(C.Structure):
         ("renumber", C.c_int32), ("flux_mode", C.c_int32), ("tile_cells", C.c_int32),
         ("mu", C.c_double), ("pr", C.c_double), ("cp", C.c_double), ("r_gas", C.c_double),
         ("stage_threads", C.c_int32), ("reserved", C.c_int32),
+        ("kink_width", C.c_double), ("kink_scale", C.c_double * 5),
     ]
 
 

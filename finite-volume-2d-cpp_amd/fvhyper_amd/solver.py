@@ -117,7 +117,12 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
         flags |= _lib.FLAG_SMOOTH
     if cfg.get("diffusive", False):
         flags |= _lib.FLAG_DIFFUSIVE
+    if cfg.get("smooth_kinks", False):
+        flags |= _lib.FLAG_SMOOTH_KINKS
     c.flags = flags
+    c.kink_width = float(cfg.get("kink_width", 0.0))
+    for i, v in enumerate(cfg.get("kink_scale", [0.0] * 5)):
+        c.kink_scale[i] = float(v)
     for k in ("mu", "pr", "cp", "r_gas"):
         if k in cfg:
             setattr(c, k, float(cfg[k]))

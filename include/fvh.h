@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define FVH_ABI_VERSION 2
+#define FVH_ABI_VERSION 3
 
 enum {
     FVH_OK = 0,
@@ -93,6 +93,21 @@ enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2, F
  * reconstruction yields a face state with rho <= 0 or p <= 0, that face uses
  * the cell value instead (first order).  Off for reference parity. */
 #define FVH_FLAG_POSITIVITY 16u
+/* Extension without a reference counterpart, for derivative checks of the
+ * adjoint: every kink of the second-order path becomes an infinitely
+ * differentiable function of width w,
+ *   |x| -> x^2 / sqrt(x^2 + w^2),  min/max(a, b) -> (a + b -/+ |a - b|_w) / 2
+ * (exact at x = 0 and a = b), in the entropy cure's |min(V-c, VL-c)|, |V|
+ * and |max(V+c, VR+c)|, the summed-CFL wave speeds c + |u.n| and their max,
+ * and the limiter's neighbour extrema and running minimum; the limiter's
+ * choice of qmax or qmin by the sign of the gradient projection dqg becomes
+ * a blend by dqg / sqrt(dqg^2 + w^2) with phi(d/dqg) written without the
+ * quotient and regularised by w^2 (Venkatakrishnan's original form); the
+ * global time step's minimum (CFL kinds) becomes the p-norm
+ * (sum_i dt_i^-128)^(-1/128).  Widths: kink_width times kink_scale.  Takes
+ * the Venkatakrishnan limiter with linear reconstruction, the summed-CFL or
+ * fixed time step and FVH_F64; the positivity fallback stays a switch. */
+#define FVH_FLAG_SMOOTH_KINKS 128u
 
 #define FVH_MAX_ALPHA 8
 #define FVH_MAX_BC 16
@@ -132,6 +147,14 @@ typedef struct fvh_config {
      * (256 lanes when four workgroups fit a CU, else 512), or 256, 512 */
     int32_t stage_threads;
     int32_t reserved;          /* 0 */
+    /* FVH_FLAG_SMOOTH_KINKS: relative width (0: 1e-2) and the absolute scales
+     * it multiplies: rho, rho u, rho v, rho E (limiter extrema and blend) and
+     * a wave speed (entropy cure, time step); a scale of 0 takes the
+     * freestream of the first inlet/outlet or farfield boundary (rho,
+     * rho (|V| + c) twice, E, |V| + c), or 1 without one.  The limiter's running
+     * minimum takes the relative width itself. */
+    double kink_width;
+    double kink_scale[5];
 } fvh_config;
 
 /* solverOptions of fvhyper/explicit.h:152-160 (minus the output fields) */

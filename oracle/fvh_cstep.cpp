@@ -68,6 +68,11 @@ template <class T> inline T smin(T a, T b) { return (b < a) ? b : a; }
 template <class T> inline T smax(T a, T b) { return (a < b) ? b : a; }
 using ::fabs;
 using ::sqrt;
+// smoothed kinks (the oracle's kinks mode): |x|_w, min_w, max_w; analytic,
+// so the complex step differentiates them
+template <class T> inline T kabs(T x, double w) { return x * x / sqrt(x * x + T(w) * T(w)); }
+template <class T> inline T kmin(T a, T b, double w) { return T(0.5) * (a + b - kabs(a - b, w)); }
+template <class T> inline T kmax(T a, T b, double w) { return T(0.5) * (a + b + kabs(a - b, w)); }
 
 template <class T>
 inline T pressure(double g, const T* q) {
@@ -114,9 +119,10 @@ void flux(const orc_config* cf, T* f, const T* qi, const T* qj, const T* n) {
     const T Vn = u * n0 + v * n1;
     const T VR = uR * n0 + vR * n1;
     const T VL = uL * n0 + vL * n1;
-    const T lcm = fabs(smin(Vn - c, VL - c));
-    const T lc = fabs(Vn);
-    const T lcp = fabs(smax(Vn + c, VR + c));
+    const double w = cf->kwv;
+    const T lcm = cf->kinks ? kabs(kmin(Vn - c, VL - c, w), w) : fabs(smin(Vn - c, VL - c));
+    const T lc = cf->kinks ? kabs(Vn, w) : fabs(Vn);
+    const T lcp = cf->kinks ? kabs(kmax(Vn + c, VR + c, w), w) : fabs(smax(Vn + c, VR + c));
     const T kF1 = lcm * ((pR - pL) - rho * c * (VR - VL)) / (T(2.) * c * c);
     const T kF234_0 = lc * ((qj[0] - qi[0]) - (pR - pL) / (c * c));
     const T kF234_1 = lc * rho;
@@ -301,9 +307,16 @@ void calc_dt(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T* 
             const T* qj = &q[V * j];
             T ci = sqrt(pressure(gd, qi) * g / qi[0]);
             T cj = sqrt(pressure(gd, qj) * g / qj[0]);
-            T ei = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
-            T ej = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
-            T center = smax(ei, ej);
+            T ei, ej, center;
+            if (cf->kinks) {
+                ei = ci + kabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1, cf->kwv);
+                ej = cj + kabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1, cf->kwv);
+                center = kmax(ei, ej, cf->kwv);
+            } else {
+                ei = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
+                ej = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
+                center = smax(ei, ej);
+            }
             dt[i] += center * le;
             dt[j] += center * le;
         }
@@ -332,6 +345,43 @@ void min_dt(const orc_mesh* m, T* dt) {
     T mn = dt[0];
     for (int32_t i = 0; i < m->n_cells; ++i) mn = smin(mn, dt[i]);
     for (int32_t i = 0; i < m->n_cells; ++i) dt[i] = mn;
+}
+
+// the p-norm global step of the oracle's kinks mode: the real part as the
+// oracle forms it (fixed-point sum), the imaginary part its derivative
+// sum_i (dt / dt_i)^129 Im(dt_i)
+inline double soft_value(const orc_mesh* m, const double* re_dt) {
+    const int32_t N = m->n_cells;
+    double mn = re_dt[0];
+    for (int32_t i = 0; i < N; ++i) mn = smin(mn, re_dt[i]);
+    unsigned __int128 S = 0;
+    for (int32_t i = 0; i < N; ++i) {
+        double y = mn / re_dt[i];
+        if (!(y >= 0.0 && y <= 1.0)) { S += (unsigned __int128)1 << 100; continue; }
+        for (int k = 0; k < 7; ++k) y = y * y;
+        if (y >= 1.0) S += (unsigned __int128)1 << 64;
+        else S += (unsigned __int128)(uint64_t)(y * 18446744073709551616.0);
+    }
+    double root = (double)(uint64_t)(S >> 64) + (double)(uint64_t)S * 0x1p-64;
+    for (int k = 0; k < 7; ++k) root = ::sqrt(root);
+    return (S >> 100) != 0 ? NAN : mn / root;
+}
+inline void min_dt_soft(const orc_mesh* m, double* dt) {
+    const double v = soft_value(m, dt);
+    for (int32_t i = 0; i < m->n_cells; ++i) dt[i] = v;
+}
+inline void min_dt_soft(const orc_mesh* m, cs* dt) {
+    const int32_t N = m->n_cells;
+    std::vector<double> r(N);
+    for (int32_t i = 0; i < N; ++i) r[i] = dt[i].re;
+    const double v = soft_value(m, r.data());
+    double im = 0.0;
+    for (int32_t i = 0; i < N; ++i) {
+        double y = v / r[i], w = y;
+        for (int k = 0; k < 7; ++k) w = w * w;
+        im += w * y * dt[i].im;
+    }
+    for (int32_t i = 0; i < N; ++i) dt[i] = cs(v, im);
 }
 
 template <class T>
@@ -375,6 +425,14 @@ void limiters(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T*
     for (int32_t e = 0; e < m->n_edges; ++e) {
         int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
         for (int k = 0; k < V; ++k) {
+            if (cf->kinks) {
+                const double w = cf->kw[k];
+                qmin[V * i + k] = kmin(qmin[V * i + k], q[V * j + k], w);
+                qmin[V * j + k] = kmin(qmin[V * j + k], q[V * i + k], w);
+                qmax[V * i + k] = kmax(qmax[V * i + k], q[V * j + k], w);
+                qmax[V * j + k] = kmax(qmax[V * j + k], q[V * i + k], w);
+                continue;
+            }
             qmin[V * i + k] = smin(qmin[V * i + k], q[V * j + k]);
             qmin[V * j + k] = smin(qmin[V * j + k], q[V * i + k]);
             qmax[V * i + k] = smax(qmax[V * i + k], q[V * j + k]);
@@ -407,13 +465,18 @@ void limiters(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T*
                     sig = T(0.);
                 }
                 T l = T(1.0);
-                if (sig < T(1.0)) {
+                if (sig < T(1.0) && cf->kinks) {
+                    const double w = cf->kw[k];
+                    const T t = dqg / sqrt(dqg * dqg + T(w) * T(w));
+                    const T d = T(0.5) * (dmax + dmin) + T(0.5) * (dmax - dmin) * t;
+                    l = (d * d + T(2.0) * d * dqg + T(w) * T(w)) / (d * d + d * dqg + T(2.0) * dqg * dqg + T(w) * T(w));
+                } else if (sig < T(1.0)) {
                     if (dqg > tol) l = phi(cf, dmax / dqg);
                     else if (dqg < -tol) l = phi(cf, dmin / dqg);
                     else l = T(1.0);
                 }
                 l = sig + (T(1.0) - sig) * l;
-                lim[V * id + k] = smin(lim[V * id + k], l);
+                lim[V * id + k] = cf->kinks ? kmin(lim[V * id + k], l, cf->kwl) : smin(lim[V * id + k], l);
             }
         }
     }
@@ -516,7 +579,10 @@ void run(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, T* q, int ste
     update_bounds(m, G, cf, q, gx.data(), gy.data(), lim.data());
     for (int step = 0; step < steps; ++step) {
         calc_dt(m, G, cf, q, dt.data());
-        if (cf->gdt) min_dt(m, dt.data());
+        if (cf->gdt) {
+            if (cf->kinks && cf->dt_kind != ORC_DT_FIXED) min_dt_soft(m, dt.data());
+            else min_dt(m, dt.data());
+        }
         for (int64_t i = 0; i < n; ++i) qk[i] = q[i];
         for (int s = 0; s < cf->n_alpha; ++s) {
             if (cf->grad) gradients(m, G, qk.data(), gx.data(), gy.data());

@@ -20,6 +20,10 @@
  * min(a,b) = (b < a) ? b : a ; max(a,b) = (a < b) ? b : a. */
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
+/* smoothed kinks (extension, no reference counterpart): |x|_w, min_w, max_w */
+static inline double kabs(double x, double w) { return x * x / sqrt(x * x + w * w); }
+static inline double kmin(double a, double b, double w) { return 0.5 * (a + b - kabs(a - b, w)); }
+static inline double kmax(double a, double b, double w) { return 0.5 * (a + b + kabs(a - b, w)); }
 
 /* ---------------------------------------------------------------------- */
 /* edge lookup: (min node, max node) -> edge id  (mesh.cpp:103-126 edgesRef) */
@@ -343,9 +347,10 @@ static void roe_flux(const orc_config* cf, double* f, const double* qi, const do
     const double Vn = u * n[0] + v * n[1];
     const double VR = uR * n[0] + vR * n[1];
     const double VL = uL * n[0] + vL * n[1];
-    const double lcm = fabs(smin(Vn - c, VL - c));
-    const double lc = fabs(Vn);
-    const double lcp = fabs(smax(Vn + c, VR + c));
+    const double w = cf->kwv;
+    const double lcm = cf->kinks ? kabs(kmin(Vn - c, VL - c, w), w) : fabs(smin(Vn - c, VL - c));
+    const double lc = cf->kinks ? kabs(Vn, w) : fabs(Vn);
+    const double lcp = cf->kinks ? kabs(kmax(Vn + c, VR + c, w), w) : fabs(smax(Vn + c, VR + c));
     const double kF1 = lcm * ((pR - pL) - rho * c * (VR - VL)) / (2. * c * c);
     const double kF234_0 = lc * ((qj[0] - qi[0]) - (pR - pL) / (c * c));
     const double kF234_1 = lc * rho;
@@ -479,9 +484,16 @@ void orc_calc_dt(const orc_mesh* m, const orc_config* cf, const double* q, doubl
             const double* qj = &q[V * j];
             double ci = sqrt(pressure(g, qi) * g / qi[0]);
             double cj = sqrt(pressure(g, qj) * g / qj[0]);
-            double ei = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
-            double ej = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
-            double center = smax(ei, ej);
+            double ei, ej, center;
+            if (cf->kinks) {
+                ei = ci + kabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1, cf->kwv);
+                ej = cj + kabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1, cf->kwv);
+                center = kmax(ei, ej, cf->kwv);
+            } else {
+                ei = ci + fabs(qi[1] / qi[0] * n0 + qi[2] / qi[0] * n1);
+                ej = cj + fabs(qj[1] / qj[0] * n0 + qj[2] / qj[0] * n1);
+                center = smax(ei, ej);
+            }
             dt[i] += center * le;
             dt[j] += center * le;
         }
@@ -545,6 +557,28 @@ void orc_min_dt(const orc_mesh* m, double* dt) {
     for (int32_t i = 0; i < m->n_cells; ++i) dt[i] = mn;
 }
 
+/* Extension (smoothed kinks): the p-norm (sum_i dt_i^-128)^(-1/128) as
+ * m / S^(1/128), m = min_dt's value, S = sum_i (m/dt_i)^128; each term in
+ * [0, 1] added in 64.64 fixed point (order-free), S^(1/128) by seven square
+ * roots; a NaN term makes the step NaN. */
+void orc_min_dt_soft(const orc_mesh* m, double* dt) {
+    const int32_t N = m->n_cells;
+    double mn = dt[0];
+    for (int32_t i = 0; i < N; ++i) mn = smin(mn, dt[i]);
+    unsigned __int128 S = 0;
+    for (int32_t i = 0; i < N; ++i) {
+        double y = mn / dt[i];
+        if (!(y >= 0.0 && y <= 1.0)) { S += (unsigned __int128)1 << 100; continue; }
+        for (int k = 0; k < 7; ++k) y = y * y;
+        if (y >= 1.0) S += (unsigned __int128)1 << 64;
+        else S += (unsigned __int128)(uint64_t)(y * 18446744073709551616.0);
+    }
+    double root = (double)(uint64_t)(S >> 64) + (double)(uint64_t)S * 0x1p-64;
+    for (int k = 0; k < 7; ++k) root = sqrt(root);
+    const double v = (S >> 100) != 0 ? NAN : mn / root;
+    for (int32_t i = 0; i < N; ++i) dt[i] = v;
+}
+
 /* calc_gradients (explicit.cpp:112-171) */
 void orc_calc_gradients(const orc_mesh* m, const double* q, double* gx, double* gy) {
     const int64_t n = (int64_t)V * m->n_cells;
@@ -587,6 +621,14 @@ void orc_calc_limiters(const orc_mesh* m, const orc_config* cf, const double* q,
     for (int32_t e = 0; e < m->n_edges; ++e) {
         int32_t i = m->edge_cells[2 * e], j = m->edge_cells[2 * e + 1];
         for (int k = 0; k < V; ++k) {
+            if (cf->kinks) {
+                const double w = cf->kw[k];
+                qmin[V * i + k] = kmin(qmin[V * i + k], q[V * j + k], w);
+                qmin[V * j + k] = kmin(qmin[V * j + k], q[V * i + k], w);
+                qmax[V * i + k] = kmax(qmax[V * i + k], q[V * j + k], w);
+                qmax[V * j + k] = kmax(qmax[V * j + k], q[V * i + k], w);
+                continue;
+            }
             qmin[V * i + k] = smin(qmin[V * i + k], q[V * j + k]);
             qmin[V * j + k] = smin(qmin[V * j + k], q[V * i + k]);
             qmax[V * i + k] = smax(qmax[V * i + k], q[V * j + k]);
@@ -619,13 +661,21 @@ void orc_calc_limiters(const orc_mesh* m, const orc_config* cf, const double* q,
                     sig = 0.;
                 }
                 double l = 1.0;
-                if (sig < 1.0) {
+                if (sig < 1.0 && cf->kinks) {
+                    /* smoothed kinks: the extremum by a blend on the sign of
+                     * dqg; Venkatakrishnan's phi(d/dqg) without the quotient,
+                     * regularised with w^2 as in his original form */
+                    const double w = cf->kw[k];
+                    const double t = dqg / sqrt(dqg * dqg + w * w);
+                    const double d = 0.5 * (dmax + dmin) + 0.5 * (dmax - dmin) * t;
+                    l = (d * d + 2.0 * d * dqg + w * w) / (d * d + d * dqg + 2.0 * dqg * dqg + w * w);
+                } else if (sig < 1.0) {
                     if (dqg > tol) l = orc_phi(cf, dmax / dqg);
                     else if (dqg < -tol) l = orc_phi(cf, dmin / dqg);
                     else l = 1.0;
                 }
                 l = sig + (1.0 - sig) * l;
-                lim[V * id + k] = smin(lim[V * id + k], l);
+                lim[V * id + k] = cf->kinks ? kmin(lim[V * id + k], l, cf->kwl) : smin(lim[V * id + k], l);
             }
         }
     }
@@ -779,7 +829,10 @@ int orc_run(const orc_mesh* m, const orc_config* cf, double* q,
         else { for (int k = 0; k < V; ++k) Rmax = smax(R[k], Rmax); }
         if ((step >= max_step) | (time >= max_time) | (Rmax < tol)) break;
         orc_calc_dt(m, cf, q, dt);
-        if (cf->gdt) orc_min_dt(m, dt);
+        if (cf->gdt) {
+            if (cf->kinks && cf->dt_kind != ORC_DT_FIXED) orc_min_dt_soft(m, dt);
+            else orc_min_dt(m, dt);
+        }
         memcpy(qk, q, sizeof(double) * n);
         for (int s = 0; s < cf->n_alpha; ++s) {
             if (cf->grad) orc_calc_gradients(m, qk, gx, gy);

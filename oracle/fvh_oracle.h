@@ -51,6 +51,14 @@ typedef struct {
     int smooth;          /* solver::smooth_residuals (explicit.h:40) */
     double mu, pr, cp, r;   /* consts:: of the viscous examples (flatPlate/main.cpp:28-35) */
     int diff;            /* solver::diffusive_gradients (explicit.h:38) */
+    /* extension: smoothed kinks (the library's FVH_FLAG_SMOOTH_KINKS), no
+     * reference counterpart: |x|_w = x^2/sqrt(x^2+w^2) and min/max_w in the
+     * entropy cure, the summed-CFL wave speeds and the limiter; the limiter's
+     * extremum chosen by a blend on the sign of dqg; the global step as the
+     * p-norm (sum dt_i^-128)^(-1/128).  Widths per conserved variable, for
+     * wave speeds and for the limiter's running minimum. */
+    int kinks;
+    double kw[4], kwv, kwl;
 } orc_config;
 
 typedef struct {
@@ -80,6 +88,8 @@ void orc_mesh_free(orc_mesh* m);
 /* Per-phase functions (reference semantics, AoS q of 4*n_cells). */
 void orc_calc_dt(const orc_mesh* m, const orc_config* c, const double* q, double* dt);
 void orc_min_dt(const orc_mesh* m, double* dt);
+/* extension (smoothed kinks): the p-norm form of min_dt */
+void orc_min_dt_soft(const orc_mesh* m, double* dt);
 void orc_calc_gradients(const orc_mesh* m, const double* q, double* gx, double* gy);
 void orc_calc_limiters(const orc_mesh* m, const orc_config* c, const double* q,
                        const double* gx, const double* gy,

@@ -13,6 +13,8 @@ Solver configuration is a plain dict shared with the product's Python layer:
   gamma, alpha (list of RK stage coefficients)
   bcs    {physical name: (kind, rho, u, v, p)}, kind in
          'copy' | 'mirror' | 'inlet_outlet' | 'farfield'
+  positivity   first-order face state where the reconstruction is not physical
+  smooth_kinks (+ kink_width, kink_scale)   the smoothed-kink extension
 """
 from __future__ import annotations
 
@@ -48,7 +50,30 @@ class OrcConfig(C.Structure):
         ("smooth", C.c_int),
         ("mu", C.c_double), ("pr", C.c_double), ("cp", C.c_double), ("r", C.c_double),
         ("diff", C.c_int),
+        ("kinks", C.c_int), ("kw", C.c_double * 4), ("kwv", C.c_double), ("kwl", C.c_double),
     ]
+
+
+def kink_widths(cfg: Dict):
+    """Widths of the smoothed kinks (cfg 'smooth_kinks'), as the library forms
+    them from kink_width (default 1e-2) and kink_scale (zeros: the first
+    characteristic boundary's freestream rho, rho (|V| + c) twice, E, |V| + c;
+    1 without one): ([w_rho, w_rhou, w_rhov, w_rhoE], w_speed, w_limiter)."""
+    import math
+    g = float(cfg.get("gamma", 1.4))
+    sc = [1.0] * 5
+    for kind, *par in cfg.get("bcs", {}).values():
+        if kind not in ("inlet_outlet", "farfield", "farfield_v"):
+            continue
+        rho, u, v, p = (list(par) + [0.0] * 4)[:4]
+        speed = math.sqrt(u * u + v * v)
+        a = math.sqrt(g * p / rho)
+        sc = [rho, rho * (speed + a), rho * (speed + a), p / (g - 1.0) + 0.5 * rho * speed * speed, speed + a]
+        break
+    w = float(cfg.get("kink_width", 0.0)) or 1e-2
+    given = list(cfg.get("kink_scale", [0.0] * 5))
+    use = [given[k] if given[k] > 0.0 else sc[k] for k in range(5)]
+    return [w * use[k] for k in range(4)], w * use[4], w
 
 
 class OrcMesh(C.Structure):
@@ -85,6 +110,7 @@ def lib():
         mp = C.POINTER(OrcMesh)
         L.orc_calc_dt.argtypes = [mp, cfgp, dp, dp]
         L.orc_min_dt.argtypes = [mp, dp]
+        L.orc_min_dt_soft.argtypes = [mp, dp]
         L.orc_calc_gradients.argtypes = [mp, dp, dp, dp]
         L.orc_calc_limiters.argtypes = [mp, cfgp, dp, dp, dp, dp, dp, dp]
         L.orc_calc_time_derivatives.argtypes = [mp, cfgp, dp, dp, dp, dp, dp]
@@ -127,6 +153,11 @@ def make_config(cfg: Dict, names: List[str]) -> OrcConfig:
     c.r = float(cfg.get("r_gas", VISCOUS_DEFAULTS["r_gas"]))
     c.K = float(cfg.get("K", 7.5))
     c.gamma = float(cfg.get("gamma", 1.4))
+    c.kinks = int(bool(cfg.get("smooth_kinks", False)))
+    kw, kwv, kwl = kink_widths(cfg)
+    for k in range(4):
+        c.kw[k] = kw[k]
+    c.kwv, c.kwl = kwv, kwl
     alpha = list(cfg.get("alpha", RK5))
     c.n_alpha = len(alpha)
     for i, a in enumerate(alpha):

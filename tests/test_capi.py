@@ -29,7 +29,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_abi_version_and_defaults():
     L = _lib.load()
-    assert L.fvh_abi_version() == 2
+    assert L.fvh_abi_version() == 3
     c = _lib.Config()
     L.fvh_config_default(C.byref(c))
     assert c.n_alpha == 5 and list(c.alpha)[:5] == [0.0695, 0.1602, 0.2898, 0.506, 1.0]
