@@ -502,7 +502,8 @@ def adj_direction(n_cells, n_real, k):
 
 def cstep_worker(a):
     """CPU only (no HIP call): the complex-step derivative of J along the
-    directions in --cstep-worker on the adjoint workload, from the state and
+    directions in --cstep-worker on the adjoint workload (an entry "s<k>":
+    direction k of the scheme with smoothed kinks), from the state and
     weights the parent wrote to --cstep-dir; prints one JSON line per
     direction."""
     import numpy as np
@@ -512,20 +513,26 @@ def cstep_worker(a):
     om = O.Mesh(m)
     q0 = np.load(os.path.join(a.cstep_dir, "q0.npy"))
     w = np.load(os.path.join(a.cstep_dir, "w.npy"))
-    for k in [int(x) for x in a.cstep_worker.split(",")]:
+    for item in a.cstep_worker.split(","):
+        smooth = item.startswith("s")
+        k = int(item.lstrip("s"))
         t0 = time.perf_counter()
-        J, dJ = O.cstep_derivative(om, cfg, q0, adj_direction(om.n_cells, om.n_real, k), a.adj_steps, wx=w)
-        print(json.dumps({"dir": k, "J": J, "dJ": dJ, "seconds": time.perf_counter() - t0}), flush=True)
+        J, dJ = O.cstep_derivative(om, dict(cfg, smooth_kinks=smooth), q0, adj_direction(om.n_cells, om.n_real, k),
+                                   a.adj_steps, wx=w)
+        print(json.dumps({"dir": k, "smooth": smooth, "J": J, "dJ": dJ, "seconds": time.perf_counter() - t0}),
+              flush=True)
 
 
 def start_cstep_workers(a, q0, w, n_dirs, procs):
     """Spawn `procs` CPU worker processes (lowest priority) for n_dirs
-    directions; returns the handles."""
+    directions of the scheme and n_dirs of its smoothed-kink form; returns
+    the handles."""
     import numpy as np
     d = tempfile.mkdtemp(prefix="fvh_cstep_")
     np.save(os.path.join(d, "q0.npy"), q0)
     np.save(os.path.join(d, "w.npy"), w)
-    groups = [list(range(i, n_dirs, procs)) for i in range(min(procs, n_dirs))]
+    items = [str(k) for k in range(n_dirs)] + ["s%d" % k for k in range(n_dirs)]
+    groups = [items[i::procs] for i in range(min(procs, len(items)))]
     out = []
     for g in groups:
         cmd = ["nice", "-n", "19", sys.executable, os.path.abspath(__file__), "--cstep-worker",
@@ -542,7 +549,7 @@ def collect_cstep_workers(procs, timeout=900):
             print("complex-step worker failed: %s" % e[-1000:], file=sys.stderr)
             continue
         res += [json.loads(l) for l in o.strip().splitlines() if l.startswith("{")]
-    return sorted(res, key=lambda r: r["dir"])
+    return sorted(res, key=lambda r: (r.get("smooth", False), r["dir"]))
 
 
 def adjoint(a):
@@ -586,6 +593,54 @@ def adjoint(a):
         for eps, out in ((1e-6, rel6), (1e-8, rel8)):
             fd = (s.objective(q0 + eps * v, N, wx=w) - s.objective(q0 - eps * v, N, wx=w)) / (2 * eps)
             out.append(abs(ad - fd) / max(abs(fd), 1e-300))
+    # the same run with smoothed kinks (FVH_FLAG_SMOOTH_KINKS): its own
+    # adjoint/forward ratio, and fourth- and second-order central differences
+    # at eps 1e-6 on the same directions
+    cfg_s = dict(cfg, smooth_kinks=True)
+    ss = Solver(pm, cfg_s)
+    ss.use_graphs(True)
+    ss.objective(q0, 2, wx=w)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Js = ss.objective(q0, N, wx=w)
+    t_fwd_s = time.perf_counter() - t0
+    ss.adjoint(q0, 2, wx=w)
+    t0 = time.perf_counter()
+    _, grad_s, st_s = ss.adjoint(q0, N, wx=w, mem_cap_bytes=8e9)
+    t_adj_s = time.perf_counter() - t0
+    eps_list = (1e-6, 5e-7, 2.5e-7)
+    fd4 = {e: [] for e in eps_list}
+    fd2 = []
+    for k in range(a.adj_dirs):
+        v = adj_direction(pm.n_cells, pm.n_real, 1000 + k)
+        ad = float(grad_s @ v)
+        for eps in eps_list:
+            Jv = {j: ss.objective(q0 + j * eps * v, N, wx=w) for j in (-2, -1, 1, 2)}
+            d4 = (8 * (Jv[1] - Jv[-1]) - (Jv[2] - Jv[-2])) / (12 * eps)
+            fd4[eps].append(abs(ad - d4) / max(abs(d4), 1e-300))
+            if eps == 1e-6:
+                d2 = (Jv[1] - Jv[-1]) / (2 * eps)
+                fd2.append(abs(ad - d2) / max(abs(d2), 1e-300))
+
+    def fd_stats(errs):
+        return None if not errs else {
+            "directions": len(errs), "median_rel_err": float(np.median(errs)), "max_rel_err": float(np.max(errs)),
+            "frac_below_1e-6": float(np.mean(np.array(errs) <= 1e-6))}
+    kw, kwv, kwl = _oracle().kink_widths(cfg_s)
+    smoothed = {
+        "what": "the scheme with smoothed kinks (|.|_w, min_w, max_w in the entropy cure, wave speeds and "
+                "limiter; sign blend and Venkatakrishnan's regularised form in the limiter; p-norm global step)",
+        "widths": {"rho_rhou_rhov_rhoE": kw, "wave_speed": kwv, "limiter_min": kwl},
+        "J": Js, "J_minus_reference_scheme": Js - J,
+        "forward_s": round(t_fwd_s, 4), "adjoint_s": round(t_adj_s, 4),
+        "adjoint_over_forward": round(t_adj_s / t_fwd_s, 3), "recorded_steps": st_s["recorded_steps"],
+        "fd4_stencil": "(8 (J(+e) - J(-e)) - (J(+2e) - J(-2e))) / (12 e)",
+        "fd4_eps_1e-6": fd_stats(fd4[1e-6]),
+        "fd4_eps_5e-7": fd_stats(fd4[5e-7]),
+        "fd4_eps_2.5e-7": fd_stats(fd4[2.5e-7]),
+        "fd2_eps_1e-6": fd_stats(fd2),
+        "complex_step": None,
+    }
     cs = None
     C = pm.n_real
     stages = len(cfg["alpha"])
@@ -607,7 +662,8 @@ def adjoint(a):
             "max_rel_err": float(np.max(rel6)), "frac_below_1e-4": float(np.mean(np.array(rel6) < 1e-4))},
         "fd_eps_1e-8": None if not rel8 else {
             "directions": len(rel8), "median_rel_err": float(np.median(rel8)), "max_rel_err": float(np.max(rel8))},
-        "complex_step": cs, "cpu_baseline": None, "_mesh": m, "_cfg": cfg, "_grad": grad,
+        "complex_step": cs, "smoothed_kinks": smoothed, "cpu_baseline": None, "_mesh": m, "_cfg": cfg,
+        "_grad": grad, "_grad_s": grad_s, "_J_s": Js,
         "_n_cells": pm.n_cells, "_n_real": pm.n_real,
     }
 
@@ -761,6 +817,7 @@ def headline(a):
     cs = collect_cstep_workers(workers) if workers else []
     if cs:
         adj["complex_step"] = adjoint_complex_step_check(adj, cs)
+        adj["smoothed_kinks"]["complex_step"] = adjoint_complex_step_check(adj, cs, smooth=True)
     if not a.no_cpu:
         step_res["cpu_baseline"] = cpu_baseline(m, cfg, a.cpu_steps, "step", info["ranks"])
         items = workloads.naca_sweep(a.instances)
@@ -782,20 +839,30 @@ def headline(a):
                        "sweep_rel_err_vs_fp64": sw["rel_err_vs_fp64"]["max"],
                        "adjoint_over_forward": adj["adjoint_over_forward"],
                        "adjoint_complex_step_max_rel_err": (adj["complex_step"] or {}).get("max_rel_err"),
+                       "adjoint_smoothed_fd4_eps_1e-6_max_rel_err":
+                           (adj["smoothed_kinks"]["fd4_eps_1e-6"] or {}).get("max_rel_err"),
+                       "adjoint_smoothed_complex_step_max_rel_err":
+                           (adj["smoothed_kinks"]["complex_step"] or {}).get("max_rel_err"),
                        "cpu": info["model"], "cpu_cores_used": info["ranks"]}
     out["wall_s"] = {"gpu_part": round(t_gpu, 1), "total": round(time.perf_counter() - t_start, 1)}
     print(json.dumps(out))
 
 
-def adjoint_complex_step_check(adj, cs):
+def adjoint_complex_step_check(adj, cs, smooth=False):
     """Compare the complex-step derivatives of the CPU workers with the
-    adjoint gradient along the same directions."""
+    adjoint gradient along the same directions (the scheme, or its
+    smoothed-kink form)."""
     import numpy as np
-    grad, n_cells, n_real = adj["_grad"], adj["_n_cells"], adj["_n_real"]
+    cs = [r for r in cs if r.get("smooth", False) == smooth]
+    if not cs:
+        return None
+    grad = adj["_grad_s"] if smooth else adj["_grad"]
+    J = adj["_J_s"] if smooth else adj["J"]
+    n_cells, n_real = adj["_n_cells"], adj["_n_real"]
     errs = [abs(float(grad @ adj_direction(n_cells, n_real, r["dir"])) - r["dJ"]) / abs(r["dJ"]) for r in cs]
     return {"cells": n_real, "steps": adj["steps"], "directions": len(errs), "max_rel_err": float(max(errs)),
             "median_rel_err": float(np.median(errs)),
-            "J_cpu_minus_gpu": float(max(abs(r["J"] - adj["J"]) for r in cs)),
+            "J_cpu_minus_gpu": float(max(abs(r["J"] - J) for r in cs)),
             "cpu_s_per_direction": round(float(np.median([r["seconds"] for r in cs])), 1),
             "where": "CPU worker processes, the templated restatement on complex numbers (h = 1e-30), run while "
                      "the GPU measured the other workloads"}
@@ -871,6 +938,7 @@ def main():
         cs = collect_cstep_workers(workers) if workers else []
         if cs:
             r["complex_step"] = adjoint_complex_step_check(r, cs)
+            r["smoothed_kinks"]["complex_step"] = adjoint_complex_step_check(r, cs, smooth=True)
         print(json.dumps(line({k: v for k, v in r.items() if not k.startswith("_")}, a)))
         return
 

@@ -376,8 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_gradlim(MeshView<T> m, PhysParam
             if (spread <= K3a) sig = T(1);
             else if (spread <= T(2) * K3a) {
                 const T y = (spread / K3a - T(1));
-                sig = T(2) * y * y * y - T(3) * y * y + T(1);
-                dsig = (T(6) * y * y - T(6) * y) / K3a;   // d sig / d spread
+                sig = sigma_smooth(y);
+                dsig = sigma_smooth_d(y) / K3a;   // d sig / d spread
             } else sig = T(0);
             // forward: per half-edge limiter values and the running minimum
             T lv[kMaxHe], Lv[kMaxHe], run[kMaxHe + 1];

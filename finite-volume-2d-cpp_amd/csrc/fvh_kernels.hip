@@ -203,7 +203,8 @@ __global__ __launch_bounds__(kBlock) void k_dt_soft_inst(InstView iv, const uint
 // sigma + (1 - sigma) phi(r), starting from the reset value in lim.
 // PHI_VENKAT_S (smoothed kinks): r's numerator blends dmax and dmin by
 // t = dqg / sqrt(dqg^2 + w^2) instead of choosing by the sign of dqg,
-// phi = venkat_dq regularised by w^2, and the running minimum is min_w.
+// phi = venkat_dq regularised by w^2, sigma is sigma_smooth, and the running
+// minimum is min_w.
 template <class T, int PHI>
 __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, int c, int h0, int h1, const Q4<T>& qc,
                                   const G2<T>& cc, const T (&qmn)[4], const T (&qmx)[4], const T (&gx)[4],
@@ -224,7 +225,7 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
             sigs[k] = T(1);
         } else if (spread <= T(2) * K3a) {
             const T y = (spread / K3a - T(1));
-            sigs[k] = T(2) * y * y * y - T(3) * y * y + T(1);
+            sigs[k] = PHI == PHI_VENKAT_S ? sigma_smooth(y) : sigma_cubic(y);
         } else {
             sigs[k] = T(0);
         }

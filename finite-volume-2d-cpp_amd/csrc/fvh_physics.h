@@ -45,6 +45,21 @@ template <class T> FVH_HD T kabs_d(T x, T w) {
     const T s2 = x * x + w * w;
     return x * (x * x + T(2) * w * w) / (s2 * sqrt(s2));
 }
+// The limiter's sigma blend (explicit.cpp:226-238) over y = spread / (K sqrt A)^3 - 1
+// in (0, 1]: the reference's cubic 2y^3 - 3y^2 + 1, once differentiable at
+// both ends; with smoothed kinks 1 - S(y), S the quartic-order smoothstep
+// y^5 (126 - 420y + 540y^2 - 315y^3 + 70y^4), four times differentiable at
+// both ends (fine meshes make the blend narrow in state space: a jump in the
+// second derivative there spoils difference quotients); and d sigma / dy.
+template <class T> FVH_HD T sigma_cubic(T y) { return T(2) * y * y * y - T(3) * y * y + T(1); }
+template <class T> FVH_HD T sigma_smooth(T y) {
+    const T y2 = y * y;
+    return T(1) - y2 * y2 * y * (T(126) + y * (T(-420) + y * (T(540) + y * (T(-315) + T(70) * y))));
+}
+template <class T> FVH_HD T sigma_smooth_d(T y) {
+    const T y2 = y * y, z = T(1) - y, z2 = z * z;
+    return T(-630) * (y2 * y2) * (z2 * z2);
+}
 // Venkatakrishnan's phi(d / q) = (r^2 + 2r) / (r^2 + r + 2) with r = d / q,
 // written without the quotient and regularised by e2 > 0 as in his original
 // form: (d^2 + 2 d q + e2) / (d^2 + d q + 2 q^2 + e2), smooth everywhere

@@ -100,7 +100,9 @@ enum { FVH_FLUX_MODE_AUTO = 0, FVH_FLUX_MODE_CELL = 1, FVH_FLUX_MODE_TILE = 2, F
  * (exact at x = 0 and a = b), in the entropy cure's |min(V-c, VL-c)|, |V|
  * and |max(V+c, VR+c)|, the summed-CFL wave speeds c + |u.n| and their max,
  * and the limiter's neighbour extrema and running minimum; the limiter's
- * choice of qmax or qmin by the sign of the gradient projection dqg becomes
+ * sigma blend becomes 1 - y^5 (126 - 420y + 540y^2 - 315y^3 + 70y^4)
+ * (four times differentiable where the reference's cubic is once); the
+ * limiter's choice of qmax or qmin by the sign of the gradient projection dqg becomes
  * a blend by dqg / sqrt(dqg^2 + w^2) with phi(d/dqg) written without the
  * quotient and regularised by w^2 (Venkatakrishnan's original form); the
  * global time step's minimum (CFL kinds) becomes the p-norm

@@ -458,6 +458,11 @@ void limiters(const orc_mesh* m, const Geo<T>& G, const orc_config* cf, const T*
                 T sig;
                 if (d2 <= K3a) {
                     sig = T(1.);
+                } else if (d2 <= T(2) * K3a && cf->kinks) {
+                    T yy = (d2 / K3a - T(1.0));
+                    T y2 = yy * yy;
+                    sig = T(1.0) - y2 * y2 * yy *
+                          (T(126.0) + yy * (T(-420.0) + yy * (T(540.0) + yy * (T(-315.0) + T(70.0) * yy))));
                 } else if (d2 <= T(2) * K3a) {
                     T yy = (d2 / K3a - T(1.0));
                     sig = T(2.0) * yy * yy * yy - T(3.0) * yy * yy + T(1.0);

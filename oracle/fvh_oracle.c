@@ -612,6 +612,17 @@ void orc_calc_gradients(const orc_mesh* m, const double* q, double* gx, double* 
         for (int k = 0; k < V; ++k) { gx[V * i + k] = 0.; gy[V * i + k] = 0.; }
 }
 
+/* the sigma blend of calc_limiters over y = d2 / K3a - 1 in (0, 1]
+ * (explicit.cpp:230-233); with smoothed kinks (extension) 1 - the
+ * quartic-order smoothstep, four times differentiable at both ends */
+double orc_sigma(const orc_config* cf, double yy) {
+    if (cf->kinks) {
+        double y2 = yy * yy;
+        return 1.0 - y2 * y2 * yy * (126.0 + yy * (-420.0 + yy * (540.0 + yy * (-315.0 + 70.0 * yy))));
+    }
+    return 2.0 * yy * yy * yy - 3.0 * yy * yy + 1.0;
+}
+
 /* calc_limiters (explicit.cpp:174-258) */
 void orc_calc_limiters(const orc_mesh* m, const orc_config* cf, const double* q,
                        const double* gx, const double* gy,
@@ -655,8 +666,7 @@ void orc_calc_limiters(const orc_mesh* m, const orc_config* cf, const double* q,
                 if (d2 <= K3a) {
                     sig = 1.;
                 } else if (d2 <= 2 * K3a) {
-                    double yy = (d2 / K3a - 1.0);
-                    sig = 2.0 * yy * yy * yy - 3.0 * yy * yy + 1.0;
+                    sig = orc_sigma(cf, d2 / K3a - 1.0);
                 } else {
                     sig = 0.;
                 }

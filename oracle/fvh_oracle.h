@@ -88,6 +88,8 @@ void orc_mesh_free(orc_mesh* m);
 /* Per-phase functions (reference semantics, AoS q of 4*n_cells). */
 void orc_calc_dt(const orc_mesh* m, const orc_config* c, const double* q, double* dt);
 void orc_min_dt(const orc_mesh* m, double* dt);
+/* the limiter's sigma blend at y = spread / K3a - 1 in (0, 1] */
+double orc_sigma(const orc_config* cf, double y);
 /* extension (smoothed kinks): the p-norm form of min_dt */
 void orc_min_dt_soft(const orc_mesh* m, double* dt);
 void orc_calc_gradients(const orc_mesh* m, const double* q, double* gx, double* gy);

@@ -111,6 +111,8 @@ def lib():
         L.orc_calc_dt.argtypes = [mp, cfgp, dp, dp]
         L.orc_min_dt.argtypes = [mp, dp]
         L.orc_min_dt_soft.argtypes = [mp, dp]
+        L.orc_sigma.argtypes = [cfgp, C.c_double]
+        L.orc_sigma.restype = C.c_double
         L.orc_calc_gradients.argtypes = [mp, dp, dp, dp]
         L.orc_calc_limiters.argtypes = [mp, cfgp, dp, dp, dp, dp, dp, dp]
         L.orc_calc_time_derivatives.argtypes = [mp, cfgp, dp, dp, dp, dp, dp]

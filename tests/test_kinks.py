@@ -5,7 +5,8 @@ second-order path by an infinitely differentiable function of width w, so
 that finite differences of an objective converge.  These tests pin its
 definition without retyping it:
   * each smoothed piece tends to the reference's piece as w -> 0 and is exact
-    where the reference's kink sits (|0|_w = 0, min_w(a, a) = a);
+    where the reference's kink sits (|0|_w = 0, min_w(a, a) = a); the sigma
+    blend is a smoothstep of fourth order (value, ends, derivative);
   * the p-norm global step has closed forms (M equal steps, one dominant);
   * with a fixed step, a whole run tends to the reference scheme's run as
     w -> 0, so no term is dropped;
@@ -67,6 +68,28 @@ def test_smoothed_flux_tends_to_the_roe_flux(flux):
         np.testing.assert_array_equal(O.flux(dict(base, smooth_kinks=True), qL, qL, n), O.flux(base, qL, qL, n))
 
 
+def test_sigma_blend_is_a_fourth_order_smoothstep():
+    """The oracle's sigma over y in (0, 1]: the reference's cubic has a
+    nonzero second derivative at both ends; with smoothed kinks sigma falls
+    from 1 to 0 through 1/2 at y = 1/2 (antisymmetric about it) with its
+    first four derivatives zero at both ends -- ten conditions that fix a
+    degree-9 polynomial.  Derivatives from a least-squares polynomial fit of
+    the sampled values."""
+    y = np.linspace(0.0, 1.0, 401)
+    for kinks in (False, True):
+        c = O.make_config(dict(SMOOTH, smooth_kinks=kinks), [])
+        sig = np.array([O.lib().orc_sigma(C.byref(c), float(t)) for t in y])
+        p = np.polynomial.Polynomial.fit(y, sig, 9, domain=[0, 1], window=[0, 1])
+        assert np.max(np.abs(p(y) - sig)) < 1e-12
+        assert abs(sig[0] - 1) < 1e-15 and abs(sig[-1]) < 1e-15 and abs(sig[200] - 0.5) < 1e-15
+        np.testing.assert_allclose(sig + sig[::-1], 1.0, atol=1e-14)
+        ends = [abs(p.deriv(k)(e)) for k in range(1, 5) for e in (0.0, 1.0)]
+        if kinks:
+            assert max(ends) < 1e-6 * 630, ends
+        else:
+            assert abs(p.deriv(2)(0.0) + 6) < 1e-8 and abs(p.deriv(2)(1.0) - 6) < 1e-8
+
+
 def _soft(values):
     L = O.lib()
     m = O.OrcMesh()
@@ -95,13 +118,15 @@ def test_p_norm_time_step_closed_forms():
 
 @pytest.mark.parametrize("mesh_name", ["mixed", "step"])
 def test_smoothed_run_tends_to_the_reference_scheme(mesh_name):
-    """With a fixed step (no p-norm), every smoothed piece tends to the
-    reference's: after 10 steps from a rough random state, the run at width
-    1e-10 is within 1e-8 of the reference scheme's and far closer than the
-    run at width 1e-6 (a dropped or altered term would not vanish)."""
+    """With a fixed step (no p-norm), every smoothed piece of width w tends
+    to the reference's: after 10 steps from a rough random state, the run at
+    width 1e-10 is within 1e-8 of the reference scheme's and far closer than
+    the run at width 1e-6 (a dropped or altered term would not vanish).  K is
+    tiny, so that sigma is 0 in every cell (the smoothed mode's sigma blend
+    is a different polynomial on purpose and does not tend to the cubic)."""
     m = MESHES[mesh_name]()
     om = O.Mesh(m)
-    cfg = dict(CONFIGS["second_venkat_fixed"], dt_fixed=2e-4)
+    cfg = dict(CONFIGS["second_venkat_fixed"], dt_fixed=2e-4, K=1e-3)
     q0 = random_state(om.n_cells, seed=77, mach=0.5)
     ref, *_ = om.run(cfg, q0, 10)
     diffs = []
