@@ -392,6 +392,29 @@ def sweep(a, with_cpu=True):
     ev_steps = max(2, a.steps // 10)
     kt = event_times(b, ev_steps)
     qb = b.split(b.get_state())
+    # the persistent form of the fused stage pass on the same batch: resident
+    # workgroups pulling the tiles from per-XCD work queues
+    persist = None
+    if tuned["flux_mode"] == "fused":
+        bp = Batch(meshes, [dict(c, stage_launch=1) for c in cfgs], precision=a.precision, flux_mode="fused",
+                   tile_cells=tuned["tile_cells"])
+        bp.set_state(bp.join(q0))
+        bp.apply_bounds()
+        bp.use_graphs(True)
+        bp.advance(a.warmup)
+        bp.synchronize()
+        torch.cuda.synchronize()
+        t0p = time.perf_counter()
+        bp.advance(a.steps)
+        bp.synchronize()
+        torch.cuda.synchronize()
+        secs_p = time.perf_counter() - t0p
+        persist = {"value": round(C * stages * a.steps / secs_p / 1e6, 2),
+                   "ms_per_step": round(secs_p / a.steps * 1e3, 4),
+                   "launch": "persistent: resident workgroups (occupancy x CUs) pull the tiles from per-XCD work "
+                             "queues with stealing, one launch per stage, hipGraph per step"}
+        del bp
+        note("sweep: persistent batched leg done")
     if not all(np.all(np.isfinite(x)) for x in qb):
         print("non-finite batch state after the timed steps", file=sys.stderr)
         sys.exit(3)
@@ -468,6 +491,7 @@ def sweep(a, with_cpu=True):
         "per_mesh_two_pass_eager": {"value": round(v_old, 2), "seconds": round(s_old, 3),
                                     "launch": "one solver per instance, two-pass tiled stage kernels, eager "
                                               "launches, synchronised per instance"},
+        "persistent": persist,
         "speedup_vs_per_mesh": round(value / v_pm, 3),
         "speedup_vs_per_mesh_two_pass_eager": round(value / v_old, 3),
         "rel_err_vs_fp64": {"max": float(max(errs)), "instances": len(sample), "steps": total,
@@ -689,6 +713,7 @@ def step_workload(a, precision):
         m, cfg, ic = workloads.forward_step(n_per_unit=a.n_per_unit)
     if a.stage_threads:
         cfg["stage_threads"] = a.stage_threads
+    cfg["stage_launch"] = a.stage_launch
     pm = Mesh.from_arrays(m)
     s = Solver(pm, cfg, precision=precision, flux_mode=a.flux_mode, tile_cells=a.tile_cells)
     q0 = workloads.initial_state(pm, ic)
@@ -766,6 +791,7 @@ def step_workload(a, precision):
               "order": 2, "limiter": "venkatakrishnan", "K": cfg["K"], "rk_stages": stages,
               "cfl": cfg["cfl"], "dt": "global", "precision": precision, "stage_form": a.flux_mode}
     wl["parallelism"] = "single_gpu"
+    wl["stage_launch"] = "persistent" if a.stage_launch else "grid"
     wl["tile_cells"] = s.tile_cells
     wl["stage_layout"] = s.stage_info
     out = {"value": round(value, 2), "ms_per_step": round(secs / a.steps * 1e3, 4),
@@ -900,6 +926,8 @@ def main():
     ap.add_argument("--cells", type=int, default=2_000_000, help="airfoil mesh size")
     ap.add_argument("--stage-threads", type=int, default=0, choices=[0, 256, 512],
                     help="fused stage pass workgroup size (0: the library's choice)")
+    ap.add_argument("--stage-launch", type=int, default=0, choices=[0, 1],
+                    help="single workloads: fused stage pass as one workgroup per tile (0) or persistent (1)")
     ap.add_argument("--tile-cells", type=int, default=0,
                     help="step/airfoil/plate: cells per tile (32..256; 0: the library's automatic choice)")
     ap.add_argument("--converge", action="store_true",

@@ -199,14 +199,17 @@ struct StageRecord {
 // edge fluxes, gather, RK update and boundary update in one pass per tile.
 // dt_kind >= 0 also computes the tile cells' local time step (first stage).
 // threads: workgroup size, 256 or 512; 0 picks 256 when four workgroups'
-// halos fit a CU's LDS, else 512.
+// halos fit a CU's LDS, else 512.  queue: null for one workgroup per tile;
+// else kXcd + 1 zeroed counters (left zeroed) of the persistent form, whose
+// resident workgroups pull the tiles from per-XCD queues.  Both forms give
+// the same bits.
 // Returns false when the tile halo does not fit the workgroup's LDS.
 template <class T, class S>
 bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads = 0,
-                  const StageRecord<T>& rec = StageRecord<T>());
+                  const StageRecord<T>& rec = StageRecord<T>(), int32_t* queue = nullptr);
 // LDS of the fused stage pass for a tile of n_ext local cells and n_edges
 // edges: the states of its local cell list in the storage type, then, from
 // the next 32-byte boundary, two face states per tile edge in the

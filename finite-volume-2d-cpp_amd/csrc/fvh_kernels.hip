@@ -746,18 +746,17 @@ __global__ __launch_bounds__(kBlock) void k_smooth(MeshView<T> m, TileMap tm, Ph
 // the flux in the slot; phase 3 gathers per cell in reference edge order and
 // finishes the stage.  No face state goes through HBM; the boundary
 // increments of the tile's own cells do (bincr), for phase 3.
+// The stage of tile lb by one workgroup (LDS: the tile halo's states, then
+// the edge slots).
 template <class T, int PHI, int DT, bool JOINED, int NB, class S>
-__global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
-                                                  int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
-                                                  const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
-                                                  Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
-                                                  T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
-                                                  double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
-                                                  StageRecord<T> rec) {
+__device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView<T>& m, const TileMap& tm,
+                                  const TileView& tv, const HaloView& hv, const PhysParams& pp, int grad, int lim_on,
+                                  const BcParam<T>* __restrict__ bcs, const Q4<S>* __restrict__ qin,
+                                  const Q4<S>* __restrict__ q0, Q4<S>* __restrict__ qout,
+                                  Q4<T>* __restrict__ bincr, T* __restrict__ dt, const T* __restrict__ dt_inst,
+                                  T alpha, double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
+                                  const StageRecord<T>& rec) {
     static_assert(NB >= kBlock, "phase 3 gives each tile cell one lane");
-    extern __shared__ __align__(32) unsigned char lds_raw[];
-    const int lb = xcd_block(tm.n_rt);
-    if (lb < 0) return;
     const int inst = tm.inst[lb];
     if (stopped(ctrl, inst)) return;
     const int c0 = tm.c0[lb], nt = tm.c0[lb + 1] - c0;
@@ -901,6 +900,86 @@ __global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileVie
         if (threadIdx.x == 0)
             for (int k = 0; k < 4; ++k) res_blk[4 * lb + k] = rs[k];
     }
+}
+
+// One launch grid over the tiles: workgroup b takes logical tile
+// xcd_block(n_rt) (each XCD one contiguous range of tiles).
+template <class T, int PHI, int DT, bool JOINED, int NB, class S>
+__global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+                                                  int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
+                                                  const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
+                                                  Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
+                                                  T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
+                                                  double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
+                                                  StageRecord<T> rec) {
+    extern __shared__ __align__(32) unsigned char lds_raw[];
+    const int lb = xcd_block(tm.n_rt);
+    if (lb < 0) return;
+    stage_tile<T, PHI, DT, JOINED, NB, S>(lb, lds_raw, m, tm, tv, hv, pp, grad, lim_on, bcs, qin, q0, qout, bincr, dt,
+                                          dt_inst, alpha, res_blk, ctrl, rec);
+}
+
+// Persistent form: as many workgroups as the GPU holds at once, each pulling
+// tiles from work queues until all are taken.  Queue x (x < kXcd) hands out
+// the contiguous tile range the one-shot grid gives XCD x; workgroup b
+// (resident on XCD b % kXcd) takes from its own XCD's queue, then from the
+// others' in turn.  queue[0..kXcd-1] are ticket counters, queue[kXcd] counts
+// the workgroups done; the last one resets all to zero, so the next launch
+// (a graph replay too) starts from zero without a memset.  Thread 0 issues
+// the atomic for a workgroup's next ticket before the current tile's work and
+// reads it after, so its latency overlaps the tile.  Every workgroup leaves
+// the loop once every queue is exhausted; there is no waiting on other
+// workgroups.
+template <class T, int PHI, int DT, bool JOINED, int NB, class S>
+__global__ __launch_bounds__(NB) void k_stage_q(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+                                                    int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
+                                                    const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
+                                                    Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
+                                                    T* __restrict__ dt, const T* __restrict__ dt_inst, T alpha,
+                                                    double* __restrict__ res_blk, const Ctrl* __restrict__ ctrl,
+                                                    StageRecord<T> rec, int32_t* __restrict__ queue) {
+    extern __shared__ __align__(32) unsigned char lds_raw[];
+    __shared__ int next_tile;
+    const int n = tm.n_rt;
+    const int per = (n + kXcd - 1) / kXcd;
+    const int home = (int)(blockIdx.x % kXcd);
+    const bool t0 = threadIdx.x == 0;
+    // thread 0's ticket state: the queue in use, the queues found empty, the
+    // pending ticket
+    int qx = home, t_raw = 0;
+    unsigned empty = 0;
+    auto resolve = [&]() -> int {
+        for (;;) {
+            const int lo = qx * per, hi = min(lo + per, n);
+            if (lo + t_raw < hi) return lo + t_raw;
+            empty |= 1u << qx;
+            int a = 1;
+            while (a < kXcd && (empty & (1u << ((home + a) % kXcd)))) ++a;
+            if (a == kXcd) return -1;
+            qx = (home + a) % kXcd;
+            t_raw = atomicAdd(&queue[qx], 1);
+        }
+    };
+    int tile = -1;
+    if (t0) {
+        t_raw = atomicAdd(&queue[qx], 1);
+        tile = resolve();
+    }
+    for (;;) {
+        if (t0) {
+            next_tile = tile;
+            if (tile >= 0) t_raw = atomicAdd(&queue[qx], 1);   // the next ticket, read after this tile
+        }
+        __syncthreads();
+        const int lb = next_tile;
+        __syncthreads();
+        if (lb < 0) break;
+        stage_tile<T, PHI, DT, JOINED, NB, S>(lb, lds_raw, m, tm, tv, hv, pp, grad, lim_on, bcs, qin, q0, qout, bincr,
+                                              dt, dt_inst, alpha, res_blk, ctrl, rec);
+        if (t0) tile = resolve();
+    }
+    if (t0 && atomicAdd(&queue[kXcd], 1) == (int)gridDim.x - 1)
+        for (int k = 0; k <= kXcd; ++k) atomicExch(&queue[k], 0);
 }
 
 // Boundary update from the cells' current states (update_bounds,
@@ -1300,17 +1379,41 @@ template <class T, int PHI, int DT, bool J, int NB, class S>
 void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                      const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<S>* qin,
                      const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
-                     double* res_blk, const Ctrl* ctrl, size_t lds, const StageRecord<T>& rec) {
+                     double* res_blk, const Ctrl* ctrl, size_t lds, const StageRecord<T>& rec, int32_t* queue) {
     static bool attr_set = false;
+    static int resident = 0;   // workgroups of the persistent form the GPU holds at once at this LDS size
+    static size_t resident_lds = 0;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void*)k_stage<T, PHI, DT, J, NB, S>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 160 * 1024) != hipSuccess)
             (void)hipGetLastError();
+        if (hipFuncSetAttribute((const void*)k_stage_q<T, PHI, DT, J, NB, S>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+            (void)hipGetLastError();
         attr_set = true;
     }
-    const int nb = xcd_grid_blocks(tm.n_rt);
-    hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB, S>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
-                       (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl, rec);
+    if (!queue) {
+        const int nb = xcd_grid_blocks(tm.n_rt);
+        hipLaunchKernelGGL((k_stage<T, PHI, DT, J, NB, S>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
+                           (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl, rec);
+        return;
+    }
+    if (resident == 0 || resident_lds != lds) {
+        // workgroups per CU by the 160 KB of LDS (with 1 KB for the static
+        // arrays) and by the 32 wave slots; the registers (under 128 VGPRs)
+        // do not bind.  More workgroups than fit at once would only queue
+        // behind the resident ones (none waits on another).
+        int dev = 0, cus = 0;
+        const int per_cu = std::max(1, std::min((int)(160 * 1024 / (lds + 1024)), 32 / (NB / 64)));
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = kXcd;
+        (void)hipGetLastError();
+        resident = std::max(kXcd, per_cu * cus / kXcd * kXcd);
+        resident_lds = lds;
+    }
+    const int nb = std::min(resident, xcd_grid_blocks(tm.n_rt));
+    hipLaunchKernelGGL((k_stage_q<T, PHI, DT, J, NB, S>), dim3(nb), dim3(NB), lds, s, m, tm, tv, hv, pp, (int)grad,
+                       (int)lim, bcs, qin, q0, qout, bincr, dt, dt_inst, alpha, res_blk, ctrl, rec, queue);
 }
 
 // Workgroup size: a tile per workgroup either way.  With 256 lanes the
@@ -1324,14 +1427,14 @@ template <class T, int PHI, int DT, bool J, class S>
 bool launch_stage_k(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const TileView& tv, const HaloView& hv,
                     const PhysParams& pp, bool grad, bool lim, const BcParam<T>* bcs, const Q4<S>* qin,
                     const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt, const T* dt_inst, T alpha,
-                    double* res_blk, const Ctrl* ctrl, int threads, const StageRecord<T>& rec) {
+                    double* res_blk, const Ctrl* ctrl, int threads, const StageRecord<T>& rec, int32_t* queue) {
     const size_t lds = stage_lds_bytes<T, S>(hv);
     if (stage_lanes(lds, threads, hv.mean_g) == kStageBlock / 2)
         launch_stage_nb<T, PHI, DT, J, kStageBlock / 2, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout,
-                                                           bincr, dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
+                                                           bincr, dt, dt_inst, alpha, res_blk, ctrl, lds, rec, queue);
     else
         launch_stage_nb<T, PHI, DT, J, kStageBlock, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr,
-                                                       dt, dt_inst, alpha, res_blk, ctrl, lds, rec);
+                                                       dt, dt_inst, alpha, res_blk, ctrl, lds, rec, queue);
     return true;
 }
 
@@ -1340,11 +1443,11 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
                   const PhysParams& pp, int phi_kind, bool joined, bool grad, bool lim, int dt_kind,
                   const BcParam<T>* bcs, const Q4<S>* qin, const Q4<S>* q0, Q4<S>* qout, Q4<T>* bincr, T* dt,
                   const T* dt_inst, T alpha, double* res_blk, const Ctrl* ctrl, int threads,
-                  const StageRecord<T>& rec) {
+                  const StageRecord<T>& rec, int32_t* queue) {
     if (tm.n_rt == 0) return true;
     if (stage_lds_bytes<T, S>(hv) > 160 * 1024) return false;
 #define FVH_ST(P, D, J) launch_stage_k<T, P, D, J, S>(s, m, tm, tv, hv, pp, grad, lim, bcs, qin, q0, qout, bincr, \
-                                                     dt, dt_inst, alpha, res_blk, ctrl, threads, rec)
+                                                     dt, dt_inst, alpha, res_blk, ctrl, threads, rec, queue)
 #define FVH_STJ(P, D) (joined ? FVH_ST(P, D, true) : FVH_ST(P, D, false))
 #define FVH_STD(P) (dt_kind == 1 ? FVH_STJ(P, 1) : dt_kind == 2 ? FVH_STJ(P, 2) : FVH_STJ(P, 0))
     switch (phi_kind) {
@@ -1468,7 +1571,8 @@ void launch_sums(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const I
     template bool launch_stage<T, S>(hipStream_t, const MeshView<T>&, const TileMap&, const TileView&,       \
                                      const HaloView&, const PhysParams&, int, bool, bool, bool, int,         \
                                      const BcParam<T>*, const Q4<S>*, const Q4<S>*, Q4<S>*, Q4<T>*, T*,     \
-                                     const T*, T, double*, const Ctrl*, int, const StageRecord<T>&);         \
+                                     const T*, T, double*, const Ctrl*, int, const StageRecord<T>&,          \
+                                     int32_t*);                                                              \
     template size_t stage_lds_bytes<T, S>(const HaloView&);
 FVH_INST(double, double)
 FVH_INST(float, float)

@@ -109,6 +109,7 @@ struct fvh_solver {
     int32_t x_max_ext = 0, x_max_g = 0, x_max_edges = 0, x_mean_g = 0, x_lds = 0;
     fvh::DevBuf q[3], dt, dt_global, face, bincr, blk_min, res_blk, ctrl, bcs, sums_blk, sums;
     fvh::DevBuf soft_part;   // smoothed kinks: fixed-point partial sums of the p-norm time step, per tile
+    fvh::DevBuf stage_queue; // persistent stage form: kXcd ticket counters and a done counter (zero between launches)
     fvh::DevBuf unphysical;                  // per instance: a stage left a non-physical cell state
     fvh::DevBuf dbg[4];
     fvh::DevBuf ph[6], ph_part;              // per-phase entry points: inputs and outputs, min_dt partials
@@ -393,6 +394,10 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
         set_error("hipMalloc failed");
         return FVH_E_HIP;
     }
+    if (s->stage_queue.alloc(64) || hipMemset(s->stage_queue.p, 0, 64) != hipSuccess) {
+        set_error("hipMalloc failed");
+        return FVH_E_HIP;
+    }
     if (s->cfg.flags & FVH_FLAG_SMOOTH)
         for (auto& b : s->qt_s)
             if (b.alloc((size_t)(s->n_real + 1) * sizeof(Q4<T>))) {
@@ -419,6 +424,12 @@ int32_t upload_geometry(fvh_solver* s, const HostGeo& h) {
 // buffer s->cur.  run_mode adds the stop test and bookkeeping kernels.
 // dbg (tests): 4 device buffers for gx, gy, limiter and qt of one stage run
 // with alpha_override, leaving the current state buffer unchanged.
+// The work-queue counters of the persistent stage form, or null for one
+// workgroup per tile.
+inline int32_t* stage_queue(fvh_solver* s) {
+    return s->cfg.stage_launch == 1 ? (int32_t*)s->stage_queue.p : nullptr;
+}
+
 // The global time step of each instance from the per-cell steps: min_dt,
 // or with smoothed kinks and a CFL step its p-norm form.
 template <class T>
@@ -485,7 +496,7 @@ int32_t enqueue_step(fvh_solver* s, bool run_mode, const fvh_run_options* o, int
                             (k == 0 && !gdt && !kk) ? c.dt_kind : -1, bcs, qin, q0, out, (Q4<T>*)s->bincr.p, dt,
                             gdt ? dtg : nullptr,
                             T(c.alpha[k]), (last && need_res) ? (double*)s->res_blk.p : nullptr, cc,
-                            c.stage_threads);
+                            c.stage_threads, StageRecord<T>(), stage_queue(s));
             qin = out;
             out = (out == qa) ? qb : qa;
         }
@@ -941,6 +952,10 @@ int32_t check_config(const fvh_config* c) {
         set_error("stage_threads must be 0, 256 or 512");
         return FVH_E_ARG;
     }
+    if (c->stage_launch != 0 && c->stage_launch != 1) {
+        set_error("stage_launch must be 0 (a workgroup per tile) or 1 (persistent work queues)");
+        return FVH_E_ARG;
+    }
     if (c->tile_cells != 0 && (c->tile_cells < 32 || c->tile_cells > kBlock)) {
         set_error("tile_cells must be 0 or 32..256");
         return FVH_E_ARG;
@@ -981,7 +996,8 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
         a.phi_yt != b.phi_yt || a.K != b.K || a.dt_kind != b.dt_kind || a.dt_fixed != b.dt_fixed ||
         a.cfl != b.cfl || a.flags != b.flags || a.n_alpha != b.n_alpha || a.renumber != b.renumber ||
         a.flux_mode != b.flux_mode || a.tile_cells != b.tile_cells || a.mu != b.mu || a.pr != b.pr ||
-        a.cp != b.cp || a.r_gas != b.r_gas || a.stage_threads != b.stage_threads || a.kink_width != b.kink_width)
+        a.cp != b.cp || a.r_gas != b.r_gas || a.stage_threads != b.stage_threads || a.kink_width != b.kink_width ||
+        a.stage_launch != b.stage_launch)
         return false;
     for (int k = 0; k < 5; ++k)
         if (a.kink_scale[k] != b.kink_scale[k]) return false;
@@ -1529,7 +1545,7 @@ struct Adjoint {
                                           bcs, (const Q4<double>*)R.qs[k], (const Q4<double>*)R.qs[0],
                                           (Q4<double>*)R.qs[k + 1], (Q4<double>*)s->bincr.p, dt,
                                           gdt ? (const double*)R.dtg : nullptr, c.alpha[k], nullptr, nullptr,
-                                          c.stage_threads, rec)) {
+                                          c.stage_threads, rec, stage_queue(s))) {
                     set_error("fused stage pass: tile halo does not fit LDS");
                     return FVH_E_UNSUPPORTED;
                 }

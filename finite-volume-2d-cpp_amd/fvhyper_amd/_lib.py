@@ -41,7 +41,7 @@ class Config(C.Structure):
         ("n_bc", C.c_int32), ("bc", BcSpec * FVH_MAX_BC),
         ("renumber", C.c_int32), ("flux_mode", C.c_int32), ("tile_cells", C.c_int32),
         ("mu", C.c_double), ("pr", C.c_double), ("cp", C.c_double), ("r_gas", C.c_double),
-        ("stage_threads", C.c_int32), ("reserved", C.c_int32),
+        ("stage_threads", C.c_int32), ("stage_launch", C.c_int32),
         ("kink_width", C.c_double), ("kink_scale", C.c_double * 5),
     ]
 

@@ -127,6 +127,7 @@ def make_config(cfg: Dict, precision: str = "f64", renumber="tiles", flux_mode: 
         if k in cfg:
             setattr(c, k, float(cfg[k]))
     c.stage_threads = int(cfg.get("stage_threads", 0))
+    c.stage_launch = int(cfg.get("stage_launch", 0))
     alpha = list(cfg.get("alpha", _lib.RK5))
     if not 1 <= len(alpha) <= _lib.FVH_MAX_ALPHA:
         raise ValueError("1..%d RK stages supported" % _lib.FVH_MAX_ALPHA)

@@ -148,7 +148,10 @@ typedef struct fvh_config {
     /* workgroup size of the fused stage pass: 0 = by the tile halo's LDS
      * (256 lanes when four workgroups fit a CU, else 512), or 256, 512 */
     int32_t stage_threads;
-    int32_t reserved;          /* 0 */
+    /* launch form of the fused stage pass: 0 = one workgroup per tile (the
+     * default), 1 = persistent workgroups pulling tiles from per-XCD work
+     * queues (same results) */
+    int32_t stage_launch;
     /* FVH_FLAG_SMOOTH_KINKS: relative width (0: 1e-2) and the absolute scales
      * it multiplies: rho, rho u, rho v, rho E (limiter extrema and blend) and
      * a wave speed (entropy cure, time step); a scale of 0 takes the
