@@ -411,8 +411,8 @@ def sweep(a, with_cpu=True):
         secs_p = time.perf_counter() - t0p
         persist = {"value": round(C * stages * a.steps / secs_p / 1e6, 2),
                    "ms_per_step": round(secs_p / a.steps * 1e3, 4),
-                   "launch": "persistent: resident workgroups (occupancy x CUs) pull the tiles from per-XCD work "
-                             "queues with stealing, one launch per stage, hipGraph per step"}
+                   "launch": "persistent: resident workgroups (as many as the LDS lets a CU hold, times the CUs) pull the tiles "
+                             "from per-XCD work queues with stealing, one launch per stage, hipGraph per step"}
         del bp
         note("sweep: persistent batched leg done")
     if not all(np.all(np.isfinite(x)) for x in qb):
