@@ -763,9 +763,11 @@ def step_workload(a, precision):
     if roof is not None:
         achievable(roof)
         roof["events_window_steps"] = ev_steps
-        tr = pmc_traffic(a.workload, precision, "stage" if "stage" in roof["kernel"] else "flux") \
+        # the default line's step leg runs the step workload
+        wname = "step" if a.workload == "headline" else a.workload
+        tr = pmc_traffic(wname, precision, "stage" if "stage" in roof["kernel"] else "flux") \
             if a.workload != "plate" else None
-        if tr is not None and (a.workload != "step" or a.n_per_unit == 440) and (
+        if tr is not None and (wname != "step" or a.n_per_unit == 440) and (
                 a.workload != "airfoil" or a.cells == 2_000_000):
             roof["traffic"] = tr["bytes"]
             roof["traffic_source"] = tr["source"]
