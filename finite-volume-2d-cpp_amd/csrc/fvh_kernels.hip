@@ -270,6 +270,16 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
     }
 }
 
+// The first chunk of a cell's half-edges as grad_lim_cell reads it (edge
+// records, where the neighbours are read from, edge geometry), for callers
+// that load it ahead.
+template <class T>
+struct HeFirst {
+    uint32_t ue[kHeChunk];
+    int nb[kHeChunk];
+    Q4<T> geo[kHeChunk];
+};
+
 // ---------------------------------------------------------------------------
 // Green-Gauss gradient + neighbour min/max + limiter of one real cell
 // (explicit.cpp:112-258), optionally its time step.  nidx(h) yields where
@@ -277,10 +287,11 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
 // and whether it is the cell itself; the two-pass and the fused forms differ
 // only in where they read from.
 // ---------------------------------------------------------------------------
-template <class T, int PHI, int DT, class NIdx, class NGet>
+template <class T, int PHI, int DT, bool PRE = false, class NIdx, class NGet>
 __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp, int grad, int lim_on, bool real,
                                      bool do_dt, int c, int h0, int h1, const Q4<T>& qc, const G2<T>& cc, NIdx nidx,
-                                     NGet nget, T (&gx)[4], T (&gy)[4], T (&lim)[4], T& dt_out) {
+                                     NGet nget, T (&gx)[4], T (&gy)[4], T (&lim)[4], T& dt_out,
+                                     const HeFirst<T>* first = nullptr) {
     const T g = T(pp.gamma);
 #pragma unroll
     for (int k = 0; k < 4; ++k) { gx[k] = T(0); gy[k] = T(0); }
@@ -300,21 +311,10 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
     // explicit.cpp:125-155), the neighbour min/max (:193-204) and the
     // time-step sum or minimum
     if ((real && (grad || lim_on)) || (DT && do_dt)) {
-        for (int hb = h0; hb < h1; hb += kHeChunk) {
-            // edge records and geometry of a chunk of half-edges loaded
-            // together (indices clamped to the cell's last half-edge), so
-            // that their latencies overlap
-            uint32_t uev[kHeChunk];
-            int nbv[kHeChunk];
-            Q4<T> geov[kHeChunk];
-#pragma unroll
-            for (int i = 0; i < kHeChunk; ++i) {
-                const int hh = min(hb + i, h1 - 1);
-                uev[i] = m.he_edge[hh];
-                nbv[i] = nidx(hh);
-            }
-#pragma unroll
-            for (int i = 0; i < kHeChunk; ++i) geov[i] = m.e_geo[uev[i] & kEdgeMask];
+        // one chunk of half-edges from their records, neighbour slots and
+        // edge geometry
+        auto chunk = [&](int hb, const uint32_t(&uev)[kHeChunk], const int(&nbv)[kHeChunk],
+                         const Q4<T>(&geov)[kHeChunk]) {
 #pragma unroll
             for (int i = 0; i < kHeChunk; ++i) {
                 const int h = hb + i;
@@ -374,6 +374,30 @@ __device__ inline void grad_lim_cell(const MeshView<T>& m, const PhysParams& pp,
                     }
                 }
             }
+        };
+        int hb0 = h0;
+        if (PRE) {
+            // the first chunk, loaded ahead by the caller (every cell has a
+            // half-edge)
+            chunk(h0, first->ue, first->nb, first->geo);
+            hb0 += kHeChunk;
+        }
+        for (int hb = hb0; hb < h1; hb += kHeChunk) {
+            // edge records and geometry of a chunk of half-edges loaded
+            // together (indices clamped to the cell's last half-edge), so
+            // that their latencies overlap
+            uint32_t uev[kHeChunk];
+            int nbv[kHeChunk];
+            Q4<T> geov[kHeChunk];
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(hb + i, h1 - 1);
+                uev[i] = m.he_edge[hh];
+                nbv[i] = nidx(hh);
+            }
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) geov[i] = m.e_geo[uev[i] & kEdgeMask];
+            chunk(hb, uev, nbv, geov);
         }
     }
     if (DT && do_dt) dt_out = (DT == 1 || DT == 4) ? T(pp.cfl) * m.c_area[c] / dts : dts;
@@ -450,13 +474,15 @@ __global__ __launch_bounds__(kBlock) void k_gradlim(MeshView<T> m, TileMap tm, P
 // to the just-updated state: the reference's stale-gradient boundary).
 // ---------------------------------------------------------------------------
 template <class T, bool LIN, class S, class Inc>
-__device__ inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
+__device__ __attribute__((always_inline)) inline void finish_cell(const MeshView<T>& m, const PhysParams& pp, const BcParam<T>* bcs, int c,
                                    int inst, int h0, int h1, const T (&qt)[4], const Q4<S>* __restrict__ q0,
                                    Q4<S>* __restrict__ qout, Inc inc_of,
                                    const T d, T alpha, bool want_rs,
-                                   double (&rs)[4], Q4<T>* __restrict__ dbg_qt) {
+                                   double (&rs)[4], Q4<T>* __restrict__ dbg_qt,
+                                   const Q4<T>* qs_pre = nullptr) {
     const T g = T(pp.gamma);
-    const Q4<T> qs = ldq<T>(q0 + c);
+    // the stage input, or the caller's copy of it loaded ahead
+    const Q4<T> qs = qs_pre ? *qs_pre : ldq<T>(q0 + c);
     Q4<T> out;
 #pragma unroll
     for (int k = 0; k < 4; ++k) out.v[k] = qs.v[k] + qt[k] * d * alpha;
@@ -768,16 +794,66 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
     const int32_t* xc = hv.cells + ex.x;
     const T g = T(pp.gamma);
     const T lim0 = lim_on ? T(1) : T(0);
-    // phase 0: states of tile, ring 1 and ring 2
-    for (int j = threadIdx.x; j < n_ext; j += NB) sq[j] = qin[j < nt ? c0 + j : xc[j - nt]];
+    // Each phase starts with a chain of dependent global loads (cell, its
+    // half-edge range, their records, the edge geometry).  The first links
+    // of a lane's next item are issued before the barrier that ends the
+    // previous phase, so that their latency overlaps the wait.
+    struct CellHead {
+        int c, h0, h1, r1;
+    };
+    auto cell_head = [&](int j) {
+        CellHead x;
+        const bool own = j < nt;
+        x.c = own ? c0 + j : xc[j - nt];
+        x.h0 = m.he_off[x.c];
+        x.h1 = m.he_off[x.c + 1];
+        x.r1 = own ? 0 : hv.r1_start[ex.x + j - nt];
+        return x;
+    };
+    // a lane without a phase-1 cell takes the last one, so that every load
+    // below stays in bounds
+    const int j_first = min((int)threadIdx.x, n_g - 1);
+    const CellHead head = cell_head(j_first);
+    // phase 0: states of tile, ring 1 and ring 2.  A lane's first two
+    // entries are loaded together (index loads, then state loads), so that
+    // their latencies overlap; a lane without an entry loads the tile's
+    // first cell (a tile has at least one) and stores nothing.
+    {
+        const int ja = threadIdx.x, jb = threadIdx.x + NB;
+        const bool va = ja < n_ext, vb = jb < n_ext;
+        const int ia = va ? (ja < nt ? c0 + ja : xc[ja - nt]) : c0;
+        const int ib = vb ? (jb < nt ? c0 + jb : xc[jb - nt]) : c0;
+        const Q4<S> sa = qin[ia], sb = qin[ib];
+        if (va) sq[ja] = sa;
+        if (vb) sq[jb] = sb;
+        for (int j = threadIdx.x + 2 * NB; j < n_ext; j += NB) sq[j] = qin[j < nt ? c0 + j : xc[j - nt]];
+    }
+    // the next links of the lane's first phase-1 cell: its centre and the
+    // first chunk of its half-edges (records, neighbour slots, geometry;
+    // every cell has a half-edge)
+    HeFirst<T> hf;
+    {
+        const uint16_t* loc = j_first < nt ? hv.he_loc + head.h0 : hv.r1_loc + head.r1;
+#pragma unroll
+        for (int i = 0; i < kHeChunk; ++i) {
+            const int hh = min(head.h0 + i, head.h1 - 1);
+            hf.ue[i] = m.he_edge[hh];
+            hf.nb[i] = loc[hh - head.h0];
+        }
+#pragma unroll
+        for (int i = 0; i < kHeChunk; ++i) hf.geo[i] = m.e_geo[hf.ue[i] & kEdgeMask];
+    }
+    const G2<T> cc_first = m.c_c[head.c];
     __syncthreads();
     // phase 1: gradients and limiters of the tile and its real ring-1 cells,
-    // and their face states on the tile's edges
-    for (int j = threadIdx.x; j < n_g; j += NB) {
+    // and their face states on the tile's edges; a lane's first cell from
+    // the links loaded ahead (PRE), further ones (tiles whose cells with
+    // their real ring 1 outnumber the lanes) from scratch
+    const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
+    auto p1_cell = [&](int j, const CellHead& hd, auto pre) {
+        constexpr bool PRE = decltype(pre)::value;
         const bool own = j < nt;
-        const int c = own ? c0 + j : xc[j - nt];
-        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-        const int r1 = own ? 0 : hv.r1_start[ex.x + j - nt];
+        const int c = hd.c, h0 = hd.h0, h1 = hd.h1, r1 = hd.r1;
         const uint16_t* loc = own ? hv.he_loc + h0 : hv.r1_loc + r1;
         T gx[4], gy[4], lim[4], dtc = T(0);
         auto nidx = [&](int h) { return (int)loc[h - h0]; };
@@ -786,8 +862,9 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
             return ldq<T>(sq + l);
         };
         const Q4<T> qc = ldq<T>(sq + j);
-        const G2<T> cc = m.c_c[c];
-        grad_lim_cell<T, PHI, DT>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim, dtc);
+        const G2<T> cc = PRE ? cc_first : m.c_c[c];
+        grad_lim_cell<T, PHI, DT, PRE>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim,
+                                       dtc, &hf);
         if (DT && own) dt[c] = dtc;
         if (rec.gx && own) {
             Q4<T> a, b, d;
@@ -846,12 +923,18 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
                 }
             }
         }
+    };
+    if ((int)threadIdx.x < n_g) {
+        p1_cell(threadIdx.x, head, std::integral_constant<bool, true>{});
+        for (int j = threadIdx.x + NB; j < n_g; j += NB) p1_cell(j, cell_head(j), std::integral_constant<bool, false>{});
     }
+    Q4<T> geo_next;
+    if (e0 + (int)threadIdx.x < e1) geo_next = m.e_geo[tv.edge[e0 + threadIdx.x]];
     __syncthreads();
     // phase 2: each edge of the tile once, flux times length left in the slot
-    const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
     for (int k = e0 + (int)threadIdx.x; k < e1; k += NB) {
-        const Q4<T> geo = m.e_geo[tv.edge[k]];
+        const Q4<T> geo = geo_next;
+        if (k + NB < e1) geo_next = m.e_geo[tv.edge[k + NB]];
         const int sl = k - e0;
         const Q4<T> a = fs[2 * sl], b = fs[2 * sl + 1];
         T f[4];
@@ -861,14 +944,37 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
         for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
         fs[2 * sl] = r;
     }
+    // phase 3's first links: the cell's stage input, time step (a local one
+    // was stored in phase 1 by this same lane), half-edge range, records and
+    // area
+    const int j = threadIdx.x;
+    int p3_h0 = 0, p3_h1 = 0;
+    uint32_t p3_ue[kHeChunk];
+    int p3_slot[kHeChunk];
+    T p3_A = T(1), p3_dt = T(0);
+    Q4<T> p3_q0;
+    if (j < nt) {
+        p3_q0 = ldq<T>(q0 + c0 + j);
+        p3_dt = dt_inst ? dt_inst[inst] : dt[c0 + j];
+        p3_h0 = m.he_off[c0 + j];
+        p3_h1 = m.he_off[c0 + j + 1];
+        p3_A = m.c_area[c0 + j];
+        if (p3_h1 > p3_h0) {
+#pragma unroll
+            for (int i = 0; i < kHeChunk; ++i) {
+                const int hh = min(p3_h0 + i, p3_h1 - 1);
+                p3_ue[i] = m.he_edge[hh];
+                p3_slot[i] = m.he_slot[hh];
+            }
+        }
+    }
     __syncthreads();
     // phase 3: ordered gather, RK update, boundary update
     double rs[4] = {0.0, 0.0, 0.0, 0.0};
-    const int j = threadIdx.x;
     if (j < nt) {
         const int c = c0 + j;
-        const int h0 = m.he_off[c], h1 = m.he_off[c + 1];
-        const T A = m.c_area[c];
+        const int h0 = p3_h0, h1 = p3_h1;
+        const T A = p3_A;
         T qt[4] = {T(0), T(0), T(0), T(0)};
         for (int hb = h0; hb < h1; hb += kHeChunk) {
             uint32_t uev[kHeChunk];
@@ -876,8 +982,8 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
 #pragma unroll
             for (int i = 0; i < kHeChunk; ++i) {
                 const int hh = min(hb + i, h1 - 1);
-                uev[i] = m.he_edge[hh];
-                slotv[i] = m.he_slot[hh];
+                uev[i] = hb == h0 ? p3_ue[i] : m.he_edge[hh];
+                slotv[i] = hb == h0 ? p3_slot[i] : (int)m.he_slot[hh];
             }
 #pragma unroll
             for (int i = 0; i < kHeChunk; ++i) {
@@ -892,8 +998,8 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
             }
         }
         auto inc = [&](int, int nb) { return ld(bincr + (nb - m.n_real)); };
-        finish_cell<T, true>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, dt_inst ? dt_inst[inst] : dt[c], alpha,
-                             res_blk != nullptr, rs, rec.qt);
+        finish_cell<T, true>(m, pp, bcs, c, inst, h0, h1, qt, q0, qout, inc, p3_dt, alpha, res_blk != nullptr, rs,
+                             rec.qt, &p3_q0);
     }
     if (res_blk) {
         block_sum4<NB>(rs);
