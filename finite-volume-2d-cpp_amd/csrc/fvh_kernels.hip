@@ -257,10 +257,12 @@ __device__ inline void limit_cell(const MeshView<T>& m, const PhysParams& pp, in
                         const T t = dqg / sqrt(dqg * dqg + w * w);
                         const T d = T(0.5) * (dmax + dmin) + T(0.5) * (dmax - dmin) * t;
                         l = venkat_dq(d, dqg, w * w);
-                    } else if (dqg > tol) {
-                        l = phi<T, PHI>(dmax / dqg, yt);
-                    } else if (dqg < -tol) {
-                        l = phi<T, PHI>(dmin / dqg, yt);
+                    } else if (dqg > tol || dqg < -tol) {
+                        // phi(dmax / dqg) or phi(dmin / dqg) by the sign of
+                        // dqg: one quotient and one phi per lane, so a wave
+                        // whose lanes differ in sign issues them once
+                        const T num = dqg > tol ? dmax : dmin;
+                        l = phi<T, PHI>(num / dqg, yt);
                     }
                 }
                 l = sig + (T(1) - sig) * l;
