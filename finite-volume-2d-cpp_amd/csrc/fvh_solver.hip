@@ -1009,7 +1009,7 @@ bool same_physics(const fvh_config& a, const fvh_config& b) {
 // Build the device layout of n instances and upload it.  splits marks, per
 // instance, the RCB tiles to bisect once more (null: none).
 int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_solver** out,
-               const std::vector<std::vector<uint8_t>>* splits = nullptr) {
+               const std::vector<std::vector<uint8_t>>* splits = nullptr, bool auto_shrunk = false) {
     if (n < 1 || !ms || !cs || !out) { set_error("null argument or empty batch"); return FVH_E_ARG; }
     int32_t rc;
     for (int32_t i = 0; i < n; ++i) {
@@ -1053,7 +1053,6 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         return FVH_E_UNSUPPORTED;
     }
     if (hipGetDevice(&s->device) != hipSuccess) { set_error("no HIP device"); return FVH_E_HIP; }
-    FVH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->n_real = (int32_t)tot_real;
     s->n_total = (int32_t)tot_cells;
     s->n_edges = (int32_t)tot_edges;
@@ -1210,15 +1209,14 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         if (he_bc[h] >= 0) w |= kBcBit;
         he_edge[h] = w;
     }
+    std::vector<int4> ep(E, make_int4(-1, -1, -1, -1));
     {
-        std::vector<int4> ep(E, make_int4(-1, -1, -1, -1));
         for (int32_t d = 0; d < N; ++d)
             for (int32_t h = off[d]; h < off[d + 1]; ++h) {
                 const uint32_t ue = he_edge[h];
                 int4& x = ep[ue & kEdgeMask];
                 if (ue & kSideBit) { x.y = h; x.w = d; } else { x.x = h; x.z = d; }
             }
-        if ((rc = upload(s->epair, ep.data(), ep.size() * sizeof(int4)))) return rc;
         const fvh_mesh& m0 = *ms[0];
         for (size_t b = 0; b < m0.bnd_edge.size(); ++b) {
             const int32_t e = m0.bnd_edge[b];
@@ -1279,9 +1277,22 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
         // halo bytes scale slightly slower than the cell count: 5% margin
         const bool three_fit = (double)lds * t / kBlock * 1.05 + 1024 <= 160.0 * 1024 / 3;
         if (t < kBlock && mean_g > half && (lanes == half || (c.stage_threads == 0 && three_fit))) {
+            // the layout so far is host data only (the stream and the
+            // device arrays come after these decisions)
             std::vector<fvh_config> shrunk(cs, cs + n);
             for (fvh_config& x : shrunk) x.tile_cells = t;
-            return create(n, ms, shrunk.data(), out);
+            return create(n, ms, shrunk.data(), out, nullptr, true);
+        }
+    }
+    // A shrink is kept only if it gave what it was made for: 256-lane
+    // workgroups whose gradient phase takes one round; otherwise the tiles
+    // of 256 cells are rebuilt.
+    if (auto_shrunk && s->fused && s->n_rt > 0) {
+        const int32_t mean_g = (int32_t)((hh.sum_g + s->n_rt - 1) / s->n_rt);
+        if (mean_g > kStageBlock / 2 || stage_lanes(halo_lds_bytes(hh), c.stage_threads, mean_g) != kStageBlock / 2) {
+            std::vector<fvh_config> full(cs, cs + n);
+            for (fvh_config& x : full) x.tile_cells = kBlock;
+            return create(n, ms, full.data(), out);
         }
     }
     // Oversized tiles: when a few tiles' halos set an LDS size that lets one
@@ -1313,6 +1324,9 @@ int32_t create(int32_t n, const fvh_mesh* const* ms, const fvh_config* cs, fvh_s
                   "residual smoothing and a tile halo that fits in LDS");
         return FVH_E_UNSUPPORTED;
     }
+    // the layout is final: device work starts here
+    FVH_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    if ((rc = upload(s->epair, ep.data(), ep.size() * sizeof(int4)))) return rc;
     if (s->fused) {
         s->x_max_ext = hh.max_ext;
         s->x_max_g = hh.max_g;

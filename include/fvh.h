@@ -196,7 +196,7 @@ int32_t fvh_abi_version(void);
 int32_t fvh_mesh_read_msh(const char* path, fvh_mesh** out);
 
 /* A 2-D SU2 ASCII mesh (triangles and quadrilaterals, line markers), as the
- * reference's examples/naca/*.su2; no counterpart in the reference, which
+ * reference's examples/naca/NACA_M.su2; no counterpart in the reference, which
  * converts such files to MSH 4.1 with gmsh (examples/naca/naca.geo:1-5).
  * Cells, nodes and marker elements keep file order; marker tags are the
  * physical names bound to boundary kinds. */
