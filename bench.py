@@ -168,19 +168,41 @@ def sweep_write_meshes(items, work, part=0, parts=1):
     return sweep_mesh_names(items)
 
 
+def wait_for_release(a):
+    """A gated worker blocks until its parent closes its stdin (release);
+    it exits if the parent ended without releasing it."""
+    if a.wait_stdin:
+        parent = os.getppid()
+        sys.stdin.read()
+        if os.getppid() != parent:
+            sys.exit(1)
+
+
+def release(procs):
+    """Let gated workers start: close the stdin each one waits on."""
+    for p in procs:
+        if p.stdin is not None:
+            p.stdin.close()
+            p.stdin = None
+
+
 def mesh_writer_worker(a):
     """CPU only: write this worker's share of the sweep meshes."""
+    wait_for_release(a)
     from fvhyper_amd import workloads
     part, parts = (int(x) for x in a.mesh_writer.split("/"))
     sweep_write_meshes(workloads.naca_sweep(a.instances), a.mesh_dir, part, parts)
 
 
 def start_mesh_writers(a, work, procs):
+    """Spawn `procs` mesh writers (lowest priority), gated: each waits for
+    release()."""
     out = []
     for k in range(procs):
         cmd = ["nice", "-n", "19", sys.executable, os.path.abspath(__file__), "--mesh-writer", "%d/%d" % (k, procs),
-               "--mesh-dir", work, "--instances", str(a.instances)]
-        out.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+               "--mesh-dir", work, "--instances", str(a.instances), "--wait-stdin"]
+        out.append(subprocess.Popen(cmd, stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                    text=True))
     return out
 
 
@@ -530,6 +552,7 @@ def cstep_worker(a):
     direction k of the scheme with smoothed kinks), from the state and
     weights the parent wrote to --cstep-dir; prints one JSON line per
     direction."""
+    wait_for_release(a)
     import numpy as np
     O = _oracle()
     from fvhyper_amd import workloads
@@ -549,8 +572,9 @@ def cstep_worker(a):
 
 def start_cstep_workers(a, q0, w, n_dirs, procs):
     """Spawn `procs` CPU worker processes (lowest priority) for n_dirs
-    directions of the scheme and n_dirs of its smoothed-kink form; returns
-    the handles."""
+    directions of the scheme and n_dirs of its smoothed-kink form, gated:
+    each waits for release(), so that no GPU leg is timed beside them;
+    returns the handles."""
     import numpy as np
     d = tempfile.mkdtemp(prefix="fvh_cstep_")
     np.save(os.path.join(d, "q0.npy"), q0)
@@ -560,8 +584,9 @@ def start_cstep_workers(a, q0, w, n_dirs, procs):
     out = []
     for g in groups:
         cmd = ["nice", "-n", "19", sys.executable, os.path.abspath(__file__), "--cstep-worker",
-               ",".join(str(k) for k in g), "--cstep-dir", d, "--adj-steps", str(a.adj_steps)]
-        out.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True))
+               ",".join(str(k) for k in g), "--cstep-dir", d, "--adj-steps", str(a.adj_steps), "--wait-stdin"]
+        out.append(subprocess.Popen(cmd, stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                    text=True))
     return out
 
 
@@ -818,8 +843,9 @@ def line(result, a):
 
 def headline(a):
     """The default line: the step workload at top level, the sweep and the
-    adjoint as sub-objects, CPU baselines after the GPU work (the complex-step
-    workers run during it)."""
+    adjoint as sub-objects; the CPU work (complex-step workers, sweep mesh
+    writers, CPU baselines) after the GPU legs, so that none of them is
+    timed beside it."""
     import numpy as np
     import torch
 
@@ -827,7 +853,8 @@ def headline(a):
     t_start = time.perf_counter()
     info = cpu_info()
     # the CPU workers (complex step, sweep meshes for the reference runs)
-    # start before any HIP call of this process
+    # are spawned before any HIP call of this process and wait until the
+    # GPU legs are done
     sweep_dir = tempfile.mkdtemp(prefix="fvh_sweep_")
     writers = [] if a.no_cpu else start_mesh_writers(a, sweep_dir, 4)
     workers = []
@@ -841,6 +868,7 @@ def headline(a):
     sw = sweep(argparse.Namespace(**dict(vars(a), precision=a.sweep_precision)), with_cpu=False)
     adj = adjoint(a)
     t_gpu = time.perf_counter() - t_start
+    release(writers + workers)
     # the workers' results, then the CPU baselines on the freed cores
     cs = collect_cstep_workers(workers) if workers else []
     if cs:
@@ -892,8 +920,8 @@ def adjoint_complex_step_check(adj, cs, smooth=False):
             "median_rel_err": float(np.median(errs)),
             "J_cpu_minus_gpu": float(max(abs(r["J"] - J) for r in cs)),
             "cpu_s_per_direction": round(float(np.median([r["seconds"] for r in cs])), 1),
-            "where": "CPU worker processes, the templated restatement on complex numbers (h = 1e-30), run while "
-                     "the GPU measured the other workloads"}
+            "where": "CPU worker processes, the templated restatement on complex numbers (h = 1e-30), run after "
+                     "the GPU legs"}
 
 
 def main():
@@ -917,6 +945,7 @@ def main():
     ap.add_argument("--cstep-dir", default="", help=argparse.SUPPRESS)
     ap.add_argument("--mesh-writer", default="", help=argparse.SUPPRESS)
     ap.add_argument("--mesh-dir", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--wait-stdin", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--instances", type=int, default=256, help="sweep: number of airfoil instances")
     ap.add_argument("--per-mesh-steps", type=int, default=0,
                     help="sweep: steps of the one-by-one baseline (default: --steps)")
@@ -965,6 +994,7 @@ def main():
         import torch
         torch.cuda.set_device(0)
         r = adjoint(a)
+        release(workers)
         cs = collect_cstep_workers(workers) if workers else []
         if cs:
             r["complex_step"] = adjoint_complex_step_check(r, cs)
