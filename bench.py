@@ -546,6 +546,26 @@ def adj_direction(n_cells, n_real, k):
     return v.ravel()
 
 
+def adj_smooth_direction(cx, cy, n_real, k):
+    """Smooth direction k of the adjoint checks: per variable, eight Gaussian
+    bumps of seeded centres (in the cells' bounding box), widths (0.1-0.3)
+    and normal amplitudes at the cell centres, scaled to unit RMS over the
+    real cells and then per variable as the white-noise directions; zero on
+    the ghost cells."""
+    import numpy as np
+    rng = np.random.default_rng(ADJ_DIR_SEED + 5000 + k)
+    x0, x1, y0, y1 = cx[:n_real].min(), cx[:n_real].max(), cy[:n_real].min(), cy[:n_real].max()
+    v = np.zeros((cx.size, 4))
+    for var in range(4):
+        f = np.zeros(cx.size)
+        for _ in range(8):
+            bx, by, wd = rng.uniform(x0, x1), rng.uniform(y0, y1), rng.uniform(0.1, 0.3)
+            f += rng.standard_normal() * np.exp(-((cx - bx) ** 2 + (cy - by) ** 2) / (wd * wd))
+        v[:, var] = f / np.sqrt(np.mean(f[:n_real] ** 2)) * ADJ_SCALE[var]
+    v[n_real:] = 0.0
+    return v.ravel()
+
+
 def cstep_worker(a):
     """CPU only (no HIP call): the complex-step derivative of J along the
     directions in --cstep-worker on the adjoint workload (an entry "s<k>":
@@ -657,19 +677,30 @@ def adjoint(a):
     t0 = time.perf_counter()
     _, grad_s, st_s = ss.adjoint(q0, N, wx=w, mem_cap_bytes=8e9)
     t_adj_s = time.perf_counter() - t0
-    eps_list = (1e-6, 5e-7, 2.5e-7)
+    eps_list = (1e-6, 5e-7, 2.5e-7, 1e-7)
+
+    def fd4_err(v, eps):
+        ad = float(grad_s @ v)
+        Jv = {j: ss.objective(q0 + j * eps * v, N, wx=w) for j in (-2, -1, 1, 2)}
+        d4 = (8 * (Jv[1] - Jv[-1]) - (Jv[2] - Jv[-2])) / (12 * eps)
+        d2 = (Jv[1] - Jv[-1]) / (2 * eps)
+        return abs(ad - d4) / max(abs(d4), 1e-300), abs(ad - d2) / max(abs(d2), 1e-300)
     fd4 = {e: [] for e in eps_list}
     fd2 = []
     for k in range(a.adj_dirs):
         v = adj_direction(pm.n_cells, pm.n_real, 1000 + k)
-        ad = float(grad_s @ v)
         for eps in eps_list:
-            Jv = {j: ss.objective(q0 + j * eps * v, N, wx=w) for j in (-2, -1, 1, 2)}
-            d4 = (8 * (Jv[1] - Jv[-1]) - (Jv[2] - Jv[-2])) / (12 * eps)
-            fd4[eps].append(abs(ad - d4) / max(abs(d4), 1e-300))
+            e4, e2 = fd4_err(v, eps)
+            fd4[eps].append(e4)
             if eps == 1e-6:
-                d2 = (Jv[1] - Jv[-1]) / (2 * eps)
-                fd2.append(abs(ad - d2) / max(abs(d2), 1e-300))
+                fd2.append(e2)
+    # smooth random directions (Gaussian bumps): no grid-scale content
+    cx, cy = pm.export("cellsCentersX"), pm.export("cellsCentersY")
+    fd4_sm = {e: [] for e in (1e-6, 1e-7)}
+    for k in range(min(a.adj_dirs, 16)):
+        v = adj_smooth_direction(cx, cy, pm.n_real, k)
+        for eps in fd4_sm:
+            fd4_sm[eps].append(fd4_err(v, eps)[0])
 
     def fd_stats(errs):
         return None if not errs else {
@@ -687,7 +718,12 @@ def adjoint(a):
         "fd4_eps_1e-6": fd_stats(fd4[1e-6]),
         "fd4_eps_5e-7": fd_stats(fd4[5e-7]),
         "fd4_eps_2.5e-7": fd_stats(fd4[2.5e-7]),
+        "fd4_eps_1e-7": fd_stats(fd4[1e-7]),
         "fd2_eps_1e-6": fd_stats(fd2),
+        "smooth_directions": {
+            "what": "Gaussian-bump fields (8 per variable, widths 0.1-0.3, unit RMS, scaled as the white-noise "
+                    "directions)",
+            "fd4_eps_1e-6": fd_stats(fd4_sm[1e-6]), "fd4_eps_1e-7": fd_stats(fd4_sm[1e-7])},
         "complex_step": None,
     }
     cs = None
@@ -897,6 +933,8 @@ def headline(a):
                        "adjoint_complex_step_max_rel_err": (adj["complex_step"] or {}).get("max_rel_err"),
                        "adjoint_smoothed_fd4_eps_1e-6_max_rel_err":
                            (adj["smoothed_kinks"]["fd4_eps_1e-6"] or {}).get("max_rel_err"),
+                       "adjoint_smoothed_fd4_eps_1e-7_max_rel_err":
+                           (adj["smoothed_kinks"]["fd4_eps_1e-7"] or {}).get("max_rel_err"),
                        "adjoint_smoothed_complex_step_max_rel_err":
                            (adj["smoothed_kinks"]["complex_step"] or {}).get("max_rel_err"),
                        "cpu": info["model"], "cpu_cores_used": info["ranks"]}
