@@ -124,9 +124,20 @@ __global__ __launch_bounds__(kBlock) void k_dt_min(InstView iv, const double* __
     const int i = blockIdx.x;
     if (stopped(ctrl, i)) return;
     const int4 r = iv.tiles[i];
-    double v = kInf;
-    for (int t = r.x + threadIdx.x; t < r.y; t += kBlock) v = smin(v, blk_min[t]);
-    for (int t = r.z + threadIdx.x; t < r.w; t += kBlock) v = smin(v, blk_min[t]);
+    // four loads in flight per lane; the minimum of the same values in any
+    // order (a running minimum from +inf never becomes NaN under smin)
+    double v0 = kInf, v1 = kInf, v2 = kInf, v3 = kInf;
+    int t = r.x + threadIdx.x;
+    for (; t + 3 * kBlock < r.y; t += 4 * kBlock) {
+        const double a = blk_min[t], b = blk_min[t + kBlock], c = blk_min[t + 2 * kBlock], d = blk_min[t + 3 * kBlock];
+        v0 = smin(v0, a);
+        v1 = smin(v1, b);
+        v2 = smin(v2, c);
+        v3 = smin(v3, d);
+    }
+    for (; t < r.y; t += kBlock) v0 = smin(v0, blk_min[t]);
+    for (t = r.z + threadIdx.x; t < r.w; t += kBlock) v1 = smin(v1, blk_min[t]);
+    double v = smin(smin(v0, v1), smin(v2, v3));
     v = block_min(v);
     if (threadIdx.x == 0) {
         // min_dt starts from dt[0] (explicit.cpp:557): a NaN there survives
