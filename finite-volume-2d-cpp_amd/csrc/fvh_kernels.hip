@@ -876,9 +876,12 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
         };
         const Q4<T> qc = ldq<T>(sq + j);
         const G2<T> cc = PRE ? cc_first : m.c_c[c];
-        grad_lim_cell<T, PHI, DT, PRE>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim,
-                                       dtc, &hf);
-        if (DT && own) dt[c] = dtc;
+        // the summed-CFL local step (DT 1) is formed after the face states
+        // below, where the gradient phase's registers are free
+        constexpr int DTG = DT == 1 ? 0 : DT;
+        grad_lim_cell<T, PHI, DTG, PRE>(m, pp, grad, lim_on, true, own, c, h0, h1, qc, cc, nidx, nget, gx, gy, lim,
+                                        dtc, &hf);
+        if (DTG && own) dt[c] = dtc;
         if (rec.gx && own) {
             Q4<T> a, b, d;
 #pragma unroll
@@ -935,6 +938,22 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
                     fs[2 * slot + 1 - side] = fg;
                 }
             }
+        }
+        if (DT == 1 && own) {
+            // summed-CFL local step (forwardStep/main.cpp:149-184): the
+            // terms grad_lim_cell sums, in the same half-edge order
+            const EigCell<T> eo_c = eig_cell(g, qc.v);
+            T sum = T(0);
+            for (int h = h0; h < h1; ++h) {
+                const uint32_t ue = m.he_edge[h];
+                const Q4<T> geo = m.e_geo[ue & kEdgeMask];
+                const Q4<T> qn = ldq<T>(sq + loc[h - h0]);
+                const bool side = (ue & kSideBit) != 0;
+                const T eo = eig_sum(eo_c, geo.v[0], geo.v[1]);
+                const T en = eig_sum(g, qn.v, geo.v[0], geo.v[1]);
+                sum = sum + (side ? smax(en, eo) : smax(eo, en)) * geo.v[2];
+            }
+            dt[c] = T(pp.cfl) * m.c_area[c] / sum;
         }
     };
     if ((int)threadIdx.x < n_g) {
@@ -1022,9 +1041,13 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
 }
 
 // One launch grid over the tiles: workgroup b takes logical tile
-// xcd_block(n_rt) (each XCD one contiguous range of tiles).
+// xcd_block(n_rt) (each XCD one contiguous range of tiles).  At least four
+// waves per SIMD (at most 128 VGPRs): the halos of the production meshes
+// let four 256-lane workgroups share a CU, and the local-time-step and
+// smoothed-kink stages, which took 131-137 VGPRs (three waves), fit with a
+// few registers spilled.
 template <class T, int PHI, int DT, bool JOINED, int NB, class S>
-__global__ __launch_bounds__(NB) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+__global__ __launch_bounds__(NB, 4) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
                                                   int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
                                                   const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
                                                   Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
