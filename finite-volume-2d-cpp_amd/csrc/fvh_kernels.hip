@@ -1542,9 +1542,10 @@ void launch_stage_nb(hipStream_t s, const MeshView<T>& m, const TileMap& tm, con
     }
     if (resident == 0 || resident_lds != lds) {
         // workgroups per CU by the 160 KB of LDS (with 1 KB for the static
-        // arrays) and by the 32 wave slots; the registers (under 128 VGPRs)
-        // do not bind.  More workgroups than fit at once would only queue
-        // behind the resident ones (none waits on another).
+        // arrays) and by the 32 wave slots.  The persistent form's registers
+        // (about 150 VGPRs: three waves per SIMD) can allow fewer; more
+        // workgroups than fit at once only queue behind the resident ones
+        // (none waits on another).
         int dev = 0, cus = 0;
         const int per_cu = std::max(1, std::min((int)(160 * 1024 / (lds + 1024)), 32 / (NB / 64)));
         (void)hipGetDevice(&dev);
