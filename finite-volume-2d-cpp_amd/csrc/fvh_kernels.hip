@@ -1041,13 +1041,14 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
 }
 
 // One launch grid over the tiles: workgroup b takes logical tile
-// xcd_block(n_rt) (each XCD one contiguous range of tiles).  At least four
-// waves per SIMD (at most 128 VGPRs): the halos of the production meshes
-// let four 256-lane workgroups share a CU, and the local-time-step and
-// smoothed-kink stages, which took 131-137 VGPRs (three waves), fit with a
-// few registers spilled.
+// xcd_block(n_rt) (each XCD one contiguous range of tiles).  The stage that
+// forms the summed-CFL local time step and the smoothed-kink stage ask for
+// four waves per SIMD (at most 128 VGPRs; they took 131-137, three waves):
+// the halos of the production meshes let four 256-lane workgroups share a
+// CU, and they fit with a few registers spilled (measured faster, and
+// bit-exact in the tests).  The other forms keep the compiler's choice.
 template <class T, int PHI, int DT, bool JOINED, int NB, class S>
-__global__ __launch_bounds__(NB, 4) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
+__global__ __launch_bounds__(NB, (DT == 1 || PHI == PHI_VENKAT_S) ? 4 : 1) void k_stage(MeshView<T> m, TileMap tm, TileView tv, HaloView hv, PhysParams pp,
                                                   int grad, int lim_on, const BcParam<T>* __restrict__ bcs,
                                                   const Q4<S>* __restrict__ qin, const Q4<S>* __restrict__ q0,
                                                   Q4<S>* __restrict__ qout, Q4<T>* __restrict__ bincr,
