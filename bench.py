@@ -224,7 +224,8 @@ def sweep_cpu_baseline(items, files, work, steps: int, ranks: int):
     flags = O.ref_flags(cfg2)
 
     def one(k):
-        r = O.run_ref("run", work, files[k], flags=flags, timeout=900, **kv)
+        # instances share meshes: a dump file of its own per instance
+        r = O.run_ref("run", work, files[k], flags=flags, timeout=900, tag="i%d" % k, **kv)
         return float(r["stdout"].strip().splitlines()[-1].split(",")[2])
 
     t0 = time.perf_counter()

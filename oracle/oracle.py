@@ -531,10 +531,11 @@ def ref_available() -> bool:
 
 
 def run_ref(mode: str, workdir: str, mesh_name: str, flags: str = "", timeout: float = 600,
-            **kv) -> Dict:
+            tag: str = "", **kv) -> Dict:
     """Run the reference harness in ``workdir`` (which holds mesh_name.msh).
-    Returns the dump dict plus 'stdout'."""
-    out = os.path.join(workdir, mesh_name + "." + mode + ".dump")
+    Returns the dump dict plus 'stdout'.  Runs that may overlap on the same
+    mesh need distinct tags: the dump file is named after mesh, mode and tag."""
+    out = os.path.join(workdir, mesh_name + "." + mode + ("." + tag if tag else "") + ".dump")
     args = [REF_BIN, mode, mesh_name, out] + ["%s=%s" % (k, v) for k, v in kv.items()]
     env = dict(os.environ)
     env["FVH_REF_FLAGS"] = flags
