@@ -108,3 +108,27 @@ def test_oracle_matches_live_reference_on_larger_mesh(tmp_path):
     r = O.run_ref("run", str(tmp_path), "case", flags=O.ref_flags(cfg2), **kv)
     q, steps, t, R, R0, _ = om.run(cfg2, q0, 20, print_interval=1000)
     assert bits_equal(q, r["q"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference harness not built (needs the reference tree)")
+def test_concurrent_reference_runs_on_one_mesh_keep_their_dumps(tmp_path):
+    """Runs of the reference harness that overlap on one mesh (as the
+    sweep's CPU baseline starts them, several instances sharing a mesh)
+    write dump files of their own when tagged, and every one reads back
+    whole: the same final state from each of eight concurrent runs."""
+    import concurrent.futures as cf
+    import gen_fixtures as G
+    m = meshgen.forward_step(n_per_unit=20)
+    cfg = G.FIX_CONFIGS["second_venkat_global"]
+    m2, cfg2 = G.renamed(m, cfg)
+    meshgen.write_msh(str(tmp_path / "case.msh"), m2)
+    kv = O.ref_args(cfg2)
+    kv.update(mach="3", ic="fs", max_step="3", print="1000")
+
+    def one(k):
+        return O.run_ref("run", str(tmp_path), "case", flags=O.ref_flags(cfg2), tag="i%d" % k, **kv)["q"]
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        qs = list(ex.map(one, range(8)))
+    assert len(set(os.listdir(tmp_path)) & {"case.run.i%d.dump" % k for k in range(8)}) == 8
+    for q in qs[1:]:
+        assert bits_equal(q, qs[0])
