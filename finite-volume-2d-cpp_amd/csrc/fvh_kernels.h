@@ -213,14 +213,16 @@ bool launch_stage(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const 
 // LDS of the fused stage pass for a tile of n_ext local cells and n_edges
 // edges: the states of its local cell list in the storage type, then, from
 // the next 32-byte boundary, two face states per tile edge in the
-// arithmetic type.  A launch takes the largest tile's (stage_lds_bytes).
+// arithmetic type, then (positivity fallback on) the pressures of those
+// face states.  A launch takes the largest tile's (stage_lds_bytes).
 template <class S>
 __host__ __device__ inline size_t stage_lds_states(int32_t n_ext) {
     return ((size_t)n_ext * sizeof(Q4<S>) + 31) / 32 * 32;
 }
 template <class T, class S>
-__host__ __device__ inline size_t stage_lds_tile(int32_t n_ext, int32_t n_edges) {
-    return stage_lds_states<S>(n_ext) + (size_t)(2 * n_edges) * sizeof(Q4<T>);
+__host__ __device__ inline size_t stage_lds_tile(int32_t n_ext, int32_t n_edges, bool pressures = false) {
+    return stage_lds_states<S>(n_ext) + (size_t)(2 * n_edges) * sizeof(Q4<T>) +
+           (pressures ? (size_t)(2 * n_edges) * sizeof(T) : 0);
 }
 template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv);

@@ -863,6 +863,9 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
     // the links loaded ahead (PRE), further ones (tiles whose cells with
     // their real ring 1 outnumber the lanes) from scratch
     const int e0 = tv.off[lb], e1 = tv.off[lb + 1];
+    // positivity fallback on: each face state's pressure, computed for the
+    // fallback test in phase 1, kept for the edge's flux in phase 2
+    T* fpr = (T*)(fs + 2 * (e1 - e0));
     auto p1_cell = [&](int j, const CellHead& hd, auto pre) {
         constexpr bool PRE = decltype(pre)::value;
         const bool own = j < nt;
@@ -915,7 +918,15 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
                 Q4<T> f;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) f.v[k] = qc.v[k] + inc.v[k];
-                if (pp.positivity && !physical(g, f.v)) f = qc;
+                if (pp.positivity) {
+                    // physical(): rho > 0 and p > 0
+                    T pf = pressure(g, f.v);
+                    if (!(f.v[0] > T(0) && pf > T(0))) {
+                        f = qc;
+                        pf = pressure(g, qc.v);
+                    }
+                    fpr[2 * slot + side] = pf;
+                }
                 fs[2 * slot + side] = f;
                 if (own && (ue & kBcBit)) {
                     // the ghost's face: its state with a zero gradient and the reset limiter
@@ -934,7 +945,14 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
                     Q4<T> fg;
 #pragma unroll
                     for (int k = 0; k < 4; ++k) fg.v[k] = qg.v[k] + ginc.v[k];
-                    if (pp.positivity && !physical(g, fg.v)) fg = qg;
+                    if (pp.positivity) {
+                        T pg = pressure(g, fg.v);
+                        if (!(fg.v[0] > T(0) && pg > T(0))) {
+                            fg = qg;
+                            pg = pressure(g, qg.v);
+                        }
+                        fpr[2 * slot + 1 - side] = pg;
+                    }
                     fs[2 * slot + 1 - side] = fg;
                 }
             }
@@ -970,7 +988,11 @@ __device__ inline void stage_tile(int lb, unsigned char* lds_raw, const MeshView
         const int sl = k - e0;
         const Q4<T> a = fs[2 * sl], b = fs[2 * sl + 1];
         T f[4];
-        roe_flux<T, JOINED, PHI == PHI_VENKAT_S>(g, f, a.v, b.v, geo.v[0], geo.v[1], T(pp.kwv));
+        if (pp.positivity)
+            roe_flux<T, JOINED, PHI == PHI_VENKAT_S, true>(g, f, a.v, b.v, geo.v[0], geo.v[1], T(pp.kwv),
+                                                           fpr[2 * sl], fpr[2 * sl + 1]);
+        else
+            roe_flux<T, JOINED, PHI == PHI_VENKAT_S>(g, f, a.v, b.v, geo.v[0], geo.v[1], T(pp.kwv));
         Q4<T> r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) r.v[j] = f[j] * geo.v[2];
@@ -1515,7 +1537,7 @@ void launch_smooth(hipStream_t s, const MeshView<T>& m, const TileMap& tm, const
 
 template <class T, class S>
 size_t stage_lds_bytes(const HaloView& hv) {
-    return hv.lds_bytes > 0 ? (size_t)hv.lds_bytes : stage_lds_tile<T, S>(hv.max_ext, hv.max_edges);
+    return hv.lds_bytes > 0 ? (size_t)hv.lds_bytes : stage_lds_tile<T, S>(hv.max_ext, hv.max_edges, true);
 }
 
 template <class T, int PHI, int DT, bool J, int NB, class S>

@@ -79,11 +79,14 @@ FVH_HD T pressure(T g, const T* q) {
 //  joined=true : examples/forwardStep/main.cpp:72-135
 // The two differ only in how the central flux is associated.
 // SM: the smoothed entropy cure, |.| -> |.|_w, min/max -> min_w/max_w with
-// the velocity width w (FVH_FLAG_SMOOTH_KINKS).
-template <class T, bool JOINED, bool SM = false>
-FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1, T w = T(0)) {
-    const T pi = pressure(g, qi);
-    const T pj = pressure(g, qj);
+// the velocity width w (FVH_FLAG_SMOOTH_KINKS).  PRE: the caller passes the
+// two face states' pressures (pressure(g, qi), pressure(g, qj), computed
+// already) as pi_in, pj_in.
+template <class T, bool JOINED, bool SM = false, bool PRE = false>
+FVH_HD void roe_flux(T g, T* f, const T* qi, const T* qj, T n0, T n1, T w = T(0), T pi_in = T(0),
+                     T pj_in = T(0)) {
+    const T pi = PRE ? pi_in : pressure(g, qi);
+    const T pj = PRE ? pj_in : pressure(g, qj);
     const T Vi = (qi[1] * n0 + qi[2] * n1) / qi[0];
     const T Vj = (qj[1] * n0 + qj[2] * n1) / qj[0];
     if (JOINED) {

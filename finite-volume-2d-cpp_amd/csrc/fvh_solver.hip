@@ -841,8 +841,12 @@ bool build_tiles(fvh_solver* s, const std::vector<int32_t>& tile_c0, const std::
 // LDS bytes of the fused stage pass for a tile with n_ext local cells and
 // n_edges edges, in the solver's precision.
 size_t halo_lds_bytes(const fvh_solver* s, int32_t n_ext, int32_t n_edges) {
-    if (s->mixed) return stage_lds_tile<double, float>(n_ext, n_edges);
-    return s->f32 ? stage_lds_tile<float, float>(n_ext, n_edges) : stage_lds_tile<double, double>(n_ext, n_edges);
+    // with the positivity fallback, the face states' pressures follow the
+    // edge slots (two per tile edge, in the arithmetic type)
+    const bool pos = (s->cfg.flags & FVH_FLAG_POSITIVITY) != 0;
+    if (s->mixed) return stage_lds_tile<double, float>(n_ext, n_edges, pos);
+    return s->f32 ? stage_lds_tile<float, float>(n_ext, n_edges, pos)
+                  : stage_lds_tile<double, double>(n_ext, n_edges, pos);
 }
 
 // Halos of the real tiles for the fused stage pass.  Returns false when a
